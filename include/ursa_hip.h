@@ -1,0 +1,85 @@
+/*
+ * ursa_hip.h — C ABI of the MI355X (gfx950) backend for ursa-core's CSR
+ * algorithm kernels.
+ *
+ * Each entry point replaces one ursa-core function (cited file:line relative
+ * to the cldixon/ursa 0.2.0 tree) and keeps its argument meaning, output
+ * layout and edge-case behaviour. Outputs go to caller-allocated buffers of
+ * the stated length; a buffer may be host memory or device memory (device
+ * pointers are written in place, with no copy back). Inputs likewise.
+ *
+ * Error convention: every function returns URSA_HIP_OK (0) or a negative
+ * status; ursa_hip_last_error() returns a thread-local message for the last
+ * failure on the calling thread. No function aborts the process.
+ *
+ * Threading: any number of threads may call concurrently on one topology.
+ * The lazy incoming/undirected builds run once per topology (topology.rs:
+ * 257-260). Each calling thread gets its own HIP stream.
+ */
+#ifndef URSA_HIP_H
+#define URSA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define URSA_HIP_OK 0
+#define URSA_HIP_EINVAL (-22)  /* bad argument (length mismatch, id out of range, ...) */
+#define URSA_HIP_ENOMEM (-12)  /* device or host allocation failed */
+#define URSA_HIP_EDEVICE (-5)  /* HIP runtime error */
+#define URSA_HIP_ERANGE (-34)  /* size beyond a documented cap (edge cap, output cap) */
+#define URSA_HIP_EINTERNAL (-1)
+
+/* Direction — ursa-core/src/topology.rs:27-32 */
+typedef enum { URSA_HIP_OUT = 0, URSA_HIP_IN = 1, URSA_HIP_BOTH = 2 } ursa_hip_direction;
+
+typedef struct ursa_hip_topo ursa_hip_topo;
+
+/* ---- lifecycle ---- */
+/* Select and initialise the device (optional: the first call on a thread
+ * initialises device 0). */
+int ursa_hip_init(int device);
+int ursa_hip_shutdown(void);
+const char* ursa_hip_last_error(void);
+/* Synchronise the calling thread's stream. */
+int ursa_hip_synchronize(void);
+
+/* ---- topology — ursa-core/src/topology.rs ---- */
+/* Topology::build (topology.rs:276-298): stable grouping of the m edge rows by
+ * source, within-segment order = original row order (topology.rs:109-114).
+ * src_dense/dst_dense: u32[m], each < n. m must be <= UINT32_MAX
+ * (topology.rs:61-64; URSA_HIP_ERANGE otherwise). */
+int ursa_hip_topology_create(uint64_t n, uint64_t m, const uint32_t* src_dense,
+                             const uint32_t* dst_dense, ursa_hip_topo** out);
+int ursa_hip_topology_destroy(ursa_hip_topo* t);
+int ursa_hip_topology_size(const ursa_hip_topo* t, uint64_t* n, uint64_t* m);
+/* Topology::incoming (topology.rs:343-348): build the transpose now. */
+int ursa_hip_topology_incoming(ursa_hip_topo* t);
+/* Topology::undirected (topology.rs:353-389): build now; *slots = target count. */
+int ursa_hip_topology_undirected(ursa_hip_topo* t, uint64_t* slots);
+/* Export a CSR to caller buffers (host or device): which = 0 out, 1 incoming,
+ * 2 undirected (edge_ids must be NULL). offsets are u64[n+1] as in
+ * Adjacency::offsets (topology.rs:35-47); targets/edge_ids u32[m] (or
+ * u32[slots] for undirected). Any output pointer may be NULL. */
+int ursa_hip_topology_export(ursa_hip_topo* t, int which, uint64_t* offsets, uint32_t* targets,
+                             uint32_t* edge_ids);
+
+/* ---- kernels — ursa-core/src/algo/ ---- */
+/* degree (degree.rs:12-28) -> u32[n] */
+int ursa_hip_degree(ursa_hip_topo* t, int direction, uint32_t* out);
+
+/* pagerank / pagerank_weighted (pagerank.rs:98-169) with PageRankParams
+ * (pagerank.rs:21-36) -> f64[n]. weights: f64[m] by original edge row, or NULL.
+ * iters_out (may be NULL): iterations executed; the loop stops after the first
+ * iteration whose L1 delta < tol (pagerank.rs:87-90). */
+int ursa_hip_pagerank(ursa_hip_topo* t, const double* weights, double damping,
+                      uint32_t max_iter, double tol, double* out, uint32_t* iters_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* URSA_HIP_H */

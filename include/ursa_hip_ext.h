@@ -1,0 +1,70 @@
+/*
+ * ursa_hip_ext.h — entry points of the gfx950 backend that have no
+ * counterpart in ursa-core: kernel timing and seeded workload synthesis.
+ * Callers: benchmarks and tests (they need device-side timing and large
+ * synthetic graphs without a host round trip).
+ */
+#ifndef URSA_HIP_EXT_H
+#define URSA_HIP_EXT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- per-kernel timing ----
+ * When enabled, every launch of a named kernel is bracketed by HIP events on
+ * the stream it is launched on. query() synchronises those events and
+ * returns the summed duration, the launch count and the algorithmic bytes
+ * booked against the kernel (its per-launch traffic model). */
+int ursa_hip_profile_enable(int on);
+int ursa_hip_profile_reset(void);
+int ursa_hip_profile_query(const char* kernel, double* total_ms, uint64_t* launches,
+                           double* algo_bytes);
+/* Names of all kernels seen since the last reset, '\n'-separated. Returns the
+ * length needed (excluding the terminator). */
+uint64_t ursa_hip_profile_names(char* buf, uint64_t cap);
+
+/* ---- device memory for callers without a HIP runtime binding ---- */
+int ursa_hip_device_alloc(uint64_t bytes, void** out);
+int ursa_hip_device_free(void* p);
+int ursa_hip_memcpy_to_host(void* dst, const void* src_device, uint64_t bytes);
+
+/* ---- seeded workload synthesis on the device ----
+ * R-MAT: edge_factor << scale edge draws; each draw's endpoints come from
+ * `scale` quadrant choices (MSB first) with probabilities a, b, c, 1-a-b-c,
+ * using splitmix64 in counter form (draw t of stream `seed` is
+ * mix(seed + t * 0x9E3779B97F4A7C15)). Erdos-Renyi: edge_factor * n draws,
+ * src = draw(2e+1) % n, dst = draw(2e+2) % n. Both drop self-loops and later
+ * duplicates (first occurrence kept, row order = generation order);
+ * symmetrize appends every kept (v,u) after the kept rows and dedups again.
+ * Node ids are then interned first-seen, src then dst per row
+ * (ursa-core/src/id_map.rs:220-292), so the dense numbering is the one ursa
+ * would assign to the same int64 edge list. */
+typedef struct ursa_hip_edges ursa_hip_edges;
+int ursa_hip_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                      uint64_t seed, int symmetrize, ursa_hip_edges** out);
+int ursa_hip_gen_er(uint64_t n, uint32_t edge_factor, uint64_t seed, int symmetrize,
+                    ursa_hip_edges** out);
+int ursa_hip_edges_size(const ursa_hip_edges* g, uint64_t* n, uint64_t* m);
+/* Any pointer may be NULL; host or device buffers: user_src/user_dst int64[m]
+ * (generated ids), src_dense/dst_dense u32[m], user_ids int64[n]. */
+int ursa_hip_edges_export(ursa_hip_edges* g, int64_t* user_src, int64_t* user_dst,
+                          uint32_t* src_dense, uint32_t* dst_dense, int64_t* user_ids);
+/* Device addresses of the dense endpoint arrays (owned by the handle). */
+int ursa_hip_edges_dense_ptrs(ursa_hip_edges* g, const uint32_t** src_dense,
+                              const uint32_t** dst_dense);
+/* Topology::build straight from the device-resident dense arrays. */
+typedef struct ursa_hip_topo ursa_hip_topo;
+int ursa_hip_edges_topology(ursa_hip_edges* g, ursa_hip_topo** out);
+int ursa_hip_edges_destroy(ursa_hip_edges* g);
+/* f32-exact edge weights in (0,1]: w = ((x >> 40) + 1) / 2^24 with x drawn
+ * from the stream seed ^ 0x5745494748540000 (draw e+1 for edge e). */
+int ursa_hip_gen_weights(uint64_t m, uint64_t seed, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* URSA_HIP_EXT_H */

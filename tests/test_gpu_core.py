@@ -1,0 +1,174 @@
+"""GPU parity: device topology, degree, PageRank and the device graph generator,
+each through the C ABI, against the CPU oracle on the same seeded inputs.
+
+Bars: integer outputs bit-exact; f64 PageRank within 1e-12 relative per node with
+the same iteration count (the reference stops at the first delta < tol,
+pagerank.rs:87-90).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+FX = np.load(os.path.join(os.path.dirname(__file__), "golden", "nx_fixtures.npz"))
+PR_RTOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def ua():
+    import ursa_amd
+
+    ursa_amd.init(0)
+    return ursa_amd
+
+
+def graphs():
+    """(name, n, src_dense, dst_dense) cases covering the reference's edge cases."""
+    out = []
+    for name in ("gnp100", "sparse400", "ba500", "gnp300w"):
+        n, s, d, _ = orc.intern_i64(FX[f"{name}_src"], FX[f"{name}_dst"])
+        out.append((name, n, s, d))
+    out.append(("diamond", 3, np.array([0, 0, 1, 2]), np.array([1, 2, 2, 0])))
+    out.append(("loops_dups", 3, np.array([0, 0, 1, 2, 1, 0]), np.array([1, 2, 2, 0, 1, 1])))
+    rng = np.random.default_rng(17)
+    out.append(("multigraph", 3000, rng.integers(0, 3000, 40000), rng.integers(0, 3000, 40000)))
+    # One hub row spanning ~15 merge-path tiles (leaves -> 0, with random edges
+    # among the leaves so their ranks differ), plus a long chain out of the hub.
+    leaves = 30000
+    la = rng.integers(1, leaves + 1, leaves)
+    lb = rng.integers(1, leaves + 1, leaves)
+    s = np.concatenate([np.arange(1, leaves + 1), la, [0], np.arange(leaves + 1, leaves + 200)])
+    d = np.concatenate([np.zeros(leaves, np.int64), lb, [leaves + 1], np.arange(leaves + 2, leaves + 201)])
+    out.append(("hub", leaves + 201, s, d))
+    out.append(("isolated_only", 5, np.array([], np.int64), np.array([], np.int64)))
+    return out
+
+
+GRAPHS = graphs()
+IDS = [g[0] for g in GRAPHS]
+
+
+@pytest.mark.parametrize("case", GRAPHS, ids=IDS)
+def test_topology_matches_oracle(ua, case):
+    name, n, s, d = case
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    for which in ("out", "in"):
+        got = t.out() if which == "out" else t.incoming()
+        ref = o.adjacency(which)
+        for g, r in zip(got, ref):
+            assert np.array_equal(g, r), (name, which)
+    gu, ru = t.undirected(), o.undirected()
+    assert np.array_equal(gu[0], ru[0]) and np.array_equal(gu[1], ru[1]), name
+
+
+@pytest.mark.parametrize("case", GRAPHS, ids=IDS)
+def test_degree_matches_oracle(ua, case):
+    name, n, s, d = case
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    for direction in ("out", "in", "both"):
+        assert np.array_equal(ua.degree(t, direction), orc.degree(o, direction)), (name, direction)
+
+
+def assert_pr_close(got, ref, name):
+    assert got.shape == ref.shape
+    err = np.abs(got - ref)
+    bad = err > PR_RTOL * np.abs(ref)
+    assert not bad.any(), f"{name}: {bad.sum()} nodes beyond 1e-12 rel, worst {np.max(err / np.abs(ref)):.3e}"
+
+
+@pytest.mark.parametrize("case", GRAPHS, ids=IDS)
+@pytest.mark.parametrize("params", [(0.85, 30, 1e-6), (0.85, 100, 1e-10), (0.5, 7, 0.0)])
+def test_pagerank_matches_oracle(ua, case, params):
+    name, n, s, d = case
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    got, it = ua._pagerank(t, None, ua.PageRankParams(*params))
+    ref, rit, rdelta = orc.pagerank(o, *params)
+    assert it == rit, (name, it, rit, rdelta)
+    assert_pr_close(got, ref, name)
+
+
+@pytest.mark.parametrize("case", GRAPHS[:7], ids=IDS[:7])
+def test_pagerank_weighted_matches_oracle(ua, case):
+    name, n, s, d = case
+    m = len(s)
+    w = orc.gen_weights(m, 3)  # f32-exact weights: the f32 edge-weight path
+    w[::7] = 0.0
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    params = (0.85, 100, 1e-10)
+    got, it = ua._pagerank(t, w, ua.PageRankParams(*params))
+    ref, rit, _ = orc.pagerank(o, *params, weights=w)
+    assert it == rit
+    assert_pr_close(got, ref, name)
+    # weights not exact in f32 take the f64 edge-weight path
+    w64 = w * (1.0 + 1e-9)
+    got, it = ua._pagerank(t, w64, ua.PageRankParams(*params))
+    ref, rit, _ = orc.pagerank(o, *params, weights=w64)
+    assert it == rit
+    assert_pr_close(got, ref, name)
+
+
+def test_pagerank_edge_cases(ua):
+    t = ua.Topology.build(0, [], [])
+    assert ua.pagerank(t).size == 0
+    t = ua.Topology.build(3, [0, 1], [1, 2])
+    pr, it = ua._pagerank(t, None, ua.PageRankParams(0.85, 0, 1e-6))
+    assert it == 0 and np.all(pr == 1.0 / 3.0)
+    with pytest.raises(ValueError):
+        ua.pagerank_weighted(t, [1.0], ua.PageRankParams())
+    # reference goldens (pagerank.rs:176-219)
+    t = ua.Topology.build(4, [0, 1, 3, 2], [2, 2, 2, 0])
+    pr = ua.pagerank(t)
+    assert abs(pr.sum() - 1.0) < 1e-6 and all(pr[i] < pr[2] for i in (0, 1, 3))
+
+
+def test_pagerank_is_deterministic(ua):
+    rng = np.random.default_rng(2)
+    t = ua.Topology.build(20000, rng.integers(0, 20000, 300000), rng.integers(0, 20000, 300000))
+    a = ua.pagerank(t, ua.PageRankParams(0.85, 50, 1e-12))
+    b = ua.pagerank(t, ua.PageRankParams(0.85, 50, 1e-12))
+    assert np.array_equal(a, b)
+
+
+def test_out_of_range_endpoint_is_einval(ua):
+    with pytest.raises(ua.UrsaHipError) as e:
+        ua.Topology.build(3, [0, 5], [1, 2])
+    assert e.value.code == -22
+
+
+def test_generator_matches_oracle_generator(ua):
+    from ursa_amd import gen
+
+    for kw in (dict(scale=12, seed=1), dict(scale=11, seed=4, symmetrize=True)):
+        e = gen.rmat(kw["scale"], 16, seed=kw["seed"], symmetrize=kw.get("symmetrize", False))
+        us, ud, sd, dd, ids = e.export()
+        rs, rd = orc.gen_rmat(kw["scale"], 16, seed=kw["seed"], symmetrize=kw.get("symmetrize", False))
+        assert np.array_equal(us, rs) and np.array_equal(ud, rd), kw
+        n, osd, odd, oids = orc.intern_i64(rs, rd)
+        assert n == e.n and np.array_equal(osd, sd) and np.array_equal(odd, dd) and np.array_equal(oids, ids)
+    e = gen.erdos_renyi(5000, 8, 1007)
+    us, ud, _, _, _ = e.export()
+    rs, rd = orc.gen_er(5000, 8, 1007)
+    assert np.array_equal(us, rs) and np.array_equal(ud, rd)
+    assert np.array_equal(gen.weights(1000, 3), orc.gen_weights(1000, 3))
+
+
+def test_config1_pagerank_rmat_s16(ua):
+    """R-MAT scale 16, seed 1, damping 0.85, tol 1e-6, max_iter 100, f64."""
+    from ursa_amd import gen
+
+    e = gen.rmat(16, 16, seed=1)
+    _, _, sd, dd, _ = e.export()
+    t = e.topology()
+    o = orc.Topology(e.n, sd, dd)
+    got, it = ua._pagerank(t, None, ua.PageRankParams(0.85, 100, 1e-6))
+    ref, rit, rdelta = orc.pagerank(o, 0.85, 100, 1e-6)
+    assert it == rit, (it, rit, rdelta)
+    assert_pr_close(got, ref, "rmat16")

@@ -1,0 +1,191 @@
+"""ursa_amd — MI355X (gfx950) backend for ursa-core's CSR algorithm kernels.
+
+The Python surface mirrors ``ursa_core`` (ursa-core/src/lib.rs:25-30,
+ursa-core/src/algo/mod.rs:35-50): ``Topology.build``, ``Direction``,
+``PageRankParams`` and one function per kernel with the reference's name,
+argument meaning and edge-case behaviour. Every call goes through the C ABI of
+``libursa_hip.so``; there is no CPU path.
+
+Outputs are numpy arrays on the host. Arguments documented as "array" accept
+numpy arrays (host) or integer device addresses via the ``*_ptr`` helpers used
+by the benchmark.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import UrsaHipError, check, load
+
+__all__ = [
+    "Direction",
+    "PageRankParams",
+    "Topology",
+    "UrsaHipError",
+    "degree",
+    "pagerank",
+    "pagerank_weighted",
+    "init",
+    "profile",
+]
+
+
+class Direction(enum.IntEnum):
+    """ursa-core/src/topology.rs:27-32"""
+
+    Out = 0
+    In = 1
+    Both = 2
+
+
+def _dir(d) -> int:
+    if isinstance(d, str):
+        return {"out": 0, "in": 1, "both": 2}[d.lower()]
+    return int(d)
+
+
+@dataclass(frozen=True)
+class PageRankParams:
+    """ursa-core/src/algo/pagerank.rs:21-36 (defaults :28-35)."""
+
+    damping: float = 0.85
+    max_iter: int = 30
+    tol: float = 1e-6
+
+
+def init(device: int = 0) -> None:
+    check(load().ursa_hip_init(device))
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
+
+
+class Topology:
+    """Device-resident CSR topology (ursa-core/src/topology.rs:251-404).
+
+    ``build`` groups the edge rows by source with original row order kept inside each
+    segment (topology.rs:109-114); the transpose and the undirected view are built on
+    first use and cached.
+    """
+
+    def __init__(self, handle: C.c_void_p, n: int, m: int):
+        self._h = handle
+        self._n = n
+        self._m = m
+
+    @classmethod
+    def build(cls, n_nodes: int, src_dense, dst_dense) -> "Topology":
+        """Topology::build (topology.rs:276-298)."""
+        s = _u32(src_dense)
+        d = _u32(dst_dense)
+        if s.shape != d.shape:
+            raise ValueError("src/dst length mismatch")
+        return cls._from_ptrs(int(n_nodes), int(s.shape[0]), _ptr(s), _ptr(d))
+
+    @classmethod
+    def _from_ptrs(cls, n: int, m: int, src_ptr, dst_ptr) -> "Topology":
+        lib = load()
+        h = C.c_void_p()
+        check(lib.ursa_hip_topology_create(n, m, src_ptr, dst_ptr, C.byref(h)))
+        return cls(h, n, m)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib._lib is not None:
+            _lib._lib.ursa_hip_topology_destroy(h)
+            self._h = None
+
+    def n_nodes(self) -> int:
+        return self._n
+
+    def n_edges(self) -> int:
+        return self._m
+
+    def _export(self, which: int, slots: int, with_eids: bool):
+        off = np.zeros(self._n + 1, np.uint64)
+        tg = np.zeros(max(slots, 1), np.uint32)
+        ei = np.zeros(max(slots, 1), np.uint32) if with_eids else None
+        check(load().ursa_hip_topology_export(self._h, which, _ptr(off), _ptr(tg), _ptr(ei) if with_eids else None))
+        return off, tg[:slots], (ei[:slots] if with_eids else None)
+
+    def out(self):
+        """(offsets u64[n+1], targets u32[m], edge_ids u32[m]) — Topology::out."""
+        return self._export(0, self._m, True)
+
+    def incoming(self):
+        """Topology::incoming (topology.rs:343-348)."""
+        check(load().ursa_hip_topology_incoming(self._h))
+        return self._export(1, self._m, True)
+
+    def undirected(self):
+        """Topology::undirected (topology.rs:353-389): (offsets u64[n+1], targets u32[slots])."""
+        slots = C.c_uint64(0)
+        check(load().ursa_hip_topology_undirected(self._h, C.byref(slots)))
+        off, tg, _ = self._export(2, slots.value, False)
+        return off, tg
+
+
+def degree(topo: Topology, direction=Direction.Out) -> np.ndarray:
+    """degree (ursa-core/src/algo/degree.rs:12-28) -> u32[n]."""
+    out = np.zeros(max(topo.n_nodes(), 1), np.uint32)
+    check(load().ursa_hip_degree(topo._h, _dir(direction), _ptr(out)))
+    return out[: topo.n_nodes()]
+
+
+def _pagerank(topo: Topology, weights, params: PageRankParams):
+    n = topo.n_nodes()
+    out = np.zeros(max(n, 1), np.float64)
+    it = C.c_uint32(0)
+    w = None
+    if weights is not None:
+        w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64))
+        if w.shape[0] != topo.n_edges():
+            raise ValueError("weights length must equal the edge count")  # pagerank.rs:140-144
+    check(load().ursa_hip_pagerank(topo._h, _ptr(w) if w is not None else None, params.damping,
+                                   params.max_iter, params.tol, _ptr(out), C.byref(it)))
+    return out[:n], it.value
+
+
+def pagerank(topo: Topology, params: PageRankParams = PageRankParams()) -> np.ndarray:
+    """pagerank (ursa-core/src/algo/pagerank.rs:98-124) -> f64[n]."""
+    return _pagerank(topo, None, params)[0]
+
+
+def pagerank_weighted(topo: Topology, weights, params: PageRankParams = PageRankParams()) -> np.ndarray:
+    """pagerank_weighted (ursa-core/src/algo/pagerank.rs:135-169) -> f64[n]."""
+    return _pagerank(topo, weights, params)[0]
+
+
+class profile:
+    """Per-kernel event timing inside the library (ursa_hip_profile_*)."""
+
+    @staticmethod
+    def enable(on: bool = True) -> None:
+        check(load().ursa_hip_profile_enable(int(on)))
+
+    @staticmethod
+    def reset() -> None:
+        check(load().ursa_hip_profile_reset())
+
+    @staticmethod
+    def query(kernel: str):
+        ms, n, b = C.c_double(0), C.c_uint64(0), C.c_double(0)
+        check(load().ursa_hip_profile_query(kernel.encode(), C.byref(ms), C.byref(n), C.byref(b)))
+        return {"total_ms": ms.value, "launches": n.value, "algo_bytes": b.value}
+
+    @staticmethod
+    def names():
+        lib = load()
+        need = lib.ursa_hip_profile_names(None, 0)
+        buf = C.create_string_buffer(int(need) + 1)
+        lib.ursa_hip_profile_names(buf, need + 1)
+        return [x for x in buf.value.decode().split("\n") if x]

@@ -1,0 +1,91 @@
+"""ctypes binding of ``libursa_hip.so`` (the C ABI declared in ``include/ursa_hip.h``
+and ``include/ursa_hip_ext.h``).
+
+The library is built in-tree by ``ursa_amd/build.py`` (hipcc, gfx950). Importing this
+module fails loudly when it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libursa_hip.so")
+
+OK = 0
+EINVAL = -22
+ENOMEM = -12
+EDEVICE = -5
+ERANGE = -34
+EINTERNAL = -1
+
+
+class UrsaHipError(RuntimeError):
+    """A non-zero status from the C ABI, with the library's thread-local message."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+        self.message = message
+
+
+P = C.c_void_p
+u64, u32, i32, f64, cint = C.c_uint64, C.c_uint32, C.c_int32, C.c_double, C.c_int
+
+# name -> (restype, argtypes); every symbol the headers declare.
+SIGNATURES = {
+    "ursa_hip_init": (cint, [cint]),
+    "ursa_hip_shutdown": (cint, []),
+    "ursa_hip_last_error": (C.c_char_p, []),
+    "ursa_hip_synchronize": (cint, []),
+    "ursa_hip_topology_create": (cint, [u64, u64, P, P, C.POINTER(P)]),
+    "ursa_hip_topology_destroy": (cint, [P]),
+    "ursa_hip_topology_size": (cint, [P, C.POINTER(u64), C.POINTER(u64)]),
+    "ursa_hip_topology_incoming": (cint, [P]),
+    "ursa_hip_topology_undirected": (cint, [P, C.POINTER(u64)]),
+    "ursa_hip_topology_export": (cint, [P, cint, P, P, P]),
+    "ursa_hip_degree": (cint, [P, cint, P]),
+    "ursa_hip_pagerank": (cint, [P, P, f64, u32, f64, P, C.POINTER(u32)]),
+    "ursa_hip_profile_enable": (cint, [cint]),
+    "ursa_hip_profile_reset": (cint, []),
+    "ursa_hip_profile_query": (cint, [C.c_char_p, C.POINTER(f64), C.POINTER(u64), C.POINTER(f64)]),
+    "ursa_hip_profile_names": (u64, [C.c_char_p, u64]),
+    "ursa_hip_device_alloc": (cint, [u64, C.POINTER(P)]),
+    "ursa_hip_device_free": (cint, [P]),
+    "ursa_hip_memcpy_to_host": (cint, [P, P, u64]),
+    "ursa_hip_gen_rmat": (cint, [u32, u32, f64, f64, f64, u64, cint, C.POINTER(P)]),
+    "ursa_hip_gen_er": (cint, [u64, u32, u64, cint, C.POINTER(P)]),
+    "ursa_hip_edges_size": (cint, [P, C.POINTER(u64), C.POINTER(u64)]),
+    "ursa_hip_edges_export": (cint, [P, P, P, P, P, P]),
+    "ursa_hip_edges_dense_ptrs": (cint, [P, C.POINTER(P), C.POINTER(P)]),
+    "ursa_hip_edges_topology": (cint, [P, C.POINTER(P)]),
+    "ursa_hip_edges_destroy": (cint, [P]),
+    "ursa_hip_gen_weights": (cint, [u64, u64, P]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the HIP library (once). Raises ImportError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(hipcc, gfx950). There is no CPU fallback."
+        )
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status != OK:
+        msg = load().ursa_hip_last_error()
+        raise UrsaHipError(status, msg.decode() if msg else "")

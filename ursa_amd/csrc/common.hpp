@@ -1,0 +1,186 @@
+// common.hpp — shared host-side plumbing of the gfx950 backend: error
+// propagation into C status codes, per-thread streams, device buffers,
+// host/device argument staging and optional per-kernel event timing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <utility>
+
+#include "../../include/ursa_hip.h"
+#include "../../include/ursa_hip_ext.h"
+
+namespace ursa {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(int code, const std::string& msg);
+[[noreturn]] void hip_fail(hipError_t e, const char* expr, const char* file, int line);
+
+#define HIP_OK(expr)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (expr);                                                   \
+        if (e_ != hipSuccess) ::ursa::hip_fail(e_, #expr, __FILE__, __LINE__);    \
+    } while (0)
+
+#define URSA_REQUIRE(cond, code, msg)          \
+    do {                                       \
+        if (!(cond)) ::ursa::fail(code, msg);  \
+    } while (0)
+
+void set_last_error(const char* msg);
+
+// Run f, translating exceptions into a status code and a thread-local message.
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        return URSA_HIP_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return URSA_HIP_ENOMEM;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return URSA_HIP_EINTERNAL;
+    } catch (...) {
+        set_last_error("unknown internal error");
+        return URSA_HIP_EINTERNAL;
+    }
+}
+
+// Device selected for this process (ursa_hip_init, default 0); makes it
+// current on the calling thread.
+int ensure_device();
+// The calling thread's stream (non-blocking, default priority).
+hipStream_t stream();
+
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+
+template <class T>
+class DevBuf {
+   public:
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    ~DevBuf() { reset(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            reset();
+            p_ = o.p_; n_ = o.n_;
+            o.p_ = nullptr; o.n_ = 0;
+        }
+        return *this;
+    }
+    void alloc(size_t n) {
+        reset();
+        p_ = static_cast<T*>(dev_alloc((n ? n : 1) * sizeof(T)));
+        n_ = n;
+    }
+    // Grow-only (contents not preserved).
+    void ensure(size_t n) { if (!p_ || n_ < n) alloc(n); }
+    void reset() {
+        if (p_) dev_free(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    T* get() const { return p_; }
+    size_t size() const { return n_; }
+    size_t bytes() const { return n_ * sizeof(T); }
+    explicit operator bool() const { return p_ != nullptr; }
+
+   private:
+    T* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+bool is_device_ptr(const void* p);
+
+// A read-only argument: device pointers are used in place, host buffers
+// are uploaded to a temporary device copy.
+template <class T>
+struct DevIn {
+    const T* p = nullptr;
+    DevBuf<T> own;
+    DevIn(const T* user, size_t n, hipStream_t s) {
+        if (!user || n == 0) { p = nullptr; return; }
+        if (is_device_ptr(user)) { p = user; return; }
+        own.alloc(n);
+        HIP_OK(hipMemcpyAsync(own.get(), user, n * sizeof(T), hipMemcpyHostToDevice, s));
+        p = own.get();
+    }
+};
+
+// An output argument: device pointers are written in place; host buffers get
+// a device staging copy that finish() copies back (and synchronises).
+template <class T>
+struct DevOut {
+    T* user = nullptr;
+    T* p = nullptr;
+    size_t n = 0;
+    bool dev = false;
+    DevBuf<T> own;
+    DevOut(T* u, size_t count) : user(u), n(count) {
+        if (!user || n == 0) { p = nullptr; return; }
+        dev = is_device_ptr(user);
+        if (dev) { p = user; return; }
+        own.alloc(n);
+        p = own.get();
+    }
+    void finish(hipStream_t s) {
+        if (user && n && !dev)
+            HIP_OK(hipMemcpyAsync(user, own.get(), n * sizeof(T), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+    }
+};
+
+// ---- per-kernel timing (ursa_hip_profile_*) ----
+bool prof_enabled();
+void prof_push(const char* name, hipEvent_t a, hipEvent_t b, double bytes, hipStream_t s);
+
+// Launch through `launch()`; when profiling is on, bracket it with events on
+// the launch stream and book `bytes` algorithmic bytes against `name`.
+template <class L>
+inline void timed(const char* name, double bytes, hipStream_t s, L&& launch) {
+    if (!prof_enabled()) {
+        launch();
+        return;
+    }
+    hipEvent_t a, b;
+    HIP_OK(hipEventCreate(&a));
+    HIP_OK(hipEventCreate(&b));
+    HIP_OK(hipEventRecord(a, s));
+    launch();
+    HIP_OK(hipEventRecord(b, s));
+    prof_push(name, a, b, bytes, s);
+}
+
+inline unsigned blocks_for(uint64_t n, unsigned block, unsigned cap = 1u << 20) {
+    uint64_t b = (n + block - 1) / block;
+    if (b == 0) b = 1;
+    if (b > cap) b = cap;
+    return (unsigned)b;
+}
+
+inline int bits_for(uint64_t n) {  // bits needed to represent values < n
+    int b = 0;
+    while (b < 64 && (1ull << b) < n) ++b;
+    return b == 0 ? 1 : b;
+}
+
+inline void check_launch() { HIP_OK(hipGetLastError()); }
+
+}  // namespace ursa

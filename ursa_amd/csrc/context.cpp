@@ -1,0 +1,238 @@
+// context.cpp — device selection, per-thread streams, error strings,
+// allocation and the kernel-timing registry behind ursa_hip_profile_*.
+#include <atomic>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace ursa {
+
+namespace {
+thread_local std::string t_last_error;
+std::atomic<int> g_device{-1};
+std::mutex g_init_mu;
+
+struct ThreadStream {
+    hipStream_t s = nullptr;
+    int device = -1;
+    // Streams are intentionally not destroyed at thread exit: the runtime may
+    // already be torn down when thread-local destructors run at process exit.
+};
+thread_local ThreadStream t_stream;
+thread_local int t_current_device = -1;
+
+struct ProfRec {
+    std::string name;
+    hipEvent_t a, b;
+    double bytes;
+};
+struct ProfAgg {
+    double ms = 0.0;
+    uint64_t launches = 0;
+    double bytes = 0.0;
+};
+std::atomic<bool> g_prof_on{false};
+std::mutex g_prof_mu;
+std::vector<ProfRec> g_prof_pending;
+std::map<std::string, ProfAgg> g_prof_agg;
+
+void prof_drain_locked() {
+    for (auto& r : g_prof_pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            auto& g = g_prof_agg[r.name];
+            g.ms += ms;
+            g.launches += 1;
+            g.bytes += r.bytes;
+        }
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    g_prof_pending.clear();
+}
+}  // namespace
+
+void fail(int code, const std::string& msg) { throw Error(code, msg); }
+
+void hip_fail(hipError_t e, const char* expr, const char* file, int line) {
+    int code = (e == hipErrorOutOfMemory) ? URSA_HIP_ENOMEM : URSA_HIP_EDEVICE;
+    std::string f = file;
+    auto p = f.find_last_of('/');
+    if (p != std::string::npos) f = f.substr(p + 1);
+    throw Error(code, std::string(hipGetErrorString(e)) + " in " + expr + " (" + f + ":" +
+                          std::to_string(line) + ")");
+}
+
+void set_last_error(const char* msg) { t_last_error = msg ? msg : ""; }
+
+int ensure_device() {
+    int dev = g_device.load();
+    if (dev < 0) {
+        std::lock_guard<std::mutex> lk(g_init_mu);
+        dev = g_device.load();
+        if (dev < 0) {
+            int count = 0;
+            HIP_OK(hipGetDeviceCount(&count));
+            URSA_REQUIRE(count > 0, URSA_HIP_EDEVICE, "no HIP device visible");
+            dev = 0;
+            hipDeviceProp_t prop;
+            HIP_OK(hipGetDeviceProperties(&prop, dev));
+            URSA_REQUIRE(std::strncmp(prop.gcnArchName, "gfx950", 6) == 0, URSA_HIP_EDEVICE,
+                         std::string("this build targets gfx950 (MI355X); device 0 is ") + prop.gcnArchName);
+            g_device.store(dev);
+        }
+    }
+    if (t_current_device != dev) {
+        HIP_OK(hipSetDevice(dev));
+        t_current_device = dev;
+    }
+    return dev;
+}
+
+hipStream_t stream() {
+    int dev = ensure_device();
+    if (!t_stream.s || t_stream.device != dev) {
+        HIP_OK(hipStreamCreateWithFlags(&t_stream.s, hipStreamNonBlocking));
+        t_stream.device = dev;
+    }
+    return t_stream.s;
+}
+
+void* dev_alloc(size_t bytes) {
+    ensure_device();
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(URSA_HIP_ENOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed: " +
+                                         hipGetErrorString(e));
+    }
+    return p;
+}
+
+void dev_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+bool is_device_ptr(const void* p) {
+    if (!p) return false;
+    hipPointerAttribute_t attr;
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice;
+}
+
+bool prof_enabled() { return g_prof_on.load(std::memory_order_relaxed); }
+
+void prof_push(const char* name, hipEvent_t a, hipEvent_t b, double bytes, hipStream_t) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof_pending.push_back({name, a, b, bytes});
+    if (g_prof_pending.size() > 4096) prof_drain_locked();
+}
+
+}  // namespace ursa
+
+using namespace ursa;
+
+extern "C" {
+
+int ursa_hip_init(int device) {
+    return guard([&] {
+        int count = 0;
+        HIP_OK(hipGetDeviceCount(&count));
+        URSA_REQUIRE(device >= 0 && device < count, URSA_HIP_EINVAL, "device index out of range");
+        hipDeviceProp_t prop;
+        HIP_OK(hipGetDeviceProperties(&prop, device));
+        URSA_REQUIRE(std::strncmp(prop.gcnArchName, "gfx950", 6) == 0, URSA_HIP_EDEVICE,
+                     std::string("this build targets gfx950 (MI355X); device is ") + prop.gcnArchName);
+        {
+            std::lock_guard<std::mutex> lk(g_init_mu);
+            g_device.store(device);
+        }
+        ensure_device();
+    });
+}
+
+int ursa_hip_shutdown(void) {
+    return guard([&] {
+        if (g_device.load() >= 0) HIP_OK(hipDeviceSynchronize());
+    });
+}
+
+const char* ursa_hip_last_error(void) { return t_last_error.c_str(); }
+
+int ursa_hip_synchronize(void) {
+    return guard([&] { HIP_OK(hipStreamSynchronize(stream())); });
+}
+
+int ursa_hip_profile_enable(int on) {
+    g_prof_on.store(on != 0);
+    return URSA_HIP_OK;
+}
+
+int ursa_hip_profile_reset(void) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    prof_drain_locked();
+    g_prof_agg.clear();
+    return URSA_HIP_OK;
+}
+
+int ursa_hip_profile_query(const char* kernel, double* total_ms, uint64_t* launches, double* algo_bytes) {
+    return guard([&] {
+        URSA_REQUIRE(kernel != nullptr, URSA_HIP_EINVAL, "kernel name is NULL");
+        std::lock_guard<std::mutex> lk(g_prof_mu);
+        prof_drain_locked();
+        auto it = g_prof_agg.find(kernel);
+        ProfAgg a = it == g_prof_agg.end() ? ProfAgg{} : it->second;
+        if (total_ms) *total_ms = a.ms;
+        if (launches) *launches = a.launches;
+        if (algo_bytes) *algo_bytes = a.bytes;
+    });
+}
+
+int ursa_hip_device_alloc(uint64_t bytes, void** out) {
+    return guard([&] {
+        URSA_REQUIRE(out != nullptr, URSA_HIP_EINVAL, "out is NULL");
+        *out = dev_alloc(bytes);
+    });
+}
+
+int ursa_hip_device_free(void* p) {
+    return guard([&] {
+        ensure_device();
+        dev_free(p);
+    });
+}
+
+int ursa_hip_memcpy_to_host(void* dst, const void* src_device, uint64_t bytes) {
+    return guard([&] {
+        hipStream_t s = stream();
+        HIP_OK(hipMemcpyAsync(dst, src_device, bytes, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+    });
+}
+
+uint64_t ursa_hip_profile_names(char* buf, uint64_t cap) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    prof_drain_locked();
+    std::string all;
+    for (auto& kv : g_prof_agg) {
+        all += kv.first;
+        all += '\n';
+    }
+    if (buf && cap) {
+        uint64_t k = std::min<uint64_t>(cap - 1, all.size());
+        std::memcpy(buf, all.data(), k);
+        buf[k] = 0;
+    }
+    return all.size();
+}
+
+}  // extern "C"

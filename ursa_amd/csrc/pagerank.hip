@@ -1,0 +1,502 @@
+// pagerank.hip — pull PageRank over the device in-CSR (pagerank.rs:63-169).
+//
+// One iteration = three launches:
+//   k_pr_tiles   merge-path SpMV: tile t owns 2048 consecutive items of the
+//                merged (row-ends, in-edges) path, so every workgroup does the
+//                same work whatever the degree skew. In-edge columns are read
+//                coalesced, their contributions gathered into LDS, each thread
+//                walks 8 path items, and a wave-level segmented scan joins row
+//                pieces that straddle threads. Rows that end inside the tile
+//                get the fused epilogue at once:
+//                   next[u]    = base + d * acc                (pagerank.rs:84)
+//                   contrib[u] = next[u] / outdeg[u]           (pagerank.rs:116-120)
+//                   delta     += |next[u] - rank[u]|           (pagerank.rs:44-52)
+//                   dangling  += next[u] if outdeg[u] == 0     (pagerank.rs:78)
+//   k_pr_fixup   rows that began in an earlier tile: their per-tile pieces are
+//                summed in tile order and finished with the same epilogue.
+//   k_pr_reduce  one workgroup sums the per-tile partials in a fixed order,
+//                records delta, derives base = (1-d)/n + d*dangling/n for the
+//                next iteration (pagerank.rs:79) and raises `done` once
+//                delta < tol (pagerank.rs:87-90) or max_iter is reached.
+// Every reduction has a fixed order, so results are bit-identical run to run.
+// Because `done` lives on the device and every launch returns at once when
+// it is set, the host queues iterations in batches and reads the flag only
+// between batches.
+//
+// HBM traffic per iteration (algorithmic): 12 B per in-edge (u32 column + f64
+// contribution gather) and 32 B per node (row end 4, rank 8, out-degree 4,
+// next 8, contribution 8) — 12m + 32n. The weighted form adds the edge weight
+// (f32 when every weight is exactly representable, else f64) and reads an f64
+// out-strength instead of the u32 out-degree.
+#include <cmath>
+#include <vector>
+
+#include "topology.hpp"
+
+namespace ursa {
+
+namespace {
+
+constexpr int PR_BLOCK = 256;
+constexpr int PR_ITEMS = 8;
+constexpr int PR_TILE = PR_BLOCK * PR_ITEMS;
+
+struct PrCtl {
+    uint32_t done;
+    uint32_t iters;
+    double delta;
+    double base[2];
+};
+
+// Split point of the merge path at diagonal `diag`: x row-ends and y in-edges
+// precede it. row_end[i] = in_off[i + 1]; a row end is taken before in-edge j
+// exactly when row_end <= j.
+__device__ __forceinline__ void merge_path_search(uint64_t diag, const uint32_t* row_end, uint64_t n, uint64_t m,
+                                                  uint64_t& x, uint64_t& y) {
+    uint64_t lo = diag > m ? diag - m : 0;
+    uint64_t hi = diag < n ? diag : n;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if ((uint64_t)row_end[mid] + mid + 1 <= diag)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    x = lo;
+    y = diag - lo;
+}
+
+__global__ void k_pr_partition(const uint32_t* in_off, uint64_t n, uint64_t m, uint64_t tiles, uint32_t* tx,
+                               uint32_t* ty) {
+    uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (t > tiles) return;
+    uint64_t diag = t * (uint64_t)PR_TILE;
+    if (diag > n + m) diag = n + m;
+    uint64_t x, y;
+    merge_path_search(diag, in_off + 1, n, m, x, y);
+    tx[t] = (uint32_t)x;
+    ty[t] = (uint32_t)y;
+}
+
+template <class W>
+struct EdgeWeight {
+    __device__ static double get(const W* w, uint64_t k) { return (double)w[k]; }
+};
+
+// Deterministic sum over a workgroup; the result is valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    double tot = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += lds[w];
+    __syncthreads();
+    return tot;
+}
+
+// Row epilogue shared by the tile and fix-up kernels. `scale` is the
+// out-degree (unweighted) or out-strength (weighted) of u; scale > 0 means u
+// has out-mass, otherwise u is dangling.
+__device__ __forceinline__ void pr_finish_row(uint64_t u, double acc, double base, double damping, double scale,
+                                              const double* rank, double* next, double* contrib_next,
+                                              double& delta, double& dangling) {
+    const double v = base + damping * acc;
+    next[u] = v;
+    const bool has_out = scale > 0.0;
+    contrib_next[u] = has_out ? v / scale : 0.0;
+    delta += fabs(v - rank[u]);
+    if (!has_out) dangling += v;
+}
+
+template <bool WEIGHTED>
+__device__ __forceinline__ double node_scale(const uint32_t* out_deg, const double* out_str, uint64_t u) {
+    if constexpr (WEIGHTED)
+        return out_str[u];
+    else
+        return (double)out_deg[u];
+}
+
+template <class W, bool WEIGHTED>
+__global__ __launch_bounds__(PR_BLOCK) void k_pr_tiles(
+    const uint32_t* __restrict__ in_off, const uint32_t* __restrict__ in_tgt, const W* __restrict__ w_in,
+    const uint32_t* __restrict__ tile_x, const uint32_t* __restrict__ tile_y, uint64_t n,
+    const uint32_t* __restrict__ out_deg, const double* __restrict__ out_str, const double* __restrict__ rank,
+    const double* __restrict__ contrib, double* __restrict__ next, double* __restrict__ contrib_next,
+    double* __restrict__ carry_val, double* __restrict__ tail_val, double* __restrict__ tile_delta,
+    double* __restrict__ tile_dm, PrCtl* ctl, int parity, double damping) {
+    __shared__ double s_val[PR_TILE];
+    __shared__ uint32_t s_end[PR_TILE + 1];
+    __shared__ double s_wv[PR_BLOCK / 64];
+    __shared__ int s_wf[PR_BLOCK / 64];
+    __shared__ double s_red[PR_BLOCK / 64];
+
+    if (ctl->done) return;
+    const uint64_t t = blockIdx.x;
+    const uint64_t x0 = tile_x[t], y0 = tile_y[t];
+    const uint64_t x1 = tile_x[t + 1], y1 = tile_y[t + 1];
+    const uint64_t rows = x1 - x0, nnz = y1 - y0, len = rows + nnz;
+    const uint64_t n_end = (x1 < n ? x1 : n - 1) - x0 + 1;  // row ends staged (incl. trailing row)
+    const double base = ctl->base[parity];
+
+    for (uint64_t i = threadIdx.x; i < nnz; i += PR_BLOCK) {
+        const uint64_t k = y0 + i;
+        double c = contrib[in_tgt[k]];
+        if constexpr (WEIGHTED) c = EdgeWeight<W>::get(w_in, k) * c;
+        s_val[i] = c;
+    }
+    if (x0 < n)
+        for (uint64_t i = threadIdx.x; i < n_end; i += PR_BLOCK) s_end[i] = in_off[x0 + 1 + i];
+    const bool starts_mid = (x0 < n) && ((uint64_t)in_off[x0] < y0);
+    __syncthreads();
+
+    // Thread's start on the tile-local path.
+    const uint64_t d0 = (uint64_t)threadIdx.x * PR_ITEMS;
+    uint64_t x = x0, y = y0;
+    if (d0 < len) {
+        uint64_t lo = d0 > nnz ? d0 - nnz : 0, hi = d0 < rows ? d0 : rows;
+        while (lo < hi) {
+            uint64_t mid = (lo + hi) >> 1;
+            if ((uint64_t)s_end[mid] + mid + 1 <= y0 + d0)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        x = x0 + lo;
+        y = y0 + d0 - lo;
+    }
+
+    double sum = 0.0;
+    double ev[PR_ITEMS];
+    uint32_t er[PR_ITEMS];
+    uint32_t emask = 0;
+#pragma unroll
+    for (int it = 0; it < PR_ITEMS; ++it) {
+        ev[it] = 0.0;
+        er[it] = 0;
+        if (d0 + it < len) {
+            if (x < n && y < (uint64_t)s_end[x - x0]) {
+                sum += s_val[y - y0];
+                ++y;
+            } else {
+                er[it] = (uint32_t)x;
+                ev[it] = sum;
+                emask |= 1u << it;
+                sum = 0.0;
+                ++x;
+            }
+        }
+    }
+
+    // Segmented inclusive scan of (sum, emitted) across the workgroup: a
+    // thread's incoming carry is the piece of its first row accumulated by
+    // the preceding threads since their last row end.
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double v = sum;
+    int f = emask != 0;
+    for (int off = 1; off < 64; off <<= 1) {
+        double vu = __shfl_up(v, off, 64);
+        int fu = __shfl_up(f, off, 64);
+        if (lane >= off) {
+            if (!f) v = vu + v;
+            f |= fu;
+        }
+    }
+    if (lane == 63) {
+        s_wv[wave] = v;
+        s_wf[wave] = f;
+    }
+    __syncthreads();
+    double pv = 0.0;
+    for (int w = 0; w < wave; ++w) pv = s_wf[w] ? s_wv[w] : pv + s_wv[w];
+    const double v_incl = f ? v : pv + v;
+    double c_in = __shfl_up(v_incl, 1, 64);
+    if (lane == 0) c_in = pv;
+    if (threadIdx.x == 0) c_in = 0.0;
+    if (threadIdx.x == PR_BLOCK - 1) carry_val[t] = (x1 < n) ? v_incl : 0.0;
+
+    double delta = 0.0, dm = 0.0;
+    bool first = true;
+#pragma unroll
+    for (int it = 0; it < PR_ITEMS; ++it) {
+        if (emask & (1u << it)) {
+            double val = ev[it];
+            if (first) {
+                val = c_in + val;
+                first = false;
+            }
+            const uint64_t u = er[it];
+            if (u == x0 && starts_mid) {
+                tail_val[t] = val;
+            } else {
+                pr_finish_row(u, val, base, damping, node_scale<WEIGHTED>(out_deg, out_str, u), rank, next,
+                              contrib_next, delta, dm);
+            }
+        }
+    }
+    const double bd = block_sum(delta, s_red);
+    const double bm = block_sum(dm, s_red);
+    if (threadIdx.x == 0) {
+        tile_delta[t] = bd;
+        tile_dm[t] = bm;
+    }
+}
+
+template <bool WEIGHTED>
+__global__ void k_pr_fixup(const uint32_t* __restrict__ in_off, const uint32_t* __restrict__ tile_x,
+                           const uint32_t* __restrict__ tile_y, uint64_t tiles, uint64_t n,
+                           const uint32_t* __restrict__ out_deg, const double* __restrict__ out_str,
+                           const double* __restrict__ rank, double* __restrict__ next,
+                           double* __restrict__ contrib_next, const double* __restrict__ carry_val,
+                           const double* __restrict__ tail_val, double* __restrict__ fix_delta,
+                           double* __restrict__ fix_dm, const PrCtl* ctl, int parity, double damping) {
+    if (ctl->done) return;
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (t >= tiles) return;
+    double delta = 0.0, dm = 0.0;
+    const uint64_t x0 = tile_x[t], y0 = tile_y[t], x1 = tile_x[t + 1];
+    if (x0 < n && x1 > x0 && (uint64_t)in_off[x0] < y0) {
+        // Tiles j..t-1 all end inside row x0; tile t holds its end.
+        uint64_t j = t - 1;
+        while (j > 0 && tile_x[j] == x0) --j;
+        double acc = 0.0;
+        for (uint64_t q = j; q < t; ++q) acc += carry_val[q];
+        acc += tail_val[t];
+        pr_finish_row(x0, acc, ctl->base[parity], damping, node_scale<WEIGHTED>(out_deg, out_str, x0), rank,
+                      next, contrib_next, delta, dm);
+    }
+    fix_delta[t] = delta;
+    fix_dm[t] = dm;
+}
+
+__global__ __launch_bounds__(1024) void k_pr_reduce(const double* __restrict__ a_delta,
+                                                    const double* __restrict__ b_delta,
+                                                    const double* __restrict__ a_dm,
+                                                    const double* __restrict__ b_dm, uint64_t count, PrCtl* ctl,
+                                                    int parity, uint32_t iter, uint32_t max_iter, double tol,
+                                                    double damping, double nf) {
+    __shared__ double s_red[1024 / 64];
+    if (ctl->done) return;
+    double d = 0.0, m = 0.0;
+    for (uint64_t i = threadIdx.x; i < count; i += blockDim.x) {
+        d += a_delta[i] + b_delta[i];
+        m += a_dm[i] + b_dm[i];
+    }
+    const double delta = block_sum(d, s_red);
+    const double dm = block_sum(m, s_red);
+    if (threadIdx.x == 0) {
+        ctl->delta = delta;
+        ctl->iters = iter + 1;
+        ctl->base[parity ^ 1] = (1.0 - damping) / nf + damping * dm / nf;
+        if (delta < tol || iter + 1 >= max_iter) ctl->done = 1;
+    }
+}
+
+// rank0 = 1/n, contrib0 = rank0 / scale, per-block dangling partials.
+template <bool WEIGHTED>
+__global__ void k_pr_init(uint64_t n, double nf, const uint32_t* __restrict__ out_deg,
+                          const double* __restrict__ out_str, double* __restrict__ rank,
+                          double* __restrict__ contrib, double* __restrict__ part_dm) {
+    __shared__ double s_red[256 / 64];
+    double dm = 0.0;
+    const uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (u < n) {
+        const double r = 1.0 / nf;
+        rank[u] = r;
+        const double sc = node_scale<WEIGHTED>(out_deg, out_str, u);
+        contrib[u] = sc > 0.0 ? r / sc : 0.0;
+        if (!(sc > 0.0)) dm = r;
+    }
+    const double b = block_sum(dm, s_red);
+    if (threadIdx.x == 0) part_dm[blockIdx.x] = b;
+}
+
+__global__ void k_pr_ctl_init(PrCtl* ctl, const double* part_dm, uint64_t count, double damping, double nf) {
+    // Single thread: fixed-order sum of the init partials.
+    double dm = 0.0;
+    for (uint64_t i = 0; i < count; ++i) dm += part_dm[i];
+    ctl->done = 0;
+    ctl->iters = 0;
+    ctl->delta = 0.0;
+    ctl->base[0] = (1.0 - damping) / nf + damping * dm / nf;
+    ctl->base[1] = 0.0;
+}
+
+// out_str[v] = sum of v's out-edge weights in out-CSR order (pagerank.rs:148-150).
+__global__ void k_out_strength(const uint32_t* __restrict__ out_off, const uint32_t* __restrict__ out_eid,
+                               const double* __restrict__ w, uint64_t n, double* __restrict__ out_str) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        double s = -0.0;
+        for (uint32_t k = out_off[v]; k < out_off[v + 1]; ++k) s += w[out_eid[k]];
+        out_str[v] = s;
+    }
+}
+
+// Weights permuted into in-CSR slot order; flags any weight not exact in f32.
+__global__ void k_permute_weights(const uint32_t* __restrict__ in_eid, const double* __restrict__ w, uint64_t m,
+                                  double* __restrict__ w64, float* __restrict__ w32, uint32_t* inexact) {
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < m;
+         k += (uint64_t)gridDim.x * blockDim.x) {
+        const double x = w[in_eid[k]];
+        const float f = (float)x;
+        w64[k] = x;
+        w32[k] = f;
+        if ((double)f != x) atomicOr(inexact, 1u);
+    }
+}
+
+__global__ void k_fill(double* p, uint64_t n, double v) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+}  // namespace
+
+// Cached merge-path partition of the in-CSR.
+static void pr_partition(ursa_hip_topo* t, const DevCsr& in, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(t->cache_mu);
+    if (t->pr_tiles && t->pr_tile_items == PR_TILE) return;
+    const uint64_t len = t->n + t->m;
+    const uint64_t tiles = (len + PR_TILE - 1) / PR_TILE;
+    t->pr_tile_x.alloc(tiles + 1);
+    t->pr_tile_y.alloc(tiles + 1);
+    k_pr_partition<<<blocks_for(tiles + 1, 256), 256, 0, s>>>(in.off.get(), t->n, t->m, tiles, t->pr_tile_x.get(),
+                                                              t->pr_tile_y.get());
+    check_launch();
+    HIP_OK(hipStreamSynchronize(s));
+    t->pr_tiles = tiles;
+    t->pr_tile_items = PR_TILE;
+}
+
+void pagerank_f64(ursa_hip_topo* t, const double* weights_user, double damping, uint32_t max_iter, double tol,
+                  double* out_user, uint32_t* iters_out) {
+    hipStream_t s = stream();
+    const uint64_t n = t->n, m = t->m;
+    if (iters_out) *iters_out = 0;
+    if (n == 0) return;
+    URSA_REQUIRE(out_user != nullptr, URSA_HIP_EINVAL, "out is NULL");
+    const double nf = (double)n;
+    DevOut<double> out(out_user, n);
+    if (max_iter == 0) {  // pagerank.rs:73-75: the loop body never runs
+        k_fill<<<blocks_for(n, 256), 256, 0, s>>>(out.p, n, 1.0 / nf);
+        check_launch();
+        out.finish(s);
+        return;
+    }
+    const DevCsr& in = t->incoming(s);
+    pr_partition(t, in, s);
+    const uint64_t tiles = t->pr_tiles;
+
+    const bool weighted = weights_user != nullptr;
+    DevIn<double> w(weights_user, m, s);
+    DevBuf<double> out_str, w64;
+    DevBuf<float> w32;
+    bool use_f32 = false;
+    if (weighted) {
+        out_str.alloc(n);
+        k_out_strength<<<blocks_for(n, 256), 256, 0, s>>>(t->out.off.get(), t->out.eid.get(), w.p, n, out_str.get());
+        check_launch();
+        w64.alloc(m);
+        w32.alloc(m);
+        DevBuf<uint32_t> inexact(1);
+        HIP_OK(hipMemsetAsync(inexact.get(), 0, sizeof(uint32_t), s));
+        if (m) {
+            k_permute_weights<<<blocks_for(m, 256), 256, 0, s>>>(in.eid.get(), w.p, m, w64.get(), w32.get(),
+                                                                 inexact.get());
+            check_launch();
+        }
+        uint32_t h = 0;
+        HIP_OK(hipMemcpyAsync(&h, inexact.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        use_f32 = (h == 0);
+        if (use_f32)
+            w64.reset();
+        else
+            w32.reset();
+    }
+
+    DevBuf<double> rank[2] = {DevBuf<double>(n), DevBuf<double>(n)};
+    DevBuf<double> contrib[2] = {DevBuf<double>(n), DevBuf<double>(n)};
+    DevBuf<double> carry(tiles), tail(tiles), td(tiles), tm(tiles), fd(tiles), fm(tiles);
+    DevBuf<PrCtl> ctl(1);
+    const unsigned init_blocks = blocks_for(n, 256, 1u << 30);
+    DevBuf<double> part_dm(init_blocks);
+    if (weighted)
+        k_pr_init<true><<<init_blocks, 256, 0, s>>>(n, nf, t->out_deg.get(), out_str.get(), rank[0].get(),
+                                                    contrib[0].get(), part_dm.get());
+    else
+        k_pr_init<false><<<init_blocks, 256, 0, s>>>(n, nf, t->out_deg.get(), nullptr, rank[0].get(),
+                                                     contrib[0].get(), part_dm.get());
+    k_pr_ctl_init<<<1, 1, 0, s>>>(ctl.get(), part_dm.get(), init_blocks, damping, nf);
+    check_launch();
+
+    const double edge_bytes = weighted ? (use_f32 ? 16.0 : 20.0) : 12.0;
+    const double node_bytes = weighted ? 36.0 : 32.0;
+    const double spmv_bytes = edge_bytes * (double)m + node_bytes * (double)n;
+    const unsigned fix_blocks = blocks_for(tiles, 256, 1u << 30);
+
+    PrCtl h{};
+    // Iterations are queued in batches; with per-kernel timing on, one at a
+    // time so no launch after convergence is booked.
+    const uint32_t batch = prof_enabled() ? 1 : 8;
+    for (uint32_t it0 = 0; it0 < max_iter; it0 += batch) {
+        const uint32_t it_end = std::min<uint32_t>(max_iter, it0 + batch);
+        for (uint32_t it = it0; it < it_end; ++it) {
+            const int p = (int)(it & 1);
+            const double* rk = rank[p].get();
+            const double* cb = contrib[p].get();
+            double* nx = rank[p ^ 1].get();
+            double* cn = contrib[p ^ 1].get();
+            timed("pagerank_spmv", spmv_bytes, s, [&] {
+                if (!weighted)
+                    k_pr_tiles<double, false><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
+                        in.off.get(), in.tgt.get(), nullptr, t->pr_tile_x.get(), t->pr_tile_y.get(), n,
+                        t->out_deg.get(), nullptr, rk, cb, nx, cn, carry.get(), tail.get(), td.get(), tm.get(),
+                        ctl.get(), p, damping);
+                else if (use_f32)
+                    k_pr_tiles<float, true><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
+                        in.off.get(), in.tgt.get(), w32.get(), t->pr_tile_x.get(), t->pr_tile_y.get(), n,
+                        t->out_deg.get(), out_str.get(), rk, cb, nx, cn, carry.get(), tail.get(), td.get(),
+                        tm.get(), ctl.get(), p, damping);
+                else
+                    k_pr_tiles<double, true><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
+                        in.off.get(), in.tgt.get(), w64.get(), t->pr_tile_x.get(), t->pr_tile_y.get(), n,
+                        t->out_deg.get(), out_str.get(), rk, cb, nx, cn, carry.get(), tail.get(), td.get(),
+                        tm.get(), ctl.get(), p, damping);
+            });
+            check_launch();
+            if (weighted)
+                k_pr_fixup<true><<<fix_blocks, 256, 0, s>>>(in.off.get(), t->pr_tile_x.get(), t->pr_tile_y.get(),
+                                                            tiles, n, t->out_deg.get(), out_str.get(), rk, nx, cn,
+                                                            carry.get(), tail.get(), fd.get(), fm.get(), ctl.get(),
+                                                            p, damping);
+            else
+                k_pr_fixup<false><<<fix_blocks, 256, 0, s>>>(in.off.get(), t->pr_tile_x.get(), t->pr_tile_y.get(),
+                                                             tiles, n, t->out_deg.get(), nullptr, rk, nx, cn,
+                                                             carry.get(), tail.get(), fd.get(), fm.get(), ctl.get(),
+                                                             p, damping);
+            k_pr_reduce<<<1, 1024, 0, s>>>(td.get(), fd.get(), tm.get(), fm.get(), tiles, ctl.get(), p, it, max_iter,
+                                           tol, damping, nf);
+            check_launch();
+        }
+        HIP_OK(hipMemcpyAsync(&h, ctl.get(), sizeof(PrCtl), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        if (h.done) break;
+    }
+    const uint32_t iters = h.iters;
+    HIP_OK(hipMemcpyAsync(out.p, rank[iters & 1].get(), n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    out.finish(s);
+    if (iters_out) *iters_out = iters;
+}
+
+}  // namespace ursa
+
+extern "C" int ursa_hip_pagerank(ursa_hip_topo* t, const double* weights, double damping, uint32_t max_iter,
+                                 double tol, double* out, uint32_t* iters_out) {
+    return ursa::guard([&] {
+        URSA_REQUIRE(t != nullptr, URSA_HIP_EINVAL, "topology is NULL");
+        ursa::pagerank_f64(t, weights, damping, max_iter, tol, out, iters_out);
+    });
+}

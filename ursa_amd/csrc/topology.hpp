@@ -1,0 +1,50 @@
+// topology.hpp — the device-resident CSR topology shared by every kernel.
+//
+// HBM layout (all u32; m <= UINT32_MAX is enforced, topology.rs:61-64):
+//   out.off[n+1], out.tgt[m], out.eid[m]   — built eagerly (Topology::build)
+//   in.off[n+1],  in.tgt[m],  in.eid[m]    — lazy transpose (Topology::incoming)
+//   und.off[n+1], und.tgt[slots]           — lazy undirected view (Topology::undirected)
+//   out_deg[n]                             — cached with the out-CSR
+// Offsets are u32 on the device (ursa-core keeps u64 on the host,
+// topology.rs:40-44); every offset value is <= m, so u32 is lossless.
+#pragma once
+
+#include <mutex>
+
+#include "common.hpp"
+
+namespace ursa {
+
+struct DevCsr {
+    DevBuf<uint32_t> off;
+    DevBuf<uint32_t> tgt;
+    DevBuf<uint32_t> eid;
+    uint64_t slots = 0;
+    bool built = false;
+};
+
+// Stable grouping of m rows by key (< n): the device counterpart of
+// Adjacency::build (topology.rs:54-187). Within each key segment rows keep
+// their original order, so the result is byte-identical to the CPU build.
+void build_grouped(uint64_t n, uint64_t m, const uint32_t* keys, const uint32_t* other, DevCsr& out,
+                   hipStream_t s);
+
+// owner[k] = node whose segment holds slot k of `csr` (needs csr.slots > 0).
+void slot_owners(uint64_t n, const DevCsr& csr, uint32_t* owner, hipStream_t s);
+
+}  // namespace ursa
+
+struct ursa_hip_topo {
+    uint64_t n = 0, m = 0;
+    ursa::DevCsr out, in, und;
+    ursa::DevBuf<uint32_t> out_deg;
+    std::mutex build_mu;  // guards the lazy in / und builds
+    std::mutex cache_mu;  // guards algorithm-side caches below
+    // PageRank merge-path partition of the in-CSR (cached, see pagerank.hip).
+    ursa::DevBuf<uint32_t> pr_tile_x, pr_tile_y;
+    uint64_t pr_tiles = 0;
+    int pr_tile_items = 0;
+
+    const ursa::DevCsr& incoming(hipStream_t s);
+    const ursa::DevCsr& undirected(hipStream_t s);
+};
