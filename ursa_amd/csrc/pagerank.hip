@@ -312,15 +312,19 @@ __global__ void k_pr_init(uint64_t n, double nf, const uint32_t* __restrict__ ou
     if (threadIdx.x == 0) part_dm[blockIdx.x] = b;
 }
 
-__global__ void k_pr_ctl_init(PrCtl* ctl, const double* part_dm, uint64_t count, double damping, double nf) {
-    // Single thread: fixed-order sum of the init partials.
-    double dm = 0.0;
-    for (uint64_t i = 0; i < count; ++i) dm += part_dm[i];
-    ctl->done = 0;
-    ctl->iters = 0;
-    ctl->delta = 0.0;
-    ctl->base[0] = (1.0 - damping) / nf + damping * dm / nf;
-    ctl->base[1] = 0.0;
+__global__ __launch_bounds__(1024) void k_pr_ctl_init(PrCtl* ctl, const double* part_dm, uint64_t count,
+                                                      double damping, double nf) {
+    __shared__ double s_red[1024 / 64];
+    double m = 0.0;
+    for (uint64_t i = threadIdx.x; i < count; i += blockDim.x) m += part_dm[i];
+    const double dm = block_sum(m, s_red);
+    if (threadIdx.x == 0) {
+        ctl->done = 0;
+        ctl->iters = 0;
+        ctl->delta = 0.0;
+        ctl->base[0] = (1.0 - damping) / nf + damping * dm / nf;
+        ctl->base[1] = 0.0;
+    }
 }
 
 // out_str[v] = sum of v's out-edge weights in out-CSR order (pagerank.rs:148-150).
@@ -429,7 +433,7 @@ void pagerank_f64(ursa_hip_topo* t, const double* weights_user, double damping, 
     else
         k_pr_init<false><<<init_blocks, 256, 0, s>>>(n, nf, t->out_deg.get(), nullptr, rank[0].get(),
                                                      contrib[0].get(), part_dm.get());
-    k_pr_ctl_init<<<1, 1, 0, s>>>(ctl.get(), part_dm.get(), init_blocks, damping, nf);
+    k_pr_ctl_init<<<1, 1024, 0, s>>>(ctl.get(), part_dm.get(), init_blocks, damping, nf);
     check_launch();
 
     const double edge_bytes = weighted ? (use_f32 ? 16.0 : 20.0) : 12.0;
