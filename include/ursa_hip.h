@@ -79,6 +79,50 @@ int ursa_hip_degree(ursa_hip_topo* t, int direction, uint32_t* out);
 int ursa_hip_pagerank(ursa_hip_topo* t, const double* weights, double damping,
                       uint32_t max_iter, double tol, double* out, uint32_t* iters_out);
 
+/* bfs_distances (bfs.rs:18-42) for k sources -> i32[k * n] row-major (row i =
+ * levels from sources[i]; -1 = unreachable). An out-of-range source yields a
+ * row of -1 (bfs.rs:21-23). */
+int ursa_hip_bfs(ursa_hip_topo* t, const uint32_t* sources, uint32_t k, int direction,
+                 int32_t* levels);
+
+/* k_hop (hop.rs:52-75) -> (seeds_out, reached) pairs in the reference's order:
+ * seeds in input order, reached nodes in discovery order. k == 0 or an
+ * unknown seed contributes nothing; repeated seeds repeat their pairs. */
+typedef struct ursa_hip_pairs ursa_hip_pairs;
+int ursa_hip_k_hop(ursa_hip_topo* t, const uint32_t* seeds, uint64_t nseeds, uint32_t k,
+                   int direction, ursa_hip_pairs** out);
+int ursa_hip_pairs_size(const ursa_hip_pairs* p, uint64_t* count);
+int ursa_hip_pairs_export(const ursa_hip_pairs* p, uint32_t* src, uint32_t* dst);
+int ursa_hip_pairs_destroy(ursa_hip_pairs* p);
+
+/* shortest_path (bfs.rs:48-102): *len = 0 for None, else the node count of
+ * the path written to path[0..*len) (source first). The path is the one the
+ * reference returns (first discoverer of every node). URSA_HIP_ERANGE if
+ * *len > cap. */
+int ursa_hip_shortest_path(ursa_hip_topo* t, uint32_t source, uint32_t target, int direction,
+                           uint32_t* path, uint64_t cap, uint64_t* len);
+
+/* betweenness (betweenness.rs:153-177): has_sample = 0 for None (exact),
+ * otherwise ceil(sample*n) seed-shuffled sources and an n/k scale
+ * (betweenness.rs:349-362); has_seed = 0 uses the reference's default seed.
+ * -> f64[n]. */
+int ursa_hip_betweenness(ursa_hip_topo* t, int has_sample, double sample, int has_seed,
+                         uint64_t seed, double* out);
+/* Brandes over an explicit source list (accumulate_sources,
+ * betweenness.rs:45-70); out[v] *= scale afterwards when apply_scale. Sources
+ * >= n are skipped. */
+int ursa_hip_betweenness_sources(ursa_hip_topo* t, const uint32_t* sources, uint64_t k,
+                                 int apply_scale, double scale, double* out);
+
+/* triangle_count (triangle.rs:58-63) -> u64[n] (the reference accumulates in
+ * u32, which wraps above 2^32 triangles per node). */
+int ursa_hip_triangle_count(ursa_hip_topo* t, uint64_t* out);
+
+/* connected_components_weak (components.rs:58-68) -> u32[n]. Each label is
+ * the minimum dense id of the node's component (a valid representative under
+ * the reference's contract, and its canonical form). */
+int ursa_hip_cc_weak(ursa_hip_topo* t, uint32_t* labels);
+
 #ifdef __cplusplus
 }
 #endif

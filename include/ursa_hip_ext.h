@@ -14,6 +14,8 @@
 extern "C" {
 #endif
 
+typedef struct ursa_hip_topo ursa_hip_topo;
+
 /* ---- per-kernel timing ----
  * When enabled, every launch of a named kernel is bracketed by HIP events on
  * the stream it is launched on. query() synchronises those events and
@@ -26,6 +28,18 @@ int ursa_hip_profile_query(const char* kernel, double* total_ms, uint64_t* launc
 /* Names of all kernels seen since the last reset, '\n'-separated. Returns the
  * length needed (excluding the terminator). */
 uint64_t ursa_hip_profile_names(char* buf, uint64_t cap);
+
+/* ---- traversal statistics for benchmarks ----
+ * As ursa_hip_bfs, also returning the TEPS numerator: the sum over all k
+ * traversals of the forward degrees of the reached vertices. */
+int ursa_hip_bfs_ex(ursa_hip_topo* t, const uint32_t* sources, uint32_t k, int direction,
+                    int32_t* levels, uint64_t* reached_edges);
+/* As ursa_hip_betweenness_sources with an explicit batch size (1..64 sources
+ * advancing together; 1 = one traversal per launch sequence) and the TEPS
+ * numerator (sum of out-degrees of reached vertices over all sources). */
+int ursa_hip_betweenness_sources_ex(ursa_hip_topo* t, const uint32_t* sources, uint64_t k,
+                                    int apply_scale, double scale, uint32_t batch, double* out,
+                                    uint64_t* reached_edges);
 
 /* ---- device memory for callers without a HIP runtime binding ---- */
 int ursa_hip_device_alloc(uint64_t bytes, void** out);
@@ -57,7 +71,6 @@ int ursa_hip_edges_export(ursa_hip_edges* g, int64_t* user_src, int64_t* user_ds
 int ursa_hip_edges_dense_ptrs(ursa_hip_edges* g, const uint32_t** src_dense,
                               const uint32_t** dst_dense);
 /* Topology::build straight from the device-resident dense arrays. */
-typedef struct ursa_hip_topo ursa_hip_topo;
 int ursa_hip_edges_topology(ursa_hip_edges* g, ursa_hip_topo** out);
 int ursa_hip_edges_destroy(ursa_hip_edges* g);
 /* f32-exact edge weights in (0,1]: w = ((x >> 40) + 1) / 2^24 with x drawn

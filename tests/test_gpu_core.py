@@ -5,16 +5,14 @@ Bars: integer outputs bit-exact; f64 PageRank within 1e-12 relative per node wit
 the same iteration count (the reference stops at the first delta < tol,
 pagerank.rs:87-90).
 """
-import os
-
 import numpy as np
 import pytest
 
 import oracle as orc
+from graphs import cases
 
 pytestmark = pytest.mark.gpu
 
-FX = np.load(os.path.join(os.path.dirname(__file__), "golden", "nx_fixtures.npz"))
 PR_RTOL = 1e-12
 
 
@@ -26,29 +24,7 @@ def ua():
     return ursa_amd
 
 
-def graphs():
-    """(name, n, src_dense, dst_dense) cases covering the reference's edge cases."""
-    out = []
-    for name in ("gnp100", "sparse400", "ba500", "gnp300w"):
-        n, s, d, _ = orc.intern_i64(FX[f"{name}_src"], FX[f"{name}_dst"])
-        out.append((name, n, s, d))
-    out.append(("diamond", 3, np.array([0, 0, 1, 2]), np.array([1, 2, 2, 0])))
-    out.append(("loops_dups", 3, np.array([0, 0, 1, 2, 1, 0]), np.array([1, 2, 2, 0, 1, 1])))
-    rng = np.random.default_rng(17)
-    out.append(("multigraph", 3000, rng.integers(0, 3000, 40000), rng.integers(0, 3000, 40000)))
-    # One hub row spanning ~15 merge-path tiles (leaves -> 0, with random edges
-    # among the leaves so their ranks differ), plus a long chain out of the hub.
-    leaves = 30000
-    la = rng.integers(1, leaves + 1, leaves)
-    lb = rng.integers(1, leaves + 1, leaves)
-    s = np.concatenate([np.arange(1, leaves + 1), la, [0], np.arange(leaves + 1, leaves + 200)])
-    d = np.concatenate([np.zeros(leaves, np.int64), lb, [leaves + 1], np.arange(leaves + 2, leaves + 201)])
-    out.append(("hub", leaves + 201, s, d))
-    out.append(("isolated_only", 5, np.array([], np.int64), np.array([], np.int64)))
-    return out
-
-
-GRAPHS = graphs()
+GRAPHS = cases()
 IDS = [g[0] for g in GRAPHS]
 
 

@@ -1,0 +1,239 @@
+"""GPU parity for the traversal and intersection kernels, through the C ABI,
+against the CPU oracle on the same inputs.
+
+Bars: BFS levels, k_hop pairs (order included), shortest paths, triangle counts
+and canonical component labels bit-exact; betweenness within 1e-12 relative per
+node.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+from graphs import cases, rmat_dense
+
+pytestmark = pytest.mark.gpu
+
+BC_RTOL = 1e-12
+GRAPHS = cases()
+IDS = [g[0] for g in GRAPHS]
+DIRS = ("out", "in", "both")
+
+
+@pytest.fixture(scope="module")
+def ua():
+    import ursa_amd
+
+    ursa_amd.init(0)
+    return ursa_amd
+
+
+@pytest.fixture(scope="module")
+def rmat14(ua):
+    n, s, d = rmat_dense(14, 5)
+    return n, s, d, ua.Topology.build(n, s, d), orc.Topology(n, s, d)
+
+
+@pytest.fixture(scope="module")
+def rmat12sym(ua):
+    n, s, d = rmat_dense(12, 4, symmetrize=True)
+    return n, s, d, ua.Topology.build(n, s, d), orc.Topology(n, s, d)
+
+
+def sources_for(n):
+    return sorted({0, n // 2, max(n - 1, 0), n // 7}) + [n + 5]
+
+
+# ---------------------------------------------------------------- BFS
+@pytest.mark.parametrize("case", GRAPHS, ids=IDS)
+def test_bfs_levels_match(ua, case):
+    name, n, s, d = case
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    src = sources_for(n)
+    for direction in DIRS:
+        got = ua.bfs_multi(t, src, direction)
+        for i, x in enumerate(src):
+            assert np.array_equal(got[i], orc.bfs(o, x, direction)), (name, direction, x)
+
+
+def test_bfs_rmat_direction_optimizing(ua, rmat14):
+    n, _, _, t, o = rmat14
+    rng = np.random.default_rng(1)
+    src = rng.integers(0, n, 12)
+    for direction in DIRS:
+        got = ua.bfs_multi(t, src, direction)
+        for i, x in enumerate(src):
+            assert np.array_equal(got[i], orc.bfs(o, int(x), direction)), (direction, int(x))
+
+
+def test_bfs_goldens(ua):  # bfs.rs:266-298
+    t = ua.Topology.build(5, [0, 1, 2, 1], [1, 2, 3, 4])
+    assert list(ua.bfs_distances(t, 0)) == [0, 1, 2, 3, 2]
+    d = ua.bfs_distances(t, 3, "both")
+    assert d[0] == 3 and d[4] == 3
+    assert list(ua.bfs_distances(t, 99)) == [-1] * 5
+
+
+# ---------------------------------------------------------------- k_hop / path
+@pytest.mark.parametrize("case", GRAPHS, ids=IDS)
+def test_k_hop_matches_in_order(ua, case):
+    name, n, s, d = case
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    seeds = [0, 1, 1, n // 3, n + 7] if n > 1 else [0, n + 7]
+    for direction in DIRS:
+        for k in (0, 1, 2, 3):
+            ga, gb = ua.k_hop(t, seeds, k, direction)
+            ra, rb = orc.k_hop(o, seeds, k, direction)
+            assert np.array_equal(ga, ra) and np.array_equal(gb, rb), (name, direction, k)
+
+
+def test_k_hop_rmat(ua, rmat14):
+    n, _, _, t, o = rmat14
+    seeds = [3, 77, 1000, 5]
+    for direction in DIRS:
+        ga, gb = ua.k_hop(t, seeds, 3, direction)
+        ra, rb = orc.k_hop(o, seeds, 3, direction)
+        assert np.array_equal(ga, ra) and np.array_equal(gb, rb), direction
+
+
+def test_k_hop_goldens(ua):  # hop.rs:148-191
+    t = ua.Topology.build(5, [0, 1, 2, 1], [1, 2, 3, 4])
+    pairs = lambda r: sorted(zip(r[0].tolist(), r[1].tolist()))
+    assert pairs(ua.k_hop(t, [0], 2)) == [(0, 1), (0, 2), (0, 4)]
+    assert pairs(ua.k_hop(t, [3], 2, "in")) == [(3, 1), (3, 2)]
+    assert ua.k_hop(t, [99], 3)[0].size == 0 and ua.k_hop(t, [0], 0)[0].size == 0
+
+
+@pytest.mark.parametrize("case", GRAPHS, ids=IDS)
+def test_shortest_path_matches(ua, case):
+    name, n, s, d = case
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    rng = np.random.default_rng(3)
+    pairs = [(0, n - 1), (n // 2, 0), (1 % max(n, 1), 1 % max(n, 1)), (0, n + 3)]
+    pairs += [tuple(int(x) for x in rng.integers(0, n, 2)) for _ in range(6)] if n else []
+    for direction in DIRS:
+        for a, b in pairs:
+            g = ua.shortest_path(t, a, b, direction)
+            r = orc.shortest_path(o, a, b, direction)
+            if r is None:
+                assert g is None, (name, direction, a, b)
+            else:
+                assert g is not None and np.array_equal(g, r), (name, direction, a, b)
+
+
+def test_shortest_path_rmat(ua, rmat14):
+    n, _, _, t, o = rmat14
+    rng = np.random.default_rng(9)
+    for _ in range(10):
+        a, b = (int(x) for x in rng.integers(0, n, 2))
+        for direction in DIRS:
+            g, r = ua.shortest_path(t, a, b, direction), orc.shortest_path(o, a, b, direction)
+            assert (g is None and r is None) or np.array_equal(g, r), (a, b, direction)
+
+
+def test_shortest_path_goldens(ua):  # bfs.rs:301-332
+    t = ua.Topology.build(5, [0, 1, 2, 1], [1, 2, 3, 4])
+    assert list(ua.shortest_path(t, 0, 3)) == [0, 1, 2, 3]
+    assert list(ua.shortest_path(t, 2, 2)) == [2]
+    assert ua.shortest_path(t, 3, 0) is None and ua.shortest_path(t, 0, 99) is None
+    assert list(ua.shortest_path(t, 3, 0, "in")) == [3, 2, 1, 0]
+
+
+# ---------------------------------------------------------------- components
+@pytest.mark.parametrize("case", GRAPHS, ids=IDS)
+def test_cc_canonical_labels_match(ua, case):
+    name, n, s, d = case
+    o = orc.Topology(n, s, d)
+    ref = orc.canonicalize(orc.cc_weak(o))
+    t = ua.Topology.build(n, s, d)
+    assert np.array_equal(ua.connected_components_weak(t), ref), name  # no transpose: full sweep
+    t.incoming()
+    assert np.array_equal(ua.connected_components_weak(t), ref), name  # giant-component skip path
+
+
+def test_cc_rmat(ua, rmat14):
+    n, _, _, t, o = rmat14
+    t.incoming()
+    assert np.array_equal(ua.connected_components_weak(t), orc.canonicalize(orc.cc_weak(o)))
+
+
+def test_cc_many_small_components(ua):
+    # 20000 disjoint 3-paths plus isolated pairs: many roots, no giant component
+    k = 20000
+    base = np.arange(k) * 3
+    s = np.concatenate([base, base + 1, np.arange(3 * k, 3 * k + 1000, 2)])
+    d = np.concatenate([base + 1, base + 2, np.arange(3 * k + 1, 3 * k + 1001, 2)])
+    n = 3 * k + 1000
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    assert np.array_equal(ua.connected_components_weak(t), orc.canonicalize(orc.cc_weak(o)))
+
+
+# ---------------------------------------------------------------- triangles
+@pytest.mark.parametrize("case", GRAPHS, ids=IDS)
+def test_triangles_match(ua, case):
+    name, n, s, d = case
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    got = ua.triangle_count(t)
+    assert np.array_equal(got, orc.triangle_count(o)), name
+    assert int(got.sum()) % 3 == 0
+
+
+def test_triangles_rmat_symmetric(ua, rmat12sym):
+    n, _, _, t, o = rmat12sym
+    assert np.array_equal(ua.triangle_count(t), orc.triangle_count(o, nthreads=8))
+
+
+def test_triangles_goldens(ua):  # triangle.rs:70-99
+    t = ua.Topology.build(4, [0, 0, 0, 1, 1, 2], [1, 2, 3, 2, 3, 3])
+    assert list(ua.triangle_count(t)) == [3, 3, 3, 3]
+    t = ua.Topology.build(3, [0, 0, 1, 2, 0, 0], [1, 2, 2, 0, 0, 1])
+    assert list(ua.triangle_count(t)) == [1, 1, 1]
+
+
+def test_triangles_dense_clique(ua):
+    # K_300: every node in C(299, 2) triangles; oriented lists exceed one LDS stage
+    k = 300
+    a, b = np.triu_indices(k, 1)
+    t = ua.Topology.build(k, a, b)
+    assert np.all(ua.triangle_count(t) == (k - 1) * (k - 2) // 2)
+
+
+# ---------------------------------------------------------------- betweenness
+def assert_bc_close(got, ref, name):
+    err = np.abs(got - ref)
+    bad = err > BC_RTOL * np.abs(ref)
+    assert not bad.any(), f"{name}: {bad.sum()} nodes off, worst abs {err.max():.3e}"
+
+
+@pytest.mark.parametrize("case", GRAPHS[:7], ids=IDS[:7])
+def test_betweenness_exact_matches(ua, case):
+    name, n, s, d = case
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    assert_bc_close(ua.betweenness(t), orc.betweenness(o, nthreads=8), name)
+
+
+def test_betweenness_sampled_and_batching(ua, rmat12sym):
+    n, _, _, t, o = rmat12sym
+    ref = orc.betweenness(o, 0.05, 7, nthreads=8)
+    assert_bc_close(ua.betweenness(t, 0.05, 7), ref, "sampled")
+    src = orc.shuffled_order(n, 4)[:256]
+    ref = orc.betweenness_sources(o, src, scale=n / 256, nthreads=8)
+    for batch in (64, 16, 1):
+        assert_bc_close(ua.betweenness_sources(t, src, scale=n / 256, batch=batch), ref, f"batch{batch}")
+
+
+def test_betweenness_goldens(ua):  # betweenness.rs:369-439
+    t = ua.Topology.build(5, [0, 1, 2, 3], [1, 2, 3, 4])
+    assert list(ua.betweenness(t)) == [0.0, 3.0, 4.0, 3.0, 0.0]
+    n = 80
+    t = ua.Topology.build(n, list(range(n)), [(i + 1) % n for i in range(n)])
+    bc = ua.betweenness(t)
+    assert np.all(np.abs(bc - bc[0]) < 1e-9) and abs(bc[0] - sum(L - 1 for L in range(2, n))) < 1e-9
+    t = ua.Topology.build(4, [0, 0, 0, 1, 2], [1, 1, 2, 3, 3])  # parallel edge counts twice
+    bc = ua.betweenness(t)
+    assert abs(bc[1] - 2.0 / 3.0) < 1e-15 and abs(bc[2] - 1.0 / 3.0) < 1e-15

@@ -29,6 +29,14 @@ __all__ = [
     "degree",
     "pagerank",
     "pagerank_weighted",
+    "bfs_distances",
+    "bfs_multi",
+    "k_hop",
+    "shortest_path",
+    "betweenness",
+    "betweenness_sources",
+    "triangle_count",
+    "connected_components_weak",
     "init",
     "profile",
 ]
@@ -163,6 +171,87 @@ def pagerank(topo: Topology, params: PageRankParams = PageRankParams()) -> np.nd
 def pagerank_weighted(topo: Topology, weights, params: PageRankParams = PageRankParams()) -> np.ndarray:
     """pagerank_weighted (ursa-core/src/algo/pagerank.rs:135-169) -> f64[n]."""
     return _pagerank(topo, weights, params)[0]
+
+
+def bfs_multi(topo: Topology, sources, direction=Direction.Out) -> np.ndarray:
+    """bfs_distances from each source -> i32[k, n] (ursa-core/src/algo/bfs.rs:18-42)."""
+    src = _u32(sources)
+    k, n = src.shape[0], topo.n_nodes()
+    out = np.zeros((max(k, 1), max(n, 1)), np.int32)
+    if k and n:
+        check(load().ursa_hip_bfs(topo._h, _ptr(src), k, _dir(direction), _ptr(out)))
+    return out[:k, :n]
+
+
+def bfs_distances(topo: Topology, source: int, direction=Direction.Out) -> np.ndarray:
+    """bfs_distances (ursa-core/src/algo/bfs.rs:18-42) -> i32[n]; -1 = unreachable."""
+    n = topo.n_nodes()
+    if n == 0:
+        return np.zeros(0, np.int32)
+    return bfs_multi(topo, [source], direction)[0]
+
+
+def k_hop(topo: Topology, seeds, k: int, direction=Direction.Out):
+    """k_hop (ursa-core/src/algo/hop.rs:52-75) -> (seeds_out u32[], reached u32[])."""
+    s = _u32(seeds)
+    lib = load()
+    h = C.c_void_p()
+    check(lib.ursa_hip_k_hop(topo._h, _ptr(s) if s.size else None, s.shape[0], k, _dir(direction), C.byref(h)))
+    try:
+        cnt = C.c_uint64(0)
+        check(lib.ursa_hip_pairs_size(h, C.byref(cnt)))
+        a = np.zeros(max(cnt.value, 1), np.uint32)
+        b = np.zeros(max(cnt.value, 1), np.uint32)
+        check(lib.ursa_hip_pairs_export(h, _ptr(a), _ptr(b)))
+    finally:
+        lib.ursa_hip_pairs_destroy(h)
+    return a[: cnt.value], b[: cnt.value]
+
+
+def shortest_path(topo: Topology, source: int, target: int, direction=Direction.Out):
+    """shortest_path (ursa-core/src/algo/bfs.rs:48-102) -> u32[] or None."""
+    cap = max(topo.n_nodes(), 1)
+    buf = np.zeros(cap, np.uint32)
+    ln = C.c_uint64(0)
+    check(load().ursa_hip_shortest_path(topo._h, source, target, _dir(direction), _ptr(buf), cap, C.byref(ln)))
+    return None if ln.value == 0 else buf[: ln.value].copy()
+
+
+def betweenness(topo: Topology, sample=None, seed=None) -> np.ndarray:
+    """betweenness (ursa-core/src/algo/betweenness.rs:153-177) -> f64[n]."""
+    n = topo.n_nodes()
+    out = np.zeros(max(n, 1), np.float64)
+    check(load().ursa_hip_betweenness(topo._h, int(sample is not None), 0.0 if sample is None else float(sample),
+                                      int(seed is not None), 0 if seed is None else int(seed), _ptr(out)))
+    return out[:n]
+
+
+def betweenness_sources(topo: Topology, sources, scale=None, batch: int = 64) -> np.ndarray:
+    """Brandes over explicit sources (betweenness.rs:45-70), optional scale; batch = sources per pass."""
+    n = topo.n_nodes()
+    src = _u32(sources)
+    out = np.zeros(max(n, 1), np.float64)
+    re = C.c_uint64(0)
+    check(load().ursa_hip_betweenness_sources_ex(topo._h, _ptr(src) if src.size else None, src.shape[0],
+                                                 int(scale is not None), 1.0 if scale is None else float(scale),
+                                                 batch, _ptr(out), C.byref(re)))
+    return out[:n]
+
+
+def triangle_count(topo: Topology) -> np.ndarray:
+    """triangle_count (ursa-core/src/algo/triangle.rs:58-63) -> u64[n]."""
+    n = topo.n_nodes()
+    out = np.zeros(max(n, 1), np.uint64)
+    check(load().ursa_hip_triangle_count(topo._h, _ptr(out)))
+    return out[:n]
+
+
+def connected_components_weak(topo: Topology) -> np.ndarray:
+    """connected_components_weak (ursa-core/src/algo/components.rs:58-68) -> u32[n] (min-id labels)."""
+    n = topo.n_nodes()
+    out = np.zeros(max(n, 1), np.uint32)
+    check(load().ursa_hip_cc_weak(topo._h, _ptr(out)))
+    return out[:n]
 
 
 class profile:

@@ -46,6 +46,18 @@ SIGNATURES = {
     "ursa_hip_topology_export": (cint, [P, cint, P, P, P]),
     "ursa_hip_degree": (cint, [P, cint, P]),
     "ursa_hip_pagerank": (cint, [P, P, f64, u32, f64, P, C.POINTER(u32)]),
+    "ursa_hip_bfs": (cint, [P, P, u32, cint, P]),
+    "ursa_hip_bfs_ex": (cint, [P, P, u32, cint, P, C.POINTER(u64)]),
+    "ursa_hip_k_hop": (cint, [P, P, u64, u32, cint, C.POINTER(P)]),
+    "ursa_hip_pairs_size": (cint, [P, C.POINTER(u64)]),
+    "ursa_hip_pairs_export": (cint, [P, P, P]),
+    "ursa_hip_pairs_destroy": (cint, [P]),
+    "ursa_hip_shortest_path": (cint, [P, u32, u32, cint, P, u64, C.POINTER(u64)]),
+    "ursa_hip_betweenness": (cint, [P, cint, f64, cint, u64, P]),
+    "ursa_hip_betweenness_sources": (cint, [P, P, u64, cint, f64, P]),
+    "ursa_hip_betweenness_sources_ex": (cint, [P, P, u64, cint, f64, u32, P, C.POINTER(u64)]),
+    "ursa_hip_triangle_count": (cint, [P, P]),
+    "ursa_hip_cc_weak": (cint, [P, P]),
     "ursa_hip_profile_enable": (cint, [cint]),
     "ursa_hip_profile_reset": (cint, []),
     "ursa_hip_profile_query": (cint, [C.c_char_p, C.POINTER(f64), C.POINTER(u64), C.POINTER(f64)]),

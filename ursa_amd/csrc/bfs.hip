@@ -1,0 +1,342 @@
+// bfs.hip — direction-optimizing BFS hop levels (bfs_distances,
+// ursa-core/src/algo/bfs.rs:18-42).
+//
+// Level-synchronous, switching per level between
+//   top-down  (k_bfs_td): the frontier is a queue; each wave takes 64 queue
+//             vertices, prefix-sums their degrees and walks the combined edge
+//             list 64 edges at a time (owner found by a 6-step shuffle search),
+//             claims unvisited targets with a CAS on level[] and appends them
+//             to the next queue with one atomic per wave;
+//   bottom-up (k_bfs_bu): the frontier is a bitmap; each unvisited vertex scans
+//             its reverse neighbours and stops at the first one in the
+//             frontier; each wave owns 64 consecutive vertices, so it writes its
+//             two next-frontier bitmap words whole (ballot) with no atomics.
+// The switch follows Beamer et al.: go bottom-up when the frontier's edge count
+// exceeds the unexplored edges / 15, go back top-down once the frontier shrinks
+// below n / 18. Hop levels are unique, so every schedule gives the same i32
+// output as the reference's serial frontier loop.
+//
+// Directions (topology.rs:395-404): Out expands out-edges and checks
+// in-neighbours bottom-up; In the reverse; Both concatenates out and in for both.
+//
+// Algorithmic bytes per source: 4 B per inspected edge (E_reach = sum of the
+// forward degrees of reached vertices) + 12 B per node (offsets, level write,
+// level/visited read).
+#include "frontier.hpp"
+
+namespace ursa {
+
+namespace {
+
+constexpr int BFS_BLOCK = 256;
+constexpr int WAVES = BFS_BLOCK / 64;
+
+struct BfsCtr {
+    uint32_t qlen;   // next-queue length (top-down) / newly visited count (bottom-up)
+    uint32_t pad;
+    unsigned long long edges;  // forward degree sum of the newly visited vertices
+};
+
+__global__ void k_bfs_init(int32_t* level, uint64_t n, uint32_t src, uint32_t* queue, BfsCtr* ctr, NbrView fwd) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        level[i] = (i == src) ? 0 : -1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        queue[0] = src;
+        ctr->qlen = 1;
+        ctr->edges = nv_degree(fwd, src);
+    }
+}
+
+__global__ void k_ctr_reset(BfsCtr* ctr) {
+    ctr->qlen = 0;
+    ctr->edges = 0;
+}
+
+__global__ __launch_bounds__(BFS_BLOCK) void k_bfs_td(const uint32_t* __restrict__ q, uint32_t qlen, NbrView fwd,
+                                                      int32_t* __restrict__ level, int32_t L,
+                                                      uint32_t* __restrict__ nq, BfsCtr* ctr) {
+    const int lane = lane_id();
+    const uint64_t gw = blockIdx.x * (uint64_t)WAVES + (threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
+    uint64_t edges = 0;
+    for (uint64_t base = gw * 64; base < qlen; base += nwaves * 64) {
+        const uint64_t idx = base + lane;
+        uint32_t u = 0, d = 0, da = 0, sa = 0, sb = 0;
+        if (idx < qlen) {
+            u = q[idx];
+            d = nv_degree(fwd, u, da);
+            sa = fwd.off_a[u];
+            if (fwd.off_b) sb = fwd.off_b[u];
+        }
+        const uint32_t incl = wave_incl_scan(d);
+        const uint32_t excl = incl - d;
+        const uint32_t total = __shfl(incl, 63, 64);
+        for (uint32_t t0 = 0; t0 < total; t0 += 64) {
+            const uint32_t t = t0 + lane;
+            const int j = wave_owner(excl, t);
+            const uint32_t e = t - __shfl(excl, j, 64);
+            const uint32_t oa = __shfl(da, j, 64);
+            const uint32_t osa = __shfl(sa, j, 64);
+            const uint32_t osb = __shfl(sb, j, 64);
+            bool claimed = false;
+            uint32_t v = 0;
+            if (t < total) {
+                v = e < oa ? fwd.tgt_a[osa + e] : fwd.tgt_b[osb + (e - oa)];
+                if (level[v] < 0) claimed = atomicCAS(&level[v], -1, L) == -1;
+            }
+            const uint32_t slot = wave_append_slot(claimed, &ctr->qlen);
+            if (claimed) {
+                nq[slot] = v;
+                edges += nv_degree(fwd, v);
+            }
+        }
+    }
+    edges = wave_sum_u64(edges);
+    if (lane == 0 && edges) atomicAdd(&ctr->edges, edges);
+}
+
+// Bottom-up step: vertices of level L are those unvisited with a reverse
+// neighbour in `front` (level L-1). Writes the whole next bitmap.
+__global__ __launch_bounds__(BFS_BLOCK) void k_bfs_bu(uint64_t n, NbrView rev, NbrView fwd,
+                                                      const uint32_t* __restrict__ front,
+                                                      uint32_t* __restrict__ next, int32_t* __restrict__ level,
+                                                      int32_t L, BfsCtr* ctr) {
+    const int lane = lane_id();
+    const uint64_t gw = blockIdx.x * (uint64_t)WAVES + (threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
+    uint64_t edges = 0;
+    uint32_t awake = 0;
+    for (uint64_t base = gw * 64; base < n; base += nwaves * 64) {
+        const uint64_t v = base + lane;
+        bool found = false;
+        if (v < n && level[v] < 0) {
+            uint32_t k = rev.off_a[v], ke = rev.off_a[v + 1];
+            for (; k < ke && !found; ++k) {
+                const uint32_t w = rev.tgt_a[k];
+                found = (front[w >> 5] >> (w & 31)) & 1u;
+            }
+            if (!found && rev.off_b) {
+                k = rev.off_b[v];
+                ke = rev.off_b[v + 1];
+                for (; k < ke && !found; ++k) {
+                    const uint32_t w = rev.tgt_b[k];
+                    found = (front[w >> 5] >> (w & 31)) & 1u;
+                }
+            }
+            if (found) {
+                level[v] = L;
+                edges += nv_degree(fwd, (uint32_t)v);
+            }
+        }
+        const uint64_t mask = __ballot(found);
+        if (lane == 0) {
+            next[base >> 5] = (uint32_t)mask;
+            next[(base >> 5) + 1] = (uint32_t)(mask >> 32);
+            awake += (uint32_t)__popcll(mask);
+        }
+    }
+    edges = wave_sum_u64(edges);
+    if (lane == 0) {
+        if (edges) atomicAdd(&ctr->edges, edges);
+        if (awake) atomicAdd(&ctr->qlen, awake);
+    }
+}
+
+// front bitmap of the vertices at level L (whole words per wave).
+__global__ __launch_bounds__(BFS_BLOCK) void k_level_to_bits(const int32_t* __restrict__ level, uint64_t n, int32_t L,
+                                                             uint32_t* __restrict__ bits) {
+    const int lane = lane_id();
+    const uint64_t gw = blockIdx.x * (uint64_t)WAVES + (threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
+    for (uint64_t base = gw * 64; base < n; base += nwaves * 64) {
+        const uint64_t v = base + lane;
+        const uint64_t mask = __ballot(v < n && level[v] == L);
+        if (lane == 0) {
+            bits[base >> 5] = (uint32_t)mask;
+            bits[(base >> 5) + 1] = (uint32_t)(mask >> 32);
+        }
+    }
+}
+
+// queue of the vertices at level L.
+__global__ __launch_bounds__(BFS_BLOCK) void k_level_to_queue(const int32_t* __restrict__ level, uint64_t n, int32_t L,
+                                                              uint32_t* __restrict__ q, BfsCtr* ctr) {
+    const uint64_t gw = blockIdx.x * (uint64_t)WAVES + (threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
+    for (uint64_t base = gw * 64; base < n; base += nwaves * 64) {
+        const uint64_t v = base + lane_id();
+        const bool take = v < n && level[v] == L;
+        const uint32_t slot = wave_append_slot(take, &ctr->qlen);
+        if (take) q[slot] = (uint32_t)v;
+    }
+}
+
+__global__ void k_fill_i32(int32_t* p, uint64_t n, int32_t v) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+}  // namespace
+
+// Direction views of a topology (forward = expansion direction, reverse = the
+// direction bottom-up checks).
+void direction_views(ursa_hip_topo* t, int dir, hipStream_t s, NbrView& fwd, NbrView& rev) {
+    const DevCsr& out = t->out;
+    const DevCsr& in = t->incoming(s);
+    const NbrView vo{out.off.get(), out.tgt.get(), nullptr, nullptr};
+    const NbrView vi{in.off.get(), in.tgt.get(), nullptr, nullptr};
+    const NbrView vb{out.off.get(), out.tgt.get(), in.off.get(), in.tgt.get()};
+    if (dir == URSA_HIP_OUT) {
+        fwd = vo;
+        rev = vi;
+    } else if (dir == URSA_HIP_IN) {
+        fwd = vi;
+        rev = vo;
+    } else {
+        fwd = vb;
+        rev = vb;
+    }
+}
+
+struct BfsWork {
+    DevBuf<uint32_t> q0, q1, b0, b1;
+    DevBuf<BfsCtr> ctr;
+    BfsCtr* h = nullptr;  // pinned mirror of ctr
+    explicit BfsWork(uint64_t n) : q0(n ? n : 1), q1(n ? n : 1), b0(((n + 63) / 64) * 2), b1(((n + 63) / 64) * 2), ctr(1) {
+        HIP_OK(hipHostMalloc((void**)&h, sizeof(BfsCtr), hipHostMallocDefault));
+    }
+    ~BfsWork() {
+        if (h) (void)hipHostFree(h);
+    }
+};
+
+// One direction-optimizing BFS into level (device, n entries). Returns the
+// number of levels run; *reached_edges = sum of forward degrees of reached
+// vertices (the TEPS numerator).
+static void bfs_one(ursa_hip_topo* t, uint32_t src, const NbrView& fwd, const NbrView& rev, uint64_t fwd_edges,
+                    int32_t* level, BfsWork& w, hipStream_t s, uint64_t* reached_edges) {
+    const uint64_t n = t->n;
+    const unsigned gb = blocks_for((n + 63) / 64, WAVES, 1u << 16);
+    auto read_ctr = [&]() {
+        HIP_OK(hipMemcpyAsync(w.h, w.ctr.get(), sizeof(BfsCtr), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        return *w.h;
+    };
+    k_bfs_init<<<blocks_for(n, 256), 256, 0, s>>>(level, n, src, w.q0.get(), w.ctr.get(), fwd);
+    check_launch();
+    BfsCtr c = read_ctr();
+    uint32_t* q = w.q0.get();
+    uint32_t* nq = w.q1.get();
+    uint32_t* front = w.b0.get();
+    uint32_t* next = w.b1.get();
+    uint64_t qlen = c.qlen;
+    uint64_t scout = c.edges;
+    uint64_t total_edges = scout;
+    int64_t edges_to_check = (int64_t)fwd_edges;
+    int32_t L = 1;
+    const double alpha = 15.0, beta = 18.0;
+    const double bytes_node = 12.0 * (double)n;
+    while (qlen > 0) {
+        if ((double)scout > (double)edges_to_check / alpha) {
+            // bottom-up phase
+            k_level_to_bits<<<gb, BFS_BLOCK, 0, s>>>(level, n, L - 1, front);
+            check_launch();
+            uint64_t awake = qlen, old_awake;
+            do {
+                old_awake = awake;
+                k_ctr_reset<<<1, 1, 0, s>>>(w.ctr.get());
+                timed("bfs_bottom_up", bytes_node, s, [&] {
+                    k_bfs_bu<<<gb, BFS_BLOCK, 0, s>>>(n, rev, fwd, front, next, level, L, w.ctr.get());
+                });
+                check_launch();
+                c = read_ctr();
+                awake = c.qlen;
+                total_edges += c.edges;
+                std::swap(front, next);
+                ++L;
+            } while (awake > 0 && (awake >= old_awake || (double)awake > (double)n / beta));
+            if (awake == 0) break;
+            k_ctr_reset<<<1, 1, 0, s>>>(w.ctr.get());
+            k_level_to_queue<<<gb, BFS_BLOCK, 0, s>>>(level, n, L - 1, q, w.ctr.get());
+            check_launch();
+            c = read_ctr();
+            qlen = c.qlen;
+            scout = 1;
+        } else {
+            edges_to_check -= (int64_t)scout;
+            k_ctr_reset<<<1, 1, 0, s>>>(w.ctr.get());
+            const unsigned tb = blocks_for((qlen + 63) / 64, WAVES, 1u << 16);
+            timed("bfs_top_down", 4.0 * (double)scout + 8.0 * (double)qlen, s, [&] {
+                k_bfs_td<<<tb, BFS_BLOCK, 0, s>>>(q, (uint32_t)qlen, fwd, level, L, nq, w.ctr.get());
+            });
+            check_launch();
+            c = read_ctr();
+            qlen = c.qlen;
+            scout = c.edges;
+            total_edges += c.edges;
+            std::swap(q, nq);
+            ++L;
+        }
+    }
+    if (reached_edges) *reached_edges = total_edges;
+}
+
+void bfs_levels(ursa_hip_topo* t, const uint32_t* sources, uint64_t k, int dir, int32_t* out_user,
+                uint64_t* reached_edges) {
+    const uint64_t n = t->n;
+    if (reached_edges) *reached_edges = 0;
+    if (n == 0 || k == 0) return;
+    hipStream_t s = stream();
+    NbrView fwd, rev;
+    direction_views(t, dir, s, fwd, rev);
+    const uint64_t fwd_edges = dir == URSA_HIP_BOTH ? 2 * t->m : t->m;
+    DevOut<int32_t> out(out_user, n * k);
+    BfsWork w(n);
+    std::vector<uint32_t> src(k);
+    if (is_device_ptr(sources))
+        HIP_OK(hipMemcpy(src.data(), sources, k * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    else
+        std::copy(sources, sources + k, src.begin());
+    uint64_t sum = 0;
+    for (uint64_t i = 0; i < k; ++i) {
+        int32_t* lv = out.p + i * n;
+        if (src[i] >= n) {  // bfs.rs:21-23: out-of-range source -> all -1
+            k_fill_i32<<<blocks_for(n, 256), 256, 0, s>>>(lv, n, -1);
+            check_launch();
+            continue;
+        }
+        uint64_t re = 0;
+        bfs_one(t, src[i], fwd, rev, fwd_edges, lv, w, s, &re);
+        sum += re;
+    }
+    out.finish(s);
+    if (reached_edges) *reached_edges = sum;
+}
+
+}  // namespace ursa
+
+extern "C" {
+
+int ursa_hip_bfs(ursa_hip_topo* t, const uint32_t* sources, uint32_t k, int direction, int32_t* levels) {
+    return ursa::guard([&] {
+        URSA_REQUIRE(t != nullptr, URSA_HIP_EINVAL, "topology is NULL");
+        URSA_REQUIRE(direction >= 0 && direction <= 2, URSA_HIP_EINVAL, "bad direction");
+        if (t->n == 0 || k == 0) return;
+        URSA_REQUIRE(sources != nullptr && levels != nullptr, URSA_HIP_EINVAL, "sources/levels is NULL");
+        ursa::bfs_levels(t, sources, k, direction, levels, nullptr);
+    });
+}
+
+int ursa_hip_bfs_ex(ursa_hip_topo* t, const uint32_t* sources, uint32_t k, int direction, int32_t* levels,
+                    uint64_t* reached_edges) {
+    return ursa::guard([&] {
+        URSA_REQUIRE(t != nullptr, URSA_HIP_EINVAL, "topology is NULL");
+        URSA_REQUIRE(direction >= 0 && direction <= 2, URSA_HIP_EINVAL, "bad direction");
+        if (reached_edges) *reached_edges = 0;
+        if (t->n == 0 || k == 0) return;
+        URSA_REQUIRE(sources != nullptr && levels != nullptr, URSA_HIP_EINVAL, "sources/levels is NULL");
+        ursa::bfs_levels(t, sources, k, direction, levels, reached_edges);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,340 @@
+// brandes.hip — Brandes betweenness (betweenness, ursa-core/src/algo/
+// betweenness.rs:45-230, 349-362) over out-edges, sigma and delta in f64.
+//
+// Sources are processed in batches of up to 64, which is exactly the
+// reference's accumulation chunk (SRC_CHUNK, betweenness.rs:46): a batch's
+// per-source dependencies are added into a chunk partial in source order, and
+// the chunk partials are added into bc in chunk order, as accumulate_sources
+// does (betweenness.rs:63-68). Within a batch all sources advance level by level
+// together, so one launch per level serves up to 64 traversals.
+//
+// Forward (one launch per level): the level's (source, vertex) entries are
+// expanded wave-cooperatively (degree prefix-sum over 64 lanes); an unvisited
+// target is claimed with a CAS on its distance, and every out-slot from level
+// L-1 into level L adds sigma[u] to sigma[v] (betweenness.rs:204-214: per CSR
+// slot, so parallel edges count as distinct paths). Path counts are integers
+// held in f64; their sums are exact below 2^53 and so independent of the order
+// of the atomic adds.
+// Backward (one launch per level, deepest first): each vertex v at level L-1
+// pulls delta[v] = sum over out-slots w at level L of
+// sigma[v] * ((1 + delta[w]) / sigma[w]) — the same per-term arithmetic as
+// reverse_accumulate (betweenness.rs:89-97), summed in out-CSR slot order.
+// Vertices with more than 128 out-slots are summed by a whole wave with a
+// fixed-shape reduction, so results are bit-identical run to run.
+//
+// Algorithmic bytes per source: 32 B per reached out-edge (forward column +
+// distance, backward column + distance + sigma + delta) + 44 B per node.
+#include <algorithm>
+#include <vector>
+
+#include "frontier.hpp"
+#include "rng.hpp"
+
+namespace ursa {
+
+namespace {
+
+constexpr int BB = 256;
+constexpr int BW = BB / 64;
+constexpr uint32_t HEAVY_OUT = 128;
+
+struct BrCtr {
+    uint32_t len;
+    uint32_t heavy;
+    unsigned long long edges;
+};
+
+__global__ void k_br_init(int32_t* dist, double* sigma, double* delta, uint64_t total) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+        dist[i] = -1;
+        sigma[i] = 0.0;
+        delta[i] = 0.0;
+    }
+}
+
+__global__ void k_br_seed(const uint32_t* src, uint32_t nb, uint64_t n, int32_t* dist, double* sigma,
+                          unsigned long long* q, BrCtr* ctr) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    uint32_t len = 0;
+    for (uint32_t b = 0; b < nb; ++b) {
+        const uint32_t s = src[b];
+        if (s >= n) continue;
+        dist[b * n + s] = 0;
+        sigma[b * n + s] = 1.0;
+        q[len++] = ((unsigned long long)b << 32) | s;
+    }
+    ctr->len = len;
+}
+
+__global__ __launch_bounds__(BB) void k_br_forward(const unsigned long long* __restrict__ q, uint32_t qlen,
+                                                   const uint32_t* __restrict__ off, const uint32_t* __restrict__ tgt,
+                                                   uint64_t n, int32_t* __restrict__ dist, double* __restrict__ sigma,
+                                                   int32_t L, unsigned long long* __restrict__ nq, BrCtr* ctr) {
+    const int lane = lane_id();
+    const uint64_t gw = blockIdx.x * (uint64_t)BW + (threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * BW;
+    unsigned long long edges = 0;
+    for (uint64_t base = gw * 64; base < qlen; base += nwaves * 64) {
+        const uint64_t idx = base + lane;
+        uint32_t d = 0, st = 0, b = 0;
+        double su = 0.0;
+        if (idx < qlen) {
+            const unsigned long long e = q[idx];
+            b = (uint32_t)(e >> 32);
+            const uint32_t u = (uint32_t)e;
+            st = off[u];
+            d = off[u + 1] - st;
+            su = sigma[b * n + u];
+        }
+        edges += d;
+        const uint32_t incl = wave_incl_scan(d);
+        const uint32_t excl = incl - d;
+        const uint32_t total = __shfl(incl, 63, 64);
+        for (uint32_t t0 = 0; t0 < total; t0 += 64) {
+            const uint32_t t = t0 + lane;
+            const int j = wave_owner(excl, t);
+            const uint32_t e = t - __shfl(excl, j, 64);
+            const uint32_t ob = __shfl(b, j, 64);
+            const uint32_t ost = __shfl(st, j, 64);
+            const double osu = __shfl(su, j, 64);
+            bool claimed = false;
+            uint32_t v = 0;
+            if (t < total) {
+                v = tgt[ost + e];
+                int32_t* dv = &dist[ob * n + v];
+                int32_t cur = *dv;
+                if (cur < 0) {
+                    const int32_t old = atomicCAS(dv, -1, L);
+                    claimed = old == -1;
+                    cur = claimed ? L : old;
+                }
+                if (cur == L) atomicAdd(&sigma[ob * n + v], osu);
+            }
+            const uint32_t slot = wave_append_slot(claimed, &ctr->len);
+            if (claimed) nq[slot] = ((unsigned long long)ob << 32) | v;
+        }
+    }
+    edges = wave_sum_u64(edges);
+    if (lane == 0 && edges) atomicAdd(&ctr->edges, edges);
+}
+
+// delta for the entries of one level; heavy entries are deferred to a wave each.
+__global__ void k_br_backward(const unsigned long long* __restrict__ q, uint32_t qlen,
+                              const uint32_t* __restrict__ off, const uint32_t* __restrict__ tgt, uint64_t n,
+                              const int32_t* __restrict__ dist, const double* __restrict__ sigma,
+                              double* __restrict__ delta, int32_t L, uint32_t* __restrict__ heavy, BrCtr* ctr) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < qlen; i += (uint64_t)gridDim.x * blockDim.x) {
+        const unsigned long long e = q[i];
+        const uint64_t b = e >> 32;
+        const uint32_t v = (uint32_t)e;
+        const uint32_t a = off[v], z = off[v + 1];
+        if (z - a > HEAVY_OUT) {
+            heavy[atomicAdd(&ctr->heavy, 1u)] = (uint32_t)i;
+            continue;
+        }
+        const uint64_t bo = b * n;
+        const double sv = sigma[bo + v];
+        double acc = 0.0;
+        for (uint32_t k = a; k < z; ++k) {
+            const uint32_t w = tgt[k];
+            if (dist[bo + w] == L) acc += sv * ((1.0 + delta[bo + w]) / sigma[bo + w]);
+        }
+        delta[bo + v] = acc;
+    }
+}
+
+__global__ __launch_bounds__(BB) void k_br_backward_heavy(const unsigned long long* __restrict__ q,
+                                                          const uint32_t* __restrict__ heavy, const BrCtr* ctr,
+                                                          const uint32_t* __restrict__ off,
+                                                          const uint32_t* __restrict__ tgt, uint64_t n,
+                                                          const int32_t* __restrict__ dist,
+                                                          const double* __restrict__ sigma,
+                                                          double* __restrict__ delta, int32_t L) {
+    const uint64_t gw = blockIdx.x * (uint64_t)BW + (threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * BW;
+    const uint32_t cnt = ctr->heavy;
+    for (uint64_t h = gw; h < cnt; h += nwaves) {
+        const unsigned long long e = q[heavy[h]];
+        const uint64_t bo = (e >> 32) * n;
+        const uint32_t v = (uint32_t)e;
+        const double sv = sigma[bo + v];
+        double acc = 0.0;
+        for (uint32_t k = off[v] + lane_id(); k < off[v + 1]; k += 64) {
+            const uint32_t w = tgt[k];
+            if (dist[bo + w] == L) acc += sv * ((1.0 + delta[bo + w]) / sigma[bo + w]);
+        }
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+        if (lane_id() == 0) delta[bo + v] = acc;
+    }
+}
+
+// chunk partial in source order (betweenness.rs:57-60, :94-96), then
+// bc += partial in chunk order (betweenness.rs:63-68).
+__global__ void k_br_accumulate(const double* __restrict__ delta, const int32_t* __restrict__ dist, uint32_t nb,
+                                uint64_t n, double* __restrict__ bc) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (uint32_t b = 0; b < nb; ++b)
+            if (dist[b * n + v] > 0) acc += delta[b * n + v];
+        bc[v] += acc;
+    }
+}
+
+__global__ void k_add(const double* __restrict__ a, uint64_t n, double* __restrict__ bc) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
+        bc[v] += a[v];
+}
+
+__global__ void k_scale(double* p, uint64_t n, double s) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = p[i] * s;
+}
+
+}  // namespace
+
+// bc over an explicit source list; `batch` sources advance together (1..64).
+void betweenness_sources(ursa_hip_topo* t, const uint32_t* sources_user, uint64_t k, bool apply_scale, double scale,
+                         uint32_t batch, double* out_user, uint64_t* reached_edges) {
+    const uint64_t n = t->n;
+    if (reached_edges) *reached_edges = 0;
+    if (n == 0) return;
+    hipStream_t s = stream();
+    DevOut<double> out(out_user, n);
+    HIP_OK(hipMemsetAsync(out.p, 0, n * sizeof(double), s));
+    std::vector<uint32_t> src(k);
+    if (k) {
+        if (is_device_ptr(sources_user))
+            HIP_OK(hipMemcpy(src.data(), sources_user, k * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        else
+            std::copy(sources_user, sources_user + k, src.begin());
+    }
+    batch = std::max<uint32_t>(1, std::min<uint32_t>(batch, 64));
+    const uint32_t B = (uint32_t)std::min<uint64_t>(batch, std::max<uint64_t>(k, 1));
+    const DevCsr& o = t->out;
+    DevBuf<int32_t> dist(B * n);
+    DevBuf<double> sigma(B * n), delta(B * n);
+    DevBuf<unsigned long long> order(B * n);
+    DevBuf<uint32_t> heavy(B * n), dsrc(B);
+    DevBuf<BrCtr> ctr(1);
+    BrCtr* h = nullptr;
+    HIP_OK(hipHostMalloc((void**)&h, sizeof(BrCtr), hipHostMallocDefault));
+    auto read = [&]() {
+        HIP_OK(hipMemcpyAsync(h, ctr.get(), sizeof(BrCtr), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        return *h;
+    };
+    uint64_t total_edges = 0;
+    try {
+        // Batches never straddle a 64-source chunk, so with batch 64 one batch is
+        // one reference chunk; smaller batches are summed into the same chunk
+        // partial in source order.
+        DevBuf<double> partial(n);
+        for (uint64_t c0 = 0; c0 < k; c0 += 64) {
+            const uint64_t c1 = std::min<uint64_t>(k, c0 + 64);
+            const bool chunk_is_batch = (c1 - c0) <= B;
+            double* acc_into = chunk_is_batch ? out.p : partial.get();
+            if (!chunk_is_batch) HIP_OK(hipMemsetAsync(partial.get(), 0, n * sizeof(double), s));
+            for (uint64_t b0 = c0; b0 < c1; b0 += B) {
+                const uint32_t nb = (uint32_t)std::min<uint64_t>(B, c1 - b0);
+                k_br_init<<<blocks_for(nb * n, 256, 1u << 16), 256, 0, s>>>(dist.get(), sigma.get(), delta.get(),
+                                                                           nb * n);
+                HIP_OK(hipMemcpyAsync(dsrc.get(), src.data() + b0, nb * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+                HIP_OK(hipMemsetAsync(ctr.get(), 0, sizeof(BrCtr), s));
+                k_br_seed<<<1, 1, 0, s>>>(dsrc.get(), nb, n, dist.get(), sigma.get(), order.get(), ctr.get());
+                check_launch();
+                std::vector<uint64_t> lvl{0, read().len};
+                for (int32_t L = 1;; ++L) {
+                    const uint64_t a = lvl[L - 1], z = lvl[L];
+                    if (z == a) break;
+                    HIP_OK(hipMemsetAsync(ctr.get(), 0, sizeof(BrCtr), s));
+                    const uint32_t qlen = (uint32_t)(z - a);
+                    timed("brandes_forward", 8.0 * (double)qlen, s, [&] {
+                        k_br_forward<<<blocks_for((qlen + 63) / 64, BW, 1u << 16), BB, 0, s>>>(
+                            order.get() + a, qlen, o.off.get(), o.tgt.get(), n, dist.get(), sigma.get(), L,
+                            order.get() + z, ctr.get());
+                    });
+                    check_launch();
+                    const BrCtr c = read();
+                    total_edges += c.edges;
+                    lvl.push_back(z + c.len);
+                }
+                // Backward: levels deepest-1 .. 1 (level-0 sources need no delta).
+                const int32_t levels = (int32_t)lvl.size() - 1;
+                for (int32_t L = levels - 1; L >= 1; --L) {
+                    const uint64_t a = lvl[L], z = lvl[L + 1];
+                    if (z == a) continue;
+                    const uint32_t qlen = (uint32_t)(z - a);
+                    HIP_OK(hipMemsetAsync(ctr.get(), 0, sizeof(BrCtr), s));
+                    timed("brandes_backward", 0.0, s, [&] {
+                        k_br_backward<<<blocks_for(qlen, 256, 1u << 16), 256, 0, s>>>(
+                            order.get() + a, qlen, o.off.get(), o.tgt.get(), n, dist.get(), sigma.get(), delta.get(),
+                            L + 1, heavy.get(), ctr.get());
+                        k_br_backward_heavy<<<blocks_for(qlen, BW, 1u << 14), BB, 0, s>>>(
+                            order.get() + a, heavy.get(), ctr.get(), o.off.get(), o.tgt.get(), n, dist.get(),
+                            sigma.get(), delta.get(), L + 1);
+                    });
+                    check_launch();
+                }
+                k_br_accumulate<<<blocks_for(n, 256, 1u << 16), 256, 0, s>>>(delta.get(), dist.get(), nb, n, acc_into);
+                check_launch();
+            }
+            if (!chunk_is_batch) {
+                // bc += partial (chunk order)
+                k_add<<<blocks_for(n, 256, 1u << 16), 256, 0, s>>>(partial.get(), n, out.p);
+                check_launch();
+            }
+        }
+    } catch (...) {
+        (void)hipHostFree(h);
+        throw;
+    }
+    (void)hipHostFree(h);
+    if (apply_scale) {
+        k_scale<<<blocks_for(n, 256, 1u << 16), 256, 0, s>>>(out.p, n, scale);
+        check_launch();
+    }
+    out.finish(s);
+    if (reached_edges) *reached_edges = total_edges;
+}
+
+}  // namespace ursa
+
+extern "C" {
+
+int ursa_hip_betweenness_sources(ursa_hip_topo* t, const uint32_t* sources, uint64_t k, int apply_scale,
+                                 double scale, double* out) {
+    return ursa::guard([&] {
+        URSA_REQUIRE(t != nullptr, URSA_HIP_EINVAL, "topology is NULL");
+        if (t->n == 0) return;
+        URSA_REQUIRE(out != nullptr, URSA_HIP_EINVAL, "out is NULL");
+        URSA_REQUIRE(k == 0 || sources != nullptr, URSA_HIP_EINVAL, "sources is NULL");
+        ursa::betweenness_sources(t, sources, k, apply_scale != 0, scale, 64, out, nullptr);
+    });
+}
+
+// betweenness.rs:153-177: sources from sample_sources, n/k scaling when sampled.
+int ursa_hip_betweenness(ursa_hip_topo* t, int has_sample, double sample, int has_seed, uint64_t seed, double* out) {
+    return ursa::guard([&] {
+        URSA_REQUIRE(t != nullptr, URSA_HIP_EINVAL, "topology is NULL");
+        const uint64_t n = t->n;
+        if (n == 0) return;  // betweenness.rs:155-157
+        URSA_REQUIRE(out != nullptr, URSA_HIP_EINVAL, "out is NULL");
+        std::vector<uint32_t> src = ursa::sample_sources(n, has_sample != 0, sample, has_seed != 0, seed);
+        const uint64_t k = src.size();
+        ursa::betweenness_sources(t, src.data(), k, has_sample != 0, (double)n / (double)k, 64, out, nullptr);
+    });
+}
+
+int ursa_hip_betweenness_sources_ex(ursa_hip_topo* t, const uint32_t* sources, uint64_t k, int apply_scale,
+                                    double scale, uint32_t batch, double* out, uint64_t* reached_edges) {
+    return ursa::guard([&] {
+        URSA_REQUIRE(t != nullptr, URSA_HIP_EINVAL, "topology is NULL");
+        if (reached_edges) *reached_edges = 0;
+        if (t->n == 0) return;
+        URSA_REQUIRE(out != nullptr, URSA_HIP_EINVAL, "out is NULL");
+        URSA_REQUIRE(k == 0 || sources != nullptr, URSA_HIP_EINVAL, "sources is NULL");
+        ursa::betweenness_sources(t, sources, k, apply_scale != 0, scale, batch, out, reached_edges);
+    });
+}
+
+}  // extern "C"

@@ -129,6 +129,11 @@ void build_grouped(uint64_t n, uint64_t m, const uint32_t* keys, const uint32_t*
     out.built = true;
 }
 
+void offsets_from_sorted(const uint32_t* keys, uint64_t m, uint64_t n, uint32_t* off, hipStream_t s) {
+    k_offsets_from_sorted<<<blocks_for(m + 1, B), B, 0, s>>>(keys, m, n, off);
+    check_launch();
+}
+
 void slot_owners(uint64_t n, const DevCsr& csr, uint32_t* owner, hipStream_t s) {
     uint64_t m = csr.slots;
     DevBuf<uint32_t> flags(m);

@@ -32,6 +32,9 @@ void build_grouped(uint64_t n, uint64_t m, const uint32_t* keys, const uint32_t*
 // owner[k] = node whose segment holds slot k of `csr` (needs csr.slots > 0).
 void slot_owners(uint64_t n, const DevCsr& csr, uint32_t* owner, hipStream_t s);
 
+// off[u] = first index whose key >= u (u in [0, n]) for keys sorted ascending.
+void offsets_from_sorted(const uint32_t* keys, uint64_t m, uint64_t n, uint32_t* off, hipStream_t s);
+
 }  // namespace ursa
 
 struct ursa_hip_topo {
@@ -44,6 +47,11 @@ struct ursa_hip_topo {
     ursa::DevBuf<uint32_t> pr_tile_x, pr_tile_y;
     uint64_t pr_tiles = 0;
     int pr_tile_items = 0;
+    // Degree-oriented undirected DAG for triangle counting (see triangle.hip).
+    ursa::DevBuf<uint32_t> tc_off, tc_tgt, tc_work;
+    uint64_t tc_slots = 0, tc_nwork = 0;
+    double tc_pair_work = 0.0;
+    bool tc_built = false;
 
     const ursa::DevCsr& incoming(hipStream_t s);
     const ursa::DevCsr& undirected(hipStream_t s);

@@ -1,0 +1,315 @@
+// traverse.hip — order-exact frontier traversals: k_hop (ursa-core/src/algo/
+// hop.rs:52-130) and unweighted shortest_path (ursa-core/src/algo/bfs.rs:48-102).
+//
+// Both outputs depend on the serial discovery order: k_hop emits reached nodes
+// in the order they are first seen, and shortest_path follows the first
+// discoverer of each node. In the serial loop, level-L frontier vertex at
+// position p scans its neighbour slots j = 0, 1, ... (out then in for Both), so
+// a node v first reached at level L+1 is discovered by the smallest (p, j) over
+// the frontier edges that hit it. The device version computes exactly that:
+// every frontier edge into a not-yet-visited v does atomicMin(key[v], p<<32|j);
+// the new nodes are then sorted by key, which is their serial discovery order,
+// and parent[v] = frontier[key >> 32] is their serial first discoverer.
+// Outputs are therefore byte-identical to the reference's.
+#include <algorithm>
+#include <vector>
+
+#include "frontier.hpp"
+#include "prim.hpp"
+
+struct ursa_hip_pairs {
+    std::vector<uint32_t> src, dst;
+};
+
+namespace ursa {
+
+void direction_views(ursa_hip_topo* t, int dir, hipStream_t s, NbrView& fwd, NbrView& rev);
+
+namespace {
+
+constexpr int TB = 256;
+constexpr int TW = TB / 64;
+
+struct OCtr {
+    uint32_t count;
+    uint32_t hit;  // target reached (shortest_path)
+};
+
+// Expand frontier F[0..flen): candidates get atomicMin(key, p<<32 | j) and are
+// appended to cand once (the thread whose atomicMin saw the initial key).
+__global__ __launch_bounds__(TB) void k_oexpand(const uint32_t* __restrict__ F, uint32_t flen, NbrView fwd,
+                                                const int32_t* __restrict__ level,
+                                                unsigned long long* __restrict__ key, uint32_t* __restrict__ cand,
+                                                OCtr* ctr) {
+    const int lane = lane_id();
+    const uint64_t gw = blockIdx.x * (uint64_t)TW + (threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * TW;
+    for (uint64_t base = gw * 64; base < flen; base += nwaves * 64) {
+        const uint64_t idx = base + lane;
+        uint32_t u = 0, d = 0, da = 0, sa = 0, sb = 0;
+        if (idx < flen) {
+            u = F[idx];
+            d = nv_degree(fwd, u, da);
+            sa = fwd.off_a[u];
+            if (fwd.off_b) sb = fwd.off_b[u];
+        }
+        const uint32_t incl = wave_incl_scan(d);
+        const uint32_t excl = incl - d;
+        const uint32_t total = __shfl(incl, 63, 64);
+        for (uint32_t t0 = 0; t0 < total; t0 += 64) {
+            const uint32_t t = t0 + lane;
+            const int j = wave_owner(excl, t);
+            const uint32_t e = t - __shfl(excl, j, 64);
+            const uint32_t oa = __shfl(da, j, 64);
+            const uint32_t osa = __shfl(sa, j, 64);
+            const uint32_t osb = __shfl(sb, j, 64);
+            bool fresh = false;
+            uint32_t v = 0;
+            if (t < total) {
+                v = e < oa ? fwd.tgt_a[osa + e] : fwd.tgt_b[osb + (e - oa)];
+                if (level[v] < 0) {
+                    const unsigned long long k = ((unsigned long long)(base + j) << 32) | e;
+                    fresh = atomicMin(&key[v], k) == ~0ull;
+                }
+            }
+            const uint32_t slot = wave_append_slot(fresh, &ctr->count);
+            if (fresh) cand[slot] = v;
+        }
+    }
+}
+
+__global__ void k_ogather_keys(const uint32_t* __restrict__ cand, uint32_t cnt,
+                               const unsigned long long* __restrict__ key, uint64_t* __restrict__ out) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < cnt; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = key[cand[i]];
+}
+
+// Commit a level: the sorted new nodes get their level and parent and their
+// key is reset; they are appended to `order` at `at`.
+__global__ void k_ocommit(const uint32_t* __restrict__ sorted, const uint64_t* __restrict__ skeys, uint32_t cnt,
+                          const uint32_t* __restrict__ F, int32_t* __restrict__ level, int32_t L,
+                          unsigned long long* __restrict__ key, uint32_t* __restrict__ parent,
+                          uint32_t* __restrict__ order, uint64_t at, uint32_t target, OCtr* ctr) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < cnt; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t v = sorted[i];
+        level[v] = L;
+        key[v] = ~0ull;
+        if (parent) parent[v] = F[skeys[i] >> 32];
+        order[at + i] = v;
+        if (v == target) ctr->hit = 1;
+    }
+}
+
+__global__ void k_oreset(const uint32_t* __restrict__ order, uint64_t cnt, int32_t* __restrict__ level) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < cnt; i += (uint64_t)gridDim.x * blockDim.x)
+        level[order[i]] = -1;
+}
+
+__global__ void k_oseed(uint32_t seed, int32_t* level, uint32_t* order, uint32_t* parent) {
+    level[seed] = 0;
+    order[0] = seed;
+    if (parent) parent[seed] = seed;
+}
+
+__global__ void k_fill_i32b(int32_t* p, uint64_t n, int32_t v) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Scratch for ordered traversals over one topology; level[] is kept all -1
+// between traversals (each traversal resets exactly what it touched).
+struct OWork {
+    uint64_t n;
+    DevBuf<int32_t> level;
+    DevBuf<unsigned long long> key;
+    DevBuf<uint32_t> order, cand, sorted, parent;
+    DevBuf<uint64_t> ckeys, skeys;
+    DevBuf<OCtr> ctr;
+    OCtr* h = nullptr;
+    OWork(uint64_t n_, bool with_parent, hipStream_t s)
+        : n(n_), level(n_), key(n_), order(n_), cand(n_), sorted(n_), ckeys(n_), skeys(n_), ctr(1) {
+        if (with_parent) parent.alloc(n_);
+        k_fill_i32b<<<blocks_for(n, 256), 256, 0, s>>>(level.get(), n, -1);
+        k_fill_u64<<<blocks_for(n, 256), 256, 0, s>>>(key.get(), n, ~0ull);
+        check_launch();
+        HIP_OK(hipHostMalloc((void**)&h, sizeof(OCtr), hipHostMallocDefault));
+    }
+    ~OWork() {
+        if (h) (void)hipHostFree(h);
+    }
+    OCtr read(hipStream_t s) {
+        HIP_OK(hipMemcpyAsync(h, ctr.get(), sizeof(OCtr), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        return *h;
+    }
+};
+
+// Ordered BFS from `seed` for at most `max_levels` levels (0 = unbounded),
+// stopping early once `target` is reached. Leaves the discovery order in
+// w.order[0..*visited) and returns whether the target was reached.
+bool ordered_bfs(uint32_t seed, uint32_t max_levels, uint32_t target, const NbrView& fwd, OWork& w, hipStream_t s,
+                 uint64_t* visited) {
+    k_oseed<<<1, 1, 0, s>>>(seed, w.level.get(), w.order.get(), w.parent.get());
+    check_launch();
+    uint64_t fstart = 0, flen = 1, total = 1;
+    bool hit = false;
+    for (int32_t L = 1; flen > 0 && (max_levels == 0 || (uint32_t)L <= max_levels); ++L) {
+        HIP_OK(hipMemsetAsync(w.ctr.get(), 0, sizeof(OCtr), s));
+        const uint32_t* F = w.order.get() + fstart;
+        k_oexpand<<<blocks_for((flen + 63) / 64, TW, 1u << 16), TB, 0, s>>>(F, (uint32_t)flen, fwd, w.level.get(),
+                                                                           w.key.get(), w.cand.get(), w.ctr.get());
+        check_launch();
+        const uint32_t cnt = w.read(s).count;
+        if (cnt == 0) break;
+        k_ogather_keys<<<blocks_for(cnt, 256), 256, 0, s>>>(w.cand.get(), cnt, w.key.get(), w.ckeys.get());
+        check_launch();
+        prim::sort_pairs_u64_u32(w.ckeys.get(), w.skeys.get(), w.cand.get(), w.sorted.get(), cnt,
+                                 32 + bits_for(flen), s);
+        k_ocommit<<<blocks_for(cnt, 256), 256, 0, s>>>(w.sorted.get(), w.skeys.get(), cnt, F, w.level.get(), L,
+                                                      w.key.get(), w.parent.get(), w.order.get(), total, target,
+                                                      w.ctr.get());
+        check_launch();
+        fstart = total;
+        flen = cnt;
+        total += cnt;
+        if (target != 0xFFFFFFFFu && w.read(s).hit) {
+            hit = true;
+            break;
+        }
+    }
+    *visited = total;
+    return hit;
+}
+
+void reset_touched(OWork& w, uint64_t visited, hipStream_t s) {
+    k_oreset<<<blocks_for(visited, 256), 256, 0, s>>>(w.order.get(), visited, w.level.get());
+    check_launch();
+}
+
+}  // namespace
+
+void k_hop(ursa_hip_topo* t, const uint32_t* seeds_user, uint64_t nseeds, uint32_t k, int dir, ursa_hip_pairs* res) {
+    const uint64_t n = t->n;
+    if (k == 0 || n == 0 || nseeds == 0) return;  // hop.rs:53-56
+    hipStream_t s = stream();
+    NbrView fwd, rev;
+    direction_views(t, dir, s, fwd, rev);
+    std::vector<uint32_t> seeds(nseeds);
+    if (is_device_ptr(seeds_user))
+        HIP_OK(hipMemcpy(seeds.data(), seeds_user, nseeds * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    else
+        std::copy(seeds_user, seeds_user + nseeds, seeds.begin());
+    OWork w(n, false, s);
+    std::vector<uint32_t> buf;
+    for (uint32_t seed : seeds) {
+        if (seed >= n) continue;  // hop.rs:91-93: unknown seed contributes nothing
+        uint64_t visited = 0;
+        ordered_bfs(seed, k, 0xFFFFFFFFu, fwd, w, s, &visited);
+        const uint64_t reached = visited - 1;
+        if (reached) {
+            buf.resize(reached);
+            HIP_OK(hipMemcpyAsync(buf.data(), w.order.get() + 1, reached * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        }
+        reset_touched(w, visited, s);
+        HIP_OK(hipStreamSynchronize(s));
+        res->src.insert(res->src.end(), reached, seed);
+        res->dst.insert(res->dst.end(), buf.begin(), buf.begin() + reached);
+    }
+}
+
+// Returns the path length (0 = None).
+uint64_t shortest_path(ursa_hip_topo* t, uint32_t src, uint32_t tgt, int dir, std::vector<uint32_t>& path) {
+    const uint64_t n = t->n;
+    path.clear();
+    if (src >= n || tgt >= n) return 0;  // bfs.rs:55-57
+    if (src == tgt) {                    // bfs.rs:58-60
+        path.push_back(src);
+        return 1;
+    }
+    hipStream_t s = stream();
+    NbrView fwd, rev;
+    direction_views(t, dir, s, fwd, rev);
+    OWork w(n, true, s);
+    uint64_t visited = 0;
+    const bool hit = ordered_bfs(src, 0, tgt, fwd, w, s, &visited);
+    if (!hit) return 0;
+    std::vector<uint32_t> parent(n);
+    HIP_OK(hipMemcpyAsync(parent.data(), w.parent.get(), n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    uint32_t cur = tgt;
+    path.push_back(cur);
+    while (cur != src) {
+        cur = parent[cur];
+        path.push_back(cur);
+    }
+    std::reverse(path.begin(), path.end());
+    return path.size();
+}
+
+}  // namespace ursa
+
+using namespace ursa;
+
+extern "C" {
+
+int ursa_hip_k_hop(ursa_hip_topo* t, const uint32_t* seeds, uint64_t nseeds, uint32_t k, int direction,
+                   ursa_hip_pairs** out) {
+    return guard([&] {
+        URSA_REQUIRE(t != nullptr && out != nullptr, URSA_HIP_EINVAL, "topology/out is NULL");
+        URSA_REQUIRE(direction >= 0 && direction <= 2, URSA_HIP_EINVAL, "bad direction");
+        URSA_REQUIRE(nseeds == 0 || seeds != nullptr, URSA_HIP_EINVAL, "seeds is NULL");
+        *out = nullptr;
+        auto* r = new ursa_hip_pairs;
+        try {
+            ursa::k_hop(t, seeds, nseeds, k, direction, r);
+        } catch (...) {
+            delete r;
+            throw;
+        }
+        *out = r;
+    });
+}
+
+int ursa_hip_pairs_size(const ursa_hip_pairs* p, uint64_t* count) {
+    return guard([&] {
+        URSA_REQUIRE(p != nullptr && count != nullptr, URSA_HIP_EINVAL, "NULL argument");
+        *count = p->src.size();
+    });
+}
+
+int ursa_hip_pairs_export(const ursa_hip_pairs* p, uint32_t* src, uint32_t* dst) {
+    return guard([&] {
+        URSA_REQUIRE(p != nullptr, URSA_HIP_EINVAL, "pairs handle is NULL");
+        const uint64_t c = p->src.size();
+        if (!c) return;
+        if (src) std::copy(p->src.begin(), p->src.end(), src);
+        if (dst) std::copy(p->dst.begin(), p->dst.end(), dst);
+    });
+}
+
+int ursa_hip_pairs_destroy(ursa_hip_pairs* p) {
+    delete p;
+    return URSA_HIP_OK;
+}
+
+int ursa_hip_shortest_path(ursa_hip_topo* t, uint32_t source, uint32_t target, int direction, uint32_t* path,
+                           uint64_t cap, uint64_t* len) {
+    return guard([&] {
+        URSA_REQUIRE(t != nullptr && len != nullptr, URSA_HIP_EINVAL, "topology/len is NULL");
+        URSA_REQUIRE(direction >= 0 && direction <= 2, URSA_HIP_EINVAL, "bad direction");
+        std::vector<uint32_t> p;
+        *len = ursa::shortest_path(t, source, target, direction, p);
+        URSA_REQUIRE(p.size() <= cap, URSA_HIP_ERANGE, "path longer than the output capacity (see *len)");
+        if (!p.empty()) {
+            URSA_REQUIRE(path != nullptr, URSA_HIP_EINVAL, "path is NULL");
+            std::copy(p.begin(), p.end(), path);
+        }
+    });
+}
+
+}  // extern "C"
