@@ -41,6 +41,16 @@ int ursa_hip_betweenness_sources_ex(ursa_hip_topo* t, const uint32_t* sources, u
                                     int apply_scale, double scale, uint32_t batch, double* out,
                                     uint64_t* reached_edges);
 
+/* ---- mixed-precision PageRank ----
+ * Same contract as ursa_hip_pagerank (pagerank.rs:98-169). The first sweeps
+ * gather f32 contributions and sum each row piece with compensated (Neumaier)
+ * f32 additions; once the L1 delta falls below 2e-6 or stops shrinking, f64
+ * sweeps refine the same iterate until delta < tol. f32_iters_out (may be
+ * NULL) = how many of *iters_out were f32 sweeps. */
+int ursa_hip_pagerank_mixed(ursa_hip_topo* t, const double* weights, double damping,
+                            uint32_t max_iter, double tol, double* out, uint32_t* iters_out,
+                            uint32_t* f32_iters_out);
+
 /* ---- device memory for callers without a HIP runtime binding ---- */
 int ursa_hip_device_alloc(uint64_t bytes, void** out);
 int ursa_hip_device_free(void* p);

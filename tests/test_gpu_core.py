@@ -148,3 +148,34 @@ def test_config1_pagerank_rmat_s16(ua):
     ref, rit, rdelta = orc.pagerank(o, 0.85, 100, 1e-6)
     assert it == rit, (it, rit, rdelta)
     assert_pr_close(got, ref, "rmat16")
+
+
+@pytest.mark.parametrize("case", GRAPHS[:8], ids=IDS[:8])
+def test_pagerank_mixed_within_1e9_l1(ua, case):
+    """f32 compensated sweeps + f64 refinement: L1 distance to the f64 oracle < 1e-9."""
+    name, n, s, d = case
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    params = ua.PageRankParams(0.85, 100, 1e-10)
+    got, it, it32 = ua.pagerank_mixed(t, params)
+    ref, rit, _ = orc.pagerank(o, 0.85, 100, 1e-10)
+    assert np.abs(got - ref).sum() < 1e-9, (name, it, it32, np.abs(got - ref).sum())
+    w = orc.gen_weights(len(s), 5)
+    got, it, it32 = ua.pagerank_mixed(t, params, weights=w)
+    ref, _, _ = orc.pagerank(o, 0.85, 100, 1e-10, weights=w)
+    assert np.abs(got - ref).sum() < 1e-9, (name, "weighted", np.abs(got - ref).sum())
+
+
+def test_pagerank_mixed_rmat(ua):
+    from ursa_amd import gen
+
+    e = gen.rmat(16, 16, seed=3)
+    _, _, sd, dd, _ = e.export()
+    t = e.topology()
+    o = orc.Topology(e.n, sd, dd)
+    got, it, it32 = ua.pagerank_mixed(t, ua.PageRankParams(0.85, 100, 1e-10))
+    ref, rit, _ = orc.pagerank(o, 0.85, 100, 1e-10)
+    assert it32 >= 1 and np.abs(got - ref).sum() < 1e-9
+    top_g = np.argsort(-got, kind="stable")[:1000]
+    top_r = np.argsort(-ref, kind="stable")[:1000]
+    assert len(set(top_g) & set(top_r)) >= 995

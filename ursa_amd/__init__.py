@@ -29,6 +29,7 @@ __all__ = [
     "degree",
     "pagerank",
     "pagerank_weighted",
+    "pagerank_mixed",
     "bfs_distances",
     "bfs_multi",
     "k_hop",
@@ -161,6 +162,21 @@ def _pagerank(topo: Topology, weights, params: PageRankParams):
     check(load().ursa_hip_pagerank(topo._h, _ptr(w) if w is not None else None, params.damping,
                                    params.max_iter, params.tol, _ptr(out), C.byref(it)))
     return out[:n], it.value
+
+
+def pagerank_mixed(topo: Topology, params: PageRankParams = PageRankParams(), weights=None):
+    """PageRank with f32 compensated sweeps refined by f64 sweeps -> (f64[n], iterations, f32 iterations)."""
+    n = topo.n_nodes()
+    out = np.zeros(max(n, 1), np.float64)
+    it, it32 = C.c_uint32(0), C.c_uint32(0)
+    w = None
+    if weights is not None:
+        w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64))
+        if w.shape[0] != topo.n_edges():
+            raise ValueError("weights length must equal the edge count")
+    check(load().ursa_hip_pagerank_mixed(topo._h, _ptr(w) if w is not None else None, params.damping,
+                                         params.max_iter, params.tol, _ptr(out), C.byref(it), C.byref(it32)))
+    return out[:n], it.value, it32.value
 
 
 def pagerank(topo: Topology, params: PageRankParams = PageRankParams()) -> np.ndarray:

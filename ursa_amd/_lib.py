@@ -46,6 +46,7 @@ SIGNATURES = {
     "ursa_hip_topology_export": (cint, [P, cint, P, P, P]),
     "ursa_hip_degree": (cint, [P, cint, P]),
     "ursa_hip_pagerank": (cint, [P, P, f64, u32, f64, P, C.POINTER(u32)]),
+    "ursa_hip_pagerank_mixed": (cint, [P, P, f64, u32, f64, P, C.POINTER(u32), C.POINTER(u32)]),
     "ursa_hip_bfs": (cint, [P, P, u32, cint, P]),
     "ursa_hip_bfs_ex": (cint, [P, P, u32, cint, P, C.POINTER(u64)]),
     "ursa_hip_k_hop": (cint, [P, P, u64, u32, cint, C.POINTER(P)]),

@@ -78,11 +78,6 @@ __global__ void k_pr_partition(const uint32_t* in_off, uint64_t n, uint64_t m, u
     ty[t] = (uint32_t)y;
 }
 
-template <class W>
-struct EdgeWeight {
-    __device__ static double get(const W* w, uint64_t k) { return (double)w[k]; }
-};
-
 // Deterministic sum over a workgroup; the result is valid in thread 0.
 __device__ __forceinline__ double block_sum(double v, double* lds) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -96,16 +91,52 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
     return tot;
 }
 
+// Per-thread running sum of a row piece. f64 contributions add plainly; f32
+// contributions use Neumaier's compensated two-sum (the rounding error of each
+// add is carried in c), so an f32 row piece is accurate to about one f32 ulp of
+// its magnitude before it is widened to f64.
+template <class CT>
+struct RowAcc;
+
+template <>
+struct RowAcc<double> {
+    double s = 0.0;
+    __device__ __forceinline__ void add(double x) { s += x; }
+    __device__ __forceinline__ double value() const { return s; }
+    __device__ __forceinline__ void reset() { s = 0.0; }
+};
+
+template <>
+struct RowAcc<float> {
+    float s = 0.f, c = 0.f;
+    __device__ __forceinline__ void add(float x) {
+#pragma clang fp contract(off)
+        const float t = s + x;
+        if (fabsf(s) >= fabsf(x))
+            c += (s - t) + x;
+        else
+            c += (x - t) + s;
+        s = t;
+    }
+    __device__ __forceinline__ double value() const { return (double)s + (double)c; }
+    __device__ __forceinline__ void reset() {
+        s = 0.f;
+        c = 0.f;
+    }
+};
+
 // Row epilogue shared by the tile and fix-up kernels. `scale` is the
 // out-degree (unweighted) or out-strength (weighted) of u; scale > 0 means u
-// has out-mass, otherwise u is dangling.
+// has out-mass, otherwise u is dangling. The next contribution is stored in
+// the contribution type CT of the following sweep.
+template <class CT>
 __device__ __forceinline__ void pr_finish_row(uint64_t u, double acc, double base, double damping, double scale,
-                                              const double* rank, double* next, double* contrib_next,
+                                              const double* rank, double* next, CT* contrib_next,
                                               double& delta, double& dangling) {
     const double v = base + damping * acc;
     next[u] = v;
     const bool has_out = scale > 0.0;
-    contrib_next[u] = has_out ? v / scale : 0.0;
+    contrib_next[u] = (CT)(has_out ? v / scale : 0.0);
     delta += fabs(v - rank[u]);
     if (!has_out) dangling += v;
 }
@@ -118,15 +149,15 @@ __device__ __forceinline__ double node_scale(const uint32_t* out_deg, const doub
         return (double)out_deg[u];
 }
 
-template <class W, bool WEIGHTED>
+template <class CT, class W, bool WEIGHTED>
 __global__ __launch_bounds__(PR_BLOCK) void k_pr_tiles(
     const uint32_t* __restrict__ in_off, const uint32_t* __restrict__ in_tgt, const W* __restrict__ w_in,
     const uint32_t* __restrict__ tile_x, const uint32_t* __restrict__ tile_y, uint64_t n,
     const uint32_t* __restrict__ out_deg, const double* __restrict__ out_str, const double* __restrict__ rank,
-    const double* __restrict__ contrib, double* __restrict__ next, double* __restrict__ contrib_next,
+    const CT* __restrict__ contrib, double* __restrict__ next, CT* __restrict__ contrib_next,
     double* __restrict__ carry_val, double* __restrict__ tail_val, double* __restrict__ tile_delta,
     double* __restrict__ tile_dm, PrCtl* ctl, int parity, double damping) {
-    __shared__ double s_val[PR_TILE];
+    __shared__ CT s_val[PR_TILE];
     __shared__ uint32_t s_end[PR_TILE + 1];
     __shared__ double s_wv[PR_BLOCK / 64];
     __shared__ int s_wf[PR_BLOCK / 64];
@@ -142,8 +173,8 @@ __global__ __launch_bounds__(PR_BLOCK) void k_pr_tiles(
 
     for (uint64_t i = threadIdx.x; i < nnz; i += PR_BLOCK) {
         const uint64_t k = y0 + i;
-        double c = contrib[in_tgt[k]];
-        if constexpr (WEIGHTED) c = EdgeWeight<W>::get(w_in, k) * c;
+        CT c = contrib[in_tgt[k]];
+        if constexpr (WEIGHTED) c = (CT)w_in[k] * c;
         s_val[i] = c;
     }
     if (x0 < n)
@@ -167,7 +198,7 @@ __global__ __launch_bounds__(PR_BLOCK) void k_pr_tiles(
         y = y0 + d0 - lo;
     }
 
-    double sum = 0.0;
+    RowAcc<CT> sum;
     double ev[PR_ITEMS];
     uint32_t er[PR_ITEMS];
     uint32_t emask = 0;
@@ -177,13 +208,13 @@ __global__ __launch_bounds__(PR_BLOCK) void k_pr_tiles(
         er[it] = 0;
         if (d0 + it < len) {
             if (x < n && y < (uint64_t)s_end[x - x0]) {
-                sum += s_val[y - y0];
+                sum.add(s_val[y - y0]);
                 ++y;
             } else {
                 er[it] = (uint32_t)x;
-                ev[it] = sum;
+                ev[it] = sum.value();
                 emask |= 1u << it;
-                sum = 0.0;
+                sum.reset();
                 ++x;
             }
         }
@@ -193,7 +224,7 @@ __global__ __launch_bounds__(PR_BLOCK) void k_pr_tiles(
     // thread's incoming carry is the piece of its first row accumulated by
     // the preceding threads since their last row end.
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double v = sum;
+    double v = sum.value();
     int f = emask != 0;
     for (int off = 1; off < 64; off <<= 1) {
         double vu = __shfl_up(v, off, 64);
@@ -243,12 +274,12 @@ __global__ __launch_bounds__(PR_BLOCK) void k_pr_tiles(
     }
 }
 
-template <bool WEIGHTED>
+template <class CT, bool WEIGHTED>
 __global__ void k_pr_fixup(const uint32_t* __restrict__ in_off, const uint32_t* __restrict__ tile_x,
                            const uint32_t* __restrict__ tile_y, uint64_t tiles, uint64_t n,
                            const uint32_t* __restrict__ out_deg, const double* __restrict__ out_str,
                            const double* __restrict__ rank, double* __restrict__ next,
-                           double* __restrict__ contrib_next, const double* __restrict__ carry_val,
+                           CT* __restrict__ contrib_next, const double* __restrict__ carry_val,
                            const double* __restrict__ tail_val, double* __restrict__ fix_delta,
                            double* __restrict__ fix_dm, const PrCtl* ctl, int parity, double damping) {
     if (ctl->done) return;
@@ -294,10 +325,10 @@ __global__ __launch_bounds__(1024) void k_pr_reduce(const double* __restrict__ a
 }
 
 // rank0 = 1/n, contrib0 = rank0 / scale, per-block dangling partials.
-template <bool WEIGHTED>
+template <class CT, bool WEIGHTED>
 __global__ void k_pr_init(uint64_t n, double nf, const uint32_t* __restrict__ out_deg,
                           const double* __restrict__ out_str, double* __restrict__ rank,
-                          double* __restrict__ contrib, double* __restrict__ part_dm) {
+                          CT* __restrict__ contrib, double* __restrict__ part_dm) {
     __shared__ double s_red[256 / 64];
     double dm = 0.0;
     const uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
@@ -305,7 +336,7 @@ __global__ void k_pr_init(uint64_t n, double nf, const uint32_t* __restrict__ ou
         const double r = 1.0 / nf;
         rank[u] = r;
         const double sc = node_scale<WEIGHTED>(out_deg, out_str, u);
-        contrib[u] = sc > 0.0 ? r / sc : 0.0;
+        contrib[u] = (CT)(sc > 0.0 ? r / sc : 0.0);
         if (!(sc > 0.0)) dm = r;
     }
     const double b = block_sum(dm, s_red);
@@ -351,6 +382,17 @@ __global__ void k_permute_weights(const uint32_t* __restrict__ in_eid, const dou
     }
 }
 
+// contrib[u] = rank[u] / scale (f64), used when the mixed-precision solve
+// hands over from its f32 sweeps to its f64 sweeps.
+template <bool WEIGHTED>
+__global__ void k_pr_contrib64(uint64_t n, const uint32_t* __restrict__ out_deg, const double* __restrict__ out_str,
+                               const double* __restrict__ rank, double* __restrict__ contrib) {
+    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n; u += (uint64_t)gridDim.x * blockDim.x) {
+        const double sc = node_scale<WEIGHTED>(out_deg, out_str, u);
+        contrib[u] = sc > 0.0 ? rank[u] / sc : 0.0;
+    }
+}
+
 __global__ void k_fill(double* p, uint64_t n, double v) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -374,11 +416,18 @@ static void pr_partition(ursa_hip_topo* t, const DevCsr& in, hipStream_t s) {
     t->pr_tile_items = PR_TILE;
 }
 
-void pagerank_f64(ursa_hip_topo* t, const double* weights_user, double damping, uint32_t max_iter, double tol,
-                  double* out_user, uint32_t* iters_out) {
+// Solve PageRank. With `mixed`, sweeps first run with f32 contributions and
+// compensated f32 row sums (8 B per edge instead of 12) until the L1 delta
+// stops shrinking quickly or falls below 2e-6, then hand over to f64 sweeps,
+// which refine the same iterate until delta < tol. Every iteration of either
+// kind counts toward max_iter, and the answer is always the last f64 iterate
+// (or the uniform start when max_iter == 0).
+void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, uint32_t max_iter, double tol,
+                  double* out_user, uint32_t* iters_out, bool mixed, uint32_t* f32_iters_out) {
     hipStream_t s = stream();
     const uint64_t n = t->n, m = t->m;
     if (iters_out) *iters_out = 0;
+    if (f32_iters_out) *f32_iters_out = 0;
     if (n == 0) return;
     URSA_REQUIRE(out_user != nullptr, URSA_HIP_EINVAL, "out is NULL");
     const double nf = (double)n;
@@ -397,7 +446,7 @@ void pagerank_f64(ursa_hip_topo* t, const double* weights_user, double damping, 
     DevIn<double> w(weights_user, m, s);
     DevBuf<double> out_str, w64;
     DevBuf<float> w32;
-    bool use_f32 = false;
+    bool use_f32w = false;
     if (weighted) {
         out_str.alloc(n);
         k_out_strength<<<blocks_for(n, 256), 256, 0, s>>>(t->out.off.get(), t->out.eid.get(), w.p, n, out_str.get());
@@ -414,85 +463,162 @@ void pagerank_f64(ursa_hip_topo* t, const double* weights_user, double damping, 
         uint32_t h = 0;
         HIP_OK(hipMemcpyAsync(&h, inexact.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
-        use_f32 = (h == 0);
-        if (use_f32)
-            w64.reset();
-        else
-            w32.reset();
+        use_f32w = (h == 0);
+        // The f32 sweeps always read f32 weights; keep both when f64 ones are needed.
+        if (use_f32w) w64.reset();
+        else if (!mixed) w32.reset();
     }
 
     DevBuf<double> rank[2] = {DevBuf<double>(n), DevBuf<double>(n)};
     DevBuf<double> contrib[2] = {DevBuf<double>(n), DevBuf<double>(n)};
+    DevBuf<float> contrib32[2];
+    if (mixed) {
+        contrib32[0].alloc(n);
+        contrib32[1].alloc(n);
+    }
     DevBuf<double> carry(tiles), tail(tiles), td(tiles), tm(tiles), fd(tiles), fm(tiles);
     DevBuf<PrCtl> ctl(1);
     const unsigned init_blocks = blocks_for(n, 256, 1u << 30);
     DevBuf<double> part_dm(init_blocks);
-    if (weighted)
-        k_pr_init<true><<<init_blocks, 256, 0, s>>>(n, nf, t->out_deg.get(), out_str.get(), rank[0].get(),
-                                                    contrib[0].get(), part_dm.get());
-    else
-        k_pr_init<false><<<init_blocks, 256, 0, s>>>(n, nf, t->out_deg.get(), nullptr, rank[0].get(),
-                                                     contrib[0].get(), part_dm.get());
+    const uint32_t* od = t->out_deg.get();
+    const double* os = weighted ? out_str.get() : nullptr;
+    if (mixed) {
+        if (weighted)
+            k_pr_init<float, true><<<init_blocks, 256, 0, s>>>(n, nf, od, os, rank[0].get(), contrib32[0].get(), part_dm.get());
+        else
+            k_pr_init<float, false><<<init_blocks, 256, 0, s>>>(n, nf, od, os, rank[0].get(), contrib32[0].get(), part_dm.get());
+    } else {
+        if (weighted)
+            k_pr_init<double, true><<<init_blocks, 256, 0, s>>>(n, nf, od, os, rank[0].get(), contrib[0].get(), part_dm.get());
+        else
+            k_pr_init<double, false><<<init_blocks, 256, 0, s>>>(n, nf, od, os, rank[0].get(), contrib[0].get(), part_dm.get());
+    }
     k_pr_ctl_init<<<1, 1024, 0, s>>>(ctl.get(), part_dm.get(), init_blocks, damping, nf);
     check_launch();
 
-    const double edge_bytes = weighted ? (use_f32 ? 16.0 : 20.0) : 12.0;
     const double node_bytes = weighted ? 36.0 : 32.0;
-    const double spmv_bytes = edge_bytes * (double)m + node_bytes * (double)n;
+    const double f64_edge = weighted ? (use_f32w ? 16.0 : 20.0) : 12.0;
+    const double f32_edge = weighted ? 12.0 : 8.0;
     const unsigned fix_blocks = blocks_for(tiles, 256, 1u << 30);
+    const uint32_t* tx = t->pr_tile_x.get();
+    const uint32_t* ty = t->pr_tile_y.get();
 
-    PrCtl h{};
+    // One iteration `it` (parity it & 1) with contribution type CT.
+    auto iteration = [&](uint32_t it, bool f32) {
+        const int p = (int)(it & 1);
+        const double* rk = rank[p].get();
+        double* nx = rank[p ^ 1].get();
+        if (f32) {
+            const float* cb = contrib32[p].get();
+            float* cn = contrib32[p ^ 1].get();
+            timed("pagerank_spmv_f32", f32_edge * (double)m + (node_bytes - 4.0) * (double)n, s, [&] {
+                if (weighted)
+                    k_pr_tiles<float, float, true><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
+                        in.off.get(), in.tgt.get(), w32.get(), tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
+                        tail.get(), td.get(), tm.get(), ctl.get(), p, damping);
+                else
+                    k_pr_tiles<float, float, false><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
+                        in.off.get(), in.tgt.get(), nullptr, tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
+                        tail.get(), td.get(), tm.get(), ctl.get(), p, damping);
+            });
+            if (weighted)
+                k_pr_fixup<float, true><<<fix_blocks, 256, 0, s>>>(in.off.get(), tx, ty, tiles, n, od, os, rk, nx, cn,
+                                                                   carry.get(), tail.get(), fd.get(), fm.get(),
+                                                                   ctl.get(), p, damping);
+            else
+                k_pr_fixup<float, false><<<fix_blocks, 256, 0, s>>>(in.off.get(), tx, ty, tiles, n, od, os, rk, nx, cn,
+                                                                    carry.get(), tail.get(), fd.get(), fm.get(),
+                                                                    ctl.get(), p, damping);
+        } else {
+            const double* cb = contrib[p].get();
+            double* cn = contrib[p ^ 1].get();
+            timed("pagerank_spmv", f64_edge * (double)m + node_bytes * (double)n, s, [&] {
+                if (!weighted)
+                    k_pr_tiles<double, double, false><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
+                        in.off.get(), in.tgt.get(), nullptr, tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
+                        tail.get(), td.get(), tm.get(), ctl.get(), p, damping);
+                else if (use_f32w)
+                    k_pr_tiles<double, float, true><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
+                        in.off.get(), in.tgt.get(), w32.get(), tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
+                        tail.get(), td.get(), tm.get(), ctl.get(), p, damping);
+                else
+                    k_pr_tiles<double, double, true><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
+                        in.off.get(), in.tgt.get(), w64.get(), tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
+                        tail.get(), td.get(), tm.get(), ctl.get(), p, damping);
+            });
+            if (weighted)
+                k_pr_fixup<double, true><<<fix_blocks, 256, 0, s>>>(in.off.get(), tx, ty, tiles, n, od, os, rk, nx,
+                                                                    cn, carry.get(), tail.get(), fd.get(), fm.get(),
+                                                                    ctl.get(), p, damping);
+            else
+                k_pr_fixup<double, false><<<fix_blocks, 256, 0, s>>>(in.off.get(), tx, ty, tiles, n, od, os, rk, nx,
+                                                                     cn, carry.get(), tail.get(), fd.get(), fm.get(),
+                                                                     ctl.get(), p, damping);
+        }
+        check_launch();
+    };
+    auto reduce = [&](uint32_t it, double tol_it) {
+        k_pr_reduce<<<1, 1024, 0, s>>>(td.get(), fd.get(), tm.get(), fm.get(), tiles, ctl.get(), (int)(it & 1), it,
+                                       max_iter, tol_it, damping, nf);
+        check_launch();
+    };
+    auto read_ctl = [&]() {
+        PrCtl h{};
+        HIP_OK(hipMemcpyAsync(&h, ctl.get(), sizeof(PrCtl), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        return h;
+    };
+
+    uint32_t it = 0;
+    uint32_t f32_iters = 0;
+    if (mixed) {
+        // f32 sweeps, one at a time (the host watches the delta). The reduce
+        // kernel's own convergence test is disabled (tol -1) so it only raises
+        // `done` at max_iter.
+        double prev = 0.0;
+        while (it < max_iter) {
+            iteration(it, true);
+            reduce(it, -1.0);
+            const PrCtl h = read_ctl();
+            ++it;
+            ++f32_iters;
+            if (h.done) break;
+            const bool stalled = f32_iters >= 3 && h.delta > 0.7 * prev;
+            prev = h.delta;
+            if (h.delta < 2e-6 || stalled) break;
+        }
+        // Hand over: f64 contributions of the current iterate.
+        const int p = (int)(it & 1);
+        if (weighted)
+            k_pr_contrib64<true><<<blocks_for(n, 256), 256, 0, s>>>(n, od, os, rank[p].get(), contrib[p].get());
+        else
+            k_pr_contrib64<false><<<blocks_for(n, 256), 256, 0, s>>>(n, od, os, rank[p].get(), contrib[p].get());
+        check_launch();
+        const PrCtl h = read_ctl();
+        if (h.done) {
+            // max_iter was spent in f32 sweeps; one more f64 sweep is not allowed,
+            // so the f32 result stands.
+            it = max_iter;
+        }
+    }
+    PrCtl h = read_ctl();
     // Iterations are queued in batches; with per-kernel timing on, one at a
     // time so no launch after convergence is booked.
     const uint32_t batch = prof_enabled() ? 1 : 8;
-    for (uint32_t it0 = 0; it0 < max_iter; it0 += batch) {
-        const uint32_t it_end = std::min<uint32_t>(max_iter, it0 + batch);
-        for (uint32_t it = it0; it < it_end; ++it) {
-            const int p = (int)(it & 1);
-            const double* rk = rank[p].get();
-            const double* cb = contrib[p].get();
-            double* nx = rank[p ^ 1].get();
-            double* cn = contrib[p ^ 1].get();
-            timed("pagerank_spmv", spmv_bytes, s, [&] {
-                if (!weighted)
-                    k_pr_tiles<double, false><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
-                        in.off.get(), in.tgt.get(), nullptr, t->pr_tile_x.get(), t->pr_tile_y.get(), n,
-                        t->out_deg.get(), nullptr, rk, cb, nx, cn, carry.get(), tail.get(), td.get(), tm.get(),
-                        ctl.get(), p, damping);
-                else if (use_f32)
-                    k_pr_tiles<float, true><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
-                        in.off.get(), in.tgt.get(), w32.get(), t->pr_tile_x.get(), t->pr_tile_y.get(), n,
-                        t->out_deg.get(), out_str.get(), rk, cb, nx, cn, carry.get(), tail.get(), td.get(),
-                        tm.get(), ctl.get(), p, damping);
-                else
-                    k_pr_tiles<double, true><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
-                        in.off.get(), in.tgt.get(), w64.get(), t->pr_tile_x.get(), t->pr_tile_y.get(), n,
-                        t->out_deg.get(), out_str.get(), rk, cb, nx, cn, carry.get(), tail.get(), td.get(),
-                        tm.get(), ctl.get(), p, damping);
-            });
-            check_launch();
-            if (weighted)
-                k_pr_fixup<true><<<fix_blocks, 256, 0, s>>>(in.off.get(), t->pr_tile_x.get(), t->pr_tile_y.get(),
-                                                            tiles, n, t->out_deg.get(), out_str.get(), rk, nx, cn,
-                                                            carry.get(), tail.get(), fd.get(), fm.get(), ctl.get(),
-                                                            p, damping);
-            else
-                k_pr_fixup<false><<<fix_blocks, 256, 0, s>>>(in.off.get(), t->pr_tile_x.get(), t->pr_tile_y.get(),
-                                                             tiles, n, t->out_deg.get(), nullptr, rk, nx, cn,
-                                                             carry.get(), tail.get(), fd.get(), fm.get(), ctl.get(),
-                                                             p, damping);
-            k_pr_reduce<<<1, 1024, 0, s>>>(td.get(), fd.get(), tm.get(), fm.get(), tiles, ctl.get(), p, it, max_iter,
-                                           tol, damping, nf);
-            check_launch();
+    while (it < max_iter && !h.done) {
+        const uint32_t it_end = std::min<uint32_t>(max_iter, it + batch);
+        for (; it < it_end; ++it) {
+            iteration(it, false);
+            reduce(it, tol);
         }
-        HIP_OK(hipMemcpyAsync(&h, ctl.get(), sizeof(PrCtl), hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
-        if (h.done) break;
+        h = read_ctl();
     }
+    h = read_ctl();
     const uint32_t iters = h.iters;
     HIP_OK(hipMemcpyAsync(out.p, rank[iters & 1].get(), n * sizeof(double), hipMemcpyDeviceToDevice, s));
     out.finish(s);
     if (iters_out) *iters_out = iters;
+    if (f32_iters_out) *f32_iters_out = f32_iters;
 }
 
 }  // namespace ursa
@@ -501,6 +627,14 @@ extern "C" int ursa_hip_pagerank(ursa_hip_topo* t, const double* weights, double
                                  double tol, double* out, uint32_t* iters_out) {
     return ursa::guard([&] {
         URSA_REQUIRE(t != nullptr, URSA_HIP_EINVAL, "topology is NULL");
-        ursa::pagerank_f64(t, weights, damping, max_iter, tol, out, iters_out);
+        ursa::pagerank_run(t, weights, damping, max_iter, tol, out, iters_out, false, nullptr);
+    });
+}
+
+extern "C" int ursa_hip_pagerank_mixed(ursa_hip_topo* t, const double* weights, double damping, uint32_t max_iter,
+                                       double tol, double* out, uint32_t* iters_out, uint32_t* f32_iters_out) {
+    return ursa::guard([&] {
+        URSA_REQUIRE(t != nullptr, URSA_HIP_EINVAL, "topology is NULL");
+        ursa::pagerank_run(t, weights, damping, max_iter, tol, out, iters_out, true, f32_iters_out);
     });
 }
