@@ -1,20 +1,30 @@
 #!/usr/bin/env python3
-"""Benchmark of the CSR hot path on one MI355X.
+"""Benchmark of ursa-core's CSR hot path on one MI355X. Prints one JSON line.
 
-Default workload: pull PageRank (f64) on R-MAT scale 24, edge factor 16,
-(a,b,c,d) = (0.57,0.19,0.19,0.05), seed 3, deduplicated, self-loops removed,
-damping 0.85, tol 1e-10 (L1), max_iter 100. The graph is generated and interned on
-the device, the topology (CSR + transpose) is built before timing, so every timed
-step starts with its inputs resident in HBM and writes its output to HBM.
+Default workload (the integer-kernel configuration): R-MAT scale 22, edge factor 16,
+(a,b,c,d) = (0.57,0.19,0.19,0.05), seed 2, self-loops and duplicates dropped,
+directed, u32 CSR plus transpose. One step runs, on that graph:
+  * degree, out and in                       -> u32[n] each
+  * BFS from 64 sources (Out), i32 levels     -> i32[64, n]; the sources are the
+    first 64 of the reference's seeded shuffle (seed 2) of the nodes with
+    out-degree > 0, in dense-id order
+  * weakly connected components               -> u32[n] (min-id labels)
+  * triangle_count on the undirected view     -> u64[n]
+The graph is generated and interned on the device and its CSR, transpose,
+undirected view and triangle orientation are built before timing (ursa caches
+them on the topology too), so a timed step starts with everything in HBM and
+writes its outputs to HBM.
 
-A step is one full PageRank solve (all iterations until delta < tol). `value` is
-GTEPS = m * iterations / step time. The roofline object describes the SpMV tile
-kernel (k_pr_tiles), timed with HIP events on its launch stream inside the library;
-its algorithmic bytes are 12 B per edge + 32 B per node per iteration. The CPU
-baseline is the oracle (a C++ restatement of ursa-core's rayon PageRank) on the
-same graph for a bounded number of iterations, with all host threads.
+value = aggregate GTEPS of the step: (sum of BFS E_reach over the 64 sources
++ m for components + undirected m for triangles) / step time, the TEPS
+numerators of each kernel. `kernels` gives per-kernel GTEPS and %HBM from HIP
+events on the kernels' own stream. `roofline` describes the dominant kernel of
+the step. `cpu_baseline` is the oracle's serial BFS (the reference's
+bfs_distances loop) on a sample of the same sources, one core.
 
-Prints one JSON line.
+The `pagerank` section (R-MAT scale 24, seed 3, damping 0.85, tol 1e-10) times
+the plain f64 solve and the f32+refinement solve on the same graph and reports
+their error against the f64 oracle. Skip it with --no-pagerank.
 """
 from __future__ import annotations
 
@@ -24,6 +34,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
@@ -36,120 +48,302 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles(kernel: str, workload: str):
+def traffic_from_profiles(workload: str, kernel: str):
     """HBM bytes per launch measured by rocprofv3 PMC runs, if committed."""
-    path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
-        with open(path) as f:
-            data = json.load(f)
-        return data.get(workload, {}).get(kernel)
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            return json.load(f).get(workload, {}).get(kernel)
     except (OSError, ValueError):
         return None
 
 
-def bench_pagerank(args, ua, gen):
-    scale = args.scale or 24
-    seed = 3
-    params = ua.PageRankParams(0.85, 100, 1e-10)
-    t0 = time.time()
-    e = gen.rmat(scale, 16, 0.57, 0.19, 0.19, seed=seed)
-    topo = e.topology()
+class Dev:
+    """A device buffer owned by the library's allocator."""
+
+    def __init__(self, lib, nbytes):
+        self.lib = lib
+        self.p = C.c_void_p()
+        check(lib.ursa_hip_device_alloc(max(int(nbytes), 8), C.byref(self.p)))
+
+    def free(self):
+        if self.p:
+            self.lib.ursa_hip_device_free(self.p)
+            self.p = C.c_void_p()
+
+
+def check(rc):
+    from ursa_amd import _lib
+
+    _lib.check(rc)
+
+
+def kernel_stats(ua, names):
+    tot_ms, launches, nbytes = 0.0, 0, 0.0
+    for k in names:
+        q = ua.profile.query(k)
+        tot_ms += q["total_ms"]
+        launches += q["launches"]
+        nbytes += q["algo_bytes"]
+    return tot_ms, launches, nbytes
+
+
+# --------------------------------------------------------------------------- C2
+def bench_c2(args, ua, gen):
     lib = ua.load()
-    ua._lib.check(lib.ursa_hip_topology_incoming(topo._h))
+    scale = args.scale or 22
+    t0 = time.time()
+    e = gen.rmat(scale, 16, 0.57, 0.19, 0.19, seed=2)
+    topo = e.topology()
     n, m = e.n, e.m
-    log(f"graph R-MAT s{scale}: n={n} m={m} built in {time.time() - t0:.1f}s")
+    outdeg = ua.degree(topo, "out")
+    cand = np.nonzero(outdeg > 0)[0].astype(np.uint32)
+    perm = ua.shuffled_order(len(cand), 2)
+    sources = np.ascontiguousarray(cand[perm[:64]])
+    # Lazily built topology structures (cached on the topology, as in ursa).
+    tb = time.time()
+    check(lib.ursa_hip_topology_incoming(topo._h))
+    t_in = time.time() - tb
+    und_slots = C.c_uint64(0)
+    tb = time.time()
+    check(lib.ursa_hip_topology_undirected(topo._h, C.byref(und_slots)))
+    t_und = time.time() - tb
+    m_und = und_slots.value // 2
+    log(f"graph R-MAT s{scale}: n={n} m={m} undirected m={m_und}; generated in {time.time() - t0:.1f}s "
+        f"(transpose {t_in * 1e3:.0f} ms, undirected view {t_und * 1e3:.0f} ms)")
 
-    out = C.c_void_p()
-    ua._lib.check(lib.ursa_hip_device_alloc(8 * n, C.byref(out)))
-    iters = C.c_uint32(0)
+    lv = Dev(lib, 4 * 64 * n)
+    d_out, d_in, d_cc, d_tri = Dev(lib, 4 * n), Dev(lib, 4 * n), Dev(lib, 4 * n), Dev(lib, 8 * n)
+    src_ptr = sources.ctypes.data_as(C.c_void_p)
+    reached = C.c_uint64(0)
+    walls = {}
 
-    def step():
-        ua._lib.check(lib.ursa_hip_pagerank(topo._h, None, params.damping, params.max_iter, params.tol, out,
-                                            C.byref(iters)))
+    def step(record=False):
+        def run(name, fn):
+            if record:
+                check(lib.ursa_hip_synchronize())
+                w0 = time.perf_counter()
+            fn()
+            if record:
+                check(lib.ursa_hip_synchronize())
+                walls[name] = walls.get(name, 0.0) + time.perf_counter() - w0
+
+        run("degree", lambda: (check(lib.ursa_hip_degree(topo._h, 0, d_out.p)),
+                               check(lib.ursa_hip_degree(topo._h, 1, d_in.p))))
+        run("bfs", lambda: check(lib.ursa_hip_bfs_ex(topo._h, src_ptr, 64, 0, lv.p, C.byref(reached))))
+        run("cc", lambda: check(lib.ursa_hip_cc_weak(topo._h, d_cc.p)))
+        run("triangle", lambda: check(lib.ursa_hip_triangle_count(topo._h, d_tri.p)))
 
     for i in range(args.warmup):
         step()
-        log(f"warmup {i + 1}/{args.warmup}: iters={iters.value}")
-    ua._lib.check(lib.ursa_hip_synchronize())
+        log(f"warmup {i + 1}/{args.warmup}")
+    check(lib.ursa_hip_synchronize())
     t_start = time.perf_counter()
-    for i in range(args.steps):
+    for _ in range(args.steps):
         step()
-    ua._lib.check(lib.ursa_hip_synchronize())
+    check(lib.ursa_hip_synchronize())
     elapsed = time.perf_counter() - t_start
     ms_per_step = elapsed / args.steps * 1e3
-    gteps = m * iters.value * args.steps / elapsed / 1e9
-    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, iters={iters.value}, {gteps:.2f} GTEPS")
+    e_reach = reached.value
+    numer = e_reach + m + m_und
+    gteps = numer * args.steps / elapsed / 1e9
+    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step; BFS E_reach={e_reach}; aggregate {gteps:.2f} GTEPS")
 
-    # Roofline of the dominant kernel: HIP events around each k_pr_tiles launch.
+    # Per-kernel event timing (one profiled step) and wall time per call (one more).
     ua.profile.enable(True)
     ua.profile.reset()
     step()
-    q = ua.profile.query("pagerank_spmv")
+    groups = {
+        "bfs": ["bfs_top_down", "bfs_bottom_up"],
+        "cc": ["cc_hook_round", "cc_compress", "cc_hook_out", "cc_hook_in"],
+        "triangle": ["triangle_count"],
+        "degree": ["degree"],
+    }
+    stats = {g: kernel_stats(ua, ks) for g, ks in groups.items()}
     ua.profile.enable(False)
-    avg_ms = q["total_ms"] / max(q["launches"], 1)
-    per_launch_bytes = q["algo_bytes"] / max(q["launches"], 1)
-    achieved = per_launch_bytes / (avg_ms / 1e3) / 1e9
-    log(f"k_pr_tiles: {q['launches']} launches, avg {avg_ms * 1e3:.1f} us, {achieved:.0f} GB/s algorithmic")
+    step(record=True)
+    tri_q = ua.profile.query("triangle_count")
+    algo = {
+        "bfs": 4.0 * e_reach + 12.0 * n * 64,
+        "cc": stats["cc"][2],
+        "triangle": stats["triangle"][2],
+        "degree": stats["degree"][2],
+    }
+    teps = {"bfs": e_reach, "cc": m, "triangle": m_und, "degree": None}
+    kernels = {}
+    for g, (ms, launches, _) in stats.items():
+        gbs = algo[g] / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+        kernels[g] = {
+            "kernel_ms": round(ms, 3),
+            "call_ms": round(walls.get(g, 0.0) * 1e3, 3),
+            "launches": launches,
+            "algo_GB": round(algo[g] / 1e9, 4),
+            "GBps": round(gbs, 1),
+            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "GTEPS": round(teps[g] / (ms / 1e3) / 1e9, 3) if teps[g] and ms > 0 else None,
+        }
+    log("kernels: " + json.dumps(kernels))
+    dom = max(("bfs", "cc", "triangle"), key=lambda g: stats[g][0])
+    if dom == "bfs":
+        ms = stats["bfs"][0]
+        per_launch = algo["bfs"] / 64
+        avg_us = ms / 64 * 1e3
+        kname = "k_bfs_top_down+k_bfs_bottom_up (one 64-source row = 64 traversals)"
+        launches = 64
+    else:
+        ms, launches, b = stats[dom]
+        per_launch = b / max(launches, 1)
+        avg_us = ms / max(launches, 1) * 1e3
+        kname = {"cc": "k_cc_* (Afforest)", "triangle": "k_tc_count"}[dom]
+    achieved = per_launch / (avg_us / 1e6) / 1e9
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic_from_profiles("k_pr_tiles", f"pagerank_rmat{scale}"),
-        "kernel": "k_pr_tiles",
-        "avg_launch_us": round(avg_ms * 1e3, 2),
-        "algo_bytes_per_launch": per_launch_bytes,
+        "traffic": traffic_from_profiles(f"c2_rmat{scale}", dom),
+        "kernel": kname,
+        "avg_launch_us": round(avg_us, 2),
+        "algo_bytes_per_launch": per_launch,
+        "launches": launches,
     }
-
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline_pagerank(e, params, args.cpu_iters)
-    ua._lib.check(lib.ursa_hip_device_free(out))
+        cpu = cpu_baseline_bfs(e, sources, args.cpu_sources)
+    for b in (lv, d_out, d_in, d_cc, d_tri):
+        b.free()
     config = {
-        "workload": f"pagerank_f64 rmat{scale}",
-        "graph": f"R-MAT scale {scale}, edge factor 16, (0.57,0.19,0.19,0.05), seed {seed}, dedup, no self-loops",
+        "workload": f"integer kernels on R-MAT scale {scale}: degree in/out, BFS x64 (Out), weak CC, triangle_count",
+        "graph": f"R-MAT scale {scale}, edge factor 16, (0.57,0.19,0.19,0.05), seed 2, dedup, no self-loops",
         "n": n,
         "m": m,
-        "damping": params.damping,
-        "tol": params.tol,
-        "max_iter": params.max_iter,
-        "iterations": iters.value,
+        "m_undirected": m_und,
+        "bfs_sources": 64,
+        "bfs_E_reach": e_reach,
+        "build_ms": {"transpose": round(t_in * 1e3, 1), "undirected": round(t_und * 1e3, 1)},
     }
     return {
-        "metric": "GTEPS (pull PageRank SpMV, m*iterations/s)",
+        "metric": "GTEPS (aggregate over the step's kernels: BFS E_reach + CC m + triangle m_und per second)",
         "value": round(gteps, 3),
         "unit": "GTEPS",
         "ms_per_step": round(ms_per_step, 3),
-        "dtype": "f64",
+        "dtype": "u32/i32/u64 (integer kernels)",
         "config": config,
         "roofline": roofline,
+        "kernels": kernels,
         "cpu_baseline": cpu,
+        "tri_launch": tri_q,
     }
 
 
-def cpu_baseline_pagerank(e, params, cpu_iters):
+def cpu_baseline_bfs(e, sources, count):
     import oracle as orc
 
-    threads = os.cpu_count() or 1
     t0 = time.time()
     _, _, sd, dd, _ = e.export()
     t = orc.Topology(e.n, sd, dd)
-    orc.degree(t, "in")  # builds the transpose outside the timed region
     log(f"cpu baseline: oracle topology built in {time.time() - t0:.1f}s")
-    t1 = time.perf_counter()
-    _, it, _ = orc.pagerank(t, params.damping, cpu_iters, params.tol, nthreads=threads)
-    el = time.perf_counter() - t1
-    gteps = e.m * it / el / 1e9
-    log(f"cpu baseline: {it} iterations in {el:.2f}s on {threads} threads: {gteps:.3f} GTEPS")
+    outdeg = orc.degree(t, "out").astype(np.int64)
+    sample = [int(x) for x in sources[:count]]
+    tot_edges = 0
+    el = 0.0
+    for s in sample:
+        t1 = time.perf_counter()
+        lv = orc.bfs(t, s, "out")
+        el += time.perf_counter() - t1
+        tot_edges += int(outdeg[lv >= 0].sum())
+    gteps = tot_edges / el / 1e9
+    log(f"cpu baseline: {len(sample)} serial BFS in {el:.2f}s: {gteps:.4f} GTEPS")
     return {
-        "value": round(gteps, 4),
+        "value": round(gteps, 5),
         "unit": "GTEPS",
-        "cores": threads,
+        "cores": 1,
         "kind": "port",
-        "sample": f"same graph, {it} PageRank iterations (max_iter={cpu_iters}) of the oracle's "
-                  f"rayon-structured pull loop, {threads} threads",
+        "sample": f"oracle bfs_distances (the reference's serial frontier loop) from the first {len(sample)} of "
+                  f"the 64 benchmark sources on the same graph",
     }
+
+
+# --------------------------------------------------------------------- PageRank
+def bench_pagerank(args, ua, gen):
+    lib = ua.load()
+    scale = args.pr_scale
+    params = ua.PageRankParams(0.85, 100, 1e-10)
+    t0 = time.time()
+    e = gen.rmat(scale, 16, 0.57, 0.19, 0.19, seed=3)
+    topo = e.topology()
+    check(lib.ursa_hip_topology_incoming(topo._h))
+    n, m = e.n, e.m
+    log(f"pagerank graph R-MAT s{scale}: n={n} m={m} built in {time.time() - t0:.1f}s")
+    out = Dev(lib, 8 * n)
+    it, it32 = C.c_uint32(0), C.c_uint32(0)
+
+    def plain():
+        check(lib.ursa_hip_pagerank(topo._h, None, params.damping, params.max_iter, params.tol, out.p, C.byref(it)))
+
+    def mixed():
+        check(lib.ursa_hip_pagerank_mixed(topo._h, None, params.damping, params.max_iter, params.tol, out.p,
+                                          C.byref(it), C.byref(it32)))
+
+    res = {}
+    for name, fn in (("f64", plain), ("mixed", mixed)):
+        fn()
+        check(lib.ursa_hip_synchronize())
+        t1 = time.perf_counter()
+        for _ in range(args.pr_steps):
+            fn()
+        check(lib.ursa_hip_synchronize())
+        ms = (time.perf_counter() - t1) / args.pr_steps * 1e3
+        res[name] = {"ms": round(ms, 3), "iters": it.value, "GTEPS": round(m * it.value / (ms / 1e3) / 1e9, 3)}
+        if name == "mixed":
+            res[name]["f32_iters"] = it32.value
+        log(f"pagerank {name}: {ms:.2f} ms, {it.value} iterations")
+    # kernel roofline of the f64 SpMV tile kernel
+    ua.profile.enable(True)
+    ua.profile.reset()
+    plain()
+    q = ua.profile.query("pagerank_spmv")
+    mixed()
+    q32 = ua.profile.query("pagerank_spmv_f32")
+    ua.profile.enable(False)
+    for key, qq in (("k_pr_tiles_f64", q), ("k_pr_tiles_f32", q32)):
+        if qq["launches"]:
+            avg = qq["total_ms"] / qq["launches"]
+            bpl = qq["algo_bytes"] / qq["launches"]
+            gbs = bpl / (avg / 1e3) / 1e9
+            res[key] = {"avg_launch_us": round(avg * 1e3, 2), "algo_bytes_per_launch": bpl, "GBps": round(gbs, 1),
+                        "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    res["mixed_speedup"] = round(res["f64"]["ms"] / res["mixed"]["ms"], 3)
+    # accuracy against the f64 oracle
+    if not args.no_cpu_baseline:
+        import oracle as orc
+
+        got64 = np.zeros(n, np.float64)
+        check(lib.ursa_hip_pagerank(topo._h, None, params.damping, params.max_iter, params.tol,
+                                    got64.ctypes.data_as(C.c_void_p), C.byref(it)))
+        gotmx = np.zeros(n, np.float64)
+        check(lib.ursa_hip_pagerank_mixed(topo._h, None, params.damping, params.max_iter, params.tol,
+                                          gotmx.ctypes.data_as(C.c_void_p), C.byref(it), C.byref(it32)))
+        t2 = time.time()
+        _, _, sd, dd, _ = e.export()
+        ot = orc.Topology(n, sd, dd)
+        orc.degree(ot, "in")
+        threads = os.cpu_count() or 1
+        t3 = time.perf_counter()
+        ref, rit, _ = orc.pagerank(ot, params.damping, params.max_iter, params.tol, nthreads=threads)
+        cpu_s = time.perf_counter() - t3
+        log(f"pagerank oracle: {rit} iterations in {cpu_s:.1f}s on {threads} threads (setup {time.time() - t2:.0f}s)")
+        top_r = set(np.argsort(-ref, kind="stable")[:1000].tolist())
+        for name, g in (("f64", got64), ("mixed", gotmx)):
+            rel = np.abs(g - ref) / np.abs(ref)
+            res[name]["L1_vs_oracle"] = float(np.abs(g - ref).sum())
+            res[name]["maxabs_vs_oracle"] = float(np.abs(g - ref).max())
+            res[name]["max_rel_vs_oracle"] = float(rel.max())
+            res[name]["top1000_overlap"] = len(top_r & set(np.argsort(-g, kind="stable")[:1000].tolist()))
+        res["cpu_oracle"] = {"GTEPS": round(m * rit / cpu_s / 1e9, 4), "iters": rit, "threads": threads}
+    out.free()
+    res["graph"] = {"scale": scale, "seed": 3, "n": n, "m": m, "damping": 0.85, "tol": 1e-10, "max_iter": 100}
+    return res
 
 
 def main():
@@ -157,21 +351,24 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="pagerank", choices=["pagerank"])
-    ap.add_argument("--scale", type=int, default=None)
-    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--scale", type=int, default=None, help="R-MAT scale of the integer-kernel graph (22)")
+    ap.add_argument("--pr-scale", type=int, default=24)
+    ap.add_argument("--pr-steps", type=int, default=3)
+    ap.add_argument("--no-pagerank", action="store_true")
+    ap.add_argument("--cpu-sources", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
-                                   "described in DESIGN.md, not implemented)"}))
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (how the path would shard over "
+                                   "several GPUs is described in DESIGN.md, not implemented)"}))
         sys.exit(2)
 
     import ursa_amd as ua
     from ursa_amd import gen
 
     ua.init(0)
-    res = bench_pagerank(args, ua, gen)
+    res = bench_c2(args, ua, gen)
+    pr = None if args.no_pagerank else bench_pagerank(args, ua, gen)
     line = {
         "metric": res["metric"],
         "value": res["value"],
@@ -188,6 +385,8 @@ def main():
         "config": res["config"],
         "roofline": res["roofline"],
         "cpu_baseline": res["cpu_baseline"],
+        "kernels": res["kernels"],
+        "pagerank": pr,
     }
     print(json.dumps(line), flush=True)
 

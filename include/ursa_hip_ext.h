@@ -30,16 +30,23 @@ int ursa_hip_profile_query(const char* kernel, double* total_ms, uint64_t* launc
 uint64_t ursa_hip_profile_names(char* buf, uint64_t cap);
 
 /* ---- traversal statistics for benchmarks ----
- * As ursa_hip_bfs, also returning the TEPS numerator: the sum over all k
- * traversals of the forward degrees of the reached vertices. */
+ * As ursa_hip_bfs, also returning the TEPS numerator (the sum over all k
+ * traversals of the forward degrees of the reached vertices) and choosing the
+ * schedule: mode 0 = automatic (what ursa_hip_bfs does: 64-lane multi-source
+ * BFS for k >= 4), 1 = one direction-optimizing traversal per source,
+ * 2 = 64-lane multi-source BFS. All modes give identical levels. */
 int ursa_hip_bfs_ex(ursa_hip_topo* t, const uint32_t* sources, uint32_t k, int direction,
-                    int32_t* levels, uint64_t* reached_edges);
+                    int32_t* levels, uint64_t* reached_edges, int mode);
 /* As ursa_hip_betweenness_sources with an explicit batch size (1..64 sources
  * advancing together; 1 = one traversal per launch sequence) and the TEPS
  * numerator (sum of out-degrees of reached vertices over all sources). */
 int ursa_hip_betweenness_sources_ex(ursa_hip_topo* t, const uint32_t* sources, uint64_t k,
                                     int apply_scale, double scale, uint32_t batch, double* out,
                                     uint64_t* reached_edges);
+
+/* The reference's seeded permutation of 0..n-1 (shuffled_order, rng.rs:64-73),
+ * used to pick sampled sources. Host code; out is u32[n]. */
+int ursa_hip_shuffled_order(uint64_t n, uint64_t seed, uint32_t* out);
 
 /* ---- mixed-precision PageRank ----
  * Same contract as ursa_hip_pagerank (pagerank.rs:98-169). The first sweeps

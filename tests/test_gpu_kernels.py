@@ -44,26 +44,49 @@ def sources_for(n):
 
 
 # ---------------------------------------------------------------- BFS
+@pytest.mark.parametrize("mode", [1, 2], ids=["per_source", "multi_source"])
 @pytest.mark.parametrize("case", GRAPHS, ids=IDS)
-def test_bfs_levels_match(ua, case):
+def test_bfs_levels_match(ua, case, mode):
     name, n, s, d = case
     t = ua.Topology.build(n, s, d)
     o = orc.Topology(n, s, d)
     src = sources_for(n)
     for direction in DIRS:
-        got = ua.bfs_multi(t, src, direction)
+        got = ua.bfs_multi(t, src, direction, mode=mode)
         for i, x in enumerate(src):
             assert np.array_equal(got[i], orc.bfs(o, x, direction)), (name, direction, x)
 
 
-def test_bfs_rmat_direction_optimizing(ua, rmat14):
+@pytest.mark.parametrize("mode", [1, 2], ids=["per_source", "multi_source"])
+def test_bfs_rmat_direction_optimizing(ua, rmat14, mode):
     n, _, _, t, o = rmat14
     rng = np.random.default_rng(1)
     src = rng.integers(0, n, 12)
     for direction in DIRS:
-        got = ua.bfs_multi(t, src, direction)
+        got = ua.bfs_multi(t, src, direction, mode=mode)
         for i, x in enumerate(src):
             assert np.array_equal(got[i], orc.bfs(o, int(x), direction)), (direction, int(x))
+
+
+def test_msbfs_two_batches_with_duplicates(ua, rmat14):
+    n, _, _, t, o = rmat14
+    rng = np.random.default_rng(2)
+    src = np.concatenate([rng.integers(0, n, 97), [5, 5, n + 3]])
+    got = ua.bfs_multi(t, src, "out", mode=2)
+    for i, x in enumerate(src.tolist()):
+        assert np.array_equal(got[i], orc.bfs(o, int(x), "out")), (i, int(x))
+
+
+def test_bfs_hub_frontier(ua):
+    # source -> hub with 20000 out-edges: the top-down level runs the grid-wide path
+    hub_deg = 20000
+    s = np.concatenate([[0], np.ones(hub_deg, np.int64), np.arange(2, 2 + hub_deg)])
+    d = np.concatenate([[1], np.arange(2, 2 + hub_deg), np.arange(3, 3 + hub_deg) % (2 + hub_deg)])
+    n = 2 + hub_deg
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    for mode in (1, 2):
+        assert np.array_equal(ua.bfs_multi(t, [0], "out", mode=mode)[0], orc.bfs(o, 0, "out"))
 
 
 def test_bfs_goldens(ua):  # bfs.rs:266-298

@@ -189,13 +189,17 @@ def pagerank_weighted(topo: Topology, weights, params: PageRankParams = PageRank
     return _pagerank(topo, weights, params)[0]
 
 
-def bfs_multi(topo: Topology, sources, direction=Direction.Out) -> np.ndarray:
-    """bfs_distances from each source -> i32[k, n] (ursa-core/src/algo/bfs.rs:18-42)."""
+def bfs_multi(topo: Topology, sources, direction=Direction.Out, mode: int = 0) -> np.ndarray:
+    """bfs_distances from each source -> i32[k, n] (ursa-core/src/algo/bfs.rs:18-42).
+
+    mode 0 = automatic, 1 = one traversal per source, 2 = 64-lane multi-source BFS.
+    """
     src = _u32(sources)
     k, n = src.shape[0], topo.n_nodes()
     out = np.zeros((max(k, 1), max(n, 1)), np.int32)
+    re = C.c_uint64(0)
     if k and n:
-        check(load().ursa_hip_bfs(topo._h, _ptr(src), k, _dir(direction), _ptr(out)))
+        check(load().ursa_hip_bfs_ex(topo._h, _ptr(src), k, _dir(direction), _ptr(out), C.byref(re), mode))
     return out[:k, :n]
 
 
@@ -267,6 +271,13 @@ def connected_components_weak(topo: Topology) -> np.ndarray:
     n = topo.n_nodes()
     out = np.zeros(max(n, 1), np.uint32)
     check(load().ursa_hip_cc_weak(topo._h, _ptr(out)))
+    return out[:n]
+
+
+def shuffled_order(n: int, seed: int) -> np.ndarray:
+    """The reference's seeded permutation of 0..n-1 (ursa-core/src/algo/rng.rs:64-73)."""
+    out = np.zeros(max(n, 1), np.uint32)
+    check(load().ursa_hip_shuffled_order(n, seed, _ptr(out)))
     return out[:n]
 
 

@@ -48,7 +48,7 @@ SIGNATURES = {
     "ursa_hip_pagerank": (cint, [P, P, f64, u32, f64, P, C.POINTER(u32)]),
     "ursa_hip_pagerank_mixed": (cint, [P, P, f64, u32, f64, P, C.POINTER(u32), C.POINTER(u32)]),
     "ursa_hip_bfs": (cint, [P, P, u32, cint, P]),
-    "ursa_hip_bfs_ex": (cint, [P, P, u32, cint, P, C.POINTER(u64)]),
+    "ursa_hip_bfs_ex": (cint, [P, P, u32, cint, P, C.POINTER(u64), cint]),
     "ursa_hip_k_hop": (cint, [P, P, u64, u32, cint, C.POINTER(P)]),
     "ursa_hip_pairs_size": (cint, [P, C.POINTER(u64)]),
     "ursa_hip_pairs_export": (cint, [P, P, P]),
@@ -58,6 +58,7 @@ SIGNATURES = {
     "ursa_hip_betweenness_sources": (cint, [P, P, u64, cint, f64, P]),
     "ursa_hip_betweenness_sources_ex": (cint, [P, P, u64, cint, f64, u32, P, C.POINTER(u64)]),
     "ursa_hip_triangle_count": (cint, [P, P]),
+    "ursa_hip_shuffled_order": (cint, [u64, u64, P]),
     "ursa_hip_cc_weak": (cint, [P, P]),
     "ursa_hip_profile_enable": (cint, [cint]),
     "ursa_hip_profile_reset": (cint, []),

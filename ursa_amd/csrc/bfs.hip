@@ -33,9 +33,13 @@ constexpr int WAVES = BFS_BLOCK / 64;
 
 struct BfsCtr {
     uint32_t qlen;   // next-queue length (top-down) / newly visited count (bottom-up)
-    uint32_t pad;
+    uint32_t heavy;  // frontier vertices deferred to the grid-wide expansion
     unsigned long long edges;  // forward degree sum of the newly visited vertices
 };
+
+// Frontier vertices with more forward edges than this are expanded by the
+// whole grid (k_bfs_td_heavy) instead of by one wave.
+constexpr uint32_t TD_HEAVY = 1024;
 
 __global__ void k_bfs_init(int32_t* level, uint64_t n, uint32_t src, uint32_t* queue, BfsCtr* ctr, NbrView fwd) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -49,12 +53,26 @@ __global__ void k_bfs_init(int32_t* level, uint64_t n, uint32_t src, uint32_t* q
 
 __global__ void k_ctr_reset(BfsCtr* ctr) {
     ctr->qlen = 0;
+    ctr->heavy = 0;
     ctr->edges = 0;
+}
+
+// Claim v for level L and append it to the next queue (wave-aggregated).
+__device__ __forceinline__ void td_visit(bool active, uint32_t v, const NbrView& fwd, int32_t* level, int32_t L,
+                                         uint32_t* nq, BfsCtr* ctr, uint64_t& edges) {
+    bool claimed = false;
+    if (active && level[v] < 0) claimed = atomicCAS(&level[v], -1, L) == -1;
+    const uint32_t slot = wave_append_slot(claimed, &ctr->qlen);
+    if (claimed) {
+        nq[slot] = v;
+        edges += nv_degree(fwd, v);
+    }
 }
 
 __global__ __launch_bounds__(BFS_BLOCK) void k_bfs_td(const uint32_t* __restrict__ q, uint32_t qlen, NbrView fwd,
                                                       int32_t* __restrict__ level, int32_t L,
-                                                      uint32_t* __restrict__ nq, BfsCtr* ctr) {
+                                                      uint32_t* __restrict__ nq, BfsCtr* ctr,
+                                                      uint32_t* __restrict__ heavy) {
     const int lane = lane_id();
     const uint64_t gw = blockIdx.x * (uint64_t)WAVES + (threadIdx.x >> 6);
     const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
@@ -67,6 +85,10 @@ __global__ __launch_bounds__(BFS_BLOCK) void k_bfs_td(const uint32_t* __restrict
             d = nv_degree(fwd, u, da);
             sa = fwd.off_a[u];
             if (fwd.off_b) sb = fwd.off_b[u];
+            if (d > TD_HEAVY) {
+                heavy[atomicAdd(&ctr->heavy, 1u)] = u;
+                d = 0;
+            }
         }
         const uint32_t incl = wave_incl_scan(d);
         const uint32_t excl = incl - d;
@@ -78,21 +100,39 @@ __global__ __launch_bounds__(BFS_BLOCK) void k_bfs_td(const uint32_t* __restrict
             const uint32_t oa = __shfl(da, j, 64);
             const uint32_t osa = __shfl(sa, j, 64);
             const uint32_t osb = __shfl(sb, j, 64);
-            bool claimed = false;
             uint32_t v = 0;
-            if (t < total) {
-                v = e < oa ? fwd.tgt_a[osa + e] : fwd.tgt_b[osb + (e - oa)];
-                if (level[v] < 0) claimed = atomicCAS(&level[v], -1, L) == -1;
-            }
-            const uint32_t slot = wave_append_slot(claimed, &ctr->qlen);
-            if (claimed) {
-                nq[slot] = v;
-                edges += nv_degree(fwd, v);
-            }
+            if (t < total) v = e < oa ? fwd.tgt_a[osa + e] : fwd.tgt_b[osb + (e - oa)];
+            td_visit(t < total, v, fwd, level, L, nq, ctr, edges);
         }
     }
     edges = wave_sum_u64(edges);
     if (lane == 0 && edges) atomicAdd(&ctr->edges, edges);
+}
+
+// Heavy frontier vertices: every wave of the grid takes a stride of each
+// heavy vertex's edge list, so a hub's edges spread over the whole device.
+__global__ __launch_bounds__(BFS_BLOCK) void k_bfs_td_heavy(const uint32_t* __restrict__ heavy, NbrView fwd,
+                                                            int32_t* __restrict__ level, int32_t L,
+                                                            uint32_t* __restrict__ nq, BfsCtr* ctr) {
+    const uint32_t cnt = ctr->heavy;
+    const uint64_t wave_base = (blockIdx.x * (uint64_t)WAVES + (threadIdx.x >> 6)) * 64;
+    const uint64_t stride = (uint64_t)gridDim.x * BFS_BLOCK;
+    uint64_t edges = 0;
+    for (uint32_t h = 0; h < cnt; ++h) {
+        const uint32_t u = heavy[h];
+        uint32_t da;
+        const uint32_t d = nv_degree(fwd, u, da);
+        const uint32_t sa = fwd.off_a[u];
+        const uint32_t sb = fwd.off_b ? fwd.off_b[u] : 0;
+        for (uint64_t e0 = wave_base; e0 < d; e0 += stride) {
+            const uint64_t e = e0 + lane_id();
+            uint32_t v = 0;
+            if (e < d) v = e < da ? fwd.tgt_a[sa + e] : fwd.tgt_b[sb + (e - da)];
+            td_visit(e < d, v, fwd, level, L, nq, ctr, edges);
+        }
+    }
+    edges = wave_sum_u64(edges);
+    if (lane_id() == 0 && edges) atomicAdd(&ctr->edges, edges);
 }
 
 // Bottom-up step: vertices of level L are those unvisited with a reverse
@@ -199,10 +239,11 @@ void direction_views(ursa_hip_topo* t, int dir, hipStream_t s, NbrView& fwd, Nbr
 }
 
 struct BfsWork {
-    DevBuf<uint32_t> q0, q1, b0, b1;
+    DevBuf<uint32_t> q0, q1, b0, b1, heavy;
     DevBuf<BfsCtr> ctr;
     BfsCtr* h = nullptr;  // pinned mirror of ctr
-    explicit BfsWork(uint64_t n) : q0(n ? n : 1), q1(n ? n : 1), b0(((n + 63) / 64) * 2), b1(((n + 63) / 64) * 2), ctr(1) {
+    explicit BfsWork(uint64_t n)
+        : q0(n ? n : 1), q1(n ? n : 1), b0(((n + 63) / 64) * 2), b1(((n + 63) / 64) * 2), heavy(n ? n : 1), ctr(1) {
         HIP_OK(hipHostMalloc((void**)&h, sizeof(BfsCtr), hipHostMallocDefault));
     }
     ~BfsWork() {
@@ -267,7 +308,11 @@ static void bfs_one(ursa_hip_topo* t, uint32_t src, const NbrView& fwd, const Nb
             k_ctr_reset<<<1, 1, 0, s>>>(w.ctr.get());
             const unsigned tb = blocks_for((qlen + 63) / 64, WAVES, 1u << 16);
             timed("bfs_top_down", 4.0 * (double)scout + 8.0 * (double)qlen, s, [&] {
-                k_bfs_td<<<tb, BFS_BLOCK, 0, s>>>(q, (uint32_t)qlen, fwd, level, L, nq, w.ctr.get());
+                k_bfs_td<<<tb, BFS_BLOCK, 0, s>>>(q, (uint32_t)qlen, fwd, level, L, nq, w.ctr.get(), w.heavy.get());
+                if (scout > TD_HEAVY) {
+                    const unsigned hb = blocks_for(scout, BFS_BLOCK, 2048);
+                    k_bfs_td_heavy<<<hb, BFS_BLOCK, 0, s>>>(w.heavy.get(), fwd, level, L, nq, w.ctr.get());
+                }
             });
             check_launch();
             c = read_ctr();
@@ -281,8 +326,13 @@ static void bfs_one(ursa_hip_topo* t, uint32_t src, const NbrView& fwd, const Nb
     if (reached_edges) *reached_edges = total_edges;
 }
 
+void msbfs_levels(ursa_hip_topo* t, const std::vector<uint32_t>& src, int dir, int32_t* out_dev,
+                  uint64_t* reached_edges, uint32_t push_div);
+
+// mode: 0 = automatic (64-lane multi-source BFS for 4 or more sources),
+// 1 = one direction-optimizing traversal per source, 2 = multi-source.
 void bfs_levels(ursa_hip_topo* t, const uint32_t* sources, uint64_t k, int dir, int32_t* out_user,
-                uint64_t* reached_edges) {
+                uint64_t* reached_edges, int mode) {
     const uint64_t n = t->n;
     if (reached_edges) *reached_edges = 0;
     if (n == 0 || k == 0) return;
@@ -297,6 +347,14 @@ void bfs_levels(ursa_hip_topo* t, const uint32_t* sources, uint64_t k, int dir, 
         HIP_OK(hipMemcpy(src.data(), sources, k * sizeof(uint32_t), hipMemcpyDeviceToHost));
     else
         std::copy(sources, sources + k, src.begin());
+    const bool multi = mode == 2 || (mode == 0 && k >= 4);
+    if (multi) {
+        uint64_t re = 0;
+        msbfs_levels(t, src, dir, out.p, reached_edges ? &re : nullptr, 20);
+        out.finish(s);
+        if (reached_edges) *reached_edges = re;
+        return;
+    }
     uint64_t sum = 0;
     for (uint64_t i = 0; i < k; ++i) {
         int32_t* lv = out.p + i * n;
@@ -323,19 +381,20 @@ int ursa_hip_bfs(ursa_hip_topo* t, const uint32_t* sources, uint32_t k, int dire
         URSA_REQUIRE(direction >= 0 && direction <= 2, URSA_HIP_EINVAL, "bad direction");
         if (t->n == 0 || k == 0) return;
         URSA_REQUIRE(sources != nullptr && levels != nullptr, URSA_HIP_EINVAL, "sources/levels is NULL");
-        ursa::bfs_levels(t, sources, k, direction, levels, nullptr);
+        ursa::bfs_levels(t, sources, k, direction, levels, nullptr, 0);
     });
 }
 
 int ursa_hip_bfs_ex(ursa_hip_topo* t, const uint32_t* sources, uint32_t k, int direction, int32_t* levels,
-                    uint64_t* reached_edges) {
+                    uint64_t* reached_edges, int mode) {
     return ursa::guard([&] {
         URSA_REQUIRE(t != nullptr, URSA_HIP_EINVAL, "topology is NULL");
         URSA_REQUIRE(direction >= 0 && direction <= 2, URSA_HIP_EINVAL, "bad direction");
         if (reached_edges) *reached_edges = 0;
         if (t->n == 0 || k == 0) return;
         URSA_REQUIRE(sources != nullptr && levels != nullptr, URSA_HIP_EINVAL, "sources/levels is NULL");
-        ursa::bfs_levels(t, sources, k, direction, levels, reached_edges);
+        URSA_REQUIRE(mode >= 0 && mode <= 2, URSA_HIP_EINVAL, "mode must be 0, 1 or 2");
+        ursa::bfs_levels(t, sources, k, direction, levels, reached_edges, mode);
     });
 }
 

@@ -325,6 +325,15 @@ int ursa_hip_betweenness(ursa_hip_topo* t, int has_sample, double sample, int ha
     });
 }
 
+int ursa_hip_shuffled_order(uint64_t n, uint64_t seed, uint32_t* out) {
+    return ursa::guard([&] {
+        if (n == 0) return;
+        URSA_REQUIRE(out != nullptr, URSA_HIP_EINVAL, "out is NULL");
+        std::vector<uint32_t> o = ursa::shuffled_order(n, seed);
+        std::copy(o.begin(), o.end(), out);
+    });
+}
+
 int ursa_hip_betweenness_sources_ex(ursa_hip_topo* t, const uint32_t* sources, uint64_t k, int apply_scale,
                                     double scale, uint32_t batch, double* out, uint64_t* reached_edges) {
     return ursa::guard([&] {
