@@ -129,7 +129,7 @@ def bench_c2(args, ua, gen):
 
         run("degree", lambda: (check(lib.ursa_hip_degree(topo._h, 0, d_out.p)),
                                check(lib.ursa_hip_degree(topo._h, 1, d_in.p))))
-        run("bfs", lambda: check(lib.ursa_hip_bfs_ex(topo._h, src_ptr, 64, 0, lv.p, C.byref(reached))))
+        run("bfs", lambda: check(lib.ursa_hip_bfs_ex(topo._h, src_ptr, 64, 0, lv.p, C.byref(reached), 0)))
         run("cc", lambda: check(lib.ursa_hip_cc_weak(topo._h, d_cc.p)))
         run("triangle", lambda: check(lib.ursa_hip_triangle_count(topo._h, d_tri.p)))
 
@@ -153,7 +153,7 @@ def bench_c2(args, ua, gen):
     ua.profile.reset()
     step()
     groups = {
-        "bfs": ["bfs_top_down", "bfs_bottom_up"],
+        "bfs": ["bfs_top_down", "bfs_bottom_up", "msbfs_push", "msbfs_pull"],
         "cc": ["cc_hook_round", "cc_compress", "cc_hook_out", "cc_hook_in"],
         "triangle": ["triangle_count"],
         "degree": ["degree"],
@@ -187,7 +187,7 @@ def bench_c2(args, ua, gen):
         ms = stats["bfs"][0]
         per_launch = algo["bfs"] / 64
         avg_us = ms / 64 * 1e3
-        kname = "k_bfs_top_down+k_bfs_bottom_up (one 64-source row = 64 traversals)"
+        kname = "k_ms_push+k_ms_pull (64-lane multi-source BFS; per-traversal average)"
         launches = 64
     else:
         ms, launches, b = stats[dom]
