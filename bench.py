@@ -357,6 +357,10 @@ def main():
     ap.add_argument("--no-pagerank", action="store_true")
     ap.add_argument("--cpu-sources", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["c2", "pagerank"], default=None,
+                    help="run one section only (for profiling runs)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run the calibration copy kernel over this many MiB")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (how the path would shard over "
@@ -367,7 +371,21 @@ def main():
     from ursa_amd import gen
 
     ua.init(0)
+    if args.copy_probe:
+        lib = ua.load()
+        nbytes = args.copy_probe << 20
+        a, b = Dev(lib, nbytes), Dev(lib, nbytes)
+        for _ in range(3):
+            check(lib.ursa_hip_copy_probe(a.p, b.p, nbytes))
+        a.free()
+        b.free()
+        log(f"copy probe: 3 x {args.copy_probe} MiB")
+    if args.only == "pagerank":
+        print(json.dumps({"pagerank": bench_pagerank(args, ua, gen)}), flush=True)
+        return
     res = bench_c2(args, ua, gen)
+    if args.only == "c2":
+        args.no_pagerank = True
     pr = None if args.no_pagerank else bench_pagerank(args, ua, gen)
     line = {
         "metric": res["metric"],

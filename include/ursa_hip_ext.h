@@ -58,6 +58,11 @@ int ursa_hip_pagerank_mixed(ursa_hip_topo* t, const double* weights, double damp
                             uint32_t max_iter, double tol, double* out, uint32_t* iters_out,
                             uint32_t* f32_iters_out);
 
+/* Streaming device-to-device copy of `bytes` (multiple of 16) by a plain
+ * kernel: the reference point for hardware byte counters (k_copy_probe moves
+ * exactly `bytes` in and `bytes` out). */
+int ursa_hip_copy_probe(const void* src_device, void* dst_device, uint64_t bytes);
+
 /* ---- device memory for callers without a HIP runtime binding ---- */
 int ursa_hip_device_alloc(uint64_t bytes, void** out);
 int ursa_hip_device_free(void* p);

@@ -105,6 +105,13 @@ __global__ void k_check_range(const uint32_t* a, const uint32_t* b, uint64_t m, 
 
 constexpr unsigned B = 256;
 
+// Plain streaming copy, 16 B per thread per step: the calibration kernel for
+// hardware byte counters (its traffic is exactly `bytes` read + `bytes` written).
+__global__ void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n16) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace
 
 void build_grouped(uint64_t n, uint64_t m, const uint32_t* keys, const uint32_t* other, DevCsr& out,
@@ -309,6 +316,21 @@ int ursa_hip_topology_export(ursa_hip_topo* t, int which, uint64_t* offsets, uin
             HIP_OK(hipMemcpyAsync(o.p, c.eid.get(), c.slots * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
             o.finish(s);
         }
+        HIP_OK(hipStreamSynchronize(s));
+    });
+}
+
+int ursa_hip_copy_probe(const void* src_device, void* dst_device, uint64_t bytes) {
+    return guard([&] {
+        URSA_REQUIRE(bytes % 16 == 0, URSA_HIP_EINVAL, "bytes must be a multiple of 16");
+        URSA_REQUIRE(is_device_ptr(src_device) && is_device_ptr(dst_device), URSA_HIP_EINVAL,
+                     "copy probe takes device buffers");
+        hipStream_t s = stream();
+        timed("copy_probe", 2.0 * (double)bytes, s, [&] {
+            k_copy_probe<<<blocks_for(bytes / 16, 256, 1u << 16), 256, 0, s>>>((const uint4*)src_device,
+                                                                              (uint4*)dst_device, bytes / 16);
+        });
+        check_launch();
         HIP_OK(hipStreamSynchronize(s));
     });
 }
