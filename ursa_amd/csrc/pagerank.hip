@@ -31,6 +31,7 @@
 #include <cmath>
 #include <vector>
 
+#include "prim.hpp"
 #include "topology.hpp"
 
 namespace ursa {
@@ -134,19 +135,19 @@ __device__ __forceinline__ void pr_finish_row(uint64_t u, double acc, double bas
                                               const double* rank, double* next, CT* contrib_next,
                                               double& delta, double& dangling) {
     const double v = base + damping * acc;
-    next[u] = v;
+    __builtin_nontemporal_store(v, &next[u]);
     const bool has_out = scale > 0.0;
     contrib_next[u] = (CT)(has_out ? v / scale : 0.0);
-    delta += fabs(v - rank[u]);
+    delta += fabs(v - __builtin_nontemporal_load(&rank[u]));
     if (!has_out) dangling += v;
 }
 
 template <bool WEIGHTED>
 __device__ __forceinline__ double node_scale(const uint32_t* out_deg, const double* out_str, uint64_t u) {
     if constexpr (WEIGHTED)
-        return out_str[u];
+        return __builtin_nontemporal_load(&out_str[u]);
     else
-        return (double)out_deg[u];
+        return (double)__builtin_nontemporal_load(&out_deg[u]);
 }
 
 template <class CT, class W, bool WEIGHTED>
@@ -173,12 +174,12 @@ __global__ __launch_bounds__(PR_BLOCK) void k_pr_tiles(
 
     for (uint64_t i = threadIdx.x; i < nnz; i += PR_BLOCK) {
         const uint64_t k = y0 + i;
-        CT c = contrib[in_tgt[k]];
-        if constexpr (WEIGHTED) c = (CT)w_in[k] * c;
+        CT c = contrib[__builtin_nontemporal_load(&in_tgt[k])];
+        if constexpr (WEIGHTED) c = (CT)__builtin_nontemporal_load(&w_in[k]) * c;
         s_val[i] = c;
     }
     if (x0 < n)
-        for (uint64_t i = threadIdx.x; i < n_end; i += PR_BLOCK) s_end[i] = in_off[x0 + 1 + i];
+        for (uint64_t i = threadIdx.x; i < n_end; i += PR_BLOCK) s_end[i] = __builtin_nontemporal_load(&in_off[x0 + 1 + i]);
     const bool starts_mid = (x0 < n) && ((uint64_t)in_off[x0] < y0);
     __syncthreads();
 
@@ -393,6 +394,45 @@ __global__ void k_pr_contrib64(uint64_t n, const uint32_t* __restrict__ out_deg,
     }
 }
 
+// ---- renumbering by decreasing out-degree ----
+__global__ void k_rl_keys(const uint32_t* __restrict__ out_deg, uint64_t n, uint32_t* keys, uint32_t* iota) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        keys[v] = ~out_deg[v];
+        iota[v] = (uint32_t)v;
+    }
+}
+
+__global__ void k_rl_invert(const uint32_t* __restrict__ old_of_new, uint64_t n, const uint32_t* __restrict__ in_off,
+                            const uint32_t* __restrict__ out_deg, uint32_t* __restrict__ new_of_old,
+                            uint32_t* __restrict__ rdeg, uint32_t* __restrict__ routdeg) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t o = old_of_new[i];
+        new_of_old[o] = (uint32_t)i;
+        rdeg[i] = in_off[o + 1] - in_off[o];
+        routdeg[i] = out_deg[o];
+    }
+}
+
+// Row u's in-slot j moves to row new(u) slot j: the order inside each row, and
+// so every row's summation order, is unchanged.
+__global__ void k_rl_slots(const uint32_t* __restrict__ owner, const uint32_t* __restrict__ in_off,
+                           const uint32_t* __restrict__ in_tgt, const uint32_t* __restrict__ in_eid,
+                           const uint32_t* __restrict__ new_of_old, const uint32_t* __restrict__ r_off, uint64_t m,
+                           uint32_t* __restrict__ r_tgt, uint32_t* __restrict__ r_eid) {
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < m; k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t u = owner[k];
+        const uint64_t dest = (uint64_t)r_off[new_of_old[u]] + (k - in_off[u]);
+        r_tgt[dest] = new_of_old[in_tgt[k]];
+        r_eid[dest] = in_eid[k];
+    }
+}
+
+__global__ void k_gather_f64(const double* __restrict__ src, const uint32_t* __restrict__ idx, uint64_t n,
+                             double* __restrict__ out) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = src[idx[i]];
+}
+
 __global__ void k_fill(double* p, uint64_t n, double v) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -400,20 +440,49 @@ __global__ void k_fill(double* p, uint64_t n, double v) {
 
 }  // namespace
 
-// Cached merge-path partition of the in-CSR.
-static void pr_partition(ursa_hip_topo* t, const DevCsr& in, hipStream_t s) {
+// PageRank's working copy of the in-CSR, built once per topology: vertices
+// renumbered by decreasing out-degree (ties by id), rows moved with their
+// in-slot order intact, sources relabelled. The most-gathered contributions
+// (high out-degree sources) then sit in a few megabytes at the front of the
+// contribution vector and stay in L2, instead of being spread over it.
+// Also caches the merge-path tile partition of that CSR.
+static void pr_prepare(ursa_hip_topo* t, const DevCsr& in, hipStream_t s) {
     std::lock_guard<std::mutex> lk(t->cache_mu);
-    if (t->pr_tiles && t->pr_tile_items == PR_TILE) return;
-    const uint64_t len = t->n + t->m;
-    const uint64_t tiles = (len + PR_TILE - 1) / PR_TILE;
+    if (t->pr_ready) return;
+    const uint64_t n = t->n, m = t->m;
+    t->pr_old_of_new.alloc(n);
+    t->pr_new_of_old.alloc(n);
+    t->pr_off.alloc(n + 1);
+    t->pr_tgt.alloc(m ? m : 1);
+    t->pr_eid.alloc(m ? m : 1);
+    t->pr_outdeg.alloc(n);
+    {
+        DevBuf<uint32_t> keys(n), iota(n), skeys(n), rdeg(n);
+        k_rl_keys<<<blocks_for(n, 256), 256, 0, s>>>(t->out_deg.get(), n, keys.get(), iota.get());
+        check_launch();
+        prim::sort_pairs_u32(keys.get(), skeys.get(), iota.get(), t->pr_old_of_new.get(), n, 32, s);
+        k_rl_invert<<<blocks_for(n, 256), 256, 0, s>>>(t->pr_old_of_new.get(), n, in.off.get(), t->out_deg.get(),
+                                                       t->pr_new_of_old.get(), rdeg.get(), t->pr_outdeg.get());
+        check_launch();
+        prim::exclusive_sum_u32(rdeg.get(), t->pr_off.get(), n, s);
+    }
+    if (m) {
+        DevBuf<uint32_t> owner(m);
+        slot_owners(n, in, owner.get(), s);
+        k_rl_slots<<<blocks_for(m, 256), 256, 0, s>>>(owner.get(), in.off.get(), in.tgt.get(), in.eid.get(),
+                                                      t->pr_new_of_old.get(), t->pr_off.get(), m, t->pr_tgt.get(),
+                                                      t->pr_eid.get());
+        check_launch();
+    }
+    const uint64_t tiles = (n + m + PR_TILE - 1) / PR_TILE;
     t->pr_tile_x.alloc(tiles + 1);
     t->pr_tile_y.alloc(tiles + 1);
-    k_pr_partition<<<blocks_for(tiles + 1, 256), 256, 0, s>>>(in.off.get(), t->n, t->m, tiles, t->pr_tile_x.get(),
+    k_pr_partition<<<blocks_for(tiles + 1, 256), 256, 0, s>>>(t->pr_off.get(), n, m, tiles, t->pr_tile_x.get(),
                                                               t->pr_tile_y.get());
     check_launch();
     HIP_OK(hipStreamSynchronize(s));
     t->pr_tiles = tiles;
-    t->pr_tile_items = PR_TILE;
+    t->pr_ready = true;
 }
 
 // Solve PageRank. With `mixed`, sweeps first run with f32 contributions and
@@ -438,9 +507,14 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
         out.finish(s);
         return;
     }
-    const DevCsr& in = t->incoming(s);
-    pr_partition(t, in, s);
+    pr_prepare(t, t->incoming(s), s);
     const uint64_t tiles = t->pr_tiles;
+    // All solve-time arrays below are in the renumbered vertex order.
+    struct {
+        const uint32_t* off;
+        const uint32_t* tgt;
+        const uint32_t* eid;
+    } in{t->pr_off.get(), t->pr_tgt.get(), t->pr_eid.get()};
 
     const bool weighted = weights_user != nullptr;
     DevIn<double> w(weights_user, m, s);
@@ -448,15 +522,18 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
     DevBuf<float> w32;
     bool use_f32w = false;
     if (weighted) {
+        DevBuf<double> os_old(n);
         out_str.alloc(n);
-        k_out_strength<<<blocks_for(n, 256), 256, 0, s>>>(t->out.off.get(), t->out.eid.get(), w.p, n, out_str.get());
+        k_out_strength<<<blocks_for(n, 256), 256, 0, s>>>(t->out.off.get(), t->out.eid.get(), w.p, n, os_old.get());
+        k_gather_f64<<<blocks_for(n, 256), 256, 0, s>>>(os_old.get(), t->pr_old_of_new.get(), n, out_str.get());
         check_launch();
+        HIP_OK(hipStreamSynchronize(s));
         w64.alloc(m);
         w32.alloc(m);
         DevBuf<uint32_t> inexact(1);
         HIP_OK(hipMemsetAsync(inexact.get(), 0, sizeof(uint32_t), s));
         if (m) {
-            k_permute_weights<<<blocks_for(m, 256), 256, 0, s>>>(in.eid.get(), w.p, m, w64.get(), w32.get(),
+            k_permute_weights<<<blocks_for(m, 256), 256, 0, s>>>(in.eid, w.p, m, w64.get(), w32.get(),
                                                                  inexact.get());
             check_launch();
         }
@@ -480,7 +557,7 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
     DevBuf<PrCtl> ctl(1);
     const unsigned init_blocks = blocks_for(n, 256, 1u << 30);
     DevBuf<double> part_dm(init_blocks);
-    const uint32_t* od = t->out_deg.get();
+    const uint32_t* od = t->pr_outdeg.get();
     const double* os = weighted ? out_str.get() : nullptr;
     if (mixed) {
         if (weighted)
@@ -514,19 +591,19 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
             timed("pagerank_spmv_f32", f32_edge * (double)m + (node_bytes - 4.0) * (double)n, s, [&] {
                 if (weighted)
                     k_pr_tiles<float, float, true><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
-                        in.off.get(), in.tgt.get(), w32.get(), tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
+                        in.off, in.tgt, w32.get(), tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
                         tail.get(), td.get(), tm.get(), ctl.get(), p, damping);
                 else
                     k_pr_tiles<float, float, false><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
-                        in.off.get(), in.tgt.get(), nullptr, tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
+                        in.off, in.tgt, nullptr, tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
                         tail.get(), td.get(), tm.get(), ctl.get(), p, damping);
             });
             if (weighted)
-                k_pr_fixup<float, true><<<fix_blocks, 256, 0, s>>>(in.off.get(), tx, ty, tiles, n, od, os, rk, nx, cn,
+                k_pr_fixup<float, true><<<fix_blocks, 256, 0, s>>>(in.off, tx, ty, tiles, n, od, os, rk, nx, cn,
                                                                    carry.get(), tail.get(), fd.get(), fm.get(),
                                                                    ctl.get(), p, damping);
             else
-                k_pr_fixup<float, false><<<fix_blocks, 256, 0, s>>>(in.off.get(), tx, ty, tiles, n, od, os, rk, nx, cn,
+                k_pr_fixup<float, false><<<fix_blocks, 256, 0, s>>>(in.off, tx, ty, tiles, n, od, os, rk, nx, cn,
                                                                     carry.get(), tail.get(), fd.get(), fm.get(),
                                                                     ctl.get(), p, damping);
         } else {
@@ -535,23 +612,23 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
             timed("pagerank_spmv", f64_edge * (double)m + node_bytes * (double)n, s, [&] {
                 if (!weighted)
                     k_pr_tiles<double, double, false><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
-                        in.off.get(), in.tgt.get(), nullptr, tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
+                        in.off, in.tgt, nullptr, tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
                         tail.get(), td.get(), tm.get(), ctl.get(), p, damping);
                 else if (use_f32w)
                     k_pr_tiles<double, float, true><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
-                        in.off.get(), in.tgt.get(), w32.get(), tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
+                        in.off, in.tgt, w32.get(), tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
                         tail.get(), td.get(), tm.get(), ctl.get(), p, damping);
                 else
                     k_pr_tiles<double, double, true><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
-                        in.off.get(), in.tgt.get(), w64.get(), tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
+                        in.off, in.tgt, w64.get(), tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
                         tail.get(), td.get(), tm.get(), ctl.get(), p, damping);
             });
             if (weighted)
-                k_pr_fixup<double, true><<<fix_blocks, 256, 0, s>>>(in.off.get(), tx, ty, tiles, n, od, os, rk, nx,
+                k_pr_fixup<double, true><<<fix_blocks, 256, 0, s>>>(in.off, tx, ty, tiles, n, od, os, rk, nx,
                                                                     cn, carry.get(), tail.get(), fd.get(), fm.get(),
                                                                     ctl.get(), p, damping);
             else
-                k_pr_fixup<double, false><<<fix_blocks, 256, 0, s>>>(in.off.get(), tx, ty, tiles, n, od, os, rk, nx,
+                k_pr_fixup<double, false><<<fix_blocks, 256, 0, s>>>(in.off, tx, ty, tiles, n, od, os, rk, nx,
                                                                      cn, carry.get(), tail.get(), fd.get(), fm.get(),
                                                                      ctl.get(), p, damping);
         }
@@ -615,7 +692,8 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
     }
     h = read_ctl();
     const uint32_t iters = h.iters;
-    HIP_OK(hipMemcpyAsync(out.p, rank[iters & 1].get(), n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    k_gather_f64<<<blocks_for(n, 256), 256, 0, s>>>(rank[iters & 1].get(), t->pr_new_of_old.get(), n, out.p);
+    check_launch();
     out.finish(s);
     if (iters_out) *iters_out = iters;
     if (f32_iters_out) *f32_iters_out = f32_iters;

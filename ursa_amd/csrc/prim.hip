@@ -58,6 +58,15 @@ void exclusive_sum_u32_to_u64(const uint32_t* in, uint64_t* out, size_t n, hipSt
     }, s);
 }
 
+void exclusive_sum_u32(const uint32_t* in, uint32_t* out, size_t n, hipStream_t s) {
+    // Inclusive scan into out[1..n], out[0] = 0.
+    HIP_OK(hipMemsetAsync(out, 0, sizeof(uint32_t), s));
+    if (n == 0) return;
+    run_twice([&](void* t, size_t& b) {
+        return rocprim::inclusive_scan(t, b, in, out + 1, n, rocprim::plus<uint32_t>(), s);
+    }, s);
+}
+
 void unique_u64(const uint64_t* in, uint64_t* out, uint64_t* d_count, size_t n, hipStream_t s) {
     run_twice([&](void* t, size_t& b) {
         return rocprim::unique(t, b, in, out, d_count, n, rocprim::equal_to<uint64_t>(), s);

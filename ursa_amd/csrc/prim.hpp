@@ -22,6 +22,8 @@ void sort_keys_u64(const uint64_t* kin, uint64_t* kout, size_t n, int end_bit, h
 void inclusive_max_scan_u32(const uint32_t* in, uint32_t* out, size_t n, hipStream_t s);
 // out[i] = sum(in[0..i)) (u64 accumulation); returns nothing, total at out[n] if with_total.
 void exclusive_sum_u32_to_u64(const uint32_t* in, uint64_t* out, size_t n, hipStream_t s);
+// out[i] = sum(in[0..i)) in u32, for i in [0, n]; out has n + 1 entries.
+void exclusive_sum_u32(const uint32_t* in, uint32_t* out, size_t n, hipStream_t s);
 // Adjacent-duplicate removal; the count is written to *d_count (device).
 void unique_u64(const uint64_t* in, uint64_t* out, uint64_t* d_count, size_t n, hipStream_t s);
 // Stable compaction of u32 values whose flag is non-zero; count to *d_count.

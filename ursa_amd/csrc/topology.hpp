@@ -43,10 +43,12 @@ struct ursa_hip_topo {
     ursa::DevBuf<uint32_t> out_deg;
     std::mutex build_mu;  // guards the lazy in / und builds
     std::mutex cache_mu;  // guards algorithm-side caches below
-    // PageRank merge-path partition of the in-CSR (cached, see pagerank.hip).
+    // PageRank's private copy of the in-CSR with vertices renumbered by
+    // decreasing out-degree, and its merge-path partition (see pagerank.hip).
+    ursa::DevBuf<uint32_t> pr_old_of_new, pr_new_of_old, pr_off, pr_tgt, pr_eid, pr_outdeg;
     ursa::DevBuf<uint32_t> pr_tile_x, pr_tile_y;
     uint64_t pr_tiles = 0;
-    int pr_tile_items = 0;
+    bool pr_ready = false;
     // Degree-oriented undirected DAG for triangle counting (see triangle.hip).
     ursa::DevBuf<uint32_t> tc_off, tc_tgt, tc_work;
     uint64_t tc_slots = 0, tc_nwork = 0;
