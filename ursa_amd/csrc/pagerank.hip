@@ -150,6 +150,13 @@ __device__ __forceinline__ double node_scale(const uint32_t* out_deg, const doub
         return (double)__builtin_nontemporal_load(&out_deg[u]);
 }
 
+// LDS slot of tile item i: one pad word per 32 words, so the 8-item stride of
+// the per-thread walk falls into distinct banks.
+template <class CT>
+__device__ __forceinline__ uint32_t lds_slot(uint32_t i) {
+    return sizeof(CT) == 8 ? i + (i >> 4) : i + (i >> 5);
+}
+
 template <class CT, class W, bool WEIGHTED>
 __global__ __launch_bounds__(PR_BLOCK) void k_pr_tiles(
     const uint32_t* __restrict__ in_off, const uint32_t* __restrict__ in_tgt, const W* __restrict__ w_in,
@@ -158,7 +165,7 @@ __global__ __launch_bounds__(PR_BLOCK) void k_pr_tiles(
     const CT* __restrict__ contrib, double* __restrict__ next, CT* __restrict__ contrib_next,
     double* __restrict__ carry_val, double* __restrict__ tail_val, double* __restrict__ tile_delta,
     double* __restrict__ tile_dm, PrCtl* ctl, int parity, double damping) {
-    __shared__ CT s_val[PR_TILE];
+    __shared__ CT s_val[PR_TILE + PR_TILE / 16];
     __shared__ uint32_t s_end[PR_TILE + 1];
     __shared__ double s_wv[PR_BLOCK / 64];
     __shared__ int s_wf[PR_BLOCK / 64];
@@ -176,7 +183,7 @@ __global__ __launch_bounds__(PR_BLOCK) void k_pr_tiles(
         const uint64_t k = y0 + i;
         CT c = contrib[__builtin_nontemporal_load(&in_tgt[k])];
         if constexpr (WEIGHTED) c = (CT)__builtin_nontemporal_load(&w_in[k]) * c;
-        s_val[i] = c;
+        s_val[lds_slot<CT>((uint32_t)i)] = c;
     }
     if (x0 < n)
         for (uint64_t i = threadIdx.x; i < n_end; i += PR_BLOCK) s_end[i] = __builtin_nontemporal_load(&in_off[x0 + 1 + i]);
@@ -209,7 +216,7 @@ __global__ __launch_bounds__(PR_BLOCK) void k_pr_tiles(
         er[it] = 0;
         if (d0 + it < len) {
             if (x < n && y < (uint64_t)s_end[x - x0]) {
-                sum.add(s_val[y - y0]);
+                sum.add(s_val[lds_slot<CT>((uint32_t)(y - y0))]);
                 ++y;
             } else {
                 er[it] = (uint32_t)x;
@@ -275,45 +282,54 @@ __global__ __launch_bounds__(PR_BLOCK) void k_pr_tiles(
     }
 }
 
+// Finishes rows that began in an earlier tile, then folds the per-tile delta
+// and dangling partials of 256 consecutive tiles (in tile order) into one
+// partial per workgroup, so the final reduce reads few values.
 template <class CT, bool WEIGHTED>
-__global__ void k_pr_fixup(const uint32_t* __restrict__ in_off, const uint32_t* __restrict__ tile_x,
-                           const uint32_t* __restrict__ tile_y, uint64_t tiles, uint64_t n,
-                           const uint32_t* __restrict__ out_deg, const double* __restrict__ out_str,
-                           const double* __restrict__ rank, double* __restrict__ next,
-                           CT* __restrict__ contrib_next, const double* __restrict__ carry_val,
-                           const double* __restrict__ tail_val, double* __restrict__ fix_delta,
-                           double* __restrict__ fix_dm, const PrCtl* ctl, int parity, double damping) {
+__global__ __launch_bounds__(256) void k_pr_fixup(
+    const uint32_t* __restrict__ in_off, const uint32_t* __restrict__ tile_x, const uint32_t* __restrict__ tile_y,
+    uint64_t tiles, uint64_t n, const uint32_t* __restrict__ out_deg, const double* __restrict__ out_str,
+    const double* __restrict__ rank, double* __restrict__ next, CT* __restrict__ contrib_next,
+    const double* __restrict__ carry_val, const double* __restrict__ tail_val, const double* __restrict__ tile_delta,
+    const double* __restrict__ tile_dm, double* __restrict__ blk_delta, double* __restrict__ blk_dm, const PrCtl* ctl,
+    int parity, double damping) {
+    __shared__ double s_red[256 / 64];
     if (ctl->done) return;
     const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (t >= tiles) return;
     double delta = 0.0, dm = 0.0;
-    const uint64_t x0 = tile_x[t], y0 = tile_y[t], x1 = tile_x[t + 1];
-    if (x0 < n && x1 > x0 && (uint64_t)in_off[x0] < y0) {
-        // Tiles j..t-1 all end inside row x0; tile t holds its end.
-        uint64_t j = t - 1;
-        while (j > 0 && tile_x[j] == x0) --j;
-        double acc = 0.0;
-        for (uint64_t q = j; q < t; ++q) acc += carry_val[q];
-        acc += tail_val[t];
-        pr_finish_row(x0, acc, ctl->base[parity], damping, node_scale<WEIGHTED>(out_deg, out_str, x0), rank,
-                      next, contrib_next, delta, dm);
+    if (t < tiles) {
+        const uint64_t x0 = tile_x[t], y0 = tile_y[t], x1 = tile_x[t + 1];
+        if (x0 < n && x1 > x0 && (uint64_t)in_off[x0] < y0) {
+            // Tiles j..t-1 all end inside row x0; tile t holds its end.
+            uint64_t j = t - 1;
+            while (j > 0 && tile_x[j] == x0) --j;
+            double acc = 0.0;
+            for (uint64_t q = j; q < t; ++q) acc += carry_val[q];
+            acc += tail_val[t];
+            pr_finish_row(x0, acc, ctl->base[parity], damping, node_scale<WEIGHTED>(out_deg, out_str, x0), rank,
+                          next, contrib_next, delta, dm);
+        }
+        delta = tile_delta[t] + delta;
+        dm = tile_dm[t] + dm;
     }
-    fix_delta[t] = delta;
-    fix_dm[t] = dm;
+    const double bd = block_sum(delta, s_red);
+    const double bm = block_sum(dm, s_red);
+    if (threadIdx.x == 0) {
+        blk_delta[blockIdx.x] = bd;
+        blk_dm[blockIdx.x] = bm;
+    }
 }
 
-__global__ __launch_bounds__(1024) void k_pr_reduce(const double* __restrict__ a_delta,
-                                                    const double* __restrict__ b_delta,
-                                                    const double* __restrict__ a_dm,
-                                                    const double* __restrict__ b_dm, uint64_t count, PrCtl* ctl,
+__global__ __launch_bounds__(1024) void k_pr_reduce(const double* __restrict__ blk_delta,
+                                                    const double* __restrict__ blk_dm, uint64_t count, PrCtl* ctl,
                                                     int parity, uint32_t iter, uint32_t max_iter, double tol,
                                                     double damping, double nf) {
     __shared__ double s_red[1024 / 64];
     if (ctl->done) return;
     double d = 0.0, m = 0.0;
     for (uint64_t i = threadIdx.x; i < count; i += blockDim.x) {
-        d += a_delta[i] + b_delta[i];
-        m += a_dm[i] + b_dm[i];
+        d += blk_delta[i];
+        m += blk_dm[i];
     }
     const double delta = block_sum(d, s_red);
     const double dm = block_sum(m, s_red);
@@ -600,11 +616,11 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
             });
             if (weighted)
                 k_pr_fixup<float, true><<<fix_blocks, 256, 0, s>>>(in.off, tx, ty, tiles, n, od, os, rk, nx, cn,
-                                                                   carry.get(), tail.get(), fd.get(), fm.get(),
+                                                                   carry.get(), tail.get(), td.get(), tm.get(), fd.get(), fm.get(),
                                                                    ctl.get(), p, damping);
             else
                 k_pr_fixup<float, false><<<fix_blocks, 256, 0, s>>>(in.off, tx, ty, tiles, n, od, os, rk, nx, cn,
-                                                                    carry.get(), tail.get(), fd.get(), fm.get(),
+                                                                    carry.get(), tail.get(), td.get(), tm.get(), fd.get(), fm.get(),
                                                                     ctl.get(), p, damping);
         } else {
             const double* cb = contrib[p].get();
@@ -625,17 +641,17 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
             });
             if (weighted)
                 k_pr_fixup<double, true><<<fix_blocks, 256, 0, s>>>(in.off, tx, ty, tiles, n, od, os, rk, nx,
-                                                                    cn, carry.get(), tail.get(), fd.get(), fm.get(),
+                                                                    cn, carry.get(), tail.get(), td.get(), tm.get(), fd.get(), fm.get(),
                                                                     ctl.get(), p, damping);
             else
                 k_pr_fixup<double, false><<<fix_blocks, 256, 0, s>>>(in.off, tx, ty, tiles, n, od, os, rk, nx,
-                                                                     cn, carry.get(), tail.get(), fd.get(), fm.get(),
+                                                                     cn, carry.get(), tail.get(), td.get(), tm.get(), fd.get(), fm.get(),
                                                                      ctl.get(), p, damping);
         }
         check_launch();
     };
     auto reduce = [&](uint32_t it, double tol_it) {
-        k_pr_reduce<<<1, 1024, 0, s>>>(td.get(), fd.get(), tm.get(), fm.get(), tiles, ctl.get(), (int)(it & 1), it,
+        k_pr_reduce<<<1, 1024, 0, s>>>(fd.get(), fm.get(), fix_blocks, ctl.get(), (int)(it & 1), it,
                                        max_iter, tol_it, damping, nf);
         check_launch();
     };
