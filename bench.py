@@ -312,7 +312,9 @@ def bench_pagerank(args, ua, gen):
             bpl = qq["algo_bytes"] / qq["launches"]
             gbs = bpl / (avg / 1e3) / 1e9
             res[key] = {"avg_launch_us": round(avg * 1e3, 2), "algo_bytes_per_launch": bpl, "GBps": round(gbs, 1),
-                        "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+                        "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                        "traffic": traffic_from_profiles(f"pagerank_rmat{scale}",
+                                                         "k_pr_tiles" if key.endswith("f64") else "k_pr_tiles_f32")}
     res["mixed_speedup"] = round(res["f64"]["ms"] / res["mixed"]["ms"], 3)
     # accuracy against the f64 oracle
     if not args.no_cpu_baseline:
@@ -346,6 +348,60 @@ def bench_pagerank(args, ua, gen):
     return res
 
 
+# ------------------------------------------------------------------ batched (C4)
+def bench_batched(args, ua, gen):
+    """R-MAT scale 20, seed 4, symmetrised: Brandes from k=256 seeded sources and
+    BFS from 4096 seeded sources, batched 64 per pass vs one traversal per launch
+    sequence, on the same GPU with identical results."""
+    lib = ua.load()
+    scale = args.batch_scale
+    e = gen.rmat(scale, 16, 0.57, 0.19, 0.19, seed=4, symmetrize=True)
+    topo = e.topology()
+    check(lib.ursa_hip_topology_incoming(topo._h))
+    n, m = e.n, e.m
+    res = {"graph": {"scale": scale, "seed": 4, "symmetrised": True, "n": n, "m": m}}
+    # Brandes, k = 256 sources = shuffled_order(n, 4)[:256], scaled by n/256.
+    src = np.ascontiguousarray(ua.shuffled_order(n, 4)[:256])
+    out_b = np.zeros(n, np.float64)
+    out_1 = np.zeros(n, np.float64)
+    re = C.c_uint64(0)
+    for name, batch, out in (("batched64", 64, out_b), ("one_per_launch", 1, out_1)):
+        check(lib.ursa_hip_betweenness_sources_ex(topo._h, src.ctypes.data_as(C.c_void_p), 256, 1, n / 256.0,
+                                                  batch, out.ctypes.data_as(C.c_void_p), C.byref(re)))
+        t0 = time.perf_counter()
+        check(lib.ursa_hip_betweenness_sources_ex(topo._h, src.ctypes.data_as(C.c_void_p), 256, 1, n / 256.0,
+                                                  batch, out.ctypes.data_as(C.c_void_p), C.byref(re)))
+        el = time.perf_counter() - t0
+        res[f"brandes_{name}"] = {"s": round(el, 4), "sources_per_s": round(256 / el, 1),
+                                  "GTEPS": round(re.value / el / 1e9, 3)}
+        log(f"brandes k=256 {name}: {el * 1e3:.1f} ms")
+    rel = np.abs(out_b - out_1) / np.maximum(np.abs(out_1), 1e-300)
+    res["brandes_speedup"] = round(res["brandes_one_per_launch"]["s"] / res["brandes_batched64"]["s"], 2)
+    res["brandes_max_rel_diff_batched_vs_single"] = float(rel[out_1 != 0].max()) if np.any(out_1 != 0) else 0.0
+    # Multi-source BFS from 4096 sources = shuffled_order(n, 4 ^ 1)[:4096].
+    nsrc = args.msbfs_sources
+    bsrc = np.ascontiguousarray(ua.shuffled_order(n, 4 ^ 1)[:nsrc])
+    lv = Dev(lib, 4 * nsrc * n)
+    for name, mode in (("batched64", 2), ("one_per_launch", 1)):
+        check(lib.ursa_hip_bfs_ex(topo._h, bsrc.ctypes.data_as(C.c_void_p), 64, 0, lv.p, C.byref(re), mode))
+        t0 = time.perf_counter()
+        check(lib.ursa_hip_bfs_ex(topo._h, bsrc.ctypes.data_as(C.c_void_p), nsrc, 0, lv.p, C.byref(re), mode))
+        check(lib.ursa_hip_synchronize())
+        el = time.perf_counter() - t0
+        res[f"msbfs_{name}"] = {"s": round(el, 4), "sources_per_s": round(nsrc / el, 1),
+                                "GTEPS": round(re.value / el / 1e9, 3)}
+        log(f"bfs x{nsrc} {name}: {el * 1e3:.1f} ms, {re.value / el / 1e9:.1f} GTEPS")
+        if name == "batched64":
+            first = np.zeros((64, n), np.int32)
+            check(lib.ursa_hip_memcpy_to_host(first.ctypes.data_as(C.c_void_p), lv.p, first.nbytes))
+    single = np.zeros((64, n), np.int32)
+    check(lib.ursa_hip_memcpy_to_host(single.ctypes.data_as(C.c_void_p), lv.p, single.nbytes))
+    res["msbfs_first64_identical"] = bool(np.array_equal(first, single))
+    res["msbfs_speedup"] = round(res["msbfs_one_per_launch"]["s"] / res["msbfs_batched64"]["s"], 2)
+    lv.free()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -357,7 +413,10 @@ def main():
     ap.add_argument("--no-pagerank", action="store_true")
     ap.add_argument("--cpu-sources", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--only", choices=["c2", "pagerank"], default=None,
+    ap.add_argument("--batch-scale", type=int, default=20)
+    ap.add_argument("--msbfs-sources", type=int, default=4096)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--only", choices=["c2", "pagerank", "batched"], default=None,
                     help="run one section only (for profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run the calibration copy kernel over this many MiB")
@@ -383,10 +442,15 @@ def main():
     if args.only == "pagerank":
         print(json.dumps({"pagerank": bench_pagerank(args, ua, gen)}), flush=True)
         return
+    if args.only == "batched":
+        print(json.dumps({"batched": bench_batched(args, ua, gen)}), flush=True)
+        return
     res = bench_c2(args, ua, gen)
     if args.only == "c2":
         args.no_pagerank = True
+        args.no_batched = True
     pr = None if args.no_pagerank else bench_pagerank(args, ua, gen)
+    batched = None if args.no_batched else bench_batched(args, ua, gen)
     line = {
         "metric": res["metric"],
         "value": res["value"],
@@ -405,6 +469,7 @@ def main():
         "cpu_baseline": res["cpu_baseline"],
         "kernels": res["kernels"],
         "pagerank": pr,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
 
