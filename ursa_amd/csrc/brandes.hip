@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "frontier.hpp"
+#include "prim.hpp"
 #include "rng.hpp"
 
 namespace ursa {
@@ -190,6 +191,329 @@ __global__ void k_scale(double* p, uint64_t n, double s) {
         p[i] = p[i] * s;
 }
 
+
+// ---- vertex-major batches: the 64 sources of a batch are the 64 lanes of a
+// wavefront. State for (vertex v, source b) lives at v * 64 + b, so one edge
+// scan of v serves every source whose frontier holds v, and the per-source
+// reads and atomics for a neighbour w are one contiguous 256 / 512 B access.
+constexpr uint32_t VM_HEAVY = 2048;  // vertices with more out-edges are split into chunks of this size
+
+struct VmCtr {
+    uint32_t len;
+    uint32_t pad;
+    unsigned long long edges;
+};
+
+__global__ void k_vm_init(int32_t* dist, double* sigma, double* delta, uint32_t* mark, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n * 64;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        dist[i] = -1;
+        sigma[i] = 0.0;
+        delta[i] = 0.0;
+        if (i < n) mark[i] = 0;
+    }
+}
+
+__global__ void k_vm_seed(const uint32_t* src, uint32_t nb, uint64_t n, int32_t* dist, double* sigma, uint32_t* mark,
+                          uint32_t* order, VmCtr* ctr) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    uint32_t len = 0;
+    for (uint32_t b = 0; b < nb; ++b) {
+        const uint32_t s = src[b];
+        if (s >= n) continue;
+        dist[(uint64_t)s * 64 + b] = 0;
+        sigma[(uint64_t)s * 64 + b] = 1.0;
+        if (mark[s] == 0) {
+            mark[s] = 1;  // queued at level 0 (mark = level + 1)
+            order[len++] = s;
+        }
+    }
+    ctr->len = len;
+}
+
+// Expand v's out-edges [e0, e1) for the lanes whose level at v is L-1.
+__device__ __forceinline__ void vm_expand(uint32_t v, uint32_t e0, uint32_t e1, const uint32_t* __restrict__ tgt,
+                                          int32_t* __restrict__ dist, double* __restrict__ sigma, int32_t L,
+                                          uint32_t* __restrict__ mark, uint32_t* __restrict__ nq, VmCtr* ctr,
+                                          bool count_edges, uint32_t deg, unsigned long long& edges) {
+    const int b = lane_id();
+    const bool fl = dist[(uint64_t)v * 64 + b] == L - 1;
+    const uint64_t fmask = __ballot(fl);
+    if (!fmask) return;
+    if (count_edges && b == 0) edges += (unsigned long long)deg * __popcll(fmask);
+    const double sv = fl ? sigma[(uint64_t)v * 64 + b] : 0.0;
+    for (uint32_t e = e0; e < e1; ++e) {
+        const uint32_t w = tgt[e];
+        bool claimed = false;
+        if (fl) {
+            int32_t* dp = &dist[(uint64_t)w * 64 + b];
+            int32_t dw = *dp;
+            if (dw < 0) {
+                const int32_t old = atomicCAS(dp, -1, L);
+                claimed = old == -1;
+                dw = claimed ? L : old;
+            }
+            if (dw == L) unsafeAtomicAdd(&sigma[(uint64_t)w * 64 + b], sv);
+        }
+        const uint64_t cm = __ballot(claimed);
+        if (cm && b == __ffsll((unsigned long long)cm) - 1) {
+            if (atomicMax(&mark[w], (uint32_t)(L + 1)) < (uint32_t)(L + 1)) nq[atomicAdd(&ctr->len, 1u)] = w;
+        }
+    }
+}
+
+__global__ __launch_bounds__(BB) void k_vm_forward(const uint32_t* __restrict__ q, uint32_t qlen,
+                                                   const uint32_t* __restrict__ off, const uint32_t* __restrict__ tgt,
+                                                   int32_t* __restrict__ dist, double* __restrict__ sigma, int32_t L,
+                                                   uint32_t* __restrict__ mark, uint32_t* __restrict__ nq, VmCtr* ctr) {
+    const uint64_t gw = blockIdx.x * (uint64_t)BW + (threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * BW;
+    unsigned long long edges = 0;
+    for (uint64_t i = gw; i < qlen; i += nwaves) {
+        const uint32_t v = q[i];
+        const uint32_t a = off[v], z = off[v + 1];
+        if (z - a > VM_HEAVY) continue;  // chunked by k_vm_forward_heavy
+        vm_expand(v, a, z, tgt, dist, sigma, L, mark, nq, ctr, true, z - a, edges);
+    }
+    if (lane_id() == 0 && edges) atomicAdd(&ctr->edges, edges);
+}
+
+// Heavy vertices, one wave per chunk of VM_HEAVY edges (chunk c of heavy
+// vertex h lies in [cbase[h], cbase[h+1])).
+__device__ __forceinline__ uint32_t vm_chunk_owner(const uint32_t* cbase, uint32_t nh, uint32_t c) {
+    uint32_t lo = 0, hi = nh;  // largest h with cbase[h] <= c
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cbase[mid] <= c)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(BB) void k_vm_forward_heavy(const uint32_t* __restrict__ heavy,
+                                                         const uint32_t* __restrict__ cbase, uint32_t nh,
+                                                         uint32_t chunks, const uint32_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ tgt, int32_t* __restrict__ dist,
+                                                         double* __restrict__ sigma, int32_t L,
+                                                         uint32_t* __restrict__ mark, uint32_t* __restrict__ nq,
+                                                         VmCtr* ctr) {
+    const uint64_t gw = blockIdx.x * (uint64_t)BW + (threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * BW;
+    unsigned long long edges = 0;
+    for (uint64_t c = gw; c < chunks; c += nwaves) {
+        const uint32_t h = vm_chunk_owner(cbase, nh, (uint32_t)c);
+        const uint32_t v = heavy[h];
+        const uint32_t a = off[v], z = off[v + 1];
+        const uint32_t e0 = a + ((uint32_t)c - cbase[h]) * VM_HEAVY;
+        const uint32_t e1 = min(z, e0 + VM_HEAVY);
+        vm_expand(v, e0, e1, tgt, dist, sigma, L, mark, nq, ctr, c == cbase[h], z - a, edges);
+    }
+    if (lane_id() == 0 && edges) atomicAdd(&ctr->edges, edges);
+}
+
+// delta for lanes at level L: sum over out-slots w at level L+1 of
+// sigma[v] * ((1 + delta[w]) / sigma[w]) in slot order.
+__device__ __forceinline__ double vm_pull(uint32_t v, uint32_t e0, uint32_t e1, const uint32_t* __restrict__ tgt,
+                                          const int32_t* __restrict__ dist, const double* __restrict__ sigma,
+                                          const double* __restrict__ delta, int32_t L, bool fl) {
+    const int b = lane_id();
+    double acc = 0.0;
+    if (!fl) return acc;
+    const double sv = sigma[(uint64_t)v * 64 + b];
+    for (uint32_t e = e0; e < e1; ++e) {
+        const uint64_t wi = (uint64_t)tgt[e] * 64 + b;
+        if (dist[wi] == L + 1) acc += sv * ((1.0 + delta[wi]) / sigma[wi]);
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(BB) void k_vm_backward(const uint32_t* __restrict__ q, uint32_t qlen,
+                                                    const uint32_t* __restrict__ off, const uint32_t* __restrict__ tgt,
+                                                    const int32_t* __restrict__ dist,
+                                                    const double* __restrict__ sigma, double* __restrict__ delta,
+                                                    int32_t L) {
+    const uint64_t gw = blockIdx.x * (uint64_t)BW + (threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * BW;
+    for (uint64_t i = gw; i < qlen; i += nwaves) {
+        const uint32_t v = q[i];
+        const uint32_t a = off[v], z = off[v + 1];
+        if (z - a > VM_HEAVY) continue;
+        const bool fl = dist[(uint64_t)v * 64 + lane_id()] == L;
+        if (!__ballot(fl)) continue;
+        const double acc = vm_pull(v, a, z, tgt, dist, sigma, delta, L, fl);
+        if (fl) delta[(uint64_t)v * 64 + lane_id()] = acc;
+    }
+}
+
+__global__ __launch_bounds__(BB) void k_vm_backward_heavy(const uint32_t* __restrict__ heavy,
+                                                          const uint32_t* __restrict__ cbase, uint32_t nh,
+                                                          uint32_t chunks, const uint32_t* __restrict__ off,
+                                                          const uint32_t* __restrict__ tgt,
+                                                          const int32_t* __restrict__ dist,
+                                                          const double* __restrict__ sigma,
+                                                          const double* __restrict__ delta, int32_t L,
+                                                          double* __restrict__ part) {
+    const uint64_t gw = blockIdx.x * (uint64_t)BW + (threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * BW;
+    for (uint64_t c = gw; c < chunks; c += nwaves) {
+        const uint32_t h = vm_chunk_owner(cbase, nh, (uint32_t)c);
+        const uint32_t v = heavy[h];
+        const uint32_t a = off[v], z = off[v + 1];
+        const uint32_t e0 = a + ((uint32_t)c - cbase[h]) * VM_HEAVY;
+        const uint32_t e1 = min(z, e0 + VM_HEAVY);
+        const bool fl = dist[(uint64_t)v * 64 + lane_id()] == L;
+        part[c * 64 + lane_id()] = __ballot(fl) ? vm_pull(v, e0, e1, tgt, dist, sigma, delta, L, fl) : 0.0;
+    }
+}
+
+// delta of heavy vertices at level L: chunk partials summed in chunk order.
+__global__ void k_vm_backward_combine(const uint32_t* __restrict__ heavy, const uint32_t* __restrict__ cbase,
+                                      uint32_t nh, const int32_t* __restrict__ dist, const double* __restrict__ part,
+                                      double* __restrict__ delta, int32_t L) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)nh * 64) return;
+    const uint32_t h = (uint32_t)(i >> 6), b = (uint32_t)(i & 63);
+    const uint64_t vi = (uint64_t)heavy[h] * 64 + b;
+    if (dist[vi] != L) return;
+    double acc = 0.0;
+    for (uint32_t c = cbase[h]; c < cbase[h + 1]; ++c) acc += part[(uint64_t)c * 64 + b];
+    delta[vi] = acc;
+}
+
+// chunk partial over the batch's sources in source order, then bc += partial.
+__global__ void k_vm_accumulate(const int32_t* __restrict__ dist, const double* __restrict__ delta, uint32_t nb,
+                                uint64_t n, double* __restrict__ bc) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (uint32_t b = 0; b < nb; ++b)
+            if (dist[v * 64 + b] > 0) acc += delta[v * 64 + b];
+        bc[v] += acc;
+    }
+}
+
+__global__ void k_vm_heavy_flags(const uint32_t* __restrict__ off, uint64_t n, uint8_t* flag, uint32_t* iota,
+                                 uint32_t* chunks) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t d = off[v + 1] - off[v];
+        flag[v] = d > VM_HEAVY;
+        iota[v] = (uint32_t)v;
+        chunks[v] = (d + VM_HEAVY - 1) / VM_HEAVY;
+    }
+}
+
+__global__ void k_vm_gather_chunks(const uint32_t* __restrict__ heavy, uint32_t nh, const uint32_t* __restrict__ chunks,
+                                   uint32_t* __restrict__ hchunks) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < nh) hchunks[i] = chunks[heavy[i]];
+}
+
+}  // namespace
+
+// Vertex-major Brandes over the sources, 64 per batch (one reference chunk
+// per batch); bc accumulated into out (device).
+static void betweenness_vm(ursa_hip_topo* t, const std::vector<uint32_t>& src, double* out, hipStream_t s,
+                           uint64_t* reached_edges) {
+    const uint64_t n = t->n;
+    const DevCsr& o = t->out;
+    // Heavy vertices and their chunk bases (static for the topology).
+    DevBuf<uint8_t> flag(n);
+    DevBuf<uint32_t> iota(n), chunks(n), heavy(n), dcnt_buf(2);
+    DevBuf<uint64_t> dcnt(1);
+    k_vm_heavy_flags<<<blocks_for(n, 256, 1u << 16), 256, 0, s>>>(o.off.get(), n, flag.get(), iota.get(), chunks.get());
+    check_launch();
+    prim::select_flagged_u32(iota.get(), flag.get(), heavy.get(), dcnt.get(), n, s);
+    uint64_t nh64 = 0;
+    HIP_OK(hipMemcpyAsync(&nh64, dcnt.get(), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    const uint32_t nh = (uint32_t)nh64;
+    DevBuf<uint32_t> hch(nh ? nh : 1), cbase(nh + 1);
+    uint32_t nchunks = 0;
+    if (nh) {
+        k_vm_gather_chunks<<<blocks_for(nh, 256), 256, 0, s>>>(heavy.get(), nh, chunks.get(), hch.get());
+        check_launch();
+        prim::exclusive_sum_u32(hch.get(), cbase.get(), nh, s);
+        HIP_OK(hipMemcpyAsync(&nchunks, cbase.get() + nh, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+    }
+    flag.reset();
+    iota.reset();
+    chunks.reset();
+    DevBuf<int32_t> dist(n * 64);
+    DevBuf<double> sigma(n * 64), delta(n * 64), part((uint64_t)std::max<uint32_t>(nchunks, 1) * 64);
+    DevBuf<uint32_t> mark(n), order(n * 64), dsrc(64);
+    DevBuf<VmCtr> ctr(1);
+    VmCtr* h = nullptr;
+    HIP_OK(hipHostMalloc((void**)&h, sizeof(VmCtr), hipHostMallocDefault));
+    auto read = [&]() {
+        HIP_OK(hipMemcpyAsync(h, ctr.get(), sizeof(VmCtr), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        return *h;
+    };
+    uint64_t total_edges = 0;
+    const unsigned hgrid = blocks_for(std::max<uint32_t>(nchunks, 1), BW, 1u << 16);
+    try {
+        for (uint64_t b0 = 0; b0 < src.size(); b0 += 64) {
+            const uint32_t nb = (uint32_t)std::min<uint64_t>(64, src.size() - b0);
+            k_vm_init<<<blocks_for(n * 64, 256, 1u << 16), 256, 0, s>>>(dist.get(), sigma.get(), delta.get(),
+                                                                       mark.get(), n);
+            HIP_OK(hipMemcpyAsync(dsrc.get(), src.data() + b0, nb * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+            HIP_OK(hipMemsetAsync(ctr.get(), 0, sizeof(VmCtr), s));
+            k_vm_seed<<<1, 1, 0, s>>>(dsrc.get(), nb, n, dist.get(), sigma.get(), mark.get(), order.get(), ctr.get());
+            check_launch();
+            std::vector<uint64_t> lvl{0, read().len};
+            for (int32_t L = 1;; ++L) {
+                const uint64_t a = lvl[L - 1], z = lvl[L];
+                if (z == a) break;
+                // Each vertex enters the list of a level at most once and reaches at
+                // most 64 distinct levels over the batch, so order[] (64 n) cannot overflow.
+                HIP_OK(hipMemsetAsync(ctr.get(), 0, sizeof(VmCtr), s));
+                const uint32_t qlen = (uint32_t)(z - a);
+                timed("brandes_forward", 0.0, s, [&] {
+                    k_vm_forward<<<blocks_for(qlen, BW, 1u << 16), BB, 0, s>>>(order.get() + a, qlen, o.off.get(),
+                                                                             o.tgt.get(), dist.get(), sigma.get(), L,
+                                                                             mark.get(), order.get() + z, ctr.get());
+                    if (nchunks)
+                        k_vm_forward_heavy<<<hgrid, BB, 0, s>>>(heavy.get(), cbase.get(), nh, nchunks, o.off.get(),
+                                                                o.tgt.get(), dist.get(), sigma.get(), L, mark.get(),
+                                                                order.get() + z, ctr.get());
+                });
+                check_launch();
+                const VmCtr c = read();
+                total_edges += c.edges;
+                lvl.push_back(z + c.len);
+            }
+            const int32_t levels = (int32_t)lvl.size() - 1;
+            for (int32_t L = levels - 2; L >= 1; --L) {
+                const uint64_t a = lvl[L], z = lvl[L + 1];
+                const uint32_t qlen = (uint32_t)(z - a);
+                timed("brandes_backward", 0.0, s, [&] {
+                    if (qlen)
+                        k_vm_backward<<<blocks_for(qlen, BW, 1u << 16), BB, 0, s>>>(
+                            order.get() + a, qlen, o.off.get(), o.tgt.get(), dist.get(), sigma.get(), delta.get(), L);
+                    if (nchunks) {
+                        k_vm_backward_heavy<<<hgrid, BB, 0, s>>>(heavy.get(), cbase.get(), nh, nchunks, o.off.get(),
+                                                                 o.tgt.get(), dist.get(), sigma.get(), delta.get(), L,
+                                                                 part.get());
+                        k_vm_backward_combine<<<blocks_for((uint64_t)nh * 64, 256), 256, 0, s>>>(
+                            heavy.get(), cbase.get(), nh, dist.get(), part.get(), delta.get(), L);
+                    }
+                });
+                check_launch();
+            }
+            k_vm_accumulate<<<blocks_for(n, 256, 1u << 16), 256, 0, s>>>(dist.get(), delta.get(), nb, n, out);
+            check_launch();
+        }
+    } catch (...) {
+        (void)hipHostFree(h);
+        throw;
+    }
+    (void)hipHostFree(h);
+    if (reached_edges) *reached_edges += total_edges;
+}
+
+namespace {
 }  // namespace
 
 // bc over an explicit source list; `batch` sources advance together (1..64).
@@ -209,6 +533,18 @@ void betweenness_sources(ursa_hip_topo* t, const uint32_t* sources_user, uint64_
             std::copy(sources_user, sources_user + k, src.begin());
     }
     batch = std::max<uint32_t>(1, std::min<uint32_t>(batch, 64));
+    if (batch == 64 && k > 0) {
+        // Batches of 64 = the reference's chunks: vertex-major lanes.
+        uint64_t re = 0;
+        betweenness_vm(t, src, out.p, s, &re);
+        if (apply_scale) {
+            k_scale<<<blocks_for(n, 256, 1u << 16), 256, 0, s>>>(out.p, n, scale);
+            check_launch();
+        }
+        out.finish(s);
+        if (reached_edges) *reached_edges = re;
+        return;
+    }
     const uint32_t B = (uint32_t)std::min<uint64_t>(batch, std::max<uint64_t>(k, 1));
     const DevCsr& o = t->out;
     DevBuf<int32_t> dist(B * n);

@@ -32,9 +32,12 @@ constexpr int MW = MB / 64;
 
 struct MsCtr {
     uint32_t qlen;
-    uint32_t pad;
+    uint32_t heavy;              // push-frontier vertices deferred to the grid-wide pass
     unsigned long long newbits;  // (source, vertex) pairs settled this level
+    unsigned long long edges;    // forward degree sum of the next frontier (push levels)
 };
+
+constexpr uint32_t MS_HEAVY = 1024;
 
 __device__ __forceinline__ void write_levels(uint64_t bits, int32_t* levels, uint64_t n, uint64_t v, int32_t L) {
     while (bits) {
@@ -44,25 +47,46 @@ __device__ __forceinline__ void write_levels(uint64_t bits, int32_t* levels, uin
     }
 }
 
+
+// Push one frontier edge u -> v carrying lane mask fm; returns whether this
+// edge gave v its first mask of the level.
+__device__ __forceinline__ bool ms_push_edge(uint32_t v, unsigned long long fm, unsigned long long* visit,
+                                             unsigned long long* next, int32_t* levels, uint64_t n, int32_t L,
+                                             unsigned long long& cnt) {
+    const unsigned long long bits = fm & ~visit[v];
+    if (!bits) return false;
+    const unsigned long long old = atomicOr(&visit[v], bits);
+    const unsigned long long mine = bits & ~old;
+    if (!mine) return false;
+    write_levels(mine, levels, n, v, L);
+    cnt += __popcll(mine);
+    return atomicOr(&next[v], mine) == 0;
+}
+
 __global__ void k_ms_fill_levels(int32_t* levels, uint64_t total) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x)
         levels[i] = -1;
 }
 
 // Seed: visit/front bits of each source, level 0, unique source queue.
-__global__ void k_ms_seed(const uint32_t* src, uint32_t nb, uint64_t n, unsigned long long* visit,
+__global__ void k_ms_seed(const uint32_t* src, uint32_t nb, uint64_t n, NbrView fwd, unsigned long long* visit,
                           unsigned long long* front, int32_t* levels, uint32_t* queue, MsCtr* ctr) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     uint32_t q = 0;
+    unsigned long long edges = 0;
     for (uint32_t b = 0; b < nb; ++b) {
         const uint32_t s = src[b];
         if (s >= n) continue;
-        if (front[s] == 0) queue[q++] = s;
+        if (front[s] == 0) {
+            queue[q++] = s;
+            edges += nv_degree(fwd, s);
+        }
         front[s] |= 1ull << b;
         visit[s] |= 1ull << b;
         levels[(uint64_t)b * n + s] = 0;
     }
     ctr->qlen = q;
+    ctr->edges = edges;
 }
 
 __global__ __launch_bounds__(MB) void k_ms_pull(uint64_t n, NbrView rev, const unsigned long long* __restrict__ front,
@@ -102,11 +126,12 @@ __global__ __launch_bounds__(MB) void k_ms_push(const uint32_t* __restrict__ q, 
                                                 const unsigned long long* __restrict__ front,
                                                 unsigned long long* __restrict__ next,
                                                 unsigned long long* __restrict__ visit, int32_t* __restrict__ levels,
-                                                uint64_t n, int32_t L, uint32_t* __restrict__ nq, MsCtr* ctr) {
+                                                uint64_t n, int32_t L, uint32_t* __restrict__ nq, MsCtr* ctr,
+                                                uint32_t* __restrict__ heavy) {
     const int lane = lane_id();
     const uint64_t gw = blockIdx.x * (uint64_t)MW + (threadIdx.x >> 6);
     const uint64_t nwaves = (uint64_t)gridDim.x * MW;
-    unsigned long long cnt = 0;
+    unsigned long long cnt = 0, edges = 0;
     for (uint64_t base = gw * 64; base < qlen; base += nwaves * 64) {
         const uint64_t idx = base + lane;
         uint32_t u = 0, d = 0, da = 0, sa = 0, sb = 0;
@@ -117,6 +142,10 @@ __global__ __launch_bounds__(MB) void k_ms_push(const uint32_t* __restrict__ q, 
             sa = fwd.off_a[u];
             if (fwd.off_b) sb = fwd.off_b[u];
             fm = front[u];
+            if (d > MS_HEAVY) {
+                heavy[atomicAdd(&ctr->heavy, 1u)] = u;
+                d = 0;
+            }
         }
         const uint32_t incl = wave_incl_scan(d);
         const uint32_t excl = incl - d;
@@ -133,36 +162,82 @@ __global__ __launch_bounds__(MB) void k_ms_push(const uint32_t* __restrict__ q, 
             uint32_t v = 0;
             if (t < total) {
                 v = e < oa ? fwd.tgt_a[osa + e] : fwd.tgt_b[osb + (e - oa)];
-                const unsigned long long bits = ofm & ~visit[v];
-                if (bits) {
-                    const unsigned long long old = atomicOr(&visit[v], bits);
-                    const unsigned long long mine = bits & ~old;
-                    if (mine) {
-                        write_levels(mine, levels, n, v, L);
-                        cnt += __popcll(mine);
-                        first = atomicOr(&next[v], mine) == 0;
-                    }
-                }
+                first = ms_push_edge(v, ofm, visit, next, levels, n, L, cnt);
             }
             const uint32_t slot = wave_append_slot(first, &ctr->qlen);
-            if (first) nq[slot] = v;
+            if (first) {
+                nq[slot] = v;
+                edges += nv_degree(fwd, v);
+            }
         }
     }
     cnt = wave_sum_u64(cnt);
-    if (lane == 0 && cnt) atomicAdd(&ctr->newbits, cnt);
+    edges = wave_sum_u64(edges);
+    if (lane == 0) {
+        if (cnt) atomicAdd(&ctr->newbits, cnt);
+        if (edges) atomicAdd(&ctr->edges, edges);
+    }
+}
+
+// Heavy push-frontier vertices: every wave of the grid takes a stride of each
+// heavy vertex's edges.
+__global__ __launch_bounds__(MB) void k_ms_push_heavy(const uint32_t* __restrict__ heavy, NbrView fwd,
+                                                      const unsigned long long* __restrict__ front,
+                                                      unsigned long long* __restrict__ next,
+                                                      unsigned long long* __restrict__ visit,
+                                                      int32_t* __restrict__ levels, uint64_t n, int32_t L,
+                                                      uint32_t* __restrict__ nq, MsCtr* ctr) {
+    const uint32_t cnt_h = ctr->heavy;
+    const uint64_t wave_base = (blockIdx.x * (uint64_t)MW + (threadIdx.x >> 6)) * 64;
+    const uint64_t stride = (uint64_t)gridDim.x * MB;
+    unsigned long long cnt = 0, edges = 0;
+    for (uint32_t h = 0; h < cnt_h; ++h) {
+        const uint32_t u = heavy[h];
+        uint32_t da;
+        const uint32_t d = nv_degree(fwd, u, da);
+        const uint32_t sa = fwd.off_a[u];
+        const uint32_t sb = fwd.off_b ? fwd.off_b[u] : 0;
+        const unsigned long long fm = front[u];
+        for (uint64_t e0 = wave_base; e0 < d; e0 += stride) {
+            const uint64_t e = e0 + lane_id();
+            bool first = false;
+            uint32_t v = 0;
+            if (e < d) {
+                v = e < da ? fwd.tgt_a[sa + e] : fwd.tgt_b[sb + (e - da)];
+                first = ms_push_edge(v, fm, visit, next, levels, n, L, cnt);
+            }
+            const uint32_t slot = wave_append_slot(first, &ctr->qlen);
+            if (first) {
+                nq[slot] = v;
+                edges += nv_degree(fwd, v);
+            }
+        }
+    }
+    cnt = wave_sum_u64(cnt);
+    edges = wave_sum_u64(edges);
+    if (lane_id() == 0) {
+        if (cnt) atomicAdd(&ctr->newbits, cnt);
+        if (edges) atomicAdd(&ctr->edges, edges);
+    }
 }
 
 // queue of vertices with a non-zero front mask (pull -> push hand-over).
 __global__ __launch_bounds__(MB) void k_ms_compact(const unsigned long long* __restrict__ front, uint64_t n,
-                                                   uint32_t* __restrict__ q, MsCtr* ctr) {
+                                                   NbrView fwd, uint32_t* __restrict__ q, MsCtr* ctr) {
     const uint64_t gw = blockIdx.x * (uint64_t)MW + (threadIdx.x >> 6);
     const uint64_t nwaves = (uint64_t)gridDim.x * MW;
+    unsigned long long edges = 0;
     for (uint64_t base = gw * 64; base < n; base += nwaves * 64) {
         const uint64_t v = base + lane_id();
         const bool take = v < n && front[v] != 0;
         const uint32_t slot = wave_append_slot(take, &ctr->qlen);
-        if (take) q[slot] = (uint32_t)v;
+        if (take) {
+            q[slot] = (uint32_t)v;
+            edges += nv_degree(fwd, (uint32_t)v);
+        }
     }
+    edges = wave_sum_u64(edges);
+    if (lane_id() == 0 && edges) atomicAdd(&ctr->edges, edges);
 }
 
 // TEPS numerator: sum over vertices of forward degree x reached lanes.
@@ -183,10 +258,11 @@ void direction_views(ursa_hip_topo* t, int dir, hipStream_t s, NbrView& fwd, Nbr
 struct MsWork {
     uint64_t n;
     DevBuf<unsigned long long> visit, f0, f1, redges;
-    DevBuf<uint32_t> q0, q1, dsrc;
+    DevBuf<uint32_t> q0, q1, dsrc, heavy;
     DevBuf<MsCtr> ctr;
     MsCtr* h = nullptr;
-    explicit MsWork(uint64_t n_) : n(n_), visit(n_), f0(n_), f1(n_), redges(1), q0(n_), q1(n_), dsrc(64), ctr(1) {
+    explicit MsWork(uint64_t n_)
+        : n(n_), visit(n_), f0(n_), f1(n_), redges(1), q0(n_), q1(n_), dsrc(64), heavy(n_), ctr(1) {
         HIP_OK(hipHostMalloc((void**)&h, sizeof(MsCtr), hipHostMallocDefault));
     }
     ~MsWork() {
@@ -199,17 +275,19 @@ struct MsWork {
     }
 };
 
-// One batch of nb <= 64 sources. Push while the frontier holds fewer than
-// n / push_div vertices, pull otherwise.
-void msbfs_batch(uint64_t n, const uint32_t* src_host, uint32_t nb, const NbrView& fwd, const NbrView& rev,
-                 int32_t* levels, MsWork& w, hipStream_t s, uint64_t* reached_edges, uint32_t push_div) {
+// One batch of nb <= 64 sources. A level pushes while its frontier's forward
+// edges number fewer than fwd_edges / push_div (and the frontier is a known
+// queue), and pulls otherwise.
+void msbfs_batch(uint64_t n, uint64_t fwd_edges, const uint32_t* src_host, uint32_t nb, const NbrView& fwd,
+                 const NbrView& rev, int32_t* levels, MsWork& w, hipStream_t s, uint64_t* reached_edges,
+                 uint32_t push_div) {
     const unsigned gv = blocks_for(n, MB, 1u << 16);
     k_ms_fill_levels<<<blocks_for(n * nb, 256, 1u << 16), 256, 0, s>>>(levels, n * nb);
     HIP_OK(hipMemsetAsync(w.visit.get(), 0, n * sizeof(unsigned long long), s));
     HIP_OK(hipMemsetAsync(w.f0.get(), 0, n * sizeof(unsigned long long), s));
     HIP_OK(hipMemcpyAsync(w.dsrc.get(), src_host, nb * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     HIP_OK(hipMemsetAsync(w.ctr.get(), 0, sizeof(MsCtr), s));
-    k_ms_seed<<<1, 1, 0, s>>>(w.dsrc.get(), nb, n, w.visit.get(), w.f0.get(), levels, w.q0.get(), w.ctr.get());
+    k_ms_seed<<<1, 1, 0, s>>>(w.dsrc.get(), nb, n, fwd, w.visit.get(), w.f0.get(), levels, w.q0.get(), w.ctr.get());
     check_launch();
     unsigned long long valid = 0;
     for (uint32_t b = 0; b < nb; ++b)
@@ -218,28 +296,45 @@ void msbfs_batch(uint64_t n, const uint32_t* src_host, uint32_t nb, const NbrVie
     unsigned long long* next = w.f1.get();
     uint32_t* q = w.q0.get();
     uint32_t* nq = w.q1.get();
-    uint64_t qlen = w.read(s).qlen;
+    MsCtr c0 = w.read(s);
+    uint64_t qlen = c0.qlen;
+    uint64_t fedges = c0.edges;
     bool have_queue = true;
+    bool known_edges = true;
     for (int32_t L = 1; qlen > 0; ++L) {
+        // Frontier edge count for the direction choice: known after a push or a
+        // compaction; after a pull it is only bounded, so pull continues until
+        // the settled count shows a small frontier, then a compaction measures it.
+        bool push = have_queue && known_edges ? fedges * (uint64_t)push_div < fwd_edges
+                                              : (have_queue && L == 1);
+        if (!have_queue && qlen * 64 < n) {
+            HIP_OK(hipMemsetAsync(w.ctr.get(), 0, sizeof(MsCtr), s));
+            k_ms_compact<<<gv / MW + 1, MB, 0, s>>>(front, n, fwd, q, w.ctr.get());
+            check_launch();
+            const MsCtr c = w.read(s);
+            qlen = c.qlen;
+            fedges = c.edges;
+            have_queue = true;
+            known_edges = true;
+            if (qlen == 0) break;
+            push = fedges * (uint64_t)push_div < fwd_edges;
+        }
         HIP_OK(hipMemsetAsync(w.ctr.get(), 0, sizeof(MsCtr), s));
-        const bool push = qlen * (uint64_t)push_div < n;
         if (push) {
-            if (!have_queue) {
-                k_ms_compact<<<gv / MW + 1, MB, 0, s>>>(front, n, q, w.ctr.get());
-                check_launch();
-                qlen = w.read(s).qlen;
-                HIP_OK(hipMemsetAsync(w.ctr.get(), 0, sizeof(MsCtr), s));
-                if (qlen == 0) break;
-            }
             HIP_OK(hipMemsetAsync(next, 0, n * sizeof(unsigned long long), s));
             timed("msbfs_push", 0.0, s, [&] {
                 k_ms_push<<<blocks_for((qlen + 63) / 64, MW, 1u << 16), MB, 0, s>>>(q, (uint32_t)qlen, fwd, front, next,
                                                                                   w.visit.get(), levels, n, L, nq,
-                                                                                  w.ctr.get());
+                                                                                  w.ctr.get(), w.heavy.get());
+                if (fedges > MS_HEAVY)
+                    k_ms_push_heavy<<<blocks_for(std::max<uint64_t>(fedges, 64), MB, 2048), MB, 0, s>>>(
+                        w.heavy.get(), fwd, front, next, w.visit.get(), levels, n, L, nq, w.ctr.get());
             });
             check_launch();
             const MsCtr c = w.read(s);
             qlen = c.qlen;
+            fedges = c.edges;
+            known_edges = true;
             std::swap(q, nq);
             have_queue = true;
         } else {
@@ -248,10 +343,11 @@ void msbfs_batch(uint64_t n, const uint32_t* src_host, uint32_t nb, const NbrVie
             });
             check_launch();
             const MsCtr c = w.read(s);
-            // The vertex count of the new frontier is not known after a pull; the
-            // settled-pair count bounds it and picks the next direction.
+            // After a pull only the settled-pair count is known; it bounds the
+            // new frontier's vertex count.
             qlen = c.newbits;
             have_queue = false;
+            known_edges = false;
         }
         std::swap(front, next);
     }
@@ -273,9 +369,10 @@ void msbfs_levels(ursa_hip_topo* t, const std::vector<uint32_t>& src, int dir, i
     NbrView fwd, rev;
     direction_views(t, dir, s, fwd, rev);
     MsWork w(n);
+    const uint64_t fwd_edges = dir == URSA_HIP_BOTH ? 2 * t->m : t->m;
     for (uint64_t b0 = 0; b0 < src.size(); b0 += 64) {
         const uint32_t nb = (uint32_t)std::min<uint64_t>(64, src.size() - b0);
-        msbfs_batch(n, src.data() + b0, nb, fwd, rev, out_dev + b0 * n, w, s, reached_edges, push_div);
+        msbfs_batch(n, fwd_edges, src.data() + b0, nb, fwd, rev, out_dev + b0 * n, w, s, reached_edges, push_div);
     }
 }
 
