@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "frontier.hpp"
+#include "prim.hpp"
 
 namespace ursa {
 
@@ -89,12 +90,18 @@ __global__ void k_ms_seed(const uint32_t* src, uint32_t nb, uint64_t n, NbrView 
     ctr->edges = edges;
 }
 
+// Pull-side vertices with more reverse edges than this are ORed chunk by chunk
+// by whole waves (k_ms_pull_heavy); OR is order-free, so the result is exact.
+constexpr uint32_t PULL_HEAVY = 256;
+constexpr uint32_t PULL_CHUNK = 4096;
+
 __global__ __launch_bounds__(MB) void k_ms_pull(uint64_t n, NbrView rev, const unsigned long long* __restrict__ front,
                                                 unsigned long long* __restrict__ next,
                                                 unsigned long long* __restrict__ visit, unsigned long long valid,
                                                 int32_t* __restrict__ levels, int32_t L, MsCtr* ctr) {
     unsigned long long cnt = 0;
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        if (nv_degree(rev, (uint32_t)v) > PULL_HEAVY) continue;  // k_ms_pull_heavy + k_ms_pull_finish
         const unsigned long long vis = visit[v];
         const unsigned long long want = valid & ~vis;
         unsigned long long acc = 0;
@@ -120,6 +127,78 @@ __global__ __launch_bounds__(MB) void k_ms_pull(uint64_t n, NbrView rev, const u
     }
     cnt = wave_sum_u64(cnt);
     if (lane_id() == 0 && cnt) atomicAdd(&ctr->newbits, cnt);
+}
+
+// One wave per chunk of a heavy vertex's reverse list (out list then in list
+// for Both): OR of the frontier masks, merged into hacc[h] with atomic OR.
+__global__ __launch_bounds__(MB) void k_ms_pull_heavy(const uint32_t* __restrict__ heavy,
+                                                      const uint32_t* __restrict__ cbase, uint32_t nh, uint32_t chunks,
+                                                      NbrView rev, const unsigned long long* __restrict__ front,
+                                                      const unsigned long long* __restrict__ visit,
+                                                      unsigned long long valid, unsigned long long* __restrict__ hacc) {
+    const uint64_t gw = blockIdx.x * (uint64_t)MW + (threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * MW;
+    for (uint64_t c = gw; c < chunks; c += nwaves) {
+        uint32_t lo = 0, hi = nh;  // owner: largest h with cbase[h] <= c
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (cbase[mid] <= c)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const uint32_t h = lo, v = heavy[h];
+        const unsigned long long want = valid & ~visit[v];
+        if (!want) continue;
+        uint32_t da;
+        const uint32_t d = nv_degree(rev, v, da);
+        const uint32_t e0 = ((uint32_t)c - cbase[h]) * PULL_CHUNK;
+        const uint32_t e1 = min(d, e0 + PULL_CHUNK);
+        unsigned long long acc = 0;
+        for (uint32_t e = e0 + lane_id(); e < e1; e += 64) {
+            const uint32_t w = e < da ? rev.tgt_a[rev.off_a[v] + e] : rev.tgt_b[rev.off_b[v] + (e - da)];
+            acc |= front[w];
+        }
+        for (int off = 32; off > 0; off >>= 1) acc |= __shfl_down(acc, off, 64);
+        acc &= want;
+        if (lane_id() == 0 && acc) atomicOr(&hacc[h], acc);
+    }
+}
+
+__global__ void k_ms_pull_finish(const uint32_t* __restrict__ heavy, uint32_t nh,
+                                 unsigned long long* __restrict__ hacc, unsigned long long* __restrict__ next,
+                                 unsigned long long* __restrict__ visit, int32_t* __restrict__ levels, uint64_t n,
+                                 int32_t L, MsCtr* ctr) {
+    const uint64_t h = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    unsigned long long cnt = 0;
+    if (h < nh) {
+        const uint32_t v = heavy[h];
+        const unsigned long long acc = hacc[h] & ~visit[v];
+        hacc[h] = 0;
+        next[v] = acc;
+        if (acc) {
+            visit[v] |= acc;
+            write_levels(acc, levels, n, v, L);
+            cnt = __popcll(acc);
+        }
+    }
+    cnt = wave_sum_u64(cnt);
+    if (lane_id() == 0 && cnt) atomicAdd(&ctr->newbits, cnt);
+}
+
+__global__ void k_ms_heavy_flags(NbrView rev, uint64_t n, uint8_t* flag, uint32_t* iota, uint32_t* chunks) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t d = nv_degree(rev, (uint32_t)v);
+        flag[v] = d > PULL_HEAVY;
+        iota[v] = (uint32_t)v;
+        chunks[v] = (d + PULL_CHUNK - 1) / PULL_CHUNK;
+    }
+}
+
+__global__ void k_ms_gather(const uint32_t* __restrict__ idx, uint32_t cnt, const uint32_t* __restrict__ src,
+                            uint32_t* __restrict__ out) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < cnt) out[i] = src[idx[i]];
 }
 
 __global__ __launch_bounds__(MB) void k_ms_push(const uint32_t* __restrict__ q, uint32_t qlen, NbrView fwd,
@@ -257,8 +336,9 @@ void direction_views(ursa_hip_topo* t, int dir, hipStream_t s, NbrView& fwd, Nbr
 // Levels for up to 64 sources (row b of `levels` = source b) on one stream.
 struct MsWork {
     uint64_t n;
-    DevBuf<unsigned long long> visit, f0, f1, redges;
-    DevBuf<uint32_t> q0, q1, dsrc, heavy;
+    DevBuf<unsigned long long> visit, f0, f1, redges, hacc;
+    DevBuf<uint32_t> q0, q1, dsrc, heavy, pheavy, pcbase;
+    uint32_t pnh = 0, pchunks = 0;  // pull-side heavy vertices and their chunk count
     DevBuf<MsCtr> ctr;
     MsCtr* h = nullptr;
     explicit MsWork(uint64_t n_)
@@ -272,6 +352,31 @@ struct MsWork {
         HIP_OK(hipMemcpyAsync(h, ctr.get(), sizeof(MsCtr), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
         return *h;
+    }
+    // Heavy reverse-list vertices and chunk bases for the pull levels.
+    void prepare_pull(const NbrView& rev, hipStream_t s) {
+        DevBuf<uint8_t> flag(n);
+        DevBuf<uint32_t> iota(n), chunks(n);
+        DevBuf<uint64_t> cnt(1);
+        pheavy.alloc(n);
+        k_ms_heavy_flags<<<blocks_for(n, 256, 1u << 16), 256, 0, s>>>(rev, n, flag.get(), iota.get(), chunks.get());
+        check_launch();
+        prim::select_flagged_u32(iota.get(), flag.get(), pheavy.get(), cnt.get(), n, s);
+        uint64_t c = 0;
+        HIP_OK(hipMemcpyAsync(&c, cnt.get(), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        pnh = (uint32_t)c;
+        pcbase.alloc(pnh + 1);
+        hacc.alloc(pnh ? pnh : 1);
+        HIP_OK(hipMemsetAsync(hacc.get(), 0, (pnh ? pnh : 1) * sizeof(unsigned long long), s));
+        if (pnh) {
+            DevBuf<uint32_t> hch(pnh);
+            k_ms_gather<<<blocks_for(pnh, 256), 256, 0, s>>>(pheavy.get(), pnh, chunks.get(), hch.get());
+            check_launch();
+            prim::exclusive_sum_u32(hch.get(), pcbase.get(), pnh, s);
+            HIP_OK(hipMemcpyAsync(&pchunks, pcbase.get() + pnh, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        }
+        HIP_OK(hipStreamSynchronize(s));
     }
 };
 
@@ -340,6 +445,13 @@ void msbfs_batch(uint64_t n, uint64_t fwd_edges, const uint32_t* src_host, uint3
         } else {
             timed("msbfs_pull", 0.0, s, [&] {
                 k_ms_pull<<<gv, MB, 0, s>>>(n, rev, front, next, w.visit.get(), valid, levels, L, w.ctr.get());
+                if (w.pnh) {
+                    k_ms_pull_heavy<<<blocks_for(w.pchunks, MW, 1u << 16), MB, 0, s>>>(
+                        w.pheavy.get(), w.pcbase.get(), w.pnh, w.pchunks, rev, front, w.visit.get(), valid,
+                        w.hacc.get());
+                    k_ms_pull_finish<<<blocks_for(w.pnh, 256), 256, 0, s>>>(w.pheavy.get(), w.pnh, w.hacc.get(), next,
+                                                                           w.visit.get(), levels, n, L, w.ctr.get());
+                }
             });
             check_launch();
             const MsCtr c = w.read(s);
@@ -369,6 +481,7 @@ void msbfs_levels(ursa_hip_topo* t, const std::vector<uint32_t>& src, int dir, i
     NbrView fwd, rev;
     direction_views(t, dir, s, fwd, rev);
     MsWork w(n);
+    w.prepare_pull(rev, s);
     const uint64_t fwd_edges = dir == URSA_HIP_BOTH ? 2 * t->m : t->m;
     for (uint64_t b0 = 0; b0 < src.size(); b0 += 64) {
         const uint32_t nb = (uint32_t)std::min<uint64_t>(64, src.size() - b0);
