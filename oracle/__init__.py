@@ -51,6 +51,7 @@ def _load() -> C.CDLL:
         "orc_degree": (None, [P, i, P]),
         "orc_pagerank": (None, [P, P, f64, u32, f64, i, P, P, P]),
         "orc_bfs": (None, [P, u32, i, P]),
+        "orc_ppr": (None, [P, P, u64, f64, u32, f64, P, P]),
         "orc_k_hop": (u64, [P, P, u64, u32, i, P, P, u64]),
         "orc_shortest_path": (u64, [P, u32, u32, i, P, u64]),
         "orc_betweenness_sources": (None, [P, P, u64, i, f64, i, P]),
@@ -141,6 +142,15 @@ def pagerank(t: Topology, damping=0.85, max_iter=30, tol=1e-6, weights=None, nth
             raise ValueError("weights length must equal the edge count")
     lib().orc_pagerank(t._h, _p(w), damping, max_iter, tol, nthreads, _p(out), C.byref(it), C.byref(dl))
     return out, it.value, dl.value
+
+
+def ppr(t: Topology, seeds, damping=0.85, max_iter=100, tol=1e-8):
+    """Personalized PageRank -> (f64[n], iterations)."""
+    s = _u32(seeds)
+    out = np.zeros(max(t.n, 1), np.float64)
+    it = C.c_uint32(0)
+    lib().orc_ppr(t._h, _p(s) if s.size else None, s.shape[0], damping, max_iter, tol, _p(out), C.byref(it))
+    return out[: t.n], it.value
 
 
 def bfs(t: Topology, source: int, direction="out") -> np.ndarray:

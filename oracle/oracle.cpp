@@ -367,6 +367,48 @@ void orc_pagerank(orc_topo* t, const double* weights, double damping, uint32_t m
     if (last_delta_out) *last_delta_out = delta;
 }
 
+// Personalized PageRank: pagerank.rs:63-124's loop with the personalization
+// vector p in place of the uniform teleport (and dangling) distribution.
+void orc_ppr(orc_topo* t, const uint32_t* seeds, uint64_t nseeds, double damping, uint32_t max_iter, double tol,
+             double* out, uint32_t* iters_out) {
+    const uint64_t n = t->n;
+    if (iters_out) *iters_out = 0;
+    if (n == 0) return;
+    std::vector<double> p(n, 0.0);
+    std::vector<uint32_t> uniq;
+    for (uint64_t i = 0; i < nseeds; ++i)
+        if (seeds[i] < n) uniq.push_back(seeds[i]);
+    std::sort(uniq.begin(), uniq.end());
+    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    for (uint32_t u : uniq) p[u] = 1.0 / (double)uniq.size();
+    const Adj& inc = t->incoming();
+    std::vector<uint32_t> od(n);
+    std::vector<uint64_t> dangling;
+    for (uint64_t u = 0; u < n; ++u) {
+        od[u] = t->out.degree((uint32_t)u);
+        if (od[u] == 0) dangling.push_back(u);
+    }
+    const double d = damping;
+    std::vector<double> rank(n, 1.0 / (double)n), next(n, 0.0);
+    uint32_t iters = 0;
+    for (uint32_t it = 0; it < max_iter && !uniq.empty(); ++it) {
+        double dm = -0.0;
+        for (uint64_t u : dangling) dm += rank[u];
+        const double pb = (1.0 - d) + d * dm;
+        for (uint64_t u = 0; u < n; ++u) {
+            double acc = 0.0;
+            for (uint64_t k = inc.off[u]; k < inc.off[u + 1]; ++k) acc += rank[inc.tgt[k]] / (double)od[inc.tgt[k]];
+            next[u] = d * acc + pb * p[u];
+        }
+        const double delta = l1_delta(next, rank);
+        std::swap(rank, next);
+        iters = it + 1;
+        if (delta < tol) break;
+    }
+    std::memcpy(out, rank.data(), n * sizeof(double));
+    if (iters_out) *iters_out = iters;
+}
+
 // ---- bfs.rs:18-42 ----
 void orc_bfs(orc_topo* t, uint32_t source, int dir, int32_t* dist) {
     const uint64_t n = t->n;

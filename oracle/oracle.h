@@ -63,6 +63,13 @@ void orc_degree(orc_topo* t, int dir, uint32_t* out);
 void orc_pagerank(orc_topo* t, const double* weights, double damping, uint32_t max_iter,
                   double tol, int nthreads, double* out, uint32_t* iters_out,
                   double* last_delta_out);
+/* Personalized PageRank (serving extension; no ursa-core counterpart): the
+ * pull iteration of pagerank.rs:63-124 with teleport and dangling mass sent to
+ * the seed set, p = uniform over the distinct seeds (networkx personalization
+ * semantics): next[u] = d * acc[u] + ((1 - d) + d * dangling) * p[u].
+ * Start 1/n, stop after the first iteration with L1 delta < tol. */
+void orc_ppr(orc_topo* t, const uint32_t* seeds, uint64_t nseeds, double damping, uint32_t max_iter,
+             double tol, double* out, uint32_t* iters_out);
 void orc_bfs(orc_topo* t, uint32_t source, int dir, int32_t* out);
 /* hop.rs:52-130. Returns number of pairs; writes min(cap, count). */
 uint64_t orc_k_hop(orc_topo* t, const uint32_t* seeds, uint64_t nseeds, uint32_t k, int dir,

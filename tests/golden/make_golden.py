@@ -130,6 +130,16 @@ def main():
     g4d = nx.DiGraph(); g4d.add_edges_from(e4)
     fx["ba500_bc_ids"], fx["ba500_bc"] = keyed(nx.betweenness_centrality(g4d, normalized=False, endpoints=False))
 
+    # 5) Personalized PageRank (networkx personalization: teleport and dangling
+    #    mass to the seed set) on the dangling-heavy sparse graph and gnp100.
+    for name, gg in (("sparse400", g3d), ("gnp100", gd)):
+        nodes_sorted = sorted(gg.nodes())
+        for qi, seeds in enumerate(([nodes_sorted[0]], nodes_sorted[3:9:2], nodes_sorted[-5:])):
+            pers = {v: 1.0 for v in seeds}
+            ref = nx.pagerank(gg, alpha=0.85, personalization=pers, tol=1e-14, max_iter=10000)
+            fx[f"{name}_ppr{qi}_seeds"] = np.array(seeds, dtype=np.int64)
+            fx[f"{name}_ppr{qi}_ids"], fx[f"{name}_ppr{qi}"] = keyed(ref)
+
     # splitmix64 streams
     seeds = np.array([0, 4, 7, 42, 0x5EED5EED5EED5EED, 1234567], dtype=np.uint64)
     fx["splitmix_seeds"] = seeds

@@ -436,3 +436,25 @@ def test_er_generator_and_weights():
     w = orc.gen_weights(1000, 3)
     assert np.all(w > 0) and np.all(w <= 1.0)
     assert np.array_equal(w.astype(np.float32).astype(np.float64), w)
+
+
+# ---------------------------------------------------------------- personalized PageRank
+@pytest.mark.parametrize("name", ["sparse400", "gnp100"])
+@pytest.mark.parametrize("qi", [0, 1, 2])
+def test_ppr_vs_networkx_personalization(name, qi):
+    t, users = topo_from_user(FX[f"{name}_src"], FX[f"{name}_dst"])
+    dense = {int(u): i for i, u in enumerate(users)}
+    seeds = [dense[int(x)] for x in FX[f"{name}_ppr{qi}_seeds"]]
+    pr, _ = orc.ppr(t, seeds, 0.85, 10000, 1e-15)
+    got = by_user(users, pr)
+    for k, v in zip(FX[f"{name}_ppr{qi}_ids"].tolist(), FX[f"{name}_ppr{qi}"].tolist()):
+        assert abs(got[k] - v) < 1e-11
+
+
+def test_ppr_with_all_nodes_as_seeds_is_pagerank():
+    # p uniform over every node: teleport and dangling terms match ursa's
+    # global PageRank algebraically (within rounding).
+    t, _ = topo_from_user(FX["sparse400_src"], FX["sparse400_dst"])
+    pr, _, _ = orc.pagerank(t, 0.85, 10000, 1e-15)
+    pp, _ = orc.ppr(t, list(range(t.n)), 0.85, 10000, 1e-15)
+    assert np.allclose(pr, pp, rtol=1e-12, atol=0)
