@@ -58,6 +58,23 @@ int ursa_hip_pagerank_mixed(ursa_hip_topo* t, const double* weights, double damp
                             uint32_t max_iter, double tol, double* out, uint32_t* iters_out,
                             uint32_t* f32_iters_out);
 
+/* ---- personalized PageRank (serving extension; ursa-core has none) ----
+ * pagerank.rs:63-124's iteration with the teleport and dangling mass sent to
+ * the seed set (networkx personalization semantics): x0 = 1/n,
+ * next[u] = d * sum_{v in in(u)} x[v]/od[v] + ((1-d) + d*dm) * p[u], p uniform
+ * over the distinct in-range seeds, stop after the first iteration whose L1
+ * delta < tol. f32 compensated sweeps first, then f64 sweeps refine to tol.
+ * No seed in range (or max_iter 0): out = 1/n, *iters_out = 0. out is f64[n]. */
+int ursa_hip_ppr(ursa_hip_topo* t, const uint32_t* seeds, uint64_t nseeds, double damping,
+                 uint32_t max_iter, double tol, double* out, uint32_t* iters_out);
+/* k queries on one graph solved together, up to 16 per device pass; query q's
+ * seeds are seeds[seed_offsets[q] .. seed_offsets[q+1]), its result
+ * out[q*n .. (q+1)*n) and its iteration count iters_out[q]. Each query's result
+ * is bit-identical to ursa_hip_ppr on the same arguments. */
+int ursa_hip_ppr_batch(ursa_hip_topo* t, uint32_t k, const uint64_t* seed_offsets,
+                       const uint32_t* seeds, double damping, uint32_t max_iter, double tol,
+                       double* out, uint32_t* iters_out);
+
 /* Streaming device-to-device copy of `bytes` (multiple of 16) by a plain
  * kernel: the reference point for hardware byte counters (k_copy_probe moves
  * exactly `bytes` in and `bytes` out). */

@@ -1,0 +1,235 @@
+"""GPU parity for personalized PageRank and the request server, through the C ABI.
+
+Bars:
+  * PPR vs the oracle: L1 <= 1e-10 and max-abs <= 1e-11 at tol 1e-12 (both sides
+    stop at delta < tol, so each is within tol*d/(1-d) ~ 6e-12 L1 of the f64 fixed
+    point); L1 <= 2e-7 at the serving tolerance 1e-8.
+  * PPR lanes: every query of a batch is bit-identical to the same query alone.
+  * Server: every query's result bytes (digest) equal the plain serial loop's, the
+    same query run alone through the server, and the oracle's (k-hop pairs, paths
+    and triangle counts bit-exact; PPR within the bar above).
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+from graphs import cases, rmat_dense
+
+pytestmark = pytest.mark.gpu
+
+GRAPHS = [g for g in cases() if g[1] > 0]
+IDS = [g[0] for g in GRAPHS]
+
+
+@pytest.fixture(scope="module")
+def ua():
+    import ursa_amd
+
+    ursa_amd.init(0)
+    return ursa_amd
+
+
+@pytest.fixture(scope="module")
+def sv(ua):
+    from ursa_amd import serve
+
+    return serve
+
+
+def seed_sets(n):
+    return [[0], [n // 2, n // 3, n - 1], list(range(0, n, max(1, n // 16)))[:16], [n - 1, n - 1, 0]]
+
+
+@pytest.mark.parametrize("case", GRAPHS, ids=IDS)
+def test_ppr_matches_oracle(ua, case):
+    name, n, s, d = case
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    for seeds in seed_sets(n):
+        got, it = ua.ppr(t, seeds, 0.85, 1000, 1e-12)
+        ref, rit = orc.ppr(o, seeds, 0.85, 1000, 1e-12)
+        assert np.abs(got - ref).sum() <= 1e-10, (name, seeds)
+        assert np.abs(got - ref).max() <= 1e-11, (name, seeds)
+        assert it > 0 and rit > 0, name
+
+
+def test_ppr_serving_tolerance_rmat(ua):
+    n, s, d = rmat_dense(13, 6)
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    rng = np.random.default_rng(2)
+    for k in (1, 5, 16):
+        seeds = rng.integers(0, n, k)
+        got, _ = ua.ppr(t, seeds, 0.85, 100, 1e-8)
+        ref, _ = orc.ppr(o, seeds, 0.85, 100, 1e-8)
+        assert np.abs(got - ref).sum() <= 2e-7
+        assert abs(got.sum() - 1.0) < 1e-9
+
+
+def test_ppr_networkx_fixtures(ua):
+    from graphs import FX
+
+    for name in ("sparse400", "gnp100"):
+        n, s, d, users = orc.intern_i64(FX[f"{name}_src"], FX[f"{name}_dst"])
+        dense = {int(u): i for i, u in enumerate(users)}
+        t = ua.Topology.build(n, s, d)
+        for qi in range(3):
+            seeds = [dense[int(x)] for x in FX[f"{name}_ppr{qi}_seeds"]]
+            got, _ = ua.ppr(t, seeds, 0.85, 10000, 1e-15)
+            want = dict(zip(FX[f"{name}_ppr{qi}_ids"].tolist(), FX[f"{name}_ppr{qi}"].tolist()))
+            for u, i in dense.items():
+                assert abs(got[i] - want[u]) < 1e-11, (name, qi, u)
+
+
+def test_ppr_batch_lanes_are_bit_identical_to_alone(ua):
+    n, s, d = rmat_dense(12, 3)
+    t = ua.Topology.build(n, s, d)
+    rng = np.random.default_rng(5)
+    lists = [rng.integers(0, n, int(rng.integers(1, 17))).tolist() for _ in range(21)]
+    lists[4] = [n + 3]  # no seed in range
+    lists[7] = lists[2]  # duplicate query
+    tols = 1e-8
+    alone = [ua.ppr(t, sl, 0.85, 100, tols) for sl in lists]
+    for group in (lists, lists[:3], lists[5:13][::-1]):
+        out, iters = ua.ppr_batch(t, group, 0.85, 100, tols)
+        for q, sl in enumerate(group):
+            ref, rit = alone[lists.index(sl)]
+            assert np.array_equal(out[q], ref) and iters[q] == rit, q
+    x, it = alone[4]
+    assert it == 0 and np.all(x == 1.0 / n)
+
+
+def test_ppr_edge_cases(ua):
+    t = ua.Topology.build(4, [0, 1, 2], [1, 2, 0])  # node 3 isolated and dangling
+    o = orc.Topology(4, [0, 1, 2], [1, 2, 0])
+    x, it = ua.ppr(t, [3], 0.85, 1000, 1e-13)
+    r, _ = orc.ppr(o, [3], 0.85, 1000, 1e-13)
+    assert np.abs(x - r).max() < 1e-12 and it > 0
+    x, it = ua.ppr(t, [1], 0.85, 0, 1e-8)
+    assert it == 0 and np.all(x == 0.25)
+    empty = ua.Topology.build(0, [], [])
+    x, it = ua.ppr(empty, [0], 0.85, 10, 1e-8)
+    assert x.size == 0 and it == 0
+
+
+# ---------------------------------------------------------------- server
+@pytest.fixture(scope="module")
+def pool(ua, sv):
+    specs = []
+    for i, (kind, a, ef) in enumerate([("rmat", 11, 8), ("er", 3000, 6), ("rmat", 12, 16), ("er", 1500, 20),
+                                        ("rmat", 10, 4), ("rmat", 13, 12)]):
+        sp = sv.GraphSpec(i, kind, (1 << a) if kind == "rmat" else a, ef, 1000 + i, a if kind == "rmat" else 0)
+        specs.append(sp)
+    topos = [sv.build_serving_graph(sp) for sp in specs]
+    dense = []
+    from ursa_amd import gen
+
+    for sp in specs:
+        e = gen.rmat(sp.scale, sp.edge_factor, seed=sp.seed) if sp.kind == "rmat" else \
+            gen.erdos_renyi(sp.n_target, sp.edge_factor, sp.seed)
+        _, _, sd, dd, ids = e.export()
+        dense.append((ids.shape[0], sd, dd))
+    return topos, dense
+
+
+def oracle_result(o, q, sv):
+    seeds = list(q.seeds)[: q.nseeds]
+    dirn = ("out", "in", "both")[q.direction]
+    if q.kind == sv.PPR:
+        return orc.ppr(o, seeds, q.damping, q.max_iter, q.tol)[0]
+    if q.kind == sv.KHOP:
+        return orc.k_hop(o, seeds, q.k, dirn)
+    if q.kind == sv.PATH:
+        return orc.shortest_path(o, seeds[0], seeds[1], dirn)
+    return orc.triangle_count(o)
+
+
+def same(kind, a, b, sv):
+    if kind == sv.PPR:
+        return np.abs(a - b).sum() <= 2e-7
+    if kind == sv.KHOP:
+        return np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    if kind == sv.PATH:
+        return (a is None and b is None) or (a is not None and b is not None and np.array_equal(a, b))
+    return np.array_equal(a, b)
+
+
+def test_server_mix_equals_serial_alone_and_oracle(ua, sv, pool):
+    topos, dense = pool
+    ns = [t.n_nodes() for t in topos]
+    trace = sv.serving_trace(ns, 240, qps=0, seed=11)
+    # Pile several queries onto one graph so batches form.
+    trace += [sv.ppr_query(2, [i, i + 1]) for i in range(20)] + [sv.triangles_query(1) for _ in range(3)]
+    trace += [sv.khop_query(5, [i * 7]) for i in range(10)] + [sv.path_query(0, i, i * 3 + 1) for i in range(10)]
+
+    with sv.Server(serial=True) as ser:
+        for t in topos:
+            ser.add_graph(t)
+        rs = ser.run(trace)
+    with sv.Server(slots=4, max_batch=16, keep_results=True) as srv:
+        for t in topos:
+            srv.add_graph(t)
+        rm = srv.run(trace)
+        assert max(r.batch for r in rm) > 1
+        orcs = [orc.Topology(n, s, d) for (n, s, d) in dense]
+        for i, q in enumerate(trace):
+            assert rs[i].status == 0 and rm[i].status == 0, (i, rs[i].status, rm[i].status)
+            assert rs[i].digest == rm[i].digest and rs[i].len == rm[i].len, (i, q.kind)
+            if q.kind == sv.PPR:
+                assert rs[i].iters == rm[i].iters
+            got = srv.result(i, q)
+            assert sv.digest(np.frombuffer(srv.result_bytes(i), np.uint8)) == rm[i].digest
+            assert same(q.kind, got, oracle_result(orcs[q.graph], q, sv), sv), (i, q.kind)
+        # Alone: each query as the only one in a run.
+        for i in range(0, len(trace), 7):
+            (r1,) = srv.run([trace[i]])
+            assert r1.digest == rm[i].digest and r1.batch == 1, i
+
+
+def test_server_timed_arrivals_and_latency_fields(ua, sv, pool):
+    topos, _ = pool
+    ns = [t.n_nodes() for t in topos]
+    trace = sv.serving_trace(ns, 60, qps=400, seed=3)
+    with sv.Server(slots=4) as srv:
+        for t in topos:
+            srv.add_graph(t)
+        res = srv.run(trace, timed_arrivals=True)
+    for q, r in zip(trace, res):
+        assert r.status == 0
+        assert abs(r.arrival_s - q.arrival_s) < 1e-6
+        assert r.arrival_s <= r.start_s + 1e-6 <= r.done_s + 2e-6
+    summ = sv.latency_summary(res)
+    assert summ["failed"] == 0 and summ["p99_ms"] >= summ["p50_ms"] > 0
+
+
+def test_server_admission_by_workspace_budget(ua, sv, pool):
+    topos, _ = pool
+    big = 5  # R-MAT scale 13
+    # 4 MiB holds a k-hop/path pass on the scale-10 graph (about 2.1 MiB) but
+    # not a 16-lane PPR pass on the scale-13 graph (about 7 MiB).
+    with sv.Server(slots=4, workspace_bytes=4 << 20) as srv:
+        for t in topos:
+            srv.add_graph(t)
+        qs = [sv.ppr_query(big, [i]) for i in range(16)] + [sv.path_query(4, 0, 5)]
+        res = srv.run(qs)
+        assert all(r.status == ua._lib.ENOMEM for r in res[:16])
+        assert res[16].status == 0
+    # One pass at a time fits: unbatched passes queue for memory and all finish.
+    with sv.Server(slots=4, max_batch=1, workspace_bytes=4 << 20) as srv:
+        for t in topos:
+            srv.add_graph(t)
+        qs = [sv.khop_query(4, [i]) for i in range(12)]
+        res = srv.run(qs)
+        assert all(r.status == 0 and r.batch == 1 for r in res)
+        spans = sorted((r.start_s, r.done_s) for r in res)
+        assert all(b[0] >= a[1] - 1e-6 for a, b in zip(spans, spans[1:]))
+
+
+def test_server_rejects_bad_queries(ua, sv, pool):
+    topos, _ = pool
+    with sv.Server(slots=2) as srv:
+        srv.add_graph(topos[0])
+        bad = sv.khop_query(0, [1])
+        bad.direction = 7
+        res = srv.run([sv.ppr_query(3, [1]), bad, sv.triangles_query(0)])
+        assert res[0].status == ua._lib.EINVAL and res[1].status == ua._lib.EINVAL and res[2].status == 0

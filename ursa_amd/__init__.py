@@ -30,6 +30,8 @@ __all__ = [
     "pagerank",
     "pagerank_weighted",
     "pagerank_mixed",
+    "ppr",
+    "ppr_batch",
     "bfs_distances",
     "bfs_multi",
     "k_hop",
@@ -177,6 +179,35 @@ def pagerank_mixed(topo: Topology, params: PageRankParams = PageRankParams(), we
     check(load().ursa_hip_pagerank_mixed(topo._h, _ptr(w) if w is not None else None, params.damping,
                                          params.max_iter, params.tol, _ptr(out), C.byref(it), C.byref(it32)))
     return out[:n], it.value, it32.value
+
+
+def ppr(topo: Topology, seeds, damping=0.85, max_iter=100, tol=1e-8):
+    """Personalized PageRank (networkx personalization semantics; f32 compensated
+    sweeps refined in f64) -> (f64[n], iterations)."""
+    n = topo.n_nodes()
+    s = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint32))
+    out = np.zeros(max(n, 1), np.float64)
+    it = C.c_uint32(0)
+    check(load().ursa_hip_ppr(topo._h, _ptr(s) if s.size else None, s.shape[0], damping, max_iter, tol,
+                              _ptr(out), C.byref(it)))
+    return out[:n], it.value
+
+
+def ppr_batch(topo: Topology, seed_lists, damping=0.85, max_iter=100, tol=1e-8):
+    """Several personalized PageRank queries on one graph solved together (16 per
+    device pass) -> (f64[k, n], iterations u32[k]); row q equals ppr(seed_lists[q])."""
+    n = topo.n_nodes()
+    k = len(seed_lists)
+    offs = np.zeros(k + 1, np.uint64)
+    for i, sl in enumerate(seed_lists):
+        offs[i + 1] = offs[i] + len(sl)
+    flat = np.ascontiguousarray(np.concatenate([np.asarray(x, np.uint32) for x in seed_lists])
+                                if k else np.zeros(0, np.uint32))
+    out = np.zeros((k, max(n, 1)), np.float64)
+    iters = np.zeros(max(k, 1), np.uint32)
+    check(load().ursa_hip_ppr_batch(topo._h, k, _ptr(offs), _ptr(flat) if flat.size else None, damping, max_iter,
+                                    tol, _ptr(out), _ptr(iters)))
+    return out[:, :n], iters[:k]
 
 
 def pagerank(topo: Topology, params: PageRankParams = PageRankParams()) -> np.ndarray:

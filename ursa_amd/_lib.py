@@ -1,5 +1,5 @@
-"""ctypes binding of ``libursa_hip.so`` (the C ABI declared in ``include/ursa_hip.h``
-and ``include/ursa_hip_ext.h``).
+"""ctypes binding of ``libursa_hip.so`` (the C ABI declared in ``include/ursa_hip.h``,
+``include/ursa_hip_ext.h`` and ``include/ursa_hip_serve.h``).
 
 The library is built in-tree by ``ursa_amd/build.py`` (hipcc, gfx950). Importing this
 module fails loudly when it is missing: there is no CPU fallback.
@@ -76,6 +76,15 @@ SIGNATURES = {
     "ursa_hip_edges_topology": (cint, [P, C.POINTER(P)]),
     "ursa_hip_edges_destroy": (cint, [P]),
     "ursa_hip_gen_weights": (cint, [u64, u64, P]),
+    "ursa_hip_ppr": (cint, [P, P, u64, f64, u32, f64, P, C.POINTER(u32)]),
+    "ursa_hip_ppr_batch": (cint, [P, u32, P, P, f64, u32, f64, P, P]),
+    "ursa_hip_server_create": (cint, [P, C.POINTER(P)]),
+    "ursa_hip_server_add_graph": (cint, [P, P, C.POINTER(u32)]),
+    "ursa_hip_server_graph_bytes": (cint, [P, C.POINTER(u64)]),
+    "ursa_hip_server_run": (cint, [P, P, u64, cint, P]),
+    "ursa_hip_server_result_data": (cint, [P, u64, P, u64, C.POINTER(u64)]),
+    "ursa_hip_server_destroy": (cint, [P]),
+    "ursa_hip_digest": (u64, [P, u64]),
 }
 
 _lib = None

@@ -68,6 +68,61 @@ hipStream_t stream();
 void* dev_alloc(size_t bytes);
 void dev_free(void* p);
 
+// Workspace arena. While a ScopedArena is active on a thread, every dev_alloc
+// on that thread is carved from the arena's range (256-byte aligned bump
+// allocation, no HIP call) and dev_free of such a pointer does nothing; the
+// range is reclaimed as a whole by its owner. The request server uses this to
+// run queries inside the HBM reservation they were admitted with, so that the
+// query path never calls hipMalloc/hipFree (hipFree waits for the device).
+// Exceeding the range throws URSA_HIP_ENOMEM.
+// Frees in reverse allocation order (the order DevBuf destructors run in)
+// give their bytes back at once; other frees wait for the owner's reset.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+    size_t peak = 0;
+    size_t stack[64];
+    int depth = 0;
+    void reset(char* b, size_t c) {
+        base = b;
+        cap = c;
+        used = peak = 0;
+        depth = 0;
+    }
+};
+Arena* arena_current();
+class ScopedArena {
+   public:
+    explicit ScopedArena(Arena* a);
+    ~ScopedArena();
+    ScopedArena(const ScopedArena&) = delete;
+    ScopedArena& operator=(const ScopedArena&) = delete;
+
+   private:
+    Arena* prev_;
+};
+// Suspends the thread's arena: allocations that outlive the current call
+// (topology caches built lazily) must come from hipMalloc.
+class NoArena {
+   public:
+    NoArena();
+    ~NoArena();
+    NoArena(const NoArena&) = delete;
+    NoArena& operator=(const NoArena&) = delete;
+
+   private:
+    Arena* prev_;
+};
+// Registers [base, base+bytes) as arena memory, so dev_free ignores pointers
+// into it on any thread.
+void arena_register(const void* base, size_t bytes);
+void arena_unregister(const void* base);
+
+// Per-thread pinned host scratch of at least `bytes` (kept for the life of
+// the thread; grows by reallocation).
+void* pinned_scratch(size_t bytes);
+
 template <class T>
 class DevBuf {
    public:

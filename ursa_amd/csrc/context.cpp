@@ -24,6 +24,25 @@ struct ThreadStream {
 };
 thread_local ThreadStream t_stream;
 thread_local int t_current_device = -1;
+thread_local Arena* t_arena = nullptr;
+
+struct PinnedScratch {
+    void* p = nullptr;
+    size_t bytes = 0;
+    // Not freed at thread exit, for the same reason as the streams.
+};
+thread_local PinnedScratch t_pinned;
+
+std::mutex g_arena_mu;
+std::vector<std::pair<uintptr_t, uintptr_t>> g_arenas;
+
+bool in_arena(const void* p) {
+    const uintptr_t a = (uintptr_t)p;
+    std::lock_guard<std::mutex> lk(g_arena_mu);
+    for (auto& r : g_arenas)
+        if (a >= r.first && a < r.second) return true;
+    return false;
+}
 
 struct ProfRec {
     std::string name;
@@ -104,6 +123,17 @@ hipStream_t stream() {
 
 void* dev_alloc(size_t bytes) {
     ensure_device();
+    if (Arena* a = t_arena) {
+        const size_t at = (a->used + 255) & ~(size_t)255;
+        if (at + bytes > a->cap)
+            throw Error(URSA_HIP_ENOMEM, "workspace of " + std::to_string(bytes) +
+                                             " bytes exceeds the query's admitted reservation (" +
+                                             std::to_string(a->cap) + " bytes, " + std::to_string(at) + " used)");
+        if (a->depth < 64) a->stack[a->depth++] = a->used;
+        a->used = at + bytes;
+        if (a->used > a->peak) a->peak = a->used;
+        return a->base + at;
+    }
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
@@ -115,7 +145,56 @@ void* dev_alloc(size_t bytes) {
 }
 
 void dev_free(void* p) {
-    if (p) (void)hipFree(p);
+    if (!p) return;
+    if (Arena* a = t_arena) {
+        char* c = static_cast<char*>(p);
+        if (c >= a->base && c < a->base + a->cap) {
+            if (a->depth > 0) {
+                const size_t prev = a->stack[a->depth - 1];
+                if (c == a->base + ((prev + 255) & ~(size_t)255)) {
+                    a->used = prev;
+                    --a->depth;
+                }
+            }
+            return;
+        }
+    }
+    if (in_arena(p)) return;
+    (void)hipFree(p);
+}
+
+Arena* arena_current() { return t_arena; }
+ScopedArena::ScopedArena(Arena* a) : prev_(t_arena) { t_arena = a; }
+ScopedArena::~ScopedArena() { t_arena = prev_; }
+NoArena::NoArena() : prev_(t_arena) { t_arena = nullptr; }
+NoArena::~NoArena() { t_arena = prev_; }
+
+void arena_register(const void* base, size_t bytes) {
+    std::lock_guard<std::mutex> lk(g_arena_mu);
+    g_arenas.emplace_back((uintptr_t)base, (uintptr_t)base + bytes);
+}
+
+void arena_unregister(const void* base) {
+    std::lock_guard<std::mutex> lk(g_arena_mu);
+    for (size_t i = 0; i < g_arenas.size(); ++i)
+        if (g_arenas[i].first == (uintptr_t)base) {
+            g_arenas.erase(g_arenas.begin() + i);
+            return;
+        }
+}
+
+void* pinned_scratch(size_t bytes) {
+    ensure_device();
+    if (t_pinned.bytes < bytes) {
+        if (t_pinned.p) (void)hipHostFree(t_pinned.p);
+        t_pinned.p = nullptr;
+        t_pinned.bytes = 0;
+        size_t want = 4096;
+        while (want < bytes) want <<= 1;
+        HIP_OK(hipHostMalloc(&t_pinned.p, want, hipHostMallocDefault));
+        t_pinned.bytes = want;
+    }
+    return t_pinned.p;
 }
 
 bool is_device_ptr(const void* p) {

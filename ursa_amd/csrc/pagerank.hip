@@ -463,6 +463,7 @@ __global__ void k_fill(double* p, uint64_t n, double v) {
 // contribution vector and stay in L2, instead of being spread over it.
 // Also caches the merge-path tile partition of that CSR.
 static void pr_prepare(ursa_hip_topo* t, const DevCsr& in, hipStream_t s) {
+    NoArena keep_out_of_arena;
     std::lock_guard<std::mutex> lk(t->cache_mu);
     if (t->pr_ready) return;
     const uint64_t n = t->n, m = t->m;

@@ -1,0 +1,587 @@
+// ppr.hip — personalized PageRank, up to 16 queries on one graph per solve.
+//
+// ursa-core has no personalized PageRank; the definition is the global
+// iteration of pagerank.rs:63-124 with the teleport and dangling mass sent to
+// the query's seed set (networkx `pagerank(personalization=...)` semantics):
+//   x0[u] = 1/n
+//   next[u] = d * sum_{v in in(u)} x[v] / od[v] + ((1 - d) + d * dm) * p[u]
+//   dm = sum of x over dangling vertices, p = uniform over the distinct seeds,
+// stopping after the first iteration whose L1 delta is below tol. A query
+// with no seed in range returns x0 after zero iterations.
+//
+// Layout: the queries of one solve are lanes q of a vertex-major state, value
+// of vertex v for lane q at v*KP + q (KP = lane count rounded up to a power of
+// two), so one in-edge gathers KP adjacent values for all queries at once.
+//
+// Precision: a lane first runs f32 sweeps with compensated (Neumaier) row
+// sums until its delta is below max(tol, 2e-6) or stops falling, then f64
+// sweeps refine the same iterate until delta < tol. The first f64 sweep reads
+// the f32 iterate widened. Lanes switch and finish independently.
+//
+// Determinism: a lane's arithmetic does not depend on the other lanes. Each
+// row is summed by one thread in in-CSR order; a row with more than PPR_HEAVY
+// in-edges is cut into segments fixed by its degree alone, whose partial sums
+// are added in segment order; the L1 delta and dangling mass are reduced over
+// fixed 256-row blocks with a fixed shuffle tree and then in block order. A
+// query gives the same bits alone, in any batch and in any lane.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "ppr.hpp"
+#include "topology.hpp"
+
+namespace ursa {
+
+namespace {
+
+constexpr int PB = 256;
+constexpr uint32_t PPR_HEAVY = 256;
+constexpr uint32_t PPR_MAXSEG = 1024;
+
+struct PprLane {
+    double d, tol, inv_s, pb, prev, delta;
+    uint32_t max_iter, iters, f32_iters;
+    int32_t phase;    // 0 f32 sweeps, 1 first f64 sweep, 2 f64 sweeps, 3 done
+    int32_t final64;  // result is the f64 iterate (else the f32 one)
+    int32_t pad;
+};
+
+struct PprCtl {
+    uint32_t active;
+};
+
+__device__ __forceinline__ void neu_add(float& s, float& c, float x) {
+    const float t = s + x;
+    if (fabsf(s) >= fabsf(x))
+        c += (s - t) + x;
+    else
+        c += (x - t) + s;
+    s = t;
+}
+
+template <int KP>
+__device__ __forceinline__ void load_phases(const PprLane* __restrict__ lanes, int* ph) {
+#pragma unroll
+    for (int q = 0; q < KP; ++q) ph[q] = lanes[q].phase;
+}
+
+// Row sum over in-slots [b, e) for every running lane.
+template <int KP>
+__device__ __forceinline__ void ppr_row_sum(uint32_t b, uint32_t e, const uint32_t* __restrict__ tgt,
+                                            const int* ph, const float* __restrict__ c32,
+                                            const double* __restrict__ c64, float* s, float* c, double* a) {
+#pragma unroll
+    for (int q = 0; q < KP; ++q) {
+        s[q] = 0.f;
+        c[q] = 0.f;
+        a[q] = 0.0;
+    }
+#pragma unroll 4
+    for (uint32_t k = b; k < e; ++k) {
+        const uint64_t v = tgt[k];
+#pragma unroll
+        for (int q = 0; q < KP; ++q) {
+            if (ph[q] == 0)
+                neu_add(s[q], c[q], c32[v * KP + q]);
+            else if (ph[q] == 1)
+                a[q] += (double)c32[v * KP + q];
+            else if (ph[q] == 2)
+                a[q] += c64[v * KP + q];
+        }
+    }
+}
+
+// New value of row u for every running lane; books |next - cur| and, for a
+// dangling row, next into the dangling mass.
+template <int KP>
+__device__ __forceinline__ void ppr_epilogue(uint64_t u, uint32_t seedb, uint32_t odu,
+                                             const PprLane* __restrict__ lanes, const int* ph, const float* s,
+                                             const float* c, const double* a, float* __restrict__ r32,
+                                             float* __restrict__ c32n, double* __restrict__ r64,
+                                             double* __restrict__ c64n, double* dl, double* dm) {
+#pragma unroll
+    for (int q = 0; q < KP; ++q) {
+        if (ph[q] == 3) continue;
+        const bool seed = (seedb >> q) & 1u;
+        const uint64_t i = u * KP + q;
+        if (ph[q] == 0) {
+            const float acc = s[q] + c[q];
+            const float tele = seed ? (float)(lanes[q].pb * lanes[q].inv_s) : 0.f;
+            const float nx = fmaf((float)lanes[q].d, acc, tele);
+            const float cur = r32[i];
+            r32[i] = nx;
+            dl[q] = fabs((double)nx - (double)cur);
+            if (odu)
+                c32n[i] = nx / (float)odu;
+            else
+                dm[q] = (double)nx;
+        } else {
+            const double tele = seed ? lanes[q].pb * lanes[q].inv_s : 0.0;
+            const double nx = fma(lanes[q].d, a[q], tele);
+            const double cur = ph[q] == 1 ? (double)r32[i] : r64[i];
+            r64[i] = nx;
+            dl[q] = fabs(nx - cur);
+            if (odu)
+                c64n[i] = nx / (double)odu;
+            else
+                dm[q] = nx;
+        }
+    }
+}
+
+// Fixed-order block reduction of (delta, dangling) per lane into out[KP*2].
+template <int KP>
+__device__ __forceinline__ void ppr_block_reduce(double* dl, double* dm, double* __restrict__ out) {
+    __shared__ double sh[PB / 64][KP * 2];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < KP; ++q) {
+        double x = dl[q], y = dm[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            x += __shfl_down(x, off, 64);
+            y += __shfl_down(y, off, 64);
+        }
+        if (lane == 0) {
+            sh[w][2 * q] = x;
+            sh[w][2 * q + 1] = y;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < KP * 2) {
+        double acc = sh[0][threadIdx.x];
+#pragma unroll
+        for (int j = 1; j < PB / 64; ++j) acc += sh[j][threadIdx.x];
+        out[threadIdx.x] = acc;
+    }
+}
+
+// One sweep over the light rows (blocks [0, light_blocks)) and the heavy
+// segments (blocks after them).
+template <int KP>
+__global__ __launch_bounds__(PB) void k_ppr_rows(uint64_t n, const uint32_t* __restrict__ off,
+                                                 const uint32_t* __restrict__ tgt, const uint32_t* __restrict__ od,
+                                                 const uint32_t* __restrict__ seedbits,
+                                                 const PprLane* __restrict__ lanes, const float* __restrict__ c32,
+                                                 float* __restrict__ c32n, float* __restrict__ r32,
+                                                 const double* __restrict__ c64, double* __restrict__ c64n,
+                                                 double* __restrict__ r64, double* __restrict__ part,
+                                                 uint32_t light_blocks, uint64_t nseg,
+                                                 const uint32_t* __restrict__ sbeg,
+                                                 const uint32_t* __restrict__ send, double* __restrict__ spart) {
+    int ph[KP];
+    load_phases<KP>(lanes, ph);
+    float s[KP], c[KP];
+    double a[KP];
+    if (blockIdx.x >= light_blocks) {
+        const uint64_t sg = (uint64_t)(blockIdx.x - light_blocks) * PB + threadIdx.x;
+        if (sg >= nseg) return;
+        ppr_row_sum<KP>(sbeg[sg], send[sg], tgt, ph, c32, c64, s, c, a);
+#pragma unroll
+        for (int q = 0; q < KP; ++q)
+            if (ph[q] != 3) spart[sg * KP + q] = ph[q] == 0 ? (double)s[q] + (double)c[q] : a[q];
+        return;
+    }
+    double dl[KP], dm[KP];
+#pragma unroll
+    for (int q = 0; q < KP; ++q) dl[q] = dm[q] = 0.0;
+    const uint64_t u = (uint64_t)blockIdx.x * PB + threadIdx.x;
+    if (u < n) {
+        const uint32_t b = off[u], e = off[u + 1];
+        if (e - b <= PPR_HEAVY) {
+            ppr_row_sum<KP>(b, e, tgt, ph, c32, c64, s, c, a);
+            ppr_epilogue<KP>(u, seedbits[u], od[u], lanes, ph, s, c, a, r32, c32n, r64, c64n, dl, dm);
+        }
+    }
+    ppr_block_reduce<KP>(dl, dm, part + (uint64_t)blockIdx.x * KP * 2);
+}
+
+// Heavy rows: add the row's segment partials in order, then the epilogue.
+template <int KP>
+__global__ __launch_bounds__(PB) void k_ppr_heavy(uint64_t H, const uint32_t* __restrict__ hrow,
+                                                  const uint32_t* __restrict__ hsoff,
+                                                  const double* __restrict__ spart, const uint32_t* __restrict__ od,
+                                                  const uint32_t* __restrict__ seedbits,
+                                                  const PprLane* __restrict__ lanes, float* __restrict__ c32n,
+                                                  float* __restrict__ r32, double* __restrict__ c64n,
+                                                  double* __restrict__ r64, double* __restrict__ part) {
+    int ph[KP];
+    load_phases<KP>(lanes, ph);
+    double dl[KP], dm[KP];
+#pragma unroll
+    for (int q = 0; q < KP; ++q) dl[q] = dm[q] = 0.0;
+    const uint64_t h = (uint64_t)blockIdx.x * PB + threadIdx.x;
+    if (h < H) {
+        const uint64_t u = hrow[h];
+        const uint32_t sb = hsoff[h], se = hsoff[h + 1];
+        float s[KP], c[KP];
+        double a[KP];
+#pragma unroll
+        for (int q = 0; q < KP; ++q) {
+            double acc = 0.0;
+            if (ph[q] != 3)
+                for (uint32_t j = sb; j < se; ++j) acc += spart[(uint64_t)j * KP + q];
+            a[q] = acc;
+            s[q] = (float)acc;
+            c[q] = 0.f;
+        }
+        ppr_epilogue<KP>(u, seedbits[u], od[u], lanes, ph, s, c, a, r32, c32n, r64, c64n, dl, dm);
+    }
+    ppr_block_reduce<KP>(dl, dm, part + (uint64_t)blockIdx.x * KP * 2);
+}
+
+// Sums the block partials per lane in a fixed order and advances each lane.
+template <int KP>
+__global__ __launch_bounds__(PB) void k_ppr_reduce(const double* __restrict__ part, uint32_t nparts,
+                                                   PprLane* __restrict__ lanes, PprCtl* __restrict__ ctl) {
+    __shared__ double sh[PB];
+    __shared__ double res[KP * 2];
+    for (int q = 0; q < KP; ++q) {
+        if (lanes[q].phase == 3) continue;
+        for (int w = 0; w < 2; ++w) {
+            double acc = 0.0;
+            for (uint32_t b = threadIdx.x; b < nparts; b += PB) acc += part[(uint64_t)b * KP * 2 + q * 2 + w];
+            sh[threadIdx.x] = acc;
+            __syncthreads();
+            for (int st = PB / 2; st > 0; st >>= 1) {
+                if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) res[q * 2 + w] = sh[0];
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x != 0) return;
+    uint32_t active = 0;
+    for (int q = 0; q < KP; ++q) {
+        PprLane L = lanes[q];
+        if (L.phase == 3) continue;
+        const double delta = res[q * 2], dmass = res[q * 2 + 1];
+        L.iters += 1;
+        L.delta = delta;
+        L.pb = (1.0 - L.d) + L.d * dmass;
+        if (L.phase == 0) {
+            L.f32_iters += 1;
+            const bool stalled = L.f32_iters >= 3 && delta > 0.7 * L.prev;
+            L.prev = delta;
+            if (L.iters >= L.max_iter) {
+                L.phase = 3;
+                L.final64 = 0;
+            } else if (delta < fmax(L.tol, 2e-6) || stalled) {
+                L.phase = 1;
+            }
+        } else {
+            if (delta < L.tol || L.iters >= L.max_iter) {
+                L.phase = 3;
+                L.final64 = 1;
+            } else {
+                L.phase = 2;
+            }
+        }
+        lanes[q] = L;
+        if (L.phase != 3) ++active;
+    }
+    ctl->active = active;
+}
+
+template <int KP>
+__global__ void k_ppr_init(uint64_t n, const uint32_t* __restrict__ od, float* __restrict__ r32,
+                           float* __restrict__ c32, double* __restrict__ r64) {
+    const float x32 = (float)(1.0 / (double)n);
+    const double x64 = 1.0 / (double)n;
+    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n; u += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t o = od[u];
+#pragma unroll
+        for (int q = 0; q < KP; ++q) {
+            r32[u * KP + q] = x32;
+            r64[u * KP + q] = x64;
+            c32[u * KP + q] = o ? x32 / (float)o : 0.f;
+        }
+    }
+}
+
+__global__ void k_ppr_seedbits(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ lane_of,
+                               uint32_t count, uint32_t* __restrict__ bits) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) atomicOr(&bits[seeds[i]], 1u << lane_of[i]);
+}
+
+template <int KP>
+__global__ void k_ppr_out(uint64_t n, int q, const PprLane* __restrict__ lanes, const float* __restrict__ r32,
+                          const double* __restrict__ r64, double* __restrict__ out) {
+    const bool f64 = lanes[q].final64 != 0;
+    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n; u += (uint64_t)gridDim.x * blockDim.x)
+        out[u] = f64 ? r64[u * KP + q] : (double)r32[u * KP + q];
+}
+
+int lanes_padded(int k) {
+    int kp = 1;
+    while (kp < k) kp <<= 1;
+    return kp;
+}
+
+struct PprBufs {
+    DevBuf<float> r32, c32a, c32b;
+    DevBuf<double> r64, c64a, c64b, part, spart;
+    DevBuf<uint32_t> seedbits, seeds, lane_of;
+    DevBuf<PprLane> lanes;
+    DevBuf<PprCtl> ctl;
+};
+
+template <int KP>
+void ppr_iterate(ursa_hip_topo* t, const DevCsr& in, PprBufs& B, uint32_t max_iter_all, hipStream_t s) {
+    const uint64_t n = t->n;
+    const uint32_t light_blocks = (uint32_t)((n + PB - 1) / PB);
+    const uint32_t seg_blocks = (uint32_t)((t->ppr_S + PB - 1) / PB);
+    const uint32_t heavy_blocks = (uint32_t)((t->ppr_H + PB - 1) / PB);
+    const uint32_t nparts = light_blocks + heavy_blocks;
+    const double edge_bytes = 4.0 + 4.0 * KP;
+    uint32_t it = 0;
+    while (it < max_iter_all) {
+        const uint32_t end = std::min<uint32_t>(max_iter_all, it + 8);
+        for (; it < end; ++it) {
+            const int p = (int)(it & 1);
+            const float* c32 = p ? B.c32b.get() : B.c32a.get();
+            float* c32n = p ? B.c32a.get() : B.c32b.get();
+            const double* c64 = p ? B.c64b.get() : B.c64a.get();
+            double* c64n = p ? B.c64a.get() : B.c64b.get();
+            timed("ppr_rows", edge_bytes * (double)t->m + (16.0 + 8.0 * KP) * (double)n, s, [&] {
+                k_ppr_rows<KP><<<light_blocks + seg_blocks, PB, 0, s>>>(
+                    n, in.off.get(), in.tgt.get(), t->out_deg.get(), B.seedbits.get(), B.lanes.get(), c32, c32n,
+                    B.r32.get(), c64, c64n, B.r64.get(), B.part.get(), light_blocks, t->ppr_S, t->ppr_sbeg.get(),
+                    t->ppr_send.get(), B.spart.get());
+            });
+            check_launch();
+            if (t->ppr_H) {
+                k_ppr_heavy<KP><<<heavy_blocks, PB, 0, s>>>(t->ppr_H, t->ppr_hrow.get(), t->ppr_hsoff.get(),
+                                                            B.spart.get(), t->out_deg.get(), B.seedbits.get(),
+                                                            B.lanes.get(), c32n, B.r32.get(), c64n, B.r64.get(),
+                                                            B.part.get() + (uint64_t)light_blocks * KP * 2);
+                check_launch();
+            }
+            k_ppr_reduce<KP><<<1, PB, 0, s>>>(B.part.get(), nparts, B.lanes.get(), B.ctl.get());
+            check_launch();
+        }
+        PprCtl* h = static_cast<PprCtl*>(pinned_scratch(sizeof(PprCtl)));
+        HIP_OK(hipMemcpyAsync(h, B.ctl.get(), sizeof(PprCtl), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        if (h->active == 0) break;
+    }
+}
+
+template <int KP>
+void ppr_write_out(uint64_t n, PprBufs& B, int k, double* const* dst, hipStream_t s) {
+    for (int q = 0; q < k; ++q) {
+        k_ppr_out<KP><<<blocks_for(n, 256, 4096), 256, 0, s>>>(n, q, B.lanes.get(), B.r32.get(), B.r64.get(), dst[q]);
+        check_launch();
+    }
+}
+
+}  // namespace
+
+void ppr_prepare(ursa_hip_topo* t, hipStream_t s) {
+    NoArena keep_out_of_arena;
+    const DevCsr& in = t->incoming(s);
+    std::lock_guard<std::mutex> lk(t->cache_mu);
+    if (t->ppr_ready) return;
+    const uint64_t n = t->n;
+    std::vector<uint32_t> off(n + 1), od(n);
+    HIP_OK(hipMemcpyAsync(off.data(), in.off.get(), (n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    if (n) HIP_OK(hipMemcpyAsync(od.data(), t->out_deg.get(), n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    std::vector<uint32_t> hrow, hsoff{0}, sbeg, send;
+    uint64_t dangling = 0;
+    for (uint64_t u = 0; u < n; ++u) {
+        if (od[u] == 0) ++dangling;
+        const uint32_t b = off[u], e = off[u + 1], d = e - b;
+        if (d <= PPR_HEAVY) continue;
+        const uint32_t len = std::max<uint32_t>(PPR_HEAVY, (d + PPR_MAXSEG - 1) / PPR_MAXSEG);
+        for (uint32_t x = b; x < e; x += len) {
+            sbeg.push_back(x);
+            send.push_back(std::min<uint32_t>(e, x + len));
+        }
+        hrow.push_back((uint32_t)u);
+        hsoff.push_back((uint32_t)sbeg.size());
+    }
+    auto up = [&](DevBuf<uint32_t>& d, const std::vector<uint32_t>& h) {
+        d.alloc(h.size());
+        if (!h.empty())
+            HIP_OK(hipMemcpyAsync(d.get(), h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    };
+    up(t->ppr_hrow, hrow);
+    up(t->ppr_hsoff, hsoff);
+    up(t->ppr_sbeg, sbeg);
+    up(t->ppr_send, send);
+    HIP_OK(hipStreamSynchronize(s));
+    t->ppr_H = hrow.size();
+    t->ppr_S = sbeg.size();
+    t->ppr_dangling = dangling;
+    t->ppr_ready = true;
+}
+
+uint64_t ppr_workspace_bytes(const ursa_hip_topo* t, int k, bool host_out) {
+    const uint64_t n = t->n, kp = (uint64_t)lanes_padded(std::max(k, 1));
+    const uint64_t light_blocks = (n + PB - 1) / PB, heavy_blocks = (t->ppr_H + PB - 1) / PB;
+    uint64_t b = n * kp * (4 + 8 + 8 + 16 + 16)  // r32, c32 x2, r64, c64 x2
+                 + n * 4                         // seedbits
+                 + (light_blocks + heavy_blocks + 1) * kp * 16 + (t->ppr_S + 1) * kp * 8 + 16 * 64 * 8;
+    if (host_out) b += (uint64_t)k * n * 8;
+    return b + (b >> 4) + (64u << 10) * 16;
+}
+
+void ppr_solve(ursa_hip_topo* t, const PprQuery* qs, int k, double* const* outs, uint32_t* iters_out) {
+    URSA_REQUIRE(k >= 1 && k <= PPR_MAX_LANES, URSA_HIP_EINVAL, "a PPR solve takes 1..16 queries");
+    const uint64_t n = t->n;
+    for (int q = 0; q < k; ++q) {
+        if (iters_out) iters_out[q] = 0;
+        URSA_REQUIRE(qs[q].nseeds == 0 || qs[q].seeds != nullptr, URSA_HIP_EINVAL, "seeds is NULL");
+        URSA_REQUIRE(qs[q].damping >= 0.0 && qs[q].damping <= 1.0, URSA_HIP_EINVAL, "damping must be in [0, 1]");
+        URSA_REQUIRE(n == 0 || outs[q] != nullptr, URSA_HIP_EINVAL, "out is NULL");
+    }
+    if (n == 0) return;
+    hipStream_t s = stream();
+    ppr_prepare(t, s);
+    const DevCsr& in = t->incoming(s);
+    const int KP = lanes_padded(k);
+
+    // Host-side lane set-up: distinct in-range seeds per lane.
+    std::vector<PprLane> lanes(KP);
+    std::vector<uint32_t> seeds, lane_of;
+    uint32_t max_iter_all = 0;
+    for (int q = 0; q < KP; ++q) {
+        PprLane& L = lanes[q];
+        L = PprLane{};
+        L.phase = 3;
+        L.final64 = 1;
+        if (q >= k) continue;
+        std::vector<uint32_t> u;
+        for (uint32_t i = 0; i < qs[q].nseeds; ++i)
+            if (qs[q].seeds[i] < n) u.push_back(qs[q].seeds[i]);
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        L.d = qs[q].damping;
+        L.tol = qs[q].tol;
+        L.max_iter = qs[q].max_iter;
+        if (u.empty() || qs[q].max_iter == 0) continue;
+        L.inv_s = 1.0 / (double)u.size();
+        L.pb = (1.0 - L.d) + L.d * ((double)t->ppr_dangling / (double)n);
+        L.phase = 0;
+        L.final64 = 0;
+        max_iter_all = std::max(max_iter_all, L.max_iter);
+        for (uint32_t v : u) {
+            seeds.push_back(v);
+            lane_of.push_back((uint32_t)q);
+        }
+    }
+
+    PprBufs B;
+    B.r32.alloc(n * KP);
+    B.c32a.alloc(n * KP);
+    B.c32b.alloc(n * KP);
+    B.r64.alloc(n * KP);
+    B.c64a.alloc(n * KP);
+    B.c64b.alloc(n * KP);
+    const uint64_t light_blocks = (n + PB - 1) / PB, heavy_blocks = (t->ppr_H + PB - 1) / PB;
+    B.part.alloc((light_blocks + heavy_blocks) * KP * 2);
+    B.spart.alloc(std::max<uint64_t>(t->ppr_S, 1) * KP);
+    B.seedbits.alloc(n);
+    B.lanes.alloc(KP);
+    B.ctl.alloc(1);
+    HIP_OK(hipMemsetAsync(B.seedbits.get(), 0, n * sizeof(uint32_t), s));
+    HIP_OK(hipMemcpyAsync(B.lanes.get(), lanes.data(), KP * sizeof(PprLane), hipMemcpyHostToDevice, s));
+    if (!seeds.empty()) {
+        B.seeds.alloc(seeds.size());
+        B.lane_of.alloc(seeds.size());
+        HIP_OK(hipMemcpyAsync(B.seeds.get(), seeds.data(), seeds.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(B.lane_of.get(), lane_of.data(), seeds.size() * 4, hipMemcpyHostToDevice, s));
+        k_ppr_seedbits<<<blocks_for(seeds.size(), 256), 256, 0, s>>>(B.seeds.get(), B.lane_of.get(),
+                                                                     (uint32_t)seeds.size(), B.seedbits.get());
+        check_launch();
+    }
+
+    std::vector<DevOut<double>> outs_dev;
+    outs_dev.reserve(k);
+    std::vector<double*> dst(k);
+    for (int q = 0; q < k; ++q) {
+        outs_dev.emplace_back(outs[q], n);
+        dst[q] = outs_dev.back().p;
+    }
+
+    auto run = [&](auto kp_tag) {
+        constexpr int KPc = decltype(kp_tag)::value;
+        k_ppr_init<KPc><<<blocks_for(n, 256, 4096), 256, 0, s>>>(n, t->out_deg.get(), B.r32.get(), B.c32a.get(),
+                                                                 B.r64.get());
+        check_launch();
+        ppr_iterate<KPc>(t, in, B, max_iter_all, s);
+        ppr_write_out<KPc>(n, B, k, dst.data(), s);
+    };
+    switch (KP) {
+        case 1: run(std::integral_constant<int, 1>{}); break;
+        case 2: run(std::integral_constant<int, 2>{}); break;
+        case 4: run(std::integral_constant<int, 4>{}); break;
+        case 8: run(std::integral_constant<int, 8>{}); break;
+        default: run(std::integral_constant<int, 16>{}); break;
+    }
+    HIP_OK(hipMemcpyAsync(lanes.data(), B.lanes.get(), KP * sizeof(PprLane), hipMemcpyDeviceToHost, s));
+    for (auto& o : outs_dev) o.finish(s);
+    HIP_OK(hipStreamSynchronize(s));
+    if (iters_out)
+        for (int q = 0; q < k; ++q) iters_out[q] = lanes[q].iters;
+}
+
+}  // namespace ursa
+
+using namespace ursa;
+
+extern "C" {
+
+int ursa_hip_ppr(ursa_hip_topo* t, const uint32_t* seeds, uint64_t nseeds, double damping, uint32_t max_iter,
+                 double tol, double* out, uint32_t* iters_out) {
+    return guard([&] {
+        URSA_REQUIRE(t != nullptr, URSA_HIP_EINVAL, "topology is NULL");
+        URSA_REQUIRE(nseeds <= 0xFFFFFFFFull, URSA_HIP_EINVAL, "too many seeds");
+        std::vector<uint32_t> hs(nseeds);
+        if (nseeds) {
+            URSA_REQUIRE(seeds != nullptr, URSA_HIP_EINVAL, "seeds is NULL");
+            if (is_device_ptr(seeds))
+                HIP_OK(hipMemcpy(hs.data(), seeds, nseeds * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            else
+                std::copy(seeds, seeds + nseeds, hs.begin());
+        }
+        PprQuery q{hs.data(), (uint32_t)nseeds, damping, max_iter, tol};
+        double* o[1] = {out};
+        ursa::ppr_solve(t, &q, 1, o, iters_out);
+    });
+}
+
+int ursa_hip_ppr_batch(ursa_hip_topo* t, uint32_t k, const uint64_t* seed_offsets, const uint32_t* seeds,
+                       double damping, uint32_t max_iter, double tol, double* out, uint32_t* iters_out) {
+    return guard([&] {
+        URSA_REQUIRE(t != nullptr, URSA_HIP_EINVAL, "topology is NULL");
+        URSA_REQUIRE(k == 0 || seed_offsets != nullptr, URSA_HIP_EINVAL, "seed_offsets is NULL");
+        const uint64_t n = t->n;
+        std::vector<uint32_t> hs;
+        if (k && seeds && is_device_ptr(seeds)) {
+            hs.resize(seed_offsets[k]);
+            if (!hs.empty())
+                HIP_OK(hipMemcpy(hs.data(), seeds, hs.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            seeds = hs.data();
+        }
+        for (uint32_t base = 0; base < k; base += PPR_MAX_LANES) {
+            const int kk = (int)std::min<uint32_t>(PPR_MAX_LANES, k - base);
+            std::vector<PprQuery> qs(kk);
+            std::vector<double*> o(kk);
+            for (int q = 0; q < kk; ++q) {
+                const uint64_t a = seed_offsets[base + q], b = seed_offsets[base + q + 1];
+                URSA_REQUIRE(b >= a && b - a <= 0xFFFFFFFFull, URSA_HIP_EINVAL, "seed_offsets must not decrease");
+                URSA_REQUIRE(b == a || seeds != nullptr, URSA_HIP_EINVAL, "seeds is NULL");
+                qs[q] = PprQuery{seeds ? seeds + a : nullptr, (uint32_t)(b - a), damping, max_iter, tol};
+                o[q] = out ? out + (uint64_t)(base + q) * n : nullptr;
+            }
+            ursa::ppr_solve(t, qs.data(), kk, o.data(), iters_out ? iters_out + base : nullptr);
+        }
+    });
+}
+
+}  // extern "C"

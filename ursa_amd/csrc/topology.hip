@@ -155,6 +155,7 @@ void slot_owners(uint64_t n, const DevCsr& csr, uint32_t* owner, hipStream_t s) 
 using namespace ursa;
 
 const DevCsr& ursa_hip_topo::incoming(hipStream_t s) {
+    NoArena keep_out_of_arena;
     std::lock_guard<std::mutex> lk(build_mu);
     if (in.built) return in;
     if (m == 0) {
@@ -172,6 +173,7 @@ const DevCsr& ursa_hip_topo::incoming(hipStream_t s) {
 }
 
 const DevCsr& ursa_hip_topo::undirected(hipStream_t s) {
+    NoArena keep_out_of_arena;
     std::lock_guard<std::mutex> lk(build_mu);
     if (und.built) return und;
     und.off.alloc(n + 1);

@@ -54,6 +54,11 @@ struct ursa_hip_topo {
     uint64_t tc_slots = 0, tc_nwork = 0;
     double tc_pair_work = 0.0;
     bool tc_built = false;
+    // Personalized PageRank's split of high in-degree rows into fixed
+    // segments (see ppr.hip); depends on the graph only.
+    ursa::DevBuf<uint32_t> ppr_hrow, ppr_hsoff, ppr_sbeg, ppr_send;
+    uint64_t ppr_H = 0, ppr_S = 0, ppr_dangling = 0;
+    bool ppr_ready = false;
 
     const ursa::DevCsr& incoming(hipStream_t s);
     const ursa::DevCsr& undirected(hipStream_t s);

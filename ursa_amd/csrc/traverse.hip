@@ -16,6 +16,7 @@
 
 #include "frontier.hpp"
 #include "prim.hpp"
+#include "traverse.hpp"
 
 struct ursa_hip_pairs {
     std::vector<uint32_t> src, dst;
@@ -130,19 +131,15 @@ struct OWork {
     DevBuf<uint32_t> order, cand, sorted, parent;
     DevBuf<uint64_t> ckeys, skeys;
     DevBuf<OCtr> ctr;
-    OCtr* h = nullptr;
     OWork(uint64_t n_, bool with_parent, hipStream_t s)
         : n(n_), level(n_), key(n_), order(n_), cand(n_), sorted(n_), ckeys(n_), skeys(n_), ctr(1) {
         if (with_parent) parent.alloc(n_);
         k_fill_i32b<<<blocks_for(n, 256), 256, 0, s>>>(level.get(), n, -1);
         k_fill_u64<<<blocks_for(n, 256), 256, 0, s>>>(key.get(), n, ~0ull);
         check_launch();
-        HIP_OK(hipHostMalloc((void**)&h, sizeof(OCtr), hipHostMallocDefault));
-    }
-    ~OWork() {
-        if (h) (void)hipHostFree(h);
     }
     OCtr read(hipStream_t s) {
+        OCtr* h = static_cast<OCtr*>(pinned_scratch(sizeof(OCtr)));
         HIP_OK(hipMemcpyAsync(h, ctr.get(), sizeof(OCtr), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
         return *h;
@@ -191,63 +188,117 @@ void reset_touched(OWork& w, uint64_t visited, hipStream_t s) {
     check_launch();
 }
 
+// Walks the parent links from tgt back to src (at most cap nodes), writing the
+// path target-first.
+__global__ void k_opath(const uint32_t* __restrict__ parent, uint32_t src, uint32_t tgt, uint32_t* __restrict__ path,
+                        uint32_t cap, uint32_t* __restrict__ len) {
+    uint32_t cur = tgt, l = 0;
+    path[l++] = cur;
+    while (cur != src && l < cap) {
+        cur = parent[cur];
+        path[l++] = cur;
+    }
+    *len = l;
+}
+
 }  // namespace
 
-void k_hop(ursa_hip_topo* t, const uint32_t* seeds_user, uint64_t nseeds, uint32_t k, int dir, ursa_hip_pairs* res) {
+uint64_t traverse_workspace_bytes(uint64_t n) {
+    // OWork (level, key, order, cand, sorted, parent, ckeys, skeys: 44 B per
+    // node) plus the radix sort's temporary storage for up to n (u64, u32)
+    // pairs (a double buffer of 12 B per pair and its histograms).
+    return n * 80 + (2u << 20);
+}
+
+void k_hop_seq(ursa_hip_topo* t, const uint32_t* seeds, uint64_t nseeds, uint32_t k, int dir,
+               std::vector<uint32_t>& dst, std::vector<uint64_t>& ends) {
     const uint64_t n = t->n;
+    ends.assign(nseeds, dst.size());
     if (k == 0 || n == 0 || nseeds == 0) return;  // hop.rs:53-56
     hipStream_t s = stream();
     NbrView fwd, rev;
     direction_views(t, dir, s, fwd, rev);
-    std::vector<uint32_t> seeds(nseeds);
-    if (is_device_ptr(seeds_user))
-        HIP_OK(hipMemcpy(seeds.data(), seeds_user, nseeds * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    else
-        std::copy(seeds_user, seeds_user + nseeds, seeds.begin());
     OWork w(n, false, s);
-    std::vector<uint32_t> buf;
-    for (uint32_t seed : seeds) {
-        if (seed >= n) continue;  // hop.rs:91-93: unknown seed contributes nothing
+    for (uint64_t i = 0; i < nseeds; ++i) {
+        const uint32_t seed = seeds[i];
+        if (seed < n) {  // hop.rs:91-93: unknown seed contributes nothing
+            uint64_t visited = 0;
+            ordered_bfs(seed, k, 0xFFFFFFFFu, fwd, w, s, &visited);
+            const uint64_t reached = visited - 1;
+            if (reached) {
+                const size_t at = dst.size();
+                dst.resize(at + reached);
+                HIP_OK(hipMemcpyAsync(dst.data() + at, w.order.get() + 1, reached * sizeof(uint32_t),
+                                      hipMemcpyDeviceToHost, s));
+            }
+            reset_touched(w, visited, s);
+            HIP_OK(hipStreamSynchronize(s));
+        }
+        ends[i] = dst.size();
+    }
+}
+
+void k_hop(ursa_hip_topo* t, const uint32_t* seeds_user, uint64_t nseeds, uint32_t k, int dir, ursa_hip_pairs* res) {
+    std::vector<uint32_t> seeds(nseeds);
+    if (nseeds && is_device_ptr(seeds_user))
+        HIP_OK(hipMemcpy(seeds.data(), seeds_user, nseeds * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    else if (nseeds)
+        std::copy(seeds_user, seeds_user + nseeds, seeds.begin());
+    std::vector<uint64_t> ends;
+    k_hop_seq(t, seeds.data(), nseeds, k, dir, res->dst, ends);
+    uint64_t at = 0;
+    res->src.reserve(res->dst.size());
+    for (uint64_t i = 0; i < nseeds; ++i) {
+        res->src.insert(res->src.end(), ends[i] - at, seeds[i]);
+        at = ends[i];
+    }
+}
+
+void shortest_paths_seq(ursa_hip_topo* t, const uint32_t* src, const uint32_t* tgt, uint64_t count, int dir,
+                        std::vector<std::vector<uint32_t>>& paths) {
+    const uint64_t n = t->n;
+    paths.assign(count, {});
+    bool any = false;
+    for (uint64_t i = 0; i < count; ++i) {
+        if (src[i] >= n || tgt[i] >= n) continue;  // bfs.rs:55-57
+        if (src[i] == tgt[i]) {                    // bfs.rs:58-60
+            paths[i].push_back(src[i]);
+            continue;
+        }
+        any = true;
+    }
+    if (!any) return;
+    hipStream_t s = stream();
+    NbrView fwd, rev;
+    direction_views(t, dir, s, fwd, rev);
+    OWork w(n, true, s);
+    DevBuf<uint32_t> dpath(n), dlen(1);
+    for (uint64_t i = 0; i < count; ++i) {
+        if (src[i] >= n || tgt[i] >= n || src[i] == tgt[i]) continue;
         uint64_t visited = 0;
-        ordered_bfs(seed, k, 0xFFFFFFFFu, fwd, w, s, &visited);
-        const uint64_t reached = visited - 1;
-        if (reached) {
-            buf.resize(reached);
-            HIP_OK(hipMemcpyAsync(buf.data(), w.order.get() + 1, reached * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        const bool hit = ordered_bfs(src[i], 0, tgt[i], fwd, w, s, &visited);
+        if (hit) {
+            k_opath<<<1, 1, 0, s>>>(w.parent.get(), src[i], tgt[i], dpath.get(), (uint32_t)std::min<uint64_t>(n, visited),
+                                    dlen.get());
+            check_launch();
+            uint32_t* h = static_cast<uint32_t*>(pinned_scratch(sizeof(uint32_t)));
+            HIP_OK(hipMemcpyAsync(h, dlen.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            HIP_OK(hipStreamSynchronize(s));
+            const uint32_t len = *h;
+            paths[i].resize(len);
+            HIP_OK(hipMemcpyAsync(paths[i].data(), dpath.get(), len * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         }
         reset_touched(w, visited, s);
         HIP_OK(hipStreamSynchronize(s));
-        res->src.insert(res->src.end(), reached, seed);
-        res->dst.insert(res->dst.end(), buf.begin(), buf.begin() + reached);
+        if (hit) std::reverse(paths[i].begin(), paths[i].end());
     }
 }
 
 // Returns the path length (0 = None).
 uint64_t shortest_path(ursa_hip_topo* t, uint32_t src, uint32_t tgt, int dir, std::vector<uint32_t>& path) {
-    const uint64_t n = t->n;
-    path.clear();
-    if (src >= n || tgt >= n) return 0;  // bfs.rs:55-57
-    if (src == tgt) {                    // bfs.rs:58-60
-        path.push_back(src);
-        return 1;
-    }
-    hipStream_t s = stream();
-    NbrView fwd, rev;
-    direction_views(t, dir, s, fwd, rev);
-    OWork w(n, true, s);
-    uint64_t visited = 0;
-    const bool hit = ordered_bfs(src, 0, tgt, fwd, w, s, &visited);
-    if (!hit) return 0;
-    std::vector<uint32_t> parent(n);
-    HIP_OK(hipMemcpyAsync(parent.data(), w.parent.get(), n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
-    uint32_t cur = tgt;
-    path.push_back(cur);
-    while (cur != src) {
-        cur = parent[cur];
-        path.push_back(cur);
-    }
-    std::reverse(path.begin(), path.end());
+    std::vector<std::vector<uint32_t>> paths;
+    shortest_paths_seq(t, &src, &tgt, 1, dir, paths);
+    path.swap(paths[0]);
     return path.size();
 }
 

@@ -209,6 +209,7 @@ __global__ __launch_bounds__(TCB) void k_tc_count(const uint32_t* __restrict__ o
 }  // namespace
 
 static void tc_orient(ursa_hip_topo* t, hipStream_t s) {
+    NoArena keep_out_of_arena;
     std::lock_guard<std::mutex> lk(t->cache_mu);
     if (t->tc_built) return;
     const uint64_t n = t->n;
@@ -260,6 +261,8 @@ static void tc_orient(ursa_hip_topo* t, hipStream_t s) {
     t->tc_pair_work = (double)hp;
     t->tc_built = true;
 }
+
+void triangle_prepare(ursa_hip_topo* t) { tc_orient(t, stream()); }
 
 void triangle_count(ursa_hip_topo* t, uint64_t* out_user) {
     const uint64_t n = t->n;
