@@ -63,7 +63,9 @@ def test_ppr_serving_tolerance_rmat(ua):
         got, _ = ua.ppr(t, seeds, 0.85, 100, 1e-8)
         ref, _ = orc.ppr(o, seeds, 0.85, 100, 1e-8)
         assert np.abs(got - ref).sum() <= 2e-7
-        assert abs(got.sum() - 1.0) < 1e-9
+        # Mass: an iterate off by eps from sum 1 moves by (1-d)*eps per sweep,
+        # so delta < tol bounds the excess by tol/(1-d) ~ 6.7e-8.
+        assert abs(got.sum() - 1.0) < 1e-8 / 0.15
 
 
 def test_ppr_networkx_fixtures(ua):
