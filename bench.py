@@ -402,6 +402,82 @@ def bench_batched(args, ua, gen):
     return res
 
 
+def bench_serving(args, ua):
+    """Serving mix: resident graphs (node counts log-uniform in [1e3, 2e6], edge
+    factor U{4..32}, R-MAT or Erdos-Renyi 50/50, seeds 1000+i) and a query trace of
+    40% PPR / 30% 3-hop / 20% shortest path / 10% triangle counts over uniformly
+    chosen graphs. The plain serial loop (one query at a time through the plain
+    entry points) and the continuous-batching server run the same trace: first all
+    queries at once (throughput), then Poisson arrivals at a fixed rate (latency).
+    Every query's result digest must match between the two."""
+    from ursa_amd import serve
+
+    lib = ua.load()
+    specs = serve.serving_graph_specs(args.serve_graphs)
+    t0 = time.perf_counter()
+    topos, ns, ms = [], [], []
+    for i, sp in enumerate(specs):
+        t = serve.build_serving_graph(sp)
+        topos.append(t)
+        ns.append(t.n_nodes())
+        ms.append(t.n_edges())
+        if (i + 1) % 64 == 0:
+            log(f"serving graphs built: {i + 1}/{len(specs)} ({time.perf_counter() - t0:.1f} s)")
+    ser = serve.Server(serial=True)
+    srv = serve.Server(slots=args.serve_slots, max_batch=16)
+    t1 = time.perf_counter()
+    for t in topos:
+        ser.add_graph(t)
+        srv.add_graph(t)
+    check(lib.ursa_hip_synchronize())
+    log(f"serving pool resident: {len(topos)} graphs, {sum(ns)} nodes, {sum(ms)} edges, "
+        f"{srv.graph_bytes() / 2**30:.1f} GiB, caches {time.perf_counter() - t1:.1f} s")
+    res = {"graphs": len(topos), "nodes": int(sum(ns)), "edges": int(sum(ms)),
+           "resident_GiB": round(srv.graph_bytes() / 2**30, 2),
+           "build_s": round(t1 - t0, 1), "slots": args.serve_slots, "max_batch": 16}
+    warm = serve.serving_trace(ns, 64, qps=0, seed=99)
+    ser.run(warm)
+    srv.run(warm)
+
+    def summary(rs, trace):
+        s = serve.latency_summary(rs)
+        per = {}
+        for k, name in serve.KIND_NAMES.items():
+            lat = [(r.done_s - r.arrival_s) * 1e3 for r, q in zip(rs, trace) if q.kind == k]
+            if lat:
+                per[name] = {"n": len(lat), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                             "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+        out = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s.items()}
+        out["by_kind"] = per
+        out["mean_batch"] = round(float(np.mean([r.batch for r in rs])), 2)
+        return out
+
+    # Throughput: every query waiting at t = 0.
+    sat = serve.serving_trace(ns, args.serve_queries, qps=0, seed=7)
+    rs = ser.run(sat)
+    rm = srv.run(sat)
+    mism = sum(1 for a, b in zip(rs, rm) if a.digest != b.digest or a.status != b.status or a.status != 0)
+    res["saturation"] = {"serial": summary(rs, sat), "served": summary(rm, sat), "digest_mismatches": mism}
+    qs_serial = res["saturation"]["serial"]["qps"]
+    qs_served = res["saturation"]["served"]["qps"]
+    res["throughput_speedup"] = round(qs_served / qs_serial, 2) if qs_serial else None
+    log(f"serving saturation: serial {qs_serial:.1f} q/s, served {qs_served:.1f} q/s, mismatches {mism}")
+    # Latency at a fixed Poisson rate both can sustain (default 0.7 x serial capacity).
+    qps = args.serve_qps or round(0.7 * qs_serial, 1)
+    tr = serve.serving_trace(ns, args.serve_queries, qps=qps, seed=8)
+    ls = ser.run(tr, timed_arrivals=True)
+    lm = srv.run(tr, timed_arrivals=True)
+    mism2 = sum(1 for a, b in zip(ls, lm) if a.digest != b.digest or a.status != b.status or a.status != 0)
+    res["poisson"] = {"qps": qps, "serial": summary(ls, tr), "served": summary(lm, tr), "digest_mismatches": mism2}
+    log(f"serving poisson {qps} q/s: serial p50 {res['poisson']['serial']['p50_ms']} ms p99 "
+        f"{res['poisson']['serial']['p99_ms']} ms; served p50 {res['poisson']['served']['p50_ms']} ms p99 "
+        f"{res['poisson']['served']['p99_ms']} ms")
+    ser.close()
+    srv.close()
+    del topos
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -416,7 +492,12 @@ def main():
     ap.add_argument("--batch-scale", type=int, default=20)
     ap.add_argument("--msbfs-sources", type=int, default=4096)
     ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--only", choices=["c2", "pagerank", "batched"], default=None,
+    ap.add_argument("--serve-graphs", type=int, default=512)
+    ap.add_argument("--serve-queries", type=int, default=1000)
+    ap.add_argument("--serve-qps", type=float, default=0.0, help="Poisson rate (default 0.7 x serial capacity)")
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--only", choices=["c2", "pagerank", "batched", "serving"], default=None,
                     help="run one section only (for profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run the calibration copy kernel over this many MiB")
@@ -445,12 +526,17 @@ def main():
     if args.only == "batched":
         print(json.dumps({"batched": bench_batched(args, ua, gen)}), flush=True)
         return
+    if args.only == "serving":
+        print(json.dumps({"serving": bench_serving(args, ua)}), flush=True)
+        return
     res = bench_c2(args, ua, gen)
     if args.only == "c2":
         args.no_pagerank = True
         args.no_batched = True
+        args.no_serving = True
     pr = None if args.no_pagerank else bench_pagerank(args, ua, gen)
     batched = None if args.no_batched else bench_batched(args, ua, gen)
+    serving = None if args.no_serving else bench_serving(args, ua)
     line = {
         "metric": res["metric"],
         "value": res["value"],
@@ -470,6 +556,7 @@ def main():
         "kernels": res["kernels"],
         "pagerank": pr,
         "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
 

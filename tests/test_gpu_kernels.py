@@ -223,6 +223,32 @@ def test_triangles_dense_clique(ua):
     a, b = np.triu_indices(k, 1)
     t = ua.Topology.build(k, a, b)
     assert np.all(ua.triangle_count(t) == (k - 1) * (k - 2) // 2)
+    # K_1100: oriented lists up to 1099 entries, staged per workgroup
+    k = 1100
+    a, b = np.triu_indices(k, 1)
+    t = ua.Topology.build(k, a, b)
+    assert np.all(ua.triangle_count(t) == (k - 1) * (k - 2) // 2)
+
+
+@pytest.mark.parametrize("side", [1000, 4200])
+def test_triangles_bipartite_with_chords(ua, side):
+    # K_{side,side} plus chords a0-a1, a1-a2, a5-a6 inside side A. Closed form:
+    # a chord's endpoints each gain `side` triangles (one per B vertex), every B
+    # vertex gains one per chord. With equal degrees the higher id ranks higher,
+    # so A vertices orient all `side` B edges outwards: |N+(a)| = side, the
+    # workgroup-staged path at 1000 and the unstaged path at 4200.
+    A = np.repeat(np.arange(side), side)
+    B = side + np.tile(np.arange(side), side)
+    chords = np.array([[0, 1], [1, 2], [5, 6]])
+    src = np.concatenate([A, chords[:, 0]])
+    dst = np.concatenate([B, chords[:, 1]])
+    t = ua.Topology.build(2 * side, src, dst)
+    want = np.zeros(2 * side, np.uint64)
+    for x, y in chords:
+        want[x] += side
+        want[y] += side
+    want[side:] = len(chords)
+    assert np.array_equal(ua.triangle_count(t), want)
 
 
 # ---------------------------------------------------------------- betweenness

@@ -18,12 +18,18 @@
 // sweeps refine the same iterate until delta < tol. The first f64 sweep reads
 // the f32 iterate widened. Lanes switch and finish independently.
 //
-// Determinism: a lane's arithmetic does not depend on the other lanes. Each
-// row is summed by one thread in in-CSR order; a row with more than PPR_HEAVY
-// in-edges is cut into segments fixed by its degree alone, whose partial sums
-// are added in segment order; the L1 delta and dangling mass are reduced over
-// fixed 256-row blocks with a fixed shuffle tree and then in block order. A
-// query gives the same bits alone, in any batch and in any lane.
+// Work split: a group of PPR_G = 8 adjacent threads sums one row, thread g
+// taking in-slots g, g+8, ... (so the group reads the slot list 32 bytes at a
+// time) and the group's eight partial sums are combined in a fixed butterfly.
+// A row with more than PPR_HEAVY in-edges is cut into segments fixed by its
+// degree alone; a group sums each segment and the segment partials are added
+// in segment order. A workgroup sweeps PPR_ROWS rows and reduces the L1 delta
+// and dangling mass of those rows in a fixed tree; the host-side reduction adds
+// the per-workgroup values in workgroup order.
+//
+// Determinism: a lane's arithmetic does not depend on the other lanes (every
+// split above is fixed by the graph), so a query gives the same bits alone, in
+// any batch and in any lane.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -36,6 +42,9 @@ namespace ursa {
 namespace {
 
 constexpr int PB = 256;
+constexpr int PPR_G = 8;                     // threads per row
+constexpr int PPR_GROUPS = PB / PPR_G;       // rows per workgroup pass
+constexpr uint32_t PPR_ROWS = 512;           // rows per workgroup (one partial each)
 constexpr uint32_t PPR_HEAVY = 256;
 constexpr uint32_t PPR_MAXSEG = 1024;
 
@@ -66,9 +75,11 @@ __device__ __forceinline__ void load_phases(const PprLane* __restrict__ lanes, i
     for (int q = 0; q < KP; ++q) ph[q] = lanes[q].phase;
 }
 
-// Row sum over in-slots [b, e) for every running lane.
+// This thread's share (slots b+g, b+g+8, ...) of the row sum over in-slots
+// [b, e), combined across the group's eight threads; the result is valid in
+// the group's thread g = 0.
 template <int KP>
-__device__ __forceinline__ void ppr_row_sum(uint32_t b, uint32_t e, const uint32_t* __restrict__ tgt,
+__device__ __forceinline__ void ppr_row_sum(uint32_t b, uint32_t e, uint32_t g, const uint32_t* __restrict__ tgt,
                                             const int* ph, const float* __restrict__ c32,
                                             const double* __restrict__ c64, float* s, float* c, double* a) {
 #pragma unroll
@@ -77,8 +88,8 @@ __device__ __forceinline__ void ppr_row_sum(uint32_t b, uint32_t e, const uint32
         c[q] = 0.f;
         a[q] = 0.0;
     }
-#pragma unroll 4
-    for (uint32_t k = b; k < e; ++k) {
+#pragma unroll 2
+    for (uint32_t k = b + g; k < e; k += PPR_G) {
         const uint64_t v = tgt[k];
 #pragma unroll
         for (int q = 0; q < KP; ++q) {
@@ -90,10 +101,24 @@ __device__ __forceinline__ void ppr_row_sum(uint32_t b, uint32_t e, const uint32
                 a[q] += c64[v * KP + q];
         }
     }
+    // Butterfly over the group (lanes differing in the low three bits).
+#pragma unroll
+    for (int off = PPR_G / 2; off > 0; off >>= 1) {
+#pragma unroll
+        for (int q = 0; q < KP; ++q) {
+            if (ph[q] == 0) {
+                const float s2 = __shfl_xor(s[q], off, 64), c2 = __shfl_xor(c[q], off, 64);
+                c[q] += c2;
+                neu_add(s[q], c[q], s2);
+            } else if (ph[q] != 3) {
+                a[q] += __shfl_xor(a[q], off, 64);
+            }
+        }
+    }
 }
 
-// New value of row u for every running lane; books |next - cur| and, for a
-// dangling row, next into the dangling mass.
+// New value of row u for every running lane; adds |next - cur| to dl and,
+// for a dangling row, next to dm.
 template <int KP>
 __device__ __forceinline__ void ppr_epilogue(uint64_t u, uint32_t seedb, uint32_t odu,
                                              const PprLane* __restrict__ lanes, const int* ph, const float* s,
@@ -111,21 +136,21 @@ __device__ __forceinline__ void ppr_epilogue(uint64_t u, uint32_t seedb, uint32_
             const float nx = fmaf((float)lanes[q].d, acc, tele);
             const float cur = r32[i];
             r32[i] = nx;
-            dl[q] = fabs((double)nx - (double)cur);
+            dl[q] += fabs((double)nx - (double)cur);
             if (odu)
                 c32n[i] = nx / (float)odu;
             else
-                dm[q] = (double)nx;
+                dm[q] += (double)nx;
         } else {
             const double tele = seed ? lanes[q].pb * lanes[q].inv_s : 0.0;
             const double nx = fma(lanes[q].d, a[q], tele);
             const double cur = ph[q] == 1 ? (double)r32[i] : r64[i];
             r64[i] = nx;
-            dl[q] = fabs(nx - cur);
+            dl[q] += fabs(nx - cur);
             if (odu)
                 c64n[i] = nx / (double)odu;
             else
-                dm[q] = nx;
+                dm[q] += nx;
         }
     }
 }
@@ -157,8 +182,8 @@ __device__ __forceinline__ void ppr_block_reduce(double* dl, double* dm, double*
     }
 }
 
-// One sweep over the light rows (blocks [0, light_blocks)) and the heavy
-// segments (blocks after them).
+// One sweep over the light rows (blocks [0, light_blocks), PPR_ROWS rows
+// each) and the heavy segments (PPR_GROUPS per block after them).
 template <int KP>
 __global__ __launch_bounds__(PB) void k_ppr_rows(uint64_t n, const uint32_t* __restrict__ off,
                                                  const uint32_t* __restrict__ tgt, const uint32_t* __restrict__ od,
@@ -174,25 +199,29 @@ __global__ __launch_bounds__(PB) void k_ppr_rows(uint64_t n, const uint32_t* __r
     load_phases<KP>(lanes, ph);
     float s[KP], c[KP];
     double a[KP];
+    const uint32_t g = threadIdx.x % PPR_G, grp = threadIdx.x / PPR_G;
     if (blockIdx.x >= light_blocks) {
-        const uint64_t sg = (uint64_t)(blockIdx.x - light_blocks) * PB + threadIdx.x;
-        if (sg >= nseg) return;
-        ppr_row_sum<KP>(sbeg[sg], send[sg], tgt, ph, c32, c64, s, c, a);
+        const uint64_t sg = (uint64_t)(blockIdx.x - light_blocks) * PPR_GROUPS + grp;
+        if (sg >= nseg) return;  // whole groups leave together
+        ppr_row_sum<KP>(sbeg[sg], send[sg], g, tgt, ph, c32, c64, s, c, a);
+        if (g == 0) {
 #pragma unroll
-        for (int q = 0; q < KP; ++q)
-            if (ph[q] != 3) spart[sg * KP + q] = ph[q] == 0 ? (double)s[q] + (double)c[q] : a[q];
+            for (int q = 0; q < KP; ++q)
+                if (ph[q] != 3) spart[sg * KP + q] = ph[q] == 0 ? (double)s[q] + (double)c[q] : a[q];
+        }
         return;
     }
     double dl[KP], dm[KP];
 #pragma unroll
     for (int q = 0; q < KP; ++q) dl[q] = dm[q] = 0.0;
-    const uint64_t u = (uint64_t)blockIdx.x * PB + threadIdx.x;
-    if (u < n) {
+    const uint64_t row0 = (uint64_t)blockIdx.x * PPR_ROWS;
+    for (uint32_t r = grp; r < PPR_ROWS; r += PPR_GROUPS) {
+        const uint64_t u = row0 + r;
+        if (u >= n) break;
         const uint32_t b = off[u], e = off[u + 1];
-        if (e - b <= PPR_HEAVY) {
-            ppr_row_sum<KP>(b, e, tgt, ph, c32, c64, s, c, a);
-            ppr_epilogue<KP>(u, seedbits[u], od[u], lanes, ph, s, c, a, r32, c32n, r64, c64n, dl, dm);
-        }
+        if (e - b > PPR_HEAVY) continue;
+        ppr_row_sum<KP>(b, e, g, tgt, ph, c32, c64, s, c, a);
+        if (g == 0) ppr_epilogue<KP>(u, seedbits[u], od[u], lanes, ph, s, c, a, r32, c32n, r64, c64n, dl, dm);
     }
     ppr_block_reduce<KP>(dl, dm, part + (uint64_t)blockIdx.x * KP * 2);
 }
@@ -332,8 +361,8 @@ struct PprBufs {
 template <int KP>
 void ppr_iterate(ursa_hip_topo* t, const DevCsr& in, PprBufs& B, uint32_t max_iter_all, hipStream_t s) {
     const uint64_t n = t->n;
-    const uint32_t light_blocks = (uint32_t)((n + PB - 1) / PB);
-    const uint32_t seg_blocks = (uint32_t)((t->ppr_S + PB - 1) / PB);
+    const uint32_t light_blocks = (uint32_t)((n + PPR_ROWS - 1) / PPR_ROWS);
+    const uint32_t seg_blocks = (uint32_t)((t->ppr_S + PPR_GROUPS - 1) / PPR_GROUPS);
     const uint32_t heavy_blocks = (uint32_t)((t->ppr_H + PB - 1) / PB);
     const uint32_t nparts = light_blocks + heavy_blocks;
     const double edge_bytes = 4.0 + 4.0 * KP;
@@ -422,7 +451,7 @@ void ppr_prepare(ursa_hip_topo* t, hipStream_t s) {
 
 uint64_t ppr_workspace_bytes(const ursa_hip_topo* t, int k, bool host_out) {
     const uint64_t n = t->n, kp = (uint64_t)lanes_padded(std::max(k, 1));
-    const uint64_t light_blocks = (n + PB - 1) / PB, heavy_blocks = (t->ppr_H + PB - 1) / PB;
+    const uint64_t light_blocks = (n + PPR_ROWS - 1) / PPR_ROWS, heavy_blocks = (t->ppr_H + PB - 1) / PB;
     uint64_t b = n * kp * (4 + 8 + 8 + 16 + 16)  // r32, c32 x2, r64, c64 x2
                  + n * 4                         // seedbits
                  + (light_blocks + heavy_blocks + 1) * kp * 16 + (t->ppr_S + 1) * kp * 8 + 16 * 64 * 8;
@@ -482,7 +511,7 @@ void ppr_solve(ursa_hip_topo* t, const PprQuery* qs, int k, double* const* outs,
     B.r64.alloc(n * KP);
     B.c64a.alloc(n * KP);
     B.c64b.alloc(n * KP);
-    const uint64_t light_blocks = (n + PB - 1) / PB, heavy_blocks = (t->ppr_H + PB - 1) / PB;
+    const uint64_t light_blocks = (n + PPR_ROWS - 1) / PPR_ROWS, heavy_blocks = (t->ppr_H + PB - 1) / PB;
     B.part.alloc((light_blocks + heavy_blocks) * KP * 2);
     B.spart.alloc(std::max<uint64_t>(t->ppr_S, 1) * KP);
     B.seedbits.alloc(n);

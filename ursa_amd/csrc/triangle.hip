@@ -7,9 +7,12 @@
 // lower rank, rank(x) = (degree(x), x). Each out-list N+(u) stays sorted by id
 // and has at most sqrt(2m) entries. A triangle u < v < w (in rank) is then
 // found exactly once: while processing u, as w in N+(u) ∩ N+(v) for v in N+(u).
-// Counting: one wavefront per vertex u (four per workgroup); see k_tc_count.
-// The wave flattens its (v, candidate) pairs over 64 lanes with a degree
-// prefix-sum. Integer sums are order-independent, so counts are exact and
+// Counting: one wavefront per work item (four per workgroup); see k_tc_count.
+// A work item is a run of consecutive slots of one N+(u): the oriented slots
+// are cut at every list start and wherever the running intersection cost
+// crosses a multiple of TC_CHUNK, so a hub's list is shared by many waves
+// instead of serialising the tail of the launch on one. The wave flattens its
+// (v, candidate) pairs over 64 lanes with a degree prefix-sum. Integer sums are order-independent, so counts are exact and
 // identical to the node-iterator reference.
 //
 // Algorithmic bytes (the sorted-merge model): 4 * sum over oriented edges (u,v)
@@ -27,6 +30,55 @@ constexpr int TCB = 256;
 constexpr int TCW = TCB / 64;
 constexpr uint32_t STAGE_CAP = 512;  // N+(u) entries staged in LDS per wave
 constexpr uint32_t HASH_CAP = 1024;  // hash slots per wave (>= 2 * STAGE_CAP)
+constexpr uint32_t BIG_CAP = 2048;   // N+(u) entries staged in LDS per workgroup
+constexpr uint64_t TC_CHUNK = 8192;  // intersection steps per work item (about)
+constexpr int TC_ILP = 4;            // 64-step groups whose loads are in flight together
+
+// Intersection cost of oriented slot k = (u, v), as k_tc_count spends it:
+// d+(v) hash probes (mode A) or d+(u) binary searches (mode B).
+__device__ __forceinline__ uint32_t tc_slot_cost(uint32_t du, uint32_t dv) { return dv <= 8 * du ? dv : du; }
+
+__global__ void k_tc_cost(const uint32_t* __restrict__ ooff, const uint32_t* __restrict__ otgt,
+                          const uint32_t* __restrict__ oown, uint64_t kept, uint32_t* __restrict__ cost) {
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < kept; k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t u = oown[k], v = otgt[k];
+        const uint32_t du = ooff[u + 1] - ooff[u], dv = ooff[v + 1] - ooff[v];
+        cost[k] = du >= 2 ? tc_slot_cost(du, dv) : 0u;
+    }
+}
+
+// Item starts: the first slot of every list with at least two entries, and
+// every slot whose exclusive cost prefix enters a new TC_CHUNK window.
+__global__ void k_tc_item_flags(const uint32_t* __restrict__ ooff, const uint32_t* __restrict__ oown,
+                                const uint64_t* __restrict__ pre, uint64_t kept, uint8_t* __restrict__ flag,
+                                uint32_t* __restrict__ iota) {
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < kept; k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t u = oown[k];
+        const uint32_t a = ooff[u], du = ooff[u + 1] - a;
+        bool f = false;
+        const uint64_t win = du > STAGE_CAP ? 4 * TC_CHUNK : TC_CHUNK;  // amortise staging a long list
+        if (du >= 2) f = (k == a) || (pre[k] / win != pre[k - 1] / win);
+        flag[k] = f;
+        iota[k] = (uint32_t)k;
+    }
+}
+
+// Per item: owner, end slot (the next item's start, cut at the list end) and
+// class (1 = staged per workgroup: STAGE_CAP < |N+(u)| <= BIG_CAP).
+__global__ void k_tc_item_meta(const uint32_t* __restrict__ istart, const uint32_t* __restrict__ oown,
+                               const uint32_t* __restrict__ ooff, uint64_t nitems, uint64_t kept,
+                               uint32_t* __restrict__ iu, uint32_t* __restrict__ iend, uint8_t* __restrict__ big,
+                               uint8_t* __restrict__ small) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nitems; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t u = oown[istart[i]];
+        const uint32_t next = i + 1 < nitems ? istart[i + 1] : (uint32_t)kept;
+        const uint32_t du = ooff[u + 1] - ooff[u];
+        iu[i] = u;
+        iend[i] = min(next, ooff[u + 1]);
+        big[i] = du > STAGE_CAP && du <= BIG_CAP;
+        small[i] = !big[i];
+    }
+}
 
 __global__ void k_tc_flags(const uint32_t* __restrict__ off, const uint32_t* __restrict__ tgt,
                            const uint32_t* __restrict__ owner, uint64_t slots, uint8_t* __restrict__ keep) {
@@ -35,13 +87,6 @@ __global__ void k_tc_flags(const uint32_t* __restrict__ off, const uint32_t* __r
         const uint32_t u = owner[k], v = tgt[k];
         const uint32_t du = off[u + 1] - off[u], dv = off[v + 1] - off[v];
         keep[k] = (dv > du) || (dv == du && v > u);
-    }
-}
-
-__global__ void k_tc_workflags(const uint32_t* __restrict__ ooff, uint64_t n, uint8_t* flag, uint32_t* iota) {
-    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n; u += (uint64_t)gridDim.x * blockDim.x) {
-        flag[u] = (ooff[u + 1] - ooff[u]) >= 2;
-        iota[u] = (uint32_t)u;
     }
 }
 
@@ -74,135 +119,191 @@ __device__ __forceinline__ bool sorted_contains(const uint32_t* a, uint32_t len,
     return lo < len && a[lo] == x;
 }
 
-// Index of x in sorted a[0..len), or -1.
-__device__ __forceinline__ int sorted_find(const uint32_t* a, uint32_t len, uint32_t x) {
-    uint32_t lo = 0, hi = len;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (a[mid] < x)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return (lo < len && a[lo] == x) ? (int)lo : -1;
-}
-
 __device__ __forceinline__ uint32_t tc_hash(uint32_t x, uint32_t mask) { return (x * 2654435761u >> 7) & mask; }
 
-// One wavefront per vertex u with |N+(u)| >= 2. When N+(u) fits (<= STAGE_CAP
-// entries) it is staged in LDS as a sorted list, an open-addressing hash set
-// (slot -> list index) and a per-entry hit counter. For each v in N+(u) the
-// wave intersects N+(u) and N+(v) from the cheaper side:
-//   A  (|N+(v)| <= 8 |N+(u)|): walk N+(v) from HBM, probe the LDS hash;
-//   B  otherwise:              walk N+(u) from LDS, binary-search N+(v) in HBM.
-// A hit w closes the triangle (u, v, w): u's count is wave-reduced, and v and w
-// (both in N+(u)) are credited in the LDS counters, flushed to HBM once per u.
+// Slot of x in the LDS hash set, or -1.
+__device__ __forceinline__ int tc_probe(const uint32_t* key, uint32_t hmask, uint32_t x) {
+    uint32_t slot = tc_hash(x, hmask);
+    uint32_t k;
+    while ((k = key[slot]) != 0xFFFFFFFFu) {
+        if (k == x) return (int)slot;
+        slot = (slot + 1) & hmask;
+    }
+    return -1;
+}
+
+// The intersections of one work item, slots [s0, s1) of N+(u), by one wave
+// that takes every `step`-th group of 64 slots starting at s0 + first. With
+// `staged`, N+(u) sits in an LDS open-addressing hash set whose slots also
+// carry hit counters. For each v in the range the wave intersects N+(u) and
+// N+(v) from the cheaper side:
+//   A  (|N+(v)| <= 8 |N+(u)|): walk N+(v) from HBM, probe N+(u);
+//   B  otherwise:              walk N+(u), binary-search N+(v) in HBM.
+// A hit w closes the triangle (u, v, w): the returned count is u's share, and
+// v and w (both in N+(u)) are credited in the slot counters (or in HBM when
+// N+(u) is not staged). Integer sums are order-independent.
+__device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict__ ooff,
+                                                      const uint32_t* __restrict__ otgt, uint32_t a, uint32_t du,
+                                                      uint32_t s0, uint32_t s1, uint32_t first, uint32_t step,
+                                                      bool staged, const uint32_t* key, uint32_t* cnt,
+                                                      uint32_t hmask, unsigned long long* __restrict__ tri) {
+    const int lane = lane_id();
+    const uint32_t* ulist = otgt + a;
+    unsigned long long cu = 0;
+    for (uint32_t i0 = s0 + first; i0 < s1; i0 += step) {
+        const uint32_t i = i0 + lane;
+        uint32_t v = 0, dv = 0, sv = 0, c = 0;
+        int vslot = -1;
+        bool mode_a = true;
+        if (i < s1) {
+            v = ulist[i];
+            sv = ooff[v];
+            dv = ooff[v + 1] - sv;
+            mode_a = dv <= 8 * du;
+            c = mode_a ? dv : du;
+            if (staged) vslot = tc_probe(key, hmask, v);
+        }
+        const uint32_t incl = wave_incl_scan(c);
+        const uint32_t excl = incl - c;
+        const uint32_t total = __shfl(incl, 63, 64);
+        // TC_ILP groups of 64 steps per round: all their list reads are issued
+        // before any is probed, so each wave keeps several loads in flight.
+        for (uint32_t t0 = 0; t0 < total; t0 += 64 * TC_ILP) {
+            uint32_t w[TC_ILP], ovv[TC_ILP], osvv[TC_ILP], odvv[TC_ILP];
+            int ovs[TC_ILP];
+            bool oa[TC_ILP], ok[TC_ILP];
+            uint32_t ee[TC_ILP];
+#pragma unroll
+            for (int r = 0; r < TC_ILP; ++r) {
+                const uint32_t t = t0 + 64 * r + lane;
+                const int j = wave_owner(excl, t);
+                ee[r] = t - __shfl(excl, j, 64);
+                ovv[r] = __shfl(v, j, 64);
+                osvv[r] = __shfl(sv, j, 64);
+                odvv[r] = __shfl(dv, j, 64);
+                ovs[r] = __shfl(vslot, j, 64);
+                oa[r] = __shfl((int)mode_a, j, 64) != 0;
+                ok[r] = t < total;
+                w[r] = 0;
+                if (ok[r]) w[r] = oa[r] ? otgt[osvv[r] + ee[r]] : ulist[ee[r]];
+            }
+#pragma unroll
+            for (int r = 0; r < TC_ILP; ++r) {
+                if (!ok[r]) continue;
+                bool hit;
+                int wslot = -1;
+                if (oa[r]) {
+                    if (staged) {
+                        wslot = tc_probe(key, hmask, w[r]);
+                        hit = wslot >= 0;
+                    } else {
+                        hit = sorted_contains(ulist, du, w[r]);
+                    }
+                } else {
+                    hit = sorted_contains(otgt + osvv[r], odvv[r], w[r]);
+                    if (hit && staged) wslot = tc_probe(key, hmask, w[r]);
+                }
+                if (hit) {
+                    ++cu;
+                    if (staged) {
+                        atomicAdd(&cnt[wslot], 1u);
+                        atomicAdd(&cnt[ovs[r]], 1u);
+                    } else {
+                        atomicAdd(&tri[w[r]], 1ull);
+                        atomicAdd(&tri[ovv[r]], 1ull);
+                    }
+                }
+            }
+        }
+    }
+    return cu;
+}
+
+// Stages N+(u) into an LDS hash set of h = 2^k >= 2 du slots (zeroed
+// counters) with `nthreads` threads. Ends with a barrier; returns h - 1.
+__device__ __forceinline__ uint32_t tc_stage(const uint32_t* __restrict__ otgt, uint32_t a, uint32_t du,
+                                             uint32_t tid, uint32_t nthreads, uint32_t* key, uint32_t* cnt) {
+    uint32_t h = 16;
+    while (h < 2 * du) h <<= 1;
+    const uint32_t hmask = h - 1;
+    for (uint32_t i = tid; i < h; i += nthreads) {
+        key[i] = 0xFFFFFFFFu;
+        cnt[i] = 0;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < du; i += nthreads) {
+        const uint32_t x = otgt[a + i];
+        uint32_t slot = tc_hash(x, hmask);
+        while (atomicCAS(&key[slot], 0xFFFFFFFFu, x) != 0xFFFFFFFFu) slot = (slot + 1) & hmask;
+    }
+    __syncthreads();
+    return hmask;
+}
+
+// Adds the slot counters of a staged list into tri. Ends with a barrier.
+__device__ __forceinline__ void tc_flush(const uint32_t* key, const uint32_t* cnt, uint32_t h, uint32_t tid,
+                                         uint32_t nthreads, unsigned long long* __restrict__ tri) {
+    __syncthreads();
+    for (uint32_t i = tid; i < h; i += nthreads)
+        if (cnt[i]) atomicAdd(&tri[key[i]], (unsigned long long)cnt[i]);
+    __syncthreads();
+}
+
+// One wavefront per item whose list has at most STAGE_CAP entries (staged in
+// the wave's LDS) or more than BIG_CAP (read from HBM).
 __global__ __launch_bounds__(TCB) void k_tc_count(const uint32_t* __restrict__ ooff, const uint32_t* __restrict__ otgt,
-                                                  const uint32_t* __restrict__ work, uint64_t nwork,
+                                                  const uint32_t* __restrict__ istart, const uint32_t* __restrict__ iend,
+                                                  const uint32_t* __restrict__ iu, uint64_t nwork,
                                                   unsigned long long* __restrict__ tri) {
-    __shared__ uint32_t s_list[TCW][STAGE_CAP];
-    __shared__ uint32_t s_cnt[TCW][STAGE_CAP];
     __shared__ uint32_t s_key[TCW][HASH_CAP];
-    __shared__ uint16_t s_idx[TCW][HASH_CAP];
+    __shared__ uint32_t s_cnt[TCW][HASH_CAP];
     const int wid = threadIdx.x >> 6;
     const int lane = lane_id();
-    uint32_t* list = s_list[wid];
-    uint32_t* cnt = s_cnt[wid];
     uint32_t* key = s_key[wid];
-    uint16_t* hidx = s_idx[wid];
+    uint32_t* cnt = s_cnt[wid];
     for (uint64_t g = blockIdx.x; g * TCW < nwork; g += gridDim.x) {
         const uint64_t wi = g * TCW + wid;
         const bool active = wi < nwork;
-        const uint32_t u = active ? work[wi] : 0;
+        const uint32_t u = active ? iu[wi] : 0;
         const uint32_t a = active ? ooff[u] : 0;
         const uint32_t du = active ? ooff[u + 1] - a : 0;
+        const uint32_t s0 = active ? istart[wi] - a : 0;
+        const uint32_t s1 = active ? iend[wi] - a : 0;
         const bool staged = du <= STAGE_CAP;
-        uint32_t hmask = 0;
-        if (staged && du) {
-            uint32_t h = 16;
-            while (h < 2 * du) h <<= 1;
-            hmask = h - 1;
-            for (uint32_t i = lane; i < h; i += 64) key[i] = 0xFFFFFFFFu;
-            for (uint32_t i = lane; i < du; i += 64) {
-                list[i] = otgt[a + i];
-                cnt[i] = 0;
-            }
-        }
-        __syncthreads();
-        if (staged && du) {
-            for (uint32_t i = lane; i < du; i += 64) {
-                const uint32_t x = list[i];
-                uint32_t slot = tc_hash(x, hmask);
-                while (atomicCAS(&key[slot], 0xFFFFFFFFu, x) != 0xFFFFFFFFu) slot = (slot + 1) & hmask;
-                hidx[slot] = (uint16_t)i;
-            }
-        }
-        __syncthreads();
-        const uint32_t* ulist = staged ? list : otgt + a;
-        unsigned long long cu = 0;
-        for (uint32_t i0 = 0; i0 < du; i0 += 64) {
-            const uint32_t i = i0 + lane;
-            uint32_t v = 0, dv = 0, sv = 0, c = 0;
-            bool mode_a = true;
-            if (i < du) {
-                v = ulist[i];
-                sv = ooff[v];
-                dv = ooff[v + 1] - sv;
-                mode_a = dv <= 8 * du;
-                c = mode_a ? dv : du;
-            }
-            const uint32_t incl = wave_incl_scan(c);
-            const uint32_t excl = incl - c;
-            const uint32_t total = __shfl(incl, 63, 64);
-            for (uint32_t t0 = 0; t0 < total; t0 += 64) {
-                const uint32_t t = t0 + lane;
-                const int j = wave_owner(excl, t);
-                const uint32_t e = t - __shfl(excl, j, 64);
-                const uint32_t ov = __shfl(v, j, 64);
-                const uint32_t osv = __shfl(sv, j, 64);
-                const uint32_t odv = __shfl(dv, j, 64);
-                const bool oa = __shfl((int)mode_a, j, 64) != 0;
-                if (t >= total) continue;
-                int hit = -1;  // index of w in N+(u)
-                uint32_t w;
-                if (oa) {
-                    w = otgt[osv + e];
-                    if (staged) {
-                        uint32_t slot = tc_hash(w, hmask);
-                        uint32_t k;
-                        while ((k = key[slot]) != 0xFFFFFFFFu) {
-                            if (k == w) {
-                                hit = hidx[slot];
-                                break;
-                            }
-                            slot = (slot + 1) & hmask;
-                        }
-                    } else {
-                        hit = sorted_find(ulist, du, w);
-                    }
-                } else {
-                    w = ulist[e];
-                    if (sorted_contains(otgt + osv, odv, w)) hit = (int)e;
-                }
-                if (hit >= 0) {
-                    ++cu;
-                    if (staged) {
-                        atomicAdd(&cnt[hit], 1u);
-                        atomicAdd(&cnt[i0 + j], 1u);
-                    } else {
-                        atomicAdd(&tri[w], 1ull);
-                        atomicAdd(&tri[ov], 1ull);
-                    }
-                }
-            }
-        }
+        // Each wave stages its own list; the barriers inside are block-wide, so
+        // every wave takes part (an empty list stages nothing).
+        const uint32_t sdu = staged ? du : 0;
+        const uint32_t hmask = tc_stage(otgt, a, sdu, lane, 64, key, cnt);
+        unsigned long long cu = tc_item(ooff, otgt, a, du, s0, s1, 0, 64, staged, key, cnt, hmask, tri);
         cu = wave_sum_u64(cu);
         if (lane == 0 && cu) atomicAdd(&tri[u], cu);
-        __syncthreads();
-        if (staged)
-            for (uint32_t i = lane; i < du; i += 64)
-                if (cnt[i]) atomicAdd(&tri[list[i]], (unsigned long long)cnt[i]);
-        __syncthreads();
+        tc_flush(key, cnt, hmask + 1, lane, 64, tri);
+    }
+}
+
+// One workgroup per item whose list has STAGE_CAP < |N+(u)| <= BIG_CAP
+// entries: the list is staged once in the group's LDS and the four waves
+// share the item's slots.
+__global__ __launch_bounds__(TCB) void k_tc_count_big(const uint32_t* __restrict__ ooff,
+                                                      const uint32_t* __restrict__ otgt,
+                                                      const uint32_t* __restrict__ istart,
+                                                      const uint32_t* __restrict__ iend,
+                                                      const uint32_t* __restrict__ iu, uint64_t nwork,
+                                                      unsigned long long* __restrict__ tri) {
+    __shared__ uint32_t b_key[2 * BIG_CAP];
+    __shared__ uint32_t b_cnt[2 * BIG_CAP];
+    const int wid = threadIdx.x >> 6;
+    const int lane = lane_id();
+    for (uint64_t wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+        const uint32_t u = iu[wi];
+        const uint32_t a = ooff[u];
+        const uint32_t du = ooff[u + 1] - a;
+        const uint32_t s0 = istart[wi] - a, s1 = iend[wi] - a;
+        const uint32_t hmask = tc_stage(otgt, a, du, threadIdx.x, TCB, b_key, b_cnt);
+        unsigned long long cu = tc_item(ooff, otgt, a, du, s0, s1, 64u * wid, TCB, true, b_key, b_cnt, hmask, tri);
+        cu = wave_sum_u64(cu);
+        if (lane == 0 && cu) atomicAdd(&tri[u], cu);
+        tc_flush(b_key, b_cnt, hmask + 1, threadIdx.x, TCB, tri);
     }
 }
 
@@ -217,13 +318,14 @@ static void tc_orient(ursa_hip_topo* t, hipStream_t s) {
     const uint64_t slots = U.slots;
     t->tc_off.alloc(n + 1);
     uint64_t kept = 0;
+    DevBuf<uint32_t> oown;  // owner of each oriented slot
     if (slots) {
         DevBuf<uint32_t> owner(slots);
         slot_owners(n, U, owner.get(), s);
         DevBuf<uint8_t> keep(slots);
         k_tc_flags<<<blocks_for(slots, 256), 256, 0, s>>>(U.off.get(), U.tgt.get(), owner.get(), slots, keep.get());
         check_launch();
-        DevBuf<uint32_t> oown(slots);
+        oown.alloc(slots);
         t->tc_tgt.alloc(slots);
         DevBuf<uint64_t> cnt(1);
         prim::select_flagged_u32(U.tgt.get(), keep.get(), t->tc_tgt.get(), cnt.get(), slots, s);
@@ -235,17 +337,50 @@ static void tc_orient(ursa_hip_topo* t, hipStream_t s) {
         t->tc_tgt.alloc(1);
         offsets_from_sorted(nullptr, 0, n, t->tc_off.get(), s);
     }
-    // Work list: vertices with at least two oriented neighbours.
-    DevBuf<uint8_t> flag(n ? n : 1);
-    DevBuf<uint32_t> iota(n ? n : 1);
-    t->tc_work.alloc(n ? n : 1);
-    DevBuf<uint64_t> wc(1);
+    // Work items (see the file comment).
     uint64_t nwork = 0;
-    if (n) {
-        k_tc_workflags<<<blocks_for(n, 256), 256, 0, s>>>(t->tc_off.get(), n, flag.get(), iota.get());
+    if (kept) {
+        DevBuf<uint32_t> cost(kept), iota(kept);
+        DevBuf<uint64_t> pre(kept + 1);
+        DevBuf<uint8_t> flag(kept);
+        k_tc_cost<<<blocks_for(kept, 256), 256, 0, s>>>(t->tc_off.get(), t->tc_tgt.get(), oown.get(), kept,
+                                                       cost.get());
         check_launch();
-        prim::select_flagged_u32(iota.get(), flag.get(), t->tc_work.get(), wc.get(), n, s);
-        HIP_OK(hipMemcpyAsync(&nwork, wc.get(), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        prim::exclusive_sum_u32_to_u64(cost.get(), pre.get(), kept, s);
+        k_tc_item_flags<<<blocks_for(kept, 256), 256, 0, s>>>(t->tc_off.get(), oown.get(), pre.get(), kept,
+                                                             flag.get(), iota.get());
+        check_launch();
+        DevBuf<uint32_t> istart(kept), iu(kept), iend(kept);
+        DevBuf<uint64_t> wc(1);
+        prim::select_flagged_u32(iota.get(), flag.get(), istart.get(), wc.get(), kept, s);
+        uint64_t nitems = 0;
+        HIP_OK(hipMemcpyAsync(&nitems, wc.get(), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        DevBuf<uint8_t> big(nitems ? nitems : 1), small(nitems ? nitems : 1);
+        if (nitems) {
+            k_tc_item_meta<<<blocks_for(nitems, 256), 256, 0, s>>>(istart.get(), oown.get(), t->tc_off.get(), nitems,
+                                                                  kept, iu.get(), iend.get(), big.get(), small.get());
+            check_launch();
+        }
+        // Split the items into the per-wave list and the per-workgroup list.
+        auto part = [&](const uint8_t* f, DevBuf<uint32_t>& a, DevBuf<uint32_t>& b, DevBuf<uint32_t>& c) -> uint64_t {
+            uint64_t cnt = 0;
+            a.alloc(nitems ? nitems : 1);
+            b.alloc(nitems ? nitems : 1);
+            c.alloc(nitems ? nitems : 1);
+            if (!nitems) return 0;
+            prim::select_flagged_u32(istart.get(), f, a.get(), wc.get(), nitems, s);
+            prim::select_flagged_u32(iend.get(), f, b.get(), wc.get(), nitems, s);
+            prim::select_flagged_u32(iu.get(), f, c.get(), wc.get(), nitems, s);
+            HIP_OK(hipMemcpyAsync(&cnt, wc.get(), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+            HIP_OK(hipStreamSynchronize(s));
+            return cnt;
+        };
+        nwork = part(small.get(), t->tc_istart, t->tc_iend, t->tc_iu);
+        t->tc_nbig = part(big.get(), t->tc_bstart, t->tc_bend, t->tc_bu);
+    } else {
+        for (auto* b : {&t->tc_istart, &t->tc_iend, &t->tc_iu, &t->tc_bstart, &t->tc_bend, &t->tc_bu}) b->alloc(1);
+        t->tc_nbig = 0;
     }
     DevBuf<unsigned long long> pairs(1);
     HIP_OK(hipMemsetAsync(pairs.get(), 0, sizeof(unsigned long long), s));
@@ -271,12 +406,18 @@ void triangle_count(ursa_hip_topo* t, uint64_t* out_user) {
     tc_orient(t, s);
     DevOut<uint64_t> out(out_user, n);
     HIP_OK(hipMemsetAsync(out.p, 0, n * sizeof(uint64_t), s));
-    if (t->tc_nwork) {
+    if (t->tc_nwork || t->tc_nbig) {
         const double bytes = 4.0 * t->tc_pair_work + 12.0 * (double)n;
-        const unsigned grid = (unsigned)std::min<uint64_t>((t->tc_nwork + TCW - 1) / TCW, 1u << 16);
+        auto* tri = reinterpret_cast<unsigned long long*>(out.p);
         timed("triangle_count", bytes, s, [&] {
-            k_tc_count<<<grid, TCB, 0, s>>>(t->tc_off.get(), t->tc_tgt.get(), t->tc_work.get(), t->tc_nwork,
-                                            reinterpret_cast<unsigned long long*>(out.p));
+            if (t->tc_nbig)
+                k_tc_count_big<<<(unsigned)std::min<uint64_t>(t->tc_nbig, 1u << 16), TCB, 0, s>>>(
+                    t->tc_off.get(), t->tc_tgt.get(), t->tc_bstart.get(), t->tc_bend.get(), t->tc_bu.get(),
+                    t->tc_nbig, tri);
+            if (t->tc_nwork)
+                k_tc_count<<<(unsigned)std::min<uint64_t>((t->tc_nwork + TCW - 1) / TCW, 1u << 16), TCB, 0, s>>>(
+                    t->tc_off.get(), t->tc_tgt.get(), t->tc_istart.get(), t->tc_iend.get(), t->tc_iu.get(),
+                    t->tc_nwork, tri);
         });
         check_launch();
     }
