@@ -33,15 +33,32 @@ constexpr int TW = TB / 64;
 
 struct OCtr {
     uint32_t count;
-    uint32_t hit;  // target reached (shortest_path)
+    uint32_t hit;    // target reached (shortest_path)
+    uint32_t heavy;  // frontier positions deferred to k_oexpand_heavy
+    uint32_t pad;
 };
 
+constexpr uint32_t O_HEAVY = 1024;  // frontier vertices with more edges go grid-wide
+
+// Offers v, reached by frontier position p through its slot j: atomicMin of
+// the discovery key, and the thread that replaced the initial key appends v to
+// the candidate list (wave-aggregated).
+__device__ __forceinline__ void o_offer(bool active, uint32_t v, uint64_t p, uint32_t j,
+                                        const int32_t* __restrict__ level, unsigned long long* __restrict__ key,
+                                        uint32_t* __restrict__ cand, OCtr* ctr) {
+    bool fresh = false;
+    if (active && level[v] < 0) fresh = atomicMin(&key[v], ((unsigned long long)p << 32) | j) == ~0ull;
+    const uint32_t slot = wave_append_slot(fresh, &ctr->count);
+    if (fresh) cand[slot] = v;
+}
+
 // Expand frontier F[0..flen): candidates get atomicMin(key, p<<32 | j) and are
-// appended to cand once (the thread whose atomicMin saw the initial key).
+// appended to cand once. Frontier vertices with more than O_HEAVY slots are
+// deferred to k_oexpand_heavy.
 __global__ __launch_bounds__(TB) void k_oexpand(const uint32_t* __restrict__ F, uint32_t flen, NbrView fwd,
                                                 const int32_t* __restrict__ level,
                                                 unsigned long long* __restrict__ key, uint32_t* __restrict__ cand,
-                                                OCtr* ctr) {
+                                                OCtr* ctr, uint32_t* __restrict__ heavy) {
     const int lane = lane_id();
     const uint64_t gw = blockIdx.x * (uint64_t)TW + (threadIdx.x >> 6);
     const uint64_t nwaves = (uint64_t)gridDim.x * TW;
@@ -53,6 +70,10 @@ __global__ __launch_bounds__(TB) void k_oexpand(const uint32_t* __restrict__ F, 
             d = nv_degree(fwd, u, da);
             sa = fwd.off_a[u];
             if (fwd.off_b) sb = fwd.off_b[u];
+            if (d > O_HEAVY) {
+                heavy[atomicAdd(&ctr->heavy, 1u)] = (uint32_t)idx;
+                d = 0;
+            }
         }
         const uint32_t incl = wave_incl_scan(d);
         const uint32_t excl = incl - d;
@@ -64,17 +85,35 @@ __global__ __launch_bounds__(TB) void k_oexpand(const uint32_t* __restrict__ F, 
             const uint32_t oa = __shfl(da, j, 64);
             const uint32_t osa = __shfl(sa, j, 64);
             const uint32_t osb = __shfl(sb, j, 64);
-            bool fresh = false;
             uint32_t v = 0;
-            if (t < total) {
-                v = e < oa ? fwd.tgt_a[osa + e] : fwd.tgt_b[osb + (e - oa)];
-                if (level[v] < 0) {
-                    const unsigned long long k = ((unsigned long long)(base + j) << 32) | e;
-                    fresh = atomicMin(&key[v], k) == ~0ull;
-                }
-            }
-            const uint32_t slot = wave_append_slot(fresh, &ctr->count);
-            if (fresh) cand[slot] = v;
+            if (t < total) v = e < oa ? fwd.tgt_a[osa + e] : fwd.tgt_b[osb + (e - oa)];
+            o_offer(t < total, v, base + j, e, level, key, cand, ctr);
+        }
+    }
+}
+
+// Deferred frontier vertices: every wave of the grid takes a stride of each
+// one's slots, so a hub's edges spread over the device.
+__global__ __launch_bounds__(TB) void k_oexpand_heavy(const uint32_t* __restrict__ F, NbrView fwd,
+                                                      const int32_t* __restrict__ level,
+                                                      unsigned long long* __restrict__ key,
+                                                      uint32_t* __restrict__ cand, OCtr* ctr,
+                                                      const uint32_t* __restrict__ heavy) {
+    const uint32_t cnt = ctr->heavy;
+    const uint64_t wave_base = (blockIdx.x * (uint64_t)TW + (threadIdx.x >> 6)) * 64;
+    const uint64_t stride = (uint64_t)gridDim.x * TB;
+    for (uint32_t h = 0; h < cnt; ++h) {
+        const uint32_t p = heavy[h];
+        const uint32_t u = F[p];
+        uint32_t da;
+        const uint32_t d = nv_degree(fwd, u, da);
+        const uint32_t sa = fwd.off_a[u];
+        const uint32_t sb = fwd.off_b ? fwd.off_b[u] : 0;
+        for (uint64_t e0 = wave_base; e0 < d; e0 += stride) {
+            const uint32_t e = (uint32_t)(e0 + lane_id());
+            uint32_t v = 0;
+            if (e < d) v = e < da ? fwd.tgt_a[sa + e] : fwd.tgt_b[sb + (e - da)];
+            o_offer(e < d, v, p, e, level, key, cand, ctr);
         }
     }
 }
@@ -130,9 +169,10 @@ struct OWork {
     DevBuf<unsigned long long> key;
     DevBuf<uint32_t> order, cand, sorted, parent;
     DevBuf<uint64_t> ckeys, skeys;
+    DevBuf<uint32_t> heavy;
     DevBuf<OCtr> ctr;
     OWork(uint64_t n_, bool with_parent, hipStream_t s)
-        : n(n_), level(n_), key(n_), order(n_), cand(n_), sorted(n_), ckeys(n_), skeys(n_), ctr(1) {
+        : n(n_), level(n_), key(n_), order(n_), cand(n_), sorted(n_), ckeys(n_), skeys(n_), heavy(n_), ctr(1) {
         if (with_parent) parent.alloc(n_);
         k_fill_i32b<<<blocks_for(n, 256), 256, 0, s>>>(level.get(), n, -1);
         k_fill_u64<<<blocks_for(n, 256), 256, 0, s>>>(key.get(), n, ~0ull);
@@ -150,7 +190,7 @@ struct OWork {
 // stopping early once `target` is reached. Leaves the discovery order in
 // w.order[0..*visited) and returns whether the target was reached.
 bool ordered_bfs(uint32_t seed, uint32_t max_levels, uint32_t target, const NbrView& fwd, OWork& w, hipStream_t s,
-                 uint64_t* visited) {
+                 uint64_t* visited, bool may_have_heavy) {
     k_oseed<<<1, 1, 0, s>>>(seed, w.level.get(), w.order.get(), w.parent.get());
     check_launch();
     uint64_t fstart = 0, flen = 1, total = 1;
@@ -159,7 +199,11 @@ bool ordered_bfs(uint32_t seed, uint32_t max_levels, uint32_t target, const NbrV
         HIP_OK(hipMemsetAsync(w.ctr.get(), 0, sizeof(OCtr), s));
         const uint32_t* F = w.order.get() + fstart;
         k_oexpand<<<blocks_for((flen + 63) / 64, TW, 1u << 16), TB, 0, s>>>(F, (uint32_t)flen, fwd, w.level.get(),
-                                                                           w.key.get(), w.cand.get(), w.ctr.get());
+                                                                           w.key.get(), w.cand.get(), w.ctr.get(),
+                                                                           w.heavy.get());
+        if (may_have_heavy)
+            k_oexpand_heavy<<<512, TB, 0, s>>>(F, fwd, w.level.get(), w.key.get(), w.cand.get(), w.ctr.get(),
+                                               w.heavy.get());
         check_launch();
         const uint32_t cnt = w.read(s).count;
         if (cnt == 0) break;
@@ -204,8 +248,8 @@ __global__ void k_opath(const uint32_t* __restrict__ parent, uint32_t src, uint3
 }  // namespace
 
 uint64_t traverse_workspace_bytes(uint64_t n) {
-    // OWork (level, key, order, cand, sorted, parent, ckeys, skeys: 44 B per
-    // node) plus the radix sort's temporary storage for up to n (u64, u32)
+    // OWork (level, key, order, cand, sorted, parent, ckeys, skeys, heavy: 48 B
+    // per node) plus the radix sort's temporary storage for up to n (u64, u32)
     // pairs (a double buffer of 12 B per pair and its histograms).
     return n * 80 + (2u << 20);
 }
@@ -223,7 +267,7 @@ void k_hop_seq(ursa_hip_topo* t, const uint32_t* seeds, uint64_t nseeds, uint32_
         const uint32_t seed = seeds[i];
         if (seed < n) {  // hop.rs:91-93: unknown seed contributes nothing
             uint64_t visited = 0;
-            ordered_bfs(seed, k, 0xFFFFFFFFu, fwd, w, s, &visited);
+            ordered_bfs(seed, k, 0xFFFFFFFFu, fwd, w, s, &visited, t->m > O_HEAVY);
             const uint64_t reached = visited - 1;
             if (reached) {
                 const size_t at = dst.size();
@@ -276,7 +320,7 @@ void shortest_paths_seq(ursa_hip_topo* t, const uint32_t* src, const uint32_t* t
     for (uint64_t i = 0; i < count; ++i) {
         if (src[i] >= n || tgt[i] >= n || src[i] == tgt[i]) continue;
         uint64_t visited = 0;
-        const bool hit = ordered_bfs(src[i], 0, tgt[i], fwd, w, s, &visited);
+        const bool hit = ordered_bfs(src[i], 0, tgt[i], fwd, w, s, &visited, t->m > O_HEAVY);
         if (hit) {
             k_opath<<<1, 1, 0, s>>>(w.parent.get(), src[i], tgt[i], dpath.get(), (uint32_t)std::min<uint64_t>(n, visited),
                                     dlen.get());
