@@ -28,11 +28,15 @@ namespace {
 
 constexpr int TCB = 256;
 constexpr int TCW = TCB / 64;
-constexpr uint32_t STAGE_CAP = 512;  // N+(u) entries staged in LDS per wave
-constexpr uint32_t HASH_CAP = 1024;  // hash slots per wave (>= 2 * STAGE_CAP)
+constexpr int TCB_BIG = 128;  // threads of a workgroup sharing one long list
+constexpr uint32_t STAGE_CAP = 256;  // N+(u) entries staged in LDS per wave
+constexpr uint32_t HASH_CAP = 512;   // hash slots per wave (>= 2 * STAGE_CAP)
+constexpr uint32_t TC_BUCKET = 8;    // hash slots per bucket
 constexpr uint32_t BIG_CAP = 2048;   // N+(u) entries staged in LDS per workgroup
 constexpr uint64_t TC_CHUNK = 8192;  // intersection steps per work item (about)
 constexpr int TC_ILP = 4;            // 64-step groups whose loads are in flight together
+constexpr uint32_t TC_LONG = 32;     // walks at least this long are taken by the whole wave
+constexpr int TC_LILP = 4;           // 64-step chunks of long walks in flight together
 
 // Intersection cost of oriented slot k = (u, v), as k_tc_count spends it:
 // d+(v) hash probes (mode A) or d+(u) binary searches (mode B).
@@ -121,15 +125,57 @@ __device__ __forceinline__ bool sorted_contains(const uint32_t* a, uint32_t len,
 
 __device__ __forceinline__ uint32_t tc_hash(uint32_t x, uint32_t mask) { return (x * 2654435761u >> 7) & mask; }
 
-// Slot of x in the LDS hash set, or -1.
-__device__ __forceinline__ int tc_probe(const uint32_t* key, uint32_t hmask, uint32_t x) {
-    uint32_t slot = tc_hash(x, hmask);
-    uint32_t k;
-    while ((k = key[slot]) != 0xFFFFFFFFu) {
-        if (k == x) return (int)slot;
-        slot = (slot + 1) & hmask;
+// The LDS hash set: buckets of TC_BUCKET = 8 slots (32 bytes), key x homed
+// in bucket tc_hash(x) and placed in its first free slot, spilling to the next
+// bucket when full (load factor <= 1/2, so a spill is rare). A lookup reads
+// its home bucket with two 16-byte loads and checks all eight slots at once.
+// `bmask` is the bucket count minus one. Returns x's slot, or -1.
+__device__ __forceinline__ int tc_probe(const uint32_t* key, uint32_t bmask, uint32_t x) {
+    uint32_t b = tc_hash(x, bmask);
+    for (;;) {
+        const uint4 k0 = reinterpret_cast<const uint4*>(key)[2 * b];
+        const uint4 k1 = reinterpret_cast<const uint4*>(key)[2 * b + 1];
+        int f = -1;
+        f = k1.w == x ? 7 : f;
+        f = k1.z == x ? 6 : f;
+        f = k1.y == x ? 5 : f;
+        f = k1.x == x ? 4 : f;
+        f = k0.w == x ? 3 : f;
+        f = k0.z == x ? 2 : f;
+        f = k0.y == x ? 1 : f;
+        f = k0.x == x ? 0 : f;
+        if (f >= 0) return (int)(b * TC_BUCKET) + f;
+        const uint32_t E = 0xFFFFFFFFu;
+        if (k0.x == E || k0.y == E || k0.z == E || k0.w == E || k1.x == E || k1.y == E || k1.z == E || k1.w == E)
+            return -1;
+        b = (b + 1) & bmask;
     }
-    return -1;
+}
+
+// Records a triangle (u, v, w) found with w at hash slot wslot (staged) or
+// w itself (not staged): w's credit; v's credit and u's are added by the caller.
+// Slot counters are 16-bit, two per LDS word (a slot is credited at most
+// once per v of the item plus its own hits, both below 2^16).
+__device__ __forceinline__ void tc_cnt_add(uint32_t* cnt, int slot, uint32_t k) {
+    atomicAdd(&cnt[slot >> 1], k << (16 * (slot & 1)));
+}
+
+__device__ __forceinline__ void tc_credit_w(bool staged, int wslot, uint32_t w, uint32_t* cnt,
+                                            unsigned long long* __restrict__ tri) {
+    if (staged)
+        tc_cnt_add(cnt, wslot, 1u);
+    else
+        atomicAdd(&tri[w], 1ull);
+}
+
+// Is w in N+(u)? Sets wslot when staged.
+__device__ __forceinline__ bool tc_member(bool staged, const uint32_t* key, uint32_t hmask, const uint32_t* ulist,
+                                          uint32_t du, uint32_t w, int& wslot) {
+    if (staged) {
+        wslot = tc_probe(key, hmask, w);
+        return wslot >= 0;
+    }
+    return sorted_contains(ulist, du, w);
 }
 
 // The intersections of one work item, slots [s0, s1) of N+(u), by one wave
@@ -139,9 +185,12 @@ __device__ __forceinline__ int tc_probe(const uint32_t* key, uint32_t hmask, uin
 // N+(v) from the cheaper side:
 //   A  (|N+(v)| <= 8 |N+(u)|): walk N+(v) from HBM, probe N+(u);
 //   B  otherwise:              walk N+(u), binary-search N+(v) in HBM.
-// A hit w closes the triangle (u, v, w): the returned count is u's share, and
-// v and w (both in N+(u)) are credited in the slot counters (or in HBM when
-// N+(u) is not staged). Integer sums are order-independent.
+// A v whose walk has at least TC_LONG steps is walked by the whole wave in
+// 64-step chunks; the shorter walks of the group are flattened over the 64
+// lanes with a prefix sum and an owner search.
+// A hit w closes the triangle (u, v, w): the returned count (wave-uniform) is
+// u's share, and v and w (both in N+(u)) are credited in the slot counters (or
+// in HBM when N+(u) is not staged). Integer sums are order-independent.
 __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict__ ooff,
                                                       const uint32_t* __restrict__ otgt, uint32_t a, uint32_t du,
                                                       uint32_t s0, uint32_t s1, uint32_t first, uint32_t step,
@@ -163,21 +212,73 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
             c = mode_a ? dv : du;
             if (staged) vslot = tc_probe(key, hmask, v);
         }
+        // Long walks: cut into 64-step chunks walked by the whole wave, lane l
+        // taking step l of the chunk; TC_LILP chunks (of one or several lists)
+        // have their loads in flight together. The chunk's list is found with a
+        // ballot: lanes' chunk prefixes are non-decreasing, so the owner of
+        // chunk ch is the last lane whose prefix is <= ch.
+        const uint32_t nch = c >= TC_LONG ? (c + 63) / 64 : 0;
+        const uint32_t chi = wave_incl_scan(nch);
+        const uint32_t che = chi - nch;
+        const uint32_t nchunks = __shfl(chi, 63, 64);
+        for (uint32_t ch0 = 0; ch0 < nchunks; ch0 += TC_LILP) {
+            uint32_t w[TC_LILP];
+            bool ok[TC_LILP];
+            int own[TC_LILP];
+#pragma unroll
+            for (int r = 0; r < TC_LILP; ++r) {
+                const uint32_t ch = ch0 + r;
+                own[r] = (int)__popcll(__ballot(che <= ch)) - 1;
+                const int j = own[r];
+                const uint32_t e = (ch - __builtin_amdgcn_readlane(che, j)) * 64 + lane;
+                const uint32_t oc = __builtin_amdgcn_readlane(c, j);
+                const uint32_t osv = __builtin_amdgcn_readlane(sv, j);
+                const bool oa = __builtin_amdgcn_readlane((int)mode_a, j) != 0;
+                ok[r] = ch < nchunks && e < oc;
+                w[r] = ok[r] ? (oa ? otgt[osv + e] : ulist[e]) : 0;
+            }
+#pragma unroll
+            for (int r = 0; r < TC_LILP; ++r) {
+                if (ch0 + r >= nchunks) break;
+                const int j = own[r];
+                const bool oa = __builtin_amdgcn_readlane((int)mode_a, j) != 0;
+                int wslot = -1;
+                bool hit = false;
+                if (ok[r]) {
+                    if (oa) {
+                        hit = tc_member(staged, key, hmask, ulist, du, w[r], wslot);
+                    } else {
+                        const uint32_t osv = __builtin_amdgcn_readlane(sv, j);
+                        const uint32_t odv = __builtin_amdgcn_readlane(dv, j);
+                        hit = sorted_contains(otgt + osv, odv, w[r]);
+                        if (hit && staged) wslot = tc_probe(key, hmask, w[r]);
+                    }
+                    if (hit) tc_credit_w(staged, wslot, w[r], cnt, tri);
+                }
+                const uint32_t h = (uint32_t)__popcll(__ballot(hit));
+                cu += h;
+                if (lane == 0 && h) {
+                    if (staged)
+                        tc_cnt_add(cnt, __builtin_amdgcn_readlane(vslot, j), h);
+                    else
+                        atomicAdd(&tri[__builtin_amdgcn_readlane(v, j)], (unsigned long long)h);
+                }
+            }
+        }
+        // Short walks, flattened.
+        if (c >= TC_LONG) c = 0;
         const uint32_t incl = wave_incl_scan(c);
         const uint32_t excl = incl - c;
         const uint32_t total = __shfl(incl, 63, 64);
-        // TC_ILP groups of 64 steps per round: all their list reads are issued
-        // before any is probed, so each wave keeps several loads in flight.
         for (uint32_t t0 = 0; t0 < total; t0 += 64 * TC_ILP) {
             uint32_t w[TC_ILP], ovv[TC_ILP], osvv[TC_ILP], odvv[TC_ILP];
             int ovs[TC_ILP];
             bool oa[TC_ILP], ok[TC_ILP];
-            uint32_t ee[TC_ILP];
 #pragma unroll
             for (int r = 0; r < TC_ILP; ++r) {
                 const uint32_t t = t0 + 64 * r + lane;
                 const int j = wave_owner(excl, t);
-                ee[r] = t - __shfl(excl, j, 64);
+                const uint32_t e = t - __shfl(excl, j, 64);
                 ovv[r] = __shfl(v, j, 64);
                 osvv[r] = __shfl(sv, j, 64);
                 odvv[r] = __shfl(dv, j, 64);
@@ -185,78 +286,79 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
                 oa[r] = __shfl((int)mode_a, j, 64) != 0;
                 ok[r] = t < total;
                 w[r] = 0;
-                if (ok[r]) w[r] = oa[r] ? otgt[osvv[r] + ee[r]] : ulist[ee[r]];
+                if (ok[r]) w[r] = oa[r] ? otgt[osvv[r] + e] : ulist[e];
             }
 #pragma unroll
             for (int r = 0; r < TC_ILP; ++r) {
-                if (!ok[r]) continue;
-                bool hit;
+                bool hit = false;
                 int wslot = -1;
-                if (oa[r]) {
-                    if (staged) {
-                        wslot = tc_probe(key, hmask, w[r]);
-                        hit = wslot >= 0;
-                    } else {
-                        hit = sorted_contains(ulist, du, w[r]);
-                    }
+                if (!ok[r]) {
+                    // nothing to test
+                } else if (oa[r]) {
+                    hit = tc_member(staged, key, hmask, ulist, du, w[r], wslot);
                 } else {
                     hit = sorted_contains(otgt + osvv[r], odvv[r], w[r]);
                     if (hit && staged) wslot = tc_probe(key, hmask, w[r]);
                 }
                 if (hit) {
-                    ++cu;
-                    if (staged) {
-                        atomicAdd(&cnt[wslot], 1u);
-                        atomicAdd(&cnt[ovs[r]], 1u);
-                    } else {
-                        atomicAdd(&tri[w[r]], 1ull);
+                    tc_credit_w(staged, wslot, w[r], cnt, tri);
+                    if (staged)
+                        tc_cnt_add(cnt, ovs[r], 1u);
+                    else
                         atomicAdd(&tri[ovv[r]], 1ull);
-                    }
                 }
+                cu += (uint64_t)__popcll(__ballot(ok[r] && hit));
             }
         }
     }
     return cu;
 }
 
-// Stages N+(u) into an LDS hash set of h = 2^k >= 2 du slots (zeroed
-// counters) with `nthreads` threads. Ends with a barrier; returns h - 1.
+// Stages N+(u) into an LDS hash set of 2^k >= du / 4 buckets (at least two
+// slots per entry) with zeroed counters, using `nthreads` threads. Ends with a
+// barrier; returns the bucket mask.
 __device__ __forceinline__ uint32_t tc_stage(const uint32_t* __restrict__ otgt, uint32_t a, uint32_t du,
                                              uint32_t tid, uint32_t nthreads, uint32_t* key, uint32_t* cnt) {
-    uint32_t h = 16;
-    while (h < 2 * du) h <<= 1;
-    const uint32_t hmask = h - 1;
-    for (uint32_t i = tid; i < h; i += nthreads) {
-        key[i] = 0xFFFFFFFFu;
-        cnt[i] = 0;
-    }
+    uint32_t nb = 2;
+    while (nb * TC_BUCKET < 2 * du) nb <<= 1;
+    const uint32_t h = nb * TC_BUCKET;
+    for (uint32_t i = tid; i < h; i += nthreads) key[i] = 0xFFFFFFFFu;
+    for (uint32_t i = tid; i < h / 2; i += nthreads) cnt[i] = 0;
     __syncthreads();
     for (uint32_t i = tid; i < du; i += nthreads) {
         const uint32_t x = otgt[a + i];
-        uint32_t slot = tc_hash(x, hmask);
-        while (atomicCAS(&key[slot], 0xFFFFFFFFu, x) != 0xFFFFFFFFu) slot = (slot + 1) & hmask;
+        uint32_t b = tc_hash(x, nb - 1);
+        for (;;) {
+            bool done = false;
+            for (uint32_t k = 0; k < TC_BUCKET && !done; ++k)
+                done = atomicCAS(&key[b * TC_BUCKET + k], 0xFFFFFFFFu, x) == 0xFFFFFFFFu;
+            if (done) break;
+            b = (b + 1) & (nb - 1);
+        }
     }
     __syncthreads();
-    return hmask;
+    return nb - 1;
 }
 
 // Adds the slot counters of a staged list into tri. Ends with a barrier.
 __device__ __forceinline__ void tc_flush(const uint32_t* key, const uint32_t* cnt, uint32_t h, uint32_t tid,
                                          uint32_t nthreads, unsigned long long* __restrict__ tri) {
     __syncthreads();
-    for (uint32_t i = tid; i < h; i += nthreads)
-        if (cnt[i]) atomicAdd(&tri[key[i]], (unsigned long long)cnt[i]);
+    for (uint32_t i = tid; i < h; i += nthreads) {
+        const uint32_t c = (cnt[i >> 1] >> (16 * (i & 1))) & 0xFFFFu;
+        if (c) atomicAdd(&tri[key[i]], (unsigned long long)c);
+    }
     __syncthreads();
 }
 
 // One wavefront per item whose list has at most STAGE_CAP entries (staged in
 // the wave's LDS) or more than BIG_CAP (read from HBM).
-__global__ __launch_bounds__(TCB) void k_tc_count(const uint32_t* __restrict__ ooff, const uint32_t* __restrict__ otgt,
+__global__ __launch_bounds__(TCB, 8) void k_tc_count(const uint32_t* __restrict__ ooff, const uint32_t* __restrict__ otgt,
                                                   const uint32_t* __restrict__ istart, const uint32_t* __restrict__ iend,
                                                   const uint32_t* __restrict__ iu, uint64_t nwork,
                                                   unsigned long long* __restrict__ tri) {
-    __shared__ uint32_t s_key[TCW][HASH_CAP];
-    __shared__ uint32_t s_cnt[TCW][HASH_CAP];
+    __shared__ __align__(16) uint32_t s_key[TCW][HASH_CAP];
+    __shared__ uint32_t s_cnt[TCW][HASH_CAP / 2];
     const int wid = threadIdx.x >> 6;
     const int lane = lane_id();
     uint32_t* key = s_key[wid];
@@ -274,24 +376,23 @@ __global__ __launch_bounds__(TCB) void k_tc_count(const uint32_t* __restrict__ o
         // every wave takes part (an empty list stages nothing).
         const uint32_t sdu = staged ? du : 0;
         const uint32_t hmask = tc_stage(otgt, a, sdu, lane, 64, key, cnt);
-        unsigned long long cu = tc_item(ooff, otgt, a, du, s0, s1, 0, 64, staged, key, cnt, hmask, tri);
-        cu = wave_sum_u64(cu);
+        const unsigned long long cu = tc_item(ooff, otgt, a, du, s0, s1, 0, 64, staged, key, cnt, hmask, tri);
         if (lane == 0 && cu) atomicAdd(&tri[u], cu);
-        tc_flush(key, cnt, hmask + 1, lane, 64, tri);
+        tc_flush(key, cnt, (hmask + 1) * TC_BUCKET, lane, 64, tri);
     }
 }
 
 // One workgroup per item whose list has STAGE_CAP < |N+(u)| <= BIG_CAP
-// entries: the list is staged once in the group's LDS and the four waves
-// share the item's slots.
-__global__ __launch_bounds__(TCB) void k_tc_count_big(const uint32_t* __restrict__ ooff,
-                                                      const uint32_t* __restrict__ otgt,
-                                                      const uint32_t* __restrict__ istart,
-                                                      const uint32_t* __restrict__ iend,
-                                                      const uint32_t* __restrict__ iu, uint64_t nwork,
-                                                      unsigned long long* __restrict__ tri) {
-    __shared__ uint32_t b_key[2 * BIG_CAP];
-    __shared__ uint32_t b_cnt[2 * BIG_CAP];
+// entries: the list is staged once in the group's LDS and the group's TCB_BIG
+// / 64 waves share the item's slots.
+__global__ __launch_bounds__(TCB_BIG) void k_tc_count_big(const uint32_t* __restrict__ ooff,
+                                                             const uint32_t* __restrict__ otgt,
+                                                             const uint32_t* __restrict__ istart,
+                                                             const uint32_t* __restrict__ iend,
+                                                             const uint32_t* __restrict__ iu, uint64_t nwork,
+                                                             unsigned long long* __restrict__ tri) {
+    __shared__ __align__(16) uint32_t b_key[2 * BIG_CAP];
+    __shared__ uint32_t b_cnt[BIG_CAP];
     const int wid = threadIdx.x >> 6;
     const int lane = lane_id();
     for (uint64_t wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
@@ -299,11 +400,11 @@ __global__ __launch_bounds__(TCB) void k_tc_count_big(const uint32_t* __restrict
         const uint32_t a = ooff[u];
         const uint32_t du = ooff[u + 1] - a;
         const uint32_t s0 = istart[wi] - a, s1 = iend[wi] - a;
-        const uint32_t hmask = tc_stage(otgt, a, du, threadIdx.x, TCB, b_key, b_cnt);
-        unsigned long long cu = tc_item(ooff, otgt, a, du, s0, s1, 64u * wid, TCB, true, b_key, b_cnt, hmask, tri);
-        cu = wave_sum_u64(cu);
+        const uint32_t hmask = tc_stage(otgt, a, du, threadIdx.x, TCB_BIG, b_key, b_cnt);
+        const unsigned long long cu =
+            tc_item(ooff, otgt, a, du, s0, s1, 64u * wid, TCB_BIG, true, b_key, b_cnt, hmask, tri);
         if (lane == 0 && cu) atomicAdd(&tri[u], cu);
-        tc_flush(b_key, b_cnt, hmask + 1, threadIdx.x, TCB, tri);
+        tc_flush(b_key, b_cnt, (hmask + 1) * TC_BUCKET, threadIdx.x, TCB_BIG, tri);
     }
 }
 
@@ -411,7 +512,7 @@ void triangle_count(ursa_hip_topo* t, uint64_t* out_user) {
         auto* tri = reinterpret_cast<unsigned long long*>(out.p);
         timed("triangle_count", bytes, s, [&] {
             if (t->tc_nbig)
-                k_tc_count_big<<<(unsigned)std::min<uint64_t>(t->tc_nbig, 1u << 16), TCB, 0, s>>>(
+                k_tc_count_big<<<(unsigned)std::min<uint64_t>(t->tc_nbig, 1u << 16), TCB_BIG, 0, s>>>(
                     t->tc_off.get(), t->tc_tgt.get(), t->tc_bstart.get(), t->tc_bend.get(), t->tc_bu.get(),
                     t->tc_nbig, tri);
             if (t->tc_nwork)
