@@ -57,6 +57,11 @@ def traffic_from_profiles(workload: str, kernel: str):
         return None
 
 
+def traffic_total(t):
+    """HBM bytes per launch (FETCH + WRITE, calibrated) from a traffic.json entry, or None."""
+    return None if not t else t.get("total_bytes")
+
+
 class Dev:
     """A device buffer owned by the library's allocator."""
 
@@ -193,7 +198,7 @@ def bench_c2(args, ua, gen):
         ms, launches, b = stats[dom]
         per_launch = b / max(launches, 1)
         avg_us = ms / max(launches, 1) * 1e3
-        kname = {"cc": "k_cc_* (Afforest)", "triangle": "k_tc_count"}[dom]
+        kname = {"cc": "k_cc_* (Afforest)", "triangle": "k_tc_count + k_tc_count_big"}[dom]
     achieved = per_launch / (avg_us / 1e6) / 1e9
     roofline = {
         "bound": "hbm",
@@ -201,7 +206,8 @@ def bench_c2(args, ua, gen):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic_from_profiles(f"c2_rmat{scale}", dom),
+        "traffic": traffic_total(traffic_from_profiles(f"c2_rmat{scale}", dom)),
+        "traffic_source": (traffic_from_profiles(f"c2_rmat{scale}", dom) or {}).get("source"),
         "kernel": kname,
         "avg_launch_us": round(avg_us, 2),
         "algo_bytes_per_launch": per_launch,
