@@ -37,6 +37,17 @@ uint64_t ursa_hip_profile_names(char* buf, uint64_t cap);
  * 2 = 64-lane multi-source BFS. All modes give identical levels. */
 int ursa_hip_bfs_ex(ursa_hip_topo* t, const uint32_t* sources, uint32_t k, int direction,
                     int32_t* levels, uint64_t* reached_edges, int mode);
+/* Traversal schedule autotuner. Times the BFS direction switch (alpha, beta in
+ * {4, 15, 60} x {6, 18, 54}) on 4 single-source traversals and the multi-source
+ * push rule (divisor in {5, 20, 80}) on one 64-source batch of this graph, and
+ * keeps the fastest for `direction` on the topology. The choice is also cached
+ * under a key of the forward degree distribution (log2 n, mean degree, max/mean
+ * degree, share of zero-degree vertices); a later graph with the same key takes
+ * it without timing unless `force`. *source: 1 measured, 2 from the key cache.
+ * Results of every traversal are identical under any parameters. */
+int ursa_hip_autotune(ursa_hip_topo* t, int direction, int force, double* alpha, double* beta,
+                      uint32_t* push_div, int* source);
+
 /* As ursa_hip_betweenness_sources with an explicit batch size (1..64 sources
  * advancing together; 1 = one traversal per launch sequence) and the TEPS
  * numerator (sum of out-degrees of reached vertices over all sources). */
@@ -47,6 +58,11 @@ int ursa_hip_betweenness_sources_ex(ursa_hip_topo* t, const uint32_t* sources, u
 /* The reference's seeded permutation of 0..n-1 (shuffled_order, rng.rs:64-73),
  * used to pick sampled sources. Host code; out is u32[n]. */
 int ursa_hip_shuffled_order(uint64_t n, uint64_t seed, uint32_t* out);
+
+/* Fixed launch sequences (8-iteration batches of PageRank and personalized
+ * PageRank) are recorded once as HIP graphs and replayed (default on). With 0
+ * they are launched one by one; results are identical either way. */
+int ursa_hip_graphs_enable(int on);
 
 /* ---- mixed-precision PageRank ----
  * Same contract as ursa_hip_pagerank (pagerank.rs:98-169). The first sweeps

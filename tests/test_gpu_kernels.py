@@ -286,3 +286,27 @@ def test_betweenness_goldens(ua):  # betweenness.rs:369-439
     t = ua.Topology.build(4, [0, 0, 0, 1, 2], [1, 1, 2, 3, 3])  # parallel edge counts twice
     bc = ua.betweenness(t)
     assert abs(bc[1] - 2.0 / 3.0) < 1e-15 and abs(bc[2] - 1.0 / 3.0) < 1e-15
+
+
+# ---------------------------------------------------------------- autotuner
+def test_autotune_changes_schedule_not_results(ua):
+    n, s, d = rmat_dense(13, 9)
+    t = ua.Topology.build(n, s, d)
+    srcs = np.arange(0, n, max(1, n // 70))[:70]
+    o = orc.Topology(n, s, d)
+    for direction in ("out", "both"):
+        before1 = ua.bfs_multi(t, srcs[:3], direction, mode=1)
+        before2 = ua.bfs_multi(t, srcs, direction, mode=2)
+        p = ua.autotune(t, direction, force=True)
+        assert p["source"] == "measured"
+        assert p["alpha"] in (4.0, 15.0, 60.0) and p["beta"] in (6.0, 18.0, 54.0)
+        assert p["push_div"] in (5, 20, 80)
+        assert np.array_equal(ua.bfs_multi(t, srcs[:3], direction, mode=1), before1)
+        assert np.array_equal(ua.bfs_multi(t, srcs, direction, mode=2), before2)
+        assert np.array_equal(before1[0], orc.bfs(o, int(srcs[0]), direction))
+        # A second graph with the same degree distribution reuses the choice.
+        t2 = ua.Topology.build(n, s, d)
+        q = ua.autotune(t2, direction)
+        assert q["source"] == "cached" and (q["alpha"], q["beta"], q["push_div"]) == (
+            p["alpha"], p["beta"], p["push_div"])
+        assert np.array_equal(ua.bfs_multi(t2, srcs, direction, mode=2), before2)

@@ -235,3 +235,27 @@ def test_server_rejects_bad_queries(ua, sv, pool):
         bad.direction = 7
         res = srv.run([sv.ppr_query(3, [1]), bad, sv.triangles_query(0)])
         assert res[0].status == ua._lib.EINVAL and res[1].status == ua._lib.EINVAL and res[2].status == 0
+
+
+def test_graph_replay_matches_direct_launches(ua):
+    # The 8-sweep batches of PageRank and personalized PageRank are replayed as
+    # HIP graphs; each replay must equal the directly launched sequence, with
+    # other eager work in between.
+    n, s, d = rmat_dense(12, 8)
+    t = ua.Topology.build(n, s, d)
+    lib = ua.load()
+    params = ua.PageRankParams(0.85, 100, 1e-12)
+    seeds = [[1, 2, 3], [n - 1], list(range(0, n, n // 9))]
+    try:
+        lib.ursa_hip_graphs_enable(0)
+        pr_ref = ua.pagerank(t, params)
+        ppr_ref, it_ref = ua.ppr_batch(t, seeds, 0.85, 100, 1e-10)
+        lib.ursa_hip_graphs_enable(1)
+        for _ in range(3):
+            assert np.array_equal(ua.pagerank(t, params), pr_ref)
+            ua.bfs_distances(t, 0)
+            out, it = ua.ppr_batch(t, seeds, 0.85, 100, 1e-10)
+            assert np.array_equal(out, ppr_ref) and np.array_equal(it, it_ref)
+            ua.triangle_count(t)
+    finally:
+        lib.ursa_hip_graphs_enable(1)

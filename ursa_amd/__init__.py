@@ -34,6 +34,7 @@ __all__ = [
     "ppr_batch",
     "bfs_distances",
     "bfs_multi",
+    "autotune",
     "k_hop",
     "shortest_path",
     "betweenness",
@@ -232,6 +233,18 @@ def bfs_multi(topo: Topology, sources, direction=Direction.Out, mode: int = 0) -
     if k and n:
         check(load().ursa_hip_bfs_ex(topo._h, _ptr(src), k, _dir(direction), _ptr(out), C.byref(re), mode))
     return out[:k, :n]
+
+
+def autotune(topo: Topology, direction=Direction.Out, force: bool = False) -> dict:
+    """Pick the traversal schedule (BFS direction switch, multi-source push rule) for
+    this graph by timing candidates, or reuse the choice cached for its degree
+    distribution. Results of every traversal are unchanged."""
+    a, b = C.c_double(0), C.c_double(0)
+    pd, src = C.c_uint32(0), C.c_int(0)
+    check(load().ursa_hip_autotune(topo._h, _dir(direction), int(force), C.byref(a), C.byref(b), C.byref(pd),
+                                   C.byref(src)))
+    return {"alpha": a.value, "beta": b.value, "push_div": pd.value,
+            "source": {0: "default", 1: "measured", 2: "cached"}[src.value]}
 
 
 def bfs_distances(topo: Topology, source: int, direction=Direction.Out) -> np.ndarray:

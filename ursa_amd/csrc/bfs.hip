@@ -22,6 +22,11 @@
 // Algorithmic bytes per source: 4 B per inspected edge (E_reach = sum of the
 // forward degrees of reached vertices) + 12 B per node (offsets, level write,
 // level/visited read).
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <vector>
+
 #include "frontier.hpp"
 
 namespace ursa {
@@ -255,7 +260,7 @@ struct BfsWork {
 // number of levels run; *reached_edges = sum of forward degrees of reached
 // vertices (the TEPS numerator).
 static void bfs_one(ursa_hip_topo* t, uint32_t src, const NbrView& fwd, const NbrView& rev, uint64_t fwd_edges,
-                    int32_t* level, BfsWork& w, hipStream_t s, uint64_t* reached_edges) {
+                    int32_t* level, BfsWork& w, hipStream_t s, uint64_t* reached_edges, double alpha, double beta) {
     const uint64_t n = t->n;
     const unsigned gb = blocks_for((n + 63) / 64, WAVES, 1u << 16);
     auto read_ctr = [&]() {
@@ -275,7 +280,6 @@ static void bfs_one(ursa_hip_topo* t, uint32_t src, const NbrView& fwd, const Nb
     uint64_t total_edges = scout;
     int64_t edges_to_check = (int64_t)fwd_edges;
     int32_t L = 1;
-    const double alpha = 15.0, beta = 18.0;
     const double bytes_node = 12.0 * (double)n;
     while (qlen > 0) {
         if ((double)scout > (double)edges_to_check / alpha) {
@@ -329,6 +333,54 @@ static void bfs_one(ursa_hip_topo* t, uint32_t src, const NbrView& fwd, const Nb
 void msbfs_levels(ursa_hip_topo* t, const std::vector<uint32_t>& src, int dir, int32_t* out_dev,
                   uint64_t* reached_edges, uint32_t push_div);
 
+namespace {
+
+std::mutex g_tune_mu;
+std::map<uint64_t, TuneParams> g_tune_cache;  // degree-distribution key -> parameters
+
+// Key of the forward degree distribution: log2 n, 2 log2 of the mean degree,
+// log2 of max/mean degree, eighths of zero-degree vertices, and the direction.
+uint64_t degree_key(ursa_hip_topo* t, int dir, const NbrView& fwd, hipStream_t s) {
+    if (t->tune_key[dir]) return t->tune_key[dir];
+    const uint64_t n = t->n;
+    std::vector<uint32_t> a(n + 1), b;
+    HIP_OK(hipMemcpyAsync(a.data(), fwd.off_a, (n + 1) * 4, hipMemcpyDeviceToHost, s));
+    if (fwd.off_b) {
+        b.resize(n + 1);
+        HIP_OK(hipMemcpyAsync(b.data(), fwd.off_b, (n + 1) * 4, hipMemcpyDeviceToHost, s));
+    }
+    HIP_OK(hipStreamSynchronize(s));
+    uint64_t zero = 0, maxd = 0, tot = 0;
+    for (uint64_t u = 0; u < n; ++u) {
+        uint64_t d = a[u + 1] - a[u];
+        if (!b.empty()) d += b[u + 1] - b[u];
+        zero += d == 0;
+        maxd = std::max(maxd, d);
+        tot += d;
+    }
+    const double mean = n ? (double)tot / (double)n : 0.0;
+    auto lg = [](double x) { return x > 1.0 ? (uint64_t)std::lround(std::log2(x)) : 0ull; };
+    const uint64_t k = (lg((double)n) & 0xFF) | ((lg(mean * mean) & 0xFF) << 8) |
+                       ((lg(mean > 0 ? (double)maxd / mean : 0.0) & 0xFF) << 16) |
+                       (((uint64_t)(n ? 8 * zero / n : 0) & 0xF) << 24) | ((uint64_t)dir << 28) | (1ull << 32);
+    t->tune_key[dir] = k;
+    return k;
+}
+
+}  // namespace
+
+// The parameters a traversal in direction dir uses: measured on this graph,
+// else cached for its degree-distribution key, else the defaults.
+TuneParams effective_tune(ursa_hip_topo* t, int dir) {
+    if (t->tune[dir].source != 0 || t->tune_key[dir] == 0) return t->tune[dir];
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(t->tune_key[dir]);
+    if (it == g_tune_cache.end()) return t->tune[dir];
+    TuneParams p = it->second;
+    p.source = 2;
+    return p;
+}
+
 // mode: 0 = automatic (64-lane multi-source BFS for 4 or more sources),
 // 1 = one direction-optimizing traversal per source, 2 = multi-source.
 void bfs_levels(ursa_hip_topo* t, const uint32_t* sources, uint64_t k, int dir, int32_t* out_user,
@@ -348,9 +400,10 @@ void bfs_levels(ursa_hip_topo* t, const uint32_t* sources, uint64_t k, int dir, 
     else
         std::copy(sources, sources + k, src.begin());
     const bool multi = mode == 2 || (mode == 0 && k >= 4);
+    const TuneParams tp = effective_tune(t, dir);
     if (multi) {
         uint64_t re = 0;
-        msbfs_levels(t, src, dir, out.p, reached_edges ? &re : nullptr, 20);
+        msbfs_levels(t, src, dir, out.p, reached_edges ? &re : nullptr, tp.ms_push_div);
         out.finish(s);
         if (reached_edges) *reached_edges = re;
         return;
@@ -364,11 +417,93 @@ void bfs_levels(ursa_hip_topo* t, const uint32_t* sources, uint64_t k, int dir, 
             continue;
         }
         uint64_t re = 0;
-        bfs_one(t, src[i], fwd, rev, fwd_edges, lv, w, s, &re);
+        bfs_one(t, src[i], fwd, rev, fwd_edges, lv, w, s, &re, tp.bfs_alpha, tp.bfs_beta);
         sum += re;
     }
     out.finish(s);
     if (reached_edges) *reached_edges = sum;
+}
+
+// Empirical schedule search on this graph: the direction-switch pair (alpha,
+// beta) over a 3 x 3 grid, timed on 4 single-source traversals, and the
+// multi-source push rule over 3 divisors, timed on one 64-source batch. The
+// winner is kept on the topology and cached for its degree-distribution key,
+// so a graph with the same key reuses it without timing.
+TuneParams autotune(ursa_hip_topo* t, int dir, bool force) {
+    const uint64_t n = t->n;
+    hipStream_t s = stream();
+    NbrView fwd, rev;
+    direction_views(t, dir, s, fwd, rev);
+    const uint64_t key = degree_key(t, dir, fwd, s);
+    if (!force) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            t->tune[dir] = it->second;
+            t->tune[dir].source = 2;
+            return t->tune[dir];
+        }
+    }
+    TuneParams best;
+    if (n == 0 || t->m == 0) return best;
+    const uint64_t fwd_edges = dir == URSA_HIP_BOTH ? 2 * t->m : t->m;
+    // Sources: vertices with forward edges, picked by a fixed stride.
+    std::vector<uint32_t> a(n + 1);
+    HIP_OK(hipMemcpyAsync(a.data(), fwd.off_a, (n + 1) * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    std::vector<uint32_t> cand;
+    for (uint64_t u = 0; u < n && cand.size() < 64; u += std::max<uint64_t>(1, n / 257))
+        if (a[u + 1] > a[u] || fwd.off_b) cand.push_back((uint32_t)u);
+    if (cand.empty()) return best;
+    DevBuf<int32_t> lv(n * 64);
+    BfsWork w(n);
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    auto time_it = [&](auto&& body) {
+        HIP_OK(hipEventRecord(e0, s));
+        body();
+        HIP_OK(hipEventRecord(e1, s));
+        HIP_OK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        return ms;
+    };
+    const double alphas[3] = {4.0, 15.0, 60.0}, betas[3] = {6.0, 18.0, 54.0};
+    const size_t ns = std::min<size_t>(4, cand.size());
+    float best_ms = 1e30f;
+    bfs_one(t, cand[0], fwd, rev, fwd_edges, lv.get(), w, s, nullptr, 15.0, 18.0);  // warm-up
+    for (double al : alphas)
+        for (double be : betas) {
+            const float ms = time_it([&] {
+                for (size_t i = 0; i < ns; ++i)
+                    bfs_one(t, cand[i], fwd, rev, fwd_edges, lv.get() + i * n, w, s, nullptr, al, be);
+            });
+            if (ms < best_ms) {
+                best_ms = ms;
+                best.bfs_alpha = al;
+                best.bfs_beta = be;
+            }
+        }
+    if (cand.size() >= 4) {
+        const uint32_t divs[3] = {5, 20, 80};
+        float best_ms2 = 1e30f;
+        msbfs_levels(t, cand, dir, lv.get(), nullptr, 20);  // warm-up
+        for (uint32_t dv : divs) {
+            const float ms = time_it([&] { msbfs_levels(t, cand, dir, lv.get(), nullptr, dv); });
+            if (ms < best_ms2) {
+                best_ms2 = ms;
+                best.ms_push_div = dv;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    best.source = 1;
+    t->tune[dir] = best;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = best;
+    return best;
 }
 
 }  // namespace ursa
@@ -382,6 +517,19 @@ int ursa_hip_bfs(ursa_hip_topo* t, const uint32_t* sources, uint32_t k, int dire
         if (t->n == 0 || k == 0) return;
         URSA_REQUIRE(sources != nullptr && levels != nullptr, URSA_HIP_EINVAL, "sources/levels is NULL");
         ursa::bfs_levels(t, sources, k, direction, levels, nullptr, 0);
+    });
+}
+
+int ursa_hip_autotune(ursa_hip_topo* t, int direction, int force, double* alpha, double* beta, uint32_t* push_div,
+                      int* source) {
+    return ursa::guard([&] {
+        URSA_REQUIRE(t != nullptr, URSA_HIP_EINVAL, "topology is NULL");
+        URSA_REQUIRE(direction >= 0 && direction <= 2, URSA_HIP_EINVAL, "bad direction");
+        const ursa::TuneParams p = ursa::autotune(t, direction, force != 0);
+        if (alpha) *alpha = p.bfs_alpha;
+        if (beta) *beta = p.bfs_beta;
+        if (push_div) *push_div = p.ms_push_div;
+        if (source) *source = p.source;
     });
 }
 

@@ -223,6 +223,50 @@ inline void timed(const char* name, double bytes, hipStream_t s, L&& launch) {
     prof_push(name, a, b, bytes, s);
 }
 
+// HIP graph replay of fixed launch sequences (ursa_hip_graphs_enable).
+bool graphs_enabled();
+
+// Records the launches `record()` makes on stream s the first time run() is
+// called (stream capture, thread-local mode) and replays the instantiated
+// graph on every call. Falls back to launching directly while graphs are
+// disabled or kernel timing is on. The recorded launches must not allocate,
+// synchronise or clear memory with hipMemsetAsync.
+class CapturedLaunches {
+   public:
+    CapturedLaunches() = default;
+    CapturedLaunches(const CapturedLaunches&) = delete;
+    CapturedLaunches& operator=(const CapturedLaunches&) = delete;
+    ~CapturedLaunches() {
+        if (exec_) (void)hipGraphExecDestroy(exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+    }
+    template <class F>
+    void run(hipStream_t s, F&& record) {
+        if (!graphs_enabled() || prof_enabled()) {
+            record();
+            return;
+        }
+        if (!exec_) {
+            HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                record();
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(s, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIP_OK(hipStreamEndCapture(s, &graph_));
+            HIP_OK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+        }
+        HIP_OK(hipGraphLaunch(exec_, s));
+    }
+
+   private:
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+};
+
 inline unsigned blocks_for(uint64_t n, unsigned block, unsigned cap = 1u << 20) {
     uint64_t b = (n + block - 1) / block;
     if (b == 0) b = 1;

@@ -55,6 +55,7 @@ struct ProfAgg {
     double bytes = 0.0;
 };
 std::atomic<bool> g_prof_on{false};
+std::atomic<bool> g_graphs_on{true};
 std::mutex g_prof_mu;
 std::vector<ProfRec> g_prof_pending;
 std::map<std::string, ProfAgg> g_prof_agg;
@@ -209,6 +210,7 @@ bool is_device_ptr(const void* p) {
 }
 
 bool prof_enabled() { return g_prof_on.load(std::memory_order_relaxed); }
+bool graphs_enabled() { return g_graphs_on.load(std::memory_order_relaxed); }
 
 void prof_push(const char* name, hipEvent_t a, hipEvent_t b, double bytes, hipStream_t) {
     std::lock_guard<std::mutex> lk(g_prof_mu);
@@ -253,6 +255,11 @@ int ursa_hip_synchronize(void) {
 
 int ursa_hip_profile_enable(int on) {
     g_prof_on.store(on != 0);
+    return URSA_HIP_OK;
+}
+
+int ursa_hip_graphs_enable(int on) {
+    g_graphs_on.store(on != 0);
     return URSA_HIP_OK;
 }
 

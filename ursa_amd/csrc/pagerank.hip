@@ -322,8 +322,8 @@ __global__ __launch_bounds__(256) void k_pr_fixup(
 
 __global__ __launch_bounds__(1024) void k_pr_reduce(const double* __restrict__ blk_delta,
                                                     const double* __restrict__ blk_dm, uint64_t count, PrCtl* ctl,
-                                                    int parity, uint32_t iter, uint32_t max_iter, double tol,
-                                                    double damping, double nf) {
+                                                    int parity, uint32_t max_iter, double tol, double damping,
+                                                    double nf) {
     __shared__ double s_red[1024 / 64];
     if (ctl->done) return;
     double d = 0.0, m = 0.0;
@@ -334,10 +334,11 @@ __global__ __launch_bounds__(1024) void k_pr_reduce(const double* __restrict__ b
     const double delta = block_sum(d, s_red);
     const double dm = block_sum(m, s_red);
     if (threadIdx.x == 0) {
+        const uint32_t iters = ctl->iters + 1;  // the iteration count lives on the device
         ctl->delta = delta;
-        ctl->iters = iter + 1;
+        ctl->iters = iters;
         ctl->base[parity ^ 1] = (1.0 - damping) / nf + damping * dm / nf;
-        if (delta < tol || iter + 1 >= max_iter) ctl->done = 1;
+        if (delta < tol || iters >= max_iter) ctl->done = 1;
     }
 }
 
@@ -652,8 +653,8 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
         check_launch();
     };
     auto reduce = [&](uint32_t it, double tol_it) {
-        k_pr_reduce<<<1, 1024, 0, s>>>(fd.get(), fm.get(), fix_blocks, ctl.get(), (int)(it & 1), it,
-                                       max_iter, tol_it, damping, nf);
+        k_pr_reduce<<<1, 1024, 0, s>>>(fd.get(), fm.get(), fix_blocks, ctl.get(), (int)(it & 1), max_iter, tol_it,
+                                       damping, nf);
         check_launch();
     };
     auto read_ctl = [&]() {
@@ -697,14 +698,24 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
     }
     PrCtl h = read_ctl();
     // Iterations are queued in batches; with per-kernel timing on, one at a
-    // time so no launch after convergence is booked.
+    // time so no launch after convergence is booked. A full batch of 8 is
+    // recorded once per start parity as a HIP graph and replayed.
     const uint32_t batch = prof_enabled() ? 1 : 8;
+    CapturedLaunches graphs[2];
     while (it < max_iter && !h.done) {
         const uint32_t it_end = std::min<uint32_t>(max_iter, it + batch);
-        for (; it < it_end; ++it) {
-            iteration(it, false);
-            reduce(it, tol);
-        }
+        const uint32_t it0 = it;
+        auto launches = [&] {
+            for (uint32_t i = it0; i < it_end; ++i) {
+                iteration(i, false);
+                reduce(i, tol);
+            }
+        };
+        if (it_end - it0 == 8)
+            graphs[it0 & 1].run(s, launches);
+        else
+            launches();
+        it = it_end;
         h = read_ctl();
     }
     h = read_ctl();

@@ -197,6 +197,10 @@ __global__ __launch_bounds__(PB) void k_ppr_rows(uint64_t n, const uint32_t* __r
                                                  const uint32_t* __restrict__ send, double* __restrict__ spart) {
     int ph[KP];
     load_phases<KP>(lanes, ph);
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < KP; ++q) any |= ph[q] != 3;
+    if (!any) return;  // every query has finished (a queued sweep after the last one)
     float s[KP], c[KP];
     double a[KP];
     const uint32_t g = threadIdx.x % PPR_G, grp = threadIdx.x / PPR_G;
@@ -237,6 +241,10 @@ __global__ __launch_bounds__(PB) void k_ppr_heavy(uint64_t H, const uint32_t* __
                                                   double* __restrict__ r64, double* __restrict__ part) {
     int ph[KP];
     load_phases<KP>(lanes, ph);
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < KP; ++q) any |= ph[q] != 3;
+    if (!any) return;
     double dl[KP], dm[KP];
 #pragma unroll
     for (int q = 0; q < KP; ++q) dl[q] = dm[q] = 0.0;
@@ -367,31 +375,40 @@ void ppr_iterate(ursa_hip_topo* t, const DevCsr& in, PprBufs& B, uint32_t max_it
     const uint32_t nparts = light_blocks + heavy_blocks;
     const double edge_bytes = 4.0 + 4.0 * KP;
     uint32_t it = 0;
+    CapturedLaunches graph;  // a full batch always starts at an even sweep
     while (it < max_iter_all) {
         const uint32_t end = std::min<uint32_t>(max_iter_all, it + 8);
-        for (; it < end; ++it) {
-            const int p = (int)(it & 1);
-            const float* c32 = p ? B.c32b.get() : B.c32a.get();
-            float* c32n = p ? B.c32a.get() : B.c32b.get();
-            const double* c64 = p ? B.c64b.get() : B.c64a.get();
-            double* c64n = p ? B.c64a.get() : B.c64b.get();
-            timed("ppr_rows", edge_bytes * (double)t->m + (16.0 + 8.0 * KP) * (double)n, s, [&] {
-                k_ppr_rows<KP><<<light_blocks + seg_blocks, PB, 0, s>>>(
-                    n, in.off.get(), in.tgt.get(), t->out_deg.get(), B.seedbits.get(), B.lanes.get(), c32, c32n,
-                    B.r32.get(), c64, c64n, B.r64.get(), B.part.get(), light_blocks, t->ppr_S, t->ppr_sbeg.get(),
-                    t->ppr_send.get(), B.spart.get());
-            });
-            check_launch();
-            if (t->ppr_H) {
-                k_ppr_heavy<KP><<<heavy_blocks, PB, 0, s>>>(t->ppr_H, t->ppr_hrow.get(), t->ppr_hsoff.get(),
-                                                            B.spart.get(), t->out_deg.get(), B.seedbits.get(),
-                                                            B.lanes.get(), c32n, B.r32.get(), c64n, B.r64.get(),
-                                                            B.part.get() + (uint64_t)light_blocks * KP * 2);
+        const uint32_t it0 = it;
+        auto launches = [&] {
+            for (uint32_t i = it0; i < end; ++i) {
+                const int p = (int)(i & 1);
+                const float* c32 = p ? B.c32b.get() : B.c32a.get();
+                float* c32n = p ? B.c32a.get() : B.c32b.get();
+                const double* c64 = p ? B.c64b.get() : B.c64a.get();
+                double* c64n = p ? B.c64a.get() : B.c64b.get();
+                timed("ppr_rows", edge_bytes * (double)t->m + (16.0 + 8.0 * KP) * (double)n, s, [&] {
+                    k_ppr_rows<KP><<<light_blocks + seg_blocks, PB, 0, s>>>(
+                        n, in.off.get(), in.tgt.get(), t->out_deg.get(), B.seedbits.get(), B.lanes.get(), c32,
+                        c32n, B.r32.get(), c64, c64n, B.r64.get(), B.part.get(), light_blocks, t->ppr_S,
+                        t->ppr_sbeg.get(), t->ppr_send.get(), B.spart.get());
+                });
+                check_launch();
+                if (t->ppr_H) {
+                    k_ppr_heavy<KP><<<heavy_blocks, PB, 0, s>>>(t->ppr_H, t->ppr_hrow.get(), t->ppr_hsoff.get(),
+                                                                B.spart.get(), t->out_deg.get(), B.seedbits.get(),
+                                                                B.lanes.get(), c32n, B.r32.get(), c64n, B.r64.get(),
+                                                                B.part.get() + (uint64_t)light_blocks * KP * 2);
+                    check_launch();
+                }
+                k_ppr_reduce<KP><<<1, PB, 0, s>>>(B.part.get(), nparts, B.lanes.get(), B.ctl.get());
                 check_launch();
             }
-            k_ppr_reduce<KP><<<1, PB, 0, s>>>(B.part.get(), nparts, B.lanes.get(), B.ctl.get());
-            check_launch();
-        }
+        };
+        if (end - it0 == 8)
+            graph.run(s, launches);
+        else
+            launches();
+        it = end;
         PprCtl* h = static_cast<PprCtl*>(pinned_scratch(sizeof(PprCtl)));
         HIP_OK(hipMemcpyAsync(h, B.ctl.get(), sizeof(PprCtl), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));

@@ -37,6 +37,17 @@ void offsets_from_sorted(const uint32_t* keys, uint64_t m, uint64_t n, uint32_t*
 
 }  // namespace ursa
 
+namespace ursa {
+// Traversal schedule parameters picked by the autotuner (bfs.hip). They change
+// only which kernel runs when, never a result.
+struct TuneParams {
+    double bfs_alpha = 15.0;   // top-down -> bottom-up when frontier edges > unexplored / alpha
+    double bfs_beta = 18.0;    // bottom-up -> top-down when the frontier < n / beta
+    uint32_t ms_push_div = 20; // multi-source push when frontier edges * div < all edges
+    int source = 0;            // 0 defaults, 1 measured on this graph, 2 taken from the key cache
+};
+}  // namespace ursa
+
 struct ursa_hip_topo {
     uint64_t n = 0, m = 0;
     ursa::DevCsr out, in, und;
@@ -61,6 +72,9 @@ struct ursa_hip_topo {
     ursa::DevBuf<uint32_t> ppr_hrow, ppr_hsoff, ppr_sbeg, ppr_send;
     uint64_t ppr_H = 0, ppr_S = 0, ppr_dangling = 0;
     bool ppr_ready = false;
+
+    ursa::TuneParams tune[3];  // per traversal direction
+    uint64_t tune_key[3] = {0, 0, 0};
 
     const ursa::DevCsr& incoming(hipStream_t s);
     const ursa::DevCsr& undirected(hipStream_t s);
