@@ -47,6 +47,10 @@ int ursa_hip_shutdown(void);
 const char* ursa_hip_last_error(void);
 /* Synchronise the calling thread's stream. */
 int ursa_hip_synchronize(void);
+/* Serving hook: run this thread's work on the caller's HIP stream (a
+ * hipStream_t, passed as void*), or on the thread's own non-blocking stream
+ * again with NULL. Calls still return only when their results are written. */
+int ursa_hip_set_stream(void* stream);
 
 /* ---- topology — ursa-core/src/topology.rs ---- */
 /* Topology::build (topology.rs:276-298): stable grouping of the m edge rows by
@@ -55,6 +59,14 @@ int ursa_hip_synchronize(void);
  * (topology.rs:61-64; URSA_HIP_ERANGE otherwise). */
 int ursa_hip_topology_create(uint64_t n, uint64_t m, const uint32_t* src_dense,
                              const uint32_t* dst_dense, ursa_hip_topo** out);
+/* A topology from an existing out-CSR (the Adjacency Topology::build produces,
+ * topology.rs:35-47): offsets u64[n+1] (0 first, m last, non-decreasing),
+ * targets u32[m] < n, edge_ids u32[m] a permutation of 0..m-1 (the original
+ * row of each slot). Host or device buffers; validated (URSA_HIP_EINVAL). The
+ * transpose and undirected view are derived from it as after create. */
+int ursa_hip_topology_create_from_csr(uint64_t n, uint64_t m, const uint64_t* offsets,
+                                      const uint32_t* targets, const uint32_t* edge_ids,
+                                      ursa_hip_topo** out);
 int ursa_hip_topology_destroy(ursa_hip_topo* t);
 int ursa_hip_topology_size(const ursa_hip_topo* t, uint64_t* n, uint64_t* m);
 /* Topology::incoming (topology.rs:343-348): build the transpose now. */

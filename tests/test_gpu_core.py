@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as orc
-from graphs import cases
+from graphs import cases, rmat_dense
 
 pytestmark = pytest.mark.gpu
 
@@ -179,3 +179,52 @@ def test_pagerank_mixed_rmat(ua):
     top_g = np.argsort(-got, kind="stable")[:1000]
     top_r = np.argsort(-ref, kind="stable")[:1000]
     assert len(set(top_g) & set(top_r)) >= 995
+
+
+# ---------------------------------------------------------------- from_csr / caller stream
+def test_topology_from_csr_matches_build(ua):
+    n, s, d = rmat_dense(11, 12)
+    t = ua.Topology.build(n, s, d)
+    off, tg, ei = t.out()
+    t2 = ua.Topology.from_csr(n, off, tg, ei)
+    for a, b in zip(t.out(), t2.out()):
+        assert np.array_equal(a, b)
+    for a, b in zip(t.incoming(), t2.incoming()):
+        assert np.array_equal(a, b)
+    for a, b in zip(t.undirected(), t2.undirected()):
+        assert np.array_equal(a, b)
+    p = ua.PageRankParams(0.85, 100, 1e-10)
+    assert np.array_equal(ua.pagerank(t, p), ua.pagerank(t2, p))
+    assert np.array_equal(ua.triangle_count(t), ua.triangle_count(t2))
+
+
+def test_topology_from_csr_rejects_bad_csr(ua):
+    off = np.array([0, 2, 1, 3], np.uint64)  # decreasing
+    with pytest.raises(ua.UrsaHipError) as e:
+        ua.Topology.from_csr(3, off, [1, 2, 0], [0, 1, 2])
+    assert e.value.code == ua._lib.EINVAL
+    with pytest.raises(ua.UrsaHipError):  # duplicate edge id
+        ua.Topology.from_csr(3, [0, 1, 2, 3], [1, 2, 0], [0, 0, 2])
+    with pytest.raises(ua.UrsaHipError):  # target out of range
+        ua.Topology.from_csr(3, [0, 1, 2, 3], [1, 5, 0], [0, 1, 2])
+
+
+def test_caller_stream(ua):
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")  # the HIP runtime the library already uses
+    n, s, d = rmat_dense(10, 13)
+    t = ua.Topology.build(n, s, d)
+    ref = ua.ppr(t, [1, 2], 0.85, 100, 1e-10)[0]
+    lib = ua.load()
+    st = C.c_void_p()
+    assert hip.hipStreamCreateWithFlags(C.byref(st), 1) == 0  # hipStreamNonBlocking
+    try:
+        ua.check(lib.ursa_hip_set_stream(st))
+        got = ua.ppr(t, [1, 2], 0.85, 100, 1e-10)[0]
+        lv = ua.bfs_distances(t, 0)
+    finally:
+        ua.check(lib.ursa_hip_set_stream(None))
+        hip.hipStreamDestroy(st)
+    assert np.array_equal(got, ref)
+    assert np.array_equal(lv, ua.bfs_distances(t, 0))

@@ -110,6 +110,20 @@ class Topology:
         check(lib.ursa_hip_topology_create(n, m, src_ptr, dst_ptr, C.byref(h)))
         return cls(h, n, m)
 
+    @classmethod
+    def from_csr(cls, n_nodes: int, offsets, targets, edge_ids) -> "Topology":
+        """A topology from an existing out-CSR (Adjacency, topology.rs:35-47): offsets
+        u64[n+1], targets u32[m], edge_ids u32[m] (a permutation of the edge rows)."""
+        off = np.ascontiguousarray(np.asarray(offsets, dtype=np.uint64))
+        tg = _u32(targets)
+        ei = _u32(edge_ids)
+        if off.shape[0] != int(n_nodes) + 1 or tg.shape != ei.shape:
+            raise ValueError("offsets must have n+1 entries and targets/edge_ids equal length")
+        h = C.c_void_p()
+        check(load().ursa_hip_topology_create_from_csr(int(n_nodes), tg.shape[0], _ptr(off), _ptr(tg), _ptr(ei),
+                                                       C.byref(h)))
+        return cls(h, int(n_nodes), int(tg.shape[0]))
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h and _lib._lib is not None:

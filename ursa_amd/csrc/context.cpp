@@ -24,6 +24,7 @@ struct ThreadStream {
 };
 thread_local ThreadStream t_stream;
 thread_local int t_current_device = -1;
+thread_local hipStream_t t_user_stream = nullptr;
 thread_local Arena* t_arena = nullptr;
 
 struct PinnedScratch {
@@ -115,6 +116,7 @@ int ensure_device() {
 
 hipStream_t stream() {
     int dev = ensure_device();
+    if (t_user_stream) return t_user_stream;
     if (!t_stream.s || t_stream.device != dev) {
         HIP_OK(hipStreamCreateWithFlags(&t_stream.s, hipStreamNonBlocking));
         t_stream.device = dev;
@@ -248,6 +250,13 @@ int ursa_hip_shutdown(void) {
 }
 
 const char* ursa_hip_last_error(void) { return t_last_error.c_str(); }
+
+int ursa_hip_set_stream(void* s) {
+    return guard([&] {
+        ensure_device();
+        t_user_stream = static_cast<hipStream_t>(s);
+    });
+}
 
 int ursa_hip_synchronize(void) {
     return guard([&] { HIP_OK(hipStreamSynchronize(stream())); });

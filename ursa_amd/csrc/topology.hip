@@ -263,6 +263,96 @@ int ursa_hip_topology_create(uint64_t n, uint64_t m, const uint32_t* src_dense, 
     });
 }
 
+}  // extern "C"
+
+namespace ursa {
+namespace {
+
+// Offsets u64 -> u32 with monotonicity, ends and target range checked.
+__global__ void k_csr_offsets(const uint64_t* __restrict__ off64, uint64_t n, uint64_t m, uint32_t* __restrict__ off,
+                              uint32_t* __restrict__ bad) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t v = off64[i];
+        if (v > m || (i == 0 && v != 0) || (i == n && v != m) || (i < n && off64[i + 1] < v)) atomicOr(bad, 1u);
+        off[i] = (uint32_t)v;
+    }
+}
+
+// Targets in range; edge ids counted (a permutation of 0..m-1 counts each once).
+__global__ void k_csr_slots(const uint32_t* __restrict__ tgt, const uint32_t* __restrict__ eid, uint64_t m,
+                            uint64_t n, uint32_t* __restrict__ seen, uint32_t* __restrict__ bad) {
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < m; k += (uint64_t)gridDim.x * blockDim.x) {
+        if (tgt[k] >= n || eid[k] >= m) {
+            atomicOr(bad, 2u);
+            continue;
+        }
+        atomicAdd(&seen[eid[k]], 1u);
+    }
+}
+
+__global__ void k_csr_perm(const uint32_t* __restrict__ seen, uint64_t m, uint32_t* __restrict__ bad) {
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < m; k += (uint64_t)gridDim.x * blockDim.x)
+        if (seen[k] != 1) atomicOr(bad, 4u);
+}
+
+}  // namespace
+}  // namespace ursa
+
+extern "C" {
+
+int ursa_hip_topology_create_from_csr(uint64_t n, uint64_t m, const uint64_t* offsets, const uint32_t* targets,
+                                      const uint32_t* edge_ids, ursa_hip_topo** out) {
+    return guard([&] {
+        URSA_REQUIRE(out != nullptr, URSA_HIP_EINVAL, "out is NULL");
+        *out = nullptr;
+        URSA_REQUIRE(m <= 0xFFFFFFFFull, URSA_HIP_ERANGE, "more than u32::MAX edges is beyond the edge-id cap");
+        URSA_REQUIRE(n <= 0xFFFFFFFFull, URSA_HIP_ERANGE, "more than u32::MAX nodes");
+        URSA_REQUIRE(offsets != nullptr, URSA_HIP_EINVAL, "offsets is NULL");
+        URSA_REQUIRE(m == 0 || (targets && edge_ids), URSA_HIP_EINVAL, "targets/edge_ids is NULL");
+        hipStream_t s = stream();
+        auto* t = new ursa_hip_topo;
+        try {
+            t->n = n;
+            t->m = m;
+            DevIn<uint64_t> off64(offsets, n + 1, s);
+            DevIn<uint32_t> tg(targets, m, s), ei(edge_ids, m, s);
+            t->out.off.alloc(n + 1);
+            t->out.tgt.alloc(m ? m : 1);
+            t->out.eid.alloc(m ? m : 1);
+            DevBuf<uint32_t> bad(1), seen(m ? m : 1);
+            HIP_OK(hipMemsetAsync(bad.get(), 0, sizeof(uint32_t), s));
+            k_csr_offsets<<<blocks_for(n + 1, B), B, 0, s>>>(off64.p, n, m, t->out.off.get(), bad.get());
+            check_launch();
+            if (m) {
+                HIP_OK(hipMemsetAsync(seen.get(), 0, m * sizeof(uint32_t), s));
+                k_csr_slots<<<blocks_for(m, B), B, 0, s>>>(tg.p, ei.p, m, n, seen.get(), bad.get());
+                k_csr_perm<<<blocks_for(m, B), B, 0, s>>>(seen.get(), m, bad.get());
+                check_launch();
+                HIP_OK(hipMemcpyAsync(t->out.tgt.get(), tg.p, m * 4, hipMemcpyDeviceToDevice, s));
+                HIP_OK(hipMemcpyAsync(t->out.eid.get(), ei.p, m * 4, hipMemcpyDeviceToDevice, s));
+            }
+            uint32_t hbad = 0;
+            HIP_OK(hipMemcpyAsync(&hbad, bad.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            HIP_OK(hipStreamSynchronize(s));
+            URSA_REQUIRE(!(hbad & 1u), URSA_HIP_EINVAL, "offsets must start at 0, end at m and not decrease");
+            URSA_REQUIRE(!(hbad & 2u), URSA_HIP_EINVAL, "target or edge id out of range");
+            URSA_REQUIRE(!(hbad & 4u), URSA_HIP_EINVAL, "edge_ids is not a permutation of 0..m-1");
+            t->out.slots = m;
+            t->out.built = true;
+            t->out_deg.alloc(n ? n : 1);
+            if (n) {
+                k_degree_of<<<blocks_for(n, B), B, 0, s>>>(t->out.off.get(), n, t->out_deg.get());
+                check_launch();
+            }
+            HIP_OK(hipStreamSynchronize(s));
+        } catch (...) {
+            delete t;
+            throw;
+        }
+        *out = t;
+    });
+}
+
 int ursa_hip_topology_destroy(ursa_hip_topo* t) {
     return guard([&] {
         if (!t) return;
