@@ -335,7 +335,7 @@ def bench_pagerank(args, ua, gen):
         t2 = time.time()
         _, _, sd, dd, _ = e.export()
         ot = orc.Topology(n, sd, dd)
-        orc.degree(ot, "in")
+        indeg = orc.degree(ot, "in")
         threads = os.cpu_count() or 1
         t3 = time.perf_counter()
         ref, rit, _ = orc.pagerank(ot, params.damping, params.max_iter, params.tol, nthreads=threads)
@@ -347,6 +347,9 @@ def bench_pagerank(args, ua, gen):
             res[name]["L1_vs_oracle"] = float(np.abs(g - ref).sum())
             res[name]["maxabs_vs_oracle"] = float(np.abs(g - ref).max())
             res[name]["max_rel_vs_oracle"] = float(rel.max())
+            w = int(np.argmax(rel))
+            res[name]["worst_node"] = {"in_degree": int(indeg[w]), "rank": float(ref[w]),
+                                       "nodes_above_1e-12": int((rel > 1e-12).sum())}
             res[name]["top1000_overlap"] = len(top_r & set(np.argsort(-g, kind="stable")[:1000].tolist()))
         res["cpu_oracle"] = {"GTEPS": round(m * rit / cpu_s / 1e9, 4), "iters": rit, "threads": threads}
     out.free()
