@@ -168,10 +168,22 @@ __device__ __forceinline__ void tc_credit_w(bool staged, int wslot, uint32_t w, 
         atomicAdd(&tri[w], 1ull);
 }
 
+// Bloom bit of x: a per-wave staged list also sets one bit per entry in a
+// bitmap of 8 bits per hash slot (64 per bucket), so most absent keys are
+// rejected with one 4-byte LDS read before any bucket is read.
+__device__ __forceinline__ uint32_t tc_bloom_bit(uint32_t x, uint32_t bmask) {
+    const uint32_t lgnb = 31 - __clz(bmask + 1);  // log2 of the bucket count
+    return (x * 0x85EBCA6Bu) >> (26 - lgnb);      // bitmap of 64 * bucket-count bits
+}
+
 // Is w in N+(u)? Sets wslot when staged.
-__device__ __forceinline__ bool tc_member(bool staged, const uint32_t* key, uint32_t hmask, const uint32_t* ulist,
-                                          uint32_t du, uint32_t w, int& wslot) {
+__device__ __forceinline__ bool tc_member(bool staged, const uint32_t* key, const uint32_t* bloom, uint32_t hmask,
+                                          const uint32_t* ulist, uint32_t du, uint32_t w, int& wslot) {
     if (staged) {
+        if (bloom) {
+            const uint32_t bit = tc_bloom_bit(w, hmask);
+            if (!((bloom[bit >> 5] >> (bit & 31)) & 1u)) return false;
+        }
         wslot = tc_probe(key, hmask, w);
         return wslot >= 0;
     }
@@ -195,7 +207,8 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ otgt, uint32_t a, uint32_t du,
                                                       uint32_t s0, uint32_t s1, uint32_t first, uint32_t step,
                                                       bool staged, const uint32_t* key, uint32_t* cnt,
-                                                      uint32_t hmask, unsigned long long* __restrict__ tri) {
+                                                      const uint32_t* bloom, uint32_t hmask,
+                                                      unsigned long long* __restrict__ tri) {
     const int lane = lane_id();
     const uint32_t* ulist = otgt + a;
     unsigned long long cu = 0;
@@ -246,7 +259,7 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
                 bool hit = false;
                 if (ok[r]) {
                     if (oa) {
-                        hit = tc_member(staged, key, hmask, ulist, du, w[r], wslot);
+                        hit = tc_member(staged, key, bloom, hmask, ulist, du, w[r], wslot);
                     } else {
                         const uint32_t osv = __builtin_amdgcn_readlane(sv, j);
                         const uint32_t odv = __builtin_amdgcn_readlane(dv, j);
@@ -295,7 +308,7 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
                 if (!ok[r]) {
                     // nothing to test
                 } else if (oa[r]) {
-                    hit = tc_member(staged, key, hmask, ulist, du, w[r], wslot);
+                    hit = tc_member(staged, key, bloom, hmask, ulist, du, w[r], wslot);
                 } else {
                     hit = sorted_contains(otgt + osvv[r], odvv[r], w[r]);
                     if (hit && staged) wslot = tc_probe(key, hmask, w[r]);
@@ -315,18 +328,25 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
 }
 
 // Stages N+(u) into an LDS hash set of 2^k >= du / 4 buckets (at least two
-// slots per entry) with zeroed counters, using `nthreads` threads. Ends with a
-// barrier; returns the bucket mask.
+// slots per entry) with zeroed counters and its bloom bitmap (h / 4 words),
+// using `nthreads` threads. Ends with a barrier; returns the bucket mask.
 __device__ __forceinline__ uint32_t tc_stage(const uint32_t* __restrict__ otgt, uint32_t a, uint32_t du,
-                                             uint32_t tid, uint32_t nthreads, uint32_t* key, uint32_t* cnt) {
+                                             uint32_t tid, uint32_t nthreads, uint32_t* key, uint32_t* cnt,
+                                             uint32_t* bloom) {
     uint32_t nb = 2;
     while (nb * TC_BUCKET < 2 * du) nb <<= 1;
     const uint32_t h = nb * TC_BUCKET;
     for (uint32_t i = tid; i < h; i += nthreads) key[i] = 0xFFFFFFFFu;
     for (uint32_t i = tid; i < h / 2; i += nthreads) cnt[i] = 0;
+    if (bloom)
+        for (uint32_t i = tid; i < h / 4; i += nthreads) bloom[i] = 0;
     __syncthreads();
     for (uint32_t i = tid; i < du; i += nthreads) {
         const uint32_t x = otgt[a + i];
+        if (bloom) {
+            const uint32_t bit = tc_bloom_bit(x, nb - 1);
+            atomicOr(&bloom[bit >> 5], 1u << (bit & 31));
+        }
         uint32_t b = tc_hash(x, nb - 1);
         for (;;) {
             bool done = false;
@@ -359,10 +379,12 @@ __global__ __launch_bounds__(TCB, 8) void k_tc_count(const uint32_t* __restrict_
                                                   unsigned long long* __restrict__ tri) {
     __shared__ __align__(16) uint32_t s_key[TCW][HASH_CAP];
     __shared__ uint32_t s_cnt[TCW][HASH_CAP / 2];
+    __shared__ uint32_t s_bloom[TCW][HASH_CAP / 4];
     const int wid = threadIdx.x >> 6;
     const int lane = lane_id();
     uint32_t* key = s_key[wid];
     uint32_t* cnt = s_cnt[wid];
+    uint32_t* bloom = s_bloom[wid];
     for (uint64_t g = blockIdx.x; g * TCW < nwork; g += gridDim.x) {
         const uint64_t wi = g * TCW + wid;
         const bool active = wi < nwork;
@@ -375,8 +397,9 @@ __global__ __launch_bounds__(TCB, 8) void k_tc_count(const uint32_t* __restrict_
         // Each wave stages its own list; the barriers inside are block-wide, so
         // every wave takes part (an empty list stages nothing).
         const uint32_t sdu = staged ? du : 0;
-        const uint32_t hmask = tc_stage(otgt, a, sdu, lane, 64, key, cnt);
-        const unsigned long long cu = tc_item(ooff, otgt, a, du, s0, s1, 0, 64, staged, key, cnt, hmask, tri);
+        const uint32_t hmask = tc_stage(otgt, a, sdu, lane, 64, key, cnt, bloom);
+        const unsigned long long cu =
+            tc_item(ooff, otgt, a, du, s0, s1, 0, 64, staged, key, cnt, bloom, hmask, tri);
         if (lane == 0 && cu) atomicAdd(&tri[u], cu);
         tc_flush(key, cnt, (hmask + 1) * TC_BUCKET, lane, 64, tri);
     }
@@ -400,9 +423,10 @@ __global__ __launch_bounds__(TCB_BIG) void k_tc_count_big(const uint32_t* __rest
         const uint32_t a = ooff[u];
         const uint32_t du = ooff[u + 1] - a;
         const uint32_t s0 = istart[wi] - a, s1 = iend[wi] - a;
-        const uint32_t hmask = tc_stage(otgt, a, du, threadIdx.x, TCB_BIG, b_key, b_cnt);
+        // No bloom bitmap here: with long lists it costs more LDS than it saves.
+        const uint32_t hmask = tc_stage(otgt, a, du, threadIdx.x, TCB_BIG, b_key, b_cnt, nullptr);
         const unsigned long long cu =
-            tc_item(ooff, otgt, a, du, s0, s1, 64u * wid, TCB_BIG, true, b_key, b_cnt, hmask, tri);
+            tc_item(ooff, otgt, a, du, s0, s1, 64u * wid, TCB_BIG, true, b_key, b_cnt, nullptr, hmask, tri);
         if (lane == 0 && cu) atomicAdd(&tri[u], cu);
         tc_flush(b_key, b_cnt, (hmask + 1) * TC_BUCKET, threadIdx.x, TCB_BIG, tri);
     }
