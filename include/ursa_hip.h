@@ -52,6 +52,16 @@ int ursa_hip_synchronize(void);
  * again with NULL. Calls still return only when their results are written. */
 int ursa_hip_set_stream(void* stream);
 
+/* ---- ingest — ursa-core/src/id_map.rs ---- */
+/* IdMap::from_edge_batches / intern_i64 (id_map.rs:220-319) on the device:
+ * dense ids in first-seen order over the rows, src before dst within a row.
+ * src/dst: int64[m] (host or device); src_dense/dst_dense: u32[m];
+ * user_ids (may be NULL): int64 with room for 2m entries, receives the user id
+ * of each dense id (IdMap::user_id_array, id_map.rs:359-365); *n_out = number
+ * of distinct ids. 2m must fit in u32 (URSA_HIP_ERANGE otherwise). */
+int ursa_hip_intern_i64(uint64_t m, const int64_t* src, const int64_t* dst, uint32_t* src_dense,
+                        uint32_t* dst_dense, int64_t* user_ids, uint64_t* n_out);
+
 /* ---- topology — ursa-core/src/topology.rs ---- */
 /* Topology::build (topology.rs:276-298): stable grouping of the m edge rows by
  * source, within-segment order = original row order (topology.rs:109-114).

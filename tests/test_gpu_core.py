@@ -228,3 +228,22 @@ def test_caller_stream(ua):
         hip.hipStreamDestroy(st)
     assert np.array_equal(got, ref)
     assert np.array_equal(lv, ua.bfs_distances(t, 0))
+
+
+# ---------------------------------------------------------------- device ingest
+def test_intern_i64_matches_oracle(ua):
+    rng = np.random.default_rng(21)
+    inputs = [
+        (np.array([10, 20, 30, 10]), np.array([20, 30, 10, 40])),  # id_map.rs:459-478 shape
+        (rng.integers(-50, 50, 1000), rng.integers(-50, 50, 1000)),  # negative ids, many repeats
+        (rng.integers(0, 2**62, 20000, dtype=np.int64), rng.integers(0, 2**62, 20000, dtype=np.int64)),
+        (np.full(7, 5), np.full(7, 5)),  # one id, self-loops
+    ]
+    us, ud = orc.gen_rmat(12, 16, seed=3)
+    inputs.append((us, ud))
+    for s, d in inputs:
+        n, sd, dd, users = ua.intern_i64(s, d)
+        rn, rs, rd, ru = orc.intern_i64(s, d)
+        assert n == rn and np.array_equal(sd, rs) and np.array_equal(dd, rd) and np.array_equal(users, ru)
+    n, sd, dd, users = ua.intern_i64([], [])
+    assert n == 0 and sd.size == 0

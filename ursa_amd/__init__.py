@@ -27,6 +27,7 @@ __all__ = [
     "Topology",
     "UrsaHipError",
     "degree",
+    "intern_i64",
     "pagerank",
     "pagerank_weighted",
     "pagerank_mixed",
@@ -158,6 +159,23 @@ class Topology:
         check(load().ursa_hip_topology_undirected(self._h, C.byref(slots)))
         off, tg, _ = self._export(2, slots.value, False)
         return off, tg
+
+
+def intern_i64(src, dst):
+    """IdMap interning on the device (ursa-core/src/id_map.rs:220-319): dense ids in
+    first-seen order, src before dst per row -> (n, src_dense u32[m], dst_dense u32[m],
+    user_ids i64[n])."""
+    s = np.ascontiguousarray(np.asarray(src, dtype=np.int64))
+    d = np.ascontiguousarray(np.asarray(dst, dtype=np.int64))
+    if s.shape != d.shape:
+        raise ValueError("src/dst length mismatch")
+    m = s.shape[0]
+    sd = np.zeros(max(m, 1), np.uint32)
+    dd = np.zeros(max(m, 1), np.uint32)
+    users = np.zeros(max(2 * m, 1), np.int64)
+    n = C.c_uint64(0)
+    check(load().ursa_hip_intern_i64(m, _ptr(s), _ptr(d), _ptr(sd), _ptr(dd), _ptr(users), C.byref(n)))
+    return int(n.value), sd[:m], dd[:m], users[: n.value]
 
 
 def degree(topo: Topology, direction=Direction.Out) -> np.ndarray:
