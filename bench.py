@@ -216,6 +216,7 @@ def bench_c2(args, ua, gen):
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline_bfs(e, sources, args.cpu_sources)
+    ingest = bench_ingest(args, lib, e)
     for b in (lv, d_out, d_in, d_cc, d_tri):
         b.free()
     config = {
@@ -239,7 +240,40 @@ def bench_c2(args, ua, gen):
         "kernels": kernels,
         "cpu_baseline": cpu,
         "tri_launch": tri_q,
+        "ingest": ingest,
     }
+
+
+def bench_ingest(args, lib, e):
+    """IdMap interning (id_map.rs:220-319) of the step graph's int64 edge list on the
+    device, inputs and outputs in HBM; bytes = 16 per row read + 8 per row written +
+    8 per distinct id written. CPU baseline: the oracle's serial hash-map interning."""
+    us, ud, _, _, _ = e.export()
+    m = int(us.shape[0])
+    bufs = [Dev(lib, 8 * m), Dev(lib, 8 * m), Dev(lib, 4 * m), Dev(lib, 4 * m), Dev(lib, 16 * m), Dev(lib, 8)]
+    check(lib.ursa_hip_memcpy_to_device(bufs[0].p, us.ctypes.data_as(C.c_void_p), 8 * m))
+    check(lib.ursa_hip_memcpy_to_device(bufs[1].p, ud.ctypes.data_as(C.c_void_p), 8 * m))
+    nn = C.c_uint64(0)
+    times = []
+    for _ in range(4):
+        t0 = time.perf_counter()
+        check(lib.ursa_hip_intern_i64(m, bufs[0].p, bufs[1].p, bufs[2].p, bufs[3].p, bufs[4].p, C.byref(nn)))
+        times.append(time.perf_counter() - t0)
+    el = min(times[1:])
+    gb = (24.0 * m + 8.0 * nn.value) / 1e9
+    res = {"rows": m, "ids": int(nn.value), "ms": round(el * 1e3, 3), "GBps": round(gb / el, 1),
+           "hbm_frac": round(gb / el / HBM_PEAK_GBS, 4)}
+    if not args.no_cpu_baseline:
+        import oracle as orc
+
+        t0 = time.perf_counter()
+        rn, _, _, _ = orc.intern_i64(us, ud)
+        res["cpu_oracle_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        res["cpu_ids_equal"] = rn == nn.value
+    for b in bufs:
+        b.free()
+    log(f"ingest: {m} rows -> {nn.value} ids in {el * 1e3:.2f} ms ({gb / el:.0f} GB/s)")
+    return res
 
 
 def cpu_baseline_bfs(e, sources, count):
@@ -563,6 +597,7 @@ def main():
         "roofline": res["roofline"],
         "cpu_baseline": res["cpu_baseline"],
         "kernels": res["kernels"],
+        "ingest": res["ingest"],
         "pagerank": pr,
         "batched": batched,
         "serving": serving,

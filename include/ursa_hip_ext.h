@@ -100,6 +100,7 @@ int ursa_hip_copy_probe(const void* src_device, void* dst_device, uint64_t bytes
 int ursa_hip_device_alloc(uint64_t bytes, void** out);
 int ursa_hip_device_free(void* p);
 int ursa_hip_memcpy_to_host(void* dst, const void* src_device, uint64_t bytes);
+int ursa_hip_memcpy_to_device(void* dst_device, const void* src, uint64_t bytes);
 
 /* ---- seeded workload synthesis on the device ----
  * R-MAT: edge_factor << scale edge draws; each draw's endpoints come from

@@ -68,6 +68,7 @@ SIGNATURES = {
     "ursa_hip_device_alloc": (cint, [u64, C.POINTER(P)]),
     "ursa_hip_device_free": (cint, [P]),
     "ursa_hip_memcpy_to_host": (cint, [P, P, u64]),
+    "ursa_hip_memcpy_to_device": (cint, [P, P, u64]),
     "ursa_hip_gen_rmat": (cint, [u32, u32, f64, f64, f64, u64, cint, C.POINTER(P)]),
     "ursa_hip_gen_er": (cint, [u64, u32, u64, cint, C.POINTER(P)]),
     "ursa_hip_edges_size": (cint, [P, C.POINTER(u64), C.POINTER(u64)]),

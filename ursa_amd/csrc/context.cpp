@@ -314,6 +314,14 @@ int ursa_hip_memcpy_to_host(void* dst, const void* src_device, uint64_t bytes) {
     });
 }
 
+int ursa_hip_memcpy_to_device(void* dst_device, const void* src, uint64_t bytes) {
+    return guard([&] {
+        hipStream_t s = stream();
+        HIP_OK(hipMemcpyAsync(dst_device, src, bytes, hipMemcpyHostToDevice, s));
+        HIP_OK(hipStreamSynchronize(s));
+    });
+}
+
 uint64_t ursa_hip_profile_names(char* buf, uint64_t cap) {
     std::lock_guard<std::mutex> lk(g_prof_mu);
     prof_drain_locked();
