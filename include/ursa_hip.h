@@ -52,6 +52,20 @@ int ursa_hip_synchronize(void);
  * again with NULL. Calls still return only when their results are written. */
 int ursa_hip_set_stream(void* stream);
 
+/* ---- BFS-based statistics — ursa-plan/src/stats.rs ---- */
+/* bfs_source_scan (stats.rs:100-140): over every source (out-of-range ones
+ * reach nothing) and every vertex at finite distance d >= 1: *sum = sum of d
+ * (f64), *count = number of such pairs, *max_dist = largest d (0 if none). */
+int ursa_hip_bfs_source_scan(ursa_hip_topo* t, const uint32_t* sources, uint64_t k, int direction,
+                             double* sum, uint64_t* count, int32_t* max_dist);
+/* avg_path_length (stats.rs:142-152): sources = every node, or the strided
+ * sample of round(sample*n) nodes (stats.rs:80-98); out-edges; 0.0 when no pair
+ * is reachable. */
+int ursa_hip_avg_path_length(ursa_hip_topo* t, int has_sample, double sample, double* out);
+/* diameter (stats.rs:154-168): max distance from every node (approximate = 0)
+ * or from the strided sample of min(128/n, 1) (approximate = 1); out-edges. */
+int ursa_hip_diameter(ursa_hip_topo* t, int approximate, int64_t* out);
+
 /* ---- ingest — ursa-core/src/id_map.rs ---- */
 /* IdMap::from_edge_batches / intern_i64 (id_map.rs:220-319) on the device:
  * dense ids in first-seen order over the rows, src before dst within a row.

@@ -52,6 +52,10 @@ def _load() -> C.CDLL:
         "orc_pagerank": (None, [P, P, f64, u32, f64, i, P, P, P]),
         "orc_bfs": (None, [P, u32, i, P]),
         "orc_ppr": (None, [P, P, u64, f64, u32, f64, P, P]),
+        "orc_stats_sources": (u64, [u64, i, f64, P]),
+        "orc_bfs_source_scan": (None, [P, P, u64, i, P, P, P]),
+        "orc_avg_path_length": (f64, [P, i, f64]),
+        "orc_diameter": (C.c_int64, [P, i]),
         "orc_k_hop": (u64, [P, P, u64, u32, i, P, P, u64]),
         "orc_shortest_path": (u64, [P, u32, u32, i, P, u64]),
         "orc_betweenness_sources": (None, [P, P, u64, i, f64, i, P]),
@@ -151,6 +155,28 @@ def ppr(t: Topology, seeds, damping=0.85, max_iter=100, tol=1e-8):
     it = C.c_uint32(0)
     lib().orc_ppr(t._h, _p(s) if s.size else None, s.shape[0], damping, max_iter, tol, _p(out), C.byref(it))
     return out[: t.n], it.value
+
+
+def stats_sources(n: int, frac=None) -> np.ndarray:
+    out = np.zeros(max(n, 1), np.uint32)
+    k = lib().orc_stats_sources(n, frac is not None, 0.0 if frac is None else frac, _p(out))
+    return out[:k].copy()
+
+
+def bfs_source_scan(t: Topology, sources, direction="out"):
+    s = _u32(sources)
+    sm, c, mx = C.c_double(0), C.c_uint64(0), C.c_int32(0)
+    lib().orc_bfs_source_scan(t._h, _p(s) if s.size else None, s.shape[0], _DIRS[direction], C.byref(sm),
+                              C.byref(c), C.byref(mx))
+    return sm.value, c.value, mx.value
+
+
+def avg_path_length(t: Topology, sample=None) -> float:
+    return lib().orc_avg_path_length(t._h, sample is not None, 0.0 if sample is None else sample)
+
+
+def diameter(t: Topology, approximate=True) -> int:
+    return int(lib().orc_diameter(t._h, int(approximate)))
 
 
 def bfs(t: Topology, source: int, direction="out") -> np.ndarray:

@@ -409,6 +409,76 @@ void orc_ppr(orc_topo* t, const uint32_t* seeds, uint64_t nseeds, double damping
     if (iters_out) *iters_out = iters;
 }
 
+// ---- ursa-plan/src/stats.rs:80-98 ----
+uint64_t orc_stats_sources(uint64_t n, int has_frac, double frac, uint32_t* out) {
+    if (!has_frac) {
+        for (uint64_t i = 0; i < n; ++i) out[i] = (uint32_t)i;
+        return n;
+    }
+    const double f = std::min(std::max(frac, 0.0), 1.0);
+    uint64_t k = (uint64_t)std::max(std::round((double)n * f), 1.0);
+    k = std::min<uint64_t>(k, n);
+    if (k == 0) return 0;
+    const double step = std::max((double)n / (double)k, 1.0);
+    for (uint64_t i = 0; i < k; ++i)
+        out[i] = (uint32_t)std::min<uint64_t>((uint64_t)std::floor((double)i * step), n - 1);
+    return k;
+}
+
+// ---- ursa-plan/src/stats.rs:100-140: per-source partials folded in source order ----
+void orc_bfs_source_scan(orc_topo* t, const uint32_t* sources, uint64_t k, int dir, double* sum, uint64_t* count,
+                         int32_t* max_dist) {
+    std::vector<int32_t> d(t->n);
+    double S = 0.0;
+    uint64_t C = 0;
+    int32_t M = 0;
+    for (uint64_t i = 0; i < k; ++i) {
+        double s = 0.0;
+        uint64_t c = 0;
+        int32_t m = 0;
+        if (t->n) orc_bfs(t, sources[i], dir, d.data());
+        for (uint64_t v = 0; v < t->n; ++v)
+            if (d[v] >= 1) {
+                s += (double)d[v];
+                ++c;
+                if (d[v] > m) m = d[v];
+            }
+        S += s;
+        C += c;
+        if (m > M) M = m;
+    }
+    *sum = S;
+    *count = C;
+    *max_dist = M;
+}
+
+// ---- ursa-plan/src/stats.rs:142-152 ----
+double orc_avg_path_length(orc_topo* t, int has_sample, double sample) {
+    std::vector<uint32_t> src(std::max<uint64_t>(t->n, 1));
+    const uint64_t k = orc_stats_sources(t->n, has_sample, sample, src.data());
+    double s;
+    uint64_t c;
+    int32_t m;
+    orc_bfs_source_scan(t, src.data(), k, ORC_OUT, &s, &c, &m);
+    return c == 0 ? 0.0 : s / (double)c;
+}
+
+// ---- ursa-plan/src/stats.rs:154-168 ----
+int64_t orc_diameter(orc_topo* t, int approximate) {
+    const uint64_t n = t->n;
+    std::vector<uint32_t> src(std::max<uint64_t>(n, 1));
+    uint64_t k;
+    if (approximate && n > 0)
+        k = orc_stats_sources(n, 1, std::min(128.0 / (double)n, 1.0), src.data());
+    else
+        k = orc_stats_sources(n, 0, 0.0, src.data());
+    double s;
+    uint64_t c;
+    int32_t m;
+    orc_bfs_source_scan(t, src.data(), k, ORC_OUT, &s, &c, &m);
+    return m;
+}
+
 // ---- bfs.rs:18-42 ----
 void orc_bfs(orc_topo* t, uint32_t source, int dir, int32_t* dist) {
     const uint64_t n = t->n;

@@ -71,6 +71,14 @@ void orc_pagerank(orc_topo* t, const double* weights, double damping, uint32_t m
 void orc_ppr(orc_topo* t, const uint32_t* seeds, uint64_t nseeds, double damping, uint32_t max_iter,
              double tol, double* out, uint32_t* iters_out);
 void orc_bfs(orc_topo* t, uint32_t source, int dir, int32_t* out);
+/* ursa-plan/src/stats.rs:80-168: strided stats source sample, bfs_source_scan
+ * (sum over sources in order of per-source f64 sums over vertices in id order,
+ * count, max), avg_path_length and diameter (out-edges). */
+uint64_t orc_stats_sources(uint64_t n, int has_frac, double frac, uint32_t* out);
+void orc_bfs_source_scan(orc_topo* t, const uint32_t* sources, uint64_t k, int dir, double* sum, uint64_t* count,
+                         int32_t* max_dist);
+double orc_avg_path_length(orc_topo* t, int has_sample, double sample);
+int64_t orc_diameter(orc_topo* t, int approximate);
 /* hop.rs:52-130. Returns number of pairs; writes min(cap, count). */
 uint64_t orc_k_hop(orc_topo* t, const uint32_t* seeds, uint64_t nseeds, uint32_t k, int dir,
                    uint32_t* out_src, uint32_t* out_dst, uint64_t cap);

@@ -310,3 +310,27 @@ def test_autotune_changes_schedule_not_results(ua):
         assert q["source"] == "cached" and (q["alpha"], q["beta"], q["push_div"]) == (
             p["alpha"], p["beta"], p["push_div"])
         assert np.array_equal(ua.bfs_multi(t2, srcs, direction, mode=2), before2)
+
+
+# ---------------------------------------------------------------- stats.rs on the device
+@pytest.mark.parametrize("case", GRAPHS, ids=IDS)
+def test_bfs_source_scan_matches(ua, case):
+    name, n, s, d = case
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    srcs = list(range(0, n, max(1, n // 150)))[:150] + [n + 2]
+    for direction in DIRS:
+        assert ua.bfs_source_scan(t, srcs, direction) == orc.bfs_source_scan(o, srcs, direction), (name, direction)
+    if n <= 3000:
+        assert ua.avg_path_length(t) == orc.avg_path_length(o)
+        assert ua.diameter(t, approximate=False) == orc.diameter(o, approximate=False)
+    assert ua.avg_path_length(t, 0.05) == orc.avg_path_length(o, 0.05)
+    assert ua.diameter(t) == orc.diameter(o)
+
+
+def test_stats_goldens_device(ua):  # ursa-plan/src/stats.rs:223-259
+    tri = ua.Topology.build(3, [0, 1, 2], [1, 2, 0])
+    assert ua.avg_path_length(tri) == 1.5 and ua.diameter(tri, approximate=False) == 2
+    assert ua.diameter(ua.Topology.build(4, [0, 1, 2], [1, 2, 3]), approximate=False) == 3
+    two = ua.Topology.build(4, [0, 2], [1, 3])
+    assert ua.avg_path_length(two) == 1.0 and ua.diameter(two, approximate=False) == 1

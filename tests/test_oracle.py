@@ -458,3 +458,37 @@ def test_ppr_with_all_nodes_as_seeds_is_pagerank():
     pr, _, _ = orc.pagerank(t, 0.85, 10000, 1e-15)
     pp, _ = orc.ppr(t, list(range(t.n)), 0.85, 10000, 1e-15)
     assert np.allclose(pr, pp, rtol=1e-12, atol=0)
+
+
+# ---------------------------------------------------------------- stats.rs (bfs_source_scan & co)
+def test_stats_goldens():  # ursa-plan/src/stats.rs:223-259
+    tri = orc.Topology(3, [0, 1, 2], [1, 2, 0])
+    assert abs(orc.avg_path_length(tri) - 1.5) < 1e-12
+    assert orc.diameter(tri, approximate=False) == 2
+    assert orc.diameter(tri, approximate=True) <= 2
+    line = orc.Topology(4, [0, 1, 2], [1, 2, 3])
+    assert orc.diameter(line, approximate=False) == 3
+    two = orc.Topology(4, [0, 2], [1, 3])
+    assert abs(orc.avg_path_length(two) - 1.0) < 1e-12
+    assert orc.diameter(two, approximate=False) == 1
+    assert orc.avg_path_length(orc.Topology(2, [], [])) == 0.0
+
+
+def test_stats_sources_stride():  # stats.rs:80-98
+    assert orc.stats_sources(5).tolist() == [0, 1, 2, 3, 4]
+    assert orc.stats_sources(10, 0.3).tolist() == [0, 3, 6]
+    assert orc.stats_sources(10, 0.0).tolist() == [0]
+    assert orc.stats_sources(7, 5.0).tolist() == list(range(7))
+    assert orc.stats_sources(1000, 128 / 1000).shape[0] == 128
+
+
+def test_bfs_source_scan_vs_scipy_fixture():
+    # The committed scipy BFS levels of gnp100 from four sources fix the fold.
+    t, users = topo_from_user(FX["gnp100_src"], FX["gnp100_dst"])
+    dense = {int(u): i for i, u in enumerate(users)}
+    srcs = [dense[int(x)] for x in FX["gnp100_bfs_sources"]]
+    lv = FX["gnp100_bfs_out"]
+    want_sum = float(lv[lv >= 1].sum())
+    want_cnt = int((lv >= 1).sum())
+    s, c, m = orc.bfs_source_scan(t, srcs + [10**6])
+    assert (s, c, m) == (want_sum, want_cnt, int(lv.max()))

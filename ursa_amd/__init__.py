@@ -28,6 +28,9 @@ __all__ = [
     "UrsaHipError",
     "degree",
     "intern_i64",
+    "bfs_source_scan",
+    "avg_path_length",
+    "diameter",
     "pagerank",
     "pagerank_weighted",
     "pagerank_mixed",
@@ -159,6 +162,30 @@ class Topology:
         check(load().ursa_hip_topology_undirected(self._h, C.byref(slots)))
         off, tg, _ = self._export(2, slots.value, False)
         return off, tg
+
+
+def bfs_source_scan(topo: Topology, sources, direction=Direction.Out):
+    """bfs_source_scan (ursa-plan/src/stats.rs:100-140) -> (sum f64, count, max)."""
+    src = _u32(sources)
+    sm, c, mx = C.c_double(0), C.c_uint64(0), C.c_int32(0)
+    check(load().ursa_hip_bfs_source_scan(topo._h, _ptr(src) if src.size else None, src.shape[0], _dir(direction),
+                                          C.byref(sm), C.byref(c), C.byref(mx)))
+    return sm.value, c.value, mx.value
+
+
+def avg_path_length(topo: Topology, sample=None) -> float:
+    """avg_path_length (ursa-plan/src/stats.rs:142-152)."""
+    out = C.c_double(0)
+    check(load().ursa_hip_avg_path_length(topo._h, sample is not None, 0.0 if sample is None else sample,
+                                          C.byref(out)))
+    return out.value
+
+
+def diameter(topo: Topology, approximate: bool = True) -> int:
+    """diameter (ursa-plan/src/stats.rs:154-168)."""
+    out = C.c_int64(0)
+    check(load().ursa_hip_diameter(topo._h, int(approximate), C.byref(out)))
+    return out.value
 
 
 def intern_i64(src, dst):

@@ -18,6 +18,7 @@
 //
 // Algorithmic bytes counted per source as in single-source BFS (4 B per
 // reached edge + 12 B per node); the batch shares them.
+#include <functional>
 #include <algorithm>
 #include <vector>
 
@@ -471,6 +472,22 @@ void msbfs_batch(uint64_t n, uint64_t fwd_edges, const uint32_t* src_host, uint3
         HIP_OK(hipMemcpyAsync(&hre, w.redges.get(), sizeof(hre), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
         *reached_edges += hre;
+    }
+}
+
+void msbfs_levels_each(ursa_hip_topo* t, const std::vector<uint32_t>& src, int dir, int32_t* lv64, uint32_t push_div,
+                       const std::function<void(uint64_t, uint32_t)>& each) {
+    const uint64_t n = t->n;
+    hipStream_t s = stream();
+    NbrView fwd, rev;
+    direction_views(t, dir, s, fwd, rev);
+    MsWork w(n);
+    w.prepare_pull(rev, s);
+    const uint64_t fwd_edges = dir == URSA_HIP_BOTH ? 2 * t->m : t->m;
+    for (uint64_t b0 = 0; b0 < src.size(); b0 += 64) {
+        const uint32_t nb = (uint32_t)std::min<uint64_t>(64, src.size() - b0);
+        msbfs_batch(n, fwd_edges, src.data() + b0, nb, fwd, rev, lv64, w, s, nullptr, push_div);
+        each(b0, nb);
     }
 }
 
