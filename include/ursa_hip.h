@@ -65,6 +65,14 @@ int ursa_hip_avg_path_length(ursa_hip_topo* t, int has_sample, double sample, do
 /* diameter (stats.rs:154-168): max distance from every node (approximate = 0)
  * or from the strided sample of min(128/n, 1) (approximate = 1); out-edges. */
 int ursa_hip_diameter(ursa_hip_topo* t, int approximate, int64_t* out);
+/* closeness (ursa-core/src/algo/closeness.rs:69-137, unweighted, out-edges):
+ * reachable / sum of finite distances >= 1 per source, 0.0 if it reaches
+ * nothing. sources = NULL: every node (out f64[n]); else out[i] for sources[i]
+ * (f64[k]; an out-of-range source gives 0.0). */
+int ursa_hip_closeness(ursa_hip_topo* t, const uint32_t* sources, uint64_t k, double* out);
+/* clustering_coefficient (ursa-core/src/algo/clustering.rs:21-40):
+ * 2 T(u) / (k (k - 1)) over the undirected view, 0.0 for degree < 2; f64[n]. */
+int ursa_hip_clustering_coefficient(ursa_hip_topo* t, double* out);
 
 /* ---- ingest — ursa-core/src/id_map.rs ---- */
 /* IdMap::from_edge_batches / intern_i64 (id_map.rs:220-319) on the device:

@@ -56,6 +56,8 @@ def _load() -> C.CDLL:
         "orc_bfs_source_scan": (None, [P, P, u64, i, P, P, P]),
         "orc_avg_path_length": (f64, [P, i, f64]),
         "orc_diameter": (C.c_int64, [P, i]),
+        "orc_closeness": (None, [P, P, u64, P]),
+        "orc_clustering": (None, [P, P]),
         "orc_k_hop": (u64, [P, P, u64, u32, i, P, P, u64]),
         "orc_shortest_path": (u64, [P, u32, u32, i, P, u64]),
         "orc_betweenness_sources": (None, [P, P, u64, i, f64, i, P]),
@@ -177,6 +179,19 @@ def avg_path_length(t: Topology, sample=None) -> float:
 
 def diameter(t: Topology, approximate=True) -> int:
     return int(lib().orc_diameter(t._h, int(approximate)))
+
+
+def closeness(t: Topology, sources=None) -> np.ndarray:
+    s = _u32(np.arange(t.n) if sources is None else sources)
+    out = np.zeros(max(s.shape[0], 1), np.float64)
+    lib().orc_closeness(t._h, _p(s) if s.size else None, s.shape[0], _p(out))
+    return out[: s.shape[0]]
+
+
+def clustering(t: Topology) -> np.ndarray:
+    out = np.zeros(max(t.n, 1), np.float64)
+    lib().orc_clustering(t._h, _p(out))
+    return out[: t.n]
 
 
 def bfs(t: Topology, source: int, direction="out") -> np.ndarray:

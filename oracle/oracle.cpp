@@ -479,6 +479,36 @@ int64_t orc_diameter(orc_topo* t, int approximate) {
     return m;
 }
 
+// ---- closeness.rs:69-137: exact i64 sums, one f64 division ----
+void orc_closeness(orc_topo* t, const uint32_t* sources, uint64_t k, double* out) {
+    std::vector<int32_t> d(t->n);
+    for (uint64_t i = 0; i < k; ++i) {
+        out[i] = 0.0;
+        if (sources[i] >= t->n) continue;
+        orc_bfs(t, sources[i], ORC_OUT, d.data());
+        int64_t sum = 0, reach = 0;
+        for (uint64_t v = 0; v < t->n; ++v)
+            if (d[v] >= 1) {
+                sum += d[v];
+                ++reach;
+            }
+        out[i] = sum > 0 ? (double)reach / (double)sum : 0.0;
+    }
+}
+
+// ---- clustering.rs:21-40 ----
+void orc_clustering(orc_topo* t, double* out) {
+    const uint64_t n = t->n;
+    if (!n) return;
+    std::vector<uint64_t> tri(n);
+    orc_triangle_count(t, 1, tri.data());
+    t->undirected();
+    for (uint64_t u = 0; u < n; ++u) {
+        const double k = (double)(t->uoff[u + 1] - t->uoff[u]);
+        out[u] = k < 2.0 ? 0.0 : 2.0 * (double)tri[u] / (k * (k - 1.0));
+    }
+}
+
 // ---- bfs.rs:18-42 ----
 void orc_bfs(orc_topo* t, uint32_t source, int dir, int32_t* dist) {
     const uint64_t n = t->n;

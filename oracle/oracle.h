@@ -79,6 +79,10 @@ void orc_bfs_source_scan(orc_topo* t, const uint32_t* sources, uint64_t k, int d
                          int32_t* max_dist);
 double orc_avg_path_length(orc_topo* t, int has_sample, double sample);
 int64_t orc_diameter(orc_topo* t, int approximate);
+/* closeness.rs:69-137 (unweighted, Out): out[i] for sources[i]. */
+void orc_closeness(orc_topo* t, const uint32_t* sources, uint64_t k, double* out);
+/* clustering.rs:21-40 over the undirected view and u64 triangle counts. */
+void orc_clustering(orc_topo* t, double* out);
 /* hop.rs:52-130. Returns number of pairs; writes min(cap, count). */
 uint64_t orc_k_hop(orc_topo* t, const uint32_t* seeds, uint64_t nseeds, uint32_t k, int dir,
                    uint32_t* out_src, uint32_t* out_dst, uint64_t cap);

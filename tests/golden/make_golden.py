@@ -140,6 +140,16 @@ def main():
             fx[f"{name}_ppr{qi}_seeds"] = np.array(seeds, dtype=np.int64)
             fx[f"{name}_ppr{qi}_ids"], fx[f"{name}_ppr{qi}"] = keyed(ref)
 
+    # 6) Local clustering (nx.clustering on the simple undirected graph) and
+    #    closeness over out-edges: nx measures distances *to* a node, so the
+    #    reversed graph gives distances from it; wf_improved=False is
+    #    reachable / sum of distances (closeness.rs:61-68).
+    fx["gnp100_clust_ids"], fx["gnp100_clust"] = keyed(nx.clustering(gu))
+    fx["ba500_clust_ids"], fx["ba500_clust"] = keyed(nx.clustering(g4))
+    fx["gnp100_close_ids"], fx["gnp100_close"] = keyed(nx.closeness_centrality(gd.reverse(), wf_improved=False))
+    fx["sparse400_close_ids"], fx["sparse400_close"] = keyed(
+        nx.closeness_centrality(g3d.reverse(), wf_improved=False))
+
     # splitmix64 streams
     seeds = np.array([0, 4, 7, 42, 0x5EED5EED5EED5EED, 1234567], dtype=np.uint64)
     fx["splitmix_seeds"] = seeds

@@ -334,3 +334,23 @@ def test_stats_goldens_device(ua):  # ursa-plan/src/stats.rs:223-259
     assert ua.diameter(ua.Topology.build(4, [0, 1, 2], [1, 2, 3]), approximate=False) == 3
     two = ua.Topology.build(4, [0, 2], [1, 3])
     assert ua.avg_path_length(two) == 1.0 and ua.diameter(two, approximate=False) == 1
+
+
+# ---------------------------------------------------------------- closeness / clustering on the device
+@pytest.mark.parametrize("case", GRAPHS, ids=IDS)
+def test_closeness_and_clustering_match(ua, case):
+    name, n, s, d = case
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    srcs = np.array(list(range(0, n, max(1, n // 100)))[:100] + [n + 4])
+    assert np.array_equal(ua.closeness(t, srcs), orc.closeness(o, srcs)), name
+    if n <= 3000:
+        assert np.array_equal(ua.closeness(t), orc.closeness(o)), name
+    assert np.array_equal(ua.clustering_coefficient(t), orc.clustering(o)), name
+
+
+def test_closeness_clustering_goldens(ua):  # closeness.rs:233-275, clustering.rs:46-78
+    c = ua.closeness(ua.Topology.build(4, [0, 1, 2], [1, 2, 3]))
+    assert c.tolist() == [0.5, 2.0 / 3.0, 1.0, 0.0]
+    cc = ua.clustering_coefficient(ua.Topology.build(4, [0, 0, 0, 1], [1, 2, 3, 2]))
+    assert abs(cc[0] - 1 / 3) < 1e-15 and cc[3] == 0.0

@@ -492,3 +492,40 @@ def test_bfs_source_scan_vs_scipy_fixture():
     want_cnt = int((lv >= 1).sum())
     s, c, m = orc.bfs_source_scan(t, srcs + [10**6])
     assert (s, c, m) == (want_sum, want_cnt, int(lv.max()))
+
+
+# ---------------------------------------------------------------- closeness / clustering
+def test_closeness_goldens():  # closeness.rs:233-275
+    c = orc.closeness(orc.Topology(3, [0, 1, 2], [1, 2, 0]))
+    assert np.allclose(c, 2.0 / 3.0, rtol=0, atol=1e-12)
+    c = orc.closeness(orc.Topology(4, [0, 1, 2], [1, 2, 3]))
+    assert abs(c[0] - 0.5) < 1e-12 and abs(c[1] - 2 / 3) < 1e-12 and abs(c[2] - 1.0) < 1e-12 and c[3] == 0.0
+    assert orc.closeness(orc.Topology(0, [], [])).size == 0
+    n = 100
+    c = orc.closeness(orc.Topology(n, np.arange(n), (np.arange(n) + 1) % n))
+    assert np.all(np.abs(c - c[0]) < 1e-12) and c[0] > 0
+
+
+def test_clustering_goldens():  # clustering.rs:46-78
+    assert orc.clustering(orc.Topology(3, [0, 0, 1, 2], [1, 2, 2, 0])).tolist() == [1.0, 1.0, 1.0]
+    assert orc.clustering(orc.Topology(3, [0, 1], [1, 2])).tolist() == [0.0, 0.0, 0.0]
+    cc = orc.clustering(orc.Topology(4, [0, 0, 0], [1, 2, 3]))
+    assert cc[0] == 0.0 and cc[1] == 0.0
+    cc = orc.clustering(orc.Topology(4, [0, 0, 0, 1], [1, 2, 3, 2]))
+    assert abs(cc[0] - 1 / 3) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["gnp100", "ba500"])
+def test_clustering_vs_networkx(name):
+    t, users = topo_from_user(FX[f"{name}_src"], FX[f"{name}_dst"])
+    got = by_user(users, orc.clustering(t))
+    for k, v in zip(FX[f"{name}_clust_ids"].tolist(), FX[f"{name}_clust"].tolist()):
+        assert abs(got[k] - v) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["gnp100", "sparse400"])
+def test_closeness_vs_networkx(name):
+    t, users = topo_from_user(FX[f"{name}_src"], FX[f"{name}_dst"])
+    got = by_user(users, orc.closeness(t))
+    for k, v in zip(FX[f"{name}_close_ids"].tolist(), FX[f"{name}_close"].tolist()):
+        assert abs(got[k] - v) < 1e-12

@@ -31,6 +31,8 @@ __all__ = [
     "bfs_source_scan",
     "avg_path_length",
     "diameter",
+    "closeness",
+    "clustering_coefficient",
     "pagerank",
     "pagerank_weighted",
     "pagerank_mixed",
@@ -186,6 +188,26 @@ def diameter(topo: Topology, approximate: bool = True) -> int:
     out = C.c_int64(0)
     check(load().ursa_hip_diameter(topo._h, int(approximate), C.byref(out)))
     return out.value
+
+
+def closeness(topo: Topology, sources=None) -> np.ndarray:
+    """closeness (ursa-core/src/algo/closeness.rs:69-137, unweighted) for every node,
+    or for the given sources -> f64."""
+    if sources is None:
+        out = np.zeros(max(topo.n_nodes(), 1), np.float64)
+        check(load().ursa_hip_closeness(topo._h, None, 0, _ptr(out)))
+        return out[: topo.n_nodes()]
+    src = _u32(sources)
+    out = np.zeros(max(src.shape[0], 1), np.float64)
+    check(load().ursa_hip_closeness(topo._h, _ptr(src), src.shape[0], _ptr(out)))
+    return out[: src.shape[0]]
+
+
+def clustering_coefficient(topo: Topology) -> np.ndarray:
+    """clustering_coefficient (ursa-core/src/algo/clustering.rs:21-40) -> f64[n]."""
+    out = np.zeros(max(topo.n_nodes(), 1), np.float64)
+    check(load().ursa_hip_clustering_coefficient(topo._h, _ptr(out)))
+    return out[: topo.n_nodes()]
 
 
 def intern_i64(src, dst):

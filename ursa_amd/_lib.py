@@ -84,6 +84,8 @@ SIGNATURES = {
     "ursa_hip_bfs_source_scan": (cint, [P, P, u64, cint, C.POINTER(f64), C.POINTER(u64), C.POINTER(i32)]),
     "ursa_hip_avg_path_length": (cint, [P, cint, f64, C.POINTER(f64)]),
     "ursa_hip_diameter": (cint, [P, cint, C.POINTER(C.c_int64)]),
+    "ursa_hip_closeness": (cint, [P, P, u64, P]),
+    "ursa_hip_clustering_coefficient": (cint, [P, P]),
     "ursa_hip_topology_create_from_csr": (cint, [u64, u64, P, P, P, C.POINTER(P)]),
     "ursa_hip_autotune": (cint, [P, cint, cint, C.POINTER(f64), C.POINTER(f64), C.POINTER(u32), C.POINTER(cint)]),
     "ursa_hip_ppr_batch": (cint, [P, u32, P, P, f64, u32, f64, P, P]),
