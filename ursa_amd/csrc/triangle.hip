@@ -60,7 +60,7 @@ __global__ void k_tc_item_flags(const uint32_t* __restrict__ ooff, const uint32_
         const uint32_t u = oown[k];
         const uint32_t a = ooff[u], du = ooff[u + 1] - a;
         bool f = false;
-        const uint64_t win = du > STAGE_CAP ? 4 * TC_CHUNK : TC_CHUNK;  // amortise staging a long list
+        const uint64_t win = du > STAGE_CAP ? 64 * TC_CHUNK : TC_CHUNK;  // long lists: amortise staging
         if (du >= 2) f = (k == a) || (pre[k] / win != pre[k - 1] / win);
         flag[k] = f;
         iota[k] = (uint32_t)k;
