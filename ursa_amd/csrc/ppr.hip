@@ -32,6 +32,7 @@
 // any batch and in any lane.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "ppr.hpp"
@@ -44,7 +45,7 @@ namespace {
 constexpr int PB = 256;
 constexpr int PPR_G = 8;                     // threads per row
 constexpr int PPR_GROUPS = PB / PPR_G;       // rows per workgroup pass
-constexpr uint32_t PPR_ROWS = 512;           // rows per workgroup (one partial each)
+constexpr uint32_t PPR_ROWS = 128;           // rows per workgroup (one partial each)
 constexpr uint32_t PPR_HEAVY = 256;
 constexpr uint32_t PPR_MAXSEG = 1024;
 
@@ -75,6 +76,32 @@ __device__ __forceinline__ void load_phases(const PprLane* __restrict__ lanes, i
     for (int q = 0; q < KP; ++q) ph[q] = lanes[q].phase;
 }
 
+// KP adjacent values (KP * sizeof(T) bytes, aligned to that size) with
+// 16-byte loads where they fit.
+template <int KP, class T>
+__device__ __forceinline__ void load_lanes(const T* __restrict__ p, T* x) {
+    constexpr int PER = 16 / (int)sizeof(T);  // values per 16-byte load
+    if constexpr (KP >= PER) {
+        using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+#pragma unroll
+        for (int i = 0; i < KP / PER; ++i) {
+            const V w = reinterpret_cast<const V*>(p)[i];
+            if constexpr (sizeof(T) == 4) {
+                x[4 * i] = w.x;
+                x[4 * i + 1] = w.y;
+                x[4 * i + 2] = w.z;
+                x[4 * i + 3] = w.w;
+            } else {
+                x[2 * i] = w.x;
+                x[2 * i + 1] = w.y;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < KP; ++q) x[q] = p[q];
+    }
+}
+
 // This thread's share (slots b+g, b+g+8, ...) of the row sum over in-slots
 // [b, e), combined across the group's eight threads; the result is valid in
 // the group's thread g = 0.
@@ -88,17 +115,30 @@ __device__ __forceinline__ void ppr_row_sum(uint32_t b, uint32_t e, uint32_t g, 
         c[q] = 0.f;
         a[q] = 0.0;
     }
+    // Which value arrays any lane reads this sweep (phases are uniform).
+    bool any32 = false, any64 = false;
+#pragma unroll
+    for (int q = 0; q < KP; ++q) {
+        any32 |= ph[q] == 0 || ph[q] == 1;
+        any64 |= ph[q] == 2;
+    }
 #pragma unroll 2
     for (uint32_t k = b + g; k < e; k += PPR_G) {
         const uint64_t v = tgt[k];
+        // The KP values of vertex v are adjacent: read them with wide loads,
+        // then add lane by lane (the per-lane arithmetic does not change).
+        float x32[KP];
+        double x64[KP];
+        if (any32) load_lanes<KP>(c32 + v * KP, x32);
+        if (any64) load_lanes<KP>(c64 + v * KP, x64);
 #pragma unroll
         for (int q = 0; q < KP; ++q) {
             if (ph[q] == 0)
-                neu_add(s[q], c[q], c32[v * KP + q]);
+                neu_add(s[q], c[q], x32[q]);
             else if (ph[q] == 1)
-                a[q] += (double)c32[v * KP + q];
+                a[q] += (double)x32[q];
             else if (ph[q] == 2)
-                a[q] += c64[v * KP + q];
+                a[q] += x64[q];
         }
     }
     // Butterfly over the group (lanes differing in the low three bits).
