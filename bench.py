@@ -305,6 +305,36 @@ def cpu_baseline_bfs(e, sources, count):
 
 
 # --------------------------------------------------------------------- PageRank
+def bench_bfs_large(ua, lib, topo, n, m):
+    """BFS on the PageRank graph (R-MAT s24): 64 sources as one multi-source batch and
+    4 of them as single-source direction-optimizing traversals; kernel time from HIP
+    events, algorithmic bytes 4 E_reach + 12 n per source."""
+    outdeg = ua.degree(topo, "out")
+    cand = np.nonzero(outdeg > 0)[0].astype(np.uint32)
+    sources = np.ascontiguousarray(cand[ua.shuffled_order(len(cand), 3)[:64]])
+    lv = Dev(lib, 4 * 64 * n)
+    reached = C.c_uint64(0)
+    out = {}
+    for name, k, mode, kern in (("msbfs_x64", 64, 2, ["msbfs_push", "msbfs_pull"]),
+                                ("single_x4", 4, 1, ["bfs_top_down", "bfs_bottom_up"])):
+        check(lib.ursa_hip_bfs_ex(topo._h, sources.ctypes.data_as(C.c_void_p), k, 0, lv.p, C.byref(reached), mode))
+        ua.profile.enable(True)
+        ua.profile.reset()
+        t0 = time.perf_counter()
+        check(lib.ursa_hip_bfs_ex(topo._h, sources.ctypes.data_as(C.c_void_p), k, 0, lv.p, C.byref(reached), mode))
+        wall = time.perf_counter() - t0
+        ms = sum(ua.profile.query(x)["total_ms"] for x in kern)
+        ua.profile.enable(False)
+        algo = 4.0 * reached.value + 12.0 * n * k
+        gbs = algo / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+        out[name] = {"sources": k, "E_reach": reached.value, "kernel_ms": round(ms, 3), "call_ms": round(wall * 1e3, 3),
+                     "GTEPS": round(reached.value / (ms / 1e3) / 1e9, 2), "GBps": round(gbs, 1),
+                     "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+        log(f"bfs s24 {name}: {ms:.2f} ms kernels, {gbs:.0f} GB/s ({gbs / HBM_PEAK_GBS:.1%} of HBM peak)")
+    lv.free()
+    return out
+
+
 def bench_pagerank(args, ua, gen):
     lib = ua.load()
     scale = args.pr_scale
@@ -356,6 +386,7 @@ def bench_pagerank(args, ua, gen):
                         "traffic": traffic_from_profiles(f"pagerank_rmat{scale}",
                                                          "k_pr_tiles" if key.endswith("f64") else "k_pr_tiles_f32")}
     res["mixed_speedup"] = round(res["f64"]["ms"] / res["mixed"]["ms"], 3)
+    res["bfs"] = bench_bfs_large(ua, lib, topo, n, m)
     # accuracy against the f64 oracle
     if not args.no_cpu_baseline:
         import oracle as orc
