@@ -48,6 +48,14 @@ int ursa_hip_bfs_ex(ursa_hip_topo* t, const uint32_t* sources, uint32_t k, int d
 int ursa_hip_autotune(ursa_hip_topo* t, int direction, int force, double* alpha, double* beta,
                       uint32_t* push_div, int* source);
 
+/* *out = ((0.0 + x[0]) + x[1]) + ... + x[n-1], rounded step by step as a
+ * sequential f64 loop rounds (round-to-nearest-even), computed in parallel.
+ * PageRank forms its dangling mass this way (the reference sums it with
+ * `dangling.iter().map(|&u| rank[u]).sum()`, ursa-core/src/algo/pagerank.rs:78);
+ * exported so the summation can be checked bit for bit on its own. x may be a
+ * host or device pointer; n < 2^32. */
+int ursa_hip_ordered_sum(const double* x, uint64_t n, double* out);
+
 /* As ursa_hip_betweenness_sources with an explicit batch size (1..64 sources
  * advancing together; 1 = one traversal per launch sequence) and the TEPS
  * numerator (sum of out-degrees of reached vertices over all sources). */

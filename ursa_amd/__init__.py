@@ -36,6 +36,7 @@ __all__ = [
     "pagerank",
     "pagerank_weighted",
     "pagerank_mixed",
+    "ordered_sum",
     "ppr",
     "ppr_batch",
     "bfs_distances",
@@ -326,6 +327,15 @@ def autotune(topo: Topology, direction=Direction.Out, force: bool = False) -> di
                                    C.byref(src)))
     return {"alpha": a.value, "beta": b.value, "push_div": pd.value,
             "source": {0: "default", 1: "measured", 2: "cached"}[src.value]}
+
+
+def ordered_sum(x) -> float:
+    """((0.0 + x[0]) + x[1]) + ... rounded step by step, as PageRank's dangling mass
+    is summed (ursa-core/src/algo/pagerank.rs:78), computed on the device."""
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    out = C.c_double(0.0)
+    check(load().ursa_hip_ordered_sum(_ptr(a) if a.size else None, a.shape[0], C.byref(out)))
+    return out.value
 
 
 def bfs_distances(topo: Topology, source: int, direction=Direction.Out) -> np.ndarray:

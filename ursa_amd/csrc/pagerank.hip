@@ -31,6 +31,7 @@
 #include <cmath>
 #include <vector>
 
+#include "ordered_sum.hpp"
 #include "prim.hpp"
 #include "topology.hpp"
 
@@ -47,6 +48,7 @@ struct PrCtl {
     uint32_t iters;
     double delta;
     double base[2];
+    int32_t dm_e0;  // lowest candidate binade of the ordered dangling sum
 };
 
 // Split point of the merge path at diagonal `diag`: x row-ends and y in-edges
@@ -133,13 +135,12 @@ struct RowAcc<float> {
 template <class CT>
 __device__ __forceinline__ void pr_finish_row(uint64_t u, double acc, double base, double damping, double scale,
                                               const double* rank, double* next, CT* contrib_next,
-                                              double& delta, double& dangling) {
-    const double v = base + damping * acc;
+                                              double& delta) {
+    const double v = __dadd_rn(base, __dmul_rn(damping, acc));  // no contraction: pagerank.rs:84
     __builtin_nontemporal_store(v, &next[u]);
     const bool has_out = scale > 0.0;
     contrib_next[u] = (CT)(has_out ? v / scale : 0.0);
     delta += fabs(v - __builtin_nontemporal_load(&rank[u]));
-    if (!has_out) dangling += v;
 }
 
 template <bool WEIGHTED>
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(PR_BLOCK) void k_pr_tiles(
     const uint32_t* __restrict__ out_deg, const double* __restrict__ out_str, const double* __restrict__ rank,
     const CT* __restrict__ contrib, double* __restrict__ next, CT* __restrict__ contrib_next,
     double* __restrict__ carry_val, double* __restrict__ tail_val, double* __restrict__ tile_delta,
-    double* __restrict__ tile_dm, PrCtl* ctl, int parity, double damping) {
+    PrCtl* ctl, int parity, double damping) {
     __shared__ CT s_val[PR_TILE + PR_TILE / 16];
     __shared__ uint32_t s_end[PR_TILE + 1];
     __shared__ double s_wv[PR_BLOCK / 64];
@@ -255,7 +256,7 @@ __global__ __launch_bounds__(PR_BLOCK) void k_pr_tiles(
     if (threadIdx.x == 0) c_in = 0.0;
     if (threadIdx.x == PR_BLOCK - 1) carry_val[t] = (x1 < n) ? v_incl : 0.0;
 
-    double delta = 0.0, dm = 0.0;
+    double delta = 0.0;
     bool first = true;
 #pragma unroll
     for (int it = 0; it < PR_ITEMS; ++it) {
@@ -270,33 +271,28 @@ __global__ __launch_bounds__(PR_BLOCK) void k_pr_tiles(
                 tail_val[t] = val;
             } else {
                 pr_finish_row(u, val, base, damping, node_scale<WEIGHTED>(out_deg, out_str, u), rank, next,
-                              contrib_next, delta, dm);
+                              contrib_next, delta);
             }
         }
     }
     const double bd = block_sum(delta, s_red);
-    const double bm = block_sum(dm, s_red);
-    if (threadIdx.x == 0) {
-        tile_delta[t] = bd;
-        tile_dm[t] = bm;
-    }
+    if (threadIdx.x == 0) tile_delta[t] = bd;
 }
 
 // Finishes rows that began in an earlier tile, then folds the per-tile delta
-// and dangling partials of 256 consecutive tiles (in tile order) into one
-// partial per workgroup, so the final reduce reads few values.
+// partials of 256 consecutive tiles (in tile order) into one partial per
+// workgroup, so the final reduce reads few values.
 template <class CT, bool WEIGHTED>
 __global__ __launch_bounds__(256) void k_pr_fixup(
     const uint32_t* __restrict__ in_off, const uint32_t* __restrict__ tile_x, const uint32_t* __restrict__ tile_y,
     uint64_t tiles, uint64_t n, const uint32_t* __restrict__ out_deg, const double* __restrict__ out_str,
     const double* __restrict__ rank, double* __restrict__ next, CT* __restrict__ contrib_next,
     const double* __restrict__ carry_val, const double* __restrict__ tail_val, const double* __restrict__ tile_delta,
-    const double* __restrict__ tile_dm, double* __restrict__ blk_delta, double* __restrict__ blk_dm, const PrCtl* ctl,
-    int parity, double damping) {
+    double* __restrict__ blk_delta, const PrCtl* ctl, int parity, double damping) {
     __shared__ double s_red[256 / 64];
     if (ctl->done) return;
     const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    double delta = 0.0, dm = 0.0;
+    double delta = 0.0;
     if (t < tiles) {
         const uint64_t x0 = tile_x[t], y0 = tile_y[t], x1 = tile_x[t + 1];
         if (x0 < n && x1 > x0 && (uint64_t)in_off[x0] < y0) {
@@ -307,73 +303,90 @@ __global__ __launch_bounds__(256) void k_pr_fixup(
             for (uint64_t q = j; q < t; ++q) acc += carry_val[q];
             acc += tail_val[t];
             pr_finish_row(x0, acc, ctl->base[parity], damping, node_scale<WEIGHTED>(out_deg, out_str, x0), rank,
-                          next, contrib_next, delta, dm);
+                          next, contrib_next, delta);
         }
         delta = tile_delta[t] + delta;
-        dm = tile_dm[t] + dm;
     }
     const double bd = block_sum(delta, s_red);
-    const double bm = block_sum(dm, s_red);
-    if (threadIdx.x == 0) {
-        blk_delta[blockIdx.x] = bd;
-        blk_dm[blockIdx.x] = bm;
-    }
+    if (threadIdx.x == 0) blk_delta[blockIdx.x] = bd;
 }
 
-__global__ __launch_bounds__(1024) void k_pr_reduce(const double* __restrict__ blk_delta,
-                                                    const double* __restrict__ blk_dm, uint64_t count, PrCtl* ctl,
-                                                    int parity, uint32_t max_iter, double tol, double damping,
-                                                    double nf) {
+__global__ __launch_bounds__(1024) void k_pr_reduce(const double* __restrict__ blk_delta, uint64_t count, PrCtl* ctl,
+                                                    uint32_t max_iter, double tol) {
     __shared__ double s_red[1024 / 64];
     if (ctl->done) return;
-    double d = 0.0, m = 0.0;
-    for (uint64_t i = threadIdx.x; i < count; i += blockDim.x) {
-        d += blk_delta[i];
-        m += blk_dm[i];
-    }
+    double d = 0.0;
+    for (uint64_t i = threadIdx.x; i < count; i += blockDim.x) d += blk_delta[i];
     const double delta = block_sum(d, s_red);
-    const double dm = block_sum(m, s_red);
     if (threadIdx.x == 0) {
         const uint32_t iters = ctl->iters + 1;  // the iteration count lives on the device
         ctl->delta = delta;
         ctl->iters = iters;
-        ctl->base[parity ^ 1] = (1.0 - damping) / nf + damping * dm / nf;
         if (delta < tol || iters >= max_iter) ctl->done = 1;
     }
 }
 
-// rank0 = 1/n, contrib0 = rank0 / scale, per-block dangling partials.
+// Dangling mass of the iterate `rank` summed over the dangling nodes in
+// ascending node order, rounded as the reference's sequential loop rounds
+// (pagerank.rs:76-78): the gather and the chunk / group tables here, the
+// ordered walk in k_pr_dm_walk.
+__global__ __launch_bounds__(OS_BUILD_THREADS) void k_pr_dm_build(const double* __restrict__ rank,
+                                                                  const uint32_t* __restrict__ dang, uint64_t d,
+                                                                  const PrCtl* ctl, double* __restrict__ xd,
+                                                                  uint64_t* __restrict__ cq, uint32_t* __restrict__ cm,
+                                                                  double* __restrict__ cs) {
+    if (ctl->done) return;
+    os_build(rank, dang, d, ctl->dm_e0, xd, cq, cm, cs);
+}
+
+// base = (1-d)/n + d*dangling/n for the iteration of parity `parity`
+// (pagerank.rs:79), from the ordered dangling sum.
+__global__ __launch_bounds__(OS_WALK_THREADS) void k_pr_dm_walk(const double* __restrict__ xd, uint64_t d,
+                                                                const uint64_t* __restrict__ cq,
+                                                                const uint32_t* __restrict__ cm,
+                                                                const double* __restrict__ cs, PrCtl* ctl, int parity,
+                                                                double damping, double nf) {
+    __shared__ OsLds lds;
+    if (ctl->done) return;
+    const double dm = os_walk(xd, d, cq, cm, cs, ctl->dm_e0, lds);
+    if (threadIdx.x == 0) {
+        ctl->base[parity] = __dadd_rn(__ddiv_rn(1.0 - damping, nf), __ddiv_rn(__dmul_rn(damping, dm), nf));
+        const OsState st = os_state(dm);
+        if (st.fast) ctl->dm_e0 = st.e - 13;
+    }
+}
+
+// rank0 = 1/n, contrib0 = rank0 / scale.
 template <class CT, bool WEIGHTED>
 __global__ void k_pr_init(uint64_t n, double nf, const uint32_t* __restrict__ out_deg,
                           const double* __restrict__ out_str, double* __restrict__ rank,
-                          CT* __restrict__ contrib, double* __restrict__ part_dm) {
-    __shared__ double s_red[256 / 64];
-    double dm = 0.0;
+                          CT* __restrict__ contrib) {
     const uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (u < n) {
         const double r = 1.0 / nf;
         rank[u] = r;
         const double sc = node_scale<WEIGHTED>(out_deg, out_str, u);
         contrib[u] = (CT)(sc > 0.0 ? r / sc : 0.0);
-        if (!(sc > 0.0)) dm = r;
     }
-    const double b = block_sum(dm, s_red);
-    if (threadIdx.x == 0) part_dm[blockIdx.x] = b;
 }
 
-__global__ __launch_bounds__(1024) void k_pr_ctl_init(PrCtl* ctl, const double* part_dm, uint64_t count,
-                                                      double damping, double nf) {
-    __shared__ double s_red[1024 / 64];
-    double m = 0.0;
-    for (uint64_t i = threadIdx.x; i < count; i += blockDim.x) m += part_dm[i];
-    const double dm = block_sum(m, s_red);
-    if (threadIdx.x == 0) {
-        ctl->done = 0;
-        ctl->iters = 0;
-        ctl->delta = 0.0;
-        ctl->base[0] = (1.0 - damping) / nf + damping * dm / nf;
-        ctl->base[1] = 0.0;
-    }
+// Dangling flags in the original node order: u is dangling when its
+// out-degree is 0 or its out-strength is <= 0, read at its renumbered slot
+// (pagerank.rs:109-111, 152-153).
+template <bool WEIGHTED>
+__global__ void k_pr_dang_flags(uint64_t n, const uint32_t* __restrict__ new_of_old, const uint32_t* __restrict__ out_deg,
+                                const double* __restrict__ out_str, uint8_t* __restrict__ flags) {
+    const uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (u < n) flags[u] = node_scale<WEIGHTED>(out_deg, out_str, new_of_old[u]) <= 0.0 ? 1 : 0;
+}
+
+__global__ void k_pr_ctl_init(PrCtl* ctl, int32_t dm_e0) {
+    ctl->done = 0;
+    ctl->iters = 0;
+    ctl->delta = 0.0;
+    ctl->base[0] = 0.0;
+    ctl->base[1] = 0.0;
+    ctl->dm_e0 = dm_e0;
 }
 
 // out_str[v] = sum of v's out-edge weights in out-CSR order (pagerank.rs:148-150).
@@ -571,25 +584,65 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
         contrib32[0].alloc(n);
         contrib32[1].alloc(n);
     }
-    DevBuf<double> carry(tiles), tail(tiles), td(tiles), tm(tiles), fd(tiles), fm(tiles);
+    DevBuf<double> carry(tiles), tail(tiles), td(tiles), fd(tiles);
     DevBuf<PrCtl> ctl(1);
     const unsigned init_blocks = blocks_for(n, 256, 1u << 30);
-    DevBuf<double> part_dm(init_blocks);
     const uint32_t* od = t->pr_outdeg.get();
     const double* os = weighted ? out_str.get() : nullptr;
     if (mixed) {
         if (weighted)
-            k_pr_init<float, true><<<init_blocks, 256, 0, s>>>(n, nf, od, os, rank[0].get(), contrib32[0].get(), part_dm.get());
+            k_pr_init<float, true><<<init_blocks, 256, 0, s>>>(n, nf, od, os, rank[0].get(), contrib32[0].get());
         else
-            k_pr_init<float, false><<<init_blocks, 256, 0, s>>>(n, nf, od, os, rank[0].get(), contrib32[0].get(), part_dm.get());
+            k_pr_init<float, false><<<init_blocks, 256, 0, s>>>(n, nf, od, os, rank[0].get(), contrib32[0].get());
     } else {
         if (weighted)
-            k_pr_init<double, true><<<init_blocks, 256, 0, s>>>(n, nf, od, os, rank[0].get(), contrib[0].get(), part_dm.get());
+            k_pr_init<double, true><<<init_blocks, 256, 0, s>>>(n, nf, od, os, rank[0].get(), contrib[0].get());
         else
-            k_pr_init<double, false><<<init_blocks, 256, 0, s>>>(n, nf, od, os, rank[0].get(), contrib[0].get(), part_dm.get());
+            k_pr_init<double, false><<<init_blocks, 256, 0, s>>>(n, nf, od, os, rank[0].get(), contrib[0].get());
     }
-    k_pr_ctl_init<<<1, 1024, 0, s>>>(ctl.get(), part_dm.get(), init_blocks, damping, nf);
     check_launch();
+
+    // The dangling nodes in ascending original order, as renumbered slots: the
+    // order of the dangling-mass sum (pagerank.rs:76-78, 109-111).
+    uint64_t nd = 0;
+    DevBuf<uint32_t> dang(n);
+    {
+        DevBuf<uint8_t> flags(n);
+        DevBuf<uint64_t> cnt(1);
+        if (weighted)
+            k_pr_dang_flags<true><<<init_blocks, 256, 0, s>>>(n, t->pr_new_of_old.get(), od, os, flags.get());
+        else
+            k_pr_dang_flags<false><<<init_blocks, 256, 0, s>>>(n, t->pr_new_of_old.get(), od, os, flags.get());
+        check_launch();
+        timed("pagerank_dm_list", 9.0 * (double)n, s, [&] {
+            prim::select_flagged_u32(t->pr_new_of_old.get(), flags.get(), dang.get(), cnt.get(), n, s);
+        });
+        HIP_OK(hipMemcpyAsync(&nd, cnt.get(), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+    }
+    const uint64_t nch = (nd + OS_CHUNK - 1) / OS_CHUNK, nch1 = nch ? nch : 1;
+    const unsigned dm_blocks = (unsigned)((nch + OS_BUILD_THREADS / 64 - 1) / (OS_BUILD_THREADS / 64));
+    DevBuf<double> xd(nd ? nd : 1), cs(nch1);
+    DevBuf<uint64_t> cq(nch1 * OS_BINS);
+    DevBuf<uint32_t> cm(nch1);
+    // base[parity] from the ordered dangling sum of iterate r.
+    auto dangling_base = [&](const double* r, int parity) {
+        if (nch)
+            timed("pagerank_dm_build", 20.0 * (double)nd, s, [&] {
+                k_pr_dm_build<<<dm_blocks, OS_BUILD_THREADS, 0, s>>>(r, dang.get(), nd, ctl.get(), xd.get(), cq.get(),
+                                                                      cm.get(), cs.get());
+            });
+        timed("pagerank_dm_walk", 0.0, s, [&] {
+            k_pr_dm_walk<<<1, OS_WALK_THREADS, 0, s>>>(xd.get(), nd, cq.get(), cm.get(), cs.get(), ctl.get(), parity,
+                                                       damping, nf);
+        });
+        check_launch();
+    };
+    // First candidate window from the initial sum, about nd / n.
+    const int32_t e0 = nd ? (int32_t)std::ilogb((double)nd / nf) - 13 : 0;
+    k_pr_ctl_init<<<1, 1, 0, s>>>(ctl.get(), e0);
+    check_launch();
+    dangling_base(rank[0].get(), 0);
 
     const double node_bytes = weighted ? 36.0 : 32.0;
     const double f64_edge = weighted ? (use_f32w ? 16.0 : 20.0) : 12.0;
@@ -610,19 +663,19 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
                 if (weighted)
                     k_pr_tiles<float, float, true><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
                         in.off, in.tgt, w32.get(), tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
-                        tail.get(), td.get(), tm.get(), ctl.get(), p, damping);
+                        tail.get(), td.get(), ctl.get(), p, damping);
                 else
                     k_pr_tiles<float, float, false><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
                         in.off, in.tgt, nullptr, tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
-                        tail.get(), td.get(), tm.get(), ctl.get(), p, damping);
+                        tail.get(), td.get(), ctl.get(), p, damping);
             });
             if (weighted)
                 k_pr_fixup<float, true><<<fix_blocks, 256, 0, s>>>(in.off, tx, ty, tiles, n, od, os, rk, nx, cn,
-                                                                   carry.get(), tail.get(), td.get(), tm.get(), fd.get(), fm.get(),
+                                                                   carry.get(), tail.get(), td.get(), fd.get(),
                                                                    ctl.get(), p, damping);
             else
                 k_pr_fixup<float, false><<<fix_blocks, 256, 0, s>>>(in.off, tx, ty, tiles, n, od, os, rk, nx, cn,
-                                                                    carry.get(), tail.get(), td.get(), tm.get(), fd.get(), fm.get(),
+                                                                    carry.get(), tail.get(), td.get(), fd.get(),
                                                                     ctl.get(), p, damping);
         } else {
             const double* cb = contrib[p].get();
@@ -631,31 +684,32 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
                 if (!weighted)
                     k_pr_tiles<double, double, false><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
                         in.off, in.tgt, nullptr, tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
-                        tail.get(), td.get(), tm.get(), ctl.get(), p, damping);
+                        tail.get(), td.get(), ctl.get(), p, damping);
                 else if (use_f32w)
                     k_pr_tiles<double, float, true><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
                         in.off, in.tgt, w32.get(), tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
-                        tail.get(), td.get(), tm.get(), ctl.get(), p, damping);
+                        tail.get(), td.get(), ctl.get(), p, damping);
                 else
                     k_pr_tiles<double, double, true><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
                         in.off, in.tgt, w64.get(), tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
-                        tail.get(), td.get(), tm.get(), ctl.get(), p, damping);
+                        tail.get(), td.get(), ctl.get(), p, damping);
             });
             if (weighted)
                 k_pr_fixup<double, true><<<fix_blocks, 256, 0, s>>>(in.off, tx, ty, tiles, n, od, os, rk, nx,
-                                                                    cn, carry.get(), tail.get(), td.get(), tm.get(), fd.get(), fm.get(),
+                                                                    cn, carry.get(), tail.get(), td.get(), fd.get(),
                                                                     ctl.get(), p, damping);
             else
                 k_pr_fixup<double, false><<<fix_blocks, 256, 0, s>>>(in.off, tx, ty, tiles, n, od, os, rk, nx,
-                                                                     cn, carry.get(), tail.get(), td.get(), tm.get(), fd.get(), fm.get(),
+                                                                     cn, carry.get(), tail.get(), td.get(), fd.get(),
                                                                      ctl.get(), p, damping);
         }
         check_launch();
     };
     auto reduce = [&](uint32_t it, double tol_it) {
-        k_pr_reduce<<<1, 1024, 0, s>>>(fd.get(), fm.get(), fix_blocks, ctl.get(), (int)(it & 1), max_iter, tol_it,
-                                       damping, nf);
+        k_pr_reduce<<<1, 1024, 0, s>>>(fd.get(), fix_blocks, ctl.get(), max_iter, tol_it);
         check_launch();
+        const int q = (int)((it & 1) ^ 1);
+        dangling_base(rank[q].get(), q);
     };
     auto read_ctl = [&]() {
         PrCtl h{};
@@ -742,5 +796,76 @@ extern "C" int ursa_hip_pagerank_mixed(ursa_hip_topo* t, const double* weights, 
     return ursa::guard([&] {
         URSA_REQUIRE(t != nullptr, URSA_HIP_EINVAL, "topology is NULL");
         ursa::pagerank_run(t, weights, damping, max_iter, tol, out, iters_out, true, f32_iters_out);
+    });
+}
+
+namespace ursa {
+namespace {
+
+__global__ void k_os_iota(uint32_t* p, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = (uint32_t)i;
+}
+
+// Candidate window for a free-standing ordered sum: from the sum of the finite
+// magnitudes (any window gives the same result; a close one is faster).
+__global__ __launch_bounds__(1024) void k_os_window(const double* __restrict__ x, uint64_t n, PrCtl* ctl) {
+    __shared__ double s_red[1024 / 64];
+    double a = 0.0;
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const double v = fabs(x[i]);
+        if (v <= 1.7e308) a += v;
+    }
+    const double tot = block_sum(a, s_red);
+    if (threadIdx.x == 0) {
+        const OsState st = os_state(tot);
+        ctl->done = 0;
+        ctl->dm_e0 = st.fast ? st.e - 13 : -1000;
+    }
+}
+
+__global__ __launch_bounds__(OS_WALK_THREADS) void k_os_walk_out(const double* __restrict__ xd, uint64_t d,
+                                                                 const uint64_t* __restrict__ cq,
+                                                                 const uint32_t* __restrict__ cm,
+                                                                 const double* __restrict__ cs, const PrCtl* ctl,
+                                                                 double* out) {
+    __shared__ OsLds lds;
+    const double v = os_walk(xd, d, cq, cm, cs, ctl->dm_e0, lds);
+    if (threadIdx.x == 0) *out = v;
+}
+
+}  // namespace
+
+double ordered_sum(const double* x_user, uint64_t n) {
+    hipStream_t s = stream();
+    if (n == 0) return 0.0;
+    URSA_REQUIRE(n <= 0xFFFFFFFFull, URSA_HIP_ERANGE, "ordered_sum takes fewer than 2^32 values");
+    DevIn<double> x(x_user, n, s);
+    const uint64_t nch = (n + OS_CHUNK - 1) / OS_CHUNK;
+    DevBuf<uint32_t> idx(n);
+    DevBuf<double> xd(n), res(1), cs(nch);
+    DevBuf<uint64_t> cq(nch * OS_BINS);
+    DevBuf<uint32_t> cm(nch);
+    DevBuf<PrCtl> ctl(1);
+    k_os_iota<<<blocks_for(n, 256, 4096), 256, 0, s>>>(idx.get(), n);
+    k_os_window<<<1, 1024, 0, s>>>(x.p, n, ctl.get());
+    const unsigned blocks = (unsigned)((nch + OS_BUILD_THREADS / 64 - 1) / (OS_BUILD_THREADS / 64));
+    k_pr_dm_build<<<blocks, OS_BUILD_THREADS, 0, s>>>(x.p, idx.get(), n, ctl.get(), xd.get(), cq.get(), cm.get(),
+                                                       cs.get());
+    k_os_walk_out<<<1, OS_WALK_THREADS, 0, s>>>(xd.get(), n, cq.get(), cm.get(), cs.get(), ctl.get(), res.get());
+    check_launch();
+    double h = 0.0;
+    HIP_OK(hipMemcpyAsync(&h, res.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    return h;
+}
+
+}  // namespace ursa
+
+extern "C" int ursa_hip_ordered_sum(const double* x, uint64_t n, double* out) {
+    return ursa::guard([&] {
+        URSA_REQUIRE(out != nullptr, URSA_HIP_EINVAL, "out is NULL");
+        URSA_REQUIRE(n == 0 || x != nullptr, URSA_HIP_EINVAL, "x is NULL");
+        *out = ursa::ordered_sum(x, n);
     });
 }
