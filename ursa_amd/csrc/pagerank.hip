@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "ordered_sum.hpp"
+#include "repeated_sum.hpp"
 #include "prim.hpp"
 #include "topology.hpp"
 
@@ -380,11 +381,11 @@ __global__ void k_pr_dang_flags(uint64_t n, const uint32_t* __restrict__ new_of_
     if (u < n) flags[u] = node_scale<WEIGHTED>(out_deg, out_str, new_of_old[u]) <= 0.0 ? 1 : 0;
 }
 
-__global__ void k_pr_ctl_init(PrCtl* ctl, int32_t dm_e0) {
+__global__ void k_pr_ctl_init(PrCtl* ctl, int32_t dm_e0, double base0) {
     ctl->done = 0;
     ctl->iters = 0;
     ctl->delta = 0.0;
-    ctl->base[0] = 0.0;
+    ctl->base[0] = base0;
     ctl->base[1] = 0.0;
     ctl->dm_e0 = dm_e0;
 }
@@ -638,11 +639,14 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
         });
         check_launch();
     };
-    // First candidate window from the initial sum, about nd / n.
-    const int32_t e0 = nd ? (int32_t)std::ilogb((double)nd / nf) - 13 : 0;
-    k_pr_ctl_init<<<1, 1, 0, s>>>(ctl.get(), e0);
+    // The first iterate is 1/n everywhere, so its dangling mass is 1/n added nd
+    // times in order: closed form on the host (every later one on the device).
+    const double dm0 = repeated_sum(1.0 / nf, nd);
+    const double base_a = (1.0 - damping) / nf, base_b = damping * dm0;
+    const double base0 = base_a + base_b / nf;  // pagerank.rs:79
+    const int32_t e0 = dm0 > 0.0 ? (int32_t)std::ilogb(dm0) - 13 : 0;
+    k_pr_ctl_init<<<1, 1, 0, s>>>(ctl.get(), e0, base0);
     check_launch();
-    dangling_base(rank[0].get(), 0);
 
     const double node_bytes = weighted ? 36.0 : 32.0;
     const double f64_edge = weighted ? (use_f32w ? 16.0 : 20.0) : 12.0;
