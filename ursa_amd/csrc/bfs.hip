@@ -142,6 +142,42 @@ __global__ __launch_bounds__(BFS_BLOCK) void k_bfs_td_heavy(const uint32_t* __re
 
 // Bottom-up step: vertices of level L are those unvisited with a reverse
 // neighbour in `front` (level L-1). Writes the whole next bitmap.
+// A lane checks its own vertex's reverse list four slots at a time (the four
+// slot reads, then the four bitmap reads, are independent). Vertices with more
+// than BU_HEAVY reverse slots would hold their wave for the whole list, so the
+// wave takes them together afterwards: 64 slots per step, coalesced, with the
+// early exit decided by a ballot.
+constexpr uint32_t BU_HEAVY = 64;
+
+__device__ __forceinline__ bool bu_scan_lane(const uint32_t* __restrict__ tgt, uint32_t k, uint32_t ke,
+                                             const uint32_t* __restrict__ front) {
+    for (; k + 4 <= ke; k += 4) {
+        const uint32_t w0 = tgt[k], w1 = tgt[k + 1], w2 = tgt[k + 2], w3 = tgt[k + 3];
+        const uint32_t b0 = front[w0 >> 5] >> (w0 & 31), b1 = front[w1 >> 5] >> (w1 & 31);
+        const uint32_t b2 = front[w2 >> 5] >> (w2 & 31), b3 = front[w3 >> 5] >> (w3 & 31);
+        if ((b0 | b1 | b2 | b3) & 1u) return true;
+    }
+    for (; k < ke; ++k) {
+        const uint32_t w = tgt[k];
+        if ((front[w >> 5] >> (w & 31)) & 1u) return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool bu_scan_wave(const uint32_t* __restrict__ tgt, uint32_t k, uint32_t ke,
+                                             const uint32_t* __restrict__ front) {
+    const int lane = lane_id();
+    for (; k < ke; k += 64) {  // k, ke wave-uniform
+        bool f = false;
+        if (k + lane < ke) {
+            const uint32_t w = tgt[k + lane];
+            f = (front[w >> 5] >> (w & 31)) & 1u;
+        }
+        if (__ballot(f)) return true;
+    }
+    return false;
+}
+
 __global__ __launch_bounds__(BFS_BLOCK) void k_bfs_bu(uint64_t n, NbrView rev, NbrView fwd,
                                                       const uint32_t* __restrict__ front,
                                                       uint32_t* __restrict__ next, int32_t* __restrict__ level,
@@ -153,25 +189,33 @@ __global__ __launch_bounds__(BFS_BLOCK) void k_bfs_bu(uint64_t n, NbrView rev, N
     uint32_t awake = 0;
     for (uint64_t base = gw * 64; base < n; base += nwaves * 64) {
         const uint64_t v = base + lane;
-        bool found = false;
+        bool found = false, heavy = false;
         if (v < n && level[v] < 0) {
-            uint32_t k = rev.off_a[v], ke = rev.off_a[v + 1];
-            for (; k < ke && !found; ++k) {
-                const uint32_t w = rev.tgt_a[k];
-                found = (front[w >> 5] >> (w & 31)) & 1u;
+            const uint32_t ka = rev.off_a[v], kea = rev.off_a[v + 1];
+            uint32_t kb = 0, keb = 0;
+            if (rev.off_b) {
+                kb = rev.off_b[v];
+                keb = rev.off_b[v + 1];
             }
-            if (!found && rev.off_b) {
-                k = rev.off_b[v];
-                ke = rev.off_b[v + 1];
-                for (; k < ke && !found; ++k) {
-                    const uint32_t w = rev.tgt_b[k];
-                    found = (front[w >> 5] >> (w & 31)) & 1u;
-                }
+            if ((kea - ka) + (keb - kb) > BU_HEAVY) {
+                heavy = true;
+            } else {
+                found = bu_scan_lane(rev.tgt_a, ka, kea, front);
+                if (!found && rev.off_b) found = bu_scan_lane(rev.tgt_b, kb, keb, front);
             }
-            if (found) {
-                level[v] = L;
-                edges += nv_degree(fwd, (uint32_t)v);
-            }
+        }
+        uint64_t hm = __ballot(heavy);
+        while (hm) {
+            const int hl = __builtin_ctzll(hm);
+            hm &= hm - 1;
+            const uint32_t hv = (uint32_t)(base + hl);
+            bool hf = bu_scan_wave(rev.tgt_a, rev.off_a[hv], rev.off_a[hv + 1], front);
+            if (!hf && rev.off_b) hf = bu_scan_wave(rev.tgt_b, rev.off_b[hv], rev.off_b[hv + 1], front);
+            if (lane == hl) found = hf;
+        }
+        if (found) {
+            level[v] = L;
+            edges += nv_degree(fwd, (uint32_t)v);
         }
         const uint64_t mask = __ballot(found);
         if (lane == 0) {
@@ -180,14 +224,10 @@ __global__ __launch_bounds__(BFS_BLOCK) void k_bfs_bu(uint64_t n, NbrView rev, N
             awake += (uint32_t)__popcll(mask);
         }
     }
-    edges = wave_sum_u64(edges);
-    if (lane == 0) {
-        if (edges) atomicAdd(&ctr->edges, edges);
-        if (awake) atomicAdd(&ctr->qlen, awake);
-    }
+    block_add<BFS_BLOCK>((unsigned long long)edges, &ctr->edges);
+    block_add<BFS_BLOCK>(awake, &ctr->qlen);
 }
 
-// front bitmap of the vertices at level L (whole words per wave).
 __global__ __launch_bounds__(BFS_BLOCK) void k_level_to_bits(const int32_t* __restrict__ level, uint64_t n, int32_t L,
                                                              uint32_t* __restrict__ bits) {
     const int lane = lane_id();
@@ -206,12 +246,11 @@ __global__ __launch_bounds__(BFS_BLOCK) void k_level_to_bits(const int32_t* __re
 // queue of the vertices at level L.
 __global__ __launch_bounds__(BFS_BLOCK) void k_level_to_queue(const int32_t* __restrict__ level, uint64_t n, int32_t L,
                                                               uint32_t* __restrict__ q, BfsCtr* ctr) {
-    const uint64_t gw = blockIdx.x * (uint64_t)WAVES + (threadIdx.x >> 6);
-    const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
-    for (uint64_t base = gw * 64; base < n; base += nwaves * 64) {
-        const uint64_t v = base + lane_id();
+    __shared__ uint32_t s_cnt[WAVES + 1];
+    for (uint64_t b0 = blockIdx.x * (uint64_t)BFS_BLOCK; b0 < n; b0 += (uint64_t)gridDim.x * BFS_BLOCK) {
+        const uint64_t v = b0 + threadIdx.x;
         const bool take = v < n && level[v] == L;
-        const uint32_t slot = wave_append_slot(take, &ctr->qlen);
+        const uint32_t slot = block_append_slot<BFS_BLOCK>(take, &ctr->qlen, s_cnt);
         if (take) q[slot] = (uint32_t)v;
     }
 }
@@ -262,7 +301,7 @@ struct BfsWork {
 static void bfs_one(ursa_hip_topo* t, uint32_t src, const NbrView& fwd, const NbrView& rev, uint64_t fwd_edges,
                     int32_t* level, BfsWork& w, hipStream_t s, uint64_t* reached_edges, double alpha, double beta) {
     const uint64_t n = t->n;
-    const unsigned gb = blocks_for((n + 63) / 64, WAVES, 1u << 16);
+    const unsigned gb = blocks_for((n + 63) / 64, WAVES, GRID_CAP);
     auto read_ctr = [&]() {
         HIP_OK(hipMemcpyAsync(w.h, w.ctr.get(), sizeof(BfsCtr), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));

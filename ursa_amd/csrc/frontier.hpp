@@ -72,4 +72,52 @@ __device__ __forceinline__ uint32_t wave_append_slot(bool take, uint32_t* len) {
     return base + (uint32_t)__popcll(mask & lanemask_lt());
 }
 
+// Counters shared by a whole grid: same-address atomics from every wave
+// serialise in one L2 channel (about 10 ns each on MI355X), which costs more
+// than the work of a frontier kernel over millions of vertices. These helpers
+// make it one atomic per workgroup; kernels that use them also cap their grid
+// (GRID_CAP workgroups, grid-stride loops), so a launch issues at most
+// GRID_CAP atomics per counter.
+constexpr unsigned GRID_CAP = 2048;
+
+// Adds every thread's v into *ctr with one atomic per workgroup. Called once,
+// by every thread of the block, at the end of the kernel.
+template <int BLOCK, class T>
+__device__ __forceinline__ void block_add(T v, T* ctr) {
+    __shared__ T s_part[BLOCK / 64];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane_id() == 0) s_part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T t = 0;
+        for (int w = 0; w < BLOCK / 64; ++w) t += s_part[w];
+        if (t) atomicAdd(ctr, t);
+    }
+    __syncthreads();  // s_part may be reused by another call
+}
+
+// Queue slot for each taking thread, one atomic per workgroup and call. Every
+// thread of the block must call it (block-uniform control flow); s_cnt is an
+// LDS array of BLOCK/64 + 1 words owned by the caller.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_append_slot(bool take, uint32_t* len, uint32_t* s_cnt) {
+    const int wave = (int)(threadIdx.x >> 6);
+    const uint64_t mask = __ballot(take);
+    if (lane_id() == 0) s_cnt[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < BLOCK / 64; ++w) {
+            const uint32_t c = s_cnt[w];
+            s_cnt[w] = tot;
+            tot += c;
+        }
+        s_cnt[BLOCK / 64] = tot ? atomicAdd(len, tot) : 0u;
+    }
+    __syncthreads();
+    const uint32_t slot = s_cnt[BLOCK / 64] + s_cnt[wave] + (uint32_t)__popcll(mask & lanemask_lt());
+    __syncthreads();  // s_cnt is rewritten by the next call
+    return slot;
+}
+
 }  // namespace ursa

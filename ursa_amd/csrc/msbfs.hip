@@ -126,8 +126,7 @@ __global__ __launch_bounds__(MB) void k_ms_pull(uint64_t n, NbrView rev, const u
             cnt += __popcll(acc);
         }
     }
-    cnt = wave_sum_u64(cnt);
-    if (lane_id() == 0 && cnt) atomicAdd(&ctr->newbits, cnt);
+    block_add<MB>(cnt, &ctr->newbits);
 }
 
 // One wave per chunk of a heavy vertex's reverse list (out list then in list
@@ -304,20 +303,18 @@ __global__ __launch_bounds__(MB) void k_ms_push_heavy(const uint32_t* __restrict
 // queue of vertices with a non-zero front mask (pull -> push hand-over).
 __global__ __launch_bounds__(MB) void k_ms_compact(const unsigned long long* __restrict__ front, uint64_t n,
                                                    NbrView fwd, uint32_t* __restrict__ q, MsCtr* ctr) {
-    const uint64_t gw = blockIdx.x * (uint64_t)MW + (threadIdx.x >> 6);
-    const uint64_t nwaves = (uint64_t)gridDim.x * MW;
+    __shared__ uint32_t s_cnt[MW + 1];
     unsigned long long edges = 0;
-    for (uint64_t base = gw * 64; base < n; base += nwaves * 64) {
-        const uint64_t v = base + lane_id();
+    for (uint64_t b0 = blockIdx.x * (uint64_t)MB; b0 < n; b0 += (uint64_t)gridDim.x * MB) {
+        const uint64_t v = b0 + threadIdx.x;
         const bool take = v < n && front[v] != 0;
-        const uint32_t slot = wave_append_slot(take, &ctr->qlen);
+        const uint32_t slot = block_append_slot<MB>(take, &ctr->qlen, s_cnt);
         if (take) {
             q[slot] = (uint32_t)v;
             edges += nv_degree(fwd, (uint32_t)v);
         }
     }
-    edges = wave_sum_u64(edges);
-    if (lane_id() == 0 && edges) atomicAdd(&ctr->edges, edges);
+    block_add<MB>(edges, &ctr->edges);
 }
 
 // TEPS numerator: sum over vertices of forward degree x reached lanes.
@@ -326,8 +323,7 @@ __global__ void k_ms_reached_edges(const unsigned long long* __restrict__ visit,
     unsigned long long acc = 0;
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
         acc += (unsigned long long)nv_degree(fwd, (uint32_t)v) * __popcll(visit[v]);
-    acc = wave_sum_u64(acc);
-    if (lane_id() == 0 && acc) atomicAdd(total, acc);
+    block_add<MB>(acc, total);
 }
 
 }  // namespace
@@ -387,7 +383,7 @@ struct MsWork {
 void msbfs_batch(uint64_t n, uint64_t fwd_edges, const uint32_t* src_host, uint32_t nb, const NbrView& fwd,
                  const NbrView& rev, int32_t* levels, MsWork& w, hipStream_t s, uint64_t* reached_edges,
                  uint32_t push_div) {
-    const unsigned gv = blocks_for(n, MB, 1u << 16);
+    const unsigned gv = blocks_for(n, MB, GRID_CAP);
     k_ms_fill_levels<<<blocks_for(n * nb, 256, 1u << 16), 256, 0, s>>>(levels, n * nb);
     HIP_OK(hipMemsetAsync(w.visit.get(), 0, n * sizeof(unsigned long long), s));
     HIP_OK(hipMemsetAsync(w.f0.get(), 0, n * sizeof(unsigned long long), s));
@@ -415,7 +411,7 @@ void msbfs_batch(uint64_t n, uint64_t fwd_edges, const uint32_t* src_host, uint3
                                               : (have_queue && L == 1);
         if (!have_queue && qlen * 64 < n) {
             HIP_OK(hipMemsetAsync(w.ctr.get(), 0, sizeof(MsCtr), s));
-            k_ms_compact<<<gv / MW + 1, MB, 0, s>>>(front, n, fwd, q, w.ctr.get());
+            k_ms_compact<<<gv, MB, 0, s>>>(front, n, fwd, q, w.ctr.get());
             check_launch();
             const MsCtr c = w.read(s);
             qlen = c.qlen;
