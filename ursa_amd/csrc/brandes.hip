@@ -231,11 +231,27 @@ __global__ void k_vm_seed(const uint32_t* src, uint32_t nb, uint64_t n, int32_t*
     ctr->len = len;
 }
 
+// A wave's newly queued vertices, appended 64 at a time: lane i holds the
+// i-th pending vertex, cnt is wave-uniform.
+struct VmAppend {
+    uint32_t buf = 0;
+    uint32_t cnt = 0;
+};
+
+__device__ __forceinline__ void vm_append_flush(VmAppend& ap, uint32_t* __restrict__ nq, VmCtr* ctr) {
+    if (ap.cnt == 0) return;
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(&ctr->len, ap.cnt);
+    base = __shfl(base, 0, 64);
+    if ((uint32_t)lane_id() < ap.cnt) nq[base + lane_id()] = ap.buf;
+    ap.cnt = 0;
+}
+
 // Expand v's out-edges [e0, e1) for the lanes whose level at v is L-1.
 __device__ __forceinline__ void vm_expand(uint32_t v, uint32_t e0, uint32_t e1, const uint32_t* __restrict__ tgt,
                                           int32_t* __restrict__ dist, double* __restrict__ sigma, int32_t L,
                                           uint32_t* __restrict__ mark, uint32_t* __restrict__ nq, VmCtr* ctr,
-                                          bool count_edges, uint32_t deg, unsigned long long& edges) {
+                                          bool count_edges, uint32_t deg, unsigned long long& edges, VmAppend& ap) {
     const int b = lane_id();
     const bool fl = dist[(uint64_t)v * 64 + b] == L - 1;
     const uint64_t fmask = __ballot(fl);
@@ -256,8 +272,14 @@ __device__ __forceinline__ void vm_expand(uint32_t v, uint32_t e0, uint32_t e1, 
             if (dw == L) unsafeAtomicAdd(&sigma[(uint64_t)w * 64 + b], sv);
         }
         const uint64_t cm = __ballot(claimed);
-        if (cm && b == __ffsll((unsigned long long)cm) - 1) {
-            if (atomicMax(&mark[w], (uint32_t)(L + 1)) < (uint32_t)(L + 1)) nq[atomicAdd(&ctr->len, 1u)] = w;
+        if (cm) {
+            bool fresh = false;
+            if (b == __ffsll((unsigned long long)cm) - 1)
+                fresh = atomicMax(&mark[w], (uint32_t)(L + 1)) < (uint32_t)(L + 1);
+            if (__ballot(fresh)) {  // w is the same on every lane
+                if ((uint32_t)b == ap.cnt) ap.buf = w;
+                if (++ap.cnt == 64) vm_append_flush(ap, nq, ctr);
+            }
         }
     }
 }
@@ -269,13 +291,15 @@ __global__ __launch_bounds__(BB) void k_vm_forward(const uint32_t* __restrict__ 
     const uint64_t gw = blockIdx.x * (uint64_t)BW + (threadIdx.x >> 6);
     const uint64_t nwaves = (uint64_t)gridDim.x * BW;
     unsigned long long edges = 0;
+    VmAppend ap;
     for (uint64_t i = gw; i < qlen; i += nwaves) {
         const uint32_t v = q[i];
         const uint32_t a = off[v], z = off[v + 1];
         if (z - a > VM_HEAVY) continue;  // chunked by k_vm_forward_heavy
-        vm_expand(v, a, z, tgt, dist, sigma, L, mark, nq, ctr, true, z - a, edges);
+        vm_expand(v, a, z, tgt, dist, sigma, L, mark, nq, ctr, true, z - a, edges, ap);
     }
-    if (lane_id() == 0 && edges) atomicAdd(&ctr->edges, edges);
+    vm_append_flush(ap, nq, ctr);
+    block_add<BB>(edges, &ctr->edges);
 }
 
 // Heavy vertices, one wave per chunk of VM_HEAVY edges (chunk c of heavy
@@ -302,15 +326,17 @@ __global__ __launch_bounds__(BB) void k_vm_forward_heavy(const uint32_t* __restr
     const uint64_t gw = blockIdx.x * (uint64_t)BW + (threadIdx.x >> 6);
     const uint64_t nwaves = (uint64_t)gridDim.x * BW;
     unsigned long long edges = 0;
+    VmAppend ap;
     for (uint64_t c = gw; c < chunks; c += nwaves) {
         const uint32_t h = vm_chunk_owner(cbase, nh, (uint32_t)c);
         const uint32_t v = heavy[h];
         const uint32_t a = off[v], z = off[v + 1];
         const uint32_t e0 = a + ((uint32_t)c - cbase[h]) * VM_HEAVY;
         const uint32_t e1 = min(z, e0 + VM_HEAVY);
-        vm_expand(v, e0, e1, tgt, dist, sigma, L, mark, nq, ctr, c == cbase[h], z - a, edges);
+        vm_expand(v, e0, e1, tgt, dist, sigma, L, mark, nq, ctr, c == cbase[h], z - a, edges, ap);
     }
-    if (lane_id() == 0 && edges) atomicAdd(&ctr->edges, edges);
+    vm_append_flush(ap, nq, ctr);
+    block_add<BB>(edges, &ctr->edges);
 }
 
 // delta for lanes at level L: sum over out-slots w at level L+1 of
@@ -452,7 +478,7 @@ static void betweenness_vm(ursa_hip_topo* t, const std::vector<uint32_t>& src, d
         return *h;
     };
     uint64_t total_edges = 0;
-    const unsigned hgrid = blocks_for(std::max<uint32_t>(nchunks, 1), BW, 1u << 16);
+    const unsigned hgrid = blocks_for(std::max<uint32_t>(nchunks, 1), BW, GRID_CAP);
     try {
         for (uint64_t b0 = 0; b0 < src.size(); b0 += 64) {
             const uint32_t nb = (uint32_t)std::min<uint64_t>(64, src.size() - b0);
@@ -471,7 +497,7 @@ static void betweenness_vm(ursa_hip_topo* t, const std::vector<uint32_t>& src, d
                 HIP_OK(hipMemsetAsync(ctr.get(), 0, sizeof(VmCtr), s));
                 const uint32_t qlen = (uint32_t)(z - a);
                 timed("brandes_forward", 0.0, s, [&] {
-                    k_vm_forward<<<blocks_for(qlen, BW, 1u << 16), BB, 0, s>>>(order.get() + a, qlen, o.off.get(),
+                    k_vm_forward<<<blocks_for(qlen, BW, GRID_CAP), BB, 0, s>>>(order.get() + a, qlen, o.off.get(),
                                                                              o.tgt.get(), dist.get(), sigma.get(), L,
                                                                              mark.get(), order.get() + z, ctr.get());
                     if (nchunks)

@@ -40,16 +40,36 @@ struct OCtr {
 
 constexpr uint32_t O_HEAVY = 1024;  // frontier vertices with more edges go grid-wide
 
+// A wave's new candidates wait in LDS (128 per wave) and go to the candidate
+// list 64 or more at a time, so the list's counter sees one atomic per
+// flush instead of one per 64-slot step.
+struct OBuf {
+    uint32_t* s;   // this wave's 128 LDS words
+    uint32_t cnt;  // wave-uniform
+};
+
+__device__ __forceinline__ void o_flush(OBuf& ob, uint32_t* __restrict__ cand, OCtr* ctr) {
+    if (ob.cnt == 0) return;
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(&ctr->count, ob.cnt);
+    base = __shfl(base, 0, 64);
+    for (uint32_t i = lane_id(); i < ob.cnt; i += 64) cand[base + i] = ob.s[i];
+    ob.cnt = 0;
+}
+
 // Offers v, reached by frontier position p through its slot j: atomicMin of
 // the discovery key, and the thread that replaced the initial key appends v to
-// the candidate list (wave-aggregated).
+// the candidate list.
 __device__ __forceinline__ void o_offer(bool active, uint32_t v, uint64_t p, uint32_t j,
                                         const int32_t* __restrict__ level, unsigned long long* __restrict__ key,
-                                        uint32_t* __restrict__ cand, OCtr* ctr) {
+                                        uint32_t* __restrict__ cand, OCtr* ctr, OBuf& ob) {
     bool fresh = false;
     if (active && level[v] < 0) fresh = atomicMin(&key[v], ((unsigned long long)p << 32) | j) == ~0ull;
-    const uint32_t slot = wave_append_slot(fresh, &ctr->count);
-    if (fresh) cand[slot] = v;
+    const uint64_t mask = __ballot(fresh);
+    if (!mask) return;
+    if (fresh) ob.s[ob.cnt + (uint32_t)__popcll(mask & lanemask_lt())] = v;
+    ob.cnt += (uint32_t)__popcll(mask);  // < 128: flushed below once it reaches 64
+    if (ob.cnt >= 64) o_flush(ob, cand, ctr);
 }
 
 // Expand frontier F[0..flen): candidates get atomicMin(key, p<<32 | j) and are
@@ -59,9 +79,11 @@ __global__ __launch_bounds__(TB) void k_oexpand(const uint32_t* __restrict__ F, 
                                                 const int32_t* __restrict__ level,
                                                 unsigned long long* __restrict__ key, uint32_t* __restrict__ cand,
                                                 OCtr* ctr, uint32_t* __restrict__ heavy) {
+    __shared__ uint32_t s_buf[TW][128];
     const int lane = lane_id();
     const uint64_t gw = blockIdx.x * (uint64_t)TW + (threadIdx.x >> 6);
     const uint64_t nwaves = (uint64_t)gridDim.x * TW;
+    OBuf ob{s_buf[threadIdx.x >> 6], 0};
     for (uint64_t base = gw * 64; base < flen; base += nwaves * 64) {
         const uint64_t idx = base + lane;
         uint32_t u = 0, d = 0, da = 0, sa = 0, sb = 0;
@@ -87,9 +109,10 @@ __global__ __launch_bounds__(TB) void k_oexpand(const uint32_t* __restrict__ F, 
             const uint32_t osb = __shfl(sb, j, 64);
             uint32_t v = 0;
             if (t < total) v = e < oa ? fwd.tgt_a[osa + e] : fwd.tgt_b[osb + (e - oa)];
-            o_offer(t < total, v, base + j, e, level, key, cand, ctr);
+            o_offer(t < total, v, base + j, e, level, key, cand, ctr, ob);
         }
     }
+    o_flush(ob, cand, ctr);
 }
 
 // Deferred frontier vertices: every wave of the grid takes a stride of each
@@ -99,6 +122,8 @@ __global__ __launch_bounds__(TB) void k_oexpand_heavy(const uint32_t* __restrict
                                                       unsigned long long* __restrict__ key,
                                                       uint32_t* __restrict__ cand, OCtr* ctr,
                                                       const uint32_t* __restrict__ heavy) {
+    __shared__ uint32_t s_buf[TW][128];
+    OBuf ob{s_buf[threadIdx.x >> 6], 0};
     const uint32_t cnt = ctr->heavy;
     const uint64_t wave_base = (blockIdx.x * (uint64_t)TW + (threadIdx.x >> 6)) * 64;
     const uint64_t stride = (uint64_t)gridDim.x * TB;
@@ -113,9 +138,10 @@ __global__ __launch_bounds__(TB) void k_oexpand_heavy(const uint32_t* __restrict
             const uint32_t e = (uint32_t)(e0 + lane_id());
             uint32_t v = 0;
             if (e < d) v = e < da ? fwd.tgt_a[sa + e] : fwd.tgt_b[sb + (e - da)];
-            o_offer(e < d, v, p, e, level, key, cand, ctr);
+            o_offer(e < d, v, p, e, level, key, cand, ctr, ob);
         }
     }
+    o_flush(ob, cand, ctr);
 }
 
 __global__ void k_ogather_keys(const uint32_t* __restrict__ cand, uint32_t cnt,
