@@ -335,6 +335,43 @@ def bench_bfs_large(ua, lib, topo, n, m):
     return out
 
 
+def bench_egress(ua, lib, topo, n, m, out):
+    """The formats either side of the kernels at R-MAT s24: the f64 result (8 n bytes)
+    copied to pageable host memory and to a pinned buffer from ursa_hip_host_alloc
+    (the Arrow egress), and per-row edge weights gathered into in-CSR slot order on
+    the device (algorithmic bytes 20 m: edge id 4 + weight 8 + slot write 8)."""
+    res = {}
+    nbytes = 8 * n
+    pageable = np.empty(n, np.float64)
+    pinned = ua.PinnedBuffer(nbytes)
+    for name, dst in (("pageable", pageable.ctypes.data_as(C.c_void_p)), ("pinned", C.c_void_p(pinned.ptr))):
+        check(lib.ursa_hip_memcpy_to_host(dst, out.p, nbytes))
+        t0 = time.perf_counter()
+        for _ in range(3):
+            check(lib.ursa_hip_memcpy_to_host(dst, out.p, nbytes))
+        ms = (time.perf_counter() - t0) / 3 * 1e3
+        res[f"result_to_{name}"] = {"MB": round(nbytes / 1e6, 1), "ms": round(ms, 3), "GBps": round(nbytes / ms / 1e6, 1)}
+        log(f"egress: f64 result {nbytes / 1e6:.0f} MB to {name} host memory in {ms:.2f} ms ({nbytes / ms / 1e6:.1f} GB/s)")
+    pinned.free()
+    w_host = np.random.default_rng(11).random(m)
+    w_dev, slots = Dev(lib, 8 * m), Dev(lib, 8 * m)
+    check(lib.ursa_hip_memcpy_to_device(w_dev.p, w_host.ctypes.data_as(C.c_void_p), 8 * m))
+    del w_host
+    check(lib.ursa_hip_gather_edge_weights(topo._h, 1, w_dev.p, slots.p))
+    ua.profile.enable(True)
+    ua.profile.reset()
+    check(lib.ursa_hip_gather_edge_weights(topo._h, 1, w_dev.p, slots.p))
+    q = ua.profile.query("gather_edge_weights")
+    ua.profile.enable(False)
+    w_dev.free()
+    slots.free()
+    gbs = q["algo_bytes"] / (q["total_ms"] / 1e3) / 1e9 if q["total_ms"] > 0 else 0.0
+    res["weight_gather_in_csr"] = {"m": m, "kernel_ms": round(q["total_ms"], 3), "GBps": round(gbs, 1),
+                                   "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    log(f"egress: weight gather into in-CSR order, {m} slots in {q['total_ms']:.2f} ms ({gbs:.0f} GB/s)")
+    return res
+
+
 def bench_pagerank(args, ua, gen):
     lib = ua.load()
     scale = args.pr_scale
@@ -387,6 +424,7 @@ def bench_pagerank(args, ua, gen):
                                                          "k_pr_tiles" if key.endswith("f64") else "k_pr_tiles_f32")}
     res["mixed_speedup"] = round(res["f64"]["ms"] / res["mixed"]["ms"], 3)
     res["bfs"] = bench_bfs_large(ua, lib, topo, n, m)
+    res["egress"] = bench_egress(ua, lib, topo, n, m, out)
     # accuracy against the f64 oracle
     if not args.no_cpu_baseline:
         import oracle as orc

@@ -104,6 +104,24 @@ int ursa_hip_ppr_batch(ursa_hip_topo* t, uint32_t k, const uint64_t* seed_offset
  * exactly `bytes` in and `bytes` out). */
 int ursa_hip_copy_probe(const void* src_device, void* dst_device, uint64_t bytes);
 
+/* ---- the formats either side of the kernels ---- */
+/* Edge weights in adjacency-slot order: w_slots[k] = w_rows[edge_ids[k]] for
+ * the out-CSR (which = 0) or the in-CSR (which = 1), the gather a weighted
+ * kernel does per slot (ursa-plan/src/weight.rs:8-10, ursa-core/src/algo/
+ * pagerank.rs:159-161). w_rows is evaluate_weight's one f64 per edge row
+ * (ursa-plan/src/weight.rs:51-106); like evaluate_weight, any negative weight
+ * is URSA_HIP_EINVAL ("weights must be non-negative"; w_slots is filled
+ * anyway). Host or device pointers, m entries each. */
+int ursa_hip_gather_edge_weights(ursa_hip_topo* t, int which, const double* w_rows, double* w_slots);
+
+/* Pinned host memory for result buffers: algo_array moves a kernel's Vec into
+ * an Arrow array without a copy (ursa-plan/src/result.rs:84-139); a buffer
+ * from here can back that array (freed by ursa_hip_host_free when the array
+ * is dropped), and any entry point writing a result into it moves it by DMA
+ * at link speed. bytes == 0 gives NULL. */
+int ursa_hip_host_alloc(uint64_t bytes, void** out);
+int ursa_hip_host_free(void* p);
+
 /* ---- device memory for callers without a HIP runtime binding ---- */
 int ursa_hip_device_alloc(uint64_t bytes, void** out);
 int ursa_hip_device_free(void* p);

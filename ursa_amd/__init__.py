@@ -37,6 +37,8 @@ __all__ = [
     "pagerank_weighted",
     "pagerank_mixed",
     "ordered_sum",
+    "gather_edge_weights",
+    "PinnedBuffer",
     "ppr",
     "ppr_batch",
     "bfs_distances",
@@ -327,6 +329,50 @@ def autotune(topo: Topology, direction=Direction.Out, force: bool = False) -> di
                                    C.byref(src)))
     return {"alpha": a.value, "beta": b.value, "push_div": pd.value,
             "source": {0: "default", 1: "measured", 2: "cached"}[src.value]}
+
+
+def gather_edge_weights(topo: Topology, weights, which="in") -> np.ndarray:
+    """Per-edge-row weights (evaluate_weight's output, ursa-plan/src/weight.rs:51-106)
+    put in adjacency-slot order, w[edge_ids[k]], for the out- or in-CSR; negative
+    weights raise UrsaHipError (code -22) as evaluate_weight does."""
+    m = topo.n_edges()
+    w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64))
+    if w.shape[0] != m:
+        raise ValueError("weights length must equal the edge count")
+    out = np.zeros(max(m, 1), np.float64)
+    check(load().ursa_hip_gather_edge_weights(topo._h, {"out": 0, "in": 1}[which], _ptr(w) if m else None,
+                                              _ptr(out) if m else None))
+    return out[:m]
+
+
+class PinnedBuffer:
+    """Pinned host memory from ursa_hip_host_alloc (the backing store an Arrow result
+    array can take over, ursa-plan/src/result.rs:84-139). ``array(dtype, count)`` views
+    it; the views must not outlive ``free()``."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(load().ursa_hip_host_alloc(self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def array(self, dtype, count: int) -> np.ndarray:
+        dt = np.dtype(dtype)
+        if count * dt.itemsize > self.nbytes or self.ptr is None:
+            raise ValueError("view larger than the buffer")
+        raw = (C.c_char * (count * dt.itemsize)).from_address(self.ptr)
+        return np.frombuffer(raw, dtype=dt, count=count)
+
+    def free(self):
+        if self.ptr is not None:
+            check(load().ursa_hip_host_free(C.c_void_p(self.ptr)))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def ordered_sum(x) -> float:

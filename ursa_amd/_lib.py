@@ -89,6 +89,9 @@ SIGNATURES = {
     "ursa_hip_topology_create_from_csr": (cint, [u64, u64, P, P, P, C.POINTER(P)]),
     "ursa_hip_autotune": (cint, [P, cint, cint, C.POINTER(f64), C.POINTER(f64), C.POINTER(u32), C.POINTER(cint)]),
     "ursa_hip_ordered_sum": (cint, [P, u64, C.POINTER(f64)]),
+    "ursa_hip_gather_edge_weights": (cint, [P, cint, P, P]),
+    "ursa_hip_host_alloc": (cint, [u64, C.POINTER(P)]),
+    "ursa_hip_host_free": (cint, [P]),
     "ursa_hip_ppr_batch": (cint, [P, u32, P, P, f64, u32, f64, P, P]),
     "ursa_hip_server_create": (cint, [P, C.POINTER(P)]),
     "ursa_hip_server_add_graph": (cint, [P, P, C.POINTER(u32)]),
