@@ -70,6 +70,15 @@ int ursa_hip_diameter(ursa_hip_topo* t, int approximate, int64_t* out);
  * nothing. sources = NULL: every node (out f64[n]); else out[i] for sources[i]
  * (f64[k]; an out-of-range source gives 0.0). */
 int ursa_hip_closeness(ursa_hip_topo* t, const uint32_t* sources, uint64_t k, double* out);
+
+/* closeness_weighted (closeness.rs:143-222): for sources[i] (all nodes in
+ * order when sources is NULL; out-of-range sources give 0.0), reachable /
+ * sum of the shortest-path costs over out-edges with weights[edge row]
+ * (f64[m], non-negative: URSA_HIP_EINVAL otherwise), the sum taken in node
+ * order as the reference takes it. Costs are exactly Dijkstra's (min over
+ * paths of the left-to-right rounded path cost), so values are bit-identical. */
+int ursa_hip_closeness_weighted(ursa_hip_topo* t, const double* weights, const uint32_t* sources, uint64_t k,
+                                double* out);
 /* clustering_coefficient (ursa-core/src/algo/clustering.rs:21-40):
  * 2 T(u) / (k (k - 1)) over the undirected view, 0.0 for degree < 2; f64[n]. */
 int ursa_hip_clustering_coefficient(ursa_hip_topo* t, double* out);

@@ -57,6 +57,7 @@ def _load() -> C.CDLL:
         "orc_avg_path_length": (f64, [P, i, f64]),
         "orc_diameter": (C.c_int64, [P, i]),
         "orc_closeness": (None, [P, P, u64, P]),
+        "orc_closeness_weighted": (None, [P, P, P, u64, P]),
         "orc_clustering": (None, [P, P]),
         "orc_k_hop": (u64, [P, P, u64, u32, i, P, P, u64]),
         "orc_shortest_path": (u64, [P, u32, u32, i, P, u64]),
@@ -185,6 +186,15 @@ def closeness(t: Topology, sources=None) -> np.ndarray:
     s = _u32(np.arange(t.n) if sources is None else sources)
     out = np.zeros(max(s.shape[0], 1), np.float64)
     lib().orc_closeness(t._h, _p(s) if s.size else None, s.shape[0], _p(out))
+    return out[: s.shape[0]]
+
+
+def closeness_weighted(t: Topology, weights, sources=None) -> np.ndarray:
+    w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64))
+    assert w.shape[0] == t.m, "weights length must equal the edge count"
+    s = _u32(np.arange(t.n) if sources is None else sources)
+    out = np.zeros(max(s.shape[0], 1), np.float64)
+    lib().orc_closeness_weighted(t._h, _p(w) if w.size else None, _p(s) if s.size else None, s.shape[0], _p(out))
     return out[: s.shape[0]]
 
 

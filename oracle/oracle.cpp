@@ -33,6 +33,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <queue>
 #include <thread>
 #include <unordered_map>
 #include <unordered_set>
@@ -493,6 +494,52 @@ void orc_closeness(orc_topo* t, const uint32_t* sources, uint64_t k, double* out
                 ++reach;
             }
         out[i] = sum > 0 ? (double)reach / (double)sum : 0.0;
+    }
+}
+
+// ---- closeness.rs:143-222 (weighted, Out) ----
+// Dijkstra from each source over out-edges with weight w[edge row]
+// (for_each_weighted_neighbor, topology.rs), then the node-order (0..n) f64
+// sum of the finite distances of the other nodes (closeness.rs:199-215).
+void orc_closeness_weighted(orc_topo* t, const double* w, const uint32_t* sources, uint64_t k, double* out) {
+    const uint64_t n = t->n;
+    std::vector<double> dist(n);
+    std::vector<char> settled(n);
+    typedef std::pair<double, uint32_t> Item;  // (distance, node): the reference's heap key
+    for (uint64_t i = 0; i < k; ++i) {
+        out[i] = 0.0;
+        const uint32_t u = sources[i];
+        if (u >= n) continue;
+        std::fill(dist.begin(), dist.end(), INFINITY);
+        std::fill(settled.begin(), settled.end(), 0);
+        std::priority_queue<Item, std::vector<Item>, std::greater<Item>> heap;
+        dist[u] = 0.0;
+        heap.push(Item(0.0, u));
+        while (!heap.empty()) {
+            const Item top = heap.top();
+            heap.pop();
+            const double du = top.first;
+            const uint32_t v = top.second;
+            if (settled[v]) continue;
+            settled[v] = 1;
+            for (uint64_t e = t->out.off[v]; e < t->out.off[v + 1]; ++e) {
+                const uint32_t x = t->out.tgt[e];
+                if (settled[x]) continue;
+                const double nd = du + w[t->out.eid[e]];
+                if (nd < dist[x]) {
+                    dist[x] = nd;
+                    heap.push(Item(nd, x));
+                }
+            }
+        }
+        double sum = 0.0;
+        int64_t reach = 0;
+        for (uint64_t v = 0; v < n; ++v)
+            if (v != u && std::isfinite(dist[v])) {
+                sum += dist[v];
+                ++reach;
+            }
+        out[i] = sum > 0.0 ? (double)reach / sum : 0.0;
     }
 }
 

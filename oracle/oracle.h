@@ -81,6 +81,8 @@ double orc_avg_path_length(orc_topo* t, int has_sample, double sample);
 int64_t orc_diameter(orc_topo* t, int approximate);
 /* closeness.rs:69-137 (unweighted, Out): out[i] for sources[i]. */
 void orc_closeness(orc_topo* t, const uint32_t* sources, uint64_t k, double* out);
+/* closeness.rs:143-222 (weighted, Dijkstra over Out): w[m] per edge row. */
+void orc_closeness_weighted(orc_topo* t, const double* w, const uint32_t* sources, uint64_t k, double* out);
 /* clustering.rs:21-40 over the undirected view and u64 triangle counts. */
 void orc_clustering(orc_topo* t, double* out);
 /* hop.rs:52-130. Returns number of pairs; writes min(cap, count). */

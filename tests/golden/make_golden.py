@@ -149,6 +149,17 @@ def main():
     fx["gnp100_close_ids"], fx["gnp100_close"] = keyed(nx.closeness_centrality(gd.reverse(), wf_improved=False))
     fx["sparse400_close_ids"], fx["sparse400_close"] = keyed(
         nx.closeness_centrality(g3d.reverse(), wf_improved=False))
+    # weighted closeness (closeness.rs:143-222): Dijkstra distances over
+    # seeded per-row weights, same reversal and wf_improved=False
+    for name, gg, ss, dd in (("gnp100", gd, s, d), ("sparse400", g3d, s3, d3)):
+        wr = np.random.default_rng(77).random(len(ss)) + 0.05
+        gw2 = nx.DiGraph()
+        gw2.add_nodes_from(gg.nodes())
+        for a_, b_, x_ in zip(ss.tolist(), dd.tolist(), wr.tolist()):
+            gw2.add_edge(a_, b_, weight=x_)
+        fx[f"{name}_wclose_w"] = wr
+        fx[f"{name}_wclose_ids"], fx[f"{name}_wclose"] = keyed(
+            nx.closeness_centrality(gw2.reverse(), distance="weight", wf_improved=False))
 
     # splitmix64 streams
     seeds = np.array([0, 4, 7, 42, 0x5EED5EED5EED5EED, 1234567], dtype=np.uint64)

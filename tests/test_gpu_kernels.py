@@ -354,3 +354,43 @@ def test_closeness_clustering_goldens(ua):  # closeness.rs:233-275, clustering.r
     assert c.tolist() == [0.5, 2.0 / 3.0, 1.0, 0.0]
     cc = ua.clustering_coefficient(ua.Topology.build(4, [0, 0, 0, 1], [1, 2, 3, 2]))
     assert abs(cc[0] - 1 / 3) < 1e-15 and cc[3] == 0.0
+
+
+# ---------------------------------------------------------------- weighted closeness (closeness.rs:143-222)
+@pytest.mark.parametrize("case", GRAPHS, ids=IDS)
+def test_closeness_weighted_matches_oracle(ua, case):
+    """Dijkstra costs (min over paths of the left-to-right rounded cost) and the
+    node-order sum are reproduced exactly, so the values are bit-identical."""
+    name, n, s, d = case
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    w = np.random.default_rng(21).random(len(s)) + 0.01
+    w[::9] = 0.0  # zero-cost edges: reachable nodes at cost 0 still count
+    srcs = np.array(list(range(0, n, max(1, n // 70)))[:70] + [n + 4])
+    assert np.array_equal(ua.closeness_weighted(t, w, srcs), orc.closeness_weighted(o, w, srcs)), name
+    if n <= 600:
+        assert np.array_equal(ua.closeness_weighted(t, w), orc.closeness_weighted(o, w)), name
+
+
+def test_closeness_weighted_rmat_and_edge_cases(ua):
+    from ursa_amd import gen
+
+    e = gen.rmat(12, 8, seed=6)
+    _, _, sd, dd, _ = e.export()
+    t, o = e.topology(), orc.Topology(e.n, sd, dd)
+    w = np.random.default_rng(2).random(e.m)
+    srcs = np.arange(0, e.n, 37)[:130]
+    assert np.array_equal(ua.closeness_weighted(t, w, srcs), orc.closeness_weighted(o, w, srcs))
+    # unit weights give the hop-count closeness exactly
+    assert np.array_equal(ua.closeness_weighted(t, np.ones(e.m), srcs), ua.closeness(t, srcs))
+    # reference goldens (closeness.rs:277-309)
+    c = ua.closeness_weighted(ua.Topology.build(3, [0, 1, 0], [1, 2, 2]), [1.0, 1.0, 5.0])
+    assert abs(c[0] - 2.0 / 3.0) < 1e-15 and abs(c[1] - 1.0) < 1e-15
+    c = ua.closeness_weighted(ua.Topology.build(3, [0, 1], [1, 2]), [0.0, 1.0])
+    assert c[0] == 2.0 and c[1] == 1.0
+    assert ua.closeness_weighted(ua.Topology.build(3, [0, 0], [1, 2]), [0.0, 0.0])[0] == 0.0
+    assert ua.closeness_weighted(ua.Topology.build(0, [], []), []).size == 0
+    with pytest.raises(ua.UrsaHipError):
+        ua.closeness_weighted(ua.Topology.build(3, [0, 1], [1, 2]), [1.0, -1.0])
+    with pytest.raises(ValueError):
+        ua.closeness_weighted(ua.Topology.build(3, [0, 1], [1, 2]), [1.0])

@@ -506,6 +506,31 @@ def test_closeness_goldens():  # closeness.rs:233-275
     assert np.all(np.abs(c - c[0]) < 1e-12) and c[0] > 0
 
 
+def test_closeness_weighted_goldens():  # closeness.rs:277-309
+    c = orc.closeness_weighted(orc.Topology(3, [0, 1, 0], [1, 2, 2]), [1.0, 1.0, 5.0])
+    assert abs(c[0] - 2.0 / 3.0) < 1e-12 and abs(c[1] - 1.0) < 1e-12
+    c = orc.closeness_weighted(orc.Topology(3, [0, 1], [1, 2]), [0.0, 1.0])  # zero-cost node counted
+    assert abs(c[0] - 2.0) < 1e-12 and abs(c[1] - 1.0) < 1e-12
+    assert orc.closeness_weighted(orc.Topology(3, [0, 0], [1, 2]), [0.0, 0.0])[0] == 0.0
+    # unit weights reduce to the hop-count closeness exactly (integer distances)
+    n = 300
+    rng = np.random.default_rng(4)
+    s, d = rng.integers(0, n, 1500), rng.integers(0, n, 1500)
+    t = orc.Topology(n, s, d)
+    assert np.array_equal(orc.closeness_weighted(t, np.ones(1500)), orc.closeness(t))
+    # scaling every weight by 2 halves every value exactly
+    w = rng.random(1500)
+    assert np.array_equal(orc.closeness_weighted(t, 2.0 * w) * 2.0, orc.closeness_weighted(t, w))
+
+
+@pytest.mark.parametrize("name", ["gnp100", "sparse400"])
+def test_closeness_weighted_vs_networkx(name):
+    t, users = topo_from_user(FX[f"{name}_src"], FX[f"{name}_dst"])
+    got = by_user(users, orc.closeness_weighted(t, FX[f"{name}_wclose_w"]))
+    for k, v in zip(FX[f"{name}_wclose_ids"].tolist(), FX[f"{name}_wclose"].tolist()):
+        assert abs(got[k] - v) <= 1e-14 * max(abs(v), 1e-300)
+
+
 def test_clustering_goldens():  # clustering.rs:46-78
     assert orc.clustering(orc.Topology(3, [0, 0, 1, 2], [1, 2, 2, 0])).tolist() == [1.0, 1.0, 1.0]
     assert orc.clustering(orc.Topology(3, [0, 1], [1, 2])).tolist() == [0.0, 0.0, 0.0]

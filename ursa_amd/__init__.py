@@ -32,6 +32,7 @@ __all__ = [
     "avg_path_length",
     "diameter",
     "closeness",
+    "closeness_weighted",
     "clustering_coefficient",
     "pagerank",
     "pagerank_weighted",
@@ -203,6 +204,23 @@ def closeness(topo: Topology, sources=None) -> np.ndarray:
     src = _u32(sources)
     out = np.zeros(max(src.shape[0], 1), np.float64)
     check(load().ursa_hip_closeness(topo._h, _ptr(src), src.shape[0], _ptr(out)))
+    return out[: src.shape[0]]
+
+
+def closeness_weighted(topo: Topology, weights, sources=None) -> np.ndarray:
+    """closeness_weighted (ursa-core/src/algo/closeness.rs:143-222): shortest-path costs
+    over out-edges with weights[edge row], for every node or the given sources -> f64."""
+    w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64))
+    if w.shape[0] != topo.n_edges():
+        raise ValueError("weights length must equal the edge count")
+    wp = _ptr(w) if w.size else None
+    if sources is None:
+        out = np.zeros(max(topo.n_nodes(), 1), np.float64)
+        check(load().ursa_hip_closeness_weighted(topo._h, wp, None, 0, _ptr(out)))
+        return out[: topo.n_nodes()]
+    src = _u32(sources)
+    out = np.zeros(max(src.shape[0], 1), np.float64)
+    check(load().ursa_hip_closeness_weighted(topo._h, wp, _ptr(src), src.shape[0], _ptr(out)))
     return out[: src.shape[0]]
 
 

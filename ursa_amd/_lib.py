@@ -90,6 +90,7 @@ SIGNATURES = {
     "ursa_hip_autotune": (cint, [P, cint, cint, C.POINTER(f64), C.POINTER(f64), C.POINTER(u32), C.POINTER(cint)]),
     "ursa_hip_ordered_sum": (cint, [P, u64, C.POINTER(f64)]),
     "ursa_hip_gather_edge_weights": (cint, [P, cint, P, P]),
+    "ursa_hip_closeness_weighted": (cint, [P, P, P, u64, P]),
     "ursa_hip_host_alloc": (cint, [u64, C.POINTER(P)]),
     "ursa_hip_host_free": (cint, [P]),
     "ursa_hip_ppr_batch": (cint, [P, u32, P, P, f64, u32, f64, P, P]),
