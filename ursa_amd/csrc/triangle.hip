@@ -28,7 +28,7 @@ namespace {
 
 constexpr int TCB = 256;
 constexpr int TCW = TCB / 64;
-constexpr int TCB_BIG = 128;  // threads of a workgroup sharing one long list
+constexpr int TCB_BIG = 256;  // threads of a workgroup sharing one long list
 constexpr uint32_t STAGE_CAP = 256;  // N+(u) entries staged in LDS per wave
 constexpr uint32_t HASH_CAP = 512;   // hash slots per wave (>= 2 * STAGE_CAP)
 constexpr uint32_t TC_BUCKET = 8;    // hash slots per bucket
