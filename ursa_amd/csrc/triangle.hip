@@ -60,7 +60,9 @@ __global__ void k_tc_item_flags(const uint32_t* __restrict__ ooff, const uint32_
         const uint32_t u = oown[k];
         const uint32_t a = ooff[u], du = ooff[u + 1] - a;
         bool f = false;
-        const uint64_t win = du > STAGE_CAP ? 64 * TC_CHUNK : TC_CHUNK;  // long lists: amortise staging
+        // Lists staged per workgroup or read from HBM are cut 512x coarser (about
+        // 4 M steps): their staging and count flush are paid once per item.
+        const uint64_t win = du > STAGE_CAP ? 512 * TC_CHUNK : TC_CHUNK;
         if (du >= 2) f = (k == a) || (pre[k] / win != pre[k - 1] / win);
         flag[k] = f;
         iota[k] = (uint32_t)k;
