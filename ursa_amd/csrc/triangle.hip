@@ -34,7 +34,7 @@ constexpr uint32_t HASH_CAP = 512;   // hash slots per wave (>= 2 * STAGE_CAP)
 constexpr uint32_t TC_BUCKET = 8;    // hash slots per bucket
 constexpr uint32_t BIG_CAP = 2048;   // N+(u) entries staged in LDS per workgroup
 constexpr uint64_t TC_CHUNK = 8192;  // intersection steps per work item (about)
-constexpr int TC_ILP = 4;            // 64-step groups whose loads are in flight together
+constexpr int TC_ILP = 2;            // 64-step groups whose loads are in flight together
 constexpr uint32_t TC_LONG = 32;     // walks at least this long are taken by the whole wave
 constexpr int TC_LILP = 4;           // 64-step chunks of long walks in flight together
 
