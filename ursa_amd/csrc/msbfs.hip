@@ -96,6 +96,24 @@ __global__ void k_ms_seed(const uint32_t* src, uint32_t nb, uint64_t n, NbrView 
 constexpr uint32_t PULL_HEAVY = 256;
 constexpr uint32_t PULL_CHUNK = 4096;
 
+// OR of the frontier masks over list slots [k, ke) until every wanted lane is
+// covered; four slots (and their mask reads) in flight per step.
+__device__ __forceinline__ unsigned long long ms_pull_list(const uint32_t* __restrict__ tgt, uint32_t k, uint32_t ke,
+                                                           const unsigned long long* __restrict__ front,
+                                                           unsigned long long want) {
+    unsigned long long acc = 0;
+    for (; k + 4 <= ke; k += 4) {
+        const uint32_t w0 = tgt[k], w1 = tgt[k + 1], w2 = tgt[k + 2], w3 = tgt[k + 3];
+        acc |= front[w0] | front[w1] | front[w2] | front[w3];
+        if ((acc & want) == want) return acc;
+    }
+    for (; k < ke; ++k) {
+        acc |= front[tgt[k]];
+        if ((acc & want) == want) break;
+    }
+    return acc;
+}
+
 __global__ __launch_bounds__(MB) void k_ms_pull(uint64_t n, NbrView rev, const unsigned long long* __restrict__ front,
                                                 unsigned long long* __restrict__ next,
                                                 unsigned long long* __restrict__ visit, unsigned long long valid,
@@ -107,16 +125,9 @@ __global__ __launch_bounds__(MB) void k_ms_pull(uint64_t n, NbrView rev, const u
         const unsigned long long want = valid & ~vis;
         unsigned long long acc = 0;
         if (want) {
-            for (uint32_t k = rev.off_a[v], ke = rev.off_a[v + 1]; k < ke; ++k) {
-                acc |= front[rev.tgt_a[k]];
-                if ((acc & want) == want) break;
-            }
-            if (rev.off_b && (acc & want) != want) {
-                for (uint32_t k = rev.off_b[v], ke = rev.off_b[v + 1]; k < ke; ++k) {
-                    acc |= front[rev.tgt_b[k]];
-                    if ((acc & want) == want) break;
-                }
-            }
+            acc = ms_pull_list(rev.tgt_a, rev.off_a[v], rev.off_a[v + 1], front, want);
+            if (rev.off_b && (acc & want) != want)
+                acc |= ms_pull_list(rev.tgt_b, rev.off_b[v], rev.off_b[v + 1], front, want & ~acc);
             acc &= want;
         }
         next[v] = acc;
