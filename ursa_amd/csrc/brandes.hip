@@ -258,27 +258,38 @@ __device__ __forceinline__ void vm_expand(uint32_t v, uint32_t e0, uint32_t e1, 
     if (!fmask) return;
     if (count_edges && b == 0) edges += (unsigned long long)deg * __popcll(fmask);
     const double sv = fl ? sigma[(uint64_t)v * 64 + b] : 0.0;
-    for (uint32_t e = e0; e < e1; ++e) {
-        const uint32_t w = tgt[e];
-        bool claimed = false;
-        if (fl) {
-            int32_t* dp = &dist[(uint64_t)w * 64 + b];
-            int32_t dw = *dp;
-            if (dw < 0) {
-                const int32_t old = atomicCAS(dp, -1, L);
-                claimed = old == -1;
-                dw = claimed ? L : old;
+    for (uint32_t e0q = e0; e0q < e1; e0q += 4) {
+        // four slots' targets and current levels read together, then taken in slot order
+        uint32_t wq[4];
+        int32_t dq[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) wq[j] = e0q + j < e1 ? tgt[e0q + j] : 0u;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) dq[j] = (fl && e0q + j < e1) ? dist[(uint64_t)wq[j] * 64 + b] : 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (e0q + j >= e1) break;  // wave-uniform
+            const uint32_t w = wq[j];
+            bool claimed = false;
+            if (fl) {
+                int32_t* dp = &dist[(uint64_t)w * 64 + b];
+                int32_t dw = dq[j];
+                if (dw < 0) {  // a repeated target within the four sees the claim through the CAS
+                    const int32_t old = atomicCAS(dp, -1, L);
+                    claimed = old == -1;
+                    dw = claimed ? L : old;
+                }
+                if (dw == L) unsafeAtomicAdd(&sigma[(uint64_t)w * 64 + b], sv);
             }
-            if (dw == L) unsafeAtomicAdd(&sigma[(uint64_t)w * 64 + b], sv);
-        }
-        const uint64_t cm = __ballot(claimed);
-        if (cm) {
-            bool fresh = false;
-            if (b == __ffsll((unsigned long long)cm) - 1)
-                fresh = atomicMax(&mark[w], (uint32_t)(L + 1)) < (uint32_t)(L + 1);
-            if (__ballot(fresh)) {  // w is the same on every lane
-                if ((uint32_t)b == ap.cnt) ap.buf = w;
-                if (++ap.cnt == 64) vm_append_flush(ap, nq, ctr);
+            const uint64_t cm = __ballot(claimed);
+            if (cm) {
+                bool fresh = false;
+                if (b == __ffsll((unsigned long long)cm) - 1)
+                    fresh = atomicMax(&mark[w], (uint32_t)(L + 1)) < (uint32_t)(L + 1);
+                if (__ballot(fresh)) {  // w is the same on every lane
+                    if ((uint32_t)b == ap.cnt) ap.buf = w;
+                    if (++ap.cnt == 64) vm_append_flush(ap, nq, ctr);
+                }
             }
         }
     }
