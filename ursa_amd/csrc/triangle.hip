@@ -130,28 +130,29 @@ __device__ __forceinline__ uint32_t tc_hash(uint32_t x, uint32_t mask) { return 
 // The LDS hash set: buckets of TC_BUCKET = 8 slots (32 bytes), key x homed
 // in bucket tc_hash(x) and placed in its first free slot, spilling to the next
 // bucket when full (load factor <= 1/2, so a spill is rare). A lookup reads
-// its home bucket with two 16-byte loads and checks all eight slots at once.
+// its home bucket with two 16-byte loads and compares all eight slots at once;
+// the slot of a match is assembled from the eight equality masks (keys of a
+// table are distinct, so at most one matches), and an empty slot (0xFFFFFFFF,
+// the largest u32, above every node id) ends the search, found as a maximum.
 // `bmask` is the bucket count minus one. Returns x's slot, or -1.
 __device__ __forceinline__ int tc_probe(const uint32_t* key, uint32_t bmask, uint32_t x) {
     uint32_t b = tc_hash(x, bmask);
+    int r = -1;
     for (;;) {
         const uint4 k0 = reinterpret_cast<const uint4*>(key)[2 * b];
         const uint4 k1 = reinterpret_cast<const uint4*>(key)[2 * b + 1];
-        int f = -1;
-        f = k1.w == x ? 7 : f;
-        f = k1.z == x ? 6 : f;
-        f = k1.y == x ? 5 : f;
-        f = k1.x == x ? 4 : f;
-        f = k0.w == x ? 3 : f;
-        f = k0.z == x ? 2 : f;
-        f = k0.y == x ? 1 : f;
-        f = k0.x == x ? 0 : f;
-        if (f >= 0) return (int)(b * TC_BUCKET) + f;
-        const uint32_t E = 0xFFFFFFFFu;
-        if (k0.x == E || k0.y == E || k0.z == E || k0.w == E || k1.x == E || k1.y == E || k1.z == E || k1.w == E)
-            return -1;
+        const bool e0 = k0.x == x, e1 = k0.y == x, e2 = k0.z == x, e3 = k0.w == x;
+        const bool e4 = k1.x == x, e5 = k1.y == x, e6 = k1.z == x, e7 = k1.w == x;
+        if (e0 | e1 | e2 | e3 | e4 | e5 | e6 | e7) {
+            const int f = ((e1 | e3 | e5 | e7) ? 1 : 0) | ((e2 | e3 | e6 | e7) ? 2 : 0) | ((e4 | e5 | e6 | e7) ? 4 : 0);
+            r = (int)(b * TC_BUCKET) + f;
+            break;
+        }
+        const uint32_t top = max(max(max(k0.x, k0.y), max(k0.z, k0.w)), max(max(k1.x, k1.y), max(k1.z, k1.w)));
+        if (top == 0xFFFFFFFFu) break;
         b = (b + 1) & bmask;
     }
+    return r;
 }
 
 // Records a triangle (u, v, w) found with w at hash slot wslot (staged) or
