@@ -7,7 +7,9 @@
 // lower rank, rank(x) = (degree(x), x). Each out-list N+(u) stays sorted by id
 // and has at most sqrt(2m) entries. A triangle u < v < w (in rank) is then
 // found exactly once: while processing u, as w in N+(u) ∩ N+(v) for v in N+(u).
-// Counting: one wavefront per work item (four per workgroup); see k_tc_count.
+// Counting: one wavefront per work item, alone in its workgroup (lists of up to
+// STAGE_CAP entries), or one workgroup of four per item (longer lists); see
+// k_tc_count and k_tc_count_big.
 // A work item is a run of consecutive slots of one N+(u): the oriented slots
 // are cut at every list start and wherever the running intersection cost
 // crosses a multiple of TC_CHUNK, so a hub's list is shared by many waves
@@ -26,8 +28,9 @@ namespace ursa {
 
 namespace {
 
-constexpr int TCB = 256;
+constexpr int TCB = 64;  // one wave per workgroup of per-wave items, so waves never wait for each other
 constexpr int TCW = TCB / 64;
+constexpr uint64_t TC_SMALL_GRID = 1u << 22;  // workgroups of k_tc_count (one item each up to this)
 constexpr int TCB_BIG = 256;  // threads of a workgroup sharing one long list
 constexpr uint32_t STAGE_CAP = 256;  // N+(u) entries staged in LDS per wave
 constexpr uint32_t HASH_CAP = 512;   // hash slots per wave (>= 2 * STAGE_CAP)
@@ -333,6 +336,22 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
 // Stages N+(u) into an LDS hash set of 2^k >= du / 4 buckets (at least two
 // slots per entry) with zeroed counters and its bloom bitmap (h / 4 words),
 // using `nthreads` threads. Ends with a barrier; returns the bucket mask.
+// Makes a table's LDS writes visible to its readers: the whole workgroup, or,
+// for a table owned by one wavefront (WAVE), that wavefront alone, whose LDS
+// accesses complete in program order, so only the compiler must be kept from
+// moving accesses across.
+template <bool WAVE>
+__device__ __forceinline__ void tc_sync() {
+    if (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
+template <bool WAVE>
 __device__ __forceinline__ uint32_t tc_stage(const uint32_t* __restrict__ otgt, uint32_t a, uint32_t du,
                                              uint32_t tid, uint32_t nthreads, uint32_t* key, uint32_t* cnt,
                                              uint32_t* bloom) {
@@ -343,7 +362,7 @@ __device__ __forceinline__ uint32_t tc_stage(const uint32_t* __restrict__ otgt, 
     for (uint32_t i = tid; i < h / 2; i += nthreads) cnt[i] = 0;
     if (bloom)
         for (uint32_t i = tid; i < h / 4; i += nthreads) bloom[i] = 0;
-    __syncthreads();
+    tc_sync<WAVE>();
     for (uint32_t i = tid; i < du; i += nthreads) {
         const uint32_t x = otgt[a + i];
         if (bloom) {
@@ -359,23 +378,25 @@ __device__ __forceinline__ uint32_t tc_stage(const uint32_t* __restrict__ otgt, 
             b = (b + 1) & (nb - 1);
         }
     }
-    __syncthreads();
+    tc_sync<WAVE>();
     return nb - 1;
 }
 
 // Adds the slot counters of a staged list into tri. Ends with a barrier.
+template <bool WAVE>
 __device__ __forceinline__ void tc_flush(const uint32_t* key, const uint32_t* cnt, uint32_t h, uint32_t tid,
                                          uint32_t nthreads, unsigned long long* __restrict__ tri) {
-    __syncthreads();
+    tc_sync<WAVE>();
     for (uint32_t i = tid; i < h; i += nthreads) {
         const uint32_t c = (cnt[i >> 1] >> (16 * (i & 1))) & 0xFFFFu;
         if (c) atomicAdd(&tri[key[i]], (unsigned long long)c);
     }
-    __syncthreads();
+    tc_sync<WAVE>();
 }
 
 // One wavefront per item whose list has at most STAGE_CAP entries (staged in
-// the wave's LDS) or more than BIG_CAP (read from HBM).
+// the wave's LDS) or more than BIG_CAP (read from HBM). Every wave owns its
+// table and walks its own items, never waiting for the other waves.
 __global__ __launch_bounds__(TCB, 8) void k_tc_count(const uint32_t* __restrict__ ooff, const uint32_t* __restrict__ otgt,
                                                   const uint32_t* __restrict__ istart, const uint32_t* __restrict__ iend,
                                                   const uint32_t* __restrict__ iu, uint64_t nwork,
@@ -388,23 +409,18 @@ __global__ __launch_bounds__(TCB, 8) void k_tc_count(const uint32_t* __restrict_
     uint32_t* key = s_key[wid];
     uint32_t* cnt = s_cnt[wid];
     uint32_t* bloom = s_bloom[wid];
-    for (uint64_t g = blockIdx.x; g * TCW < nwork; g += gridDim.x) {
-        const uint64_t wi = g * TCW + wid;
-        const bool active = wi < nwork;
-        const uint32_t u = active ? iu[wi] : 0;
-        const uint32_t a = active ? ooff[u] : 0;
-        const uint32_t du = active ? ooff[u + 1] - a : 0;
-        const uint32_t s0 = active ? istart[wi] - a : 0;
-        const uint32_t s1 = active ? iend[wi] - a : 0;
+    const uint64_t nwaves = (uint64_t)gridDim.x * TCW;
+    for (uint64_t wi = blockIdx.x * (uint64_t)TCW + wid; wi < nwork; wi += nwaves) {
+        const uint32_t u = iu[wi];
+        const uint32_t a = ooff[u];
+        const uint32_t du = ooff[u + 1] - a;
+        const uint32_t s0 = istart[wi] - a, s1 = iend[wi] - a;
         const bool staged = du <= STAGE_CAP;
-        // Each wave stages its own list; the barriers inside are block-wide, so
-        // every wave takes part (an empty list stages nothing).
-        const uint32_t sdu = staged ? du : 0;
-        const uint32_t hmask = tc_stage(otgt, a, sdu, lane, 64, key, cnt, bloom);
+        const uint32_t hmask = tc_stage<true>(otgt, a, staged ? du : 0, lane, 64, key, cnt, bloom);
         const unsigned long long cu =
             tc_item(ooff, otgt, a, du, s0, s1, 0, 64, staged, key, cnt, bloom, hmask, tri);
         if (lane == 0 && cu) atomicAdd(&tri[u], cu);
-        tc_flush(key, cnt, (hmask + 1) * TC_BUCKET, lane, 64, tri);
+        tc_flush<true>(key, cnt, (hmask + 1) * TC_BUCKET, lane, 64, tri);
     }
 }
 
@@ -427,11 +443,11 @@ __global__ __launch_bounds__(TCB_BIG) void k_tc_count_big(const uint32_t* __rest
         const uint32_t du = ooff[u + 1] - a;
         const uint32_t s0 = istart[wi] - a, s1 = iend[wi] - a;
         // No bloom bitmap here: with long lists it costs more LDS than it saves.
-        const uint32_t hmask = tc_stage(otgt, a, du, threadIdx.x, TCB_BIG, b_key, b_cnt, nullptr);
+        const uint32_t hmask = tc_stage<false>(otgt, a, du, threadIdx.x, TCB_BIG, b_key, b_cnt, nullptr);
         const unsigned long long cu =
             tc_item(ooff, otgt, a, du, s0, s1, 64u * wid, TCB_BIG, true, b_key, b_cnt, nullptr, hmask, tri);
         if (lane == 0 && cu) atomicAdd(&tri[u], cu);
-        tc_flush(b_key, b_cnt, (hmask + 1) * TC_BUCKET, threadIdx.x, TCB_BIG, tri);
+        tc_flush<false>(b_key, b_cnt, (hmask + 1) * TC_BUCKET, threadIdx.x, TCB_BIG, tri);
     }
 }
 
@@ -543,7 +559,7 @@ void triangle_count(ursa_hip_topo* t, uint64_t* out_user) {
                     t->tc_off.get(), t->tc_tgt.get(), t->tc_bstart.get(), t->tc_bend.get(), t->tc_bu.get(),
                     t->tc_nbig, tri);
             if (t->tc_nwork)
-                k_tc_count<<<(unsigned)std::min<uint64_t>((t->tc_nwork + TCW - 1) / TCW, 1u << 16), TCB, 0, s>>>(
+                k_tc_count<<<(unsigned)std::min<uint64_t>((t->tc_nwork + TCW - 1) / TCW, TC_SMALL_GRID), TCB, 0, s>>>(
                     t->tc_off.get(), t->tc_tgt.get(), t->tc_istart.get(), t->tc_iend.get(), t->tc_iu.get(),
                     t->tc_nwork, tri);
         });
