@@ -132,28 +132,33 @@ __device__ __forceinline__ uint32_t tc_hash(uint32_t x, uint32_t mask) { return 
 
 // The LDS hash set: buckets of TC_BUCKET = 8 slots (32 bytes), key x homed
 // in bucket tc_hash(x) and placed in its first free slot, spilling to the next
-// bucket when full (load factor <= 1/2, so a spill is rare). A lookup reads
-// its home bucket with two 16-byte loads and compares all eight slots at once;
-// the slot of a match is assembled from the eight equality masks (keys of a
-// table are distinct, so at most one matches), and an empty slot (0xFFFFFFFF,
-// the largest u32, above every node id) ends the search, found as a maximum.
-// `bmask` is the bucket count minus one. Returns x's slot, or -1.
+// bucket when full (load factor <= 1/2, so a spill is rare). Slots of a bucket
+// fill in order, so a bucket is full exactly when its last slot is taken, and
+// a key can lie past its home bucket only if that bucket is full. A lookup
+// reads its home bucket with two 16-byte loads and compares all eight slots
+// at once; the slot of a match is assembled from the eight equality masks
+// (keys of a table are distinct, so at most one matches). `bmask` is the
+// bucket count minus one. Returns x's slot, or -1.
+__device__ __forceinline__ int tc_probe_bucket(const uint32_t* key, uint32_t b, uint32_t x, bool& full) {
+    const uint4 k0 = reinterpret_cast<const uint4*>(key)[2 * b];
+    const uint4 k1 = reinterpret_cast<const uint4*>(key)[2 * b + 1];
+    const bool e0 = k0.x == x, e1 = k0.y == x, e2 = k0.z == x, e3 = k0.w == x;
+    const bool e4 = k1.x == x, e5 = k1.y == x, e6 = k1.z == x, e7 = k1.w == x;
+    full = k1.w != 0xFFFFFFFFu;
+    if (!(e0 | e1 | e2 | e3 | e4 | e5 | e6 | e7)) return -1;
+    return (int)(b * TC_BUCKET) + (((e1 | e3 | e5 | e7) ? 1 : 0) | ((e2 | e3 | e6 | e7) ? 2 : 0) | ((e4 | e5 | e6 | e7) ? 4 : 0));
+}
+
 __device__ __forceinline__ int tc_probe(const uint32_t* key, uint32_t bmask, uint32_t x) {
     uint32_t b = tc_hash(x, bmask);
-    int r = -1;
-    for (;;) {
-        const uint4 k0 = reinterpret_cast<const uint4*>(key)[2 * b];
-        const uint4 k1 = reinterpret_cast<const uint4*>(key)[2 * b + 1];
-        const bool e0 = k0.x == x, e1 = k0.y == x, e2 = k0.z == x, e3 = k0.w == x;
-        const bool e4 = k1.x == x, e5 = k1.y == x, e6 = k1.z == x, e7 = k1.w == x;
-        if (e0 | e1 | e2 | e3 | e4 | e5 | e6 | e7) {
-            const int f = ((e1 | e3 | e5 | e7) ? 1 : 0) | ((e2 | e3 | e6 | e7) ? 2 : 0) | ((e4 | e5 | e6 | e7) ? 4 : 0);
-            r = (int)(b * TC_BUCKET) + f;
-            break;
+    bool full;
+    int r = tc_probe_bucket(key, b, x, full);
+    if (r < 0 && full) {  // rare: follow the spill into the next buckets
+        for (;;) {
+            b = (b + 1) & bmask;
+            r = tc_probe_bucket(key, b, x, full);
+            if (r >= 0 || !full) break;
         }
-        const uint32_t top = max(max(max(k0.x, k0.y), max(k0.z, k0.w)), max(max(k1.x, k1.y), max(k1.z, k1.w)));
-        if (top == 0xFFFFFFFFu) break;
-        b = (b + 1) & bmask;
     }
     return r;
 }
@@ -174,22 +179,12 @@ __device__ __forceinline__ void tc_credit_w(bool staged, int wslot, uint32_t w, 
         atomicAdd(&tri[w], 1ull);
 }
 
-// Bloom bit of x: a per-wave staged list also sets one bit per entry in a
-// bitmap of 8 bits per hash slot (64 per bucket), so most absent keys are
-// rejected with one 4-byte LDS read before any bucket is read.
-__device__ __forceinline__ uint32_t tc_bloom_bit(uint32_t x, uint32_t bmask) {
-    const uint32_t lgnb = 31 - __clz(bmask + 1);  // log2 of the bucket count
-    return (x * 0x85EBCA6Bu) >> (26 - lgnb);      // bitmap of 64 * bucket-count bits
-}
-
-// Is w in N+(u)? Sets wslot when staged.
-__device__ __forceinline__ bool tc_member(bool staged, const uint32_t* key, const uint32_t* bloom, uint32_t hmask,
+// Is w in N+(u)? Sets wslot when staged. (A bloom bitmap in front of the
+// table cannot pay on a wave: the table is read whenever any of its 64 lanes
+// passes, which with about one hit in ten steps is nearly always.)
+__device__ __forceinline__ bool tc_member(bool staged, const uint32_t* key, uint32_t hmask,
                                           const uint32_t* ulist, uint32_t du, uint32_t w, int& wslot) {
     if (staged) {
-        if (bloom) {
-            const uint32_t bit = tc_bloom_bit(w, hmask);
-            if (!((bloom[bit >> 5] >> (bit & 31)) & 1u)) return false;
-        }
         wslot = tc_probe(key, hmask, w);
         return wslot >= 0;
     }
@@ -213,7 +208,7 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ otgt, uint32_t a, uint32_t du,
                                                       uint32_t s0, uint32_t s1, uint32_t first, uint32_t step,
                                                       bool staged, const uint32_t* key, uint32_t* cnt,
-                                                      const uint32_t* bloom, uint32_t hmask,
+                                                      uint32_t hmask,
                                                       unsigned long long* __restrict__ tri) {
     const int lane = lane_id();
     const uint32_t* ulist = otgt + a;
@@ -265,7 +260,7 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
                 bool hit = false;
                 if (ok[r]) {
                     if (oa) {
-                        hit = tc_member(staged, key, bloom, hmask, ulist, du, w[r], wslot);
+                        hit = tc_member(staged, key, hmask, ulist, du, w[r], wslot);
                     } else {
                         const uint32_t osv = __builtin_amdgcn_readlane(sv, j);
                         const uint32_t odv = __builtin_amdgcn_readlane(dv, j);
@@ -314,7 +309,7 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
                 if (!ok[r]) {
                     // nothing to test
                 } else if (oa[r]) {
-                    hit = tc_member(staged, key, bloom, hmask, ulist, du, w[r], wslot);
+                    hit = tc_member(staged, key, hmask, ulist, du, w[r], wslot);
                 } else {
                     hit = sorted_contains(otgt + osvv[r], odvv[r], w[r]);
                     if (hit && staged) wslot = tc_probe(key, hmask, w[r]);
@@ -333,9 +328,6 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
     return cu;
 }
 
-// Stages N+(u) into an LDS hash set of 2^k >= du / 4 buckets (at least two
-// slots per entry) with zeroed counters and its bloom bitmap (h / 4 words),
-// using `nthreads` threads. Ends with a barrier; returns the bucket mask.
 // Makes a table's LDS writes visible to its readers: the whole workgroup, or,
 // for a table owned by one wavefront (WAVE), that wavefront alone, whose LDS
 // accesses complete in program order, so only the compiler must be kept from
@@ -351,24 +343,20 @@ __device__ __forceinline__ void tc_sync() {
     }
 }
 
+// Stages N+(u) into an LDS hash set of 2^k >= du / 4 buckets (at least two
+// slots per entry) with zeroed counters, using `nthreads` threads. Ends with a
+// barrier; returns the bucket mask.
 template <bool WAVE>
 __device__ __forceinline__ uint32_t tc_stage(const uint32_t* __restrict__ otgt, uint32_t a, uint32_t du,
-                                             uint32_t tid, uint32_t nthreads, uint32_t* key, uint32_t* cnt,
-                                             uint32_t* bloom) {
+                                             uint32_t tid, uint32_t nthreads, uint32_t* key, uint32_t* cnt) {
     uint32_t nb = 2;
     while (nb * TC_BUCKET < 2 * du) nb <<= 1;
     const uint32_t h = nb * TC_BUCKET;
     for (uint32_t i = tid; i < h; i += nthreads) key[i] = 0xFFFFFFFFu;
     for (uint32_t i = tid; i < h / 2; i += nthreads) cnt[i] = 0;
-    if (bloom)
-        for (uint32_t i = tid; i < h / 4; i += nthreads) bloom[i] = 0;
     tc_sync<WAVE>();
     for (uint32_t i = tid; i < du; i += nthreads) {
         const uint32_t x = otgt[a + i];
-        if (bloom) {
-            const uint32_t bit = tc_bloom_bit(x, nb - 1);
-            atomicOr(&bloom[bit >> 5], 1u << (bit & 31));
-        }
         uint32_t b = tc_hash(x, nb - 1);
         for (;;) {
             bool done = false;
@@ -403,12 +391,10 @@ __global__ __launch_bounds__(TCB, 8) void k_tc_count(const uint32_t* __restrict_
                                                   unsigned long long* __restrict__ tri) {
     __shared__ __align__(16) uint32_t s_key[TCW][HASH_CAP];
     __shared__ uint32_t s_cnt[TCW][HASH_CAP / 2];
-    __shared__ uint32_t s_bloom[TCW][HASH_CAP / 4];
     const int wid = threadIdx.x >> 6;
     const int lane = lane_id();
     uint32_t* key = s_key[wid];
     uint32_t* cnt = s_cnt[wid];
-    uint32_t* bloom = s_bloom[wid];
     const uint64_t nwaves = (uint64_t)gridDim.x * TCW;
     for (uint64_t wi = blockIdx.x * (uint64_t)TCW + wid; wi < nwork; wi += nwaves) {
         const uint32_t u = iu[wi];
@@ -416,9 +402,9 @@ __global__ __launch_bounds__(TCB, 8) void k_tc_count(const uint32_t* __restrict_
         const uint32_t du = ooff[u + 1] - a;
         const uint32_t s0 = istart[wi] - a, s1 = iend[wi] - a;
         const bool staged = du <= STAGE_CAP;
-        const uint32_t hmask = tc_stage<true>(otgt, a, staged ? du : 0, lane, 64, key, cnt, bloom);
+        const uint32_t hmask = tc_stage<true>(otgt, a, staged ? du : 0, lane, 64, key, cnt);
         const unsigned long long cu =
-            tc_item(ooff, otgt, a, du, s0, s1, 0, 64, staged, key, cnt, bloom, hmask, tri);
+            tc_item(ooff, otgt, a, du, s0, s1, 0, 64, staged, key, cnt, hmask, tri);
         if (lane == 0 && cu) atomicAdd(&tri[u], cu);
         tc_flush<true>(key, cnt, (hmask + 1) * TC_BUCKET, lane, 64, tri);
     }
@@ -442,10 +428,9 @@ __global__ __launch_bounds__(TCB_BIG) void k_tc_count_big(const uint32_t* __rest
         const uint32_t a = ooff[u];
         const uint32_t du = ooff[u + 1] - a;
         const uint32_t s0 = istart[wi] - a, s1 = iend[wi] - a;
-        // No bloom bitmap here: with long lists it costs more LDS than it saves.
-        const uint32_t hmask = tc_stage<false>(otgt, a, du, threadIdx.x, TCB_BIG, b_key, b_cnt, nullptr);
+        const uint32_t hmask = tc_stage<false>(otgt, a, du, threadIdx.x, TCB_BIG, b_key, b_cnt);
         const unsigned long long cu =
-            tc_item(ooff, otgt, a, du, s0, s1, 64u * wid, TCB_BIG, true, b_key, b_cnt, nullptr, hmask, tri);
+            tc_item(ooff, otgt, a, du, s0, s1, 64u * wid, TCB_BIG, true, b_key, b_cnt, hmask, tri);
         if (lane == 0 && cu) atomicAdd(&tri[u], cu);
         tc_flush<false>(b_key, b_cnt, (hmask + 1) * TC_BUCKET, threadIdx.x, TCB_BIG, tri);
     }
