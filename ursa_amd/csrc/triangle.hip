@@ -128,7 +128,12 @@ __device__ __forceinline__ bool sorted_contains(const uint32_t* a, uint32_t len,
     return lo < len && a[lo] == x;
 }
 
-__device__ __forceinline__ uint32_t tc_hash(uint32_t x, uint32_t mask) { return (x * 2654435761u >> 7) & mask; }
+// Multiplicative hash, bits 7.. of x * 0x9E3779B1. Tables have fewer than 2^17
+// buckets, so only product bits below 24 are kept, and those depend only on
+// the low 24 bits of x and of the multiplier: a full-rate 24-bit multiply
+// gives the same value.
+__device__ __forceinline__ uint32_t tc_hash(uint32_t x, uint32_t mask) { return (__umul24(x, 0x3779B1u) >> 7) & mask; }
+static_assert(2 * BIG_CAP / TC_BUCKET <= (1u << 17), "tc_hash keeps product bits below 24");
 
 // The LDS hash set: buckets of TC_BUCKET = 8 slots (32 bytes), key x homed
 // in bucket tc_hash(x) and placed in its first free slot, spilling to the next
@@ -236,6 +241,33 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
         const uint32_t che = chi - nch;
         const uint32_t nchunks = __shfl(chi, 63, 64);
         for (uint32_t ch0 = 0; ch0 < nchunks; ch0 += TC_LILP) {
+            // Common case: the group's chunks all walk one N+(v) (mode A) and
+            // N+(u) is staged. One owner lookup serves the group; loads are
+            // clamped into the list instead of masked, and every lane probes.
+            {
+                const uint32_t chl = min(ch0 + TC_LILP, nchunks) - 1;
+                const int j = (int)__popcll(__ballot(che <= ch0)) - 1;
+                const int jl = (int)__popcll(__ballot(che <= chl)) - 1;
+                if (staged && j == jl && __builtin_amdgcn_readlane((int)mode_a, j) != 0) {
+                    const uint32_t oc = __builtin_amdgcn_readlane(c, j);
+                    const uint32_t osv = __builtin_amdgcn_readlane(sv, j);
+                    const uint32_t e0 = (ch0 - __builtin_amdgcn_readlane(che, j)) * 64 + lane;
+                    uint32_t w[TC_LILP];
+#pragma unroll
+                    for (int r = 0; r < TC_LILP; ++r) w[r] = otgt[osv + min(e0 + 64 * r, oc - 1)];
+                    uint32_t h = 0;
+#pragma unroll
+                    for (int r = 0; r < TC_LILP; ++r) {
+                        const int wslot = tc_probe(key, hmask, w[r]);
+                        const bool hit = e0 + 64 * r < oc && wslot >= 0;
+                        if (hit) tc_cnt_add(cnt, wslot, 1u);
+                        h += (uint32_t)__popcll(__ballot(hit));
+                    }
+                    cu += h;
+                    if (lane == 0 && h) tc_cnt_add(cnt, __builtin_amdgcn_readlane(vslot, j), h);
+                    continue;
+                }
+            }
             uint32_t w[TC_LILP];
             bool ok[TC_LILP];
             int own[TC_LILP];
