@@ -140,18 +140,23 @@ static_assert(2 * BIG_CAP / TC_BUCKET <= (1u << 17), "tc_hash keeps product bits
 // bucket when full (load factor <= 1/2, so a spill is rare). Slots of a bucket
 // fill in order, so a bucket is full exactly when its last slot is taken, and
 // a key can lie past its home bucket only if that bucket is full. A lookup
-// reads its home bucket with two 16-byte loads and compares all eight slots
-// at once; the slot of a match is assembled from the eight equality masks
-// (keys of a table are distinct, so at most one matches). `bmask` is the
-// bucket count minus one. Returns x's slot, or -1.
+// reads its home bucket with two 16-byte loads and tests all eight slots at
+// once in vector arithmetic: d_i = key_i ^ x is zero only at a match (keys of
+// a table are distinct, so at most one matches), and the match's slot number
+// is read off the minima of the d_i over the slots with each index bit set.
+// `bmask` is the bucket count minus one. Returns x's slot, or -1.
 __device__ __forceinline__ int tc_probe_bucket(const uint32_t* key, uint32_t b, uint32_t x, bool& full) {
     const uint4 k0 = reinterpret_cast<const uint4*>(key)[2 * b];
     const uint4 k1 = reinterpret_cast<const uint4*>(key)[2 * b + 1];
-    const bool e0 = k0.x == x, e1 = k0.y == x, e2 = k0.z == x, e3 = k0.w == x;
-    const bool e4 = k1.x == x, e5 = k1.y == x, e6 = k1.z == x, e7 = k1.w == x;
     full = k1.w != 0xFFFFFFFFu;
-    if (!(e0 | e1 | e2 | e3 | e4 | e5 | e6 | e7)) return -1;
-    return (int)(b * TC_BUCKET) + (((e1 | e3 | e5 | e7) ? 1 : 0) | ((e2 | e3 | e6 | e7) ? 2 : 0) | ((e4 | e5 | e6 | e7) ? 4 : 0));
+    const uint32_t d0 = k0.x ^ x, d1 = k0.y ^ x, d2 = k0.z ^ x, d3 = k0.w ^ x;
+    const uint32_t d4 = k1.x ^ x, d5 = k1.y ^ x, d6 = k1.z ^ x, d7 = k1.w ^ x;
+    const uint32_t mhi = min(min(d4, d5), min(d6, d7));
+    if (min(min(min(d0, d1), min(d2, d3)), mhi) != 0) return -1;
+    const uint32_t f0 = 1u - min(min(min(d1, d3), min(d5, d7)), 1u);  // match in an odd slot
+    const uint32_t f1 = 1u - min(min(min(d2, d3), min(d6, d7)), 1u);  // slot 2, 3, 6 or 7
+    const uint32_t f2 = 1u - min(mhi, 1u);                            // slot 4..7
+    return (int)(b * TC_BUCKET + f0 + 2 * f1 + 4 * f2);
 }
 
 __device__ __forceinline__ int tc_probe(const uint32_t* key, uint32_t bmask, uint32_t x) {
@@ -232,84 +237,55 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
             if (staged) vslot = tc_probe(key, hmask, v);
         }
         // Long walks: cut into 64-step chunks walked by the whole wave, lane l
-        // taking step l of the chunk; TC_LILP chunks (of one or several lists)
-        // have their loads in flight together. The chunk's list is found with a
-        // ballot: lanes' chunk prefixes are non-decreasing, so the owner of
-        // chunk ch is the last lane whose prefix is <= ch.
+        // taking step l of the chunk. A group of up to TC_LILP chunks of one
+        // list has its loads in flight together; groups never cross lists, so
+        // the list's data is read once per group. The owner of chunk ch is the
+        // last lane whose chunk prefix is <= ch (prefixes are non-decreasing).
+        // Loads are clamped into the list instead of masked, and lanes past
+        // its end test a repeated element and discard the result.
         const uint32_t nch = c >= TC_LONG ? (c + 63) / 64 : 0;
         const uint32_t chi = wave_incl_scan(nch);
         const uint32_t che = chi - nch;
         const uint32_t nchunks = __shfl(chi, 63, 64);
-        for (uint32_t ch0 = 0; ch0 < nchunks; ch0 += TC_LILP) {
-            // Common case: the group's chunks all walk one N+(v) (mode A) and
-            // N+(u) is staged. One owner lookup serves the group; loads are
-            // clamped into the list instead of masked, and every lane probes.
-            {
-                const uint32_t chl = min(ch0 + TC_LILP, nchunks) - 1;
-                const int j = (int)__popcll(__ballot(che <= ch0)) - 1;
-                const int jl = (int)__popcll(__ballot(che <= chl)) - 1;
-                if (staged && j == jl && __builtin_amdgcn_readlane((int)mode_a, j) != 0) {
-                    const uint32_t oc = __builtin_amdgcn_readlane(c, j);
-                    const uint32_t osv = __builtin_amdgcn_readlane(sv, j);
-                    const uint32_t e0 = (ch0 - __builtin_amdgcn_readlane(che, j)) * 64 + lane;
-                    uint32_t w[TC_LILP];
-#pragma unroll
-                    for (int r = 0; r < TC_LILP; ++r) w[r] = otgt[osv + min(e0 + 64 * r, oc - 1)];
-                    uint32_t h = 0;
-#pragma unroll
-                    for (int r = 0; r < TC_LILP; ++r) {
-                        const int wslot = tc_probe(key, hmask, w[r]);
-                        const bool hit = e0 + 64 * r < oc && wslot >= 0;
-                        if (hit) tc_cnt_add(cnt, wslot, 1u);
-                        h += (uint32_t)__popcll(__ballot(hit));
-                    }
-                    cu += h;
-                    if (lane == 0 && h) tc_cnt_add(cnt, __builtin_amdgcn_readlane(vslot, j), h);
-                    continue;
-                }
-            }
+        for (uint32_t ch0 = 0; ch0 < nchunks;) {
+            const int j = (int)__popcll(__ballot(che <= ch0)) - 1;
+            const uint32_t ng = min((uint32_t)TC_LILP, __builtin_amdgcn_readlane(chi, j) - ch0);
+            const uint32_t oc = __builtin_amdgcn_readlane(c, j);
+            const uint32_t osv = __builtin_amdgcn_readlane(sv, j);
+            const bool oa = __builtin_amdgcn_readlane((int)mode_a, j) != 0;
+            const uint32_t e0 = (ch0 - __builtin_amdgcn_readlane(che, j)) * 64 + lane;
+            const uint32_t* walk = oa ? otgt + osv : ulist;  // mode A walks N+(v), mode B N+(u)
             uint32_t w[TC_LILP];
-            bool ok[TC_LILP];
-            int own[TC_LILP];
+#pragma unroll
+            for (int r = 0; r < TC_LILP; ++r)
+                if ((uint32_t)r < ng) w[r] = walk[min(e0 + 64 * r, oc - 1)];
+            uint32_t h = 0;
 #pragma unroll
             for (int r = 0; r < TC_LILP; ++r) {
-                const uint32_t ch = ch0 + r;
-                own[r] = (int)__popcll(__ballot(che <= ch)) - 1;
-                const int j = own[r];
-                const uint32_t e = (ch - __builtin_amdgcn_readlane(che, j)) * 64 + lane;
-                const uint32_t oc = __builtin_amdgcn_readlane(c, j);
-                const uint32_t osv = __builtin_amdgcn_readlane(sv, j);
-                const bool oa = __builtin_amdgcn_readlane((int)mode_a, j) != 0;
-                ok[r] = ch < nchunks && e < oc;
-                w[r] = ok[r] ? (oa ? otgt[osv + e] : ulist[e]) : 0;
-            }
-#pragma unroll
-            for (int r = 0; r < TC_LILP; ++r) {
-                if (ch0 + r >= nchunks) break;
-                const int j = own[r];
-                const bool oa = __builtin_amdgcn_readlane((int)mode_a, j) != 0;
+                if ((uint32_t)r >= ng) break;
+                const bool ok = e0 + 64 * r < oc;
                 int wslot = -1;
-                bool hit = false;
-                if (ok[r]) {
-                    if (oa) {
-                        hit = tc_member(staged, key, hmask, ulist, du, w[r], wslot);
-                    } else {
-                        const uint32_t osv = __builtin_amdgcn_readlane(sv, j);
-                        const uint32_t odv = __builtin_amdgcn_readlane(dv, j);
-                        hit = sorted_contains(otgt + osv, odv, w[r]);
-                        if (hit && staged) wslot = tc_probe(key, hmask, w[r]);
-                    }
-                    if (hit) tc_credit_w(staged, wslot, w[r], cnt, tri);
+                bool hit;
+                if (oa && staged) {
+                    wslot = tc_probe(key, hmask, w[r]);
+                    hit = ok && wslot >= 0;
+                } else if (oa) {
+                    hit = ok && sorted_contains(ulist, du, w[r]);
+                } else {
+                    hit = ok && sorted_contains(otgt + osv, __builtin_amdgcn_readlane(dv, j), w[r]);
+                    if (hit && staged) wslot = tc_probe(key, hmask, w[r]);
                 }
-                const uint32_t h = (uint32_t)__popcll(__ballot(hit));
-                cu += h;
-                if (lane == 0 && h) {
-                    if (staged)
-                        tc_cnt_add(cnt, __builtin_amdgcn_readlane(vslot, j), h);
-                    else
-                        atomicAdd(&tri[__builtin_amdgcn_readlane(v, j)], (unsigned long long)h);
-                }
+                if (hit) tc_credit_w(staged, wslot, w[r], cnt, tri);
+                h += (uint32_t)__popcll(__ballot(hit));
             }
+            cu += h;
+            if (lane == 0 && h) {
+                if (staged)
+                    tc_cnt_add(cnt, __builtin_amdgcn_readlane(vslot, j), h);
+                else
+                    atomicAdd(&tri[__builtin_amdgcn_readlane(v, j)], (unsigned long long)h);
+            }
+            ch0 += ng;
         }
         // Short walks, flattened.
         if (c >= TC_LONG) c = 0;
