@@ -36,10 +36,10 @@ constexpr uint32_t STAGE_CAP = 256;  // N+(u) entries staged in LDS per wave
 constexpr uint32_t HASH_CAP = 512;   // hash slots per wave (>= 2 * STAGE_CAP)
 constexpr uint32_t TC_BUCKET = 8;    // hash slots per bucket
 constexpr uint32_t BIG_CAP = 2048;   // N+(u) entries staged in LDS per workgroup
-constexpr uint64_t TC_CHUNK = 8192;  // intersection steps per work item (about)
+constexpr uint64_t TC_CHUNK = 32768;  // intersection steps per work item (about)
 constexpr int TC_ILP = 2;            // 64-step groups whose loads are in flight together
 constexpr uint32_t TC_LONG = 32;     // walks at least this long are taken by the whole wave
-constexpr int TC_LILP = 4;           // 64-step chunks of long walks in flight together
+constexpr int TC_LILP = 8;           // 64-step chunks of long walks in flight together
 
 // Intersection cost of oriented slot k = (u, v), as k_tc_count spends it:
 // d+(v) hash probes (mode A) or d+(u) binary searches (mode B).
@@ -64,7 +64,7 @@ __global__ void k_tc_item_flags(const uint32_t* __restrict__ ooff, const uint32_
         const uint32_t a = ooff[u], du = ooff[u + 1] - a;
         bool f = false;
         // Lists staged per workgroup or read from HBM are cut 512x coarser (about
-        // 4 M steps): their staging and count flush are paid once per item.
+        // 16 M steps): their staging and count flush are paid once per item.
         const uint64_t win = du > STAGE_CAP ? 512 * TC_CHUNK : TC_CHUNK;
         if (du >= 2) f = (k == a) || (pre[k] / win != pre[k - 1] / win);
         flag[k] = f;
