@@ -64,7 +64,8 @@ struct ursa_hip_topo {
     ursa::DevBuf<uint32_t> tc_off, tc_tgt;
     ursa::DevBuf<uint32_t> tc_istart, tc_iend, tc_iu;  // work items for one wave each
     ursa::DevBuf<uint32_t> tc_bstart, tc_bend, tc_bu;  // work items for one workgroup each
-    uint64_t tc_slots = 0, tc_nwork = 0, tc_nbig = 0;
+    ursa::DevBuf<uint32_t> tc_hstart, tc_hend, tc_hu;  // the same for the longest staged lists
+    uint64_t tc_slots = 0, tc_nwork = 0, tc_nbig = 0, tc_nhuge = 0;
     double tc_pair_work = 0.0;
     bool tc_built = false;
     // Personalized PageRank's split of high in-degree rows into fixed

@@ -35,7 +35,8 @@ constexpr int TCB_BIG = 256;  // threads of a workgroup sharing one long list
 constexpr uint32_t STAGE_CAP = 256;  // N+(u) entries staged in LDS per wave
 constexpr uint32_t HASH_CAP = 512;   // hash slots per wave (>= 2 * STAGE_CAP)
 constexpr uint32_t TC_BUCKET = 8;    // hash slots per bucket
-constexpr uint32_t BIG_CAP = 2048;   // N+(u) entries staged in LDS per workgroup
+constexpr uint32_t MID_CAP = 1024;   // N+(u) entries staged per workgroup in the 12 KB table
+constexpr uint32_t BIG_CAP = 2048;   // N+(u) entries staged in LDS per workgroup (24 KB table)
 constexpr uint64_t TC_CHUNK = 32768;  // intersection steps per work item (about)
 constexpr int TC_ILP = 2;            // 64-step groups whose loads are in flight together
 constexpr uint32_t TC_LONG = 32;     // walks at least this long are taken by the whole wave
@@ -73,19 +74,23 @@ __global__ void k_tc_item_flags(const uint32_t* __restrict__ ooff, const uint32_
 }
 
 // Per item: owner, end slot (the next item's start, cut at the list end) and
-// class (1 = staged per workgroup: STAGE_CAP < |N+(u)| <= BIG_CAP).
+// class flags: staged per wave (|N+(u)| <= STAGE_CAP) or read from HBM
+// (> BIG_CAP), staged per workgroup in a table for up to MID_CAP entries, or
+// in one for up to BIG_CAP. The smaller table leaves room for more
+// workgroups per CU.
 __global__ void k_tc_item_meta(const uint32_t* __restrict__ istart, const uint32_t* __restrict__ oown,
                                const uint32_t* __restrict__ ooff, uint64_t nitems, uint64_t kept,
-                               uint32_t* __restrict__ iu, uint32_t* __restrict__ iend, uint8_t* __restrict__ big,
-                               uint8_t* __restrict__ small) {
+                               uint32_t* __restrict__ iu, uint32_t* __restrict__ iend, uint8_t* __restrict__ mid,
+                               uint8_t* __restrict__ big, uint8_t* __restrict__ small) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nitems; i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t u = oown[istart[i]];
         const uint32_t next = i + 1 < nitems ? istart[i + 1] : (uint32_t)kept;
         const uint32_t du = ooff[u + 1] - ooff[u];
         iu[i] = u;
         iend[i] = min(next, ooff[u + 1]);
-        big[i] = du > STAGE_CAP && du <= BIG_CAP;
-        small[i] = !big[i];
+        mid[i] = du > STAGE_CAP && du <= MID_CAP;
+        big[i] = du > MID_CAP && du <= BIG_CAP;
+        small[i] = du <= STAGE_CAP || du > BIG_CAP;
     }
 }
 
@@ -418,17 +423,18 @@ __global__ __launch_bounds__(TCB, 8) void k_tc_count(const uint32_t* __restrict_
     }
 }
 
-// One workgroup per item whose list has STAGE_CAP < |N+(u)| <= BIG_CAP
-// entries: the list is staged once in the group's LDS and the group's TCB_BIG
-// / 64 waves share the item's slots.
+// One workgroup per item whose list has more than STAGE_CAP and at most CAP
+// entries (MID_CAP or BIG_CAP, the item classes above): the list is staged once
+// in the group's LDS and the group's TCB_BIG / 64 waves share the item's slots.
+template <uint32_t CAP>
 __global__ __launch_bounds__(TCB_BIG) void k_tc_count_big(const uint32_t* __restrict__ ooff,
                                                              const uint32_t* __restrict__ otgt,
                                                              const uint32_t* __restrict__ istart,
                                                              const uint32_t* __restrict__ iend,
                                                              const uint32_t* __restrict__ iu, uint64_t nwork,
                                                              unsigned long long* __restrict__ tri) {
-    __shared__ __align__(16) uint32_t b_key[2 * BIG_CAP];
-    __shared__ uint32_t b_cnt[BIG_CAP];
+    __shared__ __align__(16) uint32_t b_key[2 * CAP];
+    __shared__ uint32_t b_cnt[CAP];
     const int wid = threadIdx.x >> 6;
     const int lane = lane_id();
     for (uint64_t wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
@@ -493,10 +499,11 @@ static void tc_orient(ursa_hip_topo* t, hipStream_t s) {
         uint64_t nitems = 0;
         HIP_OK(hipMemcpyAsync(&nitems, wc.get(), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
-        DevBuf<uint8_t> big(nitems ? nitems : 1), small(nitems ? nitems : 1);
+        DevBuf<uint8_t> mid(nitems ? nitems : 1), big(nitems ? nitems : 1), small(nitems ? nitems : 1);
         if (nitems) {
             k_tc_item_meta<<<blocks_for(nitems, 256), 256, 0, s>>>(istart.get(), oown.get(), t->tc_off.get(), nitems,
-                                                                  kept, iu.get(), iend.get(), big.get(), small.get());
+                                                                  kept, iu.get(), iend.get(), mid.get(), big.get(),
+                                                                  small.get());
             check_launch();
         }
         // Split the items into the per-wave list and the per-workgroup list.
@@ -514,10 +521,14 @@ static void tc_orient(ursa_hip_topo* t, hipStream_t s) {
             return cnt;
         };
         nwork = part(small.get(), t->tc_istart, t->tc_iend, t->tc_iu);
-        t->tc_nbig = part(big.get(), t->tc_bstart, t->tc_bend, t->tc_bu);
+        t->tc_nbig = part(mid.get(), t->tc_bstart, t->tc_bend, t->tc_bu);
+        t->tc_nhuge = part(big.get(), t->tc_hstart, t->tc_hend, t->tc_hu);
     } else {
-        for (auto* b : {&t->tc_istart, &t->tc_iend, &t->tc_iu, &t->tc_bstart, &t->tc_bend, &t->tc_bu}) b->alloc(1);
+        for (auto* b : {&t->tc_istart, &t->tc_iend, &t->tc_iu, &t->tc_bstart, &t->tc_bend, &t->tc_bu, &t->tc_hstart,
+                        &t->tc_hend, &t->tc_hu})
+            b->alloc(1);
         t->tc_nbig = 0;
+        t->tc_nhuge = 0;
     }
     DevBuf<unsigned long long> pairs(1);
     HIP_OK(hipMemsetAsync(pairs.get(), 0, sizeof(unsigned long long), s));
@@ -543,12 +554,16 @@ void triangle_count(ursa_hip_topo* t, uint64_t* out_user) {
     tc_orient(t, s);
     DevOut<uint64_t> out(out_user, n);
     HIP_OK(hipMemsetAsync(out.p, 0, n * sizeof(uint64_t), s));
-    if (t->tc_nwork || t->tc_nbig) {
+    if (t->tc_nwork || t->tc_nbig || t->tc_nhuge) {
         const double bytes = 4.0 * t->tc_pair_work + 12.0 * (double)n;
         auto* tri = reinterpret_cast<unsigned long long*>(out.p);
         timed("triangle_count", bytes, s, [&] {
+            if (t->tc_nhuge)
+                k_tc_count_big<BIG_CAP><<<(unsigned)std::min<uint64_t>(t->tc_nhuge, 1u << 16), TCB_BIG, 0, s>>>(
+                    t->tc_off.get(), t->tc_tgt.get(), t->tc_hstart.get(), t->tc_hend.get(), t->tc_hu.get(),
+                    t->tc_nhuge, tri);
             if (t->tc_nbig)
-                k_tc_count_big<<<(unsigned)std::min<uint64_t>(t->tc_nbig, 1u << 16), TCB_BIG, 0, s>>>(
+                k_tc_count_big<MID_CAP><<<(unsigned)std::min<uint64_t>(t->tc_nbig, 1u << 16), TCB_BIG, 0, s>>>(
                     t->tc_off.get(), t->tc_tgt.get(), t->tc_bstart.get(), t->tc_bend.get(), t->tc_bu.get(),
                     t->tc_nbig, tri);
             if (t->tc_nwork)
