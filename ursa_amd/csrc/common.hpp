@@ -38,11 +38,14 @@ struct Error : std::runtime_error {
 
 void set_last_error(const char* msg);
 
+void sync_kept_blocks();
+
 // Run f, translating exceptions into a status code and a thread-local message.
 template <class F>
 int guard(F&& f) {
     try {
         f();
+        sync_kept_blocks();
         return URSA_HIP_OK;
     } catch (const Error& e) {
         set_last_error(e.what());
@@ -67,6 +70,8 @@ hipStream_t stream();
 
 void* dev_alloc(size_t bytes);
 void dev_free(void* p);
+// Synchronizes the calling thread's stream if a block was kept for reuse since
+// the last call (see dev_free); guard calls it before an entry point returns.
 
 // Workspace arena. While a ScopedArena is active on a thread, every dev_alloc
 // on that thread is carved from the arena's range (256-byte aligned bump

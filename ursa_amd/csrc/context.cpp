@@ -4,6 +4,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -124,6 +125,74 @@ hipStream_t stream() {
     return t_stream.s;
 }
 
+// Device blocks freed outside a workspace arena are kept for reuse instead of
+// going back to HIP: hipMalloc and hipFree cost tens to hundreds of
+// microseconds each, and hipFree waits for the whole device. A block is kept
+// only when it is freed on the stream it was allocated on, and it is handed
+// out again only on that stream. Every kernel of the library runs on the
+// calling thread's stream, so work enqueued there later is ordered after every
+// earlier use of the block. Only the library's own per-thread streams qualify:
+// they are never destroyed, so their handles are never reused, whereas a
+// caller's stream may be. A call that keeps a block synchronizes its stream
+// before returning (sync_kept_blocks), so calls still return with their
+// results written, as they did when hipFree waited. Blocks above kKeepMax are
+// never kept, the cache holds at most kCacheCap bytes, and a failed hipMalloc
+// waits for the device and releases the whole cache before it retries.
+namespace {
+constexpr size_t kKeepMax = 512ull << 20;
+constexpr size_t kCacheCap = 4ull << 30;
+struct LiveBlock {
+    size_t bytes;
+    hipStream_t s;
+};
+std::mutex g_cache_mu;
+std::unordered_map<void*, LiveBlock> g_live;                   // blocks in use that may be kept
+std::multimap<std::pair<hipStream_t, size_t>, void*> g_kept;  // (stream, bytes) -> idle block
+size_t g_kept_bytes = 0;
+thread_local bool t_kept_since_sync = false;
+
+size_t keep_round(size_t b) {
+    return b < (1u << 20) ? (b + 4095) & ~size_t(4095) : (b + 65535) & ~size_t(65535);
+}
+
+void release_kept() {
+    std::vector<void*> blocks;
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        for (auto& e : g_kept) blocks.push_back(e.second);
+        g_kept.clear();
+        g_kept_bytes = 0;
+    }
+    if (!blocks.empty()) {
+        (void)hipDeviceSynchronize();
+        for (void* p : blocks) (void)hipFree(p);
+    }
+}
+
+void* hip_alloc_or_throw(size_t bytes) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        release_kept();
+        e = hipMalloc(&p, bytes);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(URSA_HIP_ENOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed: " +
+                                         hipGetErrorString(e));
+    }
+    return p;
+}
+}  // namespace
+
+void sync_kept_blocks() {
+    if (t_kept_since_sync) {  // blocks are only kept on the thread's own stream
+        t_kept_since_sync = false;
+        if (t_stream.s) HIP_OK(hipStreamSynchronize(t_stream.s));
+    }
+}
+
 void* dev_alloc(size_t bytes) {
     ensure_device();
     if (Arena* a = t_arena) {
@@ -137,12 +206,24 @@ void* dev_alloc(size_t bytes) {
         if (a->used > a->peak) a->peak = a->used;
         return a->base + at;
     }
-    void* p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        throw Error(URSA_HIP_ENOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed: " +
-                                         hipGetErrorString(e));
+    const size_t r = keep_round(bytes ? bytes : 1);
+    hipStream_t s = (t_user_stream || r > kKeepMax) ? nullptr : stream();
+    if (s) {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto it = g_kept.lower_bound({s, r});
+        if (it != g_kept.end() && it->first.first == s && it->first.second <= r + r / 8) {
+            void* p = it->second;
+            const size_t got = it->first.second;
+            g_kept.erase(it);
+            g_kept_bytes -= got;
+            g_live[p] = LiveBlock{got, s};
+            return p;
+        }
+    }
+    void* p = hip_alloc_or_throw(s ? r : bytes);
+    if (s) {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        g_live[p] = LiveBlock{r, s};
     }
     return p;
 }
@@ -163,6 +244,21 @@ void dev_free(void* p) {
         }
     }
     if (in_arena(p)) return;
+    {
+        hipStream_t cur = t_user_stream ? nullptr : t_stream.s;
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto it = g_live.find(p);
+        if (it != g_live.end()) {
+            const LiveBlock b = it->second;
+            g_live.erase(it);
+            if (cur && b.s == cur && g_kept_bytes + b.bytes <= kCacheCap) {
+                g_kept.emplace(std::make_pair(b.s, b.bytes), p);
+                g_kept_bytes += b.bytes;
+                t_kept_since_sync = true;
+                return;
+            }
+        }
+    }
     (void)hipFree(p);
 }
 
@@ -245,7 +341,10 @@ int ursa_hip_init(int device) {
 
 int ursa_hip_shutdown(void) {
     return guard([&] {
-        if (g_device.load() >= 0) HIP_OK(hipDeviceSynchronize());
+        if (g_device.load() >= 0) {
+            HIP_OK(hipDeviceSynchronize());
+            release_kept();
+        }
     });
 }
 
@@ -295,7 +394,9 @@ int ursa_hip_profile_query(const char* kernel, double* total_ms, uint64_t* launc
 int ursa_hip_device_alloc(uint64_t bytes, void** out) {
     return guard([&] {
         URSA_REQUIRE(out != nullptr, URSA_HIP_EINVAL, "out is NULL");
-        *out = dev_alloc(bytes);
+        ensure_device();
+        // Caller-owned memory may be used on the caller's streams: never kept.
+        *out = hip_alloc_or_throw(bytes ? bytes : 1);
     });
 }
 
