@@ -251,6 +251,41 @@ def test_triangles_bipartite_with_chords(ua, side):
     assert np.array_equal(ua.triangle_count(t), want)
 
 
+def _bucket0_ids(k, nbuckets):
+    """k node ids >= 1 whose home bucket in an LDS table of `nbuckets` buckets is
+    0 (bits 7.. of id * 0x9E3779B1, the kernel's hash): staged together they
+    overflow one bucket. Ids colliding for nbuckets also collide for every
+    smaller power of two."""
+    ids, x = [], 1
+    while len(ids) < k:
+        if ((((x * 0x9E3779B1) & 0xFFFFFFFF) >> 7) & (nbuckets - 1)) == 0:
+            ids.append(x)
+        x += 1
+    return np.array(ids, np.int64)
+
+
+@pytest.mark.parametrize("k,nb", [(14, 4), (40, 16), (300, 128)])
+def test_triangles_colliding_keys(ua, k, nb):
+    # Node 0 joined to k nodes that form a clique, all with equal degree k, so
+    # N+(0) holds the k ids and N+(w_i) the later ones. With every id homed in
+    # bucket 0, the bucket fills and the remaining keys spill into the
+    # following buckets: 6 of them at k = 14, 32 at k = 40 (per-wave tables),
+    # 292 at k = 300 (a per-workgroup table). Closed form: node 0 closes
+    # C(k, 2) triangles, each w C(k-1, 2) + (k-1).
+    w = _bucket0_ids(k, nb)
+    a, b = np.triu_indices(k, 1)
+    src = np.concatenate([np.zeros(k, np.int64), w[a]])
+    dst = np.concatenate([w, w[b]])
+    n = int(w.max()) + 1
+    t = ua.Topology.build(n, src, dst)
+    want = np.zeros(n, np.uint64)
+    want[0] = k * (k - 1) // 2
+    want[w] = (k - 1) * (k - 2) // 2 + (k - 1)
+    got = ua.triangle_count(t)
+    assert np.array_equal(got, want)
+    assert np.array_equal(got, orc.triangle_count(orc.Topology(n, src, dst)))
+
+
 # ---------------------------------------------------------------- betweenness
 def assert_bc_close(got, ref, name):
     err = np.abs(got - ref)
