@@ -60,12 +60,17 @@ struct ursa_hip_topo {
     ursa::DevBuf<uint32_t> pr_tile_x, pr_tile_y;
     uint64_t pr_tiles = 0;
     bool pr_ready = false;
-    // Degree-oriented undirected DAG for triangle counting (see triangle.hip).
-    ursa::DevBuf<uint32_t> tc_off, tc_tgt;
-    ursa::DevBuf<uint32_t> tc_istart, tc_iend, tc_iu;  // work items for one wave each
-    ursa::DevBuf<uint32_t> tc_bstart, tc_bend, tc_bu;  // work items for one workgroup each
-    ursa::DevBuf<uint32_t> tc_hstart, tc_hend, tc_hu;  // the same for the longest staged lists
-    uint64_t tc_slots = 0, tc_nwork = 0, tc_nbig = 0, tc_nhuge = 0;
+    // Degree-oriented undirected DAG for triangle counting (see triangle.hip):
+    // out-lists N+(u), in-lists In+(v), and work items over either list kind
+    // ([0] out, [1] in) in three classes ([0] per wave, [1] per workgroup with
+    // the mid-size table, [2] with the large table).
+    ursa::DevBuf<uint32_t> tc_off, tc_tgt, tc_ioff, tc_isrc;
+    struct TcItems {
+        ursa::DevBuf<uint32_t> start, end, owner;
+        uint64_t n = 0;
+    };
+    TcItems tc_items[2][3];
+    uint64_t tc_slots = 0;
     double tc_pair_work = 0.0;
     bool tc_built = false;
     // Personalized PageRank's split of high in-degree rows into fixed

@@ -6,16 +6,20 @@
 // Orientation: every undirected edge {u, v} is kept once, from the endpoint of
 // lower rank, rank(x) = (degree(x), x). Each out-list N+(u) stays sorted by id
 // and has at most sqrt(2m) entries. A triangle u < v < w (in rank) is then
-// found exactly once: while processing u, as w in N+(u) ∩ N+(v) for v in N+(u).
-// Counting: one wavefront per work item, alone in its workgroup (lists of up to
-// STAGE_CAP entries), or one workgroup of four per item (longer lists); see
+// found exactly once, as w in N+(u) ∩ N+(v) for the pair u -> v, intersected
+// from one side: N+(v) walked against a hash table of N+(u), or N+(u) walked
+// against a table of N+(v) when N+(v) is much longer (tc_walk); the second
+// kind of work runs over the in-lists In+(v) of the oriented graph.
+// Counting: one wavefront per work item, alone in its workgroup (tables of up
+// to STAGE_CAP entries), or one workgroup of four per item (longer lists); see
 // k_tc_count and k_tc_count_big.
-// A work item is a run of consecutive slots of one N+(u): the oriented slots
-// are cut at every list start and wherever the running intersection cost
-// crosses a multiple of TC_CHUNK, so a hub's list is shared by many waves
-// instead of serialising the tail of the launch on one. The wave flattens its
-// (v, candidate) pairs over 64 lanes with a degree prefix-sum. Integer sums are order-independent, so counts are exact and
-// identical to the node-iterator reference.
+// A work item is a run of consecutive slots of one list (an out-list, or an
+// in-list for the second kind): the slots are cut at every list start and
+// wherever the running intersection cost crosses a multiple of TC_CHUNK, so a
+// hub's list is shared by many waves instead of serialising the tail of the
+// launch on one. The wave flattens its (partner, candidate) pairs over 64
+// lanes with a degree prefix-sum. Integer sums are order-independent, so
+// counts are exact and identical to the node-iterator reference.
 //
 // Algorithmic bytes (the sorted-merge model): 4 * sum over oriented edges (u,v)
 // of (d+(u) + d+(v)), plus 12 B per node.
@@ -42,52 +46,65 @@ constexpr int TC_ILP = 2;            // 64-step groups whose loads are in flight
 constexpr uint32_t TC_LONG = 32;     // walks at least this long are taken by the whole wave
 constexpr int TC_LILP = 8;           // 64-step chunks of long walks in flight together
 
-// Intersection cost of oriented slot k = (u, v), as k_tc_count spends it:
-// d+(v) hash probes (mode A) or d+(u) binary searches (mode B).
-__device__ __forceinline__ uint32_t tc_slot_cost(uint32_t du, uint32_t dv) { return dv <= 8 * du ? dv : du; }
+// Each oriented edge x -> y (a pair) is intersected once: N+(y) is walked
+// against x's table when d+(y) <= TC_SIDE_RATIO * d+(x), by an item over x's
+// out-list (kind 0); otherwise N+(x) is walked against y's table, by an item
+// over y's in-list In+(y) (kind 1). A triangle x < y < z closes at w = z in
+// N+(x) and N+(y). The ratio is 2 rather than 1 (always the shorter side):
+// long walks over one list are cheaper per step than short ones, and at s22
+// 2 measured 32.4 ms against 33.9 (1), 35.2 (4) and 37.0 (8). Walk length of a
+// slot, with `owner` the table's vertex and `partner` the other end; a pair
+// needs |N+(x)| >= 2.
+constexpr uint32_t TC_SIDE_RATIO = 2;
+__device__ __forceinline__ uint32_t tc_walk(bool in, uint32_t d_owner, uint32_t d_partner) {
+    if (in) return d_partner >= 2 && TC_SIDE_RATIO * d_partner < d_owner ? d_partner : 0u;
+    return d_owner >= 2 && d_partner <= TC_SIDE_RATIO * d_owner ? d_partner : 0u;
+}
 
-__global__ void k_tc_cost(const uint32_t* __restrict__ ooff, const uint32_t* __restrict__ otgt,
-                          const uint32_t* __restrict__ oown, uint64_t kept, uint32_t* __restrict__ cost) {
+__global__ void k_tc_cost(const uint32_t* __restrict__ ooff, const uint32_t* __restrict__ partner,
+                          const uint32_t* __restrict__ owner, uint64_t kept, bool in, uint32_t* __restrict__ cost) {
     for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < kept; k += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t u = oown[k], v = otgt[k];
-        const uint32_t du = ooff[u + 1] - ooff[u], dv = ooff[v + 1] - ooff[v];
-        cost[k] = du >= 2 ? tc_slot_cost(du, dv) : 0u;
+        const uint32_t o = owner[k], p = partner[k];
+        cost[k] = tc_walk(in, ooff[o + 1] - ooff[o], ooff[p + 1] - ooff[p]);
     }
 }
 
-// Item starts: the first slot of every list with at least two entries, and
-// every slot whose exclusive cost prefix enters a new TC_CHUNK window.
-__global__ void k_tc_item_flags(const uint32_t* __restrict__ ooff, const uint32_t* __restrict__ oown,
+// Item starts: the first slot of every list with any work, and every slot
+// whose exclusive cost prefix enters a new TC_CHUNK window. `loff` holds the
+// list offsets (out- or in-lists), `ooff` the out-list offsets that size the
+// owner's table.
+__global__ void k_tc_item_flags(const uint32_t* __restrict__ loff, const uint32_t* __restrict__ ooff,
+                                const uint32_t* __restrict__ owner, const uint32_t* __restrict__ cost,
                                 const uint64_t* __restrict__ pre, uint64_t kept, uint8_t* __restrict__ flag,
                                 uint32_t* __restrict__ iota) {
     for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < kept; k += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t u = oown[k];
-        const uint32_t a = ooff[u], du = ooff[u + 1] - a;
-        bool f = false;
+        const uint32_t o = owner[k];
+        const uint32_t a = loff[o], e = loff[o + 1], du = ooff[o + 1] - ooff[o];
         // Lists staged per workgroup or read from HBM are cut 512x coarser (about
         // 16 M steps): their staging and count flush are paid once per item.
         const uint64_t win = du > STAGE_CAP ? 512 * TC_CHUNK : TC_CHUNK;
-        if (du >= 2) f = (k == a) || (pre[k] / win != pre[k - 1] / win);
-        flag[k] = f;
+        const bool work = pre[e - 1] + cost[e - 1] > pre[a];
+        flag[k] = work && ((k == a) || (pre[k] / win != pre[k - 1] / win));
         iota[k] = (uint32_t)k;
     }
 }
 
 // Per item: owner, end slot (the next item's start, cut at the list end) and
-// class flags: staged per wave (|N+(u)| <= STAGE_CAP) or read from HBM
+// class flags: staged per wave (|N+(owner)| <= STAGE_CAP) or read from HBM
 // (> BIG_CAP), staged per workgroup in a table for up to MID_CAP entries, or
 // in one for up to BIG_CAP. The smaller table leaves room for more
 // workgroups per CU.
-__global__ void k_tc_item_meta(const uint32_t* __restrict__ istart, const uint32_t* __restrict__ oown,
-                               const uint32_t* __restrict__ ooff, uint64_t nitems, uint64_t kept,
-                               uint32_t* __restrict__ iu, uint32_t* __restrict__ iend, uint8_t* __restrict__ mid,
-                               uint8_t* __restrict__ big, uint8_t* __restrict__ small) {
+__global__ void k_tc_item_meta(const uint32_t* __restrict__ istart, const uint32_t* __restrict__ owner,
+                               const uint32_t* __restrict__ loff, const uint32_t* __restrict__ ooff,
+                               uint64_t nitems, uint64_t kept, uint32_t* __restrict__ iu,
+                               uint32_t* __restrict__ iend, uint8_t* __restrict__ mid, uint8_t* __restrict__ big,
+                               uint8_t* __restrict__ small) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nitems; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t u = oown[istart[i]];
+        const uint32_t o = owner[istart[i]];
         const uint32_t next = i + 1 < nitems ? istart[i + 1] : (uint32_t)kept;
-        const uint32_t du = ooff[u + 1] - ooff[u];
-        iu[i] = u;
-        iend[i] = min(next, ooff[u + 1]);
+        const uint32_t du = ooff[o + 1] - ooff[o];
+        iu[i] = o;
+        iend[i] = min(next, loff[o + 1]);
         mid[i] = du > STAGE_CAP && du <= MID_CAP;
         big[i] = du > MID_CAP && du <= BIG_CAP;
         small[i] = du <= STAGE_CAP || du > BIG_CAP;
@@ -206,21 +223,24 @@ __device__ __forceinline__ bool tc_member(bool staged, const uint32_t* key, uint
     return sorted_contains(ulist, du, w);
 }
 
-// The intersections of one work item, slots [s0, s1) of N+(u), by one wave
-// that takes every `step`-th group of 64 slots starting at s0 + first. With
-// `staged`, N+(u) sits in an LDS open-addressing hash set whose slots also
-// carry hit counters. For each v in the range the wave intersects N+(u) and
-// N+(v) from the cheaper side:
-//   A  (|N+(v)| <= 8 |N+(u)|): walk N+(v) from HBM, probe N+(u);
-//   B  otherwise:              walk N+(u), binary-search N+(v) in HBM.
-// A v whose walk has at least TC_LONG steps is walked by the whole wave in
-// 64-step chunks; the shorter walks of the group are flattened over the 64
-// lanes with a prefix sum and an owner search.
-// A hit w closes the triangle (u, v, w): the returned count (wave-uniform) is
-// u's share, and v and w (both in N+(u)) are credited in the slot counters (or
-// in HBM when N+(u) is not staged). Integer sums are order-independent.
+// The intersections of one work item, slots [s0, s1) of a list of the
+// owner u (its out-list N+(u), or with IN its in-list In+(u); `plist` is the
+// concatenated lists of that kind), by one wave that takes every `step`-th
+// group of 64 slots starting at s0 + first. With `staged`, N+(u) sits in an
+// LDS open-addressing hash set whose slots also carry hit counters; otherwise
+// membership is a binary search of N+(u) in HBM. For each partner v of the
+// range whose pair falls to this side (tc_walk), the wave walks N+(v) from HBM
+// and tests each entry against N+(u). A v whose walk has at least TC_LONG steps
+// is walked by the whole wave in 64-step chunks; the shorter walks of the group
+// are flattened over the 64 lanes with a prefix sum and an owner search.
+// A hit w closes a triangle of u, v and w: the returned count (wave-uniform) is
+// u's share, w (in N+(u)) is credited in the slot counters (or in HBM when N+(u)
+// is not staged), and so is v when v is in N+(u) (out-lists); an in-list
+// partner is credited in HBM. Integer sums are order-independent.
+template <bool IN>
 __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict__ ooff,
-                                                      const uint32_t* __restrict__ otgt, uint32_t a, uint32_t du,
+                                                      const uint32_t* __restrict__ otgt,
+                                                      const uint32_t* __restrict__ plist, uint32_t a, uint32_t du,
                                                       uint32_t s0, uint32_t s1, uint32_t first, uint32_t step,
                                                       bool staged, const uint32_t* key, uint32_t* cnt,
                                                       uint32_t hmask,
@@ -228,18 +248,17 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
     const int lane = lane_id();
     const uint32_t* ulist = otgt + a;
     unsigned long long cu = 0;
+    // Partner credits: the slot counter of v in N+(u), or v's count in HBM.
+    const bool vtable = staged && !IN;
     for (uint32_t i0 = s0 + first; i0 < s1; i0 += step) {
         const uint32_t i = i0 + lane;
-        uint32_t v = 0, dv = 0, sv = 0, c = 0;
+        uint32_t v = 0, sv = 0, c = 0;
         int vslot = -1;
-        bool mode_a = true;
         if (i < s1) {
-            v = ulist[i];
+            v = plist[i];
             sv = ooff[v];
-            dv = ooff[v + 1] - sv;
-            mode_a = dv <= 8 * du;
-            c = mode_a ? dv : du;
-            if (staged) vslot = tc_probe(key, hmask, v);
+            c = tc_walk(IN, du, ooff[v + 1] - sv);
+            if (vtable && c) vslot = tc_probe(key, hmask, v);
         }
         // Long walks: cut into 64-step chunks walked by the whole wave, lane l
         // taking step l of the chunk. A group of up to TC_LILP chunks of one
@@ -256,10 +275,8 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
             const int j = (int)__popcll(__ballot(che <= ch0)) - 1;
             const uint32_t ng = min((uint32_t)TC_LILP, __builtin_amdgcn_readlane(chi, j) - ch0);
             const uint32_t oc = __builtin_amdgcn_readlane(c, j);
-            const uint32_t osv = __builtin_amdgcn_readlane(sv, j);
-            const bool oa = __builtin_amdgcn_readlane((int)mode_a, j) != 0;
+            const uint32_t* walk = otgt + __builtin_amdgcn_readlane(sv, j);
             const uint32_t e0 = (ch0 - __builtin_amdgcn_readlane(che, j)) * 64 + lane;
-            const uint32_t* walk = oa ? otgt + osv : ulist;  // mode A walks N+(v), mode B N+(u)
             uint32_t w[TC_LILP];
 #pragma unroll
             for (int r = 0; r < TC_LILP; ++r)
@@ -271,21 +288,18 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
                 const bool ok = e0 + 64 * r < oc;
                 int wslot = -1;
                 bool hit;
-                if (oa && staged) {
+                if (staged) {
                     wslot = tc_probe(key, hmask, w[r]);
                     hit = ok && wslot >= 0;
-                } else if (oa) {
-                    hit = ok && sorted_contains(ulist, du, w[r]);
                 } else {
-                    hit = ok && sorted_contains(otgt + osv, __builtin_amdgcn_readlane(dv, j), w[r]);
-                    if (hit && staged) wslot = tc_probe(key, hmask, w[r]);
+                    hit = ok && sorted_contains(ulist, du, w[r]);
                 }
                 if (hit) tc_credit_w(staged, wslot, w[r], cnt, tri);
                 h += (uint32_t)__popcll(__ballot(hit));
             }
             cu += h;
             if (lane == 0 && h) {
-                if (staged)
+                if (vtable)
                     tc_cnt_add(cnt, __builtin_amdgcn_readlane(vslot, j), h);
                 else
                     atomicAdd(&tri[__builtin_amdgcn_readlane(v, j)], (unsigned long long)h);
@@ -298,38 +312,29 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
         const uint32_t excl = incl - c;
         const uint32_t total = __shfl(incl, 63, 64);
         for (uint32_t t0 = 0; t0 < total; t0 += 64 * TC_ILP) {
-            uint32_t w[TC_ILP], ovv[TC_ILP], osvv[TC_ILP], odvv[TC_ILP];
+            uint32_t w[TC_ILP], ovv[TC_ILP];
             int ovs[TC_ILP];
-            bool oa[TC_ILP], ok[TC_ILP];
+            bool ok[TC_ILP];
 #pragma unroll
             for (int r = 0; r < TC_ILP; ++r) {
                 const uint32_t t = t0 + 64 * r + lane;
                 const int j = wave_owner(excl, t);
                 const uint32_t e = t - __shfl(excl, j, 64);
                 ovv[r] = __shfl(v, j, 64);
-                osvv[r] = __shfl(sv, j, 64);
-                odvv[r] = __shfl(dv, j, 64);
                 ovs[r] = __shfl(vslot, j, 64);
-                oa[r] = __shfl((int)mode_a, j, 64) != 0;
+                const uint32_t osv = __shfl(sv, j, 64);
                 ok[r] = t < total;
                 w[r] = 0;
-                if (ok[r]) w[r] = oa[r] ? otgt[osvv[r] + e] : ulist[e];
+                if (ok[r]) w[r] = otgt[osv + e];
             }
 #pragma unroll
             for (int r = 0; r < TC_ILP; ++r) {
                 bool hit = false;
                 int wslot = -1;
-                if (!ok[r]) {
-                    // nothing to test
-                } else if (oa[r]) {
-                    hit = tc_member(staged, key, hmask, ulist, du, w[r], wslot);
-                } else {
-                    hit = sorted_contains(otgt + osvv[r], odvv[r], w[r]);
-                    if (hit && staged) wslot = tc_probe(key, hmask, w[r]);
-                }
+                if (ok[r]) hit = tc_member(staged, key, hmask, ulist, du, w[r], wslot);
                 if (hit) {
                     tc_credit_w(staged, wslot, w[r], cnt, tri);
-                    if (staged)
+                    if (vtable)
                         tc_cnt_add(cnt, ovs[r], 1u);
                     else
                         atomicAdd(&tri[ovv[r]], 1ull);
@@ -395,10 +400,13 @@ __device__ __forceinline__ void tc_flush(const uint32_t* key, const uint32_t* cn
     tc_sync<WAVE>();
 }
 
-// One wavefront per item whose list has at most STAGE_CAP entries (staged in
-// the wave's LDS) or more than BIG_CAP (read from HBM). Every wave owns its
-// table and walks its own items, never waiting for the other waves.
+// One wavefront per item whose owner's out-list has at most STAGE_CAP entries
+// (staged in the wave's LDS) or more than BIG_CAP (read from HBM). Every wave
+// owns its table and walks its own items, never waiting for the other waves.
+// `plist`: the out-lists (otgt) or, with IN, the in-lists.
+template <bool IN>
 __global__ __launch_bounds__(TCB, 8) void k_tc_count(const uint32_t* __restrict__ ooff, const uint32_t* __restrict__ otgt,
+                                                  const uint32_t* __restrict__ plist,
                                                   const uint32_t* __restrict__ istart, const uint32_t* __restrict__ iend,
                                                   const uint32_t* __restrict__ iu, uint64_t nwork,
                                                   unsigned long long* __restrict__ tri) {
@@ -413,22 +421,23 @@ __global__ __launch_bounds__(TCB, 8) void k_tc_count(const uint32_t* __restrict_
         const uint32_t u = iu[wi];
         const uint32_t a = ooff[u];
         const uint32_t du = ooff[u + 1] - a;
-        const uint32_t s0 = istart[wi] - a, s1 = iend[wi] - a;
         const bool staged = du <= STAGE_CAP;
         const uint32_t hmask = tc_stage<true>(otgt, a, staged ? du : 0, lane, 64, key, cnt);
         const unsigned long long cu =
-            tc_item(ooff, otgt, a, du, s0, s1, 0, 64, staged, key, cnt, hmask, tri);
+            tc_item<IN>(ooff, otgt, plist, a, du, istart[wi], iend[wi], 0, 64, staged, key, cnt, hmask, tri);
         if (lane == 0 && cu) atomicAdd(&tri[u], cu);
         tc_flush<true>(key, cnt, (hmask + 1) * TC_BUCKET, lane, 64, tri);
     }
 }
 
-// One workgroup per item whose list has more than STAGE_CAP and at most CAP
-// entries (MID_CAP or BIG_CAP, the item classes above): the list is staged once
-// in the group's LDS and the group's TCB_BIG / 64 waves share the item's slots.
-template <uint32_t CAP>
+// One workgroup per item whose owner's out-list has more than STAGE_CAP and at
+// most CAP entries (MID_CAP or BIG_CAP, the item classes above): the list is
+// staged once in the group's LDS and the group's TCB_BIG / 64 waves share the
+// item's slots.
+template <uint32_t CAP, bool IN>
 __global__ __launch_bounds__(TCB_BIG) void k_tc_count_big(const uint32_t* __restrict__ ooff,
                                                              const uint32_t* __restrict__ otgt,
+                                                             const uint32_t* __restrict__ plist,
                                                              const uint32_t* __restrict__ istart,
                                                              const uint32_t* __restrict__ iend,
                                                              const uint32_t* __restrict__ iu, uint64_t nwork,
@@ -441,10 +450,9 @@ __global__ __launch_bounds__(TCB_BIG) void k_tc_count_big(const uint32_t* __rest
         const uint32_t u = iu[wi];
         const uint32_t a = ooff[u];
         const uint32_t du = ooff[u + 1] - a;
-        const uint32_t s0 = istart[wi] - a, s1 = iend[wi] - a;
         const uint32_t hmask = tc_stage<false>(otgt, a, du, threadIdx.x, TCB_BIG, b_key, b_cnt);
-        const unsigned long long cu =
-            tc_item(ooff, otgt, a, du, s0, s1, 64u * wid, TCB_BIG, true, b_key, b_cnt, hmask, tri);
+        const unsigned long long cu = tc_item<IN>(ooff, otgt, plist, a, du, istart[wi], iend[wi], 64u * wid,
+                                                  TCB_BIG, true, b_key, b_cnt, hmask, tri);
         if (lane == 0 && cu) atomicAdd(&tri[u], cu);
         tc_flush<false>(b_key, b_cnt, (hmask + 1) * TC_BUCKET, threadIdx.x, TCB_BIG, tri);
     }
@@ -480,18 +488,43 @@ static void tc_orient(ursa_hip_topo* t, hipStream_t s) {
         t->tc_tgt.alloc(1);
         offsets_from_sorted(nullptr, 0, n, t->tc_off.get(), s);
     }
-    // Work items (see the file comment).
-    uint64_t nwork = 0;
+    // In-lists of the DAG: the kept (owner, target) pairs regrouped by target
+    // with a stable sort, so each In+(v) lists its sources in increasing order.
+    DevBuf<uint32_t> iown;  // target of each in-list slot
+    t->tc_ioff.alloc(n + 1);
     if (kept) {
+        iown.alloc(kept);
+        t->tc_isrc.alloc(kept);
+        int bits = 1;
+        while (bits < 32 && (1ull << bits) < n) ++bits;
+        prim::sort_pairs_u32(t->tc_tgt.get(), iown.get(), oown.get(), t->tc_isrc.get(), kept, bits, s);
+        offsets_from_sorted(iown.get(), kept, n, t->tc_ioff.get(), s);
+    } else {
+        t->tc_isrc.alloc(1);
+        offsets_from_sorted(nullptr, 0, n, t->tc_ioff.get(), s);
+    }
+    // Work items (see the file comment), for out-lists ([0]) and in-lists ([1]).
+    for (int kind = 0; kind < 2; ++kind) {
+        auto& items = t->tc_items[kind];
+        const uint32_t* loff = kind ? t->tc_ioff.get() : t->tc_off.get();
+        const uint32_t* partner = kind ? t->tc_isrc.get() : t->tc_tgt.get();
+        const uint32_t* owner = kind ? iown.get() : oown.get();
+        for (auto& it : items) {
+            it.start.alloc(1);
+            it.end.alloc(1);
+            it.owner.alloc(1);
+            it.n = 0;
+        }
+        if (!kept) continue;
         DevBuf<uint32_t> cost(kept), iota(kept);
         DevBuf<uint64_t> pre(kept + 1);
         DevBuf<uint8_t> flag(kept);
-        k_tc_cost<<<blocks_for(kept, 256), 256, 0, s>>>(t->tc_off.get(), t->tc_tgt.get(), oown.get(), kept,
+        k_tc_cost<<<blocks_for(kept, 256), 256, 0, s>>>(t->tc_off.get(), partner, owner, kept, kind == 1,
                                                        cost.get());
         check_launch();
         prim::exclusive_sum_u32_to_u64(cost.get(), pre.get(), kept, s);
-        k_tc_item_flags<<<blocks_for(kept, 256), 256, 0, s>>>(t->tc_off.get(), oown.get(), pre.get(), kept,
-                                                             flag.get(), iota.get());
+        k_tc_item_flags<<<blocks_for(kept, 256), 256, 0, s>>>(loff, t->tc_off.get(), owner, cost.get(), pre.get(),
+                                                             kept, flag.get(), iota.get());
         check_launch();
         DevBuf<uint32_t> istart(kept), iu(kept), iend(kept);
         DevBuf<uint64_t> wc(1);
@@ -499,36 +532,24 @@ static void tc_orient(ursa_hip_topo* t, hipStream_t s) {
         uint64_t nitems = 0;
         HIP_OK(hipMemcpyAsync(&nitems, wc.get(), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
-        DevBuf<uint8_t> mid(nitems ? nitems : 1), big(nitems ? nitems : 1), small(nitems ? nitems : 1);
-        if (nitems) {
-            k_tc_item_meta<<<blocks_for(nitems, 256), 256, 0, s>>>(istart.get(), oown.get(), t->tc_off.get(), nitems,
-                                                                  kept, iu.get(), iend.get(), mid.get(), big.get(),
-                                                                  small.get());
-            check_launch();
-        }
-        // Split the items into the per-wave list and the per-workgroup list.
-        auto part = [&](const uint8_t* f, DevBuf<uint32_t>& a, DevBuf<uint32_t>& b, DevBuf<uint32_t>& c) -> uint64_t {
-            uint64_t cnt = 0;
-            a.alloc(nitems ? nitems : 1);
-            b.alloc(nitems ? nitems : 1);
-            c.alloc(nitems ? nitems : 1);
-            if (!nitems) return 0;
-            prim::select_flagged_u32(istart.get(), f, a.get(), wc.get(), nitems, s);
-            prim::select_flagged_u32(iend.get(), f, b.get(), wc.get(), nitems, s);
-            prim::select_flagged_u32(iu.get(), f, c.get(), wc.get(), nitems, s);
-            HIP_OK(hipMemcpyAsync(&cnt, wc.get(), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        if (!nitems) continue;
+        DevBuf<uint8_t> cls[3] = {DevBuf<uint8_t>(nitems), DevBuf<uint8_t>(nitems), DevBuf<uint8_t>(nitems)};
+        k_tc_item_meta<<<blocks_for(nitems, 256), 256, 0, s>>>(istart.get(), owner, loff, t->tc_off.get(), nitems,
+                                                              kept, iu.get(), iend.get(), cls[1].get(), cls[2].get(),
+                                                              cls[0].get());
+        check_launch();
+        // Split the items into the per-wave class and the two per-workgroup classes.
+        for (int c = 0; c < 3; ++c) {
+            auto& it = items[c];
+            it.start.alloc(nitems);
+            it.end.alloc(nitems);
+            it.owner.alloc(nitems);
+            prim::select_flagged_u32(istart.get(), cls[c].get(), it.start.get(), wc.get(), nitems, s);
+            prim::select_flagged_u32(iend.get(), cls[c].get(), it.end.get(), wc.get(), nitems, s);
+            prim::select_flagged_u32(iu.get(), cls[c].get(), it.owner.get(), wc.get(), nitems, s);
+            HIP_OK(hipMemcpyAsync(&it.n, wc.get(), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
             HIP_OK(hipStreamSynchronize(s));
-            return cnt;
-        };
-        nwork = part(small.get(), t->tc_istart, t->tc_iend, t->tc_iu);
-        t->tc_nbig = part(mid.get(), t->tc_bstart, t->tc_bend, t->tc_bu);
-        t->tc_nhuge = part(big.get(), t->tc_hstart, t->tc_hend, t->tc_hu);
-    } else {
-        for (auto* b : {&t->tc_istart, &t->tc_iend, &t->tc_iu, &t->tc_bstart, &t->tc_bend, &t->tc_bu, &t->tc_hstart,
-                        &t->tc_hend, &t->tc_hu})
-            b->alloc(1);
-        t->tc_nbig = 0;
-        t->tc_nhuge = 0;
+        }
     }
     DevBuf<unsigned long long> pairs(1);
     HIP_OK(hipMemsetAsync(pairs.get(), 0, sizeof(unsigned long long), s));
@@ -540,7 +561,6 @@ static void tc_orient(ursa_hip_topo* t, hipStream_t s) {
     HIP_OK(hipMemcpyAsync(&hp, pairs.get(), sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
     t->tc_slots = kept;
-    t->tc_nwork = nwork;
     t->tc_pair_work = (double)hp;
     t->tc_built = true;
 }
@@ -554,22 +574,42 @@ void triangle_count(ursa_hip_topo* t, uint64_t* out_user) {
     tc_orient(t, s);
     DevOut<uint64_t> out(out_user, n);
     HIP_OK(hipMemsetAsync(out.p, 0, n * sizeof(uint64_t), s));
-    if (t->tc_nwork || t->tc_nbig || t->tc_nhuge) {
+    uint64_t nitems = 0;
+    for (const auto& kind : t->tc_items)
+        for (const auto& it : kind) nitems += it.n;
+    if (nitems) {
         const double bytes = 4.0 * t->tc_pair_work + 12.0 * (double)n;
         auto* tri = reinterpret_cast<unsigned long long*>(out.p);
+        const uint32_t* ooff = t->tc_off.get();
+        const uint32_t* otgt = t->tc_tgt.get();
         timed("triangle_count", bytes, s, [&] {
-            if (t->tc_nhuge)
-                k_tc_count_big<BIG_CAP><<<(unsigned)std::min<uint64_t>(t->tc_nhuge, 1u << 16), TCB_BIG, 0, s>>>(
-                    t->tc_off.get(), t->tc_tgt.get(), t->tc_hstart.get(), t->tc_hend.get(), t->tc_hu.get(),
-                    t->tc_nhuge, tri);
-            if (t->tc_nbig)
-                k_tc_count_big<MID_CAP><<<(unsigned)std::min<uint64_t>(t->tc_nbig, 1u << 16), TCB_BIG, 0, s>>>(
-                    t->tc_off.get(), t->tc_tgt.get(), t->tc_bstart.get(), t->tc_bend.get(), t->tc_bu.get(),
-                    t->tc_nbig, tri);
-            if (t->tc_nwork)
-                k_tc_count<<<(unsigned)std::min<uint64_t>((t->tc_nwork + TCW - 1) / TCW, TC_SMALL_GRID), TCB, 0, s>>>(
-                    t->tc_off.get(), t->tc_tgt.get(), t->tc_istart.get(), t->tc_iend.get(), t->tc_iu.get(),
-                    t->tc_nwork, tri);
+            for (int kind = 0; kind < 2; ++kind) {
+                const auto& it = t->tc_items[kind];
+                const uint32_t* plist = kind ? t->tc_isrc.get() : otgt;
+                auto grid_big = [](uint64_t k) { return (unsigned)std::min<uint64_t>(k, 1u << 16); };
+                const unsigned grid_small = (unsigned)std::min<uint64_t>((it[0].n + TCW - 1) / TCW, TC_SMALL_GRID);
+                if (kind == 0) {
+                    if (it[2].n)
+                        k_tc_count_big<BIG_CAP, false><<<grid_big(it[2].n), TCB_BIG, 0, s>>>(
+                            ooff, otgt, plist, it[2].start.get(), it[2].end.get(), it[2].owner.get(), it[2].n, tri);
+                    if (it[1].n)
+                        k_tc_count_big<MID_CAP, false><<<grid_big(it[1].n), TCB_BIG, 0, s>>>(
+                            ooff, otgt, plist, it[1].start.get(), it[1].end.get(), it[1].owner.get(), it[1].n, tri);
+                    if (it[0].n)
+                        k_tc_count<false><<<grid_small, TCB, 0, s>>>(ooff, otgt, plist, it[0].start.get(),
+                                                                     it[0].end.get(), it[0].owner.get(), it[0].n, tri);
+                } else {
+                    if (it[2].n)
+                        k_tc_count_big<BIG_CAP, true><<<grid_big(it[2].n), TCB_BIG, 0, s>>>(
+                            ooff, otgt, plist, it[2].start.get(), it[2].end.get(), it[2].owner.get(), it[2].n, tri);
+                    if (it[1].n)
+                        k_tc_count_big<MID_CAP, true><<<grid_big(it[1].n), TCB_BIG, 0, s>>>(
+                            ooff, otgt, plist, it[1].start.get(), it[1].end.get(), it[1].owner.get(), it[1].n, tri);
+                    if (it[0].n)
+                        k_tc_count<true><<<grid_small, TCB, 0, s>>>(ooff, otgt, plist, it[0].start.get(),
+                                                                    it[0].end.get(), it[0].owner.get(), it[0].n, tri);
+                }
+            }
         });
         check_launch();
     }
