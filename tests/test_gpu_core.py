@@ -230,6 +230,40 @@ def test_caller_stream(ua):
     assert np.array_equal(lv, ua.bfs_distances(t, 0))
 
 
+def test_device_outputs_written_on_return(ua):
+    # Calls return only when their results are written, also when the output is
+    # device memory and the call's temporaries were kept for reuse instead of
+    # freed: a plain hipMemcpy, which does not wait for the library's
+    # non-blocking stream, must read the final values right after each call.
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n, s, d = rmat_dense(16, 31)
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    lib = ua.load()
+    srcs = np.array([0, 5, 9, 17], np.uint32)
+    want_lv = np.stack([orc.bfs(o, int(x)) for x in srcs])
+    dev = C.c_void_p()
+    ua.check(lib.ursa_hip_device_alloc(8 * n + 4 * len(srcs) * n, C.byref(dev)))
+    try:
+        dtri = dev.value
+        dlv = dev.value + 8 * n
+        reached = C.c_uint64(0)
+        for _ in range(3):  # later calls reuse the blocks the first one kept
+            ua.check(lib.ursa_hip_triangle_count(t._h, C.c_void_p(dtri)))
+            tri = np.zeros(n, np.uint64)
+            assert hip.hipMemcpy(tri.ctypes.data_as(C.c_void_p), C.c_void_p(dtri), 8 * n, 2) == 0
+            assert np.array_equal(tri, ua.triangle_count(t))
+            ua.check(lib.ursa_hip_bfs_ex(t._h, srcs.ctypes.data_as(C.c_void_p), len(srcs), 0,
+                                         C.c_void_p(dlv), C.byref(reached), 0))
+            lv = np.zeros((len(srcs), n), np.int32)
+            assert hip.hipMemcpy(lv.ctypes.data_as(C.c_void_p), C.c_void_p(dlv), 4 * len(srcs) * n, 2) == 0
+            assert np.array_equal(lv, want_lv)
+    finally:
+        ua.check(lib.ursa_hip_device_free(dev))
+
+
 # ---------------------------------------------------------------- device ingest
 def test_intern_i64_matches_oracle(ua):
     rng = np.random.default_rng(21)
