@@ -80,12 +80,14 @@ __global__ void k_tc_item_flags(const uint32_t* __restrict__ loff, const uint32_
     for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < kept; k += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t o = owner[k];
         const uint32_t a = loff[o], e = loff[o + 1], du = ooff[o + 1] - ooff[o];
-        // Lists staged per workgroup or read from HBM are cut 8x coarser (about
+        // Lists staged in the mid-size workgroup table are cut 8x coarser (about
         // 256 K steps), so their larger staging and count flush are paid less
         // often, while the longest lists still spread over several workgroups
         // instead of finishing the launch alone (at s22: 30.8 ms against 32.5
-        // uncut, 31.8 at 32x and 37.8 at 2x).
-        const uint64_t win = du > STAGE_CAP ? 8 * TC_CHUNK : TC_CHUNK;
+        // uncut, 31.8 at 32x and 37.8 at 2x). The few lists of the large table
+        // and those read from HBM (no staging) keep the plain window, so that
+        // their launches are not left to a handful of long items.
+        const uint64_t win = du > STAGE_CAP && du <= MID_CAP ? 8 * TC_CHUNK : TC_CHUNK;
         const bool work = pre[e - 1] + cost[e - 1] > pre[a];
         flag[k] = work && ((k == a) || (pre[k] / win != pre[k - 1] / win));
         iota[k] = (uint32_t)k;
