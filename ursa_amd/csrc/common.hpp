@@ -67,6 +67,12 @@ int guard(F&& f) {
 int ensure_device();
 // The calling thread's stream (non-blocking, default priority).
 hipStream_t stream();
+// Work beside `main`: fork_side returns the thread's side stream, ordered after
+// everything enqueued on `main` so far; join_side orders `main` after
+// everything enqueued on the side stream since. Memory the side work uses must
+// be allocated before the fork and freed after the join.
+void fork_side(hipStream_t main, hipStream_t* side);
+void join_side(hipStream_t main);
 
 void* dev_alloc(size_t bytes);
 void dev_free(void* p);

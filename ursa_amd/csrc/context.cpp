@@ -24,6 +24,14 @@ struct ThreadStream {
     // already be torn down when thread-local destructors run at process exit.
 };
 thread_local ThreadStream t_stream;
+// A second stream per thread for work that runs beside the main stream's
+// (joined to it by events before the call returns); default priority.
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    int device = -1;
+};
+thread_local SideStream t_side;
 thread_local int t_current_device = -1;
 thread_local hipStream_t t_user_stream = nullptr;
 thread_local Arena* t_arena = nullptr;
@@ -123,6 +131,24 @@ hipStream_t stream() {
         t_stream.device = dev;
     }
     return t_stream.s;
+}
+
+void fork_side(hipStream_t main, hipStream_t* side) {
+    const int dev = ensure_device();
+    if (!t_side.s || t_side.device != dev) {
+        HIP_OK(hipStreamCreateWithFlags(&t_side.s, hipStreamNonBlocking));
+        HIP_OK(hipEventCreateWithFlags(&t_side.fork, hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&t_side.join, hipEventDisableTiming));
+        t_side.device = dev;
+    }
+    HIP_OK(hipEventRecord(t_side.fork, main));
+    HIP_OK(hipStreamWaitEvent(t_side.s, t_side.fork, 0));
+    *side = t_side.s;
+}
+
+void join_side(hipStream_t main) {
+    HIP_OK(hipEventRecord(t_side.join, t_side.s));
+    HIP_OK(hipStreamWaitEvent(main, t_side.join, 0));
 }
 
 // Device blocks freed outside a workspace arena are kept for reuse instead of

@@ -588,33 +588,39 @@ void triangle_count(ursa_hip_topo* t, uint64_t* out_user) {
         const uint32_t* ooff = t->tc_off.get();
         const uint32_t* otgt = t->tc_tgt.get();
         timed("triangle_count", bytes, s, [&] {
+            // In-list items run on the side stream beside the out-list items, so
+            // that each launch's last items overlap the other kind's work.
+            hipStream_t s1 = nullptr;
+            fork_side(s, &s1);
             for (int kind = 0; kind < 2; ++kind) {
                 const auto& it = t->tc_items[kind];
+                const hipStream_t ks = kind ? s1 : s;
                 const uint32_t* plist = kind ? t->tc_isrc.get() : otgt;
                 auto grid_big = [](uint64_t k) { return (unsigned)std::min<uint64_t>(k, 1u << 16); };
                 const unsigned grid_small = (unsigned)std::min<uint64_t>((it[0].n + TCW - 1) / TCW, TC_SMALL_GRID);
                 if (kind == 0) {
                     if (it[2].n)
-                        k_tc_count_big<BIG_CAP, false><<<grid_big(it[2].n), TCB_BIG, 0, s>>>(
+                        k_tc_count_big<BIG_CAP, false><<<grid_big(it[2].n), TCB_BIG, 0, ks>>>(
                             ooff, otgt, plist, it[2].start.get(), it[2].end.get(), it[2].owner.get(), it[2].n, tri);
                     if (it[1].n)
-                        k_tc_count_big<MID_CAP, false><<<grid_big(it[1].n), TCB_BIG, 0, s>>>(
+                        k_tc_count_big<MID_CAP, false><<<grid_big(it[1].n), TCB_BIG, 0, ks>>>(
                             ooff, otgt, plist, it[1].start.get(), it[1].end.get(), it[1].owner.get(), it[1].n, tri);
                     if (it[0].n)
-                        k_tc_count<false><<<grid_small, TCB, 0, s>>>(ooff, otgt, plist, it[0].start.get(),
-                                                                     it[0].end.get(), it[0].owner.get(), it[0].n, tri);
+                        k_tc_count<false><<<grid_small, TCB, 0, ks>>>(ooff, otgt, plist, it[0].start.get(),
+                                                                      it[0].end.get(), it[0].owner.get(), it[0].n, tri);
                 } else {
                     if (it[2].n)
-                        k_tc_count_big<BIG_CAP, true><<<grid_big(it[2].n), TCB_BIG, 0, s>>>(
+                        k_tc_count_big<BIG_CAP, true><<<grid_big(it[2].n), TCB_BIG, 0, ks>>>(
                             ooff, otgt, plist, it[2].start.get(), it[2].end.get(), it[2].owner.get(), it[2].n, tri);
                     if (it[1].n)
-                        k_tc_count_big<MID_CAP, true><<<grid_big(it[1].n), TCB_BIG, 0, s>>>(
+                        k_tc_count_big<MID_CAP, true><<<grid_big(it[1].n), TCB_BIG, 0, ks>>>(
                             ooff, otgt, plist, it[1].start.get(), it[1].end.get(), it[1].owner.get(), it[1].n, tri);
                     if (it[0].n)
-                        k_tc_count<true><<<grid_small, TCB, 0, s>>>(ooff, otgt, plist, it[0].start.get(),
-                                                                    it[0].end.get(), it[0].owner.get(), it[0].n, tri);
+                        k_tc_count<true><<<grid_small, TCB, 0, ks>>>(ooff, otgt, plist, it[0].start.get(),
+                                                                     it[0].end.get(), it[0].owner.get(), it[0].n, tri);
                 }
             }
+            join_side(s);
         });
         check_launch();
     }
