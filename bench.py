@@ -198,7 +198,9 @@ def bench_c2(args, ua, gen):
         ms, launches, b = stats[dom]
         per_launch = b / max(launches, 1)
         avg_us = ms / max(launches, 1) * 1e3
-        kname = {"cc": "k_cc_* (Afforest)", "triangle": "k_tc_count + k_tc_count_big"}[dom]
+        kname = {"cc": "k_cc_* (Afforest)",
+                 "triangle": "k_tc_count + k_tc_count_big, out- and in-list kinds on two streams; "
+                             "avg_launch_us is one call's span"}[dom]
     achieved = per_launch / (avg_us / 1e6) / 1e9
     roofline = {
         "bound": "hbm",
