@@ -46,16 +46,14 @@ constexpr int TC_ILP = 2;            // 64-step groups whose loads are in flight
 constexpr uint32_t TC_LONG = 32;     // walks at least this long are taken by the whole wave
 constexpr int TC_LILP = 8;           // 64-step chunks of long walks in flight together
 
-// Each oriented edge x -> y (a pair) is intersected once: N+(y) is walked
-// against x's table when d+(y) <= TC_SIDE_RATIO * d+(x), by an item over x's
-// out-list (kind 0); otherwise N+(x) is walked against y's table, by an item
-// over y's in-list In+(y) (kind 1). A triangle x < y < z closes at w = z in
-// N+(x) and N+(y). The ratio is 2 rather than 1 (always the shorter side):
-// long walks over one list are cheaper per step than short ones, and at s22
-// 2 measured 32.4 ms against 33.9 (1), 35.2 (4) and 37.0 (8). Walk length of a
-// slot, with `owner` the table's vertex and `partner` the other end; a pair
-// needs |N+(x)| >= 2.
-constexpr uint32_t TC_SIDE_RATIO = 2;
+// Each oriented edge x -> y (a pair) is intersected once, from the shorter
+// side: N+(y) is walked against x's table when d+(y) <= TC_SIDE_RATIO * d+(x),
+// by an item over x's out-list (kind 0); otherwise N+(x) is walked against y's
+// table, by an item over y's in-list In+(y) (kind 1). A triangle x < y < z
+// closes at w = z in N+(x) and N+(y). At s22 the ratio 1 measured 25.4 ms
+// against 28.7 (2) and 32.0 (3). Walk length of a slot, with `owner` the
+// table's vertex and `partner` the other end; a pair needs |N+(x)| >= 2.
+constexpr uint32_t TC_SIDE_RATIO = 1;
 __device__ __forceinline__ uint32_t tc_walk(bool in, uint32_t d_owner, uint32_t d_partner) {
     if (in) return d_partner >= 2 && TC_SIDE_RATIO * d_partner < d_owner ? d_partner : 0u;
     return d_owner >= 2 && d_partner <= TC_SIDE_RATIO * d_owner ? d_partner : 0u;
