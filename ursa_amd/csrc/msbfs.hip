@@ -342,49 +342,68 @@ __global__ void k_ms_reached_edges(const unsigned long long* __restrict__ visit,
 void direction_views(ursa_hip_topo* t, int dir, hipStream_t s, NbrView& fwd, NbrView& rev);
 
 // Levels for up to 64 sources (row b of `levels` = source b) on one stream.
+// The pull side's heavy-vertex list is cached on the topology per direction;
+// the rest is per call (allocations are reused across calls by the device
+// allocator) and the counter readback goes through the thread's pinned scratch.
 struct MsWork {
     uint64_t n;
     DevBuf<unsigned long long> visit, f0, f1, redges, hacc;
-    DevBuf<uint32_t> q0, q1, dsrc, heavy, pheavy, pcbase;
+    DevBuf<uint32_t> q0, q1, dsrc, heavy;
+    const uint32_t* pheavy = nullptr;
+    const uint32_t* pcbase = nullptr;
     uint32_t pnh = 0, pchunks = 0;  // pull-side heavy vertices and their chunk count
     DevBuf<MsCtr> ctr;
-    MsCtr* h = nullptr;
     explicit MsWork(uint64_t n_)
-        : n(n_), visit(n_), f0(n_), f1(n_), redges(1), q0(n_), q1(n_), dsrc(64), heavy(n_), ctr(1) {
-        HIP_OK(hipHostMalloc((void**)&h, sizeof(MsCtr), hipHostMallocDefault));
-    }
-    ~MsWork() {
-        if (h) (void)hipHostFree(h);
-    }
+        : n(n_), visit(n_), f0(n_), f1(n_), redges(1), q0(n_), q1(n_), dsrc(64), heavy(n_), ctr(1) {}
     MsCtr read(hipStream_t s) {
+        MsCtr* h = static_cast<MsCtr*>(pinned_scratch(sizeof(MsCtr)));
         HIP_OK(hipMemcpyAsync(h, ctr.get(), sizeof(MsCtr), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
         return *h;
     }
     // Heavy reverse-list vertices and chunk bases for the pull levels.
-    void prepare_pull(const NbrView& rev, hipStream_t s) {
-        DevBuf<uint8_t> flag(n);
-        DevBuf<uint32_t> iota(n), chunks(n);
-        DevBuf<uint64_t> cnt(1);
-        pheavy.alloc(n);
-        k_ms_heavy_flags<<<blocks_for(n, 256, 1u << 16), 256, 0, s>>>(rev, n, flag.get(), iota.get(), chunks.get());
-        check_launch();
-        prim::select_flagged_u32(iota.get(), flag.get(), pheavy.get(), cnt.get(), n, s);
-        uint64_t c = 0;
-        HIP_OK(hipMemcpyAsync(&c, cnt.get(), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
-        pnh = (uint32_t)c;
-        pcbase.alloc(pnh + 1);
+    void prepare_pull(ursa_hip_topo* t, int dir, const NbrView& rev, hipStream_t s) {
+        {
+            NoArena keep_out_of_arena;
+            std::lock_guard<std::mutex> lk(t->cache_mu);
+            if (!t->ms_pull_ready[dir]) {
+                DevBuf<uint8_t> flag(n);
+                DevBuf<uint32_t> iota(n), chunks(n);
+                DevBuf<uint64_t> cnt(1);
+                DevBuf<uint32_t> ph(n);
+                k_ms_heavy_flags<<<blocks_for(n, 256, 1u << 16), 256, 0, s>>>(rev, n, flag.get(), iota.get(),
+                                                                               chunks.get());
+                check_launch();
+                prim::select_flagged_u32(iota.get(), flag.get(), ph.get(), cnt.get(), n, s);
+                uint64_t c = 0;
+                HIP_OK(hipMemcpyAsync(&c, cnt.get(), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+                HIP_OK(hipStreamSynchronize(s));
+                const uint32_t nh = (uint32_t)c;
+                DevBuf<uint32_t> cb(nh + 1);
+                uint32_t nch = 0;
+                if (nh) {
+                    DevBuf<uint32_t> hch(nh);
+                    k_ms_gather<<<blocks_for(nh, 256), 256, 0, s>>>(ph.get(), nh, chunks.get(), hch.get());
+                    check_launch();
+                    prim::exclusive_sum_u32(hch.get(), cb.get(), nh, s);
+                    HIP_OK(hipMemcpyAsync(&nch, cb.get() + nh, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+                }
+                DevBuf<uint32_t> keep(nh ? nh : 1);  // the list at its own length
+                if (nh) HIP_OK(hipMemcpyAsync(keep.get(), ph.get(), nh * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+                HIP_OK(hipStreamSynchronize(s));
+                t->ms_pheavy[dir] = std::move(keep);
+                t->ms_pcbase[dir] = std::move(cb);
+                t->ms_pnh[dir] = nh;
+                t->ms_pchunks[dir] = nch;
+                t->ms_pull_ready[dir] = true;
+            }
+            pheavy = t->ms_pheavy[dir].get();
+            pcbase = t->ms_pcbase[dir].get();
+            pnh = t->ms_pnh[dir];
+            pchunks = t->ms_pchunks[dir];
+        }
         hacc.alloc(pnh ? pnh : 1);
         HIP_OK(hipMemsetAsync(hacc.get(), 0, (pnh ? pnh : 1) * sizeof(unsigned long long), s));
-        if (pnh) {
-            DevBuf<uint32_t> hch(pnh);
-            k_ms_gather<<<blocks_for(pnh, 256), 256, 0, s>>>(pheavy.get(), pnh, chunks.get(), hch.get());
-            check_launch();
-            prim::exclusive_sum_u32(hch.get(), pcbase.get(), pnh, s);
-            HIP_OK(hipMemcpyAsync(&pchunks, pcbase.get() + pnh, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        }
-        HIP_OK(hipStreamSynchronize(s));
     }
 };
 
@@ -455,9 +474,9 @@ void msbfs_batch(uint64_t n, uint64_t fwd_edges, const uint32_t* src_host, uint3
                 k_ms_pull<<<gv, MB, 0, s>>>(n, rev, front, next, w.visit.get(), valid, levels, L, w.ctr.get());
                 if (w.pnh) {
                     k_ms_pull_heavy<<<blocks_for(w.pchunks, MW, 1u << 16), MB, 0, s>>>(
-                        w.pheavy.get(), w.pcbase.get(), w.pnh, w.pchunks, rev, front, w.visit.get(), valid,
+                        w.pheavy, w.pcbase, w.pnh, w.pchunks, rev, front, w.visit.get(), valid,
                         w.hacc.get());
-                    k_ms_pull_finish<<<blocks_for(w.pnh, 256), 256, 0, s>>>(w.pheavy.get(), w.pnh, w.hacc.get(), next,
+                    k_ms_pull_finish<<<blocks_for(w.pnh, 256), 256, 0, s>>>(w.pheavy, w.pnh, w.hacc.get(), next,
                                                                            w.visit.get(), levels, n, L, w.ctr.get());
                 }
             });
@@ -489,7 +508,7 @@ void msbfs_levels_each(ursa_hip_topo* t, const std::vector<uint32_t>& src, int d
     NbrView fwd, rev;
     direction_views(t, dir, s, fwd, rev);
     MsWork w(n);
-    w.prepare_pull(rev, s);
+    w.prepare_pull(t, dir, rev, s);
     const uint64_t fwd_edges = dir == URSA_HIP_BOTH ? 2 * t->m : t->m;
     for (uint64_t b0 = 0; b0 < src.size(); b0 += 64) {
         const uint32_t nb = (uint32_t)std::min<uint64_t>(64, src.size() - b0);
@@ -505,7 +524,7 @@ void msbfs_levels(ursa_hip_topo* t, const std::vector<uint32_t>& src, int dir, i
     NbrView fwd, rev;
     direction_views(t, dir, s, fwd, rev);
     MsWork w(n);
-    w.prepare_pull(rev, s);
+    w.prepare_pull(t, dir, rev, s);
     const uint64_t fwd_edges = dir == URSA_HIP_BOTH ? 2 * t->m : t->m;
     for (uint64_t b0 = 0; b0 < src.size(); b0 += 64) {
         const uint32_t nb = (uint32_t)std::min<uint64_t>(64, src.size() - b0);

@@ -101,6 +101,7 @@ uint64_t topo_device_bytes(const ursa_hip_topo* t) {
     for (const auto& kind : t->tc_items)
         for (const auto& it : kind) b += bytes_of(it.start) + bytes_of(it.end) + bytes_of(it.owner);
     b += bytes_of(t->ppr_hrow) + bytes_of(t->ppr_hsoff) + bytes_of(t->ppr_sbeg) + bytes_of(t->ppr_send);
+    for (int d = 0; d < 3; ++d) b += bytes_of(t->ms_pheavy[d]) + bytes_of(t->ms_pcbase[d]);
     return b;
 }
 

@@ -60,6 +60,12 @@ struct ursa_hip_topo {
     ursa::DevBuf<uint32_t> pr_tile_x, pr_tile_y;
     uint64_t pr_tiles = 0;
     bool pr_ready = false;
+    // Multi-source BFS pull side, per direction (see msbfs.hip): the vertices
+    // whose reverse lists are cut into chunks and their chunk bases; depends
+    // on the graph only.
+    ursa::DevBuf<uint32_t> ms_pheavy[3], ms_pcbase[3];
+    uint32_t ms_pnh[3] = {0, 0, 0}, ms_pchunks[3] = {0, 0, 0};
+    bool ms_pull_ready[3] = {false, false, false};
     // Degree-oriented undirected DAG for triangle counting (see triangle.hip):
     // out-lists N+(u), in-lists In+(v), and work items over either list kind
     // ([0] out, [1] in) in three classes ([0] per wave, [1] per workgroup with
