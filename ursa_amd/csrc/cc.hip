@@ -60,11 +60,63 @@ __global__ void k_cc_init(uint32_t* comp, uint64_t n) {
         comp[i] = (uint32_t)i;
 }
 
-__global__ void k_cc_round(const uint32_t* __restrict__ off, const uint32_t* __restrict__ tgt, uint64_t n, uint32_t r,
+// Round 0, first half: u's first neighbour v = tgt[off[u]], when smaller than
+// u, becomes u's parent by a plain store (each thread writes only comp[u], and
+// every comp[] is still its own id). The pointers, all to smaller ids, form a
+// forest whose trees join only adjacent vertices, as hooking would: this
+// replaces most of round 0's compare-and-swap hooks, which all ran at once on
+// a fresh forest and retried along growing parent chains.
+__global__ void k_cc_first(const uint32_t* __restrict__ off, const uint32_t* __restrict__ tgt, uint64_t n,
                            uint32_t* comp) {
     for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n; u += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t a = off[u], b = off[u + 1];
-        if (a + r < b) hook((uint32_t)u, tgt[a + r], comp);
+        if (a < b) {
+            const uint32_t v = tgt[a];
+            if (v < u) comp[u] = v;
+        }
+    }
+}
+
+// Round 0, second half: hooks the first neighbours larger than the vertex,
+// which k_cc_first left out.
+__global__ void k_cc_round0(const uint32_t* __restrict__ off, const uint32_t* __restrict__ tgt, uint64_t n,
+                            uint32_t* comp) {
+    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n; u += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t a = off[u], b = off[u + 1];
+        if (a < b) {
+            const uint32_t v = tgt[a];
+            if (v > u) hook((uint32_t)u, v, comp);
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t root_of(uint32_t x, const uint32_t* comp) {
+    uint32_t c = ld(&comp[x]);
+    while (c != x) {
+        x = c;
+        c = ld(&comp[x]);
+    }
+    return x;
+}
+
+// Round 1: the roots of u and of its second neighbour are linked, larger to
+// smaller, by a plain store while the larger is still a root. After round 0
+// whole trees share one root, and compare-and-swap hooks from all their
+// vertices on that root's word serialised; a store that races with another
+// may drop a link, never adds a wrong one (both roots are connected through
+// the edge, and every pointer still goes to a smaller id). Slot 1 is swept
+// again with exact hooks afterwards, so a dropped link is made there.
+__global__ void k_cc_round1(const uint32_t* __restrict__ off, const uint32_t* __restrict__ tgt, uint64_t n,
+                            uint32_t* comp) {
+    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n; u += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t a = off[u], b = off[u + 1];
+        if (a + 1 < b) {
+            const uint32_t p1 = root_of((uint32_t)u, comp), p2 = root_of(tgt[a + 1], comp);
+            if (p1 != p2) {
+                const uint32_t high = max(p1, p2), low = min(p1, p2);
+                if (ld(&comp[high]) == high) __hip_atomic_store(&comp[high], low, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
     }
 }
 
@@ -132,7 +184,14 @@ void cc_weak(ursa_hip_topo* t, uint32_t* out_user, uint32_t* sweeps_out) {
     check_launch();
     const uint32_t rounds = 2;
     for (uint32_t r = 0; r < rounds; ++r) {
-        timed("cc_hook_round", nb + 8.0 * (double)n, s, [&] { k_cc_round<<<g, CB, 0, s>>>(o.off.get(), o.tgt.get(), n, r, comp); });
+        timed("cc_hook_round", nb + 8.0 * (double)n, s, [&] {
+            if (r == 0) {
+                k_cc_first<<<g, CB, 0, s>>>(o.off.get(), o.tgt.get(), n, comp);
+                k_cc_round0<<<g, CB, 0, s>>>(o.off.get(), o.tgt.get(), n, comp);
+            } else {
+                k_cc_round1<<<g, CB, 0, s>>>(o.off.get(), o.tgt.get(), n, comp);
+            }
+        });
         timed("cc_compress", nb, s, [&] { k_cc_compress<<<g, CB, 0, s>>>(comp, n); });
         check_launch();
     }
@@ -173,7 +232,7 @@ void cc_weak(ursa_hip_topo* t, uint32_t* out_user, uint32_t* sweeps_out) {
         });
         check_launch();
     };
-    sweep(o, rounds, "cc_hook_out");
+    sweep(o, 1, "cc_hook_out");  // from slot 1: round 1's links may have been dropped
     if (have_in) sweep(t->in, 0, "cc_hook_in");
     timed("cc_compress", nb, s, [&] { k_cc_compress<<<g, CB, 0, s>>>(comp, n); });
     check_launch();
