@@ -14,7 +14,10 @@
  *
  * Threading: any number of threads may call concurrently on one topology.
  * The lazy incoming/undirected builds run once per topology (topology.rs:
- * 257-260). Each calling thread gets its own HIP stream.
+ * 257-260). Each calling thread gets its own HIP stream (triangle counting
+ * also uses a second one, joined before the call returns). Temporary device
+ * buffers are kept for reuse on the thread's own stream rather than freed;
+ * memory from ursa_hip_device_alloc is never kept.
  */
 #ifndef URSA_HIP_H
 #define URSA_HIP_H
