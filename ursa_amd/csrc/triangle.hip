@@ -46,17 +46,22 @@ constexpr int TC_ILP = 2;            // 64-step groups whose loads are in flight
 constexpr uint32_t TC_LONG = 32;     // walks at least this long are taken by the whole wave
 constexpr int TC_LILP = 8;           // 64-step chunks of long walks in flight together
 
-// Each oriented edge x -> y (a pair) is intersected once, from the shorter
-// side: N+(y) is walked against x's table when d+(y) <= TC_SIDE_RATIO * d+(x),
-// by an item over x's out-list (kind 0); otherwise N+(x) is walked against y's
-// table, by an item over y's in-list In+(y) (kind 1). A triangle x < y < z
-// closes at w = z in N+(x) and N+(y). At s22 the ratio 1 measured 25.4 ms
-// against 28.7 (2) and 32.0 (3). Walk length of a slot, with `owner` the
-// table's vertex and `partner` the other end; a pair needs |N+(x)| >= 2.
-constexpr uint32_t TC_SIDE_RATIO = 1;
+// Each oriented edge x -> y (a pair) is intersected once: from y's side (N+(x)
+// walked against y's table, by an item over y's in-list In+(y), kind 1) when
+// N+(y) is longer than N+(x) and than TC_HUB entries, else from x's side (N+(y)
+// walked against x's table, by an item over x's out-list, kind 0). So the
+// shorter list is walked wherever the longer one is a hub's; pairs among short
+// lists stay on x's side, where each table is staged once and no partner needs
+// an atomic credit (on graphs of even, small degrees splitting them as well
+// doubled the staging and the launches). At s22 a hub threshold of 32 to 128
+// measured 25.4 to 25.6 ms, 256 26.5 ms.
+// A triangle x < y < z closes at w = z in N+(x) and N+(y). Walk length of a
+// slot, with `owner` the table's vertex and `partner` the other end; a pair
+// needs |N+(x)| >= 2.
+constexpr uint32_t TC_HUB = 128;
 __device__ __forceinline__ uint32_t tc_walk(bool in, uint32_t d_owner, uint32_t d_partner) {
-    if (in) return d_partner >= 2 && TC_SIDE_RATIO * d_partner < d_owner ? d_partner : 0u;
-    return d_owner >= 2 && d_partner <= TC_SIDE_RATIO * d_owner ? d_partner : 0u;
+    if (in) return d_partner >= 2 && d_partner < d_owner && d_owner > TC_HUB ? d_partner : 0u;
+    return d_owner >= 2 && (d_partner <= d_owner || d_partner <= TC_HUB) ? d_partner : 0u;
 }
 
 __global__ void k_tc_cost(const uint32_t* __restrict__ ooff, const uint32_t* __restrict__ partner,
@@ -588,8 +593,10 @@ void triangle_count(ursa_hip_topo* t, uint64_t* out_user) {
         timed("triangle_count", bytes, s, [&] {
             // In-list items run on the side stream beside the out-list items, so
             // that each launch's last items overlap the other kind's work.
-            hipStream_t s1 = nullptr;
-            fork_side(s, &s1);
+            const auto& in_items = t->tc_items[1];
+            const bool side = in_items[0].n + in_items[1].n + in_items[2].n > 0;
+            hipStream_t s1 = s;
+            if (side) fork_side(s, &s1);
             for (int kind = 0; kind < 2; ++kind) {
                 const auto& it = t->tc_items[kind];
                 const hipStream_t ks = kind ? s1 : s;
@@ -618,7 +625,7 @@ void triangle_count(ursa_hip_topo* t, uint64_t* out_user) {
                                                                      it[0].end.get(), it[0].owner.get(), it[0].n, tri);
                 }
             }
-            join_side(s);
+            if (side) join_side(s);
         });
         check_launch();
     }
