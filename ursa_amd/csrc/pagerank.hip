@@ -181,11 +181,30 @@ __global__ __launch_bounds__(PR_BLOCK) void k_pr_tiles(
     const uint64_t n_end = (x1 < n ? x1 : n - 1) - x0 + 1;  // row ends staged (incl. trailing row)
     const double base = ctl->base[parity];
 
-    for (uint64_t i = threadIdx.x; i < nnz; i += PR_BLOCK) {
-        const uint64_t k = y0 + i;
-        CT c = contrib[__builtin_nontemporal_load(&in_tgt[k])];
-        if constexpr (WEIGHTED) c = (CT)__builtin_nontemporal_load(&w_in[k]) * c;
-        s_val[lds_slot<CT>((uint32_t)i)] = c;
+    {
+        // All of this thread's columns first, then all its gathers: PR_ITEMS
+        // independent gathers in flight per thread instead of one at a time.
+        uint32_t col[PR_ITEMS];
+#pragma unroll
+        for (int g = 0; g < PR_ITEMS; ++g) {
+            const uint64_t i = threadIdx.x + (uint64_t)g * PR_BLOCK;
+            col[g] = i < nnz ? __builtin_nontemporal_load(&in_tgt[y0 + i]) : 0u;
+        }
+        CT cv[PR_ITEMS];
+#pragma unroll
+        for (int g = 0; g < PR_ITEMS; ++g) {
+            const uint64_t i = threadIdx.x + (uint64_t)g * PR_BLOCK;
+            if (i < nnz) cv[g] = contrib[col[g]];
+        }
+#pragma unroll
+        for (int g = 0; g < PR_ITEMS; ++g) {
+            const uint64_t i = threadIdx.x + (uint64_t)g * PR_BLOCK;
+            if (i < nnz) {
+                CT c = cv[g];
+                if constexpr (WEIGHTED) c = (CT)__builtin_nontemporal_load(&w_in[y0 + i]) * c;
+                s_val[lds_slot<CT>((uint32_t)i)] = c;
+            }
+        }
     }
     if (x0 < n)
         for (uint64_t i = threadIdx.x; i < n_end; i += PR_BLOCK) s_end[i] = __builtin_nontemporal_load(&in_off[x0 + 1 + i]);
