@@ -48,6 +48,7 @@ constexpr int PPR_GROUPS = PB / PPR_G;       // rows per workgroup pass
 constexpr uint32_t PPR_ROWS = 128;           // rows per workgroup (one partial each)
 constexpr uint32_t PPR_HEAVY = 256;
 constexpr uint32_t PPR_MAXSEG = 1024;
+constexpr int PPR_BATCH = 4;                 // slots read together per thread (1-2 lanes)
 
 struct PprLane {
     double d, tol, inv_s, pb, prev, delta;
@@ -122,15 +123,7 @@ __device__ __forceinline__ void ppr_row_sum(uint32_t b, uint32_t e, uint32_t g, 
         any32 |= ph[q] == 0 || ph[q] == 1;
         any64 |= ph[q] == 2;
     }
-#pragma unroll 2
-    for (uint32_t k = b + g; k < e; k += PPR_G) {
-        const uint64_t v = tgt[k];
-        // The KP values of vertex v are adjacent: read them with wide loads,
-        // then add lane by lane (the per-lane arithmetic does not change).
-        float x32[KP];
-        double x64[KP];
-        if (any32) load_lanes<KP>(c32 + v * KP, x32);
-        if (any64) load_lanes<KP>(c64 + v * KP, x64);
+    auto add = [&](const float* x32, const double* x64) {
 #pragma unroll
         for (int q = 0; q < KP; ++q) {
             if (ph[q] == 0)
@@ -139,6 +132,42 @@ __device__ __forceinline__ void ppr_row_sum(uint32_t b, uint32_t e, uint32_t g, 
                 a[q] += (double)x32[q];
             else if (ph[q] == 2)
                 a[q] += x64[q];
+        }
+    };
+    if constexpr (KP <= 2) {
+        // Few lanes: this thread's next PPR_BATCH slots are read together (the
+        // columns, then their values) and added in slot order as before.
+        for (uint32_t k0 = b + g; k0 < e; k0 += PPR_BATCH * PPR_G) {
+            uint64_t vv[PPR_BATCH];
+#pragma unroll
+            for (int j = 0; j < PPR_BATCH; ++j) {
+                const uint32_t k = k0 + j * PPR_G;
+                vv[j] = k < e ? tgt[k] : 0u;
+            }
+            float x32[PPR_BATCH][KP];
+            double x64[PPR_BATCH][KP];
+#pragma unroll
+            for (int j = 0; j < PPR_BATCH; ++j) {
+                if (k0 + j * PPR_G < e) {
+                    if (any32) load_lanes<KP>(c32 + vv[j] * KP, x32[j]);
+                    if (any64) load_lanes<KP>(c64 + vv[j] * KP, x64[j]);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < PPR_BATCH; ++j)
+                if (k0 + j * PPR_G < e) add(x32[j], x64[j]);
+        }
+    } else {
+#pragma unroll 2
+        for (uint32_t k = b + g; k < e; k += PPR_G) {
+            const uint64_t v = tgt[k];
+            // The KP values of vertex v are adjacent: read them with wide loads,
+            // then add lane by lane (the per-lane arithmetic does not change).
+            float x32[KP];
+            double x64[KP];
+            if (any32) load_lanes<KP>(c32 + v * KP, x32);
+            if (any64) load_lanes<KP>(c64 + v * KP, x64);
+            add(x32, x64);
         }
     }
     // Butterfly over the group (lanes differing in the low three bits).
