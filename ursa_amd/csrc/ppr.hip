@@ -48,7 +48,7 @@ constexpr int PPR_GROUPS = PB / PPR_G;       // rows per workgroup pass
 constexpr uint32_t PPR_ROWS = 128;           // rows per workgroup (one partial each)
 constexpr uint32_t PPR_HEAVY = 256;
 constexpr uint32_t PPR_MAXSEG = 1024;
-constexpr int PPR_BATCH = 4;                 // slots read together per thread (1-2 lanes)
+constexpr int PPR_BATCH = 4;                 // slots read together per thread (2 at four lanes)
 
 struct PprLane {
     double d, tol, inv_s, pb, prev, delta;
@@ -134,27 +134,28 @@ __device__ __forceinline__ void ppr_row_sum(uint32_t b, uint32_t e, uint32_t g, 
                 a[q] += x64[q];
         }
     };
-    if constexpr (KP <= 2) {
-        // Few lanes: this thread's next PPR_BATCH slots are read together (the
+    if constexpr (KP <= 4) {
+        // Few lanes: this thread's next NB slots are read together (the
         // columns, then their values) and added in slot order as before.
-        for (uint32_t k0 = b + g; k0 < e; k0 += PPR_BATCH * PPR_G) {
-            uint64_t vv[PPR_BATCH];
+        constexpr int NB = KP <= 2 ? PPR_BATCH : 2;
+        for (uint32_t k0 = b + g; k0 < e; k0 += NB * PPR_G) {
+            uint64_t vv[NB];
 #pragma unroll
-            for (int j = 0; j < PPR_BATCH; ++j) {
+            for (int j = 0; j < NB; ++j) {
                 const uint32_t k = k0 + j * PPR_G;
                 vv[j] = k < e ? tgt[k] : 0u;
             }
-            float x32[PPR_BATCH][KP];
-            double x64[PPR_BATCH][KP];
+            float x32[NB][KP];
+            double x64[NB][KP];
 #pragma unroll
-            for (int j = 0; j < PPR_BATCH; ++j) {
+            for (int j = 0; j < NB; ++j) {
                 if (k0 + j * PPR_G < e) {
                     if (any32) load_lanes<KP>(c32 + vv[j] * KP, x32[j]);
                     if (any64) load_lanes<KP>(c64 + vv[j] * KP, x64[j]);
                 }
             }
 #pragma unroll
-            for (int j = 0; j < PPR_BATCH; ++j)
+            for (int j = 0; j < NB; ++j)
                 if (k0 + j * PPR_G < e) add(x32[j], x64[j]);
         }
     } else {
