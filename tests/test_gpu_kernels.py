@@ -194,6 +194,49 @@ def test_cc_many_small_components(ua):
     assert np.array_equal(ua.connected_components_weak(t), orc.canonicalize(orc.cc_weak(o)))
 
 
+def _cc_shapes():
+    rng = np.random.default_rng(33)
+    out = []
+    # A long path over shuffled ids: every hook round links along a chain, and
+    # the round-1 root links race along it.
+    n = 200000
+    perm = rng.permutation(n)
+    out.append(("shuffled_path", n, perm[:-1], perm[1:]))
+    # Stars whose centres have the largest ids: every leaf's first neighbour is
+    # larger than the leaf, so round 0 hooks them all by compare-and-swap.
+    k, leaves = 300, 400
+    centres = np.arange(k * leaves, k * leaves + k)
+    s = np.repeat(centres, leaves)
+    d = np.arange(k * leaves)
+    out.append(("stars_high_centres", k * leaves + k, s, d))
+    # Two dense clusters joined by one edge whose endpoints carry no other
+    # link in their first two slots.
+    a = rng.integers(0, 5000, 60000)
+    b = rng.integers(0, 5000, 60000)
+    c = rng.integers(5000, 10000, 60000)
+    e = rng.integers(5000, 10000, 60000)
+    s = np.concatenate([a, c, [4999]])
+    d = np.concatenate([b, e, [5000]])
+    out.append(("barbell", 10000, s, d))
+    return out
+
+
+@pytest.mark.parametrize("shape", _cc_shapes(), ids=lambda x: x[0])
+def test_cc_store_rounds(ua, shape):
+    # The two sampling rounds link roots by plain stores, which may drop a link
+    # when they race; the final sweep (all out-slots from 1, and all in-edges
+    # when the transpose exists) must make every dropped link exactly.
+    name, n, s, d = shape
+    o = orc.Topology(n, s, d)
+    ref = orc.canonicalize(orc.cc_weak(o))
+    t = ua.Topology.build(n, s, d)
+    for _ in range(3):
+        assert np.array_equal(ua.connected_components_weak(t), ref), name  # no transpose
+    t.incoming()
+    for _ in range(3):
+        assert np.array_equal(ua.connected_components_weak(t), ref), name  # giant-component skip
+
+
 # ---------------------------------------------------------------- triangles
 @pytest.mark.parametrize("case", GRAPHS, ids=IDS)
 def test_triangles_match(ua, case):
