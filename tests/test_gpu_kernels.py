@@ -68,6 +68,20 @@ def test_bfs_rmat_direction_optimizing(ua, rmat14, mode):
             assert np.array_equal(got[i], orc.bfs(o, int(x), direction)), (direction, int(x))
 
 
+def test_msbfs_directions_any_order(ua):
+    # The pull lists are built once per direction on first use; the order in
+    # which directions are first asked for must not change any level.
+    n, s, d = rmat_dense(14, 5)
+    o = orc.Topology(n, s, d)
+    src = np.random.default_rng(3).integers(0, n, 40)
+    want = {x: [orc.bfs(o, int(v), x) for v in src] for x in DIRS}
+    t = ua.Topology.build(n, s, d)
+    for direction in ("both", "in", "out", "out", "in", "both"):
+        got = ua.bfs_multi(t, src, direction, mode=2)
+        for i in range(len(src)):
+            assert np.array_equal(got[i], want[direction][i]), (direction, i)
+
+
 def test_msbfs_two_batches_with_duplicates(ua, rmat14):
     n, _, _, t, o = rmat14
     rng = np.random.default_rng(2)

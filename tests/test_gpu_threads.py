@@ -78,3 +78,21 @@ def test_concurrent_lazy_builds(ua):
             assert np.array_equal(ua.triangle_count(t), want_tri), (k, "triangles")
 
     _run_threads(work, 4)
+
+
+def test_concurrent_pull_lists_per_direction(ua):
+    # A fresh topology: six threads, two per direction, race to build the
+    # multi-source pull lists, which are kept per direction on the topology.
+    n, s, d = rmat_dense(14, 5)
+    srcs = np.random.default_rng(7).integers(0, n, 70)
+    dirs = (ua.Direction.Out, ua.Direction.In, ua.Direction.Both)
+    ref = ua.Topology.build(n, s, d)
+    want = [ua.bfs_multi(ref, srcs, x, mode=2) for x in dirs]
+    t = ua.Topology.build(n, s, d)
+
+    def work(k):
+        for r in range(3):
+            j = (k + r) % 3
+            assert np.array_equal(ua.bfs_multi(t, srcs, dirs[j], mode=2), want[j]), (k, r, j)
+
+    _run_threads(work, 6)
