@@ -463,18 +463,19 @@ __global__ void k_rl_invert(const uint32_t* __restrict__ old_of_new, uint64_t n,
     }
 }
 
-// Row u's in-slot j moves to row new(u) slot j: the order inside each row, and
-// so every row's summation order, is unchanged.
-__global__ void k_rl_slots(const uint32_t* __restrict__ owner, const uint32_t* __restrict__ in_off,
-                           const uint32_t* __restrict__ in_tgt, const uint32_t* __restrict__ in_eid,
-                           const uint32_t* __restrict__ new_of_old, const uint32_t* __restrict__ r_off, uint64_t m,
-                           uint32_t* __restrict__ r_tgt, uint32_t* __restrict__ r_eid) {
-    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < m; k += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t u = owner[k];
-        const uint64_t dest = (uint64_t)r_off[new_of_old[u]] + (k - in_off[u]);
-        r_tgt[dest] = new_of_old[in_tgt[k]];
-        r_eid[dest] = in_eid[k];
-    }
+// Slot k of row u becomes the key (new(u), new(source)) with its edge id
+// beside it; a stable sort of the keys then lists every row's sources in
+// ascending new id. Lanes that gather the same contribution line sit next to
+// each other, so the wave asks L2 for that line once.
+__global__ void k_rl_slot_keys(const uint32_t* __restrict__ owner, const uint32_t* __restrict__ in_tgt,
+                               const uint32_t* __restrict__ new_of_old, uint64_t m, uint64_t* __restrict__ key) {
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < m; k += (uint64_t)gridDim.x * blockDim.x)
+        key[k] = (uint64_t)new_of_old[owner[k]] << 32 | new_of_old[in_tgt[k]];
+}
+
+__global__ void k_rl_low_words(const uint64_t* __restrict__ key, uint64_t m, uint32_t* __restrict__ r_tgt) {
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < m; k += (uint64_t)gridDim.x * blockDim.x)
+        r_tgt[k] = (uint32_t)key[k];
 }
 
 __global__ void k_gather_f64(const double* __restrict__ src, const uint32_t* __restrict__ idx, uint64_t n,
@@ -491,10 +492,12 @@ __global__ void k_fill(double* p, uint64_t n, double v) {
 }  // namespace
 
 // PageRank's working copy of the in-CSR, built once per topology: vertices
-// renumbered by decreasing out-degree (ties by id), rows moved with their
-// in-slot order intact, sources relabelled. The most-gathered contributions
-// (high out-degree sources) then sit in a few megabytes at the front of the
-// contribution vector and stay in L2, instead of being spread over it.
+// renumbered by decreasing out-degree (ties by id), sources relabelled, and
+// each row's sources listed in ascending new id (parallel edges keep their
+// slot order). The most-gathered contributions (high out-degree sources) then
+// sit in a few megabytes at the front of the contribution vector and stay in
+// L2, instead of being spread over it. A row's sum runs in this order, not the
+// in-CSR's: the same terms, rounded in another order.
 // Also caches the merge-path tile partition of that CSR.
 static void pr_prepare(ursa_hip_topo* t, const DevCsr& in, hipStream_t s) {
     NoArena keep_out_of_arena;
@@ -518,11 +521,18 @@ static void pr_prepare(ursa_hip_topo* t, const DevCsr& in, hipStream_t s) {
         prim::exclusive_sum_u32(rdeg.get(), t->pr_off.get(), n, s);
     }
     if (m) {
-        DevBuf<uint32_t> owner(m);
-        slot_owners(n, in, owner.get(), s);
-        k_rl_slots<<<blocks_for(m, 256), 256, 0, s>>>(owner.get(), in.off.get(), in.tgt.get(), in.eid.get(),
-                                                      t->pr_new_of_old.get(), t->pr_off.get(), m, t->pr_tgt.get(),
-                                                      t->pr_eid.get());
+        DevBuf<uint64_t> key(m), skey(m);
+        {
+            DevBuf<uint32_t> owner(m);
+            slot_owners(n, in, owner.get(), s);
+            k_rl_slot_keys<<<blocks_for(m, 256), 256, 0, s>>>(owner.get(), in.tgt.get(), t->pr_new_of_old.get(), m,
+                                                              key.get());
+            check_launch();
+        }
+        int bits = 1;
+        while (bits < 32 && (n - 1) >> bits) ++bits;
+        prim::sort_pairs_u64_u32(key.get(), skey.get(), in.eid.get(), t->pr_eid.get(), m, 32 + bits, s);
+        k_rl_low_words<<<blocks_for(m, 256), 256, 0, s>>>(skey.get(), m, t->pr_tgt.get());
         check_launch();
     }
     const uint64_t tiles = (n + m + PR_TILE - 1) / PR_TILE;
