@@ -68,6 +68,38 @@ def test_ppr_serving_tolerance_rmat(ua):
         assert abs(got.sum() - 1.0) < 1e-8 / 0.15
 
 
+@pytest.mark.parametrize("max_iter", [1, 2, 3, 100])
+@pytest.mark.parametrize("tol", [1e-4, 1e-8])
+def test_ppr_f64_lanes_follow_the_oracle(ua, max_iter, tol):
+    """Lanes with a coarse tol (>= 2e-6) or max_iter <= 16 run f64 sweeps from the
+    uniform start: same iteration count as the oracle, values within 1e-11 relative."""
+    n, s, d = rmat_dense(12, 3)
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    for seeds in seed_sets(n):
+        got, it = ua.ppr(t, seeds, 0.85, max_iter, tol)
+        ref, rit = orc.ppr(o, seeds, 0.85, max_iter, tol)
+        if tol < 2e-6 and max_iter > 16:  # f32 sweeps, then f64 refinement
+            assert np.abs(got - ref).sum() <= 2e-7, seeds
+            continue
+        assert it == rit, (seeds, it, rit)
+        assert np.all(np.abs(got - ref) <= 1e-11 * np.abs(ref)), seeds
+
+
+def test_ppr_max_iter_inside_the_f32_phase_ends_in_f64(ua):
+    """max_iter reached while the f32 phase would still run: the last sweep is an f64
+    one, so the result is an f64 iterate; it carries the f32 phase's error (about 1e-7
+    of the mass, damped by d per f64 sweep), not the f64 solve's."""
+    n, s, d = rmat_dense(12, 3)
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    for mi in (17, 18, 20):
+        got, it = ua.ppr(t, [1, 5, 9], 0.85, mi, 1e-15)
+        ref, rit = orc.ppr(o, [1, 5, 9], 0.85, mi, 1e-15)
+        assert it == rit == mi
+        assert np.abs(got - ref).sum() <= 1e-6 and abs(got.sum() - ref.sum()) <= 1e-7
+
+
 def test_ppr_networkx_fixtures(ua):
     from graphs import FX
 

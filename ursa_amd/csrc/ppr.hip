@@ -13,10 +13,15 @@
 // of vertex v for lane q at v*KP + q (KP = lane count rounded up to a power of
 // two), so one in-edge gathers KP adjacent values for all queries at once.
 //
-// Precision: a lane first runs f32 sweeps with compensated (Neumaier) row
-// sums until its delta is below max(tol, 2e-6) or stops falling, then f64
-// sweeps refine the same iterate until delta < tol. The first f64 sweep reads
-// the f32 iterate widened. Lanes switch and finish independently.
+// Precision: a lane with tol < 2e-6 and max_iter > 16 first runs f32 sweeps
+// with compensated (Neumaier) row sums until its delta is below 2e-6 or stops
+// falling, then f64 sweeps refine the same iterate until delta < tol; the
+// first f64 sweep reads the f32 iterate widened. The f32 phase always leaves
+// at least one f64 sweep before max_iter, so every lane returns an f64
+// iterate. Any other lane (a coarse tol, or a max_iter too small for the f32
+// phase to pay) runs f64 sweeps from the uniform start, the iteration above
+// term for term, so its iteration count and values follow the f64 solve.
+// Lanes switch and finish independently.
 //
 // Work split: a group of PPR_G = 8 adjacent threads sums one row, thread g
 // taking in-slots g, g+8, ... (so the group reads the slot list 32 bytes at a
@@ -49,13 +54,14 @@ constexpr uint32_t PPR_ROWS = 128;           // rows per workgroup (one partial 
 constexpr uint32_t PPR_HEAVY = 256;
 constexpr uint32_t PPR_MAXSEG = 1024;
 constexpr int PPR_BATCH = 4;                 // slots read together per thread (2 at four lanes)
+constexpr double PPR_F32_TOL = 2e-6;         // the f32 phase ends once delta is below this
+constexpr uint32_t PPR_F32_MIN_ITER = 16;    // lanes with max_iter <= this run f64 only
 
 struct PprLane {
     double d, tol, inv_s, pb, prev, delta;
     uint32_t max_iter, iters, f32_iters;
     int32_t phase;    // 0 f32 sweeps, 1 first f64 sweep, 2 f64 sweeps, 3 done
-    int32_t final64;  // result is the f64 iterate (else the f32 one)
-    int32_t pad;
+    int32_t pad[2];
 };
 
 struct PprCtl {
@@ -372,16 +378,11 @@ __global__ __launch_bounds__(PB) void k_ppr_reduce(const double* __restrict__ pa
             L.f32_iters += 1;
             const bool stalled = L.f32_iters >= 3 && delta > 0.7 * L.prev;
             L.prev = delta;
-            if (L.iters >= L.max_iter) {
-                L.phase = 3;
-                L.final64 = 0;
-            } else if (delta < fmax(L.tol, 2e-6) || stalled) {
-                L.phase = 1;
-            }
+            // The last sweep max_iter allows is always an f64 one.
+            if (delta < PPR_F32_TOL || stalled || L.iters + 1 >= L.max_iter) L.phase = 1;
         } else {
             if (delta < L.tol || L.iters >= L.max_iter) {
                 L.phase = 3;
-                L.final64 = 1;
             } else {
                 L.phase = 2;
             }
@@ -392,9 +393,11 @@ __global__ __launch_bounds__(PB) void k_ppr_reduce(const double* __restrict__ pa
     ctl->active = active;
 }
 
+// x0 = 1/n for every lane, with its f32 and f64 contributions (lanes in the
+// f32 phase read c32, lanes that start in f64 read c64).
 template <int KP>
 __global__ void k_ppr_init(uint64_t n, const uint32_t* __restrict__ od, float* __restrict__ r32,
-                           float* __restrict__ c32, double* __restrict__ r64) {
+                           float* __restrict__ c32, double* __restrict__ r64, double* __restrict__ c64) {
     const float x32 = (float)(1.0 / (double)n);
     const double x64 = 1.0 / (double)n;
     for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n; u += (uint64_t)gridDim.x * blockDim.x) {
@@ -404,6 +407,7 @@ __global__ void k_ppr_init(uint64_t n, const uint32_t* __restrict__ od, float* _
             r32[u * KP + q] = x32;
             r64[u * KP + q] = x64;
             c32[u * KP + q] = o ? x32 / (float)o : 0.f;
+            c64[u * KP + q] = o ? x64 / (double)o : 0.0;
         }
     }
 }
@@ -414,12 +418,11 @@ __global__ void k_ppr_seedbits(const uint32_t* __restrict__ seeds, const uint32_
     if (i < count) atomicOr(&bits[seeds[i]], 1u << lane_of[i]);
 }
 
+// Every lane ends on its f64 iterate (the uniform start for a lane that never ran).
 template <int KP>
-__global__ void k_ppr_out(uint64_t n, int q, const PprLane* __restrict__ lanes, const float* __restrict__ r32,
-                          const double* __restrict__ r64, double* __restrict__ out) {
-    const bool f64 = lanes[q].final64 != 0;
+__global__ void k_ppr_out(uint64_t n, int q, const double* __restrict__ r64, double* __restrict__ out) {
     for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n; u += (uint64_t)gridDim.x * blockDim.x)
-        out[u] = f64 ? r64[u * KP + q] : (double)r32[u * KP + q];
+        out[u] = r64[u * KP + q];
 }
 
 int lanes_padded(int k) {
@@ -489,7 +492,7 @@ void ppr_iterate(ursa_hip_topo* t, const DevCsr& in, PprBufs& B, uint32_t max_it
 template <int KP>
 void ppr_write_out(uint64_t n, PprBufs& B, int k, double* const* dst, hipStream_t s) {
     for (int q = 0; q < k; ++q) {
-        k_ppr_out<KP><<<blocks_for(n, 256, 4096), 256, 0, s>>>(n, q, B.lanes.get(), B.r32.get(), B.r64.get(), dst[q]);
+        k_ppr_out<KP><<<blocks_for(n, 256, 4096), 256, 0, s>>>(n, q, B.r64.get(), dst[q]);
         check_launch();
     }
 }
@@ -569,7 +572,6 @@ void ppr_solve(ursa_hip_topo* t, const PprQuery* qs, int k, double* const* outs,
         PprLane& L = lanes[q];
         L = PprLane{};
         L.phase = 3;
-        L.final64 = 1;
         if (q >= k) continue;
         std::vector<uint32_t> u;
         for (uint32_t i = 0; i < qs[q].nseeds; ++i)
@@ -582,8 +584,7 @@ void ppr_solve(ursa_hip_topo* t, const PprQuery* qs, int k, double* const* outs,
         if (u.empty() || qs[q].max_iter == 0) continue;
         L.inv_s = 1.0 / (double)u.size();
         L.pb = (1.0 - L.d) + L.d * ((double)t->ppr_dangling / (double)n);
-        L.phase = 0;
-        L.final64 = 0;
+        L.phase = (L.tol < PPR_F32_TOL && L.max_iter > PPR_F32_MIN_ITER) ? 0 : 2;
         max_iter_all = std::max(max_iter_all, L.max_iter);
         for (uint32_t v : u) {
             seeds.push_back(v);
@@ -627,7 +628,7 @@ void ppr_solve(ursa_hip_topo* t, const PprQuery* qs, int k, double* const* outs,
     auto run = [&](auto kp_tag) {
         constexpr int KPc = decltype(kp_tag)::value;
         k_ppr_init<KPc><<<blocks_for(n, 256, 4096), 256, 0, s>>>(n, t->out_deg.get(), B.r32.get(), B.c32a.get(),
-                                                                 B.r64.get());
+                                                                 B.r64.get(), B.c64a.get());
         check_launch();
         ppr_iterate<KPc>(t, in, B, max_iter_all, s);
         ppr_write_out<KPc>(n, B, k, dst.data(), s);
