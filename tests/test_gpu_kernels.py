@@ -368,6 +368,44 @@ def test_betweenness_sampled_and_batching(ua, rmat12sym):
         assert_bc_close(ua.betweenness_sources(t, src, scale=n / 256, batch=batch), ref, f"batch{batch}")
 
 
+def _lattice(w):
+    """w x w grid, both directions: corner-to-corner path counts are C(2w-2, w-1)."""
+    idx = np.arange(w * w).reshape(w, w)
+    a = np.concatenate([idx[:, :-1].ravel(), idx[:-1, :].ravel()])
+    b = np.concatenate([idx[:, 1:].ravel(), idx[1:, :].ravel()])
+    return w * w, np.concatenate([a, b]), np.concatenate([b, a])
+
+
+def _layered(width, layers):
+    """Layers of `width` vertices, every vertex joined to every vertex of the next
+    layer (one direction): width^(layers-1) shortest paths end to end."""
+    s, d = [], []
+    for L in range(layers - 1):
+        for i in range(width):
+            for j in range(width):
+                s.append(L * width + i)
+                d.append((L + 1) * width + j)
+    return width * layers, np.array(s), np.array(d)
+
+
+@pytest.mark.parametrize("shape", ["lattice64", "layered4x40"])
+def test_betweenness_past_2_53_paths(ua, shape):
+    """Path counts far above 2^53 (C(126, 63) ~ 6e36 corner to corner; 4^39 end to
+    end): sums round, and the ordered recount keeps results bit-identical run to run
+    and within the relative bar of the oracle (betweenness.rs:200-215,
+    ursa-core/tests/determinism_threads.rs:69-94)."""
+    n, s, d = _lattice(64) if shape == "lattice64" else _layered(4, 40)
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    src = np.arange(0, n, 7, dtype=np.uint32)[:256]
+    ref = orc.betweenness_sources(o, src, nthreads=8)
+    for batch in (64, 1):
+        runs = [ua.betweenness_sources(t, src, batch=batch) for _ in range(3)]
+        assert_bc_close(runs[0], ref, f"{shape} batch{batch}")
+        for r in runs[1:]:
+            assert np.array_equal(r.view(np.uint64), runs[0].view(np.uint64)), (shape, batch)
+
+
 def test_betweenness_goldens(ua):  # betweenness.rs:369-439
     t = ua.Topology.build(5, [0, 1, 2, 3], [1, 2, 3, 4])
     assert list(ua.betweenness(t)) == [0.0, 3.0, 4.0, 3.0, 0.0]

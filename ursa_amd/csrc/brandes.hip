@@ -14,7 +14,11 @@
 // L-1 into level L adds sigma[u] to sigma[v] (betweenness.rs:204-214: per CSR
 // slot, so parallel edges count as distinct paths). Path counts are integers
 // held in f64; their sums are exact below 2^53 and so independent of the order
-// of the atomic adds.
+// of the atomic adds. An add whose result reaches 2^53 (the only way one can
+// round) raises a flag, and that level's path counts are then recomputed by an
+// ordered pull: sigma[w] = sum of sigma[u] over w's in-slots u at level L-1, in
+// in-CSR slot order. Results are therefore bit-identical run to run for any
+// path count.
 // Backward (one launch per level, deepest first): each vertex v at level L-1
 // pulls delta[v] = sum over out-slots w at level L of
 // sigma[v] * ((1 + delta[w]) / sigma[w]) — the same per-term arithmetic as
@@ -43,7 +47,16 @@ struct BrCtr {
     uint32_t len;
     uint32_t heavy;
     unsigned long long edges;
+    uint32_t big;  // a path-count sum reached 2^53 this level
+    uint32_t pad;
 };
+
+constexpr double SIGMA_EXACT = 9007199254740992.0;  // 2^53: integer sums below it are exact
+
+// Adds sv to a path count; true when the sum reached 2^53.
+__device__ __forceinline__ bool sigma_add(double* p, double sv) {
+    return unsafeAtomicAdd(p, sv) + sv >= SIGMA_EXACT;
+}
 
 __global__ void k_br_init(int32_t* dist, double* sigma, double* delta, uint64_t total) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
@@ -75,6 +88,7 @@ __global__ __launch_bounds__(BB) void k_br_forward(const unsigned long long* __r
     const uint64_t gw = blockIdx.x * (uint64_t)BW + (threadIdx.x >> 6);
     const uint64_t nwaves = (uint64_t)gridDim.x * BW;
     unsigned long long edges = 0;
+    bool big = false;
     for (uint64_t base = gw * 64; base < qlen; base += nwaves * 64) {
         const uint64_t idx = base + lane;
         uint32_t d = 0, st = 0, b = 0;
@@ -109,7 +123,7 @@ __global__ __launch_bounds__(BB) void k_br_forward(const unsigned long long* __r
                     claimed = old == -1;
                     cur = claimed ? L : old;
                 }
-                if (cur == L) atomicAdd(&sigma[ob * n + v], osu);
+                if (cur == L) big |= sigma_add(&sigma[ob * n + v], osu);
             }
             const uint32_t slot = wave_append_slot(claimed, &ctr->len);
             if (claimed) nq[slot] = ((unsigned long long)ob << 32) | v;
@@ -117,6 +131,26 @@ __global__ __launch_bounds__(BB) void k_br_forward(const unsigned long long* __r
     }
     edges = wave_sum_u64(edges);
     if (lane == 0 && edges) atomicAdd(&ctr->edges, edges);
+    if (__ballot(big) && lane == 0) atomicOr(&ctr->big, 1u);
+}
+
+// Ordered recount of a level's path counts (source-major layout): for every
+// (source b, vertex w) entry of level L, sigma = sum over w's in-slots u with
+// dist[b][u] == L - 1 of sigma[b][u], in in-CSR slot order.
+__global__ void k_br_sigma_pull(const unsigned long long* __restrict__ q, uint32_t qlen,
+                                const uint32_t* __restrict__ in_off, const uint32_t* __restrict__ in_tgt, uint64_t n,
+                                const int32_t* __restrict__ dist, double* __restrict__ sigma, int32_t L) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < qlen; i += (uint64_t)gridDim.x * blockDim.x) {
+        const unsigned long long e = q[i];
+        const uint64_t bo = (e >> 32) * n;
+        const uint32_t w = (uint32_t)e;
+        double acc = 0.0;
+        for (uint32_t k = in_off[w]; k < in_off[w + 1]; ++k) {
+            const uint32_t u = in_tgt[k];
+            if (dist[bo + u] == L - 1) acc += sigma[bo + u];
+        }
+        sigma[bo + w] = acc;
+    }
 }
 
 // delta for the entries of one level; heavy entries are deferred to a wave each.
@@ -200,7 +234,7 @@ constexpr uint32_t VM_HEAVY = 2048;  // vertices with more out-edges are split i
 
 struct VmCtr {
     uint32_t len;
-    uint32_t pad;
+    uint32_t big;  // a path-count sum reached 2^53 this level
     unsigned long long edges;
 };
 
@@ -251,7 +285,8 @@ __device__ __forceinline__ void vm_append_flush(VmAppend& ap, uint32_t* __restri
 __device__ __forceinline__ void vm_expand(uint32_t v, uint32_t e0, uint32_t e1, const uint32_t* __restrict__ tgt,
                                           int32_t* __restrict__ dist, double* __restrict__ sigma, int32_t L,
                                           uint32_t* __restrict__ mark, uint32_t* __restrict__ nq, VmCtr* ctr,
-                                          bool count_edges, uint32_t deg, unsigned long long& edges, VmAppend& ap) {
+                                          bool count_edges, uint32_t deg, unsigned long long& edges, VmAppend& ap,
+                                          bool& big) {
     const int b = lane_id();
     const bool fl = dist[(uint64_t)v * 64 + b] == L - 1;
     const uint64_t fmask = __ballot(fl);
@@ -279,7 +314,7 @@ __device__ __forceinline__ void vm_expand(uint32_t v, uint32_t e0, uint32_t e1, 
                     claimed = old == -1;
                     dw = claimed ? L : old;
                 }
-                if (dw == L) unsafeAtomicAdd(&sigma[(uint64_t)w * 64 + b], sv);
+                if (dw == L) big |= sigma_add(&sigma[(uint64_t)w * 64 + b], sv);
             }
             const uint64_t cm = __ballot(claimed);
             if (cm) {
@@ -302,14 +337,16 @@ __global__ __launch_bounds__(BB) void k_vm_forward(const uint32_t* __restrict__ 
     const uint64_t gw = blockIdx.x * (uint64_t)BW + (threadIdx.x >> 6);
     const uint64_t nwaves = (uint64_t)gridDim.x * BW;
     unsigned long long edges = 0;
+    bool big = false;
     VmAppend ap;
     for (uint64_t i = gw; i < qlen; i += nwaves) {
         const uint32_t v = q[i];
         const uint32_t a = off[v], z = off[v + 1];
         if (z - a > VM_HEAVY) continue;  // chunked by k_vm_forward_heavy
-        vm_expand(v, a, z, tgt, dist, sigma, L, mark, nq, ctr, true, z - a, edges, ap);
+        vm_expand(v, a, z, tgt, dist, sigma, L, mark, nq, ctr, true, z - a, edges, ap, big);
     }
     vm_append_flush(ap, nq, ctr);
+    if (__ballot(big) && lane_id() == 0) atomicOr(&ctr->big, 1u);
     block_add<BB>(edges, &ctr->edges);
 }
 
@@ -337,6 +374,7 @@ __global__ __launch_bounds__(BB) void k_vm_forward_heavy(const uint32_t* __restr
     const uint64_t gw = blockIdx.x * (uint64_t)BW + (threadIdx.x >> 6);
     const uint64_t nwaves = (uint64_t)gridDim.x * BW;
     unsigned long long edges = 0;
+    bool big = false;
     VmAppend ap;
     for (uint64_t c = gw; c < chunks; c += nwaves) {
         const uint32_t h = vm_chunk_owner(cbase, nh, (uint32_t)c);
@@ -344,10 +382,35 @@ __global__ __launch_bounds__(BB) void k_vm_forward_heavy(const uint32_t* __restr
         const uint32_t a = off[v], z = off[v + 1];
         const uint32_t e0 = a + ((uint32_t)c - cbase[h]) * VM_HEAVY;
         const uint32_t e1 = min(z, e0 + VM_HEAVY);
-        vm_expand(v, e0, e1, tgt, dist, sigma, L, mark, nq, ctr, c == cbase[h], z - a, edges, ap);
+        vm_expand(v, e0, e1, tgt, dist, sigma, L, mark, nq, ctr, c == cbase[h], z - a, edges, ap, big);
     }
     vm_append_flush(ap, nq, ctr);
+    if (__ballot(big) && lane_id() == 0) atomicOr(&ctr->big, 1u);
     block_add<BB>(edges, &ctr->edges);
+}
+
+// Ordered recount of a level's path counts (vertex-major layout): one wave per
+// vertex w queued at level L; lane b with dist == L sums sigma over w's
+// in-slots u at level L - 1 in in-CSR slot order.
+__global__ __launch_bounds__(BB) void k_vm_sigma_pull(const uint32_t* __restrict__ q, uint32_t qlen,
+                                                      const uint32_t* __restrict__ in_off,
+                                                      const uint32_t* __restrict__ in_tgt,
+                                                      const int32_t* __restrict__ dist, double* __restrict__ sigma,
+                                                      int32_t L) {
+    const uint64_t gw = blockIdx.x * (uint64_t)BW + (threadIdx.x >> 6);
+    const uint64_t nwaves = (uint64_t)gridDim.x * BW;
+    const int b = lane_id();
+    for (uint64_t i = gw; i < qlen; i += nwaves) {
+        const uint32_t w = q[i];
+        const bool on = dist[(uint64_t)w * 64 + b] == L;
+        if (!__ballot(on)) continue;
+        double acc = 0.0;
+        for (uint32_t k = in_off[w]; k < in_off[w + 1]; ++k) {
+            const uint64_t ui = (uint64_t)in_tgt[k] * 64 + b;
+            if (on && dist[ui] == L - 1) acc += sigma[ui];
+        }
+        if (on) sigma[(uint64_t)w * 64 + b] = acc;
+    }
 }
 
 // delta for lanes at level L: sum over out-slots w at level L+1 of
@@ -537,6 +600,12 @@ static void betweenness_vm(ursa_hip_topo* t, const std::vector<uint32_t>& src, d
                 check_launch();
                 const VmCtr c = read();
                 total_edges += c.edges;
+                if (c.big && c.len) {
+                    const DevCsr& in = t->incoming(s);
+                    k_vm_sigma_pull<<<blocks_for(c.len, BW, GRID_CAP), BB, 0, s>>>(
+                        order.get() + z, c.len, in.off.get(), in.tgt.get(), dist.get(), sigma.get(), L);
+                    check_launch();
+                }
                 lvl.push_back(z + c.len);
             }
             const int32_t levels = (int32_t)lvl.size() - 1;
@@ -647,6 +716,12 @@ void betweenness_sources(ursa_hip_topo* t, const uint32_t* sources_user, uint64_
                     check_launch();
                     const BrCtr c = read();
                     total_edges += c.edges;
+                    if (c.big && c.len) {
+                        const DevCsr& in = t->incoming(s);
+                        k_br_sigma_pull<<<blocks_for(c.len, 256, 1u << 16), 256, 0, s>>>(
+                            order.get() + z, c.len, in.off.get(), in.tgt.get(), n, dist.get(), sigma.get(), L);
+                        check_launch();
+                    }
                     lvl.push_back(z + c.len);
                 }
                 // Backward: levels deepest-1 .. 1 (level-0 sources need no delta).
