@@ -91,6 +91,27 @@ def test_pagerank_weighted_matches_oracle(ua, case):
     assert_pr_close(got, ref, name)
 
 
+def test_pagerank_weighted_rmat16_and_out_hubs(ua):
+    """Weighted PageRank (pagerank.rs:135-169) on R-MAT scale 16 and on a graph whose
+    hubs have tens of thousands of out-edges (out-strength summed by a whole wave):
+    same iteration count as the oracle and 1e-12 relative per node, f32-exact and
+    f64 weights."""
+    n16, s16, d16 = rmat_dense(16, 3)
+    rng = np.random.default_rng(8)
+    hubs = np.concatenate([np.zeros(40000, np.int64), np.ones(25000, np.int64)])
+    sh = np.concatenate([hubs, rng.integers(2, 20000, 60000)])
+    dh = np.concatenate([rng.integers(2, 20000, 65000), rng.integers(0, 20000, 60000)])
+    for name, n, s, d in (("rmat16", n16, s16, d16), ("out_hubs", 20000, sh, dh)):
+        t = ua.Topology.build(n, s, d)
+        o = orc.Topology(n, s, d)
+        w = orc.gen_weights(len(s), 3)
+        for ww in (w, w * (1.0 + 1e-9)):
+            got, it = ua._pagerank(t, ww, ua.PageRankParams(0.85, 100, 1e-10))
+            ref, rit, _ = orc.pagerank(o, 0.85, 100, 1e-10, weights=ww)
+            assert it == rit, (name, it, rit)
+            assert_pr_close(got, ref, name)
+
+
 def test_pagerank_edge_cases(ua):
     t = ua.Topology.build(0, [], [])
     assert ua.pagerank(t).size == 0

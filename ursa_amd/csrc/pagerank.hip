@@ -1,34 +1,65 @@
 // pagerank.hip — pull PageRank over the device in-CSR (pagerank.rs:63-169).
 //
-// One iteration = three launches:
-//   k_pr_tiles   merge-path SpMV: tile t owns 2048 consecutive items of the
-//                merged (row-ends, in-edges) path, so every workgroup does the
-//                same work whatever the degree skew. In-edge columns are read
-//                coalesced, their contributions gathered into LDS, each thread
-//                walks 8 path items, and a wave-level segmented scan joins row
-//                pieces that straddle threads. Rows that end inside the tile
-//                get the fused epilogue at once:
+// Working copy of the in-CSR (pr_prepare, cached on the topology): vertices
+// renumbered so that rows with in-edges come first and, within each part, by
+// decreasing out-degree (ties by id); sources relabelled and each row listing
+// them in ascending new id, so the most gathered contributions sit together
+// at the front of the contribution vector. Rows with in-edges are [0, nz).
+// Beside it: one head bit per in-slot (set on the first slot of each row)
+// and, per chunk of 512 consecutive slots, the row holding the chunk's first
+// slot and three flags (the first slot is a head / some slot is a head /
+// some row ends in the chunk).
+//
+// One f64 iteration = three launches:
+//   k_pr_sweep   one wave per chunk, 8 consecutive slots per thread: two
+//                16-byte column loads and 8 independent contribution gathers
+//                (the first HOT contributions come from a copy in LDS made
+//                once per workgroup), then the slots are summed row by row
+//                along the head bits — no row offsets are read and nothing is
+//                searched. Pieces of a row split across threads are joined by
+//                a segmented scan over the wave. A row whose first and last
+//                slot lie in the chunk is finished at once (fused epilogue):
 //                   next[u]    = base + d * acc                (pagerank.rs:84)
 //                   contrib[u] = next[u] / outdeg[u]           (pagerank.rs:116-120)
 //                   delta     += |next[u] - rank[u]|           (pagerank.rs:44-52)
-//                   dangling  += next[u] if outdeg[u] == 0     (pagerank.rs:78)
-//   k_pr_fixup   rows that began in an earlier tile: their per-tile pieces are
-//                summed in tile order and finished with the same epilogue.
-//   k_pr_reduce  one workgroup sums the per-tile partials in a fixed order,
-//                records delta, derives base = (1-d)/n + d*dangling/n for the
-//                next iteration (pagerank.rs:79) and raises `done` once
-//                delta < tol (pagerank.rs:87-90) or max_iter is reached.
+//                A row begun in an earlier chunk leaves the chunk's leading
+//                piece; a row that continues past the chunk its trailing piece.
+//   k_pr_fixup   rows spanning chunks (the trailing piece of the chunk holding
+//                their head, then the leading pieces in chunk order) and rows
+//                without in-edges (acc = 0) get the same epilogue; the delta
+//                partials are folded in a fixed order.
+//   k_pr_reduce  one workgroup finishes the delta sum in a fixed order,
+//                records it and raises `done` once delta < tol
+//                (pagerank.rs:87-90) or max_iter is reached;
+// then the ordered dangling-mass sum (k_pr_dm_build, k_pr_dm_walk) sets
+// base = (1-d)/n + d*dangling/n for the next iteration (pagerank.rs:76-79).
 // Every reduction has a fixed order, so results are bit-identical run to run.
-// Because `done` lives on the device and every launch returns at once when
-// it is set, the host queues iterations in batches and reads the flag only
-// between batches.
+// `done` lives on the device and every launch returns at once when it is set,
+// so the host queues iterations in batches of 8 (recorded once as a HIP graph
+// and replayed) and reads the flag between batches.
 //
-// HBM traffic per iteration (algorithmic): 12 B per in-edge (u32 column + f64
-// contribution gather) and 32 B per node (row end 4, rank 8, out-degree 4,
-// next 8, contribution 8) — 12m + 32n. The weighted form adds the edge weight
-// (f32 when every weight is exactly representable, else f64) and reads an f64
-// out-strength instead of the u32 out-degree.
+// HBM traffic per f64 iteration (algorithmic): 12 B per in-edge (column +
+// contribution gather) and 32 B per node — 12m + 32n. The weighted form adds
+// the edge weight in slot order (f32 when every weight is exact in f32) and
+// reads an f64 out-strength instead of the u32 out-degree. Weighted terms are
+// w * (rank / strength), the contribution being computed once per node, where
+// the reference computes (w * rank) / strength per edge (pagerank.rs:161-166):
+// the same terms, each within one rounding of the reference's.
+//
+// Mixed precision (ursa_hip_pagerank_mixed) is iterative refinement around
+// the same sweep. f32 sweeps (f32 ranks and contributions, compensated row
+// sums) run from the uniform start until delta < 1e-6 or stops shrinking;
+// an f64 sweep x1 = F(x0) of the widened iterate yields the residual
+// r = x1 - x0, computed exactly as the f64 solve computes it; f32 sweeps then
+// solve the correction e = r + d * (A e + dangling(e) / n) until their own
+// delta is below tol / 4; the corrected iterate x0 + e gets the next f64
+// sweep, and the solve ends after the first f64 sweep whose delta is below
+// tol, returning that sweep's output as the reference returns its newest
+// iterate. Only f64 sweeps decide convergence. A coarse tol (>= 1e-6) or a
+// max_iter of 16 or less runs the f64 solve itself.
+#include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "ordered_sum.hpp"
@@ -40,47 +71,31 @@ namespace ursa {
 
 namespace {
 
-constexpr int PR_BLOCK = 256;
-constexpr int PR_ITEMS = 8;
-constexpr int PR_TILE = PR_BLOCK * PR_ITEMS;
+constexpr int PR_ITEMS = 8;                  // in-slots per thread
+constexpr int PR_CHUNK = 64 * PR_ITEMS;      // in-slots per wave
+constexpr int PR_WORDS = PR_CHUNK / 32;      // head-bit words per chunk
+constexpr int PR_FIX = 256;                  // fix-up workgroup
+constexpr int PR_HOT_BYTES = 64 << 10;       // LDS copy of the hottest contributions
+constexpr int PR_HOT_BLOCK = 512;            // workgroup of the LDS-copy sweep (two per CU)
+constexpr int PR_PLAIN_BLOCK = 256;          // workgroup of the sweep without the copy
+constexpr double PR_F32_TOL = 1e-6;          // the first f32 phase ends below this delta
+constexpr uint32_t PR_F32_MIN_ITER = 16;     // max_iter at or below this: f64 solve
+constexpr uint32_t PR_MAX_CORR = 40;         // correction sweeps per refinement round at most
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 struct PrCtl {
     uint32_t done;
     uint32_t iters;
     double delta;
+    double dm;  // tree sum of the dangling values of the last sweep (mixed solve)
     double base[2];
     int32_t dm_e0;  // lowest candidate binade of the ordered dangling sum
 };
 
-// Split point of the merge path at diagonal `diag`: x row-ends and y in-edges
-// precede it. row_end[i] = in_off[i + 1]; a row end is taken before in-edge j
-// exactly when row_end <= j.
-__device__ __forceinline__ void merge_path_search(uint64_t diag, const uint32_t* row_end, uint64_t n, uint64_t m,
-                                                  uint64_t& x, uint64_t& y) {
-    uint64_t lo = diag > m ? diag - m : 0;
-    uint64_t hi = diag < n ? diag : n;
-    while (lo < hi) {
-        uint64_t mid = (lo + hi) >> 1;
-        if ((uint64_t)row_end[mid] + mid + 1 <= diag)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    x = lo;
-    y = diag - lo;
-}
-
-__global__ void k_pr_partition(const uint32_t* in_off, uint64_t n, uint64_t m, uint64_t tiles, uint32_t* tx,
-                               uint32_t* ty) {
-    uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (t > tiles) return;
-    uint64_t diag = t * (uint64_t)PR_TILE;
-    if (diag > n + m) diag = n + m;
-    uint64_t x, y;
-    merge_path_search(diag, in_off + 1, n, m, x, y);
-    tx[t] = (uint32_t)x;
-    ty[t] = (uint32_t)y;
-}
+struct PrPart {
+    double delta, dm;
+};
 
 // Deterministic sum over a workgroup; the result is valid in thread 0.
 __device__ __forceinline__ double block_sum(double v, double* lds) {
@@ -93,6 +108,16 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += lds[w];
     __syncthreads();
     return tot;
+}
+
+__device__ __forceinline__ uint32_t wave_excl_scan_u32(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+    uint32_t x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(x, off, 64);
+        if (lane >= off) x += t;
+    }
+    return x - v;
 }
 
 // Per-thread running sum of a row piece. f64 contributions add plainly; f32
@@ -129,21 +154,6 @@ struct RowAcc<float> {
     }
 };
 
-// Row epilogue shared by the tile and fix-up kernels. `scale` is the
-// out-degree (unweighted) or out-strength (weighted) of u; scale > 0 means u
-// has out-mass, otherwise u is dangling. The next contribution is stored in
-// the contribution type CT of the following sweep.
-template <class CT>
-__device__ __forceinline__ void pr_finish_row(uint64_t u, double acc, double base, double damping, double scale,
-                                              const double* rank, double* next, CT* contrib_next,
-                                              double& delta) {
-    const double v = __dadd_rn(base, __dmul_rn(damping, acc));  // no contraction: pagerank.rs:84
-    __builtin_nontemporal_store(v, &next[u]);
-    const bool has_out = scale > 0.0;
-    contrib_next[u] = (CT)(has_out ? v / scale : 0.0);
-    delta += fabs(v - __builtin_nontemporal_load(&rank[u]));
-}
-
 template <bool WEIGHTED>
 __device__ __forceinline__ double node_scale(const uint32_t* out_deg, const double* out_str, uint64_t u) {
     if constexpr (WEIGHTED)
@@ -152,195 +162,260 @@ __device__ __forceinline__ double node_scale(const uint32_t* out_deg, const doub
         return (double)__builtin_nontemporal_load(&out_deg[u]);
 }
 
-// LDS slot of tile item i: one pad word per 32 words, so the 8-item stride of
-// the per-thread walk falls into distinct banks.
-template <class CT>
-__device__ __forceinline__ uint32_t lds_slot(uint32_t i) {
-    return sizeof(CT) == 8 ? i + (i >> 4) : i + (i >> 5);
+// Row epilogue shared by the sweep and fix-up kernels: u's new value from its
+// row sum, its next contribution, and its share of delta (and, with DM, of the
+// dangling mass). `scale` > 0 means u has out-mass. `b` is the constant term
+// (base, plus the residual in a correction sweep). RT is the rank type, CT the
+// contribution type of the following sweep.
+template <class RT, class CT, bool DM>
+__device__ __forceinline__ void pr_finish(uint64_t u, double acc, double b, double damping, double scale, double cur,
+                                          RT* next, CT* cnext, double& delta, double& dm) {
+    double v = __dadd_rn(b, __dmul_rn(damping, acc));  // no contraction: pagerank.rs:84
+    if constexpr (sizeof(RT) == 4) v = (double)(float)v;
+    __builtin_nontemporal_store((RT)v, &next[u]);
+    const bool has_out = scale > 0.0;
+    cnext[u] = (CT)(has_out ? v / scale : 0.0);
+    delta += fabs(v - cur);
+    if constexpr (DM)
+        if (!has_out) dm += v;
 }
 
-template <class CT, class W, bool WEIGHTED>
-__global__ __launch_bounds__(PR_BLOCK) void k_pr_tiles(
-    const uint32_t* __restrict__ in_off, const uint32_t* __restrict__ in_tgt, const W* __restrict__ w_in,
-    const uint32_t* __restrict__ tile_x, const uint32_t* __restrict__ tile_y, uint64_t n,
-    const uint32_t* __restrict__ out_deg, const double* __restrict__ out_str, const double* __restrict__ rank,
-    const CT* __restrict__ contrib, double* __restrict__ next, CT* __restrict__ contrib_next,
-    double* __restrict__ carry_val, double* __restrict__ tail_val, double* __restrict__ tile_delta,
-    PrCtl* ctl, int parity, double damping) {
-    __shared__ CT s_val[PR_TILE + PR_TILE / 16];
-    __shared__ uint32_t s_end[PR_TILE + 1];
-    __shared__ double s_wv[PR_BLOCK / 64];
-    __shared__ int s_wf[PR_BLOCK / 64];
-    __shared__ double s_red[PR_BLOCK / 64];
-
+// One sweep: `rank`/`contrib` are the current iterate, `next`/`cnext` the new
+// one. base = ctl->base[parity] (device-driven solve, parity >= 0) or base_arg.
+// With CORR the constant term of row u is base + corr[u] (correction sweeps of
+// the mixed solve). HOT > 0: contributions [0, HOT) are read from an LDS copy.
+template <class CT, class RT, class W, bool WEIGHTED, bool CORR, bool DM, int BLOCK, int HOT>
+__global__ __launch_bounds__(BLOCK) void k_pr_sweep(
+    const uint32_t* __restrict__ tgt, const W* __restrict__ w_in, const uint32_t* __restrict__ head,
+    const uint32_t* __restrict__ crow, uint64_t m, uint64_t nchunks, const uint32_t* __restrict__ out_deg,
+    const double* __restrict__ out_str, const RT* __restrict__ rank, const CT* __restrict__ contrib,
+    RT* __restrict__ next, CT* __restrict__ cnext, const float* __restrict__ corr, double* __restrict__ lead_val,
+    double* __restrict__ tail_val, PrPart* __restrict__ part, const PrCtl* ctl, int parity, double base_arg,
+    double damping) {
+    constexpr int WAVES = BLOCK / 64;
+    __shared__ double s_red[2][WAVES];
+    __shared__ CT s_hot[HOT > 0 ? HOT : 1];
     if (ctl->done) return;
-    const uint64_t t = blockIdx.x;
-    const uint64_t x0 = tile_x[t], y0 = tile_y[t];
-    const uint64_t x1 = tile_x[t + 1], y1 = tile_y[t + 1];
-    const uint64_t rows = x1 - x0, nnz = y1 - y0, len = rows + nnz;
-    const uint64_t n_end = (x1 < n ? x1 : n - 1) - x0 + 1;  // row ends staged (incl. trailing row)
-    const double base = ctl->base[parity];
-
-    {
-        // All of this thread's columns first, then all its gathers: PR_ITEMS
-        // independent gathers in flight per thread instead of one at a time.
-        uint32_t col[PR_ITEMS];
-#pragma unroll
-        for (int g = 0; g < PR_ITEMS; ++g) {
-            const uint64_t i = threadIdx.x + (uint64_t)g * PR_BLOCK;
-            col[g] = i < nnz ? __builtin_nontemporal_load(&in_tgt[y0 + i]) : 0u;
-        }
-        CT cv[PR_ITEMS];
-#pragma unroll
-        for (int g = 0; g < PR_ITEMS; ++g) {
-            const uint64_t i = threadIdx.x + (uint64_t)g * PR_BLOCK;
-            if (i < nnz) cv[g] = contrib[col[g]];
-        }
-#pragma unroll
-        for (int g = 0; g < PR_ITEMS; ++g) {
-            const uint64_t i = threadIdx.x + (uint64_t)g * PR_BLOCK;
-            if (i < nnz) {
-                CT c = cv[g];
-                if constexpr (WEIGHTED) c = (CT)__builtin_nontemporal_load(&w_in[y0 + i]) * c;
-                s_val[lds_slot<CT>((uint32_t)i)] = c;
-            }
-        }
+    const double base = parity >= 0 ? ctl->base[parity] : base_arg;
+    if constexpr (HOT > 0) {
+        for (uint32_t i = threadIdx.x; i < (uint32_t)HOT; i += BLOCK) s_hot[i] = contrib[i];
+        __syncthreads();
     }
-    if (x0 < n)
-        for (uint64_t i = threadIdx.x; i < n_end; i += PR_BLOCK) s_end[i] = __builtin_nontemporal_load(&in_off[x0 + 1 + i]);
-    const bool starts_mid = (x0 < n) && ((uint64_t)in_off[x0] < y0);
-    __syncthreads();
-
-    // Thread's start on the tile-local path.
-    const uint64_t d0 = (uint64_t)threadIdx.x * PR_ITEMS;
-    uint64_t x = x0, y = y0;
-    if (d0 < len) {
-        uint64_t lo = d0 > nnz ? d0 - nnz : 0, hi = d0 < rows ? d0 : rows;
-        while (lo < hi) {
-            uint64_t mid = (lo + hi) >> 1;
-            if ((uint64_t)s_end[mid] + mid + 1 <= y0 + d0)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        x = x0 + lo;
-        y = y0 + d0 - lo;
-    }
-
-    RowAcc<CT> sum;
-    double ev[PR_ITEMS];
-    uint32_t er[PR_ITEMS];
-    uint32_t emask = 0;
-#pragma unroll
-    for (int it = 0; it < PR_ITEMS; ++it) {
-        ev[it] = 0.0;
-        er[it] = 0;
-        if (d0 + it < len) {
-            if (x < n && y < (uint64_t)s_end[x - x0]) {
-                sum.add(s_val[lds_slot<CT>((uint32_t)(y - y0))]);
-                ++y;
-            } else {
-                er[it] = (uint32_t)x;
-                ev[it] = sum.value();
-                emask |= 1u << it;
-                sum.reset();
-                ++x;
-            }
-        }
-    }
-
-    // Segmented inclusive scan of (sum, emitted) across the workgroup: a
-    // thread's incoming carry is the piece of its first row accumulated by
-    // the preceding threads since their last row end.
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double v = sum.value();
-    int f = emask != 0;
-    for (int off = 1; off < 64; off <<= 1) {
-        double vu = __shfl_up(v, off, 64);
-        int fu = __shfl_up(f, off, 64);
-        if (lane >= off) {
-            if (!f) v = vu + v;
-            f |= fu;
+    double delta = 0.0, dm = 0.0;
+    const uint64_t ngroups = (nchunks + WAVES - 1) / WAVES;
+    for (uint64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        const uint64_t c = g * WAVES + wave;
+        if (c >= nchunks) break;  // wave-uniform; later groups lie further out
+        const uint64_t i0 = c * PR_CHUNK + (uint64_t)lane * PR_ITEMS;
+        const int cnt = i0 >= m ? 0 : (int)(m - i0 < PR_ITEMS ? m - i0 : PR_ITEMS);
+
+        uint32_t col[PR_ITEMS];
+        if (cnt == PR_ITEMS) {
+            const u32x4* p = reinterpret_cast<const u32x4*>(tgt + i0);
+            const u32x4 a = __builtin_nontemporal_load(p), b = __builtin_nontemporal_load(p + 1);
+            col[0] = a.x; col[1] = a.y; col[2] = a.z; col[3] = a.w;
+            col[4] = b.x; col[5] = b.y; col[6] = b.z; col[7] = b.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < PR_ITEMS; ++j) col[j] = j < cnt ? tgt[i0 + j] : 0u;
         }
+        CT v[PR_ITEMS];
+#pragma unroll
+        for (int j = 0; j < PR_ITEMS; ++j) {
+            CT x = (CT)0;
+            if (j < cnt) {
+                if constexpr (HOT > 0) {
+                    if (col[j] < (uint32_t)HOT)
+                        x = s_hot[col[j]];
+                    else
+                        x = contrib[col[j]];
+                } else {
+                    x = contrib[col[j]];
+                }
+            }
+            v[j] = x;
+        }
+        if constexpr (WEIGHTED) {
+#pragma unroll
+            for (int j = 0; j < PR_ITEMS; ++j)
+                if (j < cnt) v[j] = (CT)__builtin_nontemporal_load(&w_in[i0 + j]) * v[j];
+        }
+
+        // Head bits of this thread's slots; a thread past the last slot acts
+        // as the start of a row (the end of the slot array).
+        uint32_t hb = 1u;
+        if (cnt > 0) hb = (head[i0 >> 5] >> (uint32_t)(i0 & 31)) & 0xFFu;
+        uint32_t nxt = __shfl_down(hb, 1, 64) & 1u;  // is slot i0 + 8 a row start?
+        if (lane == 63) {
+            const uint64_t i8 = (c + 1) * PR_CHUNK;
+            nxt = i8 >= m ? 1u : (head[i8 >> 5] & 1u);
+        }
+        // end bit j: slot j is the last slot of its row
+        uint32_t E = (hb >> 1) | (nxt << 7);
+        if (cnt < PR_ITEMS) E = cnt ? ((E & ((1u << cnt) - 1u)) | (1u << (cnt - 1))) : 0u;
+        // row(j) = rbase + popcount(hb & ((2 << j) - 1))
+        const uint32_t h0c = __shfl(hb, 0, 64) & 1u;
+        const uint32_t rbase = crow[c] - h0c + wave_excl_scan_u32((uint32_t)__popc(hb));
+
+        RowAcc<CT> acc;
+        double ev[PR_ITEMS];
+        uint32_t emask = 0;
+        double lead = 0.0;
+        int lead_j = -1;
+        bool open_first = (hb & 1u) == 0u;  // slot 0 continues a row begun before this thread
+#pragma unroll
+        for (int j = 0; j < PR_ITEMS; ++j) {
+            ev[j] = 0.0;
+            if (j < cnt) {
+                acc.add(v[j]);
+                if ((E >> j) & 1u) {
+                    const double sv = acc.value();
+                    acc.reset();
+                    if (open_first) {
+                        lead = sv;
+                        lead_j = j;
+                        open_first = false;
+                    } else {
+                        ev[j] = sv;
+                        emask |= 1u << j;
+                    }
+                }
+            }
+        }
+
+        // Segmented scan over the wave of (open trailing piece, has a row start):
+        // a thread's carry is the piece of its first row summed by the preceding
+        // threads since that row's start.
+        double sv = acc.value();
+        uint32_t f = hb != 0u;
+        for (int off = 1; off < 64; off <<= 1) {
+            const double vu = __shfl_up(sv, off, 64);
+            const uint32_t fu = __shfl_up(f, off, 64);
+            if (lane >= off) {
+                if (!f) sv = vu + sv;
+                f |= fu;
+            }
+        }
+        double cin = __shfl_up(sv, 1, 64);
+        uint32_t fin = __shfl_up(f, 1, 64);
+        if (lane == 0) {
+            cin = 0.0;
+            fin = 0u;
+        }
+        if (lead_j >= 0) {
+            const double val = cin + lead;
+            if (fin) {
+                ev[lead_j] = val;
+                emask |= 1u << lead_j;
+            } else {
+                lead_val[c] = val;  // the row began in an earlier chunk: k_pr_fixup finishes it
+            }
+        }
+        if (lane == 63) {
+            if (!f) {
+                if (lead_j < 0) lead_val[c] = sv;  // the whole chunk lies inside one row
+            } else if (!nxt) {
+                tail_val[c] = sv;  // the last row continues into the next chunk
+            }
+        }
+
+        // Epilogue of the rows finished here: their loads first, then the stores.
+        double sc[PR_ITEMS], cur[PR_ITEMS], bb[PR_ITEMS];
+        uint32_t er[PR_ITEMS];
+#pragma unroll
+        for (int j = 0; j < PR_ITEMS; ++j) {
+            er[j] = rbase + (uint32_t)__popc(hb & ((2u << j) - 1u));
+            if ((emask >> j) & 1u) {
+                sc[j] = node_scale<WEIGHTED>(out_deg, out_str, er[j]);
+                cur[j] = (double)__builtin_nontemporal_load(&rank[er[j]]);
+                bb[j] = base;
+                if constexpr (CORR) bb[j] = base + (double)corr[er[j]];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < PR_ITEMS; ++j)
+            if ((emask >> j) & 1u) pr_finish<RT, CT, DM>(er[j], ev[j], bb[j], damping, sc[j], cur[j], next, cnext, delta, dm);
     }
-    if (lane == 63) {
-        s_wv[wave] = v;
-        s_wf[wave] = f;
+    for (int off = 32; off > 0; off >>= 1) {
+        delta += __shfl_down(delta, off, 64);
+        dm += __shfl_down(dm, off, 64);
+    }
+    if (lane == 0) {
+        s_red[0][wave] = delta;
+        s_red[1][wave] = dm;
     }
     __syncthreads();
-    double pv = 0.0;
-    for (int w = 0; w < wave; ++w) pv = s_wf[w] ? s_wv[w] : pv + s_wv[w];
-    const double v_incl = f ? v : pv + v;
-    double c_in = __shfl_up(v_incl, 1, 64);
-    if (lane == 0) c_in = pv;
-    if (threadIdx.x == 0) c_in = 0.0;
-    if (threadIdx.x == PR_BLOCK - 1) carry_val[t] = (x1 < n) ? v_incl : 0.0;
-
-    double delta = 0.0;
-    bool first = true;
-#pragma unroll
-    for (int it = 0; it < PR_ITEMS; ++it) {
-        if (emask & (1u << it)) {
-            double val = ev[it];
-            if (first) {
-                val = c_in + val;
-                first = false;
-            }
-            const uint64_t u = er[it];
-            if (u == x0 && starts_mid) {
-                tail_val[t] = val;
-            } else {
-                pr_finish_row(u, val, base, damping, node_scale<WEIGHTED>(out_deg, out_str, u), rank, next,
-                              contrib_next, delta);
-            }
+    if (threadIdx.x == 0) {
+        PrPart r{0.0, 0.0};
+        for (int w = 0; w < WAVES; ++w) {
+            r.delta += s_red[0][w];
+            r.dm += s_red[1][w];
         }
+        part[blockIdx.x] = r;
     }
-    const double bd = block_sum(delta, s_red);
-    if (threadIdx.x == 0) tile_delta[t] = bd;
 }
 
-// Finishes rows that began in an earlier tile, then folds the per-tile delta
-// partials of 256 consecutive tiles (in tile order) into one partial per
-// workgroup, so the final reduce reads few values.
-template <class CT, bool WEIGHTED>
-__global__ __launch_bounds__(256) void k_pr_fixup(
-    const uint32_t* __restrict__ in_off, const uint32_t* __restrict__ tile_x, const uint32_t* __restrict__ tile_y,
-    uint64_t tiles, uint64_t n, const uint32_t* __restrict__ out_deg, const double* __restrict__ out_str,
-    const double* __restrict__ rank, double* __restrict__ next, CT* __restrict__ contrib_next,
-    const double* __restrict__ carry_val, const double* __restrict__ tail_val, const double* __restrict__ tile_delta,
-    double* __restrict__ blk_delta, const PrCtl* ctl, int parity, double damping) {
-    __shared__ double s_red[256 / 64];
+// Rows spanning chunks and rows without in-edges, then the sweep's partials
+// folded per workgroup in index order.
+template <class CT, class RT, bool WEIGHTED, bool CORR, bool DM>
+__global__ __launch_bounds__(PR_FIX) void k_pr_fixup(
+    const uint8_t* __restrict__ cflag, const uint32_t* __restrict__ crow, uint64_t nchunks, uint64_t nz, uint64_t n,
+    const uint32_t* __restrict__ out_deg, const double* __restrict__ out_str, const RT* __restrict__ rank,
+    RT* __restrict__ next, CT* __restrict__ cnext, const float* __restrict__ corr,
+    const double* __restrict__ lead_val, const double* __restrict__ tail_val, const PrPart* __restrict__ part,
+    uint64_t nparts, PrPart* __restrict__ fpart, const PrCtl* ctl, int parity, double base_arg, double damping) {
+    __shared__ double s_red[PR_FIX / 64];
     if (ctl->done) return;
-    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    double delta = 0.0;
-    if (t < tiles) {
-        const uint64_t x0 = tile_x[t], y0 = tile_y[t], x1 = tile_x[t + 1];
-        if (x0 < n && x1 > x0 && (uint64_t)in_off[x0] < y0) {
-            // Tiles j..t-1 all end inside row x0; tile t holds its end.
-            uint64_t j = t - 1;
-            while (j > 0 && tile_x[j] == x0) --j;
-            double acc = 0.0;
-            for (uint64_t q = j; q < t; ++q) acc += carry_val[q];
-            acc += tail_val[t];
-            pr_finish_row(x0, acc, ctl->base[parity], damping, node_scale<WEIGHTED>(out_deg, out_str, x0), rank,
-                          next, contrib_next, delta);
+    const double base = parity >= 0 ? ctl->base[parity] : base_arg;
+    const uint64_t i = blockIdx.x * (uint64_t)PR_FIX + threadIdx.x;
+    double delta = 0.0, dm = 0.0;
+    if (i < nparts) {
+        delta = part[i].delta;
+        dm = part[i].dm;
+    }
+    uint64_t u = n;
+    double acc = 0.0;
+    if (i < nchunks) {
+        const uint32_t f = cflag[i];
+        if (!(f & 1u) && (f & 4u)) {
+            // The row holding this chunk's first slot ends here and began in the
+            // last earlier chunk with a row start; chunks between lie inside it.
+            uint64_t j = i - 1;
+            while (!(cflag[j] & 2u)) --j;
+            acc = tail_val[j];
+            for (uint64_t q = j + 1; q <= i; ++q) acc += lead_val[q];
+            u = crow[i];
         }
-        delta = tile_delta[t] + delta;
+    } else if (i < nchunks + (n - nz)) {
+        u = nz + (i - nchunks);  // no in-edges: acc = 0 (pagerank.rs:113-123)
+    }
+    if (u < n) {
+        double b = base;
+        if constexpr (CORR) b = base + (double)corr[u];
+        pr_finish<RT, CT, DM>(u, acc, b, damping, node_scale<WEIGHTED>(out_deg, out_str, u), (double)rank[u], next, cnext,
+                              delta, dm);
     }
     const double bd = block_sum(delta, s_red);
-    if (threadIdx.x == 0) blk_delta[blockIdx.x] = bd;
+    const double bm = block_sum(dm, s_red);
+    if (threadIdx.x == 0) fpart[blockIdx.x] = PrPart{bd, bm};
 }
 
-__global__ __launch_bounds__(1024) void k_pr_reduce(const double* __restrict__ blk_delta, uint64_t count, PrCtl* ctl,
+__global__ __launch_bounds__(1024) void k_pr_reduce(const PrPart* __restrict__ fpart, uint64_t count, PrCtl* ctl,
                                                     uint32_t max_iter, double tol) {
     __shared__ double s_red[1024 / 64];
     if (ctl->done) return;
-    double d = 0.0;
-    for (uint64_t i = threadIdx.x; i < count; i += blockDim.x) d += blk_delta[i];
+    double d = 0.0, x = 0.0;
+    for (uint64_t i = threadIdx.x; i < count; i += blockDim.x) {
+        d += fpart[i].delta;
+        x += fpart[i].dm;
+    }
     const double delta = block_sum(d, s_red);
+    const double dmt = block_sum(x, s_red);
     if (threadIdx.x == 0) {
         const uint32_t iters = ctl->iters + 1;  // the iteration count lives on the device
         ctl->delta = delta;
+        ctl->dm = dmt;
         ctl->iters = iters;
         if (delta < tol || iters >= max_iter) ctl->done = 1;
     }
@@ -376,17 +451,16 @@ __global__ __launch_bounds__(OS_WALK_THREADS) void k_pr_dm_walk(const double* __
     }
 }
 
-// rank0 = 1/n, contrib0 = rank0 / scale.
-template <class CT, bool WEIGHTED>
+// rank0 = 1/n, contrib0 = rank0 / scale (in the precision of RT / CT).
+template <class RT, class CT, bool WEIGHTED>
 __global__ void k_pr_init(uint64_t n, double nf, const uint32_t* __restrict__ out_deg,
-                          const double* __restrict__ out_str, double* __restrict__ rank,
-                          CT* __restrict__ contrib) {
+                          const double* __restrict__ out_str, RT* __restrict__ rank, CT* __restrict__ contrib) {
     const uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (u < n) {
-        const double r = 1.0 / nf;
+        const RT r = (RT)(1.0 / nf);
         rank[u] = r;
         const double sc = node_scale<WEIGHTED>(out_deg, out_str, u);
-        contrib[u] = (CT)(sc > 0.0 ? r / sc : 0.0);
+        contrib[u] = (CT)(sc > 0.0 ? (double)r / sc : 0.0);
     }
 }
 
@@ -404,23 +478,57 @@ __global__ void k_pr_ctl_init(PrCtl* ctl, int32_t dm_e0, double base0) {
     ctl->done = 0;
     ctl->iters = 0;
     ctl->delta = 0.0;
+    ctl->dm = 0.0;
     ctl->base[0] = base0;
     ctl->base[1] = 0.0;
     ctl->dm_e0 = dm_e0;
 }
 
-// out_str[v] = sum of v's out-edge weights in out-CSR order (pagerank.rs:148-150).
+// out_str[v] = sum of v's out-edge weights (pagerank.rs:148-150): in out-CSR
+// order for vertices with at most OS_LIGHT out-edges; a wave per heavier
+// vertex (k_out_strength_heavy) sums lane-strided pieces in slot order and
+// joins them in a fixed butterfly, so a hub's sum is not one thread's serial
+// walk. The order differs from the reference's left-to-right sum only for
+// heavy vertices (same terms, within a few ulps).
+constexpr uint32_t OS_LIGHT = 64;
+
 __global__ void k_out_strength(const uint32_t* __restrict__ out_off, const uint32_t* __restrict__ out_eid,
                                const double* __restrict__ w, uint64_t n, double* __restrict__ out_str) {
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n;
          v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t a = out_off[v], b = out_off[v + 1];
+        if (b - a > OS_LIGHT) continue;
         double s = -0.0;
-        for (uint32_t k = out_off[v]; k < out_off[v + 1]; ++k) s += w[out_eid[k]];
+        for (uint32_t k = a; k < b; ++k) s += w[out_eid[k]];
         out_str[v] = s;
     }
 }
 
-// Weights permuted into in-CSR slot order; flags any weight not exact in f32.
+__global__ __launch_bounds__(256) void k_out_strength_heavy(const uint32_t* __restrict__ out_off,
+                                                            const uint32_t* __restrict__ out_eid,
+                                                            const double* __restrict__ w, uint64_t n,
+                                                            double* __restrict__ out_str) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t gw = blockIdx.x * (uint64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x >> 6);
+    for (uint64_t v0 = gw * 64; v0 < n; v0 += nw * 64) {
+        const uint64_t mine = v0 + lane;
+        const bool heavy = mine < n && out_off[mine + 1] - out_off[mine] > OS_LIGHT;
+        uint64_t mask = __ballot(heavy);
+        while (mask) {
+            const int j = __ffsll((unsigned long long)mask) - 1;
+            mask &= mask - 1;
+            const uint64_t v = v0 + j;
+            const uint32_t a = out_off[v], b = out_off[v + 1];
+            double s = -0.0;
+            for (uint32_t k = a + lane; k < b; k += 64) s += w[out_eid[k]];
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+            if (lane == 0) out_str[v] = s;
+        }
+    }
+}
+
+// Weights permuted into slot order; flags any weight not exact in f32.
 __global__ void k_permute_weights(const uint32_t* __restrict__ in_eid, const double* __restrict__ w, uint64_t m,
                                   double* __restrict__ w64, float* __restrict__ w32, uint32_t* inexact) {
     for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < m;
@@ -433,21 +541,59 @@ __global__ void k_permute_weights(const uint32_t* __restrict__ in_eid, const dou
     }
 }
 
-// contrib[u] = rank[u] / scale (f64), used when the mixed-precision solve
-// hands over from its f32 sweeps to its f64 sweeps.
+// Mixed solve: x64 = widened f32 iterate, with its f64 contributions.
 template <bool WEIGHTED>
-__global__ void k_pr_contrib64(uint64_t n, const uint32_t* __restrict__ out_deg, const double* __restrict__ out_str,
-                               const double* __restrict__ rank, double* __restrict__ contrib) {
+__global__ void k_pr_widen(uint64_t n, const uint32_t* __restrict__ out_deg, const double* __restrict__ out_str,
+                           const float* __restrict__ x32, double* __restrict__ x64, double* __restrict__ c64) {
     for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n; u += (uint64_t)gridDim.x * blockDim.x) {
+        const double x = (double)x32[u];
         const double sc = node_scale<WEIGHTED>(out_deg, out_str, u);
-        contrib[u] = sc > 0.0 ? rank[u] / sc : 0.0;
+        x64[u] = x;
+        c64[u] = sc > 0.0 ? x / sc : 0.0;
     }
 }
 
-// ---- renumbering by decreasing out-degree ----
-__global__ void k_rl_keys(const uint32_t* __restrict__ out_deg, uint64_t n, uint32_t* keys, uint32_t* iota) {
+// Mixed solve: residual r = x1 - x0 of an f64 sweep (rounded to f32), which is
+// also the correction solve's start e0 = r; per-workgroup dangling sums of r.
+template <bool WEIGHTED>
+__global__ __launch_bounds__(PR_FIX) void k_pr_residual(uint64_t n, const uint32_t* __restrict__ out_deg,
+                                                        const double* __restrict__ out_str,
+                                                        const double* __restrict__ x1, const double* __restrict__ x0,
+                                                        float* __restrict__ r32, float* __restrict__ e32,
+                                                        float* __restrict__ ce32, PrPart* __restrict__ fpart) {
+    __shared__ double s_red[PR_FIX / 64];
+    const uint64_t u = blockIdx.x * (uint64_t)PR_FIX + threadIdx.x;
+    double dm = 0.0;
+    if (u < n) {
+        const float r = (float)(x1[u] - x0[u]);
+        const double sc = node_scale<WEIGHTED>(out_deg, out_str, u);
+        r32[u] = r;
+        e32[u] = r;
+        ce32[u] = sc > 0.0 ? (float)((double)r / sc) : 0.f;
+        if (!(sc > 0.0)) dm = (double)r;
+    }
+    const double bm = block_sum(dm, s_red);
+    if (threadIdx.x == 0) fpart[blockIdx.x] = PrPart{0.0, bm};
+}
+
+// Mixed solve: x0 += e, with the f64 contributions of the corrected iterate.
+template <bool WEIGHTED>
+__global__ void k_pr_apply(uint64_t n, const uint32_t* __restrict__ out_deg, const double* __restrict__ out_str,
+                           const float* __restrict__ e32, double* __restrict__ x64, double* __restrict__ c64) {
+    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n; u += (uint64_t)gridDim.x * blockDim.x) {
+        const double x = x64[u] + (double)e32[u];
+        const double sc = node_scale<WEIGHTED>(out_deg, out_str, u);
+        x64[u] = x;
+        c64[u] = sc > 0.0 ? x / sc : 0.0;
+    }
+}
+
+// ---- renumbering: rows with in-edges first, then by decreasing out-degree ----
+__global__ void k_rl_keys(const uint32_t* __restrict__ out_deg, const uint32_t* __restrict__ in_off, uint64_t n,
+                          uint64_t* keys, uint32_t* iota) {
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
-        keys[v] = ~out_deg[v];
+        const uint64_t no_in = in_off[v + 1] == in_off[v] ? 1u : 0u;
+        keys[v] = no_in << 32 | (uint32_t)~out_deg[v];
         iota[v] = (uint32_t)v;
     }
 }
@@ -461,6 +607,12 @@ __global__ void k_rl_invert(const uint32_t* __restrict__ old_of_new, uint64_t n,
         rdeg[i] = in_off[o + 1] - in_off[o];
         routdeg[i] = out_deg[o];
     }
+}
+
+// nz = number of rows with in-edges (they come first in the new numbering).
+__global__ void k_rl_nz(const uint32_t* __restrict__ rdeg, uint64_t n, uint64_t* nz) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        if (rdeg[i] > 0 && (i + 1 == n || rdeg[i + 1] == 0)) *nz = i + 1;
 }
 
 // Slot k of row u becomes the key (new(u), new(source)) with its edge id
@@ -478,6 +630,44 @@ __global__ void k_rl_low_words(const uint64_t* __restrict__ key, uint64_t m, uin
         r_tgt[k] = (uint32_t)key[k];
 }
 
+// Head bit of the first slot of every row with in-edges.
+__global__ void k_pr_heads(const uint32_t* __restrict__ off, uint64_t nz, uint32_t* __restrict__ head) {
+    for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < nz; r += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = off[r];
+        atomicOr(&head[k >> 5], 1u << (k & 31));
+    }
+}
+
+// Per chunk: the row holding its first slot (largest r < nz with off[r] <= slot)
+// and its flags (bit 0 first slot starts a row, bit 1 some slot starts a row,
+// bit 2 some row ends in the chunk, i.e. a row starts in (first, first + 512]
+// with the end of the slot array counting as a start).
+__global__ void k_pr_chunks(const uint32_t* __restrict__ off, uint64_t nz, const uint32_t* __restrict__ head,
+                            uint64_t m, uint64_t nchunks, uint32_t* __restrict__ crow, uint8_t* __restrict__ cflag) {
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < nchunks;
+         c += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t slot = c * PR_CHUNK;
+        uint64_t lo = 0, hi = nz;  // off[lo] <= slot < off[hi]
+        while (hi - lo > 1) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (off[mid] <= slot)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        crow[c] = (uint32_t)lo;
+        uint32_t any = 0, later = 0;
+        for (int w = 0; w < PR_WORDS; ++w) {
+            const uint32_t x = head[c * PR_WORDS + w];
+            any |= x;
+            later |= w == 0 ? (x & ~1u) : x;
+        }
+        const uint64_t nxt = slot + PR_CHUNK;
+        const uint32_t next_start = nxt >= m ? 1u : (head[nxt >> 5] & 1u);
+        cflag[c] = (uint8_t)((head[c * PR_WORDS] & 1u) | (any ? 2u : 0u) | ((later || next_start) ? 4u : 0u));
+    }
+}
+
 __global__ void k_gather_f64(const double* __restrict__ src, const uint32_t* __restrict__ idx, uint64_t n,
                              double* __restrict__ out) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -489,16 +679,21 @@ __global__ void k_fill(double* p, uint64_t n, double v) {
         p[i] = v;
 }
 
+unsigned cu_count() {
+    static int cus = [] {
+        int dev = 0, v = 0;
+        HIP_OK(hipGetDevice(&dev));
+        HIP_OK(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev));
+        return v > 0 ? v : 256;
+    }();
+    return (unsigned)cus;
+}
+
 }  // namespace
 
-// PageRank's working copy of the in-CSR, built once per topology: vertices
-// renumbered by decreasing out-degree (ties by id), sources relabelled, and
-// each row's sources listed in ascending new id (parallel edges keep their
-// slot order). The most-gathered contributions (high out-degree sources) then
-// sit in a few megabytes at the front of the contribution vector and stay in
-// L2, instead of being spread over it. A row's sum runs in this order, not the
-// in-CSR's: the same terms, rounded in another order.
-// Also caches the merge-path tile partition of that CSR.
+// PageRank's working copy of the in-CSR, built once per topology (layout at
+// the top of this file). A row's sum runs in ascending new source id, not in
+// the in-CSR's order: the same terms, rounded in another order.
 static void pr_prepare(ursa_hip_topo* t, const DevCsr& in, hipStream_t s) {
     NoArena keep_out_of_arena;
     std::lock_guard<std::mutex> lk(t->cache_mu);
@@ -510,15 +705,22 @@ static void pr_prepare(ursa_hip_topo* t, const DevCsr& in, hipStream_t s) {
     t->pr_tgt.alloc(m ? m : 1);
     t->pr_eid.alloc(m ? m : 1);
     t->pr_outdeg.alloc(n);
+    uint64_t nz = 0;
     {
-        DevBuf<uint32_t> keys(n), iota(n), skeys(n), rdeg(n);
-        k_rl_keys<<<blocks_for(n, 256), 256, 0, s>>>(t->out_deg.get(), n, keys.get(), iota.get());
+        DevBuf<uint64_t> keys(n), skeys(n), dnz(1);
+        DevBuf<uint32_t> iota(n), rdeg(n);
+        k_rl_keys<<<blocks_for(n, 256), 256, 0, s>>>(t->out_deg.get(), in.off.get(), n, keys.get(), iota.get());
         check_launch();
-        prim::sort_pairs_u32(keys.get(), skeys.get(), iota.get(), t->pr_old_of_new.get(), n, 32, s);
+        prim::sort_pairs_u64_u32(keys.get(), skeys.get(), iota.get(), t->pr_old_of_new.get(), n, 33, s);
         k_rl_invert<<<blocks_for(n, 256), 256, 0, s>>>(t->pr_old_of_new.get(), n, in.off.get(), t->out_deg.get(),
                                                        t->pr_new_of_old.get(), rdeg.get(), t->pr_outdeg.get());
         check_launch();
         prim::exclusive_sum_u32(rdeg.get(), t->pr_off.get(), n, s);
+        HIP_OK(hipMemsetAsync(dnz.get(), 0, sizeof(uint64_t), s));
+        k_rl_nz<<<blocks_for(n, 256), 256, 0, s>>>(rdeg.get(), n, dnz.get());
+        check_launch();
+        HIP_OK(hipMemcpyAsync(&nz, dnz.get(), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
     }
     if (m) {
         DevBuf<uint64_t> key(m), skey(m);
@@ -535,23 +737,30 @@ static void pr_prepare(ursa_hip_topo* t, const DevCsr& in, hipStream_t s) {
         k_rl_low_words<<<blocks_for(m, 256), 256, 0, s>>>(skey.get(), m, t->pr_tgt.get());
         check_launch();
     }
-    const uint64_t tiles = (n + m + PR_TILE - 1) / PR_TILE;
-    t->pr_tile_x.alloc(tiles + 1);
-    t->pr_tile_y.alloc(tiles + 1);
-    k_pr_partition<<<blocks_for(tiles + 1, 256), 256, 0, s>>>(t->pr_off.get(), n, m, tiles, t->pr_tile_x.get(),
-                                                              t->pr_tile_y.get());
-    check_launch();
+    const uint64_t nchunks = (m + PR_CHUNK - 1) / PR_CHUNK;
+    t->pr_head.alloc(nchunks * PR_WORDS + 1);
+    HIP_OK(hipMemsetAsync(t->pr_head.get(), 0, (nchunks * PR_WORDS + 1) * sizeof(uint32_t), s));
+    t->pr_crow.alloc(nchunks ? nchunks : 1);
+    t->pr_cflag.alloc(nchunks ? nchunks : 1);
+    if (nz) {
+        k_pr_heads<<<blocks_for(nz, 256), 256, 0, s>>>(t->pr_off.get(), nz, t->pr_head.get());
+        check_launch();
+    }
+    if (nchunks) {
+        k_pr_chunks<<<blocks_for(nchunks, 256), 256, 0, s>>>(t->pr_off.get(), nz, t->pr_head.get(), m, nchunks,
+                                                             t->pr_crow.get(), t->pr_cflag.get());
+        check_launch();
+    }
     HIP_OK(hipStreamSynchronize(s));
-    t->pr_tiles = tiles;
+    t->pr_nz = nz;
+    t->pr_chunks = nchunks;
     t->pr_ready = true;
 }
 
-// Solve PageRank. With `mixed`, sweeps first run with f32 contributions and
-// compensated f32 row sums (8 B per edge instead of 12) until the L1 delta
-// stops shrinking quickly or falls below 2e-6, then hand over to f64 sweeps,
-// which refine the same iterate until delta < tol. Every iteration of either
-// kind counts toward max_iter, and the answer is always the last f64 iterate
-// (or the uniform start when max_iter == 0).
+// Solve PageRank (f64 solve, or with `mixed` the refinement solve described at
+// the top of this file). Every sweep of either kind counts toward max_iter,
+// and the answer is always the output of an f64 sweep (or the uniform start
+// when max_iter == 0).
 void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, uint32_t max_iter, double tol,
                   double* out_user, uint32_t* iters_out, bool mixed, uint32_t* f32_iters_out) {
     hipStream_t s = stream();
@@ -568,14 +777,13 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
         out.finish(s);
         return;
     }
+    if (!(tol < PR_F32_TOL) || max_iter <= PR_F32_MIN_ITER) mixed = false;
     pr_prepare(t, t->incoming(s), s);
-    const uint64_t tiles = t->pr_tiles;
-    // All solve-time arrays below are in the renumbered vertex order.
-    struct {
-        const uint32_t* off;
-        const uint32_t* tgt;
-        const uint32_t* eid;
-    } in{t->pr_off.get(), t->pr_tgt.get(), t->pr_eid.get()};
+    const uint64_t nchunks = t->pr_chunks, nz = t->pr_nz;
+    const uint32_t* tgt = t->pr_tgt.get();
+    const uint32_t* head = t->pr_head.get();
+    const uint32_t* crow = t->pr_crow.get();
+    const uint8_t* cflag = t->pr_cflag.get();
 
     const bool weighted = weights_user != nullptr;
     DevIn<double> w(weights_user, m, s);
@@ -586,6 +794,8 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
         DevBuf<double> os_old(n);
         out_str.alloc(n);
         k_out_strength<<<blocks_for(n, 256), 256, 0, s>>>(t->out.off.get(), t->out.eid.get(), w.p, n, os_old.get());
+        k_out_strength_heavy<<<blocks_for((n + 63) / 64, 4, 8192), 256, 0, s>>>(t->out.off.get(), t->out.eid.get(), w.p,
+                                                                               n, os_old.get());
         k_gather_f64<<<blocks_for(n, 256), 256, 0, s>>>(os_old.get(), t->pr_old_of_new.get(), n, out_str.get());
         check_launch();
         HIP_OK(hipStreamSynchronize(s));
@@ -594,7 +804,7 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
         DevBuf<uint32_t> inexact(1);
         HIP_OK(hipMemsetAsync(inexact.get(), 0, sizeof(uint32_t), s));
         if (m) {
-            k_permute_weights<<<blocks_for(m, 256), 256, 0, s>>>(in.eid, w.p, m, w64.get(), w32.get(),
+            k_permute_weights<<<blocks_for(m, 256), 256, 0, s>>>(t->pr_eid.get(), w.p, m, w64.get(), w32.get(),
                                                                  inexact.get());
             check_launch();
         }
@@ -607,30 +817,36 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
         else if (!mixed) w32.reset();
     }
 
+    // Launch shape: with the LDS copy, two persistent workgroups per CU;
+    // without it (graphs too small to fill the copy) one workgroup per group
+    // of four chunks.
+    const bool hot64 = n >= (uint64_t)(PR_HOT_BYTES / 8), hot32 = n >= (uint64_t)(PR_HOT_BYTES / 4);
+    auto grid_for = [&](bool hot) -> unsigned {
+        const uint64_t waves = hot ? PR_HOT_BLOCK / 64 : PR_PLAIN_BLOCK / 64;
+        const uint64_t groups = std::max<uint64_t>((nchunks + waves - 1) / waves, 1);
+        return (unsigned)(hot ? std::min<uint64_t>(groups, 2ull * cu_count()) : groups);
+    };
+    const unsigned g64 = grid_for(hot64), g32 = grid_for(hot32);
+    const uint64_t nparts = std::max(g64, g32);
+    const uint64_t fix_items = std::max<uint64_t>(nchunks + (n - nz), nparts);
+    const unsigned fix_blocks = (unsigned)((fix_items + PR_FIX - 1) / PR_FIX);
+
     DevBuf<double> rank[2] = {DevBuf<double>(n), DevBuf<double>(n)};
     DevBuf<double> contrib[2] = {DevBuf<double>(n), DevBuf<double>(n)};
-    DevBuf<float> contrib32[2];
+    DevBuf<float> x32[2], c32[2], r32;
     if (mixed) {
-        contrib32[0].alloc(n);
-        contrib32[1].alloc(n);
+        for (int i = 0; i < 2; ++i) {
+            x32[i].alloc(n);
+            c32[i].alloc(n);
+        }
+        r32.alloc(n);
     }
-    DevBuf<double> carry(tiles), tail(tiles), td(tiles), fd(tiles);
+    DevBuf<double> lead(std::max<uint64_t>(nchunks, 1)), tail(std::max<uint64_t>(nchunks, 1));
+    DevBuf<PrPart> part(nparts), fpart(std::max<uint64_t>(fix_blocks, (n + PR_FIX - 1) / PR_FIX));
     DevBuf<PrCtl> ctl(1);
     const unsigned init_blocks = blocks_for(n, 256, 1u << 30);
     const uint32_t* od = t->pr_outdeg.get();
     const double* os = weighted ? out_str.get() : nullptr;
-    if (mixed) {
-        if (weighted)
-            k_pr_init<float, true><<<init_blocks, 256, 0, s>>>(n, nf, od, os, rank[0].get(), contrib32[0].get());
-        else
-            k_pr_init<float, false><<<init_blocks, 256, 0, s>>>(n, nf, od, os, rank[0].get(), contrib32[0].get());
-    } else {
-        if (weighted)
-            k_pr_init<double, true><<<init_blocks, 256, 0, s>>>(n, nf, od, os, rank[0].get(), contrib[0].get());
-        else
-            k_pr_init<double, false><<<init_blocks, 256, 0, s>>>(n, nf, od, os, rank[0].get(), contrib[0].get());
-    }
-    check_launch();
 
     // The dangling nodes in ascending original order, as renumbered slots: the
     // order of the dangling-mass sum (pagerank.rs:76-78, 109-111).
@@ -671,8 +887,7 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
     // The first iterate is 1/n everywhere, so its dangling mass is 1/n added nd
     // times in order: closed form on the host (every later one on the device).
     const double dm0 = repeated_sum(1.0 / nf, nd);
-    const double base_a = (1.0 - damping) / nf, base_b = damping * dm0;
-    const double base0 = base_a + base_b / nf;  // pagerank.rs:79
+    const double base0 = (1.0 - damping) / nf + damping * dm0 / nf;  // pagerank.rs:79
     const int32_t e0 = dm0 > 0.0 ? (int32_t)std::ilogb(dm0) - 13 : 0;
     k_pr_ctl_init<<<1, 1, 0, s>>>(ctl.get(), e0, base0);
     check_launch();
@@ -680,69 +895,92 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
     const double node_bytes = weighted ? 36.0 : 32.0;
     const double f64_edge = weighted ? (use_f32w ? 16.0 : 20.0) : 12.0;
     const double f32_edge = weighted ? 12.0 : 8.0;
-    const unsigned fix_blocks = blocks_for(tiles, 256, 1u << 30);
-    const uint32_t* tx = t->pr_tile_x.get();
-    const uint32_t* ty = t->pr_tile_y.get();
+    const double f32_node = weighted ? 24.0 : 20.0;
 
-    // One iteration `it` (parity it & 1) with contribution type CT.
-    auto iteration = [&](uint32_t it, bool f32) {
-        const int p = (int)(it & 1);
+    // f64 sweep of rank[p] (device-driven base ctl->base[p] when dev, else base_arg).
+    auto sweep64 = [&](int p, bool dev, double base_arg) {
+        const int par = dev ? p : -1;
         const double* rk = rank[p].get();
         double* nx = rank[p ^ 1].get();
-        if (f32) {
-            const float* cb = contrib32[p].get();
-            float* cn = contrib32[p ^ 1].get();
-            timed("pagerank_spmv_f32", f32_edge * (double)m + (node_bytes - 4.0) * (double)n, s, [&] {
-                if (weighted)
-                    k_pr_tiles<float, float, true><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
-                        in.off, in.tgt, w32.get(), tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
-                        tail.get(), td.get(), ctl.get(), p, damping);
-                else
-                    k_pr_tiles<float, float, false><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
-                        in.off, in.tgt, nullptr, tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
-                        tail.get(), td.get(), ctl.get(), p, damping);
-            });
+        const double* cb = contrib[p].get();
+        double* cn = contrib[p ^ 1].get();
+        auto go = [&](auto hot_tag, auto w_tag, auto wt_tag) {
+            constexpr int HOT = decltype(hot_tag)::value;
+            constexpr bool WT = decltype(wt_tag)::value;
+            using W = typename decltype(w_tag)::type;
+            constexpr int BLOCK = HOT ? PR_HOT_BLOCK : PR_PLAIN_BLOCK;
+            const W* wp = nullptr;
+            if constexpr (WT) {
+                if constexpr (sizeof(W) == 4) wp = w32.get();
+                else wp = w64.get();
+            }
+            k_pr_sweep<double, double, W, WT, false, false, BLOCK, HOT><<<g64, BLOCK, 0, s>>>(
+                tgt, wp, head, crow, m, nchunks, od, os, rk, cb, nx, cn, nullptr, lead.get(), tail.get(), part.get(),
+                ctl.get(), par, base_arg, damping);
+        };
+        struct F { using type = float; };
+        struct D { using type = double; };
+        timed("pagerank_spmv", f64_edge * (double)m + node_bytes * (double)n, s, [&] {
+            using H0 = std::integral_constant<int, 0>;
+            using H1 = std::integral_constant<int, PR_HOT_BYTES / 8>;
+            using T = std::true_type;
+            using Fl = std::false_type;
+            if (!weighted) {
+                if (hot64) go(H1{}, D{}, Fl{}); else go(H0{}, D{}, Fl{});
+            } else if (use_f32w) {
+                if (hot64) go(H1{}, F{}, T{}); else go(H0{}, F{}, T{});
+            } else {
+                if (hot64) go(H1{}, D{}, T{}); else go(H0{}, D{}, T{});
+            }
+        });
+        check_launch();
+        timed("pagerank_fixup", 0.0, s, [&] {
             if (weighted)
-                k_pr_fixup<float, true><<<fix_blocks, 256, 0, s>>>(in.off, tx, ty, tiles, n, od, os, rk, nx, cn,
-                                                                   carry.get(), tail.get(), td.get(), fd.get(),
-                                                                   ctl.get(), p, damping);
+                k_pr_fixup<double, double, true, false, false><<<fix_blocks, PR_FIX, 0, s>>>(
+                    cflag, crow, nchunks, nz, n, od, os, rk, nx, cn, nullptr, lead.get(), tail.get(), part.get(), g64,
+                    fpart.get(), ctl.get(), par, base_arg, damping);
             else
-                k_pr_fixup<float, false><<<fix_blocks, 256, 0, s>>>(in.off, tx, ty, tiles, n, od, os, rk, nx, cn,
-                                                                    carry.get(), tail.get(), td.get(), fd.get(),
-                                                                    ctl.get(), p, damping);
-        } else {
-            const double* cb = contrib[p].get();
-            double* cn = contrib[p ^ 1].get();
-            timed("pagerank_spmv", f64_edge * (double)m + node_bytes * (double)n, s, [&] {
-                if (!weighted)
-                    k_pr_tiles<double, double, false><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
-                        in.off, in.tgt, nullptr, tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
-                        tail.get(), td.get(), ctl.get(), p, damping);
-                else if (use_f32w)
-                    k_pr_tiles<double, float, true><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
-                        in.off, in.tgt, w32.get(), tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
-                        tail.get(), td.get(), ctl.get(), p, damping);
-                else
-                    k_pr_tiles<double, double, true><<<(unsigned)tiles, PR_BLOCK, 0, s>>>(
-                        in.off, in.tgt, w64.get(), tx, ty, n, od, os, rk, cb, nx, cn, carry.get(),
-                        tail.get(), td.get(), ctl.get(), p, damping);
-            });
-            if (weighted)
-                k_pr_fixup<double, true><<<fix_blocks, 256, 0, s>>>(in.off, tx, ty, tiles, n, od, os, rk, nx,
-                                                                    cn, carry.get(), tail.get(), td.get(), fd.get(),
-                                                                    ctl.get(), p, damping);
-            else
-                k_pr_fixup<double, false><<<fix_blocks, 256, 0, s>>>(in.off, tx, ty, tiles, n, od, os, rk, nx,
-                                                                     cn, carry.get(), tail.get(), td.get(), fd.get(),
-                                                                     ctl.get(), p, damping);
-        }
+                k_pr_fixup<double, double, false, false, false><<<fix_blocks, PR_FIX, 0, s>>>(
+                    cflag, crow, nchunks, nz, n, od, os, rk, nx, cn, nullptr, lead.get(), tail.get(), part.get(), g64,
+                    fpart.get(), ctl.get(), par, base_arg, damping);
+        });
         check_launch();
     };
-    auto reduce = [&](uint32_t it, double tol_it) {
-        k_pr_reduce<<<1, 1024, 0, s>>>(fd.get(), fix_blocks, ctl.get(), max_iter, tol_it);
+    // f32 sweep xin -> xout (host-driven base; corr = residual of a correction sweep).
+    auto sweep32 = [&](const float* xin, const float* cin_, float* xout, float* cout, const float* corr,
+                       double base_arg) {
+        auto go = [&](auto hot_tag, auto wt_tag, auto corr_tag) {
+            constexpr int HOT = decltype(hot_tag)::value;
+            constexpr bool WT = decltype(wt_tag)::value;
+            constexpr bool CORR = decltype(corr_tag)::value;
+            constexpr int BLOCK = HOT ? PR_HOT_BLOCK : PR_PLAIN_BLOCK;
+            k_pr_sweep<float, float, float, WT, CORR, true, BLOCK, HOT><<<g32, BLOCK, 0, s>>>(
+                tgt, WT ? w32.get() : nullptr, head, crow, m, nchunks, od, os, xin, cin_, xout, cout, corr,
+                lead.get(), tail.get(), part.get(), ctl.get(), -1, base_arg, damping);
+            k_pr_fixup<float, float, WT, CORR, true><<<fix_blocks, PR_FIX, 0, s>>>(
+                cflag, crow, nchunks, nz, n, od, os, xin, xout, cout, corr, lead.get(), tail.get(), part.get(), g32,
+                fpart.get(), ctl.get(), -1, base_arg, damping);
+        };
+        using H0 = std::integral_constant<int, 0>;
+        using H1 = std::integral_constant<int, PR_HOT_BYTES / 4>;
+        using T = std::true_type;
+        using Fl = std::false_type;
+        const bool c = corr != nullptr;
+        timed(c ? "pagerank_spmv_f32c" : "pagerank_spmv_f32",
+              f32_edge * (double)m + (f32_node + (c ? 4.0 : 0.0)) * (double)n, s, [&] {
+                  if (hot32) {
+                      if (weighted) { if (c) go(H1{}, T{}, T{}); else go(H1{}, T{}, Fl{}); }
+                      else { if (c) go(H1{}, Fl{}, T{}); else go(H1{}, Fl{}, Fl{}); }
+                  } else {
+                      if (weighted) { if (c) go(H0{}, T{}, T{}); else go(H0{}, T{}, Fl{}); }
+                      else { if (c) go(H0{}, Fl{}, T{}); else go(H0{}, Fl{}, Fl{}); }
+                  }
+              });
         check_launch();
-        const int q = (int)((it & 1) ^ 1);
-        dangling_base(rank[q].get(), q);
+    };
+    auto reduce = [&](uint32_t mi, double tol_it) {
+        k_pr_reduce<<<1, 1024, 0, s>>>(fpart.get(), fix_blocks, ctl.get(), mi, tol_it);
+        check_launch();
     };
     auto read_ctl = [&]() {
         PrCtl h{};
@@ -751,63 +989,113 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
         return h;
     };
 
-    uint32_t it = 0;
-    uint32_t f32_iters = 0;
-    if (mixed) {
-        // f32 sweeps, one at a time (the host watches the delta). The reduce
-        // kernel's own convergence test is disabled (tol -1) so it only raises
-        // `done` at max_iter.
+    uint32_t iters = 0, f32_iters = 0;
+    int result = 0;  // rank[result] holds the answer
+    if (!mixed) {
+        if (weighted)
+            k_pr_init<double, double, true><<<init_blocks, 256, 0, s>>>(n, nf, od, os, rank[0].get(), contrib[0].get());
+        else
+            k_pr_init<double, double, false><<<init_blocks, 256, 0, s>>>(n, nf, od, os, rank[0].get(), contrib[0].get());
+        check_launch();
+        // Iterations are queued in batches; with per-kernel timing on, one at a
+        // time so no launch after convergence is booked. A full batch of 8 is
+        // recorded once per start parity as a HIP graph and replayed.
+        const uint32_t batch = prof_enabled() ? 1 : 8;
+        CapturedLaunches graphs[2];
+        uint32_t it = 0;
+        PrCtl h = read_ctl();
+        while (it < max_iter && !h.done) {
+            const uint32_t it_end = std::min<uint32_t>(max_iter, it + batch);
+            const uint32_t it0 = it;
+            auto launches = [&] {
+                for (uint32_t i = it0; i < it_end; ++i) {
+                    const int p = (int)(i & 1);
+                    sweep64(p, true, 0.0);
+                    reduce(max_iter, tol);
+                    dangling_base(rank[p ^ 1].get(), p ^ 1);
+                }
+            };
+            if (it_end - it0 == 8)
+                graphs[it0 & 1].run(s, launches);
+            else
+                launches();
+            it = it_end;
+            h = read_ctl();
+        }
+        iters = h.iters;
+        result = (int)(iters & 1);
+    } else {
+        constexpr uint32_t NO_LIMIT = 0xFFFFFFFFu;
+        // f32 sweeps from the uniform start.
+        if (weighted)
+            k_pr_init<float, float, true><<<init_blocks, 256, 0, s>>>(n, nf, od, os, x32[0].get(), c32[0].get());
+        else
+            k_pr_init<float, float, false><<<init_blocks, 256, 0, s>>>(n, nf, od, os, x32[0].get(), c32[0].get());
+        check_launch();
+        double dm = (double)nd * (double)(float)(1.0 / nf);
         double prev = 0.0;
-        while (it < max_iter) {
-            iteration(it, true);
-            reduce(it, -1.0);
+        int p = 0;
+        for (;;) {
+            sweep32(x32[p].get(), c32[p].get(), x32[p ^ 1].get(), c32[p ^ 1].get(), nullptr,
+                    (1.0 - damping) / nf + damping * dm / nf);
+            reduce(NO_LIMIT, -1.0);
             const PrCtl h = read_ctl();
-            ++it;
+            ++iters;
             ++f32_iters;
-            if (h.done) break;
+            p ^= 1;
+            dm = h.dm;
             const bool stalled = f32_iters >= 3 && h.delta > 0.7 * prev;
             prev = h.delta;
-            if (h.delta < 2e-6 || stalled) break;
+            if (h.delta < PR_F32_TOL || stalled || iters + 2 >= max_iter) break;
         }
-        // Hand over: f64 contributions of the current iterate.
-        const int p = (int)(it & 1);
         if (weighted)
-            k_pr_contrib64<true><<<blocks_for(n, 256), 256, 0, s>>>(n, od, os, rank[p].get(), contrib[p].get());
+            k_pr_widen<true><<<init_blocks, 256, 0, s>>>(n, od, os, x32[p].get(), rank[0].get(), contrib[0].get());
         else
-            k_pr_contrib64<false><<<blocks_for(n, 256), 256, 0, s>>>(n, od, os, rank[p].get(), contrib[p].get());
+            k_pr_widen<false><<<init_blocks, 256, 0, s>>>(n, od, os, x32[p].get(), rank[0].get(), contrib[0].get());
         check_launch();
-        const PrCtl h = read_ctl();
-        if (h.done) {
-            // max_iter was spent in f32 sweeps; one more f64 sweep is not allowed,
-            // so the f32 result stands.
-            it = max_iter;
+        for (;;) {
+            // f64 sweep rank[0] -> rank[1]: the residual and the stopping test.
+            dangling_base(rank[0].get(), 0);
+            sweep64(0, true, 0.0);
+            reduce(NO_LIMIT, -1.0);
+            PrCtl h = read_ctl();
+            ++iters;
+            if (h.delta < tol || iters >= max_iter) {
+                result = 1;
+                break;
+            }
+            // Correction: e = r + d (A e + dangling(e) / n), r = rank[1] - rank[0], from e0 = r.
+            const unsigned rb = (unsigned)((n + PR_FIX - 1) / PR_FIX);
+            if (weighted)
+                k_pr_residual<true><<<rb, PR_FIX, 0, s>>>(n, od, os, rank[1].get(), rank[0].get(), r32.get(),
+                                                          x32[0].get(), c32[0].get(), fpart.get());
+            else
+                k_pr_residual<false><<<rb, PR_FIX, 0, s>>>(n, od, os, rank[1].get(), rank[0].get(), r32.get(),
+                                                           x32[0].get(), c32[0].get(), fpart.get());
+            check_launch();
+            k_pr_reduce<<<1, 1024, 0, s>>>(fpart.get(), rb, ctl.get(), NO_LIMIT, -1.0);
+            check_launch();
+            double dme = read_ctl().dm;
+            int pe = 0;
+            for (uint32_t j = 0; j < PR_MAX_CORR && iters + 1 < max_iter; ++j) {
+                sweep32(x32[pe].get(), c32[pe].get(), x32[pe ^ 1].get(), c32[pe ^ 1].get(), r32.get(),
+                        damping * dme / nf);
+                reduce(NO_LIMIT, -1.0);
+                const PrCtl hc = read_ctl();
+                ++iters;
+                ++f32_iters;
+                pe ^= 1;
+                dme = hc.dm;
+                if (hc.delta < 0.25 * tol) break;
+            }
+            if (weighted)
+                k_pr_apply<true><<<init_blocks, 256, 0, s>>>(n, od, os, x32[pe].get(), rank[0].get(), contrib[0].get());
+            else
+                k_pr_apply<false><<<init_blocks, 256, 0, s>>>(n, od, os, x32[pe].get(), rank[0].get(), contrib[0].get());
+            check_launch();
         }
     }
-    PrCtl h = read_ctl();
-    // Iterations are queued in batches; with per-kernel timing on, one at a
-    // time so no launch after convergence is booked. A full batch of 8 is
-    // recorded once per start parity as a HIP graph and replayed.
-    const uint32_t batch = prof_enabled() ? 1 : 8;
-    CapturedLaunches graphs[2];
-    while (it < max_iter && !h.done) {
-        const uint32_t it_end = std::min<uint32_t>(max_iter, it + batch);
-        const uint32_t it0 = it;
-        auto launches = [&] {
-            for (uint32_t i = it0; i < it_end; ++i) {
-                iteration(i, false);
-                reduce(i, tol);
-            }
-        };
-        if (it_end - it0 == 8)
-            graphs[it0 & 1].run(s, launches);
-        else
-            launches();
-        it = it_end;
-        h = read_ctl();
-    }
-    h = read_ctl();
-    const uint32_t iters = h.iters;
-    k_gather_f64<<<blocks_for(n, 256), 256, 0, s>>>(rank[iters & 1].get(), t->pr_new_of_old.get(), n, out.p);
+    k_gather_f64<<<blocks_for(n, 256), 256, 0, s>>>(rank[result].get(), t->pr_new_of_old.get(), n, out.p);
     check_launch();
     out.finish(s);
     if (iters_out) *iters_out = iters;
@@ -831,6 +1119,7 @@ extern "C" int ursa_hip_pagerank_mixed(ursa_hip_topo* t, const double* weights, 
         ursa::pagerank_run(t, weights, damping, max_iter, tol, out, iters_out, true, f32_iters_out);
     });
 }
+
 
 namespace ursa {
 namespace {

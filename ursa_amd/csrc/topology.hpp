@@ -54,11 +54,12 @@ struct ursa_hip_topo {
     ursa::DevBuf<uint32_t> out_deg;
     std::mutex build_mu;  // guards the lazy in / und builds
     std::mutex cache_mu;  // guards algorithm-side caches below
-    // PageRank's private copy of the in-CSR with vertices renumbered by
-    // decreasing out-degree, and its merge-path partition (see pagerank.hip).
+    // PageRank's private copy of the in-CSR with renumbered vertices, its
+    // row-start bits and per-chunk row / flags (see pagerank.hip).
     ursa::DevBuf<uint32_t> pr_old_of_new, pr_new_of_old, pr_off, pr_tgt, pr_eid, pr_outdeg;
-    ursa::DevBuf<uint32_t> pr_tile_x, pr_tile_y;
-    uint64_t pr_tiles = 0;
+    ursa::DevBuf<uint32_t> pr_head, pr_crow;
+    ursa::DevBuf<uint8_t> pr_cflag;
+    uint64_t pr_nz = 0, pr_chunks = 0;
     bool pr_ready = false;
     // Multi-source BFS pull side, per direction (see msbfs.hip): the vertices
     // whose reverse lists are cut into chunks and their chunk bases; depends
