@@ -215,9 +215,9 @@ def bench_c2(args, ua, gen):
         "algo_bytes_per_launch": per_launch,
         "launches": launches,
     }
-    cpu = None
+    cpu, cpu_kernels, parity = None, None, None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline_bfs(e, sources, args.cpu_sources)
+        cpu, cpu_kernels, parity = c2_cpu_leg(args, lib, e, sources, lv, d_out, d_in, d_cc, d_tri, scale)
     ingest = bench_ingest(args, lib, e)
     for b in (lv, d_out, d_in, d_cc, d_tri):
         b.free()
@@ -241,6 +241,8 @@ def bench_c2(args, ua, gen):
         "roofline": roofline,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "cpu_kernels": cpu_kernels,
+        "parity": parity,
         "tri_launch": tri_q,
         "ingest": ingest,
     }
@@ -278,32 +280,116 @@ def bench_ingest(args, lib, e):
     return res
 
 
-def cpu_baseline_bfs(e, sources, count):
+def host_copy(lib, dev, dtype, count):
+    out = np.zeros(count, dtype)
+    if count:
+        check(lib.ursa_hip_memcpy_to_host(out.ctypes.data_as(C.c_void_p), dev.p, out.nbytes))
+    return out
+
+
+def oracle_bfs_rows(o, sources, threads, direction="out"):
+    """The oracle's serial BFS (bfs.rs:18-42) for each source, `threads` sources at a
+    time (ctypes releases the GIL during each call)."""
+    import oracle as orc
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(lambda x: orc.bfs(o, int(x), direction), sources))
+
+
+def cpu_threads():
+    """Host cores this process may use (the box's CPU share), capped at 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def c2_cpu_leg(args, lib, e, sources, lv, d_out, d_in, d_cc, d_tri, scale):
+    """CPU baseline of the step on the oracle (the reference's algorithms restated in
+    C++), timed on this host, and the parity check of every GPU output of the step
+    against it at full size: 64 BFS level rows, canonical component labels and in/out
+    degrees bit-exact; the u64 triangle counts against the committed oracle digest
+    (tests/golden/tri_rmat22.json: the node-iterator count takes the oracle ~20 min
+    at this size, so the digest was computed once in the build container)."""
+    import hashlib
+
     import oracle as orc
 
-    t0 = time.time()
+    threads = cpu_threads()
+    n = e.n
+    t0 = time.perf_counter()
     _, _, sd, dd, _ = e.export()
-    t = orc.Topology(e.n, sd, dd)
-    log(f"cpu baseline: oracle topology built in {time.time() - t0:.1f}s")
-    outdeg = orc.degree(t, "out").astype(np.int64)
-    sample = [int(x) for x in sources[:count]]
-    tot_edges = 0
-    el = 0.0
-    for s in sample:
+    o = orc.Topology(n, sd, dd)
+    build_s = time.perf_counter() - t0
+    parity = {}
+    per = {}
+    # degrees
+    t1 = time.perf_counter()
+    r_out, r_in = orc.degree(o, "out"), orc.degree(o, "in")
+    per["degree_in_out"] = {"s": round(time.perf_counter() - t1, 4), "threads": 1}
+    parity["degree_out_equal"] = bool(np.array_equal(host_copy(lib, d_out, np.uint32, n), r_out))
+    parity["degree_in_equal"] = bool(np.array_equal(host_copy(lib, d_in, np.uint32, n), r_in))
+    # BFS: one thread (a sample, cold + warm) and all threads (every source)
+    outdeg = r_out.astype(np.int64)
+    k1 = max(1, min(args.cpu_sources, len(sources)))
+    times1 = []
+    for rep in range(3):
         t1 = time.perf_counter()
-        lv = orc.bfs(t, s, "out")
-        el += time.perf_counter() - t1
-        tot_edges += int(outdeg[lv >= 0].sum())
-    gteps = tot_edges / el / 1e9
-    log(f"cpu baseline: {len(sample)} serial BFS in {el:.2f}s: {gteps:.4f} GTEPS")
-    return {
-        "value": round(gteps, 5),
+        oracle_bfs_rows(o, sources[:k1], 1)
+        times1.append(time.perf_counter() - t1)
+    t1 = time.perf_counter()
+    rows = oracle_bfs_rows(o, sources, threads)
+    tall = time.perf_counter() - t1
+    reach_all = sum(int(outdeg[r >= 0].sum()) for r in rows)
+    reach_1 = sum(int(outdeg[r >= 0].sum()) for r in rows[:k1])
+    got = host_copy(lib, lv, np.int32, 64 * n).reshape(64, n)
+    parity["bfs_rows_equal"] = int(sum(np.array_equal(got[i], rows[i]) for i in range(len(sources))))
+    parity["bfs_rows"] = len(sources)
+    del got
+    per["bfs_1thread"] = {"sources": k1, "cold_s": round(times1[0], 4), "warm_median_s": round(float(np.median(times1[1:])), 4),
+                          "warm_min_s": round(min(times1[1:]), 4),
+                          "GTEPS": round(reach_1 / float(np.median(times1[1:])) / 1e9, 5)}
+    per["bfs_all_threads"] = {"sources": len(sources), "threads": threads, "s": round(tall, 4),
+                              "GTEPS": round(reach_all / tall / 1e9, 5)}
+    # weak components: serial union-find, as the reference runs it
+    times = []
+    for rep in range(3):
+        t1 = time.perf_counter()
+        lab = orc.cc_weak(o)
+        times.append(time.perf_counter() - t1)
+    parity["cc_labels_equal"] = bool(np.array_equal(host_copy(lib, d_cc, np.uint32, n), orc.canonicalize(lab)))
+    per["cc"] = {"threads": 1, "cold_s": round(times[0], 4), "warm_median_s": round(float(np.median(times[1:])), 4),
+                 "GTEPS": round(e.m / float(np.median(times[1:])) / 1e9, 5)}
+    # triangles: digest of the full per-node u64 counts
+    tri = host_copy(lib, d_tri, np.uint64, n)
+    fx_path = os.path.join(ROOT, "tests", "golden", f"tri_rmat{scale}.json")
+    if os.path.exists(fx_path):
+        with open(fx_path) as f:
+            fx = json.load(f)
+        parity["triangles_sha256_equal"] = hashlib.sha256(tri.astype("<u8").tobytes()).hexdigest() == fx["sha256_u64_le"]
+        parity["triangles_total"] = int(tri.sum(dtype=np.uint64)) // 3
+        parity["triangles_total_oracle"] = fx["triangles"]
+        per["triangle_oracle_fixture"] = {"s": fx["oracle_seconds"], "threads": fx["oracle_threads"],
+                                          "where": "build container (node-iterator count, not rerun here)"}
+    else:
+        parity["triangles_sha256_equal"] = None
+    parity["all_equal"] = bool(parity["degree_out_equal"] and parity["degree_in_equal"]
+                               and parity["bfs_rows_equal"] == len(sources) and parity["cc_labels_equal"]
+                               and parity["triangles_sha256_equal"] is not False)
+    log(f"c2 parity vs oracle: {json.dumps(parity)}")
+    cpu = {
+        "value": per["bfs_all_threads"]["GTEPS"],
         "unit": "GTEPS",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"oracle bfs_distances (the reference's serial frontier loop) from the first {len(sample)} of "
-                  f"the 64 benchmark sources on the same graph",
+        "sample": f"oracle bfs_distances (the reference's serial frontier loop) for all {len(sources)} benchmark "
+                  f"sources of the step, one source per thread on {threads} threads; per-kernel CPU times "
+                  f"(1 thread, cold + warm) in cpu_kernels",
+        "oracle_topology_build_s": round(build_s, 3),
     }
+    return cpu, per, parity
 
 
 # --------------------------------------------------------------------- PageRank
@@ -374,6 +460,50 @@ def bench_egress(ua, lib, topo, n, m, out):
     return res
 
 
+def bench_c1(args, ua, gen):
+    """R-MAT scale 16, seed 1, damping 0.85, tol 1e-6, max_iter 100 (the reference's
+    CPU-runnable case): the f64 GPU solve through the C ABI with its inputs and output
+    in HBM, and the oracle's solve on this host at one thread and at all threads
+    (cold run + median/min of warm runs), with the GPU result checked against it."""
+    lib = ua.load()
+    e = gen.rmat(16, 16, 0.57, 0.19, 0.19, seed=1)
+    topo = e.topology()
+    n, m = e.n, e.m
+    out = Dev(lib, 8 * n)
+    it = C.c_uint32(0)
+    run = lambda p: check(lib.ursa_hip_pagerank(topo._h, None, 0.85, 100, 1e-6, p, C.byref(it)))
+    run(out.p)
+    check(lib.ursa_hip_synchronize())
+    t0 = time.perf_counter()
+    for _ in range(20):
+        run(out.p)
+    check(lib.ursa_hip_synchronize())
+    ms = (time.perf_counter() - t0) / 20 * 1e3
+    res = {"graph": {"scale": 16, "seed": 1, "n": n, "m": m, "tol": 1e-6, "max_iter": 100},
+           "gpu": {"ms": round(ms, 3), "iters": it.value, "GTEPS": round(m * it.value / (ms / 1e3) / 1e9, 3)}}
+    if not args.no_cpu_baseline:
+        import oracle as orc
+
+        got = np.zeros(n, np.float64)
+        run(got.ctypes.data_as(C.c_void_p))
+        _, _, sd, dd, _ = e.export()
+        o = orc.Topology(n, sd, dd)
+        for th in sorted({1, cpu_threads()}):
+            ts = []
+            for _ in range(6):
+                t1 = time.perf_counter()
+                ref, rit, _ = orc.pagerank(o, 0.85, 100, 1e-6, nthreads=th)
+                ts.append(time.perf_counter() - t1)
+            res[f"cpu_{th}thread"] = {"cold_ms": round(ts[0] * 1e3, 3),
+                                      "warm_median_ms": round(float(np.median(ts[1:])) * 1e3, 3),
+                                      "warm_min_ms": round(min(ts[1:]) * 1e3, 3), "iters": rit}
+        res["parity"] = {"iters_equal": it.value == rit,
+                         "max_rel": float((np.abs(got - ref) / np.abs(ref)).max())}
+    out.free()
+    log(f"c1: {json.dumps(res)}")
+    return res
+
+
 def bench_pagerank(args, ua, gen):
     lib = ua.load()
     scale = args.pr_scale
@@ -425,38 +555,94 @@ def bench_pagerank(args, ua, gen):
                         "traffic": traffic_from_profiles(f"pagerank_rmat{scale}",
                                                          "k_pr_tiles" if key.endswith("f64") else "k_pr_tiles_f32")}
     res["mixed_speedup"] = round(res["f64"]["ms"] / res["mixed"]["ms"], 3)
+    res["c1"] = bench_c1(args, ua, gen)
     res["bfs"] = bench_bfs_large(ua, lib, topo, n, m)
     res["egress"] = bench_egress(ua, lib, topo, n, m, out)
-    # accuracy against the f64 oracle
+    # Weighted variant: f32-exact weights U(0,1] per edge row (pagerank.rs:135-169).
+    w_dev = Dev(lib, 8 * m)
+    check(lib.ursa_hip_gen_weights(m, 3, C.c_void_p(w_dev.p.value)))
+
+    def plain_w():
+        check(lib.ursa_hip_pagerank(topo._h, w_dev.p, params.damping, params.max_iter, params.tol, out.p, C.byref(it)))
+
+    def mixed_w():
+        check(lib.ursa_hip_pagerank_mixed(topo._h, w_dev.p, params.damping, params.max_iter, params.tol, out.p,
+                                          C.byref(it), C.byref(it32)))
+
+    for name, fn in (("weighted_f64", plain_w), ("weighted_mixed", mixed_w)):
+        fn()
+        check(lib.ursa_hip_synchronize())
+        t1 = time.perf_counter()
+        for _ in range(args.pr_steps):
+            fn()
+        check(lib.ursa_hip_synchronize())
+        ms = (time.perf_counter() - t1) / args.pr_steps * 1e3
+        res[name] = {"ms": round(ms, 3), "iters": it.value, "GTEPS": round(m * it.value / (ms / 1e3) / 1e9, 3)}
+        if name.endswith("mixed"):
+            res[name]["f32_iters"] = it32.value
+        log(f"pagerank {name}: {ms:.2f} ms, {it.value} iterations")
+    res["weighted_mixed_speedup"] = round(res["weighted_f64"]["ms"] / res["weighted_mixed"]["ms"], 3)
+    # CPU leg: the oracle's f64 solves (the reference's pull loop restated) timed on
+    # this host, and the accuracy of every GPU solve against them.
     if not args.no_cpu_baseline:
         import oracle as orc
 
-        got64 = np.zeros(n, np.float64)
-        check(lib.ursa_hip_pagerank(topo._h, None, params.damping, params.max_iter, params.tol,
-                                    got64.ctypes.data_as(C.c_void_p), C.byref(it)))
-        gotmx = np.zeros(n, np.float64)
-        check(lib.ursa_hip_pagerank_mixed(topo._h, None, params.damping, params.max_iter, params.tol,
-                                          gotmx.ctypes.data_as(C.c_void_p), C.byref(it), C.byref(it32)))
+        threads = cpu_threads()
+        got = {}
+        for name, wp, fn in (("f64", None, lib.ursa_hip_pagerank), ("mixed", None, lib.ursa_hip_pagerank_mixed),
+                             ("weighted_f64", w_dev.p, lib.ursa_hip_pagerank),
+                             ("weighted_mixed", w_dev.p, lib.ursa_hip_pagerank_mixed)):
+            g = np.zeros(n, np.float64)
+            extra = (C.byref(it32),) if name.endswith("mixed") else ()
+            check(fn(topo._h, wp, params.damping, params.max_iter, params.tol, g.ctypes.data_as(C.c_void_p),
+                     C.byref(it), *extra))
+            got[name] = g
+        w_host = np.zeros(m, np.float64)
+        check(lib.ursa_hip_memcpy_to_host(w_host.ctypes.data_as(C.c_void_p), w_dev.p, 8 * m))
         t2 = time.time()
         _, _, sd, dd, _ = e.export()
         ot = orc.Topology(n, sd, dd)
+        del sd, dd
         indeg = orc.degree(ot, "in")
-        threads = os.cpu_count() or 1
-        t3 = time.perf_counter()
-        ref, rit, _ = orc.pagerank(ot, params.damping, params.max_iter, params.tol, nthreads=threads)
-        cpu_s = time.perf_counter() - t3
-        log(f"pagerank oracle: {rit} iterations in {cpu_s:.1f}s on {threads} threads (setup {time.time() - t2:.0f}s)")
-        top_r = set(np.argsort(-ref, kind="stable")[:1000].tolist())
-        for name, g in (("f64", got64), ("mixed", gotmx)):
+        refs, cpu = {}, {}
+        for kind, wts in (("", None), ("weighted_", w_host)):
+            t3 = time.perf_counter()
+            ref, rit, _ = orc.pagerank(ot, params.damping, params.max_iter, params.tol, weights=wts, nthreads=threads)
+            cpu_s = time.perf_counter() - t3
+            refs[kind] = (ref, rit)
+            cpu[kind + "f64"] = {"s": round(cpu_s, 3), "iters": rit, "threads": threads,
+                                 "GTEPS": round(m * rit / cpu_s / 1e9, 4)}
+            log(f"pagerank oracle {kind or 'unweighted '}: {rit} iterations in {cpu_s:.1f}s on {threads} threads")
+        log(f"pagerank oracle setup {time.time() - t2:.0f}s")
+        for name, g in got.items():
+            ref, rit = refs["weighted_" if name.startswith("weighted") else ""]
             rel = np.abs(g - ref) / np.abs(ref)
-            res[name]["L1_vs_oracle"] = float(np.abs(g - ref).sum())
-            res[name]["maxabs_vs_oracle"] = float(np.abs(g - ref).max())
-            res[name]["max_rel_vs_oracle"] = float(rel.max())
+            r = res[name]
+            r["L1_vs_oracle"] = float(np.abs(g - ref).sum())
+            r["maxabs_vs_oracle"] = float(np.abs(g - ref).max())
+            r["max_rel_vs_oracle"] = float(rel.max())
+            r["oracle_iters"] = rit
             w = int(np.argmax(rel))
-            res[name]["worst_node"] = {"in_degree": int(indeg[w]), "rank": float(ref[w]),
-                                       "nodes_above_1e-12": int((rel > 1e-12).sum())}
-            res[name]["top1000_overlap"] = len(top_r & set(np.argsort(-g, kind="stable")[:1000].tolist()))
-        res["cpu_oracle"] = {"GTEPS": round(m * rit / cpu_s / 1e9, 4), "iters": rit, "threads": threads}
+            r["worst_node"] = {"in_degree": int(indeg[w]), "rank": float(ref[w]),
+                               "nodes_above_1e-12": int((rel > 1e-12).sum())}
+            top_r = set(np.argsort(-ref, kind="stable")[:1000].tolist())
+            r["top1000_overlap"] = len(top_r & set(np.argsort(-g, kind="stable")[:1000].tolist()))
+        res["cpu_oracle"] = cpu
+        res["parity"] = {
+            "f64_max_rel": res["f64"]["max_rel_vs_oracle"],
+            "f64_iters_equal": res["f64"]["iters"] == refs[""][1],
+            "weighted_f64_max_rel": res["weighted_f64"]["max_rel_vs_oracle"],
+            "weighted_f64_iters_equal": res["weighted_f64"]["iters"] == refs["weighted_"][1],
+            "mixed_L1": res["mixed"]["L1_vs_oracle"],
+            "weighted_mixed_L1": res["weighted_mixed"]["L1_vs_oracle"],
+            "within_bars": bool(res["f64"]["max_rel_vs_oracle"] <= 1e-12
+                                and res["weighted_f64"]["max_rel_vs_oracle"] <= 1e-12
+                                and res["mixed"]["L1_vs_oracle"] <= 1e-9
+                                and res["weighted_mixed"]["L1_vs_oracle"] <= 1e-9),
+        }
+        log(f"pagerank parity: {json.dumps(res['parity'])}")
+        del w_host, ot
+    w_dev.free()
     out.free()
     res["graph"] = {"scale": scale, "seed": 3, "n": n, "m": m, "damping": 0.85, "tol": 1e-10, "max_iter": 100}
     return res
@@ -492,6 +678,28 @@ def bench_batched(args, ua, gen):
     rel = np.abs(out_b - out_1) / np.maximum(np.abs(out_1), 1e-300)
     res["brandes_speedup"] = round(res["brandes_one_per_launch"]["s"] / res["brandes_batched64"]["s"], 2)
     res["brandes_max_rel_diff_batched_vs_single"] = float(rel[out_1 != 0].max()) if np.any(out_1 != 0) else 0.0
+    o = None
+    if not args.no_cpu_baseline:
+        # CPU leg: the oracle's Brandes over the same 256 sources (betweenness.rs:45-70,
+        # 64-source chunks in parallel), timed here, and the GPU result checked against it.
+        import oracle as orc
+
+        threads = cpu_threads()
+        _, _, sd, dd, _ = e.export()
+        o = orc.Topology(n, sd, dd)
+        del sd, dd
+        t0 = time.perf_counter()
+        ref = orc.betweenness_sources(o, src, scale=n / 256.0, nthreads=threads)
+        cpu_s = time.perf_counter() - t0
+        nzr = ref != 0
+        relb = np.abs(out_b - ref)[nzr] / np.abs(ref[nzr])
+        res["brandes_cpu_oracle"] = {"s": round(cpu_s, 3), "threads": threads,
+                                     "sources_per_s": round(256 / cpu_s, 2)}
+        res["brandes_parity"] = {"max_rel_vs_oracle": float(relb.max()) if relb.size else 0.0,
+                                 "zeros_equal": bool(np.array_equal(out_b[~nzr], ref[~nzr])),
+                                 "within_1e-12": bool((relb.size == 0 or relb.max() <= 1e-12)
+                                                      and np.array_equal(out_b[~nzr], ref[~nzr]))}
+        log(f"brandes k=256 oracle on {threads} threads: {cpu_s:.2f} s; parity {json.dumps(res['brandes_parity'])}")
     # Multi-source BFS from 4096 sources = shuffled_order(n, 4 ^ 1)[:4096].
     nsrc = args.msbfs_sources
     bsrc = np.ascontiguousarray(ua.shuffled_order(n, 4 ^ 1)[:nsrc])
@@ -511,6 +719,15 @@ def bench_batched(args, ua, gen):
     single = np.zeros((64, n), np.int32)
     check(lib.ursa_hip_memcpy_to_host(single.ctypes.data_as(C.c_void_p), lv.p, single.nbytes))
     res["msbfs_first64_identical"] = bool(np.array_equal(first, single))
+    if o is not None:
+        threads = cpu_threads()
+        t0 = time.perf_counter()
+        rows = oracle_bfs_rows(o, bsrc[:64], threads)
+        cpu_s = time.perf_counter() - t0
+        res["msbfs_parity"] = {"rows_checked": 64,
+                               "rows_equal": int(sum(np.array_equal(first[i], rows[i]) for i in range(64)))}
+        res["msbfs_cpu_oracle"] = {"sources": 64, "threads": threads, "s": round(cpu_s, 3)}
+        log(f"msbfs parity vs oracle: {json.dumps(res['msbfs_parity'])}")
     res["msbfs_speedup"] = round(res["msbfs_one_per_launch"]["s"] / res["msbfs_batched64"]["s"], 2)
     lv.free()
     return res
@@ -667,6 +884,8 @@ def main():
         "config": res["config"],
         "roofline": res["roofline"],
         "cpu_baseline": res["cpu_baseline"],
+        "cpu_kernels": res["cpu_kernels"],
+        "parity": res["parity"],
         "kernels": res["kernels"],
         "ingest": res["ingest"],
         "pagerank": pr,
