@@ -179,6 +179,17 @@ int ursa_hip_triangle_count(ursa_hip_topo* t, uint64_t* out);
  * the reference's contract, and its canonical form). */
 int ursa_hip_cc_weak(ursa_hip_topo* t, uint32_t* labels);
 
+/* No counterpart in the reference. ursa_hip_cc_weak for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): same labels, but no hipMalloc / hipFree or other
+ * allocation, no host copy and no wait. `labels` must be device memory of n
+ * u32; the work is queued on the calling thread's stream (ursa_hip_set_stream,
+ * else the library's stream of that thread) and the labels are ready when that
+ * stream reaches it. Scratch is a fixed device array (16 MiB per device, kept
+ * until the process ends), so n is limited to 4194304 (URSA_HIP_ERANGE above);
+ * calls that overlap on one device share it and race. */
+int ursa_hip_cc_weak_noalloc(ursa_hip_topo* t, uint32_t* labels);
+
 #ifdef __cplusplus
 }
 #endif

@@ -101,6 +101,7 @@ SIGNATURES = {
     "ursa_hip_server_result_data": (cint, [P, u64, P, u64, C.POINTER(u64)]),
     "ursa_hip_server_destroy": (cint, [P]),
     "ursa_hip_digest": (u64, [P, u64]),
+    "ursa_hip_cc_weak_noalloc": (cint, [P, P]),
 }
 
 _lib = None

@@ -25,6 +25,7 @@
 //
 // Algorithmic bytes: 8 B per edge processed + 8 B per node per sweep.
 #include <algorithm>
+#include <mutex>
 #include <unordered_map>
 #include <vector>
 
@@ -36,6 +37,8 @@ namespace {
 
 constexpr int CB = 256;
 constexpr uint32_t HEAVY = 64;
+constexpr uint32_t CC_SAMPLES = 1024;
+constexpr uint64_t CC_SAMPLE_SEED = 0x9E3779B97F4A7C15ull;
 
 __device__ __forceinline__ uint32_t ld(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -134,8 +137,11 @@ __global__ void k_cc_compress(uint32_t* comp, uint64_t n) {
 
 // Light vertices (not in the giant component) hook their edges from slot
 // `from` on; heavy ones are queued for k_cc_heavy.
+// The skipped component is *giant_p when giant_p is set, else giant_v.
 __global__ void k_cc_light(const uint32_t* __restrict__ off, const uint32_t* __restrict__ tgt, uint64_t n,
-                           uint32_t from, uint32_t giant, uint32_t* comp, uint32_t* heavy, uint32_t* heavy_len) {
+                           uint32_t from, const uint32_t* giant_p, uint32_t giant_v, uint32_t* comp, uint32_t* heavy,
+                           uint32_t* heavy_len) {
+    const uint32_t giant = giant_p ? *giant_p : giant_v;
     for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n; u += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t a = off[u] + from, b = off[u + 1];
         bool qh = false;
@@ -167,6 +173,57 @@ __global__ void k_cc_sample(const uint32_t* __restrict__ comp, const uint32_t* _
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < k) out[i] = comp[idx[i]];
 }
+
+// The giant-component vote on the device (one workgroup of CC_SAMPLES threads):
+// sample i is the root of vertex x_{i+1} % n of the xorshift sequence the host
+// path draws, and the winner is the most frequent root, the smallest on ties —
+// the host path's choice. Also clears the heavy-queue counter.
+__global__ __launch_bounds__(CC_SAMPLES) void k_cc_giant(const uint32_t* __restrict__ comp, uint64_t n,
+                                                        uint32_t* __restrict__ giant, uint32_t* __restrict__ hlen) {
+    __shared__ uint32_t samp[CC_SAMPLES];
+    __shared__ unsigned long long best[CC_SAMPLES / 64];
+    const uint32_t i = threadIdx.x;
+    const uint32_t ks = n < CC_SAMPLES ? (uint32_t)n : CC_SAMPLES;
+    if (i < ks) {
+        uint64_t x = CC_SAMPLE_SEED;
+        for (uint32_t j = 0; j <= i; ++j) {
+            x ^= x << 13;
+            x ^= x >> 7;
+            x ^= x << 17;
+        }
+        samp[i] = __hip_atomic_load(&comp[x % n], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    unsigned long long key = 0;  // (count, ~root): the maximum is the most frequent, smallest root
+    if (i < ks) {
+        const uint32_t v = samp[i];
+        uint32_t cnt = 0;
+        for (uint32_t j = 0; j < ks; ++j) cnt += samp[j] == v;
+        key = (unsigned long long)cnt << 32 | (uint32_t)~v;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(key, off, 64);
+        key = o > key ? o : key;
+    }
+    if ((i & 63) == 0) best[i >> 6] = key;
+    __syncthreads();
+    if (i == 0) {
+        unsigned long long b = 0;
+        for (uint32_t w = 0; w < CC_SAMPLES / 64; ++w) b = best[w] > b ? best[w] : b;
+        *giant = ~(uint32_t)b;
+        *hlen = 0;
+    }
+}
+
+__global__ void k_cc_zero(uint32_t* p) { *p = 0; }
+
+// Scratch of ursa_hip_cc_weak_noalloc: the heavy-vertex queue, its counter and
+// the giant root, for graphs of at most CC_NOALLOC_MAX_N vertices. Each device
+// holds its own copy (16 MiB + 1 KiB) for the life of the process.
+constexpr uint64_t CC_NOALLOC_MAX_N = 1ull << 22;
+constexpr uint64_t CC_SCRATCH_HLEN = CC_NOALLOC_MAX_N, CC_SCRATCH_GIANT = CC_NOALLOC_MAX_N + 1;
+constexpr uint64_t CC_SCRATCH_WORDS = CC_NOALLOC_MAX_N + 256;
+static __device__ __align__(256) uint32_t scratch[CC_SCRATCH_WORDS];
 
 }  // namespace
 
@@ -226,7 +283,8 @@ void cc_weak(ursa_hip_topo* t, uint32_t* out_user, uint32_t* sweeps_out) {
     auto sweep = [&](const DevCsr& c, uint32_t from, const char* name) {
         HIP_OK(hipMemsetAsync(hlen.get(), 0, sizeof(uint32_t), s));
         timed(name, 8.0 * (double)c.slots + nb, s, [&] {
-            k_cc_light<<<g, CB, 0, s>>>(c.off.get(), c.tgt.get(), n, from, skip, comp, heavy.get(), hlen.get());
+            k_cc_light<<<g, CB, 0, s>>>(c.off.get(), c.tgt.get(), n, from, nullptr, skip, comp, heavy.get(),
+                                        hlen.get());
             k_cc_heavy<<<blocks_for(n, CB, 1u << 14), CB, 0, s>>>(c.off.get(), c.tgt.get(), heavy.get(), hlen.get(),
                                                                   from, comp);
         });
@@ -240,7 +298,88 @@ void cc_weak(ursa_hip_topo* t, uint32_t* out_user, uint32_t* sweeps_out) {
     if (sweeps_out) *sweeps_out = have_in ? 2 : 1;
 }
 
+// Device address of `scratch` per device, taken once per device with that
+// device current (ursa_hip_init, else the first no-allocation call there). The
+// array's name is a host-side stand-in; only these addresses reach kernels.
+namespace {
+std::mutex g_scratch_mu;
+uint32_t* g_scratch[64] = {};
+}  // namespace
+
+void cc_noalloc_setup() {
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    URSA_REQUIRE(dev >= 0 && dev < 64, URSA_HIP_EDEVICE, "device index beyond 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[dev]) return;
+    void* p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    URSA_REQUIRE(p != nullptr, URSA_HIP_EDEVICE, "no device address for the components scratch");
+    g_scratch[dev] = static_cast<uint32_t*>(p);
+}
+
+// cc_weak without any allocation: the caller's device labels are the parent
+// array, the heavy queue and counters live in this device's `scratch`, the
+// giant component is voted on the device, and nothing is copied to the host or
+// waited for: the work is queued on the calling thread's stream.
+void cc_weak_noalloc(ursa_hip_topo* t, uint32_t* comp) {
+    const uint64_t n = t->n;
+    if (n == 0) return;
+    URSA_REQUIRE(n <= CC_NOALLOC_MAX_N, URSA_HIP_ERANGE,
+                 "ursa_hip_cc_weak_noalloc takes at most 4194304 vertices (its scratch size)");
+    URSA_REQUIRE(is_device_ptr(comp), URSA_HIP_EINVAL, "labels must be device memory for the no-allocation entry point");
+    int dev = ensure_device();
+    HIP_OK(hipGetDevice(&dev));
+    uint32_t* sp = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        sp = (dev >= 0 && dev < 64) ? g_scratch[dev] : nullptr;
+    }
+    if (!sp) {
+        cc_noalloc_setup();
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        sp = g_scratch[dev];
+    }
+    uint32_t* heavy = sp;
+    uint32_t* hlen = sp + CC_SCRATCH_HLEN;
+    uint32_t* giant = sp + CC_SCRATCH_GIANT;
+    hipStream_t s = stream();
+    const DevCsr& o = t->out;
+    const bool have_in = t->in.built;
+    const unsigned g = blocks_for(n, CB, 1u << 16);
+    k_cc_init<<<g, CB, 0, s>>>(comp, n);
+    k_cc_first<<<g, CB, 0, s>>>(o.off.get(), o.tgt.get(), n, comp);
+    k_cc_round0<<<g, CB, 0, s>>>(o.off.get(), o.tgt.get(), n, comp);
+    k_cc_compress<<<g, CB, 0, s>>>(comp, n);
+    k_cc_round1<<<g, CB, 0, s>>>(o.off.get(), o.tgt.get(), n, comp);
+    k_cc_compress<<<g, CB, 0, s>>>(comp, n);
+    k_cc_giant<<<1, CC_SAMPLES, 0, s>>>(comp, n, giant, hlen);
+    check_launch();
+    // Without the transpose nothing may be skipped: an id no root has.
+    k_cc_light<<<g, CB, 0, s>>>(o.off.get(), o.tgt.get(), n, 1, have_in ? giant : nullptr, 0xFFFFFFFFu, comp, heavy,
+                                hlen);
+    k_cc_heavy<<<blocks_for(n, CB, 1u << 14), CB, 0, s>>>(o.off.get(), o.tgt.get(), heavy, hlen, 1, comp);
+    check_launch();
+    if (have_in) {
+        k_cc_zero<<<1, 1, 0, s>>>(hlen);
+        k_cc_light<<<g, CB, 0, s>>>(t->in.off.get(), t->in.tgt.get(), n, 0, giant, 0u, comp, heavy, hlen);
+        k_cc_heavy<<<blocks_for(n, CB, 1u << 14), CB, 0, s>>>(t->in.off.get(), t->in.tgt.get(), heavy, hlen, 0, comp);
+        check_launch();
+    }
+    k_cc_compress<<<g, CB, 0, s>>>(comp, n);
+    check_launch();
+}
+
 }  // namespace ursa
+
+extern "C" int ursa_hip_cc_weak_noalloc(ursa_hip_topo* t, uint32_t* labels) {
+    return ursa::guard([&] {
+        URSA_REQUIRE(t != nullptr, URSA_HIP_EINVAL, "topology is NULL");
+        if (t->n == 0) return;
+        URSA_REQUIRE(labels != nullptr, URSA_HIP_EINVAL, "labels is NULL");
+        ursa::cc_weak_noalloc(t, labels);
+    });
+}
 
 extern "C" int ursa_hip_cc_weak(ursa_hip_topo* t, uint32_t* labels) {
     return ursa::guard([&] {

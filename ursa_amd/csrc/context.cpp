@@ -12,6 +12,8 @@
 
 namespace ursa {
 
+void cc_noalloc_setup();
+
 namespace {
 thread_local std::string t_last_error;
 std::atomic<int> g_device{-1};
@@ -362,6 +364,7 @@ int ursa_hip_init(int device) {
             g_device.store(device);
         }
         ensure_device();
+        ursa::cc_noalloc_setup();  // device address of the no-allocation scratch
     });
 }
 
