@@ -158,7 +158,7 @@ def bench_c2(args, ua, gen):
     ua.profile.reset()
     step()
     groups = {
-        "bfs": ["bfs_top_down", "bfs_bottom_up", "msbfs_push", "msbfs_pull"],
+        "bfs": ["bfs_top_down", "bfs_bottom_up", "msbfs_expand"],
         "cc": ["cc_hook_round", "cc_compress", "cc_hook_out", "cc_hook_in"],
         "triangle": ["triangle_count"],
         "degree": ["degree"],
@@ -403,7 +403,7 @@ def bench_bfs_large(ua, lib, topo, n, m):
     lv = Dev(lib, 4 * 64 * n)
     reached = C.c_uint64(0)
     out = {}
-    for name, k, mode, kern in (("msbfs_x64", 64, 2, ["msbfs_push", "msbfs_pull"]),
+    for name, k, mode, kern in (("msbfs_x64", 64, 2, ["msbfs_expand"]),
                                 ("single_x4", 4, 1, ["bfs_top_down", "bfs_bottom_up"])):
         check(lib.ursa_hip_bfs_ex(topo._h, sources.ctypes.data_as(C.c_void_p), k, 0, lv.p, C.byref(reached), mode))
         ua.profile.enable(True)

@@ -238,6 +238,43 @@ struct VmCtr {
     unsigned long long edges;
 };
 
+// Device-side state of a batch's forward level loop: level L expands the
+// queue entries order[a, z) (level L - 1) and appends level L at order[z, ...);
+// lvl[k] is the start of level k in order[].
+struct VmCtl {
+    int32_t L;
+    uint32_t done;
+    uint32_t fresh;  // no level has run yet
+    uint32_t big;    // some level's path-count sums reached 2^53
+    unsigned long long a, z;
+    unsigned long long edges;
+    int32_t levels;  // when done: the reference loop's level count
+    int32_t pad;
+};
+
+// Closes the level just expanded (its length from the counters) and opens the
+// next, or marks the loop done once a level came out empty.
+__global__ void k_vm_level_step(VmCtl* ctl, VmCtr* ctr, unsigned long long* lvl) {
+    VmCtl c = *ctl;
+    if (c.done) return;
+    if (!c.fresh) {
+        const unsigned long long len = ctr->len;
+        c.edges += ctr->edges;
+        if (ctr->big) c.big = 1;
+        lvl[c.L + 1] = c.z + len;
+        c.a = c.z;
+        c.z = c.z + len;
+        c.L += 1;
+    }
+    c.fresh = 0;
+    if (c.z == c.a) {
+        c.done = 1;
+        c.levels = c.L;
+    }
+    *ctr = VmCtr{};
+    *ctl = c;
+}
+
 __global__ void k_vm_init(int32_t* dist, double* sigma, double* delta, uint32_t* mark, uint64_t n) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n * 64;
          i += (uint64_t)gridDim.x * blockDim.x) {
@@ -249,7 +286,7 @@ __global__ void k_vm_init(int32_t* dist, double* sigma, double* delta, uint32_t*
 }
 
 __global__ void k_vm_seed(const uint32_t* src, uint32_t nb, uint64_t n, int32_t* dist, double* sigma, uint32_t* mark,
-                          uint32_t* order, VmCtr* ctr) {
+                          uint32_t* order, VmCtr* ctr, VmCtl* ctl, unsigned long long* lvl) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     uint32_t len = 0;
     for (uint32_t b = 0; b < nb; ++b) {
@@ -262,7 +299,15 @@ __global__ void k_vm_seed(const uint32_t* src, uint32_t nb, uint64_t n, int32_t*
             order[len++] = s;
         }
     }
-    ctr->len = len;
+    *ctr = VmCtr{};
+    VmCtl c{};
+    c.L = 1;
+    c.fresh = 1;
+    c.a = 0;
+    c.z = len;
+    *ctl = c;
+    lvl[0] = 0;
+    lvl[1] = len;
 }
 
 // A wave's newly queued vertices, appended 64 at a time: lane i holds the
@@ -330,10 +375,15 @@ __device__ __forceinline__ void vm_expand(uint32_t v, uint32_t e0, uint32_t e1, 
     }
 }
 
-__global__ __launch_bounds__(BB) void k_vm_forward(const uint32_t* __restrict__ q, uint32_t qlen,
+__global__ __launch_bounds__(BB) void k_vm_forward(uint32_t* __restrict__ order, const VmCtl* ctl,
                                                    const uint32_t* __restrict__ off, const uint32_t* __restrict__ tgt,
-                                                   int32_t* __restrict__ dist, double* __restrict__ sigma, int32_t L,
-                                                   uint32_t* __restrict__ mark, uint32_t* __restrict__ nq, VmCtr* ctr) {
+                                                   int32_t* __restrict__ dist, double* __restrict__ sigma,
+                                                   uint32_t* __restrict__ mark, VmCtr* ctr) {
+    if (ctl->done) return;
+    const uint32_t* q = order + ctl->a;
+    uint32_t* nq = order + ctl->z;
+    const uint64_t qlen = ctl->z - ctl->a;
+    const int32_t L = ctl->L;
     const uint64_t gw = blockIdx.x * (uint64_t)BW + (threadIdx.x >> 6);
     const uint64_t nwaves = (uint64_t)gridDim.x * BW;
     unsigned long long edges = 0;
@@ -368,9 +418,11 @@ __global__ __launch_bounds__(BB) void k_vm_forward_heavy(const uint32_t* __restr
                                                          const uint32_t* __restrict__ cbase, uint32_t nh,
                                                          uint32_t chunks, const uint32_t* __restrict__ off,
                                                          const uint32_t* __restrict__ tgt, int32_t* __restrict__ dist,
-                                                         double* __restrict__ sigma, int32_t L,
-                                                         uint32_t* __restrict__ mark, uint32_t* __restrict__ nq,
-                                                         VmCtr* ctr) {
+                                                         double* __restrict__ sigma, uint32_t* __restrict__ mark,
+                                                         uint32_t* __restrict__ order, const VmCtl* ctl, VmCtr* ctr) {
+    if (ctl->done) return;
+    uint32_t* nq = order + ctl->z;
+    const int32_t L = ctl->L;
     const uint64_t gw = blockIdx.x * (uint64_t)BW + (threadIdx.x >> 6);
     const uint64_t nwaves = (uint64_t)gridDim.x * BW;
     unsigned long long edges = 0;
@@ -392,11 +444,14 @@ __global__ __launch_bounds__(BB) void k_vm_forward_heavy(const uint32_t* __restr
 // Ordered recount of a level's path counts (vertex-major layout): one wave per
 // vertex w queued at level L; lane b with dist == L sums sigma over w's
 // in-slots u at level L - 1 in in-CSR slot order.
-__global__ __launch_bounds__(BB) void k_vm_sigma_pull(const uint32_t* __restrict__ q, uint32_t qlen,
-                                                      const uint32_t* __restrict__ in_off,
+__global__ __launch_bounds__(BB) void k_vm_sigma_pull(const uint32_t* __restrict__ order, const VmCtl* ctl,
+                                                      const VmCtr* ctr, const uint32_t* __restrict__ in_off,
                                                       const uint32_t* __restrict__ in_tgt,
-                                                      const int32_t* __restrict__ dist, double* __restrict__ sigma,
-                                                      int32_t L) {
+                                                      const int32_t* __restrict__ dist, double* __restrict__ sigma) {
+    if (ctl->done || !ctr->big) return;
+    const uint32_t* q = order + ctl->z;
+    const uint64_t qlen = ctr->len;
+    const int32_t L = ctl->L;
     const uint64_t gw = blockIdx.x * (uint64_t)BW + (threadIdx.x >> 6);
     const uint64_t nwaves = (uint64_t)gridDim.x * BW;
     const int b = lane_id();
@@ -562,52 +617,72 @@ static void betweenness_vm(ursa_hip_topo* t, const std::vector<uint32_t>& src, d
     DevBuf<double> sigma(n * 64), delta(n * 64), part((uint64_t)std::max<uint32_t>(nchunks, 1) * 64);
     DevBuf<uint32_t> mark(n), order(n * 64), dsrc(64);
     DevBuf<VmCtr> ctr(1);
-    VmCtr* h = nullptr;
-    HIP_OK(hipHostMalloc((void**)&h, sizeof(VmCtr), hipHostMallocDefault));
+    DevBuf<VmCtl> ctl(1);
+    DevBuf<unsigned long long> dlvl(n + 2);
+    VmCtl* h = nullptr;
+    HIP_OK(hipHostMalloc((void**)&h, sizeof(VmCtl), hipHostMallocDefault));
     auto read = [&]() {
-        HIP_OK(hipMemcpyAsync(h, ctr.get(), sizeof(VmCtr), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(h, ctl.get(), sizeof(VmCtl), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
         return *h;
     };
     uint64_t total_edges = 0;
     const unsigned hgrid = blocks_for(std::max<uint32_t>(nchunks, 1), BW, GRID_CAP);
+    const unsigned fgrid = blocks_for(n, BW, GRID_CAP);
+    // Forward levels in groups of 8, every launch reading its level, queue range
+    // and counters from the control block: one recorded graph replayed for every
+    // group of every batch. Each vertex enters the list of a level at most once
+    // and reaches at most 64 distinct levels over the batch, so order[] (64 n)
+    // cannot overflow. `recount` adds the ordered path-count pull after each
+    // level (taken only when a sum reached 2^53).
+    const DevCsr* in_csr = nullptr;
+    auto group = [&](bool recount) {
+        for (int i = 0; i < 8; ++i) {
+            k_vm_level_step<<<1, 1, 0, s>>>(ctl.get(), ctr.get(), dlvl.get());
+            timed("brandes_forward", 0.0, s, [&] {
+                k_vm_forward<<<fgrid, BB, 0, s>>>(order.get(), ctl.get(), o.off.get(), o.tgt.get(), dist.get(),
+                                                  sigma.get(), mark.get(), ctr.get());
+                if (nchunks)
+                    k_vm_forward_heavy<<<hgrid, BB, 0, s>>>(heavy.get(), cbase.get(), nh, nchunks, o.off.get(),
+                                                            o.tgt.get(), dist.get(), sigma.get(), mark.get(),
+                                                            order.get(), ctl.get(), ctr.get());
+            });
+            if (recount)
+                k_vm_sigma_pull<<<fgrid, BB, 0, s>>>(order.get(), ctl.get(), ctr.get(), in_csr->off.get(),
+                                                     in_csr->tgt.get(), dist.get(), sigma.get());
+            check_launch();
+        }
+    };
+    CapturedLaunches fast;
     try {
         for (uint64_t b0 = 0; b0 < src.size(); b0 += 64) {
             const uint32_t nb = (uint32_t)std::min<uint64_t>(64, src.size() - b0);
-            k_vm_init<<<blocks_for(n * 64, 256, 1u << 16), 256, 0, s>>>(dist.get(), sigma.get(), delta.get(),
-                                                                       mark.get(), n);
             HIP_OK(hipMemcpyAsync(dsrc.get(), src.data() + b0, nb * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-            HIP_OK(hipMemsetAsync(ctr.get(), 0, sizeof(VmCtr), s));
-            k_vm_seed<<<1, 1, 0, s>>>(dsrc.get(), nb, n, dist.get(), sigma.get(), mark.get(), order.get(), ctr.get());
-            check_launch();
-            std::vector<uint64_t> lvl{0, read().len};
-            for (int32_t L = 1;; ++L) {
-                const uint64_t a = lvl[L - 1], z = lvl[L];
-                if (z == a) break;
-                // Each vertex enters the list of a level at most once and reaches at
-                // most 64 distinct levels over the batch, so order[] (64 n) cannot overflow.
-                HIP_OK(hipMemsetAsync(ctr.get(), 0, sizeof(VmCtr), s));
-                const uint32_t qlen = (uint32_t)(z - a);
-                timed("brandes_forward", 0.0, s, [&] {
-                    k_vm_forward<<<blocks_for(qlen, BW, GRID_CAP), BB, 0, s>>>(order.get() + a, qlen, o.off.get(),
-                                                                             o.tgt.get(), dist.get(), sigma.get(), L,
-                                                                             mark.get(), order.get() + z, ctr.get());
-                    if (nchunks)
-                        k_vm_forward_heavy<<<hgrid, BB, 0, s>>>(heavy.get(), cbase.get(), nh, nchunks, o.off.get(),
-                                                                o.tgt.get(), dist.get(), sigma.get(), L, mark.get(),
-                                                                order.get() + z, ctr.get());
-                });
+            VmCtl c{};
+            for (int pass = 0; pass < 2; ++pass) {
+                const bool recount = pass == 1;
+                k_vm_init<<<blocks_for(n * 64, 256, 1u << 16), 256, 0, s>>>(dist.get(), sigma.get(), delta.get(),
+                                                                           mark.get(), n);
+                k_vm_seed<<<1, 1, 0, s>>>(dsrc.get(), nb, n, dist.get(), sigma.get(), mark.get(), order.get(), ctr.get(),
+                                          ctl.get(), dlvl.get());
                 check_launch();
-                const VmCtr c = read();
-                total_edges += c.edges;
-                if (c.big && c.len) {
-                    const DevCsr& in = t->incoming(s);
-                    k_vm_sigma_pull<<<blocks_for(c.len, BW, GRID_CAP), BB, 0, s>>>(
-                        order.get() + z, c.len, in.off.get(), in.tgt.get(), dist.get(), sigma.get(), L);
-                    check_launch();
+                if (recount) {
+                    in_csr = &t->incoming(s);
+                    do {
+                        group(true);
+                    } while (!read().done);
+                } else {
+                    do {
+                        fast.run(s, [&] { group(false); });
+                    } while (!read().done);
                 }
-                lvl.push_back(z + c.len);
+                c = *h;
+                if (!c.big) break;  // no sum reached 2^53: the counts are exact
             }
+            total_edges += c.edges;
+            std::vector<uint64_t> lvl((size_t)c.levels + 1);
+            HIP_OK(hipMemcpyAsync(lvl.data(), dlvl.get(), lvl.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+            HIP_OK(hipStreamSynchronize(s));
             const int32_t levels = (int32_t)lvl.size() - 1;
             for (int32_t L = levels - 2; L >= 1; --L) {
                 const uint64_t a = lvl[L], z = lvl[L + 1];

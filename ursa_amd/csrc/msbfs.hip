@@ -16,6 +16,14 @@
 // same i32 array the reference's serial loop gives for each source, for any mix
 // of push and pull levels. All 64 traversals share each edge scan.
 //
+// The level loop runs without the host: a one-thread kernel closes the last
+// level and opens the next (level number, done test, compaction, push or
+// pull) in a device control block, every level kernel reads its arguments from
+// that block and returns at once when it is not chosen, and a level's push or
+// pull is one expand launch (plus one for its heavy vertices). The host queues
+// levels four at a time (recorded once as a HIP graph and replayed for every
+// group of every batch) and reads the done flag once per group.
+//
 // Algorithmic bytes counted per source as in single-source BFS (4 B per
 // reached edge + 12 B per node); the batch shares them.
 #include <functional>
@@ -40,6 +48,23 @@ struct MsCtr {
 };
 
 constexpr uint32_t MS_HEAVY = 1024;
+constexpr int MS_LEVELS_PER_GROUP = 4;
+
+// Device-side state of one batch's level loop.
+struct MsCtl {
+    int32_t L;                  // the level being built
+    uint32_t done;
+    uint32_t push;              // this level pushes (else it pulls)
+    uint32_t compact;           // this level first compacts the frontier into a queue
+    uint32_t qsel;              // queue buffer (0 / 1) holding the frontier queue
+    uint32_t have_queue;        // the frontier is known as a queue
+    uint32_t known_edges;       // fedges is exact
+    uint32_t fresh;             // no level has run yet (nothing to close)
+    unsigned long long qlen;    // frontier queue length, or after a pull the settled-pair count
+    unsigned long long fedges;  // forward degree sum of the frontier queue
+    unsigned long long valid;   // lanes whose source is in range
+    int32_t* levels;            // the batch's level rows
+};
 
 __device__ __forceinline__ void write_levels(uint64_t bits, int32_t* levels, uint64_t n, uint64_t v, int32_t L) {
     while (bits) {
@@ -70,15 +95,17 @@ __global__ void k_ms_fill_levels(int32_t* levels, uint64_t total) {
         levels[i] = -1;
 }
 
-// Seed: visit/front bits of each source, level 0, unique source queue.
+// Seed: visit/front bits of each source, level 0, unique source queue (in
+// queue buffer 0), and the control block of the batch's level loop.
 __global__ void k_ms_seed(const uint32_t* src, uint32_t nb, uint64_t n, NbrView fwd, unsigned long long* visit,
-                          unsigned long long* front, int32_t* levels, uint32_t* queue, MsCtr* ctr) {
+                          unsigned long long* front, int32_t* levels, uint32_t* queue, MsCtl* ctl) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     uint32_t q = 0;
-    unsigned long long edges = 0;
+    unsigned long long edges = 0, valid = 0;
     for (uint32_t b = 0; b < nb; ++b) {
         const uint32_t s = src[b];
         if (s >= n) continue;
+        valid |= 1ull << b;
         if (front[s] == 0) {
             queue[q++] = s;
             edges += nv_degree(fwd, s);
@@ -87,8 +114,83 @@ __global__ void k_ms_seed(const uint32_t* src, uint32_t nb, uint64_t n, NbrView 
         visit[s] |= 1ull << b;
         levels[(uint64_t)b * n + s] = 0;
     }
-    ctr->qlen = q;
-    ctr->edges = edges;
+    MsCtl c{};
+    c.L = 0;
+    c.qsel = 0;
+    c.have_queue = 1;
+    c.known_edges = 1;
+    c.qlen = q;
+    c.fedges = edges;
+    c.valid = valid;
+    c.levels = levels;
+    c.fresh = 1;
+    *ctl = c;
+}
+
+__global__ void k_ms_zero_u64(unsigned long long* p, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// Closes the previous level (the next frontier's size from its counters) and
+// opens the next: its number, the done test (empty frontier), and whether it
+// compacts first and whether it pushes or pulls. A level pushes while its
+// frontier's forward edges number fewer than fwd_edges / push_div and the
+// frontier is a known queue, and pulls otherwise; after a pull only the
+// settled-pair count is known, which bounds the new frontier's vertex count,
+// so pulls continue until that count shows a small frontier, and a compaction
+// then measures it.
+__global__ void k_ms_level_step(MsCtl* ctl, MsCtr* ctr, uint64_t n, unsigned long long fwd_edges,
+                                uint32_t push_div) {
+    MsCtl c = *ctl;
+    if (c.done) return;
+    if (!c.fresh) {
+        if (c.push) {
+            c.qlen = ctr->qlen;
+            c.fedges = ctr->edges;
+            c.known_edges = 1;
+            c.have_queue = 1;
+            c.qsel ^= 1u;
+        } else {
+            c.qlen = ctr->newbits;
+            c.have_queue = 0;
+            c.known_edges = 0;
+        }
+    }
+    c.fresh = 0;
+    *ctr = MsCtr{};
+    c.L += 1;
+    if (c.qlen == 0) {
+        c.done = 1;
+    } else {
+        c.compact = !c.have_queue && c.qlen * 64 < n;
+        if (!c.compact)
+            c.push = c.have_queue && c.known_edges ? c.fedges * push_div < fwd_edges : (c.have_queue && c.L == 1);
+    }
+    *ctl = c;
+}
+
+// After a compaction: the measured queue decides push or pull.
+__global__ void k_ms_after_compact(MsCtl* ctl, MsCtr* ctr, unsigned long long fwd_edges, uint32_t push_div) {
+    MsCtl c = *ctl;
+    if (c.done || !c.compact) return;
+    c.qlen = ctr->qlen;
+    c.fedges = ctr->edges;
+    c.have_queue = 1;
+    c.known_edges = 1;
+    if (c.qlen == 0)
+        c.done = 1;
+    else
+        c.push = c.fedges * push_div < fwd_edges;
+    *ctr = MsCtr{};
+    *ctl = c;
+}
+
+// next[] = 0 before a push level.
+__global__ void k_ms_clear_next(const MsCtl* ctl, unsigned long long* next, uint64_t n) {
+    if (ctl->done || !ctl->push) return;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        next[i] = 0;
 }
 
 // Pull-side vertices with more reverse edges than this are ORed chunk by chunk
@@ -114,10 +216,13 @@ __device__ __forceinline__ unsigned long long ms_pull_list(const uint32_t* __res
     return acc;
 }
 
-__global__ __launch_bounds__(MB) void k_ms_pull(uint64_t n, NbrView rev, const unsigned long long* __restrict__ front,
+__device__ __forceinline__ void ms_pull(uint64_t n, NbrView rev, const unsigned long long* __restrict__ front,
                                                 unsigned long long* __restrict__ next,
-                                                unsigned long long* __restrict__ visit, unsigned long long valid,
-                                                int32_t* __restrict__ levels, int32_t L, MsCtr* ctr) {
+                                                unsigned long long* __restrict__ visit, const MsCtl* ctl,
+                                                MsCtr* ctr) {
+    const unsigned long long valid = ctl->valid;
+    int32_t* levels = ctl->levels;
+    const int32_t L = ctl->L;
     unsigned long long cnt = 0;
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
         if (nv_degree(rev, (uint32_t)v) > PULL_HEAVY) continue;  // k_ms_pull_heavy + k_ms_pull_finish
@@ -142,11 +247,12 @@ __global__ __launch_bounds__(MB) void k_ms_pull(uint64_t n, NbrView rev, const u
 
 // One wave per chunk of a heavy vertex's reverse list (out list then in list
 // for Both): OR of the frontier masks, merged into hacc[h] with atomic OR.
-__global__ __launch_bounds__(MB) void k_ms_pull_heavy(const uint32_t* __restrict__ heavy,
+__device__ __forceinline__ void ms_pull_heavy(const uint32_t* __restrict__ heavy,
                                                       const uint32_t* __restrict__ cbase, uint32_t nh, uint32_t chunks,
                                                       NbrView rev, const unsigned long long* __restrict__ front,
-                                                      const unsigned long long* __restrict__ visit,
-                                                      unsigned long long valid, unsigned long long* __restrict__ hacc) {
+                                                      const unsigned long long* __restrict__ visit, const MsCtl* ctl,
+                                                      unsigned long long* __restrict__ hacc) {
+    const unsigned long long valid = ctl->valid;
     const uint64_t gw = blockIdx.x * (uint64_t)MW + (threadIdx.x >> 6);
     const uint64_t nwaves = (uint64_t)gridDim.x * MW;
     for (uint64_t c = gw; c < chunks; c += nwaves) {
@@ -178,8 +284,10 @@ __global__ __launch_bounds__(MB) void k_ms_pull_heavy(const uint32_t* __restrict
 
 __global__ void k_ms_pull_finish(const uint32_t* __restrict__ heavy, uint32_t nh,
                                  unsigned long long* __restrict__ hacc, unsigned long long* __restrict__ next,
-                                 unsigned long long* __restrict__ visit, int32_t* __restrict__ levels, uint64_t n,
-                                 int32_t L, MsCtr* ctr) {
+                                 unsigned long long* __restrict__ visit, uint64_t n, const MsCtl* ctl, MsCtr* ctr) {
+    if (ctl->done || ctl->push) return;
+    int32_t* levels = ctl->levels;
+    const int32_t L = ctl->L;
     const uint64_t h = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     unsigned long long cnt = 0;
     if (h < nh) {
@@ -212,12 +320,16 @@ __global__ void k_ms_gather(const uint32_t* __restrict__ idx, uint32_t cnt, cons
     if (i < cnt) out[i] = src[idx[i]];
 }
 
-__global__ __launch_bounds__(MB) void k_ms_push(const uint32_t* __restrict__ q, uint32_t qlen, NbrView fwd,
+__device__ __forceinline__ void ms_push(uint32_t* __restrict__ q0, uint32_t* __restrict__ q1, NbrView fwd,
                                                 const unsigned long long* __restrict__ front,
                                                 unsigned long long* __restrict__ next,
-                                                unsigned long long* __restrict__ visit, int32_t* __restrict__ levels,
-                                                uint64_t n, int32_t L, uint32_t* __restrict__ nq, MsCtr* ctr,
-                                                uint32_t* __restrict__ heavy) {
+                                                unsigned long long* __restrict__ visit, uint64_t n, const MsCtl* ctl,
+                                                MsCtr* ctr, uint32_t* __restrict__ heavy) {
+    const uint32_t* q = ctl->qsel ? q1 : q0;
+    uint32_t* nq = ctl->qsel ? q0 : q1;
+    const uint64_t qlen = ctl->qlen;
+    int32_t* levels = ctl->levels;
+    const int32_t L = ctl->L;
     const int lane = lane_id();
     const uint64_t gw = blockIdx.x * (uint64_t)MW + (threadIdx.x >> 6);
     const uint64_t nwaves = (uint64_t)gridDim.x * MW;
@@ -271,12 +383,15 @@ __global__ __launch_bounds__(MB) void k_ms_push(const uint32_t* __restrict__ q, 
 
 // Heavy push-frontier vertices: every wave of the grid takes a stride of each
 // heavy vertex's edges.
-__global__ __launch_bounds__(MB) void k_ms_push_heavy(const uint32_t* __restrict__ heavy, NbrView fwd,
+__device__ __forceinline__ void ms_push_heavy(const uint32_t* __restrict__ heavy, NbrView fwd,
                                                       const unsigned long long* __restrict__ front,
                                                       unsigned long long* __restrict__ next,
-                                                      unsigned long long* __restrict__ visit,
-                                                      int32_t* __restrict__ levels, uint64_t n, int32_t L,
-                                                      uint32_t* __restrict__ nq, MsCtr* ctr) {
+                                                      unsigned long long* __restrict__ visit, uint64_t n,
+                                                      uint32_t* __restrict__ q0, uint32_t* __restrict__ q1,
+                                                      const MsCtl* ctl, MsCtr* ctr) {
+    uint32_t* nq = ctl->qsel ? q0 : q1;
+    int32_t* levels = ctl->levels;
+    const int32_t L = ctl->L;
     const uint32_t cnt_h = ctr->heavy;
     const uint64_t wave_base = (blockIdx.x * (uint64_t)MW + (threadIdx.x >> 6)) * 64;
     const uint64_t stride = (uint64_t)gridDim.x * MB;
@@ -311,9 +426,44 @@ __global__ __launch_bounds__(MB) void k_ms_push_heavy(const uint32_t* __restrict
     }
 }
 
+// One level's expansion: the push over the frontier queue or the pull over all
+// vertices, as the control block chose (uniform over the grid).
+__global__ __launch_bounds__(MB) void k_ms_expand(uint32_t* __restrict__ q0, uint32_t* __restrict__ q1, NbrView fwd,
+                                                  NbrView rev, const unsigned long long* __restrict__ front,
+                                                  unsigned long long* __restrict__ next,
+                                                  unsigned long long* __restrict__ visit, uint64_t n, const MsCtl* ctl,
+                                                  MsCtr* ctr, uint32_t* __restrict__ heavy) {
+    if (ctl->done) return;
+    if (ctl->push)
+        ms_push(q0, q1, fwd, front, next, visit, n, ctl, ctr, heavy);
+    else
+        ms_pull(n, rev, front, next, visit, ctl, ctr);
+}
+
+// The heavy vertices of the level: the push frontier's long forward lists
+// (queued by the push) or the pull side's long reverse lists, chunk by chunk.
+__global__ __launch_bounds__(MB) void k_ms_expand_heavy(const uint32_t* __restrict__ heavy, NbrView fwd, NbrView rev,
+                                                        const unsigned long long* __restrict__ front,
+                                                        unsigned long long* __restrict__ next,
+                                                        unsigned long long* __restrict__ visit, uint64_t n,
+                                                        uint32_t* __restrict__ q0, uint32_t* __restrict__ q1,
+                                                        const uint32_t* __restrict__ pheavy,
+                                                        const uint32_t* __restrict__ pcbase, uint32_t pnh,
+                                                        uint32_t pchunks, unsigned long long* __restrict__ hacc,
+                                                        const MsCtl* ctl, MsCtr* ctr) {
+    if (ctl->done) return;
+    if (ctl->push)
+        ms_push_heavy(heavy, fwd, front, next, visit, n, q0, q1, ctl, ctr);
+    else if (pnh)
+        ms_pull_heavy(pheavy, pcbase, pnh, pchunks, rev, front, visit, ctl, hacc);
+}
+
 // queue of vertices with a non-zero front mask (pull -> push hand-over).
 __global__ __launch_bounds__(MB) void k_ms_compact(const unsigned long long* __restrict__ front, uint64_t n,
-                                                   NbrView fwd, uint32_t* __restrict__ q, MsCtr* ctr) {
+                                                   NbrView fwd, uint32_t* __restrict__ q0, uint32_t* __restrict__ q1,
+                                                   const MsCtl* ctl, MsCtr* ctr) {
+    if (ctl->done || !ctl->compact) return;
+    uint32_t* q = ctl->qsel ? q1 : q0;
     __shared__ uint32_t s_cnt[MW + 1];
     unsigned long long edges = 0;
     for (uint64_t b0 = blockIdx.x * (uint64_t)MB; b0 < n; b0 += (uint64_t)gridDim.x * MB) {
@@ -344,7 +494,7 @@ void direction_views(ursa_hip_topo* t, int dir, hipStream_t s, NbrView& fwd, Nbr
 // Levels for up to 64 sources (row b of `levels` = source b) on one stream.
 // The pull side's heavy-vertex list is cached on the topology per direction;
 // the rest is per call (allocations are reused across calls by the device
-// allocator) and the counter readback goes through the thread's pinned scratch.
+// allocator), and the level groups' HIP graph is recorded once per call.
 struct MsWork {
     uint64_t n;
     DevBuf<unsigned long long> visit, f0, f1, redges, hacc;
@@ -353,11 +503,13 @@ struct MsWork {
     const uint32_t* pcbase = nullptr;
     uint32_t pnh = 0, pchunks = 0;  // pull-side heavy vertices and their chunk count
     DevBuf<MsCtr> ctr;
+    DevBuf<MsCtl> ctl;
+    CapturedLaunches group;
     explicit MsWork(uint64_t n_)
-        : n(n_), visit(n_), f0(n_), f1(n_), redges(1), q0(n_), q1(n_), dsrc(64), heavy(n_), ctr(1) {}
-    MsCtr read(hipStream_t s) {
-        MsCtr* h = static_cast<MsCtr*>(pinned_scratch(sizeof(MsCtr)));
-        HIP_OK(hipMemcpyAsync(h, ctr.get(), sizeof(MsCtr), hipMemcpyDeviceToHost, s));
+        : n(n_), visit(n_), f0(n_), f1(n_), redges(1), q0(n_), q1(n_), dsrc(64), heavy(n_), ctr(1), ctl(1) {}
+    MsCtl read_ctl(hipStream_t s) {
+        MsCtl* h = static_cast<MsCtl*>(pinned_scratch(sizeof(MsCtl)));
+        HIP_OK(hipMemcpyAsync(h, ctl.get(), sizeof(MsCtl), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
         return *h;
     }
@@ -407,88 +559,48 @@ struct MsWork {
     }
 };
 
-// One batch of nb <= 64 sources. A level pushes while its frontier's forward
-// edges number fewer than fwd_edges / push_div (and the frontier is a known
-// queue), and pulls otherwise.
+// One batch of nb <= 64 sources: seed, then groups of MS_LEVELS_PER_GROUP
+// levels until the control block reports an empty frontier. Every launch in a
+// group has the same arguments for every group and batch of the call (the
+// level, the direction and the batch's level rows come from the control
+// block), so the group is one replayed graph.
 void msbfs_batch(uint64_t n, uint64_t fwd_edges, const uint32_t* src_host, uint32_t nb, const NbrView& fwd,
                  const NbrView& rev, int32_t* levels, MsWork& w, hipStream_t s, uint64_t* reached_edges,
                  uint32_t push_div) {
     const unsigned gv = blocks_for(n, MB, GRID_CAP);
     k_ms_fill_levels<<<blocks_for(n * nb, 256, 1u << 16), 256, 0, s>>>(levels, n * nb);
-    HIP_OK(hipMemsetAsync(w.visit.get(), 0, n * sizeof(unsigned long long), s));
-    HIP_OK(hipMemsetAsync(w.f0.get(), 0, n * sizeof(unsigned long long), s));
+    k_ms_zero_u64<<<gv, MB, 0, s>>>(w.visit.get(), n);
+    k_ms_zero_u64<<<gv, MB, 0, s>>>(w.f0.get(), n);
     HIP_OK(hipMemcpyAsync(w.dsrc.get(), src_host, nb * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    HIP_OK(hipMemsetAsync(w.ctr.get(), 0, sizeof(MsCtr), s));
-    k_ms_seed<<<1, 1, 0, s>>>(w.dsrc.get(), nb, n, fwd, w.visit.get(), w.f0.get(), levels, w.q0.get(), w.ctr.get());
+    k_ms_seed<<<1, 1, 0, s>>>(w.dsrc.get(), nb, n, fwd, w.visit.get(), w.f0.get(), levels, w.q0.get(), w.ctl.get());
     check_launch();
-    unsigned long long valid = 0;
-    for (uint32_t b = 0; b < nb; ++b)
-        if (src_host[b] < n) valid |= 1ull << b;
-    unsigned long long* front = w.f0.get();
-    unsigned long long* next = w.f1.get();
-    uint32_t* q = w.q0.get();
-    uint32_t* nq = w.q1.get();
-    MsCtr c0 = w.read(s);
-    uint64_t qlen = c0.qlen;
-    uint64_t fedges = c0.edges;
-    bool have_queue = true;
-    bool known_edges = true;
-    for (int32_t L = 1; qlen > 0; ++L) {
-        // Frontier edge count for the direction choice: known after a push or a
-        // compaction; after a pull it is only bounded, so pull continues until
-        // the settled count shows a small frontier, then a compaction measures it.
-        bool push = have_queue && known_edges ? fedges * (uint64_t)push_div < fwd_edges
-                                              : (have_queue && L == 1);
-        if (!have_queue && qlen * 64 < n) {
-            HIP_OK(hipMemsetAsync(w.ctr.get(), 0, sizeof(MsCtr), s));
-            k_ms_compact<<<gv, MB, 0, s>>>(front, n, fwd, q, w.ctr.get());
-            check_launch();
-            const MsCtr c = w.read(s);
-            qlen = c.qlen;
-            fedges = c.edges;
-            have_queue = true;
-            known_edges = true;
-            if (qlen == 0) break;
-            push = fedges * (uint64_t)push_div < fwd_edges;
-        }
-        HIP_OK(hipMemsetAsync(w.ctr.get(), 0, sizeof(MsCtr), s));
-        if (push) {
-            HIP_OK(hipMemsetAsync(next, 0, n * sizeof(unsigned long long), s));
-            timed("msbfs_push", 0.0, s, [&] {
-                k_ms_push<<<blocks_for((qlen + 63) / 64, MW, 1u << 16), MB, 0, s>>>(q, (uint32_t)qlen, fwd, front, next,
-                                                                                  w.visit.get(), levels, n, L, nq,
-                                                                                  w.ctr.get(), w.heavy.get());
-                if (fedges > MS_HEAVY)
-                    k_ms_push_heavy<<<blocks_for(std::max<uint64_t>(fedges, 64), MB, 2048), MB, 0, s>>>(
-                        w.heavy.get(), fwd, front, next, w.visit.get(), levels, n, L, nq, w.ctr.get());
-            });
-            check_launch();
-            const MsCtr c = w.read(s);
-            qlen = c.qlen;
-            fedges = c.edges;
-            known_edges = true;
-            std::swap(q, nq);
-            have_queue = true;
-        } else {
-            timed("msbfs_pull", 0.0, s, [&] {
-                k_ms_pull<<<gv, MB, 0, s>>>(n, rev, front, next, w.visit.get(), valid, levels, L, w.ctr.get());
-                if (w.pnh) {
-                    k_ms_pull_heavy<<<blocks_for(w.pchunks, MW, 1u << 16), MB, 0, s>>>(
-                        w.pheavy, w.pcbase, w.pnh, w.pchunks, rev, front, w.visit.get(), valid,
-                        w.hacc.get());
+    MsCtl* ctl = w.ctl.get();
+    MsCtr* ctr = w.ctr.get();
+    auto group = [&] {
+        for (int i = 0; i < MS_LEVELS_PER_GROUP; ++i) {
+            // Level L = 2 g + i + 1 of any group g: its frontier is in f0 when L is odd.
+            unsigned long long* front = (i & 1) ? w.f1.get() : w.f0.get();
+            unsigned long long* next = (i & 1) ? w.f0.get() : w.f1.get();
+            k_ms_level_step<<<1, 1, 0, s>>>(ctl, ctr, n, fwd_edges, push_div);
+            k_ms_compact<<<gv, MB, 0, s>>>(front, n, fwd, w.q0.get(), w.q1.get(), ctl, ctr);
+            k_ms_after_compact<<<1, 1, 0, s>>>(ctl, ctr, fwd_edges, push_div);
+            k_ms_clear_next<<<gv, MB, 0, s>>>(ctl, next, n);
+            timed("msbfs_expand", 0.0, s, [&] {
+                k_ms_expand<<<GRID_CAP, MB, 0, s>>>(w.q0.get(), w.q1.get(), fwd, rev, front, next, w.visit.get(), n,
+                                                    ctl, ctr, w.heavy.get());
+                k_ms_expand_heavy<<<GRID_CAP, MB, 0, s>>>(w.heavy.get(), fwd, rev, front, next, w.visit.get(), n,
+                                                          w.q0.get(), w.q1.get(), w.pheavy, w.pcbase, w.pnh,
+                                                          w.pchunks, w.hacc.get(), ctl, ctr);
+                if (w.pnh)
                     k_ms_pull_finish<<<blocks_for(w.pnh, 256), 256, 0, s>>>(w.pheavy, w.pnh, w.hacc.get(), next,
-                                                                           w.visit.get(), levels, n, L, w.ctr.get());
-                }
+                                                                           w.visit.get(), n, ctl, ctr);
             });
             check_launch();
-            const MsCtr c = w.read(s);
-            // After a pull only the settled-pair count is known; it bounds the
-            // new frontier's vertex count.
-            qlen = c.newbits;
-            have_queue = false;
-            known_edges = false;
         }
-        std::swap(front, next);
+    };
+    for (;;) {
+        w.group.run(s, group);
+        if (w.read_ctl(s).done) break;
     }
     if (reached_edges) {
         HIP_OK(hipMemsetAsync(w.redges.get(), 0, sizeof(unsigned long long), s));
