@@ -486,9 +486,9 @@ __global__ void k_pr_ctl_init(PrCtl* ctl, int32_t dm_e0, double base0) {
 
 // out_str[v] = sum of v's out-edge weights (pagerank.rs:148-150): in out-CSR
 // order for vertices with at most OS_LIGHT out-edges; a wave per heavier
-// vertex (k_out_strength_heavy) sums lane-strided pieces in slot order and
-// joins them in a fixed butterfly, so a hub's sum is not one thread's serial
-// walk. The order differs from the reference's left-to-right sum only for
+// vertex (k_out_strength_heavy) sums lane-strided pieces into eight partial
+// sums per lane and joins them in a fixed order and butterfly, so a hub's sum
+// is not one thread's serial walk. The order differs from the reference's left-to-right sum only for
 // heavy vertices (same terms, within a few ulps).
 constexpr uint32_t OS_LIGHT = 64;
 
@@ -520,8 +520,22 @@ __global__ __launch_bounds__(256) void k_out_strength_heavy(const uint32_t* __re
             mask &= mask - 1;
             const uint64_t v = v0 + j;
             const uint32_t a = out_off[v], b = out_off[v + 1];
-            double s = -0.0;
-            for (uint32_t k = a + lane; k < b; k += 64) s += w[out_eid[k]];
+            // Eight partial sums per lane (slots a + lane + 64 (8 i + j) go to
+            // partial j) keep eight weight gathers in flight; the partials are
+            // then joined in a fixed order.
+            double p[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) p[j] = -0.0;
+            uint32_t k = a + lane;
+            for (; k + 7 * 64 < b; k += 8 * 64) {
+                uint32_t e[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) e[j] = out_eid[k + j * 64];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) p[j] += w[e[j]];
+            }
+            for (int j = 0; k < b; k += 64, ++j) p[j] += w[out_eid[k]];
+            double s = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
             for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
             if (lane == 0) out_str[v] = s;
         }
@@ -756,6 +770,8 @@ static void pr_prepare(ursa_hip_topo* t, const DevCsr& in, hipStream_t s) {
     t->pr_chunks = nchunks;
     t->pr_ready = true;
 }
+
+void pagerank_prepare(ursa_hip_topo* t, hipStream_t s) { pr_prepare(t, t->incoming(s), s); }
 
 // Solve PageRank (f64 solve, or with `mixed` the refinement solve described at
 // the top of this file). Every sweep of either kind counts toward max_iter,

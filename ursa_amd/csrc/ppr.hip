@@ -9,9 +9,13 @@
 // stopping after the first iteration whose L1 delta is below tol. A query
 // with no seed in range returns x0 after zero iterations.
 //
-// Layout: the queries of one solve are lanes q of a vertex-major state, value
-// of vertex v for lane q at v*KP + q (KP = lane count rounded up to a power of
-// two), so one in-edge gathers KP adjacent values for all queries at once.
+// Layout: the sweeps run over PageRank's working copy of the in-CSR (vertices
+// renumbered with the most gathered sources first, each row's sources in
+// ascending new id; pagerank.hip), so the hot contributions share cache lines.
+// The queries of one solve are lanes q of a vertex-major state, value of
+// (renumbered) vertex v for lane q at v*KP + q (KP = lane count rounded up to
+// a power of two), so one in-edge gathers KP adjacent values for all queries at
+// once. Seeds come in and results go out in the original numbering.
 //
 // Precision: a lane with tol < 2e-6 and max_iter > 16 first runs f32 sweeps
 // with compensated (Neumaier) row sums until its delta is below 2e-6 or stops
@@ -413,16 +417,17 @@ __global__ void k_ppr_init(uint64_t n, const uint32_t* __restrict__ od, float* _
 }
 
 __global__ void k_ppr_seedbits(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ lane_of,
-                               uint32_t count, uint32_t* __restrict__ bits) {
+                               uint32_t count, const uint32_t* __restrict__ new_of_old, uint32_t* __restrict__ bits) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) atomicOr(&bits[seeds[i]], 1u << lane_of[i]);
+    if (i < count) atomicOr(&bits[new_of_old[seeds[i]]], 1u << lane_of[i]);
 }
 
 // Every lane ends on its f64 iterate (the uniform start for a lane that never ran).
 template <int KP>
-__global__ void k_ppr_out(uint64_t n, int q, const double* __restrict__ r64, double* __restrict__ out) {
+__global__ void k_ppr_out(uint64_t n, int q, const uint32_t* __restrict__ new_of_old, const double* __restrict__ r64,
+                          double* __restrict__ out) {
     for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n; u += (uint64_t)gridDim.x * blockDim.x)
-        out[u] = r64[u * KP + q];
+        out[u] = r64[(uint64_t)new_of_old[u] * KP + q];
 }
 
 int lanes_padded(int k) {
@@ -440,7 +445,7 @@ struct PprBufs {
 };
 
 template <int KP>
-void ppr_iterate(ursa_hip_topo* t, const DevCsr& in, PprBufs& B, uint32_t max_iter_all, hipStream_t s) {
+void ppr_iterate(ursa_hip_topo* t, PprBufs& B, uint32_t max_iter_all, hipStream_t s) {
     const uint64_t n = t->n;
     const uint32_t light_blocks = (uint32_t)((n + PPR_ROWS - 1) / PPR_ROWS);
     const uint32_t seg_blocks = (uint32_t)((t->ppr_S + PPR_GROUPS - 1) / PPR_GROUPS);
@@ -461,14 +466,14 @@ void ppr_iterate(ursa_hip_topo* t, const DevCsr& in, PprBufs& B, uint32_t max_it
                 double* c64n = p ? B.c64a.get() : B.c64b.get();
                 timed("ppr_rows", edge_bytes * (double)t->m + (16.0 + 8.0 * KP) * (double)n, s, [&] {
                     k_ppr_rows<KP><<<light_blocks + seg_blocks, PB, 0, s>>>(
-                        n, in.off.get(), in.tgt.get(), t->out_deg.get(), B.seedbits.get(), B.lanes.get(), c32,
+                        n, t->pr_off.get(), t->pr_tgt.get(), t->pr_outdeg.get(), B.seedbits.get(), B.lanes.get(), c32,
                         c32n, B.r32.get(), c64, c64n, B.r64.get(), B.part.get(), light_blocks, t->ppr_S,
                         t->ppr_sbeg.get(), t->ppr_send.get(), B.spart.get());
                 });
                 check_launch();
                 if (t->ppr_H) {
                     k_ppr_heavy<KP><<<heavy_blocks, PB, 0, s>>>(t->ppr_H, t->ppr_hrow.get(), t->ppr_hsoff.get(),
-                                                                B.spart.get(), t->out_deg.get(), B.seedbits.get(),
+                                                                B.spart.get(), t->pr_outdeg.get(), B.seedbits.get(),
                                                                 B.lanes.get(), c32n, B.r32.get(), c64n, B.r64.get(),
                                                                 B.part.get() + (uint64_t)light_blocks * KP * 2);
                     check_launch();
@@ -490,9 +495,9 @@ void ppr_iterate(ursa_hip_topo* t, const DevCsr& in, PprBufs& B, uint32_t max_it
 }
 
 template <int KP>
-void ppr_write_out(uint64_t n, PprBufs& B, int k, double* const* dst, hipStream_t s) {
+void ppr_write_out(const ursa_hip_topo* t, uint64_t n, PprBufs& B, int k, double* const* dst, hipStream_t s) {
     for (int q = 0; q < k; ++q) {
-        k_ppr_out<KP><<<blocks_for(n, 256, 4096), 256, 0, s>>>(n, q, B.r64.get(), dst[q]);
+        k_ppr_out<KP><<<blocks_for(n, 256, 4096), 256, 0, s>>>(n, q, t->pr_new_of_old.get(), B.r64.get(), dst[q]);
         check_launch();
     }
 }
@@ -501,13 +506,13 @@ void ppr_write_out(uint64_t n, PprBufs& B, int k, double* const* dst, hipStream_
 
 void ppr_prepare(ursa_hip_topo* t, hipStream_t s) {
     NoArena keep_out_of_arena;
-    const DevCsr& in = t->incoming(s);
+    pagerank_prepare(t, s);
     std::lock_guard<std::mutex> lk(t->cache_mu);
     if (t->ppr_ready) return;
     const uint64_t n = t->n;
     std::vector<uint32_t> off(n + 1), od(n);
-    HIP_OK(hipMemcpyAsync(off.data(), in.off.get(), (n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    if (n) HIP_OK(hipMemcpyAsync(od.data(), t->out_deg.get(), n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(off.data(), t->pr_off.get(), (n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    if (n) HIP_OK(hipMemcpyAsync(od.data(), t->pr_outdeg.get(), n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
     std::vector<uint32_t> hrow, hsoff{0}, sbeg, send;
     uint64_t dangling = 0;
@@ -561,7 +566,6 @@ void ppr_solve(ursa_hip_topo* t, const PprQuery* qs, int k, double* const* outs,
     if (n == 0) return;
     hipStream_t s = stream();
     ppr_prepare(t, s);
-    const DevCsr& in = t->incoming(s);
     const int KP = lanes_padded(k);
 
     // Host-side lane set-up: distinct in-range seeds per lane.
@@ -613,7 +617,8 @@ void ppr_solve(ursa_hip_topo* t, const PprQuery* qs, int k, double* const* outs,
         HIP_OK(hipMemcpyAsync(B.seeds.get(), seeds.data(), seeds.size() * 4, hipMemcpyHostToDevice, s));
         HIP_OK(hipMemcpyAsync(B.lane_of.get(), lane_of.data(), seeds.size() * 4, hipMemcpyHostToDevice, s));
         k_ppr_seedbits<<<blocks_for(seeds.size(), 256), 256, 0, s>>>(B.seeds.get(), B.lane_of.get(),
-                                                                     (uint32_t)seeds.size(), B.seedbits.get());
+                                                                     (uint32_t)seeds.size(), t->pr_new_of_old.get(),
+                                                                     B.seedbits.get());
         check_launch();
     }
 
@@ -627,11 +632,11 @@ void ppr_solve(ursa_hip_topo* t, const PprQuery* qs, int k, double* const* outs,
 
     auto run = [&](auto kp_tag) {
         constexpr int KPc = decltype(kp_tag)::value;
-        k_ppr_init<KPc><<<blocks_for(n, 256, 4096), 256, 0, s>>>(n, t->out_deg.get(), B.r32.get(), B.c32a.get(),
+        k_ppr_init<KPc><<<blocks_for(n, 256, 4096), 256, 0, s>>>(n, t->pr_outdeg.get(), B.r32.get(), B.c32a.get(),
                                                                  B.r64.get(), B.c64a.get());
         check_launch();
-        ppr_iterate<KPc>(t, in, B, max_iter_all, s);
-        ppr_write_out<KPc>(n, B, k, dst.data(), s);
+        ppr_iterate<KPc>(t, B, max_iter_all, s);
+        ppr_write_out<KPc>(t, n, B, k, dst.data(), s);
     };
     switch (KP) {
         case 1: run(std::integral_constant<int, 1>{}); break;

@@ -16,7 +16,10 @@ struct PprQuery {
     double tol;
 };
 
-// Builds the graph-only heavy-row segmentation (cached on the topology).
+// PageRank's working copy of the in-CSR (cached on the topology; pagerank.hip).
+void pagerank_prepare(ursa_hip_topo* t, hipStream_t s);
+// Builds the graph-only heavy-row segmentation over that copy (cached on the
+// topology).
 void ppr_prepare(ursa_hip_topo* t, hipStream_t s);
 // Device bytes one solve of k lanes allocates (outputs staged when host_out).
 uint64_t ppr_workspace_bytes(const ursa_hip_topo* t, int k, bool host_out);
