@@ -665,7 +665,8 @@ def bench_batched(args, ua, gen):
     out_b = np.zeros(n, np.float64)
     out_1 = np.zeros(n, np.float64)
     re = C.c_uint64(0)
-    for name, batch, out in (("batched64", 64, out_b), ("one_per_launch", 1, out_1)):
+    legs = (("batched64", 64, out_b),) + (() if args.no_compare else (("one_per_launch", 1, out_1),))
+    for name, batch, out in legs:
         check(lib.ursa_hip_betweenness_sources_ex(topo._h, src.ctypes.data_as(C.c_void_p), 256, 1, n / 256.0,
                                                   batch, out.ctypes.data_as(C.c_void_p), C.byref(re)))
         t0 = time.perf_counter()
@@ -675,9 +676,10 @@ def bench_batched(args, ua, gen):
         res[f"brandes_{name}"] = {"s": round(el, 4), "sources_per_s": round(256 / el, 1),
                                   "GTEPS": round(re.value / el / 1e9, 3)}
         log(f"brandes k=256 {name}: {el * 1e3:.1f} ms")
-    rel = np.abs(out_b - out_1) / np.maximum(np.abs(out_1), 1e-300)
-    res["brandes_speedup"] = round(res["brandes_one_per_launch"]["s"] / res["brandes_batched64"]["s"], 2)
-    res["brandes_max_rel_diff_batched_vs_single"] = float(rel[out_1 != 0].max()) if np.any(out_1 != 0) else 0.0
+    if not args.no_compare:
+        rel = np.abs(out_b - out_1) / np.maximum(np.abs(out_1), 1e-300)
+        res["brandes_speedup"] = round(res["brandes_one_per_launch"]["s"] / res["brandes_batched64"]["s"], 2)
+        res["brandes_max_rel_diff_batched_vs_single"] = float(rel[out_1 != 0].max()) if np.any(out_1 != 0) else 0.0
     o = None
     if not args.no_cpu_baseline:
         # CPU leg: the oracle's Brandes over the same 256 sources (betweenness.rs:45-70,
@@ -704,7 +706,7 @@ def bench_batched(args, ua, gen):
     nsrc = args.msbfs_sources
     bsrc = np.ascontiguousarray(ua.shuffled_order(n, 4 ^ 1)[:nsrc])
     lv = Dev(lib, 4 * nsrc * n)
-    for name, mode in (("batched64", 2), ("one_per_launch", 1)):
+    for name, mode in (("batched64", 2),) + (() if args.no_compare else (("one_per_launch", 1),)):
         check(lib.ursa_hip_bfs_ex(topo._h, bsrc.ctypes.data_as(C.c_void_p), 64, 0, lv.p, C.byref(re), mode))
         t0 = time.perf_counter()
         check(lib.ursa_hip_bfs_ex(topo._h, bsrc.ctypes.data_as(C.c_void_p), nsrc, 0, lv.p, C.byref(re), mode))
@@ -716,9 +718,10 @@ def bench_batched(args, ua, gen):
         if name == "batched64":
             first = np.zeros((64, n), np.int32)
             check(lib.ursa_hip_memcpy_to_host(first.ctypes.data_as(C.c_void_p), lv.p, first.nbytes))
-    single = np.zeros((64, n), np.int32)
-    check(lib.ursa_hip_memcpy_to_host(single.ctypes.data_as(C.c_void_p), lv.p, single.nbytes))
-    res["msbfs_first64_identical"] = bool(np.array_equal(first, single))
+    if not args.no_compare:
+        single = np.zeros((64, n), np.int32)
+        check(lib.ursa_hip_memcpy_to_host(single.ctypes.data_as(C.c_void_p), lv.p, single.nbytes))
+        res["msbfs_first64_identical"] = bool(np.array_equal(first, single))
     if o is not None:
         threads = cpu_threads()
         t0 = time.perf_counter()
@@ -728,8 +731,49 @@ def bench_batched(args, ua, gen):
                                "rows_equal": int(sum(np.array_equal(first[i], rows[i]) for i in range(64)))}
         res["msbfs_cpu_oracle"] = {"sources": 64, "threads": threads, "s": round(cpu_s, 3)}
         log(f"msbfs parity vs oracle: {json.dumps(res['msbfs_parity'])}")
-    res["msbfs_speedup"] = round(res["msbfs_one_per_launch"]["s"] / res["msbfs_batched64"]["s"], 2)
+    if not args.no_compare:
+        res["msbfs_speedup"] = round(res["msbfs_one_per_launch"]["s"] / res["msbfs_batched64"]["s"], 2)
     lv.free()
+    return res
+
+
+def bench_ppr(args, ua, gen):
+    """Personalized PageRank on the R-MAT scale 22 / seed 2 graph (tol 1e-8, damping
+    0.85): one query per solve and 16 queries per solve (one lane each), device
+    output; sweep time and bytes from the library's per-kernel events."""
+    lib = ua.load()
+    e = gen.rmat(args.scale or 22, 16, 0.57, 0.19, 0.19, seed=2)
+    topo = e.topology()
+    n, m = e.n, e.m
+    rng = np.random.default_rng(12)
+    seed_lists = [rng.integers(0, n, int(rng.integers(1, 17))).astype(np.uint32) for _ in range(16)]
+    out = Dev(lib, 8 * n * 16)
+    res = {"graph": {"scale": args.scale or 22, "seed": 2, "n": n, "m": m}}
+    offs = np.zeros(17, np.uint64)
+    for i, sl in enumerate(seed_lists):
+        offs[i + 1] = offs[i] + len(sl)
+    flat = np.ascontiguousarray(np.concatenate(seed_lists))
+    iters = np.zeros(16, np.uint32)
+    for name, k in (("one_query", 1), ("sixteen_queries", 16)):
+        call = lambda: check(lib.ursa_hip_ppr_batch(topo._h, k, offs.ctypes.data_as(C.c_void_p),
+                                                    flat.ctypes.data_as(C.c_void_p), 0.85, 100, 1e-8, out.p,
+                                                    iters.ctypes.data_as(C.c_void_p)))
+        call()
+        check(lib.ursa_hip_synchronize())
+        t0 = time.perf_counter()
+        call()
+        check(lib.ursa_hip_synchronize())
+        ms = (time.perf_counter() - t0) * 1e3
+        ua.profile.enable(True)
+        ua.profile.reset()
+        call()
+        q = ua.profile.query("ppr_rows")
+        ua.profile.enable(False)
+        res[name] = {"ms": round(ms, 3), "iters": iters[:k].tolist(), "sweeps": q["launches"],
+                     "sweep_us": round(q["total_ms"] / max(q["launches"], 1) * 1e3, 2),
+                     "GBps": round(q["algo_bytes"] / (q["total_ms"] / 1e3) / 1e9, 1) if q["total_ms"] else None}
+        log(f"ppr {name}: {json.dumps(res[name])}")
+    out.free()
     return res
 
 
@@ -828,10 +872,15 @@ def main():
     ap.add_argument("--serve-qps", type=float, default=0.0, help="Poisson rate (default 0.7 x serial capacity)")
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--only", choices=["c2", "pagerank", "batched", "serving"], default=None,
+    ap.add_argument("--only", choices=["c2", "pagerank", "batched", "serving", "ppr"], default=None,
                     help="run one section only (for profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run the calibration copy kernel over this many MiB")
+    ap.add_argument("--gather-probe", type=int, default=0, metavar="MIB",
+                    help="also run the calibration gather kernel: 2^24 random 8-byte reads, each on its own "
+                         "128-byte line of a buffer of this many MiB")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the one-query-per-launch comparison legs (for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (how the path would shard over "
@@ -851,6 +900,19 @@ def main():
         a.free()
         b.free()
         log(f"copy probe: 3 x {args.copy_probe} MiB")
+    if args.gather_probe:
+        lib = ua.load()
+        lines = (args.gather_probe << 20) // 128
+        count = 1 << 24
+        idx = (np.random.default_rng(5).permutation(lines)[:count] * 16).astype(np.uint32)
+        count = int(idx.shape[0])
+        src, di, out = Dev(lib, lines * 128), Dev(lib, 4 * count), Dev(lib, 8 * count)
+        check(lib.ursa_hip_memcpy_to_device(di.p, idx.ctypes.data_as(C.c_void_p), 4 * count))
+        for _ in range(3):
+            check(lib.ursa_hip_gather_probe(src.p, lines * 16, di.p, count, out.p))
+        for b in (src, di, out):
+            b.free()
+        log(f"gather probe: 3 x {count} gathers over {args.gather_probe} MiB")
     if args.only == "pagerank":
         print(json.dumps({"pagerank": bench_pagerank(args, ua, gen)}), flush=True)
         return
@@ -859,6 +921,9 @@ def main():
         return
     if args.only == "serving":
         print(json.dumps({"serving": bench_serving(args, ua)}), flush=True)
+        return
+    if args.only == "ppr":
+        print(json.dumps({"ppr": bench_ppr(args, ua, gen)}), flush=True)
         return
     res = bench_c2(args, ua, gen)
     if args.only == "c2":

@@ -104,6 +104,12 @@ int ursa_hip_ppr_batch(ursa_hip_topo* t, uint32_t k, const uint64_t* seed_offset
  * exactly `bytes` in and `bytes` out). */
 int ursa_hip_copy_probe(const void* src_device, void* dst_device, uint64_t bytes);
 
+/* out[i] = src[idx[i]] for i < count (all device memory; every idx[i] <
+ * src_len, which the caller guarantees): the reference point for hardware byte
+ * counters under random 8-byte gathers. */
+int ursa_hip_gather_probe(const double* src_device, uint64_t src_len, const uint32_t* idx_device, uint64_t count,
+                          double* out_device);
+
 /* ---- the formats either side of the kernels ---- */
 /* Edge weights in adjacency-slot order: w_slots[k] = w_rows[edge_ids[k]] for
  * the out-CSR (which = 0) or the in-CSR (which = 1), the gather a weighted

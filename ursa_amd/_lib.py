@@ -65,6 +65,7 @@ SIGNATURES = {
     "ursa_hip_profile_query": (cint, [C.c_char_p, C.POINTER(f64), C.POINTER(u64), C.POINTER(f64)]),
     "ursa_hip_profile_names": (u64, [C.c_char_p, u64]),
     "ursa_hip_copy_probe": (cint, [P, P, u64]),
+    "ursa_hip_gather_probe": (cint, [P, u64, P, u64, P]),
     "ursa_hip_device_alloc": (cint, [u64, C.POINTER(P)]),
     "ursa_hip_device_free": (cint, [P]),
     "ursa_hip_memcpy_to_host": (cint, [P, P, u64]),

@@ -112,6 +112,15 @@ __global__ void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ 
         dst[i] = src[i];
 }
 
+// out[i] = src[idx[i]]: the calibration kernel for random gathers (with every
+// index on its own cache line of a buffer much larger than the caches, each
+// gather costs one line fetch past L2, whatever the counters call it).
+__global__ void k_gather_probe(const double* __restrict__ src, const uint32_t* __restrict__ idx, uint64_t count,
+                               double* __restrict__ out) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = src[idx[i]];
+}
+
 }  // namespace
 
 void build_grouped(uint64_t n, uint64_t m, const uint32_t* keys, const uint32_t* other, DevCsr& out,
@@ -421,6 +430,22 @@ int ursa_hip_copy_probe(const void* src_device, void* dst_device, uint64_t bytes
         timed("copy_probe", 2.0 * (double)bytes, s, [&] {
             k_copy_probe<<<blocks_for(bytes / 16, 256, 1u << 16), 256, 0, s>>>((const uint4*)src_device,
                                                                               (uint4*)dst_device, bytes / 16);
+        });
+        check_launch();
+        HIP_OK(hipStreamSynchronize(s));
+    });
+}
+
+int ursa_hip_gather_probe(const double* src_device, uint64_t src_len, const uint32_t* idx_device, uint64_t count,
+                          double* out_device) {
+    return guard([&] {
+        URSA_REQUIRE(is_device_ptr(src_device) && is_device_ptr(idx_device) && is_device_ptr(out_device),
+                     URSA_HIP_EINVAL, "gather probe takes device buffers");
+        URSA_REQUIRE(src_len > 0 && src_len <= 0xFFFFFFFFull, URSA_HIP_EINVAL, "src_len must be in [1, 2^32)");
+        hipStream_t s = stream();
+        timed("gather_probe", 12.0 * (double)count, s, [&] {
+            k_gather_probe<<<blocks_for(count, 256, 1u << 16), 256, 0, s>>>(src_device, idx_device, count,
+                                                                            out_device);
         });
         check_launch();
         HIP_OK(hipStreamSynchronize(s));
