@@ -231,6 +231,7 @@ def bench_c2(args, ua, gen):
         "algo_bytes_per_launch": per_launch,
         "launches": launches,
     }
+    host_step = c2_host_step(ua, lib, topo, n, src_ptr)
     cpu, cpu_kernels, parity = None, None, None
     if not args.no_cpu_baseline:
         cpu, cpu_kernels, parity = c2_cpu_leg(args, lib, e, sources, lv, d_out, d_in, d_cc, d_tri, scale)
@@ -259,9 +260,38 @@ def bench_c2(args, ua, gen):
         "cpu_baseline": cpu,
         "cpu_kernels": cpu_kernels,
         "parity": parity,
+        "step_host_outputs": host_step,
         "tri_launch": tri_q,
         "ingest": ingest,
     }
+
+
+def c2_host_step(ua, lib, topo, n, src_ptr, reps=3):
+    """The same step through the C ABI with every output written to pinned host
+    memory (ursa_hip_host_alloc, the buffers an Arrow array can own; result.rs:84-139):
+    64 x n i32 levels, 2 x n u32 degrees, n u32 labels, n u64 triangle counts."""
+    bufs = {"levels": ua.PinnedBuffer(4 * 64 * n), "out": ua.PinnedBuffer(4 * n), "in": ua.PinnedBuffer(4 * n),
+            "cc": ua.PinnedBuffer(4 * n), "tri": ua.PinnedBuffer(8 * n)}
+    reached = C.c_uint64(0)
+    p = {k: C.c_void_p(b.ptr) for k, b in bufs.items()}
+
+    def step():
+        check(lib.ursa_hip_degree(topo._h, 0, p["out"]))
+        check(lib.ursa_hip_degree(topo._h, 1, p["in"]))
+        check(lib.ursa_hip_bfs_ex(topo._h, src_ptr, 64, 0, p["levels"], C.byref(reached), 0))
+        check(lib.ursa_hip_cc_weak(topo._h, p["cc"]))
+        check(lib.ursa_hip_triangle_count(topo._h, p["tri"]))
+
+    step()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    out_mb = (4 * 64 + 4 * 3 + 8) * n / 1e6
+    for b in bufs.values():
+        b.free()
+    log(f"step with pinned host outputs: {ms:.2f} ms ({out_mb:.0f} MB of results to host)")
+    return {"ms_per_step": round(ms, 3), "host_output_MB": round(out_mb, 1)}
 
 
 def bench_ingest(args, lib, e):
@@ -283,6 +313,26 @@ def bench_ingest(args, lib, e):
     gb = (24.0 * m + 8.0 * nn.value) / 1e9
     res = {"rows": m, "ids": int(nn.value), "ms": round(el * 1e3, 3), "GBps": round(gb / el, 1),
            "hbm_frac": round(gb / el / HBM_PEAK_GBS, 4)}
+    # The same rows handed over as 16 chunk pairs (IdMap::from_edge_batches,
+    # id_map.rs:220-292), each a device pointer into the edge list.
+    sd1, dd1 = host_copy(lib, bufs[2], np.uint32, m), host_copy(lib, bufs[3], np.uint32, m)
+    nb = 16
+    cut = [m * i // nb for i in range(nb + 1)]
+    lens = np.array([cut[i + 1] - cut[i] for i in range(nb)], np.uint64)
+    srcs = (C.c_void_p * nb)(*[bufs[0].p.value + 8 * cut[i] for i in range(nb)])
+    dsts = (C.c_void_p * nb)(*[bufs[1].p.value + 8 * cut[i] for i in range(nb)])
+    nb_n = C.c_uint64(0)
+    times = []
+    for _ in range(4):
+        t0 = time.perf_counter()
+        check(lib.ursa_hip_intern_i64_batches(nb, lens.ctypes.data_as(C.c_void_p), C.cast(srcs, C.c_void_p),
+                                              C.cast(dsts, C.c_void_p), bufs[2].p, bufs[3].p, bufs[4].p,
+                                              C.byref(nb_n)))
+        times.append(time.perf_counter() - t0)
+    res["batches16_ms"] = round(min(times[1:]) * 1e3, 3)
+    res["batches16_equal"] = bool(nb_n.value == nn.value
+                                  and np.array_equal(host_copy(lib, bufs[2], np.uint32, m), sd1)
+                                  and np.array_equal(host_copy(lib, bufs[3], np.uint32, m), dd1))
     if not args.no_cpu_baseline:
         import oracle as orc
 
@@ -292,7 +342,8 @@ def bench_ingest(args, lib, e):
         res["cpu_ids_equal"] = rn == nn.value
     for b in bufs:
         b.free()
-    log(f"ingest: {m} rows -> {nn.value} ids in {el * 1e3:.2f} ms ({gb / el:.0f} GB/s)")
+    log(f"ingest: {m} rows -> {nn.value} ids in {el * 1e3:.2f} ms ({gb / el:.0f} GB/s); "
+        f"as {nb} batches {res['batches16_ms']:.2f} ms, equal {res['batches16_equal']}")
     return res
 
 

@@ -95,6 +95,15 @@ int ursa_hip_clustering_coefficient(ursa_hip_topo* t, double* out);
  * of distinct ids. 2m must fit in u32 (URSA_HIP_ERANGE otherwise). */
 int ursa_hip_intern_i64(uint64_t m, const int64_t* src, const int64_t* dst, uint32_t* src_dense,
                         uint32_t* dst_dense, int64_t* user_ids, uint64_t* n_out);
+/* IdMap::from_edge_batches over a stream of (src, dst) batches (id_map.rs:207-292):
+ * batch b has lens[b] rows in srcs[b] / dsts[b] (int64, host or device; each
+ * batch may be anywhere). Rows are numbered across the batches in order, so the
+ * ids are those ursa_hip_intern_i64 gives the rows laid end to end;
+ * src_dense/dst_dense: u32[sum lens] in that order; user_ids and n_out as
+ * above. The batches are not concatenated. */
+int ursa_hip_intern_i64_batches(uint64_t nbatches, const uint64_t* lens, const int64_t* const* srcs,
+                                const int64_t* const* dsts, uint32_t* src_dense, uint32_t* dst_dense,
+                                int64_t* user_ids, uint64_t* n_out);
 
 /* ---- topology — ursa-core/src/topology.rs ---- */
 /* Topology::build (topology.rs:276-298): stable grouping of the m edge rows by

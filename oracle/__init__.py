@@ -299,6 +299,16 @@ def intern_i64(src, dst):
     return int(n), sd, dd, users[:n].copy()
 
 
+def intern_i64_batches(batches):
+    """IdMap::from_edge_batches (ursa-core/src/id_map.rs:220-292): ids are global across
+    the batches and rows are numbered in batch order, which is the single-array
+    interning of the rows laid end to end (id_map.rs:529-550 asserts exactly that)."""
+    bs = [(np.asarray(s, np.int64), np.asarray(d, np.int64)) for s, d in batches]
+    if not bs:
+        return 0, np.zeros(0, np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.int64)
+    return intern_i64(np.concatenate([s for s, _ in bs]), np.concatenate([d for _, d in bs]))
+
+
 def gen_rmat(scale: int, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, symmetrize=False):
     cap = (edge_factor << scale) * (2 if symmetrize else 1)
     s = np.zeros(cap, np.int64)

@@ -302,3 +302,28 @@ def test_intern_i64_matches_oracle(ua):
         assert n == rn and np.array_equal(sd, rs) and np.array_equal(dd, rd) and np.array_equal(users, ru)
     n, sd, dd, users = ua.intern_i64([], [])
     assert n == 0 and sd.size == 0
+
+
+def test_intern_i64_batches_matches_oracle(ua):
+    """Interning across batches (id_map.rs:220-292): the two-chunk golden of
+    id_map.rs:529-550, the empty batch list, empty batches between full ones, and an
+    R-MAT edge list cut at ragged points, each equal to the oracle's interning of the
+    rows laid end to end."""
+    n, sd, dd, users = ua.intern_i64_batches([([10, 10], [20, 30]), ([20, 30], [30, 10])])
+    assert n == 3 and list(sd) == [0, 0, 1, 2] and list(dd) == [1, 2, 2, 0] and list(users) == [10, 20, 30]
+    n, sd, dd, users = ua.intern_i64_batches([])
+    assert n == 0 and sd.size == 0 and users.size == 0
+    n, sd, dd, users = ua.intern_i64_batches([([], []), ([], [])])
+    assert n == 0 and sd.size == 0
+    us, ud = orc.gen_rmat(14, 16, seed=5)
+    rng = np.random.default_rng(6)
+    cuts = np.sort(rng.integers(0, us.size, 9))
+    cuts = np.concatenate([[0], cuts, [cuts[-1]], [us.size]])  # one empty batch
+    batches = [(us[a:b], ud[a:b]) for a, b in zip(cuts[:-1], cuts[1:])]
+    got = ua.intern_i64_batches(batches)
+    want = orc.intern_i64(us, ud)
+    assert got[0] == want[0]
+    for g, w in zip(got[1:], want[1:]):
+        assert np.array_equal(g, w)
+    with pytest.raises(ValueError):
+        ua.intern_i64_batches([([1, 2], [3])])

@@ -403,6 +403,13 @@ def test_intern_goldens():  # id_map.rs:459-478
     assert list(users) == [100, 7, 42]
 
 
+def test_intern_batches_goldens():  # id_map.rs:529-550, 569-574
+    n, sd, dd, users = orc.intern_i64_batches([([10, 10], [20, 30]), ([20, 30], [30, 10])])
+    assert n == 3 and list(sd) == [0, 0, 1, 2] and list(dd) == [1, 2, 2, 0] and list(users) == [10, 20, 30]
+    n, sd, dd, users = orc.intern_i64_batches([])
+    assert n == 0 and sd.size == 0 and dd.size == 0
+
+
 # ---------------------------------------------------------------- generators
 def test_rmat_generator_contract():
     s, d = orc.gen_rmat(10, 16, seed=1)

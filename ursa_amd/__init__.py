@@ -28,6 +28,7 @@ __all__ = [
     "UrsaHipError",
     "degree",
     "intern_i64",
+    "intern_i64_batches",
     "bfs_source_scan",
     "avg_path_length",
     "diameter",
@@ -245,6 +246,32 @@ def intern_i64(src, dst):
     users = np.zeros(max(2 * m, 1), np.int64)
     n = C.c_uint64(0)
     check(load().ursa_hip_intern_i64(m, _ptr(s), _ptr(d), _ptr(sd), _ptr(dd), _ptr(users), C.byref(n)))
+    return int(n.value), sd[:m], dd[:m], users[: n.value]
+
+
+def intern_i64_batches(batches):
+    """IdMap::from_edge_batches on the device (ursa-core/src/id_map.rs:207-292): interns
+    a sequence of (src, dst) int64 batches in first-seen order across the batches, src
+    before dst per row, without concatenating them -> (n, src_dense u32[m],
+    dst_dense u32[m], user_ids i64[n]) with m the total row count."""
+    keep = []
+    for src, dst in batches:
+        s = np.ascontiguousarray(np.asarray(src, dtype=np.int64))
+        d = np.ascontiguousarray(np.asarray(dst, dtype=np.int64))
+        if s.shape != d.shape or s.ndim != 1:
+            raise ValueError("src/dst length mismatch")
+        keep.append((s, d))
+    nb = len(keep)
+    m = sum(s.shape[0] for s, _ in keep)
+    lens = np.array([s.shape[0] for s, _ in keep] or [0], np.uint64)
+    srcs = (C.c_void_p * max(nb, 1))(*[s.ctypes.data for s, _ in keep])
+    dsts = (C.c_void_p * max(nb, 1))(*[d.ctypes.data for _, d in keep])
+    sd = np.zeros(max(m, 1), np.uint32)
+    dd = np.zeros(max(m, 1), np.uint32)
+    users = np.zeros(max(2 * m, 1), np.int64)
+    n = C.c_uint64(0)
+    check(load().ursa_hip_intern_i64_batches(nb, _ptr(lens), C.cast(srcs, C.c_void_p), C.cast(dsts, C.c_void_p),
+                                             _ptr(sd), _ptr(dd), _ptr(users), C.byref(n)))
     return int(n.value), sd[:m], dd[:m], users[: n.value]
 
 

@@ -103,6 +103,7 @@ SIGNATURES = {
     "ursa_hip_server_destroy": (cint, [P]),
     "ursa_hip_digest": (u64, [P, u64]),
     "ursa_hip_cc_weak_noalloc": (cint, [P, P]),
+    "ursa_hip_intern_i64_batches": (cint, [u64, P, P, P, P, P, P, C.POINTER(u64)]),
 }
 
 _lib = None

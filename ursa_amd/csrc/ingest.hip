@@ -21,13 +21,15 @@ namespace {
 
 constexpr unsigned IB = 256;
 
+// Occurrences of one batch of m rows whose first row is global row row0.
 __global__ void k_occurrences(const int64_t* __restrict__ src, const int64_t* __restrict__ dst, uint64_t m,
-                              uint64_t* __restrict__ keys, uint32_t* __restrict__ pos) {
+                              uint64_t row0, uint64_t* __restrict__ keys, uint32_t* __restrict__ pos) {
     for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < m; r += (uint64_t)gridDim.x * blockDim.x) {
-        keys[2 * r] = (uint64_t)src[r];
-        keys[2 * r + 1] = (uint64_t)dst[r];
-        pos[2 * r] = (uint32_t)(2 * r);
-        pos[2 * r + 1] = (uint32_t)(2 * r + 1);
+        const uint64_t g = row0 + r;
+        keys[2 * g] = (uint64_t)src[r];
+        keys[2 * g + 1] = (uint64_t)dst[r];
+        pos[2 * g] = (uint32_t)(2 * g);
+        pos[2 * g + 1] = (uint32_t)(2 * g + 1);
     }
 }
 
@@ -92,18 +94,34 @@ __global__ void k_inclusive_heads(const uint32_t* __restrict__ excl, const uint3
 
 }  // namespace
 
-uint64_t intern_i64(uint64_t m, const int64_t* src_user, const int64_t* dst_user, uint32_t* src_out,
-                    uint32_t* dst_out, int64_t* users_out) {
+// Interning over a sequence of (src, dst) batches (IdMap::from_edge_batches,
+// id_map.rs:220-292): rows are numbered across the batches in order, each
+// batch's endpoints are written straight into the occurrence keys at its row
+// offset (the batches are never concatenated), and the dense ids of all rows
+// come out in batch order.
+uint64_t intern_batches(uint64_t nb, const uint64_t* lens, const int64_t* const* srcs, const int64_t* const* dsts,
+                        uint32_t* src_out, uint32_t* dst_out, int64_t* users_out) {
+    uint64_t m = 0;
+    for (uint64_t b = 0; b < nb; ++b) {
+        URSA_REQUIRE(lens[b] == 0 || (srcs[b] && dsts[b]), URSA_HIP_EINVAL, "NULL edge batch");
+        m += lens[b];
+    }
     if (m == 0) return 0;
     URSA_REQUIRE(2 * m <= 0xFFFFFFFFull, URSA_HIP_ERANGE, "too many edge rows for u32 occurrence positions");
     hipStream_t s = stream();
     const uint64_t cnt = 2 * m;
-    DevIn<int64_t> src(src_user, m, s), dst(dst_user, m, s);
     DevOut<uint32_t> sd(src_out, m), dd(dst_out, m);
     DevBuf<uint64_t> keys(cnt), skeys(cnt);
     DevBuf<uint32_t> pos(cnt), spos(cnt);
-    k_occurrences<<<blocks_for(m, IB, 1u << 16), IB, 0, s>>>(src.p, dst.p, m, keys.get(), pos.get());
-    check_launch();
+    uint64_t row0 = 0;
+    for (uint64_t b = 0; b < nb; ++b) {
+        if (lens[b] == 0) continue;
+        DevIn<int64_t> src(srcs[b], lens[b], s), dst(dsts[b], lens[b], s);
+        k_occurrences<<<blocks_for(lens[b], IB, 1u << 16), IB, 0, s>>>(src.p, dst.p, lens[b], row0, keys.get(),
+                                                                      pos.get());
+        check_launch();
+        row0 += lens[b];
+    }
     prim::sort_pairs_u64_u32(keys.get(), skeys.get(), pos.get(), spos.get(), cnt, 64, s);
     keys.reset();
     pos.reset();
@@ -137,7 +155,28 @@ uint64_t intern_i64(uint64_t m, const int64_t* src_user, const int64_t* dst_user
     return nruns;
 }
 
+uint64_t intern_i64(uint64_t m, const int64_t* src_user, const int64_t* dst_user, uint32_t* src_out,
+                    uint32_t* dst_out, int64_t* users_out) {
+    return intern_batches(1, &m, &src_user, &dst_user, src_out, dst_out, users_out);
+}
+
 }  // namespace ursa
+
+extern "C" int ursa_hip_intern_i64_batches(uint64_t nbatches, const uint64_t* lens, const int64_t* const* srcs,
+                                           const int64_t* const* dsts, uint32_t* src_dense, uint32_t* dst_dense,
+                                           int64_t* user_ids, uint64_t* n_out) {
+    return ursa::guard([&] {
+        URSA_REQUIRE(n_out != nullptr, URSA_HIP_EINVAL, "n_out is NULL");
+        *n_out = 0;
+        if (nbatches == 0) return;
+        URSA_REQUIRE(lens && srcs && dsts, URSA_HIP_EINVAL, "NULL batch table");
+        uint64_t m = 0;
+        for (uint64_t b = 0; b < nbatches; ++b) m += lens[b];
+        if (m == 0) return;
+        URSA_REQUIRE(src_dense && dst_dense, URSA_HIP_EINVAL, "NULL output buffer");
+        *n_out = ursa::intern_batches(nbatches, lens, srcs, dsts, src_dense, dst_dense, user_ids);
+    });
+}
 
 extern "C" int ursa_hip_intern_i64(uint64_t m, const int64_t* src, const int64_t* dst, uint32_t* src_dense,
                                    uint32_t* dst_dense, int64_t* user_ids, uint64_t* n_out) {
