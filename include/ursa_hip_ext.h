@@ -72,6 +72,12 @@ int ursa_hip_shuffled_order(uint64_t n, uint64_t seed, uint32_t* out);
  * they are launched one by one; results are identical either way. */
 int ursa_hip_graphs_enable(int on);
 
+/* Triangle-count tables (default on): owners of long oriented lists among the
+ * 65536 highest-ranked vertices are staged as LDS bitmaps, the rest as LDS
+ * hash sets. With 0, topologies whose triangle orientation is built afterwards
+ * use hash sets only. Counts are identical either way. */
+int ursa_hip_triangle_bitmaps(int on);
+
 /* ---- mixed-precision PageRank ----
  * Same contract as ursa_hip_pagerank (pagerank.rs:98-169). The first sweeps
  * gather f32 contributions and sum each row piece with compensated (Neumaier)

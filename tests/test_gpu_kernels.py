@@ -5,6 +5,8 @@ Bars: BFS levels, k_hop pairs (order included), shortest paths, triangle counts
 and canonical component labels bit-exact; betweenness within 1e-12 relative per
 node.
 """
+import contextlib
+
 import numpy as np
 import pytest
 
@@ -262,9 +264,36 @@ def test_triangles_match(ua, case):
     assert int(got.sum()) % 3 == 0
 
 
+@contextlib.contextmanager
+def tri_tables(ua, bitmaps):
+    """Triangle tables for topologies built inside: bitmaps for eligible owners
+    (the default) or hash sets only."""
+    lib = ua.load()
+    ua.check(lib.ursa_hip_triangle_bitmaps(int(bitmaps)))
+    try:
+        yield
+    finally:
+        ua.check(lib.ursa_hip_triangle_bitmaps(1))
+
+
 def test_triangles_rmat_symmetric(ua, rmat12sym):
-    n, _, _, t, o = rmat12sym
-    assert np.array_equal(ua.triangle_count(t), orc.triangle_count(o, nthreads=8))
+    n, s, d, t, o = rmat12sym
+    want = orc.triangle_count(o, nthreads=8)
+    assert np.array_equal(ua.triangle_count(t), want)
+    with tri_tables(ua, False):
+        assert np.array_equal(ua.triangle_count(ua.Topology.build(n, s, d)), want)
+
+
+def test_triangles_dense_rmat_both_tables(ua):
+    # R-MAT scale 12, edge factor 400, symmetrised: 506 oriented lists longer
+    # than 256 (up to 407), staged as bitmaps or as per-workgroup hash sets.
+    n, s, d = rmat_dense(12, 9, symmetrize=True, edge_factor=400)
+    want = orc.triangle_count(orc.Topology(n, s, d), nthreads=8)
+    for bitmaps in (True, False):
+        with tri_tables(ua, bitmaps):
+            t = ua.Topology.build(n, s, d)
+            for _ in range(2):
+                assert np.array_equal(ua.triangle_count(t), want), bitmaps
 
 
 def test_triangles_goldens(ua):  # triangle.rs:70-99
@@ -274,16 +303,31 @@ def test_triangles_goldens(ua):  # triangle.rs:70-99
     assert list(ua.triangle_count(t)) == [1, 1, 1]
 
 
-def test_triangles_dense_clique(ua):
-    # K_300: every node in C(299, 2) triangles; oriented lists exceed one LDS stage
-    k = 300
+@pytest.mark.parametrize("bitmaps", [True, False])
+def test_triangles_dense_clique(ua, bitmaps):
+    with tri_tables(ua, bitmaps):
+        # K_300: every node in C(299, 2) triangles; oriented lists exceed one LDS stage
+        k = 300
+        a, b = np.triu_indices(k, 1)
+        t = ua.Topology.build(k, a, b)
+        assert np.all(ua.triangle_count(t) == (k - 1) * (k - 2) // 2)
+        # K_1100: oriented lists up to 1099 entries, staged per workgroup
+        k = 1100
+        a, b = np.triu_indices(k, 1)
+        t = ua.Topology.build(k, a, b)
+        assert np.all(ua.triangle_count(t) == (k - 1) * (k - 2) // 2)
+
+
+def test_triangles_disjoint_cliques(ua):
+    # 300 disjoint K_300 (90,000 nodes, all of degree 299). New ids run in
+    # decreasing id, so a clique's lowest node (|N+| = 299) has new id
+    # 89,999 - 300 c: the first 82 cliques' are beyond the bitmap range and use
+    # the per-workgroup hash table, the others the bitmap. Closed form: every
+    # node in C(299, 2) triangles.
+    k, c = 300, 300
     a, b = np.triu_indices(k, 1)
-    t = ua.Topology.build(k, a, b)
-    assert np.all(ua.triangle_count(t) == (k - 1) * (k - 2) // 2)
-    # K_1100: oriented lists up to 1099 entries, staged per workgroup
-    k = 1100
-    a, b = np.triu_indices(k, 1)
-    t = ua.Topology.build(k, a, b)
+    off = (np.arange(c) * k)[:, None]
+    t = ua.Topology.build(k * c, (a[None, :] + off).ravel(), (b[None, :] + off).ravel())
     assert np.all(ua.triangle_count(t) == (k - 1) * (k - 2) // 2)
 
 
@@ -309,7 +353,7 @@ def test_triangles_bipartite_with_chords(ua, side):
 
 
 def _bucket0_ids(k, nbuckets):
-    """k node ids >= 1 whose home bucket in an LDS table of `nbuckets` buckets is
+    """k new ids >= 1 whose home bucket in an LDS table of `nbuckets` buckets is
     0 (bits 7.. of id * 0x9E3779B1, the kernel's hash): staged together they
     overflow one bucket. Ids colliding for nbuckets also collide for every
     smaller power of two."""
@@ -321,26 +365,38 @@ def _bucket0_ids(k, nbuckets):
     return np.array(ids, np.int64)
 
 
-@pytest.mark.parametrize("k,nb", [(14, 4), (40, 16), (300, 128)])
-def test_triangles_colliding_keys(ua, k, nb):
-    # Node 0 joined to k nodes that form a clique, all with equal degree k, so
-    # N+(0) holds the k ids and N+(w_i) the later ones. With every id homed in
-    # bucket 0, the bucket fills and the remaining keys spill into the
-    # following buckets: 6 of them at k = 14, 32 at k = 40 (per-wave tables),
-    # 292 at k = 300 (a per-workgroup table). Closed form: node 0 closes
-    # C(k, 2) triangles, each w C(k-1, 2) + (k-1).
-    w = _bucket0_ids(k, nb)
+@pytest.mark.parametrize("k,nb,bitmaps", [(14, 4, True), (40, 16, True), (300, 128, False), (300, 128, True)])
+def test_triangles_colliding_keys(ua, k, nb, bitmaps):
+    # An apex joined to k nodes w that form a clique. The triangle kernel hashes
+    # new ids (the rank order: decreasing degree, then decreasing id), so the
+    # graph pins them: P = max(C) + 1 nodes rank above the apex, all of degree
+    # k, and with equal degrees new id p is caller id P - p. The w take the new
+    # ids C (all homed in bucket 0), star centres with k private leaves take
+    # the other positions below P, the apex is caller id 0 (new id P) and
+    # N+(apex) holds exactly C. The bucket fills and the remaining keys spill:
+    # 6 buckets at k = 14, 32 at k = 40 (per-wave tables), 292 at k = 300 in a
+    # per-workgroup hash table (hash sets only) or, with bitmaps, the bitmap
+    # table. Closed form: the apex closes C(k, 2) triangles, each w
+    # C(k-1, 2) + (k-1), centres and leaves none.
+    cpos = _bucket0_ids(k, nb)
+    P = int(cpos.max()) + 1
+    w = P - cpos  # caller ids of the clique
+    fill = np.setdiff1d(np.arange(P), cpos)
+    centres = P - fill
+    leaves = P + 1 + np.arange(fill.size * k)
     a, b = np.triu_indices(k, 1)
-    src = np.concatenate([np.zeros(k, np.int64), w[a]])
-    dst = np.concatenate([w, w[b]])
-    n = int(w.max()) + 1
-    t = ua.Topology.build(n, src, dst)
+    src = np.concatenate([np.zeros(k, np.int64), w[a], np.repeat(centres, k)])
+    dst = np.concatenate([w, w[b], leaves])
+    n = P + 1 + leaves.size
+    with tri_tables(ua, bitmaps):
+        t = ua.Topology.build(n, src, dst)
+        got = ua.triangle_count(t)
     want = np.zeros(n, np.uint64)
     want[0] = k * (k - 1) // 2
     want[w] = (k - 1) * (k - 2) // 2 + (k - 1)
-    got = ua.triangle_count(t)
     assert np.array_equal(got, want)
-    assert np.array_equal(got, orc.triangle_count(orc.Topology(n, src, dst)))
+    if k < 100:
+        assert np.array_equal(got, orc.triangle_count(orc.Topology(n, src, dst)))
 
 
 # ---------------------------------------------------------------- betweenness

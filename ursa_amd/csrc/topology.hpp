@@ -67,16 +67,18 @@ struct ursa_hip_topo {
     ursa::DevBuf<uint32_t> ms_pheavy[3], ms_pcbase[3];
     uint32_t ms_pnh[3] = {0, 0, 0}, ms_pchunks[3] = {0, 0, 0};
     bool ms_pull_ready[3] = {false, false, false};
-    // Degree-oriented undirected DAG for triangle counting (see triangle.hip):
-    // out-lists N+(u), in-lists In+(v), and work items over either list kind
-    // ([0] out, [1] in) in three classes ([0] per wave, [1] per workgroup with
-    // the mid-size table, [2] with the large table).
-    ursa::DevBuf<uint32_t> tc_off, tc_tgt, tc_ioff, tc_isrc;
+    // Degree-oriented undirected DAG for triangle counting (see triangle.hip),
+    // its vertices numbered by decreasing (degree, id) (tc_new: the new id of
+    // each vertex): out-lists N+(u), in-lists In+(v), and work items over
+    // either list kind ([0] out, [1] in) in four classes ([0] per wave, [1] per
+    // workgroup with the mid-size hash table, [2] with the large one, [3] with
+    // a bitmap table).
+    ursa::DevBuf<uint32_t> tc_off, tc_tgt, tc_ioff, tc_isrc, tc_new;
     struct TcItems {
         ursa::DevBuf<uint32_t> start, end, owner;
         uint64_t n = 0;
     };
-    TcItems tc_items[2][3];
+    TcItems tc_items[2][4];
     uint64_t tc_slots = 0;
     double tc_pair_work = 0.0;
     bool tc_built = false;

@@ -4,15 +4,26 @@
 // reference's u32 accumulator wraps on large graphs; u64 never does for m < 2^32).
 //
 // Orientation: every undirected edge {u, v} is kept once, from the endpoint of
-// lower rank, rank(x) = (degree(x), x). Each out-list N+(u) stays sorted by id
-// and has at most sqrt(2m) entries. A triangle u < v < w (in rank) is then
-// found exactly once, as w in N+(u) ∩ N+(v) for the pair u -> v, intersected
-// from one side: N+(v) walked against a hash table of N+(u), or N+(u) walked
-// against a table of N+(v) when N+(v) is much longer (tc_walk); the second
-// kind of work runs over the in-lists In+(v) of the oriented graph.
-// Counting: one wavefront per work item, alone in its workgroup (tables of up
-// to STAGE_CAP entries), or one workgroup of four per item (longer lists); see
-// k_tc_count and k_tc_count_big.
+// lower rank, rank(x) = (degree(x), x), and has at most sqrt(2m) entries per
+// out-list. The oriented graph is stored with its vertices numbered by
+// decreasing rank (new id 0 = highest rank), so every edge points to a smaller
+// new id and N+(u) lies in [0, u); each out-list is sorted by new id. Counts
+// are made per new id and mapped back to the caller's ids at the end.
+// A triangle u < v < w (in rank) is found exactly once, as w in
+// N+(u) ∩ N+(v) for the pair u -> v, intersected from one side: N+(v) walked
+// against a table of N+(u), or N+(u) walked against a table of N+(v) when
+// N+(v) is much longer (tc_walk); the second kind of work runs over the
+// in-lists In+(v) of the oriented graph.
+// Tables: an LDS hash set (lists of up to BIG_CAP entries), or, for an owner
+// u below BM_BITS whose list is longer than STAGE_CAP, a bitmap of N+(u) over
+// [0, u) with per-word prefix counts (a lookup is one LDS word; the member's
+// index, for its hit counter, is the prefix plus a popcount). On R-MAT graphs
+// the owners of long lists are the few highest-ranked vertices (at scale 22
+// all 35 K lists longer than 256 belong to new ids below 35.5 K; 89 % of the
+// probes go to them), so the bitmap serves nearly all the work.
+// Counting: one wavefront per work item, alone in its workgroup (hash tables
+// of up to STAGE_CAP entries), or one workgroup of four per item (longer
+// lists); see k_tc_count, k_tc_count_big and k_tc_count_bm.
 // A work item is a run of consecutive slots of one list (an out-list, or an
 // in-list for the second kind): the slots are cut at every list start and
 // wherever the running intersection cost crosses a multiple of TC_CHUNK, so a
@@ -23,6 +34,7 @@
 //
 // Algorithmic bytes (the sorted-merge model): 4 * sum over oriented edges (u,v)
 // of (d+(u) + d+(v)), plus 12 B per node.
+#include <atomic>
 #include <vector>
 
 #include "frontier.hpp"
@@ -41,7 +53,10 @@ constexpr uint32_t HASH_CAP = 512;   // hash slots per wave (>= 2 * STAGE_CAP)
 constexpr uint32_t TC_BUCKET = 8;    // hash slots per bucket
 constexpr uint32_t MID_CAP = 1024;   // N+(u) entries staged per workgroup in the 12 KB table
 constexpr uint32_t BIG_CAP = 2048;   // N+(u) entries staged in LDS per workgroup (24 KB table)
+constexpr uint32_t BM_BITS = 65536;  // bitmap tables cover new ids [0, BM_BITS) (8 KB)
+constexpr uint32_t BM_WORDS = BM_BITS / 32;
 constexpr uint64_t TC_CHUNK = 32768;  // intersection steps per work item (about)
+constexpr uint32_t TC_MAX_SLOTS = 32768;  // list slots per work item at most (keeps hit counters below 2^16)
 constexpr int TC_ILP = 2;            // 64-step groups whose loads are in flight together
 constexpr uint32_t TC_LONG = 32;     // walks at least this long are taken by the whole wave
 constexpr int TC_LILP = 8;           // 64-step chunks of long walks in flight together
@@ -92,30 +107,34 @@ __global__ void k_tc_item_flags(const uint32_t* __restrict__ loff, const uint32_
         // their launches are not left to a handful of long items.
         const uint64_t win = du > STAGE_CAP && du <= MID_CAP ? 8 * TC_CHUNK : TC_CHUNK;
         const bool work = pre[e - 1] + cost[e - 1] > pre[a];
-        flag[k] = work && ((k == a) || (pre[k] / win != pre[k - 1] / win));
+        flag[k] = work && ((k == a) || (pre[k] / win != pre[k - 1] / win) || ((k - a) % TC_MAX_SLOTS == 0));
         iota[k] = (uint32_t)k;
     }
 }
 
 // Per item: owner, end slot (the next item's start, cut at the list end) and
 // class flags: staged per wave (|N+(owner)| <= STAGE_CAP) or read from HBM
-// (> BIG_CAP), staged per workgroup in a table for up to MID_CAP entries, or
-// in one for up to BIG_CAP. The smaller table leaves room for more
-// workgroups per CU.
+// (> BIG_CAP); per workgroup in a bitmap when the owner's new id is below
+// BM_BITS; else per workgroup in a hash table for up to MID_CAP entries, or in
+// one for up to BIG_CAP. The smaller table leaves room for more workgroups per
+// CU.
 __global__ void k_tc_item_meta(const uint32_t* __restrict__ istart, const uint32_t* __restrict__ owner,
                                const uint32_t* __restrict__ loff, const uint32_t* __restrict__ ooff,
                                uint64_t nitems, uint64_t kept, uint32_t* __restrict__ iu,
                                uint32_t* __restrict__ iend, uint8_t* __restrict__ mid, uint8_t* __restrict__ big,
-                               uint8_t* __restrict__ small) {
+                               uint8_t* __restrict__ small, uint8_t* __restrict__ bm, uint32_t bm_limit) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nitems; i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t o = owner[istart[i]];
         const uint32_t next = i + 1 < nitems ? istart[i + 1] : (uint32_t)kept;
         const uint32_t du = ooff[o + 1] - ooff[o];
+        const bool group = du > STAGE_CAP && du <= BIG_CAP;
+        const bool bitmap = group && o < bm_limit;
         iu[i] = o;
         iend[i] = min(next, loff[o + 1]);
-        mid[i] = du > STAGE_CAP && du <= MID_CAP;
-        big[i] = du > MID_CAP && du <= BIG_CAP;
-        small[i] = du <= STAGE_CAP || du > BIG_CAP;
+        bm[i] = bitmap;
+        mid[i] = !bitmap && du > STAGE_CAP && du <= MID_CAP;
+        big[i] = !bitmap && du > MID_CAP && du <= BIG_CAP;
+        small[i] = !group;
     }
 }
 
@@ -203,14 +222,38 @@ __device__ __forceinline__ int tc_probe(const uint32_t* key, uint32_t bmask, uin
     return r;
 }
 
-// Records a triangle (u, v, w) found with w at hash slot wslot (staged) or
-// w itself (not staged): w's credit; v's credit and u's are added by the caller.
-// Slot counters are 16-bit, two per LDS word (a slot is credited at most
-// once per v of the item plus its own hits, both below 2^16).
+// Bitmap table lookup: w's index in the sorted N+(u), or -1. `lim` is the
+// bitmap's bit count (32 per staged word); bits at or above u are zero, since
+// N+(u) lies in [0, u).
+__device__ __forceinline__ int tc_bm_find(const uint32_t* bm, const uint16_t* pre, uint32_t lim, uint32_t w) {
+    if (w >= lim) return -1;
+    const uint32_t word = bm[w >> 5];
+    const uint32_t b = w & 31u;
+    if (!((word >> b) & 1u)) return -1;
+    return (int)pre[w >> 5] + __popc(word & ((1u << b) - 1u));
+}
+
+// A staged table's lookup, bitmap (BM) or hash: `key` holds the bitmap words
+// or the hash keys, `mask` the bit limit or the bucket mask. The returned index
+// (a hash slot, or the member's position in the list) selects a 16-bit hit
+// counter.
+template <bool BM>
+__device__ __forceinline__ int tc_find(const uint32_t* key, const uint16_t* pre, uint32_t mask, uint32_t w) {
+    if constexpr (BM)
+        return tc_bm_find(key, pre, mask, w);
+    else
+        return tc_probe(key, mask, w);
+}
+
+// Hit counters are 16-bit, two per LDS word (an entry is credited at most once
+// per v of the item plus its own hits, both at most BIG_CAP, below 2^16).
 __device__ __forceinline__ void tc_cnt_add(uint32_t* cnt, int slot, uint32_t k) {
     atomicAdd(&cnt[slot >> 1], k << (16 * (slot & 1)));
 }
 
+// Records a triangle (u, v, w) found with w at table index wslot (staged) or
+// w itself (not staged): w's credit; v's credit and u's are added by the caller.
+template <bool BM>
 __device__ __forceinline__ void tc_credit_w(bool staged, int wslot, uint32_t w, uint32_t* cnt,
                                             unsigned long long* __restrict__ tri) {
     if (staged)
@@ -220,12 +263,13 @@ __device__ __forceinline__ void tc_credit_w(bool staged, int wslot, uint32_t w, 
 }
 
 // Is w in N+(u)? Sets wslot when staged. (A bloom bitmap in front of the
-// table cannot pay on a wave: the table is read whenever any of its 64 lanes
-// passes, which with about one hit in ten steps is nearly always.)
-__device__ __forceinline__ bool tc_member(bool staged, const uint32_t* key, uint32_t hmask,
+// hash table cannot pay on a wave: the table is read whenever any of its 64
+// lanes passes, which with about one hit in ten steps is nearly always.)
+template <bool BM>
+__device__ __forceinline__ bool tc_member(bool staged, const uint32_t* key, const uint16_t* pre, uint32_t hmask,
                                           const uint32_t* ulist, uint32_t du, uint32_t w, int& wslot) {
     if (staged) {
-        wslot = tc_probe(key, hmask, w);
+        wslot = tc_find<BM>(key, pre, hmask, w);
         return wslot >= 0;
     }
     return sorted_contains(ulist, du, w);
@@ -245,14 +289,15 @@ __device__ __forceinline__ bool tc_member(bool staged, const uint32_t* key, uint
 // u's share, w (in N+(u)) is credited in the slot counters (or in HBM when N+(u)
 // is not staged), and so is v when v is in N+(u) (out-lists); an in-list
 // partner is credited in HBM. Integer sums are order-independent.
-template <bool IN>
+template <bool IN, bool BM = false>
 __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict__ ooff,
                                                       const uint32_t* __restrict__ otgt,
                                                       const uint32_t* __restrict__ plist, uint32_t a, uint32_t du,
                                                       uint32_t s0, uint32_t s1, uint32_t first, uint32_t step,
                                                       bool staged, const uint32_t* key, uint32_t* cnt,
                                                       uint32_t hmask,
-                                                      unsigned long long* __restrict__ tri) {
+                                                      unsigned long long* __restrict__ tri,
+                                                      const uint16_t* pre = nullptr) {
     const int lane = lane_id();
     const uint32_t* ulist = otgt + a;
     unsigned long long cu = 0;
@@ -266,7 +311,7 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
             v = plist[i];
             sv = ooff[v];
             c = tc_walk(IN, du, ooff[v + 1] - sv);
-            if (vtable && c) vslot = tc_probe(key, hmask, v);
+            if (vtable && c) vslot = tc_find<BM>(key, pre, hmask, v);
         }
         // Long walks: cut into 64-step chunks walked by the whole wave, lane l
         // taking step l of the chunk. A group of up to TC_LILP chunks of one
@@ -297,12 +342,12 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
                 int wslot = -1;
                 bool hit;
                 if (staged) {
-                    wslot = tc_probe(key, hmask, w[r]);
+                    wslot = tc_find<BM>(key, pre, hmask, w[r]);
                     hit = ok && wslot >= 0;
                 } else {
                     hit = ok && sorted_contains(ulist, du, w[r]);
                 }
-                if (hit) tc_credit_w(staged, wslot, w[r], cnt, tri);
+                if (hit) tc_credit_w<BM>(staged, wslot, w[r], cnt, tri);
                 h += (uint32_t)__popcll(__ballot(hit));
             }
             cu += h;
@@ -339,9 +384,9 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
             for (int r = 0; r < TC_ILP; ++r) {
                 bool hit = false;
                 int wslot = -1;
-                if (ok[r]) hit = tc_member(staged, key, hmask, ulist, du, w[r], wslot);
+                if (ok[r]) hit = tc_member<BM>(staged, key, pre, hmask, ulist, du, w[r], wslot);
                 if (hit) {
-                    tc_credit_w(staged, wslot, w[r], cnt, tri);
+                    tc_credit_w<BM>(staged, wslot, w[r], cnt, tri);
                     if (vtable)
                         tc_cnt_add(cnt, ovs[r], 1u);
                     else
@@ -466,7 +511,115 @@ __global__ __launch_bounds__(TCB_BIG) void k_tc_count_big(const uint32_t* __rest
     }
 }
 
+// One workgroup per item whose owner u (new id below BM_BITS) has more than
+// STAGE_CAP and at most BIG_CAP out-entries: N+(u), which lies in [0, u), is
+// staged as a bitmap of ceil(u / 32) words with the number of members before
+// each word, and one u32 hit counter per member in list order (the list is
+// sorted, so a member's index is its prefix count plus the set bits below it
+// in its word), two 16-bit counters per LDS word.
+template <bool IN>
+__global__ __launch_bounds__(TCB_BIG) void k_tc_count_bm(const uint32_t* __restrict__ ooff,
+                                                        const uint32_t* __restrict__ otgt,
+                                                        const uint32_t* __restrict__ plist,
+                                                        const uint32_t* __restrict__ istart,
+                                                        const uint32_t* __restrict__ iend,
+                                                        const uint32_t* __restrict__ iu, uint64_t nwork,
+                                                        unsigned long long* __restrict__ tri) {
+    constexpr int WPT = BM_WORDS / TCB_BIG;  // bitmap words per thread in the prefix pass
+    __shared__ uint32_t s_bm[BM_WORDS];
+    __shared__ uint16_t s_pre[BM_WORDS];
+    __shared__ uint32_t s_cnt[BIG_CAP / 2];
+    __shared__ uint32_t s_wave[TCB_BIG / 64];
+    const int wid = threadIdx.x >> 6;
+    const int lane = lane_id();
+    const uint32_t tid = threadIdx.x;
+    for (uint64_t wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+        const uint32_t u = iu[wi];
+        const uint32_t a = ooff[u];
+        const uint32_t du = ooff[u + 1] - a;
+        const uint32_t nw = (u + 31) / 32;  // u < BM_BITS, so nw <= BM_WORDS
+        for (uint32_t i = tid; i < nw; i += TCB_BIG) s_bm[i] = 0;
+        for (uint32_t i = tid; i < (du + 1) / 2; i += TCB_BIG) s_cnt[i] = 0;
+        __syncthreads();
+        for (uint32_t i = tid; i < du; i += TCB_BIG) {
+            const uint32_t x = otgt[a + i];  // < u by the numbering; the test only guards LDS
+            if (x < nw * 32u) atomicOr(&s_bm[x >> 5], 1u << (x & 31u));
+        }
+        __syncthreads();
+        // Prefix counts: thread t sums words [WPT t, WPT t + WPT), a block
+        // exclusive scan gives its start.
+        uint32_t loc = 0;
+#pragma unroll
+        for (int r = 0; r < WPT; ++r) {
+            const uint32_t wd = WPT * tid + r;
+            if (wd < nw) loc += __popc(s_bm[wd]);
+        }
+        const uint32_t incl = wave_incl_scan(loc);
+        if (lane == 63) s_wave[wid] = incl;
+        __syncthreads();
+        uint32_t run = incl - loc;
+        for (int w = 0; w < wid; ++w) run += s_wave[w];
+#pragma unroll
+        for (int r = 0; r < WPT; ++r) {
+            const uint32_t wd = WPT * tid + r;
+            if (wd < nw) {
+                s_pre[wd] = (uint16_t)run;
+                run += __popc(s_bm[wd]);
+            }
+        }
+        __syncthreads();
+        const unsigned long long cu = tc_item<IN, true>(ooff, otgt, plist, a, du, istart[wi], iend[wi], 64u * wid,
+                                                        TCB_BIG, true, s_bm, s_cnt, nw * 32u, tri, s_pre);
+        if (lane == 0 && cu) atomicAdd(&tri[u], cu);
+        __syncthreads();
+        for (uint32_t i = tid; i < du; i += TCB_BIG) {
+            const uint32_t c = (s_cnt[i >> 1] >> (16 * (i & 1))) & 0xFFFFu;
+            if (c) atomicAdd(&tri[otgt[a + i]], (unsigned long long)c);
+        }
+        __syncthreads();
+    }
+}
+
+// Counts per new id -> counts per caller id.
+__global__ void k_tc_unmap(const unsigned long long* __restrict__ tnew, const uint32_t* __restrict__ newid,
+                           uint64_t n, unsigned long long* __restrict__ out) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x)
+        out[v] = tnew[newid[v]];
+}
+
+// Rank keys: ascending order of (~degree, ~id) is decreasing (degree, id).
+__global__ void k_tc_rank_keys(const uint32_t* __restrict__ uoff, uint64_t n, uint64_t* __restrict__ keys,
+                               uint32_t* __restrict__ iota) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t d = uoff[v + 1] - uoff[v];
+        keys[v] = ((uint64_t)(~d) << 32) | (uint64_t)(~(uint32_t)v);
+        iota[v] = (uint32_t)v;
+    }
+}
+
+__global__ void k_tc_newid(const uint32_t* __restrict__ old_of_new, uint64_t n, uint32_t* __restrict__ newid) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        newid[old_of_new[i]] = (uint32_t)i;
+}
+
+// Oriented (owner, target) pairs in new ids, packed owner-major for the sort.
+__global__ void k_tc_pack(const uint32_t* __restrict__ own, const uint32_t* __restrict__ tgt, uint64_t m,
+                          const uint32_t* __restrict__ newid, uint64_t* __restrict__ keys) {
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < m; k += (uint64_t)gridDim.x * blockDim.x)
+        keys[k] = ((uint64_t)newid[own[k]] << 32) | newid[tgt[k]];
+}
+
+__global__ void k_tc_unpack(const uint64_t* __restrict__ keys, uint64_t m, uint32_t* __restrict__ own,
+                            uint32_t* __restrict__ tgt) {
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < m; k += (uint64_t)gridDim.x * blockDim.x) {
+        own[k] = (uint32_t)(keys[k] >> 32);
+        tgt[k] = (uint32_t)keys[k];
+    }
+}
+
 }  // namespace
+
+std::atomic<bool> g_tc_bitmaps{true};
 
 static void tc_orient(ursa_hip_topo* t, hipStream_t s) {
     NoArena keep_out_of_arena;
@@ -476,6 +629,18 @@ static void tc_orient(ursa_hip_topo* t, hipStream_t s) {
     const DevCsr& U = t->undirected(s);
     const uint64_t slots = U.slots;
     t->tc_off.alloc(n + 1);
+    // New ids: vertices in decreasing (degree, id) order, so that every
+    // oriented edge points to a smaller new id.
+    t->tc_new.alloc(n ? n : 1);
+    if (n) {
+        DevBuf<uint64_t> rk(n), rks(n);
+        DevBuf<uint32_t> iota(n), old_of_new(n);
+        k_tc_rank_keys<<<blocks_for(n, 256), 256, 0, s>>>(U.off.get(), n, rk.get(), iota.get());
+        check_launch();
+        prim::sort_pairs_u64_u32(rk.get(), rks.get(), iota.get(), old_of_new.get(), n, 64, s);
+        k_tc_newid<<<blocks_for(n, 256), 256, 0, s>>>(old_of_new.get(), n, t->tc_new.get());
+        check_launch();
+    }
     uint64_t kept = 0;
     DevBuf<uint32_t> oown;  // owner of each oriented slot
     if (slots) {
@@ -484,13 +649,24 @@ static void tc_orient(ursa_hip_topo* t, hipStream_t s) {
         DevBuf<uint8_t> keep(slots);
         k_tc_flags<<<blocks_for(slots, 256), 256, 0, s>>>(U.off.get(), U.tgt.get(), owner.get(), slots, keep.get());
         check_launch();
-        oown.alloc(slots);
-        t->tc_tgt.alloc(slots);
+        DevBuf<uint32_t> ktgt(slots), kown(slots);
         DevBuf<uint64_t> cnt(1);
-        prim::select_flagged_u32(U.tgt.get(), keep.get(), t->tc_tgt.get(), cnt.get(), slots, s);
-        prim::select_flagged_u32(owner.get(), keep.get(), oown.get(), cnt.get(), slots, s);
+        prim::select_flagged_u32(U.tgt.get(), keep.get(), ktgt.get(), cnt.get(), slots, s);
+        prim::select_flagged_u32(owner.get(), keep.get(), kown.get(), cnt.get(), slots, s);
         HIP_OK(hipMemcpyAsync(&kept, cnt.get(), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
+        // Relabel both ends and sort owner-major: the out-lists in new ids,
+        // each sorted by new id.
+        oown.alloc(kept ? kept : 1);
+        t->tc_tgt.alloc(kept ? kept : 1);
+        if (kept) {
+            DevBuf<uint64_t> pk(kept), pks(kept);
+            k_tc_pack<<<blocks_for(kept, 256), 256, 0, s>>>(kown.get(), ktgt.get(), kept, t->tc_new.get(), pk.get());
+            check_launch();
+            prim::sort_keys_u64(pk.get(), pks.get(), kept, 32 + bits_for(n), s);
+            k_tc_unpack<<<blocks_for(kept, 256), 256, 0, s>>>(pks.get(), kept, oown.get(), t->tc_tgt.get());
+            check_launch();
+        }
         offsets_from_sorted(oown.get(), kept, n, t->tc_off.get(), s);
     } else {
         t->tc_tgt.alloc(1);
@@ -541,13 +717,15 @@ static void tc_orient(ursa_hip_topo* t, hipStream_t s) {
         HIP_OK(hipMemcpyAsync(&nitems, wc.get(), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
         if (!nitems) continue;
-        DevBuf<uint8_t> cls[3] = {DevBuf<uint8_t>(nitems), DevBuf<uint8_t>(nitems), DevBuf<uint8_t>(nitems)};
+        DevBuf<uint8_t> cls[4] = {DevBuf<uint8_t>(nitems), DevBuf<uint8_t>(nitems), DevBuf<uint8_t>(nitems),
+                                  DevBuf<uint8_t>(nitems)};
         k_tc_item_meta<<<blocks_for(nitems, 256), 256, 0, s>>>(istart.get(), owner, loff, t->tc_off.get(), nitems,
                                                               kept, iu.get(), iend.get(), cls[1].get(), cls[2].get(),
-                                                              cls[0].get());
+                                                              cls[0].get(), cls[3].get(),
+                                                              g_tc_bitmaps.load() ? BM_BITS : 0u);
         check_launch();
-        // Split the items into the per-wave class and the two per-workgroup classes.
-        for (int c = 0; c < 3; ++c) {
+        // Split the items into the per-wave class and the three per-workgroup classes.
+        for (int c = 0; c < 4; ++c) {
             auto& it = items[c];
             it.start.alloc(nitems);
             it.end.alloc(nitems);
@@ -581,20 +759,21 @@ void triangle_count(ursa_hip_topo* t, uint64_t* out_user) {
     hipStream_t s = stream();
     tc_orient(t, s);
     DevOut<uint64_t> out(out_user, n);
-    HIP_OK(hipMemsetAsync(out.p, 0, n * sizeof(uint64_t), s));
+    DevBuf<unsigned long long> tnew(n);  // counts per new id
+    HIP_OK(hipMemsetAsync(tnew.get(), 0, n * sizeof(uint64_t), s));
     uint64_t nitems = 0;
     for (const auto& kind : t->tc_items)
         for (const auto& it : kind) nitems += it.n;
     if (nitems) {
         const double bytes = 4.0 * t->tc_pair_work + 12.0 * (double)n;
-        auto* tri = reinterpret_cast<unsigned long long*>(out.p);
+        auto* tri = tnew.get();
         const uint32_t* ooff = t->tc_off.get();
         const uint32_t* otgt = t->tc_tgt.get();
         timed("triangle_count", bytes, s, [&] {
             // In-list items run on the side stream beside the out-list items, so
             // that each launch's last items overlap the other kind's work.
             const auto& in_items = t->tc_items[1];
-            const bool side = in_items[0].n + in_items[1].n + in_items[2].n > 0;
+            const bool side = in_items[0].n + in_items[1].n + in_items[2].n + in_items[3].n > 0;
             hipStream_t s1 = s;
             if (side) fork_side(s, &s1);
             for (int kind = 0; kind < 2; ++kind) {
@@ -603,6 +782,14 @@ void triangle_count(ursa_hip_topo* t, uint64_t* out_user) {
                 const uint32_t* plist = kind ? t->tc_isrc.get() : otgt;
                 auto grid_big = [](uint64_t k) { return (unsigned)std::min<uint64_t>(k, 1u << 16); };
                 const unsigned grid_small = (unsigned)std::min<uint64_t>((it[0].n + TCW - 1) / TCW, TC_SMALL_GRID);
+                if (it[3].n) {
+                    if (kind == 0)
+                        k_tc_count_bm<false><<<grid_big(it[3].n), TCB_BIG, 0, ks>>>(
+                            ooff, otgt, plist, it[3].start.get(), it[3].end.get(), it[3].owner.get(), it[3].n, tri);
+                    else
+                        k_tc_count_bm<true><<<grid_big(it[3].n), TCB_BIG, 0, ks>>>(
+                            ooff, otgt, plist, it[3].start.get(), it[3].end.get(), it[3].owner.get(), it[3].n, tri);
+                }
                 if (kind == 0) {
                     if (it[2].n)
                         k_tc_count_big<BIG_CAP, false><<<grid_big(it[2].n), TCB_BIG, 0, ks>>>(
@@ -629,10 +816,18 @@ void triangle_count(ursa_hip_topo* t, uint64_t* out_user) {
         });
         check_launch();
     }
+    k_tc_unmap<<<blocks_for(n, 256), 256, 0, s>>>(tnew.get(), t->tc_new.get(), n,
+                                                  reinterpret_cast<unsigned long long*>(out.p));
+    check_launch();
     out.finish(s);
 }
 
 }  // namespace ursa
+
+extern "C" int ursa_hip_triangle_bitmaps(int on) {
+    ursa::g_tc_bitmaps.store(on != 0);
+    return URSA_HIP_OK;
+}
 
 extern "C" int ursa_hip_triangle_count(ursa_hip_topo* t, uint64_t* out) {
     return ursa::guard([&] {
