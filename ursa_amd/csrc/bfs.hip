@@ -46,20 +46,27 @@ struct BfsCtr {
 // whole grid (k_bfs_td_heavy) instead of by one wave.
 constexpr uint32_t TD_HEAVY = 1024;
 
-__global__ void k_bfs_init(int32_t* level, uint64_t n, uint32_t src, uint32_t* queue, BfsCtr* ctr, NbrView fwd) {
+// The level counters are double-buffered: the first kernel of each step
+// clears the slot the next step will use, so no launch is spent on resets.
+__device__ __forceinline__ void ctr_clear(BfsCtr* c) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c->qlen = 0;
+        c->heavy = 0;
+        c->edges = 0;
+    }
+}
+
+__global__ void k_bfs_init(int32_t* level, uint64_t n, uint32_t src, uint32_t* queue, BfsCtr* ctr, BfsCtr* next_ctr,
+                           NbrView fwd) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         level[i] = (i == src) ? 0 : -1;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         queue[0] = src;
         ctr->qlen = 1;
+        ctr->heavy = 0;
         ctr->edges = nv_degree(fwd, src);
     }
-}
-
-__global__ void k_ctr_reset(BfsCtr* ctr) {
-    ctr->qlen = 0;
-    ctr->heavy = 0;
-    ctr->edges = 0;
+    ctr_clear(next_ctr);
 }
 
 // Claim v for level L and append it to the next queue (wave-aggregated).
@@ -77,7 +84,8 @@ __device__ __forceinline__ void td_visit(bool active, uint32_t v, const NbrView&
 __global__ __launch_bounds__(BFS_BLOCK) void k_bfs_td(const uint32_t* __restrict__ q, uint32_t qlen, NbrView fwd,
                                                       int32_t* __restrict__ level, int32_t L,
                                                       uint32_t* __restrict__ nq, BfsCtr* ctr,
-                                                      uint32_t* __restrict__ heavy) {
+                                                      uint32_t* __restrict__ heavy, BfsCtr* next_ctr) {
+    ctr_clear(next_ctr);
     const int lane = lane_id();
     const uint64_t gw = blockIdx.x * (uint64_t)WAVES + (threadIdx.x >> 6);
     const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
@@ -142,15 +150,24 @@ __global__ __launch_bounds__(BFS_BLOCK) void k_bfs_td_heavy(const uint32_t* __re
 
 // Bottom-up step: vertices of level L are those unvisited with a reverse
 // neighbour in `front` (level L-1). Writes the whole next bitmap.
-// A lane checks its own vertex's reverse list four slots at a time (the four
-// slot reads, then the four bitmap reads, are independent). Vertices with more
-// than BU_HEAVY reverse slots would hold their wave for the whole list, so the
-// wave takes them together afterwards: 64 slots per step, coalesced, with the
-// early exit decided by a ballot.
+// A lane checks its own vertex's reverse list eight, then four slots at a time
+// (the slot reads, then the bitmap reads, are independent). Vertices with more
+// than BU_HEAVY reverse slots check their first four slots in their own lane
+// like the others; if no parent is among them, they would hold their wave for
+// the whole list, so the wave takes them together afterwards: 64 slots per
+// step, coalesced, with the early exit decided by a ballot.
 constexpr uint32_t BU_HEAVY = 64;
 
 __device__ __forceinline__ bool bu_scan_lane(const uint32_t* __restrict__ tgt, uint32_t k, uint32_t ke,
                                              const uint32_t* __restrict__ front) {
+    for (; k + 8 <= ke; k += 8) {
+        uint32_t wv[8], bv = 0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) wv[r] = tgt[k + r];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) bv |= front[wv[r] >> 5] >> (wv[r] & 31);
+        if (bv & 1u) return true;
+    }
     for (; k + 4 <= ke; k += 4) {
         const uint32_t w0 = tgt[k], w1 = tgt[k + 1], w2 = tgt[k + 2], w3 = tgt[k + 3];
         const uint32_t b0 = front[w0 >> 5] >> (w0 & 31), b1 = front[w1 >> 5] >> (w1 & 31);
@@ -181,7 +198,8 @@ __device__ __forceinline__ bool bu_scan_wave(const uint32_t* __restrict__ tgt, u
 __global__ __launch_bounds__(BFS_BLOCK) void k_bfs_bu(uint64_t n, NbrView rev, NbrView fwd,
                                                       const uint32_t* __restrict__ front,
                                                       uint32_t* __restrict__ next, int32_t* __restrict__ level,
-                                                      int32_t L, BfsCtr* ctr) {
+                                                      int32_t L, BfsCtr* ctr, BfsCtr* next_ctr) {
+    ctr_clear(next_ctr);
     const int lane = lane_id();
     const uint64_t gw = blockIdx.x * (uint64_t)WAVES + (threadIdx.x >> 6);
     const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
@@ -198,7 +216,9 @@ __global__ __launch_bounds__(BFS_BLOCK) void k_bfs_bu(uint64_t n, NbrView rev, N
                 keb = rev.off_b[v + 1];
             }
             if ((kea - ka) + (keb - kb) > BU_HEAVY) {
-                heavy = true;
+                // the first four slots of the first list, in this lane
+                found = bu_scan_lane(rev.tgt_a, ka, min(ka + 4, kea), front);
+                heavy = !found;
             } else {
                 found = bu_scan_lane(rev.tgt_a, ka, kea, front);
                 if (!found && rev.off_b) found = bu_scan_lane(rev.tgt_b, kb, keb, front);
@@ -209,7 +229,8 @@ __global__ __launch_bounds__(BFS_BLOCK) void k_bfs_bu(uint64_t n, NbrView rev, N
             const int hl = __builtin_ctzll(hm);
             hm &= hm - 1;
             const uint32_t hv = (uint32_t)(base + hl);
-            bool hf = bu_scan_wave(rev.tgt_a, rev.off_a[hv], rev.off_a[hv + 1], front);
+            const uint32_t ha = rev.off_a[hv], hea = rev.off_a[hv + 1];
+            bool hf = bu_scan_wave(rev.tgt_a, min(ha + 4, hea), hea, front);
             if (!hf && rev.off_b) hf = bu_scan_wave(rev.tgt_b, rev.off_b[hv], rev.off_b[hv + 1], front);
             if (lane == hl) found = hf;
         }
@@ -243,15 +264,44 @@ __global__ __launch_bounds__(BFS_BLOCK) void k_level_to_bits(const int32_t* __re
     }
 }
 
-// queue of the vertices at level L.
-__global__ __launch_bounds__(BFS_BLOCK) void k_level_to_queue(const int32_t* __restrict__ level, uint64_t n, int32_t L,
-                                                              uint32_t* __restrict__ q, BfsCtr* ctr) {
-    __shared__ uint32_t s_cnt[WAVES + 1];
-    for (uint64_t b0 = blockIdx.x * (uint64_t)BFS_BLOCK; b0 < n; b0 += (uint64_t)gridDim.x * BFS_BLOCK) {
-        const uint64_t v = b0 + threadIdx.x;
-        const bool take = v < n && level[v] == L;
-        const uint32_t slot = block_append_slot<BFS_BLOCK>(take, &ctr->qlen, s_cnt);
-        if (take) q[slot] = (uint32_t)v;
+// Queue of the vertices whose bit is set in a frontier bitmap (the last
+// bottom-up level), with one atomic per workgroup: each workgroup counts the
+// set bits of its contiguous range of words, reserves that many slots, then
+// writes the vertices in the same word order. (Appending per 256 vertices of a
+// grid-stride loop had cost 35 K same-address atomics, 0.28 ms at scale 24.)
+__global__ __launch_bounds__(BFS_BLOCK) void k_bits_to_queue(const uint32_t* __restrict__ bits, uint64_t nwords,
+                                                             uint32_t* __restrict__ q, BfsCtr* ctr,
+                                                             BfsCtr* next_ctr) {
+    __shared__ uint32_t s_w[WAVES];
+    __shared__ uint32_t s_base;
+    ctr_clear(next_ctr);
+    const uint64_t per = (nwords + gridDim.x - 1) / gridDim.x;
+    const uint64_t w0 = blockIdx.x * per;
+    const uint64_t w1 = w0 + per < nwords ? w0 + per : nwords;
+    uint32_t cnt = 0;
+    for (uint64_t w = w0 + threadIdx.x; w < w1; w += BFS_BLOCK) cnt += __popc(bits[w]);
+    const uint32_t incl = wave_incl_scan(cnt);
+    const int wid = (int)(threadIdx.x >> 6);
+    if (lane_id() == 63) s_w[wid] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int i = 0; i < WAVES; ++i) {
+            const uint32_t c = s_w[i];
+            s_w[i] = tot;
+            tot += c;
+        }
+        s_base = tot ? atomicAdd(&ctr->qlen, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t slot = s_base + s_w[wid] + incl - cnt;
+    for (uint64_t w = w0 + threadIdx.x; w < w1; w += BFS_BLOCK) {
+        uint32_t b = bits[w];
+        while (b) {
+            const int k = __ffs(b) - 1;
+            b &= b - 1;
+            q[slot++] = (uint32_t)(w * 32 + (uint64_t)k);
+        }
     }
 }
 
@@ -284,10 +334,10 @@ void direction_views(ursa_hip_topo* t, int dir, hipStream_t s, NbrView& fwd, Nbr
 
 struct BfsWork {
     DevBuf<uint32_t> q0, q1, b0, b1, heavy;
-    DevBuf<BfsCtr> ctr;
-    BfsCtr* h = nullptr;  // pinned mirror of ctr
+    DevBuf<BfsCtr> ctr;  // two slots, used by alternate steps
+    BfsCtr* h = nullptr;  // pinned mirror of the current slot
     explicit BfsWork(uint64_t n)
-        : q0(n ? n : 1), q1(n ? n : 1), b0(((n + 63) / 64) * 2), b1(((n + 63) / 64) * 2), heavy(n ? n : 1), ctr(1) {
+        : q0(n ? n : 1), q1(n ? n : 1), b0(((n + 63) / 64) * 2), b1(((n + 63) / 64) * 2), heavy(n ? n : 1), ctr(2) {
         HIP_OK(hipHostMalloc((void**)&h, sizeof(BfsCtr), hipHostMallocDefault));
     }
     ~BfsWork() {
@@ -302,12 +352,18 @@ static void bfs_one(ursa_hip_topo* t, uint32_t src, const NbrView& fwd, const Nb
                     int32_t* level, BfsWork& w, hipStream_t s, uint64_t* reached_edges, double alpha, double beta) {
     const uint64_t n = t->n;
     const unsigned gb = blocks_for((n + 63) / 64, WAVES, GRID_CAP);
+    const uint64_t nwords = ((n + 63) / 64) * 2;
+    // Step k uses counter slot k & 1 and clears slot (k + 1) & 1.
+    int step = 0;
+    auto cur = [&]() { return w.ctr.get() + (step & 1); };
+    auto nxt = [&]() { return w.ctr.get() + ((step + 1) & 1); };
     auto read_ctr = [&]() {
-        HIP_OK(hipMemcpyAsync(w.h, w.ctr.get(), sizeof(BfsCtr), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(w.h, cur(), sizeof(BfsCtr), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
+        ++step;
         return *w.h;
     };
-    k_bfs_init<<<blocks_for(n, 256), 256, 0, s>>>(level, n, src, w.q0.get(), w.ctr.get(), fwd);
+    k_bfs_init<<<blocks_for(n, 256), 256, 0, s>>>(level, n, src, w.q0.get(), cur(), nxt(), fwd);
     check_launch();
     BfsCtr c = read_ctr();
     uint32_t* q = w.q0.get();
@@ -328,9 +384,8 @@ static void bfs_one(ursa_hip_topo* t, uint32_t src, const NbrView& fwd, const Nb
             uint64_t awake = qlen, old_awake;
             do {
                 old_awake = awake;
-                k_ctr_reset<<<1, 1, 0, s>>>(w.ctr.get());
                 timed("bfs_bottom_up", bytes_node, s, [&] {
-                    k_bfs_bu<<<gb, BFS_BLOCK, 0, s>>>(n, rev, fwd, front, next, level, L, w.ctr.get());
+                    k_bfs_bu<<<gb, BFS_BLOCK, 0, s>>>(n, rev, fwd, front, next, level, L, cur(), nxt());
                 });
                 check_launch();
                 c = read_ctr();
@@ -340,21 +395,19 @@ static void bfs_one(ursa_hip_topo* t, uint32_t src, const NbrView& fwd, const Nb
                 ++L;
             } while (awake > 0 && (awake >= old_awake || (double)awake > (double)n / beta));
             if (awake == 0) break;
-            k_ctr_reset<<<1, 1, 0, s>>>(w.ctr.get());
-            k_level_to_queue<<<gb, BFS_BLOCK, 0, s>>>(level, n, L - 1, q, w.ctr.get());
+            k_bits_to_queue<<<gb, BFS_BLOCK, 0, s>>>(front, nwords, q, cur(), nxt());
             check_launch();
             c = read_ctr();
             qlen = c.qlen;
             scout = 1;
         } else {
             edges_to_check -= (int64_t)scout;
-            k_ctr_reset<<<1, 1, 0, s>>>(w.ctr.get());
             const unsigned tb = blocks_for((qlen + 63) / 64, WAVES, 1u << 16);
             timed("bfs_top_down", 4.0 * (double)scout + 8.0 * (double)qlen, s, [&] {
-                k_bfs_td<<<tb, BFS_BLOCK, 0, s>>>(q, (uint32_t)qlen, fwd, level, L, nq, w.ctr.get(), w.heavy.get());
+                k_bfs_td<<<tb, BFS_BLOCK, 0, s>>>(q, (uint32_t)qlen, fwd, level, L, nq, cur(), w.heavy.get(), nxt());
                 if (scout > TD_HEAVY) {
                     const unsigned hb = blocks_for(scout, BFS_BLOCK, 2048);
-                    k_bfs_td_heavy<<<hb, BFS_BLOCK, 0, s>>>(w.heavy.get(), fwd, level, L, nq, w.ctr.get());
+                    k_bfs_td_heavy<<<hb, BFS_BLOCK, 0, s>>>(w.heavy.get(), fwd, level, L, nq, cur());
                 }
             });
             check_launch();
