@@ -360,7 +360,8 @@ __global__ __launch_bounds__(BLOCK) void k_pr_sweep(
 // folded per workgroup in index order.
 template <class CT, class RT, bool WEIGHTED, bool CORR, bool DM>
 __global__ __launch_bounds__(PR_FIX) void k_pr_fixup(
-    const uint8_t* __restrict__ cflag, const uint32_t* __restrict__ crow, uint64_t nchunks, uint64_t nz, uint64_t n,
+    const uint8_t* __restrict__ cflag, const uint32_t* __restrict__ crow, const uint32_t* __restrict__ off,
+    uint64_t nchunks, uint64_t nz, uint64_t n,
     const uint32_t* __restrict__ out_deg, const double* __restrict__ out_str, const RT* __restrict__ rank,
     RT* __restrict__ next, CT* __restrict__ cnext, const float* __restrict__ corr,
     const double* __restrict__ lead_val, const double* __restrict__ tail_val, const PrPart* __restrict__ part,
@@ -379,13 +380,22 @@ __global__ __launch_bounds__(PR_FIX) void k_pr_fixup(
     if (i < nchunks) {
         const uint32_t f = cflag[i];
         if (!(f & 1u) && (f & 4u)) {
-            // The row holding this chunk's first slot ends here and began in the
-            // last earlier chunk with a row start; chunks between lie inside it.
-            uint64_t j = i - 1;
-            while (!(cflag[j] & 2u)) --j;
-            acc = tail_val[j];
-            for (uint64_t q = j + 1; q <= i; ++q) acc += lead_val[q];
+            // The row u holding this chunk's first slot ends here; it began in
+            // chunk j, the one holding its head slot, and the chunks between lie
+            // inside it. Its pieces are added left to right; the loads are
+            // issued eight at a time (a hub's row spans hundreds of chunks).
             u = crow[i];
+            const uint64_t j = off[u] / PR_CHUNK;
+            acc = tail_val[j];
+            uint64_t q = j + 1;
+            for (; q + 8 <= i + 1; q += 8) {
+                double v[8];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) v[r] = lead_val[q + r];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) acc += v[r];
+            }
+            for (; q <= i; ++q) acc += lead_val[q];
         }
     } else if (i < nchunks + (n - nz)) {
         u = nz + (i - nchunks);  // no in-edges: acc = 0 (pagerank.rs:113-123)
@@ -953,11 +963,13 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
         timed("pagerank_fixup", 0.0, s, [&] {
             if (weighted)
                 k_pr_fixup<double, double, true, false, false><<<fix_blocks, PR_FIX, 0, s>>>(
-                    cflag, crow, nchunks, nz, n, od, os, rk, nx, cn, nullptr, lead.get(), tail.get(), part.get(), g64,
+                    cflag, crow, t->pr_off.get(), nchunks, nz, n, od, os, rk, nx, cn, nullptr, lead.get(), tail.get(),
+                    part.get(), g64,
                     fpart.get(), ctl.get(), par, base_arg, damping);
             else
                 k_pr_fixup<double, double, false, false, false><<<fix_blocks, PR_FIX, 0, s>>>(
-                    cflag, crow, nchunks, nz, n, od, os, rk, nx, cn, nullptr, lead.get(), tail.get(), part.get(), g64,
+                    cflag, crow, t->pr_off.get(), nchunks, nz, n, od, os, rk, nx, cn, nullptr, lead.get(), tail.get(),
+                    part.get(), g64,
                     fpart.get(), ctl.get(), par, base_arg, damping);
         });
         check_launch();
@@ -974,7 +986,8 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
                 tgt, WT ? w32.get() : nullptr, head, crow, m, nchunks, od, os, xin, cin_, xout, cout, corr,
                 lead.get(), tail.get(), part.get(), ctl.get(), -1, base_arg, damping);
             k_pr_fixup<float, float, WT, CORR, true><<<fix_blocks, PR_FIX, 0, s>>>(
-                cflag, crow, nchunks, nz, n, od, os, xin, xout, cout, corr, lead.get(), tail.get(), part.get(), g32,
+                cflag, crow, t->pr_off.get(), nchunks, nz, n, od, os, xin, xout, cout, corr, lead.get(), tail.get(),
+                part.get(), g32,
                 fpart.get(), ctl.get(), -1, base_arg, damping);
         };
         using H0 = std::integral_constant<int, 0>;
