@@ -259,6 +259,35 @@ def test_server_admission_by_workspace_budget(ua, sv, pool):
         assert all(b[0] >= a[1] - 1e-6 for a, b in zip(spans, spans[1:]))
 
 
+def test_server_large_graphs_fit_their_reservations(ua, sv):
+    """Graphs whose n-sized temporaries exceed a megabyte (R-MAT scale 18, about
+    180K nodes; Erdos-Renyi with 300K): every kind of pass, alone and batched,
+    must fit the reservation the server admits it with, and give the plain
+    serial loop's bytes."""
+    specs = [sv.GraphSpec(0, "rmat", 1 << 18, 8, 2001, 18), sv.GraphSpec(1, "er", 300000, 4, 2002, 0)]
+    topos = [sv.build_serving_graph(sp) for sp in specs]
+    ns = [t.n_nodes() for t in topos]
+    assert min(ns) > (1 << 17)
+    trace = []
+    for g in range(2):
+        trace += [sv.triangles_query(g), sv.triangles_query(g)]
+        trace += [sv.ppr_query(g, [i, ns[g] - 1 - i]) for i in range(3)]
+        trace += [sv.khop_query(g, [i * 11]) for i in range(3)] + [sv.path_query(g, i, ns[g] - 1 - i) for i in range(3)]
+    with sv.Server(serial=True) as ser:
+        for t in topos:
+            ser.add_graph(t)
+        rs = ser.run(trace)
+    with sv.Server(slots=4, max_batch=16) as srv:
+        for t in topos:
+            srv.add_graph(t)
+        rm = srv.run(trace)
+        alone = [srv.run([q])[0] for q in trace if q.kind == sv.TRIANGLES]
+    for i, q in enumerate(trace):
+        assert rs[i].status == 0 and rm[i].status == 0, (i, q.kind, rs[i].status, rm[i].status)
+        assert rs[i].digest == rm[i].digest and rs[i].len == rm[i].len, (i, q.kind)
+    assert all(r.status == 0 for r in alone)
+
+
 def test_server_rejects_bad_queries(ua, sv, pool):
     topos, _ = pool
     with sv.Server(slots=2) as srv:

@@ -25,6 +25,7 @@ namespace ursa {
 
 void triangle_prepare(ursa_hip_topo* t);
 void triangle_count(ursa_hip_topo* t, uint64_t* out_user);
+uint64_t triangle_workspace_bytes(uint64_t n);
 
 namespace {
 
@@ -159,8 +160,8 @@ struct ursa_hip_server {
                 return traverse_workspace_bytes(n);
             case URSA_HIP_Q_PATH:
                 return traverse_workspace_bytes(n) + n * 4 + 4096;
-            default:
-                return n * 8 + (1u << 20);
+            default:  // the pass's n-sized output, then triangle_count's own temporaries
+                return n * 8 + 4096 + triangle_workspace_bytes(n);
         }
     }
 

@@ -753,6 +753,10 @@ static void tc_orient(ursa_hip_topo* t, hipStream_t s) {
 
 void triangle_prepare(ursa_hip_topo* t) { tc_orient(t, stream()); }
 
+// Device temporaries of one triangle_count call on a prepared topology with a
+// device output: the counts per new id (the arena aligns each block to 256 B).
+uint64_t triangle_workspace_bytes(uint64_t n) { return n * sizeof(uint64_t) + 4096; }
+
 void triangle_count(ursa_hip_topo* t, uint64_t* out_user) {
     const uint64_t n = t->n;
     if (n == 0) return;  // triangle.rs:59-61
