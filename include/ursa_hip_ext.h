@@ -78,6 +78,23 @@ int ursa_hip_graphs_enable(int on);
  * use hash sets only. Counts are identical either way. */
 int ursa_hip_triangle_bitmaps(int on);
 
+/* PageRank sweep schedule (default 0): 0 automatic (binned from 32 blocks of
+ * 8192 nodes up), 1 pull (each in-row sums its sources' contributions gathered
+ * in new-id order), 2 binned (terms streamed through a tiling of 8192-source
+ * blocks by 4096-row bins and added into exact fixed-point row sums in LDS).
+ * Both binned choices fall back to pull when the tiling would exceed 2^26
+ * tiles (above about 47 M nodes). Every schedule is deterministic and within
+ * 1e-12 relative of the reference per node; pull and binned differ in the last
+ * bits. */
+int ursa_hip_pagerank_schedule(int mode);
+
+/* *out = sum of x[0..n) computed exactly in fixed point (each value truncated
+ * below 2^-100) and rounded to nearest even: the accumulator of the binned
+ * PageRank sweep, exported so it can be checked against exact rational sums.
+ * Every |x[i]| < 4 (checked for host x) and the sum below 8 in magnitude
+ * (integer parts wrap, so partial sums may exceed it); x host or device. */
+int ursa_hip_exact_sum(const double* x, uint64_t n, double* out);
+
 /* ---- mixed-precision PageRank ----
  * Same contract as ursa_hip_pagerank (pagerank.rs:98-169). The first sweeps
  * gather f32 contributions and sum each row piece with compensated (Neumaier)

@@ -327,3 +327,111 @@ def test_intern_i64_batches_matches_oracle(ua):
         assert np.array_equal(g, w)
     with pytest.raises(ValueError):
         ua.intern_i64_batches([([1, 2], [3])])
+
+
+# ---- binned PageRank sweep (exact fixed-point row sums) ----
+import contextlib
+
+
+@contextlib.contextmanager
+def pr_schedule(ua, mode):
+    """0 automatic, 1 pull, 2 binned (ursa_hip_pagerank_schedule)."""
+    lib = ua.load()
+    ua.check(lib.ursa_hip_pagerank_schedule(mode))
+    try:
+        yield
+    finally:
+        ua.check(lib.ursa_hip_pagerank_schedule(0))
+
+
+@pytest.mark.parametrize("case", GRAPHS, ids=IDS)
+def test_pagerank_binned_matches_oracle(ua, case):
+    """Binned sweep on every fixture graph (one tile up to a few): the oracle's
+    iteration count and 1e-12 relative per node, plain and weighted (f32-exact
+    and f64 weights)."""
+    name, n, s, d = case
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    with pr_schedule(ua, 2):
+        for params in ((0.85, 30, 1e-6), (0.85, 100, 1e-10), (0.5, 7, 0.0)):
+            got, it = ua._pagerank(t, None, ua.PageRankParams(*params))
+            ref, rit, rdelta = orc.pagerank(o, *params)
+            assert it == rit, (name, params, it, rit, rdelta)
+            assert_pr_close(got, ref, name)
+        if len(s):
+            w = orc.gen_weights(len(s), 3)
+            w[::7] = 0.0
+            for ww in (w, w * (1.0 + 1e-9)):
+                got, it = ua._pagerank(t, ww, ua.PageRankParams(0.85, 100, 1e-10))
+                ref, rit, _ = orc.pagerank(o, 0.85, 100, 1e-10, weights=ww)
+                assert it == rit, (name, "weighted")
+                assert_pr_close(got, ref, name)
+
+
+def test_pagerank_binned_split_bins_rmat(ua):
+    """R-MAT scale 18 (22 source blocks, 3.9 M in-slots): the hub rows fill bins
+    of fewer than 4096 rows; plain, weighted and mixed, bit-identical across runs
+    and within 1e-13 relative of the pull sweep."""
+    from ursa_amd import gen
+
+    e = gen.rmat(18, 16, seed=3)
+    _, _, sd, dd, _ = e.export()
+    t = e.topology()
+    o = orc.Topology(e.n, sd, dd)
+    params = ua.PageRankParams(0.85, 100, 1e-10)
+    ref, rit, _ = orc.pagerank(o, 0.85, 100, 1e-10, nthreads=8)
+    with pr_schedule(ua, 1):
+        pull, pit = ua._pagerank(t, None, params)
+    with pr_schedule(ua, 2):
+        a, it = ua._pagerank(t, None, params)
+        b, _ = ua._pagerank(t, None, params)
+        assert it == rit == pit
+        assert np.array_equal(a, b)
+        assert_pr_close(a, ref, "rmat18")
+        assert np.max(np.abs(a - pull) / np.abs(pull)) <= 1e-13
+        w = orc.gen_weights(len(sd), 4)
+        gw, wit = ua._pagerank(t, w, params)
+        rw, rwit, _ = orc.pagerank(o, 0.85, 100, 1e-10, weights=w, nthreads=8)
+        assert wit == rwit
+        assert_pr_close(gw, rw, "rmat18 weighted")
+        mx, mit, m32 = ua.pagerank_mixed(t, params)
+        assert m32 >= 1 and np.abs(mx - ref).sum() < 1e-9
+        mx2, _, _ = ua.pagerank_mixed(t, params)
+        assert np.array_equal(mx, mx2)
+
+
+def test_pagerank_automatic_schedule_is_binned_from_32_blocks(ua):
+    """The automatic schedule takes the binned sweep from 32 source blocks up:
+    its result equals the forced binned result bit for bit."""
+    from ursa_amd import gen
+
+    e = gen.rmat(19, 8, seed=9)
+    t = e.topology()
+    auto = ua.pagerank(t, ua.PageRankParams(0.85, 60, 1e-11))
+    with pr_schedule(ua, 2):
+        forced = ua.pagerank(t, ua.PageRankParams(0.85, 60, 1e-11))
+    assert e.n > 31 * 8192
+    assert np.array_equal(auto, forced)
+
+
+def test_exact_sum_matches_rational_arithmetic(ua):
+    """ursa_hip_exact_sum (the binned sweep's accumulator on the device) against
+    Fraction sums of the terms truncated to multiples of 2^-100 (sums below 8
+    in magnitude, the accumulator's range)."""
+    from fractions import Fraction
+    import ctypes as C
+    import math
+
+    lib = ua.load()
+    rng = np.random.default_rng(3)
+    cases = [np.zeros(0), np.array([1.0, 2.0 ** -53]), np.array([1.0, 2.0 ** -53, 2.0 ** -100]),
+             rng.uniform(-1, 1, 5000) * 10.0 ** rng.uniform(-30, 0, 5000), np.full(100000, 7.5e-5)]
+    for x in cases:
+        x = np.ascontiguousarray(x, np.float64)
+        out = C.c_double(0.0)
+        ua.check(lib.ursa_hip_exact_sum(x.ctypes.data_as(C.c_void_p) if x.size else None, x.size, C.byref(out)))
+        tot = Fraction(0)
+        for v in x:
+            f = Fraction(float(v)) * (1 << 100)
+            tot += Fraction(math.trunc(f) if f >= 0 else -math.trunc(-f), 1 << 100)
+        assert out.value == float(tot)

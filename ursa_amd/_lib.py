@@ -81,6 +81,8 @@ SIGNATURES = {
     "ursa_hip_ppr": (cint, [P, P, u64, f64, u32, f64, P, C.POINTER(u32)]),
     "ursa_hip_graphs_enable": (cint, [cint]),
     "ursa_hip_triangle_bitmaps": (cint, [cint]),
+    "ursa_hip_pagerank_schedule": (cint, [cint]),
+    "ursa_hip_exact_sum": (cint, [P, u64, P]),
     "ursa_hip_set_stream": (cint, [P]),
     "ursa_hip_intern_i64": (cint, [u64, P, P, P, P, P, C.POINTER(u64)]),
     "ursa_hip_bfs_source_scan": (cint, [P, P, u64, cint, C.POINTER(f64), C.POINTER(u64), C.POINTER(i32)]),

@@ -58,10 +58,12 @@
 // iterate. Only f64 sweeps decide convergence. A coarse tol (>= 1e-6) or a
 // max_iter of 16 or less runs the f64 solve itself.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <type_traits>
 #include <vector>
 
+#include "fixed_sum.hpp"
 #include "ordered_sum.hpp"
 #include "repeated_sum.hpp"
 #include "prim.hpp"
@@ -703,6 +705,488 @@ __global__ void k_fill(double* p, uint64_t n, double v) {
         p[i] = v;
 }
 
+// Sweep schedule (ursa_hip_pagerank_schedule): 0 automatic, 1 pull, 2 binned.
+std::atomic<int> g_pr_schedule{0};
+
+// ---- binned sweep (propagation blocking with exact row sums) ----
+//
+// The pull sweep gathers one contribution per in-slot; on R-MAT scale 24 the
+// L2 request rate of those 8-byte gathers, not HBM, sets its time. The binned
+// sweep instead streams every byte it touches. Sources are cut into blocks of
+// BN_SSPAN (new ids) and rows into bins of at most BN_RSPAN consecutive rows
+// and at most BN_T2 in-slots (a row with more in-slots is a bin of its own),
+// so that every bin is one work item of phase 2; tile (c, b) holds the
+// in-slots from source block c into bin b, rows ascending, padded to whole
+// chunks of BN_CH slots (padding slots: source 0, row BN_PAD). The tiles are
+// stored twice over: source-block-major for phase 1 (a u16 source index per
+// slot, and per chunk its place in the other order) and bin-major for phase 2
+// (a u16 row index per slot, and the term array). Phase 1 (k_bn_scatter)
+// copies a source block's contributions into LDS, forms the terms of a chunk
+// (contribution, times its weight) per thread and stores them at the chunk's
+// bin-major place; consecutive chunks of a tile are consecutive in both
+// orders, so the stores are runs. Phase 2 (k_bn_gather) takes one bin and
+// streams its chunks: a thread adds a chunk's terms row by row in registers
+// (a tile lists its rows in ascending order, so a hub row's terms come in
+// long runs) and adds each run into the row's accumulator in LDS; a wave whose
+// 64 chunks all hold one row adds it once. Then the bin's rows get the pull
+// sweep's epilogue.
+//
+// Row sums are exact (fixed_sum.hpp): every term is a fixed-point integer in
+// units of 2^-100 (terms are at most 1 in magnitude: a contribution is
+// rank / out-degree or w * rank / out-strength with w <= out-strength, a
+// correction is below 1, and a row sums to at most 1), kept as a 40-bit low
+// part and a signed high part that integer adds accumulate. Integer sums do
+// not depend on the order of the adds, so each row gets the exact sum of its
+// terms (each truncated below 2^-100), correctly rounded to f64 — bit-identical
+// run to run, whichever wave adds first. A workgroup takes a whole bin, so no
+// partial sums cross workgroups (a cross-workgroup merge would need device-
+// scope fences, which write back and invalidate L2).
+//
+// HBM traffic per f64 sweep: 2 (source index) + 8 (term written) + 8 (term
+// read) + 2 (row index) bytes per slot, half a byte of chunk places, plus the
+// node terms; f32 sweeps write and read 4-byte terms.
+constexpr int BN_SBITS = 13;
+constexpr uint32_t BN_SSPAN = 1u << BN_SBITS;  // sources per block (their contributions fill LDS)
+constexpr uint32_t BN_RSPAN = 4096;  // rows per bin at most (16-byte accumulators: 64 KB of LDS)
+constexpr uint32_t BN_CH = 8;                  // slots per chunk; tiles are padded to whole chunks
+constexpr uint32_t BN_PAD = 0xFFFFu;           // row index of a padding slot
+constexpr int BN_P1_BLOCK = 512;
+constexpr int BN_P2_BLOCK = 512;
+constexpr int BN_EP = BN_RSPAN / BN_P2_BLOCK;  // epilogue rows per thread
+constexpr uint32_t BN_T1 = 1u << 17;         // slots per phase-1 item at most
+constexpr uint32_t BN_T2 = 1u << 18;         // in-slots per bin at most, unless one row has more
+constexpr uint64_t BN_MAX_ROW = 1ull << 24;  // in-degree limit of the f64 binned sweep (low parts of a row stay below 2^64)
+constexpr uint64_t BN_MAX_TILES = 1ull << 26;
+constexpr uint64_t BN_MIN_BLOCKS = 32;       // automatic choice: binned from 32 source blocks up
+static_assert(BN_RSPAN % (4 * BN_P2_BLOCK) == 0, "epilogue rows per thread: groups of four");
+
+// Lane-to-slot mapping of both phases: a wave takes a window of BN_WIN
+// consecutive slots, and in step i (0..3) lane L handles the slot pair
+// 128 i + 2 L, 128 i + 2 L + 1 of the window, so every load and store
+// instruction of the wave covers one contiguous span.
+constexpr uint32_t BN_WIN = 512;
+
+// 8 consecutive values from a 16-byte aligned address in 16-byte loads.
+template <class T>
+__device__ __forceinline__ void load8(const T* p, T (&v)[8]) {
+    if constexpr (sizeof(T) == 8) {
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        const d2* q = reinterpret_cast<const d2*>(p);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const d2 t = __builtin_nontemporal_load(q + i);
+            v[2 * i] = t.x;
+            v[2 * i + 1] = t.y;
+        }
+    } else {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        const f4* q = reinterpret_cast<const f4*>(p);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const f4 t = __builtin_nontemporal_load(q + i);
+            v[4 * i] = t.x;
+            v[4 * i + 1] = t.y;
+            v[4 * i + 2] = t.z;
+            v[4 * i + 3] = t.w;
+        }
+    }
+}
+
+template <class T>
+struct Pair;
+template <>
+struct Pair<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct Pair<float> {
+    typedef float type __attribute__((ext_vector_type(2)));
+};
+
+// Phase 1: terms of the chunks of items (block c, source-major chunks
+// [q0, q1)), each slot's term stored at its chunk's bin-major place
+// place[chunk] * BN_CH + (slot % BN_CH), claimed through ctr[0]; also clears
+// phase 2's claim counter ctr[1] for this sweep.
+template <class CT, class W, bool WEIGHTED>
+__global__ __launch_bounds__(BN_P1_BLOCK) void k_bn_scatter(const uint16_t* __restrict__ bsrc,
+                                                            const uint32_t* __restrict__ place,
+                                                            const W* __restrict__ w,
+                                                            const uint32_t* __restrict__ items, uint32_t nitems,
+                                                            uint64_t n, const CT* __restrict__ contrib,
+                                                            CT* __restrict__ term, uint32_t* __restrict__ ctr,
+                                                            const PrCtl* ctl) {
+    constexpr int WAVES = BN_P1_BLOCK / 64;
+    typedef typename Pair<CT>::type CT2;
+    typedef typename Pair<W>::type W2;
+    __shared__ CT s_c[BN_SSPAN];
+    __shared__ uint32_t s_it;
+    if (ctl->done) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[1] = 0;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (;;) {
+        if (threadIdx.x == 0) s_it = atomicAdd(&ctr[0], 1u);
+        __syncthreads();
+        const uint32_t it = s_it;
+        if (it >= nitems) break;
+        const uint32_t c = items[4 * (uint64_t)it];
+        const uint64_t k0 = (uint64_t)items[4 * (uint64_t)it + 1] * BN_CH, k1 = (uint64_t)items[4 * (uint64_t)it + 2] * BN_CH;
+        const uint64_t s0 = (uint64_t)c << BN_SBITS;
+        const uint32_t cnt = (uint32_t)(n - s0 < BN_SSPAN ? n - s0 : BN_SSPAN);
+        for (uint32_t i = threadIdx.x; i < cnt; i += BN_P1_BLOCK) s_c[i] = contrib[s0 + i];
+        __syncthreads();
+        for (uint64_t W0 = k0 + (uint64_t)wave * BN_WIN; W0 < k1; W0 += (uint64_t)WAVES * BN_WIN) {
+            // Loads without branches (slots past the item read its first slot and
+            // store nothing), so that all of them are issued before the first wait.
+            uint32_t sp[4], pl[4];
+            W2 wv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint64_t sl = W0 + 128 * i + 2 * lane;
+                const uint64_t sa = sl < k1 ? sl : k0;
+                sp[i] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(bsrc + sa));
+                pl[i] = place[sa / BN_CH];
+                if constexpr (WEIGHTED) wv[i] = __builtin_nontemporal_load(reinterpret_cast<const W2*>(w + sa));
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint64_t sl = W0 + 128 * i + 2 * lane;
+                if (sl < k1) {
+                    CT2 v;
+                    v.x = s_c[sp[i] & 0xFFFFu];
+                    v.y = s_c[sp[i] >> 16];
+                    if constexpr (WEIGHTED) {
+                        v.x = (CT)wv[i].x * v.x;
+                        v.y = (CT)wv[i].y * v.y;
+                    }
+                    *reinterpret_cast<CT2*>(term + (uint64_t)pl[i] * BN_CH + (sl % BN_CH)) = v;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Row accumulators of phase 2. f64 sweeps (PARTS = 2) keep the exact
+// fixed-point sum of fixed_sum.hpp: a low part (units of 2^-100) and a high
+// part (units of 2^-60) per row. f32 sweeps (PARTS = 1) keep one signed
+// 64-bit integer per row in units of 2^-K, K chosen per sweep so that no row
+// sum can reach 2^60 units (K = 60 - ceil(log2 B) for a bound B on the L1 norm
+// of the terms); each term is floor(|x| * 2^K), sign applied, computed with
+// f32 operations. Both are integer sums, so neither depends on the order of
+// the additions.
+template <class CT>
+struct BnAcc;
+
+template <>
+struct BnAcc<double> {
+    static constexpr int PARTS = 2;
+    unsigned long long lo = 0;
+    long long hi = 0;
+    __device__ __forceinline__ void add(double x, float) {
+        unsigned long long l;
+        long long h;
+        fx_split(x, l, h);
+        lo += l;
+        hi += h;
+    }
+    __device__ __forceinline__ void reset() {
+        lo = 0;
+        hi = 0;
+    }
+    __device__ __forceinline__ void flush(uint32_t r, unsigned long long* s) const {
+        atomicAdd(&s[r], lo);
+        atomicAdd(&s[BN_RSPAN + r], (unsigned long long)hi);
+    }
+    // this += the value of lane (lane - off), for lanes at or above off when take.
+    __device__ __forceinline__ void add_up(int off, bool take) {
+        const unsigned long long l = __shfl_up(lo, off, 64);
+        const long long h = __shfl_up(hi, off, 64);
+        if (take) {
+            lo += l;
+            hi += h;
+        }
+    }
+};
+
+template <>
+struct BnAcc<float> {
+    static constexpr int PARTS = 1;
+    unsigned long long t = 0;
+    // scale = 2^(K-32): |x| * scale < 2^28 by the choice of K.
+    __device__ __forceinline__ void add(float x, float scale) {
+        const float a = fabsf(x) * scale;
+        const float f = floorf(a);
+        const unsigned long long v =
+            ((unsigned long long)(uint32_t)f << 32) | (uint32_t)((a - f) * 4294967296.0f);
+        t += x < 0.f ? 0ull - v : v;
+    }
+    __device__ __forceinline__ void reset() { t = 0; }
+    __device__ __forceinline__ void flush(uint32_t r, unsigned long long* s) const { atomicAdd(&s[r], t); }
+    __device__ __forceinline__ void add_up(int off, bool take) {
+        const unsigned long long v = __shfl_up(t, off, 64);
+        if (take) t += v;
+    }
+};
+
+// Adds the 8 consecutive slots of one lane (rows d, terms x; padding rows
+// BN_PAD at the end) into the LDS accumulators. A tile lists its rows in
+// ascending order, so a row's terms come in runs: each run is summed in
+// registers; runs that end inside the lane are added at once, and the lane's
+// last run goes through a segmented scan over the wave keyed by row, so that
+// consecutive lanes holding the same row (a hub's long run) add it once, from
+// the segment's last lane. Without the scan those lanes would all add to one
+// LDS address, which serialises the atomics. Called by all 64 lanes.
+template <class CT>
+__device__ __forceinline__ void bn_add_lane(const uint32_t (&d)[8], const CT (&x)[8], uint32_t lane, float scale,
+                                            unsigned long long* s) {
+    BnAcc<CT> acc;
+    uint32_t cur = BN_PAD;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        if (d[j] == BN_PAD) break;
+        if (d[j] != cur) {
+            if (cur != BN_PAD) acc.flush(cur, s);
+            acc.reset();
+            cur = d[j];
+        }
+        acc.add(x[j], scale);
+    }
+    const uint32_t prev = __shfl_up(cur, 1, 64), next = __shfl_down(cur, 1, 64);
+    bool start = lane == 0 || prev != cur;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const bool start_up = __shfl_up(start ? 1 : 0, off, 64) != 0;
+        const bool take = lane >= (uint32_t)off && !start;
+        acc.add_up(off, take);
+        start = start || (lane >= (uint32_t)off && start_up);
+    }
+    if ((lane == 63 || next != cur) && cur != BN_PAD) acc.flush(cur, s);
+}
+
+// Phase 2: item = (bin b, bin-major chunks [q0, q1), first row r0, rows),
+// claimed through ctr[1] in the order of the item list (largest first); also
+// clears phase 1's counter ctr[0] for the next sweep. K: the f32 sweeps'
+// fixed-point scale (unused by f64 sweeps).
+template <class CT, class RT, bool WEIGHTED, bool CORR, bool DM>
+__global__ __launch_bounds__(BN_P2_BLOCK, 4) void k_bn_gather(
+    const uint16_t* __restrict__ bdst, const CT* __restrict__ term,
+    const uint32_t* __restrict__ items, uint32_t nitems, uint64_t n, const uint32_t* __restrict__ out_deg,
+    const double* __restrict__ out_str, const RT* __restrict__ rank, RT* __restrict__ next, CT* __restrict__ cnext,
+    const float* __restrict__ corr, PrPart* __restrict__ part, uint32_t* __restrict__ ctr, const PrCtl* ctl,
+    int parity, double base_arg, double damping, int K) {
+    constexpr int WAVES = BN_P2_BLOCK / 64;
+    constexpr int NW = sizeof(CT) == 8 ? 2 : 4;  // windows loaded before any is added
+    constexpr int PARTS = BnAcc<CT>::PARTS;
+    __shared__ unsigned long long s_acc[PARTS * BN_RSPAN];
+    __shared__ double s_red[WAVES];
+    __shared__ uint32_t s_it;
+    if (ctl->done) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[0] = 0;
+    const double base = parity >= 0 ? ctl->base[parity] : base_arg;
+    const float scale = sizeof(CT) == 4 ? __builtin_ldexpf(1.0f, K - 32) : 0.0f;
+    const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (;;) {
+        if (threadIdx.x == 0) s_it = atomicAdd(&ctr[1], 1u);
+        for (uint32_t i = threadIdx.x; i < PARTS * BN_RSPAN; i += BN_P2_BLOCK) s_acc[i] = 0ull;
+        __syncthreads();
+        const uint32_t it = s_it;
+        if (it >= nitems) break;
+        const uint32_t* I = items + 8 * (uint64_t)it;
+        const uint32_t b = I[0], q0 = I[1], q1 = I[2], rows = I[4];
+        const uint64_t r0 = I[3];
+        // A wave takes windows of BN_WIN slots, WAVES * BN_WIN apart; lane L
+        // takes the chunk at window offset 8 L (row indexes in one 16-byte load,
+        // terms in 16-byte loads). Chunks lie inside one tile and windows start
+        // on chunk boundaries, so a lane's 8 slots are all before the item's end
+        // or all after it; a lane past the end loads the item's first chunk and
+        // treats it as padding, so that no load sits behind a branch. NW
+        // windows are loaded before any is added.
+        const uint64_t k0 = (uint64_t)q0 * BN_CH, k1 = (uint64_t)q1 * BN_CH;
+        constexpr uint64_t STRIDE = (uint64_t)WAVES * BN_WIN;
+        for (uint64_t W = k0 + (uint64_t)wave * BN_WIN; W < k1; W += NW * STRIDE) {
+            uint32_t d[NW][8];
+            CT x[NW][8];
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                const uint64_t sl = W + w * STRIDE + 8 * lane;
+                const uint64_t sa = sl < k1 ? sl : k0;
+                const u32x4 dv = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(bdst + sa));
+                d[w][0] = dv.x & 0xFFFFu; d[w][1] = dv.x >> 16;
+                d[w][2] = dv.y & 0xFFFFu; d[w][3] = dv.y >> 16;
+                d[w][4] = dv.z & 0xFFFFu; d[w][5] = dv.z >> 16;
+                d[w][6] = dv.w & 0xFFFFu; d[w][7] = dv.w >> 16;
+                load8(term + sa, x[w]);
+            }
+#pragma unroll
+            for (int w = 0; w < NW; ++w)
+                if (W + w * STRIDE + 8 * lane >= k1) d[w][0] = BN_PAD;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) bn_add_lane<CT>(d[w], x[w], lane, scale, s_acc);
+        }
+        __syncthreads();
+        {
+            // The epilogue's loads for four of the thread's rows first, then the stores.
+            double delta = 0.0, dm = 0.0;
+            for (int k0r = 0; k0r < BN_EP; k0r += 4) {
+                double cur[4], sc[4], bb[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t i = threadIdx.x + (k0r + k) * BN_P2_BLOCK;
+                    const uint64_t u = r0 + (i < rows ? i : 0);  // no branch: row r0 always exists
+                    cur[k] = (double)rank[u];
+                    sc[k] = node_scale<WEIGHTED>(out_deg, out_str, u);
+                    bb[k] = base;
+                    if constexpr (CORR) bb[k] = base + (double)corr[u];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t i = threadIdx.x + (k0r + k) * BN_P2_BLOCK;
+                    if (i < rows) {
+                        double acc;
+                        if constexpr (PARTS == 2)
+                            acc = fx_value(s_acc[i], (long long)s_acc[BN_RSPAN + i]);
+                        else
+                            acc = __builtin_ldexp((double)(long long)s_acc[i], -K);
+                        pr_finish<RT, CT, DM>(r0 + i, acc, bb[k], damping, sc[k], cur[k], next, cnext, delta, dm);
+                    }
+                }
+            }
+            const double bd = block_sum(delta, s_red);
+            const double bm = block_sum(dm, s_red);
+            if (threadIdx.x == 0) part[b] = PrPart{bd, bm};
+        }
+        __syncthreads();
+    }
+}
+
+// Free-standing exact sum (ursa_hip_exact_sum): one workgroup adds the fixed
+// point parts of every x[i] into one accumulator.
+__global__ __launch_bounds__(256) void k_fx_sum(const double* __restrict__ x, uint64_t n, double* out) {
+    __shared__ unsigned long long s_lo, s_hi;
+    if (threadIdx.x == 0) {
+        s_lo = 0ull;
+        s_hi = 0ull;
+    }
+    __syncthreads();
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        unsigned long long lo;
+        long long hi;
+        fx_split(x[i], lo, hi);
+        atomicAdd(&s_lo, lo);
+        atomicAdd(&s_hi, (unsigned long long)hi);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *out = fx_value(s_lo, (long long)s_hi);
+}
+
+// ---- binned layout (bn_prepare) ----
+__global__ void k_bn_keys(const uint32_t* __restrict__ tgt, uint64_t m, uint32_t* __restrict__ key,
+                          uint32_t* __restrict__ iota) {
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < m; k += (uint64_t)gridDim.x * blockDim.x) {
+        key[k] = tgt[k] >> BN_SBITS;
+        iota[k] = (uint32_t)k;
+    }
+}
+
+// row_bin[r] = the bin holding row r (bstart: first row of each bin, nrb + 1 values).
+__global__ void k_bn_rowbin(const uint32_t* __restrict__ bstart, uint64_t nrb, uint64_t n,
+                            uint32_t* __restrict__ row_bin) {
+    for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t lo = 0, hi = nrb;  // bstart[lo] <= r < bstart[hi]
+        while (hi - lo > 1) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (bstart[mid] <= r)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        row_bin[r] = (uint32_t)lo;
+    }
+}
+
+// Unpadded binned slot j holds in-slot perm[j]; tile id = block * nrb + bin.
+__global__ void k_bn_tile(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ owner,
+                          const uint32_t* __restrict__ tgt, const uint32_t* __restrict__ row_bin, uint64_t m,
+                          uint64_t nrb, uint32_t* __restrict__ tile) {
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < m; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = perm[j];
+        tile[j] = (uint32_t)((uint64_t)(tgt[k] >> BN_SBITS) * nrb + row_bin[owner[k]]);
+    }
+}
+
+// toff[t] = first unpadded binned slot whose tile id is >= t, for t in [0, T];
+// the tile ids of the binned slots never decrease.
+__global__ void k_bn_toff(const uint32_t* __restrict__ tile, uint64_t m, uint64_t T, uint32_t* __restrict__ toff) {
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j <= m; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t lo = j == 0 ? 0 : (uint64_t)tile[j - 1] + 1;
+        const uint64_t hi = j == m ? T : (uint64_t)tile[j];
+        for (uint64_t t = lo; t <= hi; ++t) toff[t] = (uint32_t)j;
+    }
+}
+
+// Padded tile lengths (whole chunks); plen[T] = 0 so the scan's last value is the total.
+__global__ void k_bn_plen(const uint32_t* __restrict__ toff, uint64_t T, uint32_t* __restrict__ plen) {
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t <= T; t += (uint64_t)gridDim.x * blockDim.x)
+        plen[t] = t < T ? (toff[t + 1] - toff[t] + BN_CH - 1) / BN_CH * BN_CH : 0u;
+}
+
+// Padding slots: source 0, row BN_PAD, edge id 0 (both orders have mp slots).
+__global__ void k_bn_pad_fill(uint64_t mp, uint16_t* __restrict__ bsrc, uint16_t* __restrict__ bdst,
+                              uint32_t* __restrict__ beid) {
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < mp; p += (uint64_t)gridDim.x * blockDim.x) {
+        bsrc[p] = 0;
+        bdst[p] = (uint16_t)BN_PAD;
+        beid[p] = 0;
+    }
+}
+
+// Unpadded slot j (tile t) goes to poff[t] + (j - toff[t]) in the
+// source-major order (source index, edge id) and to bpos[b * nsb + c] +
+// (j - toff[t]) in the bin-major order (row index), t = c * nrb + b.
+__global__ void k_bn_place(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ owner,
+                           const uint32_t* __restrict__ tgt, const uint32_t* __restrict__ eid,
+                           const uint32_t* __restrict__ tile, const uint32_t* __restrict__ toff,
+                           const uint64_t* __restrict__ poff, const uint64_t* __restrict__ bpos,
+                           const uint32_t* __restrict__ bstart, uint64_t nsb, uint64_t nrb, uint64_t m,
+                           uint16_t* __restrict__ bsrc, uint16_t* __restrict__ bdst, uint32_t* __restrict__ beid) {
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < m; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = perm[j], t = tile[j];
+        const uint64_t off = j - toff[t], c = t / nrb, b = t % nrb;
+        bsrc[poff[t] + off] = (uint16_t)(tgt[k] & (BN_SSPAN - 1));
+        beid[poff[t] + off] = eid[k];
+        bdst[bpos[b * nsb + c] + off] = (uint16_t)(owner[k] - bstart[b]);
+    }
+}
+
+// Padded tile lengths in bin-major order (i = b * nsb + c); blen[T] = 0.
+__global__ void k_bn_blen(const uint32_t* __restrict__ plen, uint64_t nsb, uint64_t nrb, uint32_t* __restrict__ blen) {
+    const uint64_t T = nsb * nrb;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= T; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t b = i / nsb, c = i % nsb;
+        blen[i] = i < T ? plen[c * nrb + b] : 0u;
+    }
+}
+
+// Source-major chunk q -> its bin-major chunk: q lies in tile t (poff[t] <=
+// q * BN_CH < poff[t + 1]), whose bin-major run starts at bpos[b * nsb + c].
+__global__ void k_bn_places(const uint64_t* __restrict__ poff, const uint64_t* __restrict__ bpos, uint64_t nsb,
+                            uint64_t nrb, uint64_t Q, uint32_t* __restrict__ place) {
+    const uint64_t T = nsb * nrb;
+    for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < Q; q += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t p = q * BN_CH;
+        uint64_t lo = 0, hi = T;  // poff[lo] <= p < poff[hi]
+        while (hi - lo > 1) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (poff[mid] <= p)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const uint64_t c = lo / nrb, b = lo % nrb;
+        place[q] = (uint32_t)((bpos[b * nsb + c] + (p - poff[lo])) / BN_CH);
+    }
+}
+
 unsigned cu_count() {
     static int cus = [] {
         int dev = 0, v = 0;
@@ -781,6 +1265,131 @@ static void pr_prepare(ursa_hip_topo* t, const DevCsr& in, hipStream_t s) {
     t->pr_ready = true;
 }
 
+// Largest in-degree of the renumbered in-CSR (cached on the topology).
+static uint64_t pr_max_in_degree(ursa_hip_topo* t, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(t->cache_mu);
+    if (t->pr_max_in == ~0ull) {
+        std::vector<uint32_t> h(t->n + 1);
+        HIP_OK(hipMemcpyAsync(h.data(), t->pr_off.get(), (t->n + 1) * 4, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        uint64_t mx = 0;
+        for (uint64_t r = 0; r < t->n; ++r) mx = std::max<uint64_t>(mx, h[r + 1] - h[r]);
+        t->pr_max_in = mx;
+    }
+    return t->pr_max_in;
+}
+
+// Binned layout of PageRank's renumbered in-CSR (layout above the binned
+// kernels), built once per topology after pr_prepare. Work items are cut on
+// the host from the tile tables.
+static void bn_prepare(ursa_hip_topo* t, hipStream_t s) {
+    NoArena keep_out_of_arena;
+    std::lock_guard<std::mutex> lk(t->cache_mu);
+    if (t->bn_ready) return;
+    const uint64_t n = t->n, m = t->m;
+    // Bins: consecutive rows, at most BN_RSPAN of them and at most BN_T2
+    // in-slots (a longer row alone), from the renumbered row offsets.
+    std::vector<uint32_t> h_off(n + 1), h_bstart;
+    HIP_OK(hipMemcpyAsync(h_off.data(), t->pr_off.get(), (n + 1) * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    for (uint64_t r = 0; r < n;) {
+        h_bstart.push_back((uint32_t)r);
+        uint64_t e = r + 1;
+        while (e < n && e - r < BN_RSPAN && (uint64_t)h_off[e + 1] - h_off[r] <= BN_T2) ++e;
+        r = e;
+    }
+    h_bstart.push_back((uint32_t)n);
+    const uint64_t nsb = (n + BN_SSPAN - 1) / BN_SSPAN, nrb = h_bstart.size() - 1, T = nsb * nrb;
+    t->bn_bstart.alloc(nrb + 1);
+    HIP_OK(hipMemcpyAsync(t->bn_bstart.get(), h_bstart.data(), (nrb + 1) * 4, hipMemcpyHostToDevice, s));
+    DevBuf<uint32_t> toff(T + 1), perm, tile, owner;
+    if (m) {
+        perm.alloc(m);
+        tile.alloc(m);
+        owner.alloc(m);
+        {
+            DevBuf<uint32_t> key(m), skey(m), iota(m);
+            k_bn_keys<<<blocks_for(m, 256), 256, 0, s>>>(t->pr_tgt.get(), m, key.get(), iota.get());
+            check_launch();
+            prim::sort_pairs_u32(key.get(), skey.get(), iota.get(), perm.get(), m, bits_for(nsb), s);
+        }
+        slot_owners(n, m, t->pr_off.get(), owner.get(), s);
+        DevBuf<uint32_t> row_bin(n);
+        k_bn_rowbin<<<blocks_for(n, 256), 256, 0, s>>>(t->bn_bstart.get(), nrb, n, row_bin.get());
+        k_bn_tile<<<blocks_for(m, 256), 256, 0, s>>>(perm.get(), owner.get(), t->pr_tgt.get(), row_bin.get(), m, nrb,
+                                                     tile.get());
+        k_bn_toff<<<blocks_for(m + 1, 256), 256, 0, s>>>(tile.get(), m, T, toff.get());
+    } else {
+        k_bn_toff<<<1, 256, 0, s>>>(nullptr, 0, T, toff.get());
+    }
+    check_launch();
+    // Padded tile lengths; their running sums in source-major order (poff,
+    // indexed by t = c * nrb + b) and in bin-major order (bpos, by b * nsb + c).
+    DevBuf<uint32_t> plen(T + 1);
+    DevBuf<uint64_t> poff(T + 1), bpos(T + 1);
+    k_bn_plen<<<blocks_for(T + 1, 256), 256, 0, s>>>(toff.get(), T, plen.get());
+    check_launch();
+    prim::exclusive_sum_u32_to_u64(plen.get(), poff.get(), T + 1, s);
+    {
+        DevBuf<uint32_t> blen(T + 1);
+        k_bn_blen<<<blocks_for(T + 1, 256), 256, 0, s>>>(plen.get(), nsb, nrb, blen.get());
+        check_launch();
+        prim::exclusive_sum_u32_to_u64(blen.get(), bpos.get(), T + 1, s);
+    }
+    std::vector<uint64_t> h_poff(T + 1), h_bpos(T + 1);
+    HIP_OK(hipMemcpyAsync(h_poff.data(), poff.get(), (T + 1) * 8, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(h_bpos.data(), bpos.get(), (T + 1) * 8, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    const uint64_t mp = h_poff[T], Q = mp / BN_CH;
+    URSA_REQUIRE(Q <= 0xFFFFFFFFull, URSA_HIP_ERANGE, "binned PageRank layout exceeds 2^32 chunks");
+    t->bn_src.alloc(std::max<uint64_t>(mp, BN_CH));
+    t->bn_dst.alloc(std::max<uint64_t>(mp, BN_CH));
+    t->bn_eid.alloc(std::max<uint64_t>(mp, BN_CH));
+    t->bn_place.alloc(std::max<uint64_t>(Q, 1));
+    if (mp) {
+        k_bn_pad_fill<<<blocks_for(mp, 256), 256, 0, s>>>(mp, t->bn_src.get(), t->bn_dst.get(), t->bn_eid.get());
+        k_bn_place<<<blocks_for(m, 256), 256, 0, s>>>(perm.get(), owner.get(), t->pr_tgt.get(), t->pr_eid.get(),
+                                                      tile.get(), toff.get(), poff.get(), bpos.get(),
+                                                      t->bn_bstart.get(), nsb, nrb, m, t->bn_src.get(),
+                                                      t->bn_dst.get(), t->bn_eid.get());
+        k_bn_places<<<blocks_for(Q, 256), 256, 0, s>>>(poff.get(), bpos.get(), nsb, nrb, Q, t->bn_place.get());
+        check_launch();
+    }
+    HIP_OK(hipStreamSynchronize(s));
+
+    // Phase 1: each source block's chunks in pieces of at most BN_T1 slots.
+    std::vector<uint32_t> it1;
+    const uint64_t c1 = BN_T1 / BN_CH;
+    for (uint64_t c = 0; c < nsb; ++c) {
+        const uint64_t a = h_poff[c * nrb] / BN_CH, e = h_poff[(c + 1) * nrb] / BN_CH;
+        for (uint64_t q = a; q < e; q += c1)
+            it1.insert(it1.end(), {(uint32_t)c, (uint32_t)q, (uint32_t)std::min<uint64_t>(e, q + c1), 0u});
+    }
+    // Phase 2: one item per bin (bin, chunks, first row, rows), largest first.
+    struct It2 {
+        uint32_t b, q0, q1, r0, rows;
+    };
+    std::vector<It2> items;
+    for (uint64_t b = 0; b < nrb; ++b)
+        items.push_back({(uint32_t)b, (uint32_t)(h_bpos[b * nsb] / BN_CH), (uint32_t)(h_bpos[(b + 1) * nsb] / BN_CH),
+                         h_bstart[b], h_bstart[b + 1] - h_bstart[b]});
+    std::stable_sort(items.begin(), items.end(),
+                     [](const It2& x, const It2& y) { return x.q1 - x.q0 > y.q1 - y.q0; });
+    std::vector<uint32_t> it2;
+    for (const It2& x : items) it2.insert(it2.end(), {x.b, x.q0, x.q1, x.r0, x.rows, 0u, 0u, 0u});
+    t->bn_item1.alloc(std::max<size_t>(it1.size(), 4));
+    t->bn_item2.alloc(it2.size());
+    if (!it1.empty())
+        HIP_OK(hipMemcpyAsync(t->bn_item1.get(), it1.data(), it1.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(t->bn_item2.get(), it2.data(), it2.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_OK(hipStreamSynchronize(s));
+    t->bn_slots = mp;
+    t->bn_n1 = it1.size() / 4;
+    t->bn_n2 = items.size();
+    t->bn_bins = nrb;
+    t->bn_ready = true;
+}
+
 void pagerank_prepare(ursa_hip_topo* t, hipStream_t s) { pr_prepare(t, t->incoming(s), s); }
 
 // Solve PageRank (f64 solve, or with `mixed` the refinement solve described at
@@ -805,6 +1414,14 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
     }
     if (!(tol < PR_F32_TOL) || max_iter <= PR_F32_MIN_ITER) mixed = false;
     pr_prepare(t, t->incoming(s), s);
+    // Sweep schedule: binned (streamed terms, exact row sums) or the pull sweep.
+    // Bins hold at most BN_RSPAN rows and, on the heaviest rows, fewer: with m
+    // in-slots there are at most n / BN_RSPAN + m / BN_T2 + 1 of them.
+    const uint64_t bn_blocks = (n + BN_SSPAN - 1) / BN_SSPAN, bn_bins_max = n / BN_RSPAN + m / BN_T2 + 1;
+    const int sched = g_pr_schedule.load();
+    const bool bn_fits = bn_blocks * bn_bins_max <= BN_MAX_TILES && pr_max_in_degree(t, s) < BN_MAX_ROW;
+    const bool binned = bn_fits && (sched == 2 || (sched == 0 && bn_blocks >= BN_MIN_BLOCKS));
+    if (binned) bn_prepare(t, s);
     const uint64_t nchunks = t->pr_chunks, nz = t->pr_nz;
     const uint32_t* tgt = t->pr_tgt.get();
     const uint32_t* head = t->pr_head.get();
@@ -825,13 +1442,17 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
         k_gather_f64<<<blocks_for(n, 256), 256, 0, s>>>(os_old.get(), t->pr_old_of_new.get(), n, out_str.get());
         check_launch();
         HIP_OK(hipStreamSynchronize(s));
-        w64.alloc(m);
-        w32.alloc(m);
+        // Weights in the sweep's slot order (the binned order includes padding
+        // slots, whose edge id 0 is valid whenever there are edges).
+        const uint64_t wn = binned ? std::max<uint64_t>(t->bn_slots, BN_CH) : m;
+        w64.alloc(wn);
+        w32.alloc(wn);
         DevBuf<uint32_t> inexact(1);
         HIP_OK(hipMemsetAsync(inexact.get(), 0, sizeof(uint32_t), s));
-        if (m) {
-            k_permute_weights<<<blocks_for(m, 256), 256, 0, s>>>(t->pr_eid.get(), w.p, m, w64.get(), w32.get(),
-                                                                 inexact.get());
+        const uint64_t wcount = binned ? t->bn_slots : m;
+        if (m && wcount) {
+            k_permute_weights<<<blocks_for(wcount, 256), 256, 0, s>>>(binned ? t->bn_eid.get() : t->pr_eid.get(), w.p,
+                                                                      wcount, w64.get(), w32.get(), inexact.get());
             check_launch();
         }
         uint32_t h = 0;
@@ -871,6 +1492,26 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
     DevBuf<PrPart> part(nparts), fpart(std::max<uint64_t>(fix_blocks, (n + PR_FIX - 1) / PR_FIX));
     DevBuf<PrCtl> ctl(1);
     const unsigned init_blocks = blocks_for(n, 256, 1u << 30);
+    // Binned sweep state: the term arrays, per-bin partials and the two
+    // work-claim counters (cleared here, then by the kernels themselves, so
+    // recorded sweeps replay without a memset).
+    const uint64_t nrb = binned ? t->bn_bins : 0;
+    DevBuf<double> term64;
+    DevBuf<float> term32;
+    DevBuf<PrPart> bpart;
+    DevBuf<uint32_t> bctr;
+    unsigned g1 = 1, g2 = 1;
+    if (binned) {
+        term64.alloc(std::max<uint64_t>(t->bn_slots, BN_CH));
+        if (mixed) term32.alloc(std::max<uint64_t>(t->bn_slots, BN_CH));
+        bpart.alloc(nrb);
+        bctr.alloc(2);
+        HIP_OK(hipMemsetAsync(bctr.get(), 0, 8, s));
+        g1 = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(t->bn_n1, 4ull * cu_count()));
+        g2 = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(t->bn_n2, 2ull * cu_count()));
+    }
+    PrPart* sweep_part = binned ? bpart.get() : fpart.get();
+    const uint64_t sweep_parts = binned ? nrb : fix_blocks;
     const uint32_t* od = t->pr_outdeg.get();
     const double* os = weighted ? out_str.get() : nullptr;
 
@@ -940,6 +1581,17 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
                 if constexpr (sizeof(W) == 4) wp = w32.get();
                 else wp = w64.get();
             }
+            if (binned) {
+                if constexpr (HOT == 0) {
+                    k_bn_scatter<double, W, WT><<<g1, BN_P1_BLOCK, 0, s>>>(
+                        t->bn_src.get(), t->bn_place.get(), wp, t->bn_item1.get(), (uint32_t)t->bn_n1, n, cb,
+                        term64.get(), bctr.get(), ctl.get());
+                    k_bn_gather<double, double, WT, false, false><<<g2, BN_P2_BLOCK, 0, s>>>(
+                        t->bn_dst.get(), term64.get(), t->bn_item2.get(), (uint32_t)t->bn_n2, n,
+                        od, os, rk, nx, cn, nullptr, bpart.get(), bctr.get(), ctl.get(), par, base_arg, damping, 0);
+                }
+                return;
+            }
             k_pr_sweep<double, double, W, WT, false, false, BLOCK, HOT><<<g64, BLOCK, 0, s>>>(
                 tgt, wp, head, crow, m, nchunks, od, os, rk, cb, nx, cn, nullptr, lead.get(), tail.get(), part.get(),
                 ctl.get(), par, base_arg, damping);
@@ -952,14 +1604,15 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
             using T = std::true_type;
             using Fl = std::false_type;
             if (!weighted) {
-                if (hot64) go(H1{}, D{}, Fl{}); else go(H0{}, D{}, Fl{});
+                if (hot64 && !binned) go(H1{}, D{}, Fl{}); else go(H0{}, D{}, Fl{});
             } else if (use_f32w) {
-                if (hot64) go(H1{}, F{}, T{}); else go(H0{}, F{}, T{});
+                if (hot64 && !binned) go(H1{}, F{}, T{}); else go(H0{}, F{}, T{});
             } else {
-                if (hot64) go(H1{}, D{}, T{}); else go(H0{}, D{}, T{});
+                if (hot64 && !binned) go(H1{}, D{}, T{}); else go(H0{}, D{}, T{});
             }
         });
         check_launch();
+        if (binned) return;
         timed("pagerank_fixup", 0.0, s, [&] {
             if (weighted)
                 k_pr_fixup<double, double, true, false, false><<<fix_blocks, PR_FIX, 0, s>>>(
@@ -976,12 +1629,28 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
     };
     // f32 sweep xin -> xout (host-driven base; corr = residual of a correction sweep).
     auto sweep32 = [&](const float* xin, const float* cin_, float* xout, float* cout, const float* corr,
-                       double base_arg) {
+                       double base_arg, double l1_bound) {
+        // Binned f32 sweeps: fixed-point scale 2^K with K = 60 - ceil(log2 B), B a
+        // bound on the L1 norm of the iterate (so on every sum of its terms).
+        int fx_k = 149;
+        if (l1_bound > 0.0) fx_k = std::min(149, std::max(1, 60 - (int)std::ceil(std::log2(l1_bound))));
         auto go = [&](auto hot_tag, auto wt_tag, auto corr_tag) {
             constexpr int HOT = decltype(hot_tag)::value;
             constexpr bool WT = decltype(wt_tag)::value;
             constexpr bool CORR = decltype(corr_tag)::value;
             constexpr int BLOCK = HOT ? PR_HOT_BLOCK : PR_PLAIN_BLOCK;
+            if (binned) {
+                if constexpr (HOT == 0) {
+                    k_bn_scatter<float, float, WT><<<g1, BN_P1_BLOCK, 0, s>>>(
+                        t->bn_src.get(), t->bn_place.get(), WT ? w32.get() : nullptr, t->bn_item1.get(),
+                        (uint32_t)t->bn_n1, n, cin_, term32.get(), bctr.get(), ctl.get());
+                    k_bn_gather<float, float, WT, CORR, true><<<g2, BN_P2_BLOCK, 0, s>>>(
+                        t->bn_dst.get(), term32.get(), t->bn_item2.get(), (uint32_t)t->bn_n2, n,
+                        od, os, xin, xout, cout, corr, bpart.get(), bctr.get(), ctl.get(), -1, base_arg, damping,
+                        fx_k);
+                }
+                return;
+            }
             k_pr_sweep<float, float, float, WT, CORR, true, BLOCK, HOT><<<g32, BLOCK, 0, s>>>(
                 tgt, WT ? w32.get() : nullptr, head, crow, m, nchunks, od, os, xin, cin_, xout, cout, corr,
                 lead.get(), tail.get(), part.get(), ctl.get(), -1, base_arg, damping);
@@ -997,7 +1666,7 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
         const bool c = corr != nullptr;
         timed(c ? "pagerank_spmv_f32c" : "pagerank_spmv_f32",
               f32_edge * (double)m + (f32_node + (c ? 4.0 : 0.0)) * (double)n, s, [&] {
-                  if (hot32) {
+                  if (hot32 && !binned) {
                       if (weighted) { if (c) go(H1{}, T{}, T{}); else go(H1{}, T{}, Fl{}); }
                       else { if (c) go(H1{}, Fl{}, T{}); else go(H1{}, Fl{}, Fl{}); }
                   } else {
@@ -1008,7 +1677,7 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
         check_launch();
     };
     auto reduce = [&](uint32_t mi, double tol_it) {
-        k_pr_reduce<<<1, 1024, 0, s>>>(fpart.get(), fix_blocks, ctl.get(), mi, tol_it);
+        k_pr_reduce<<<1, 1024, 0, s>>>(sweep_part, sweep_parts, ctl.get(), mi, tol_it);
         check_launch();
     };
     auto read_ctl = [&]() {
@@ -1065,8 +1734,9 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
         double prev = 0.0;
         int p = 0;
         for (;;) {
+            // The iterate is a probability vector: its L1 norm is 1 up to f32 rounding.
             sweep32(x32[p].get(), c32[p].get(), x32[p ^ 1].get(), c32[p ^ 1].get(), nullptr,
-                    (1.0 - damping) / nf + damping * dm / nf);
+                    (1.0 - damping) / nf + damping * dm / nf, 1.01);
             reduce(NO_LIMIT, -1.0);
             const PrCtl h = read_ctl();
             ++iters;
@@ -1106,9 +1776,12 @@ void pagerank_run(ursa_hip_topo* t, const double* weights_user, double damping, 
             check_launch();
             double dme = read_ctl().dm;
             int pe = 0;
+            // Every correction iterate has L1 norm at most |r|_1 / (1 - d), |r|_1 =
+            // the f64 sweep's delta (A and the dangling spread do not grow L1).
+            const double e_bound = 1.01 * h.delta / (1.0 - damping);
             for (uint32_t j = 0; j < PR_MAX_CORR && iters + 1 < max_iter; ++j) {
                 sweep32(x32[pe].get(), c32[pe].get(), x32[pe ^ 1].get(), c32[pe ^ 1].get(), r32.get(),
-                        damping * dme / nf);
+                        damping * dme / nf, e_bound);
                 reduce(NO_LIMIT, -1.0);
                 const PrCtl hc = read_ctl();
                 ++iters;
@@ -1138,6 +1811,31 @@ extern "C" int ursa_hip_pagerank(ursa_hip_topo* t, const double* weights, double
     return ursa::guard([&] {
         URSA_REQUIRE(t != nullptr, URSA_HIP_EINVAL, "topology is NULL");
         ursa::pagerank_run(t, weights, damping, max_iter, tol, out, iters_out, false, nullptr);
+    });
+}
+
+extern "C" int ursa_hip_pagerank_schedule(int mode) {
+    if (mode < 0 || mode > 2) {
+        ursa::set_last_error("schedule must be 0 (automatic), 1 (pull) or 2 (binned)");
+        return URSA_HIP_EINVAL;
+    }
+    ursa::g_pr_schedule.store(mode);
+    return URSA_HIP_OK;
+}
+
+extern "C" int ursa_hip_exact_sum(const double* x, uint64_t n, double* out) {
+    return ursa::guard([&] {
+        URSA_REQUIRE(out != nullptr, URSA_HIP_EINVAL, "out is NULL");
+        URSA_REQUIRE(n == 0 || x != nullptr, URSA_HIP_EINVAL, "x is NULL");
+        for (uint64_t i = 0; i < n && !ursa::is_device_ptr(x); ++i)
+            URSA_REQUIRE(std::fabs(x[i]) < 4.0, URSA_HIP_ERANGE, "exact_sum takes values of magnitude below 4");
+        hipStream_t s = ursa::stream();
+        ursa::DevIn<double> xd(x, n, s);
+        ursa::DevBuf<double> res(1);
+        ursa::k_fx_sum<<<1, 256, 0, s>>>(xd.p, n, res.get());
+        ursa::check_launch();
+        HIP_OK(hipMemcpyAsync(out, res.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
     });
 }
 

@@ -150,13 +150,16 @@ void offsets_from_sorted(const uint32_t* keys, uint64_t m, uint64_t n, uint32_t*
     check_launch();
 }
 
-void slot_owners(uint64_t n, const DevCsr& csr, uint32_t* owner, hipStream_t s) {
-    uint64_t m = csr.slots;
+void slot_owners(uint64_t n, uint64_t m, const uint32_t* off, uint32_t* owner, hipStream_t s) {
     DevBuf<uint32_t> flags(m);
     HIP_OK(hipMemsetAsync(flags.get(), 0, m * sizeof(uint32_t), s));
-    k_head_flags<<<blocks_for(n, B), B, 0, s>>>(csr.off.get(), n, flags.get());
+    k_head_flags<<<blocks_for(n, B), B, 0, s>>>(off, n, flags.get());
     check_launch();
     prim::inclusive_max_scan_u32(flags.get(), owner, m, s);
+}
+
+void slot_owners(uint64_t n, const DevCsr& csr, uint32_t* owner, hipStream_t s) {
+    slot_owners(n, csr.slots, csr.off.get(), owner, s);
 }
 
 }  // namespace ursa

@@ -31,6 +31,8 @@ void build_grouped(uint64_t n, uint64_t m, const uint32_t* keys, const uint32_t*
 
 // owner[k] = node whose segment holds slot k of `csr` (needs csr.slots > 0).
 void slot_owners(uint64_t n, const DevCsr& csr, uint32_t* owner, hipStream_t s);
+// The same for offsets off[n+1] over m > 0 slots.
+void slot_owners(uint64_t n, uint64_t m, const uint32_t* off, uint32_t* owner, hipStream_t s);
 
 // off[u] = first index whose key >= u (u in [0, n]) for keys sorted ascending.
 void offsets_from_sorted(const uint32_t* keys, uint64_t m, uint64_t n, uint32_t* off, hipStream_t s);
@@ -61,6 +63,17 @@ struct ursa_hip_topo {
     ursa::DevBuf<uint8_t> pr_cflag;
     uint64_t pr_nz = 0, pr_chunks = 0;
     bool pr_ready = false;
+    uint64_t pr_max_in = ~0ull;  // largest in-degree (computed on first use)
+    // PageRank's binned sweep (see pagerank.hip): the renumbered in-slots in
+    // tiles (block of 8192 sources, bin of up to 4096 consecutive rows; rows
+    // ascending), each padded to whole chunks of 8 slots; source-block-major:
+    // each slot's source index in its block (u16) and edge id, and each
+    // chunk's place in the bin-major order; bin-major: each slot's row index
+    // in its bin (u16); the first row of each bin; the work items of both phases.
+    ursa::DevBuf<uint16_t> bn_src, bn_dst;
+    ursa::DevBuf<uint32_t> bn_eid, bn_place, bn_bstart, bn_item1, bn_item2;
+    uint64_t bn_slots = 0, bn_n1 = 0, bn_n2 = 0, bn_bins = 0;
+    bool bn_ready = false;
     // Multi-source BFS pull side, per direction (see msbfs.hip): the vertices
     // whose reverse lists are cut into chunks and their chunk bases; depends
     // on the graph only.
