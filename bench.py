@@ -597,36 +597,53 @@ def bench_pagerank(args, ua, gen):
         check(lib.ursa_hip_pagerank_mixed(topo._h, None, params.damping, params.max_iter, params.tol, out.p,
                                           C.byref(it), C.byref(it32)))
 
-    res = {}
-    for name, fn in (("f64", plain), ("mixed", mixed)):
+    def timed_solve(fn):
         fn()
         check(lib.ursa_hip_synchronize())
         t1 = time.perf_counter()
         for _ in range(args.pr_steps):
             fn()
         check(lib.ursa_hip_synchronize())
-        ms = (time.perf_counter() - t1) / args.pr_steps * 1e3
-        res[name] = {"ms": round(ms, 3), "iters": it.value, "GTEPS": round(m * it.value / (ms / 1e3) / 1e9, 3)}
-        if name == "mixed":
-            res[name]["f32_iters"] = it32.value
-        log(f"pagerank {name}: {ms:.2f} ms, {it.value} iterations")
-    # kernel roofline of the f64 SpMV tile kernel
-    ua.profile.enable(True)
-    ua.profile.reset()
-    plain()
-    q = ua.profile.query("pagerank_spmv")
-    mixed()
-    q32 = ua.profile.query("pagerank_spmv_f32")
-    ua.profile.enable(False)
-    for key, qq in (("k_pr_sweep_f64", q), ("k_pr_sweep_f32", q32)):
-        if qq["launches"]:
-            avg = qq["total_ms"] / qq["launches"]
-            bpl = qq["algo_bytes"] / qq["launches"]
-            gbs = bpl / (avg / 1e3) / 1e9
-            res[key] = {"avg_launch_us": round(avg * 1e3, 2), "algo_bytes_per_launch": bpl, "GBps": round(gbs, 1),
-                        "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
-                        "traffic": traffic_from_profiles(f"pagerank_rmat{scale}", key)}
-    res["mixed_speedup"] = round(res["f64"]["ms"] / res["mixed"]["ms"], 3)
+        return (time.perf_counter() - t1) / args.pr_steps * 1e3
+
+    def sweep_roofline(qq, traffic_key):
+        if not qq["launches"]:
+            return None
+        avg = qq["total_ms"] / qq["launches"]
+        bpl = qq["algo_bytes"] / qq["launches"]
+        gbs = bpl / (avg / 1e3) / 1e9
+        return {"avg_launch_us": round(avg * 1e3, 2), "algo_bytes_per_launch": bpl, "GBps": round(gbs, 1),
+                "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profiles(f"pagerank_rmat{scale}", traffic_key)}
+
+    # Two sweep schedules (ursa_hip_pagerank_schedule): binned (streamed terms,
+    # exact fixed-point row sums; the automatic choice at this size) and pull.
+    res = {"pull": {}}
+    for sched, sname in ((2, "binned"), (1, "pull")):
+        check(lib.ursa_hip_pagerank_schedule(sched))
+        dst = res if sched == 2 else res["pull"]
+        for name, fn in (("f64", plain), ("mixed", mixed)):
+            ms = timed_solve(fn)
+            dst[name] = {"ms": round(ms, 3), "iters": it.value, "GTEPS": round(m * it.value / (ms / 1e3) / 1e9, 3),
+                         "schedule": sname}
+            if name == "mixed":
+                dst[name]["f32_iters"] = it32.value
+            log(f"pagerank {sname} {name}: {ms:.2f} ms, {it.value} iterations")
+        # per-sweep kernel time (the binned sweep's two phases count as one sweep)
+        ua.profile.enable(True)
+        ua.profile.reset()
+        plain()
+        q = ua.profile.query("pagerank_spmv")
+        mixed()
+        q32 = ua.profile.query("pagerank_spmv_f32")
+        ua.profile.enable(False)
+        pre = "k_bn_sweep" if sched == 2 else "k_pr_sweep"
+        dst[pre + "_f64"] = sweep_roofline(q, pre + "_f64")
+        dst[pre + "_f32"] = sweep_roofline(q32, pre + "_f32")
+        dst["mixed_speedup"] = round(dst["f64"]["ms"] / dst["mixed"]["ms"], 3)
+    check(lib.ursa_hip_pagerank_schedule(0))
+    res["mixed_speedup_vs_pull_f64"] = round(res["pull"]["f64"]["ms"] / res["mixed"]["ms"], 3)
+    res["binned_f64_speedup_vs_pull"] = round(res["pull"]["f64"]["ms"] / res["f64"]["ms"], 3)
     res["c1"] = bench_c1(args, ua, gen)
     res["bfs"] = bench_bfs_large(ua, lib, topo, n, m)
     res["egress"] = bench_egress(ua, lib, topo, n, m, out)
@@ -642,17 +659,16 @@ def bench_pagerank(args, ua, gen):
                                           C.byref(it), C.byref(it32)))
 
     for name, fn in (("weighted_f64", plain_w), ("weighted_mixed", mixed_w)):
-        fn()
-        check(lib.ursa_hip_synchronize())
-        t1 = time.perf_counter()
-        for _ in range(args.pr_steps):
-            fn()
-        check(lib.ursa_hip_synchronize())
-        ms = (time.perf_counter() - t1) / args.pr_steps * 1e3
+        ms = timed_solve(fn)
         res[name] = {"ms": round(ms, 3), "iters": it.value, "GTEPS": round(m * it.value / (ms / 1e3) / 1e9, 3)}
         if name.endswith("mixed"):
             res[name]["f32_iters"] = it32.value
         log(f"pagerank {name}: {ms:.2f} ms, {it.value} iterations")
+    check(lib.ursa_hip_pagerank_schedule(1))
+    ms = timed_solve(plain_w)
+    res["pull"]["weighted_f64"] = {"ms": round(ms, 3), "iters": it.value, "schedule": "pull"}
+    log(f"pagerank pull weighted_f64: {ms:.2f} ms, {it.value} iterations")
+    check(lib.ursa_hip_pagerank_schedule(0))
     res["weighted_mixed_speedup"] = round(res["weighted_f64"]["ms"] / res["weighted_mixed"]["ms"], 3)
     # CPU leg: the oracle's f64 solves (the reference's pull loop restated) timed on
     # this host, and the accuracy of every GPU solve against them.
@@ -661,14 +677,19 @@ def bench_pagerank(args, ua, gen):
 
         threads = cpu_threads()
         got = {}
-        for name, wp, fn in (("f64", None, lib.ursa_hip_pagerank), ("mixed", None, lib.ursa_hip_pagerank_mixed),
-                             ("weighted_f64", w_dev.p, lib.ursa_hip_pagerank),
-                             ("weighted_mixed", w_dev.p, lib.ursa_hip_pagerank_mixed)):
+        for name, wp, fn, sched in (("f64", None, lib.ursa_hip_pagerank, 0),
+                                    ("mixed", None, lib.ursa_hip_pagerank_mixed, 0),
+                                    ("weighted_f64", w_dev.p, lib.ursa_hip_pagerank, 0),
+                                    ("weighted_mixed", w_dev.p, lib.ursa_hip_pagerank_mixed, 0),
+                                    ("pull_f64", None, lib.ursa_hip_pagerank, 1),
+                                    ("pull_mixed", None, lib.ursa_hip_pagerank_mixed, 1)):
+            check(lib.ursa_hip_pagerank_schedule(sched))
             g = np.zeros(n, np.float64)
             extra = (C.byref(it32),) if name.endswith("mixed") else ()
             check(fn(topo._h, wp, params.damping, params.max_iter, params.tol, g.ctypes.data_as(C.c_void_p),
                      C.byref(it), *extra))
-            got[name] = g
+            got[name] = (g, it.value)
+        check(lib.ursa_hip_pagerank_schedule(0))
         w_host = np.zeros(m, np.float64)
         check(lib.ursa_hip_memcpy_to_host(w_host.ctypes.data_as(C.c_void_p), w_dev.p, 8 * m))
         t2 = time.time()
@@ -686,10 +707,11 @@ def bench_pagerank(args, ua, gen):
                                  "GTEPS": round(m * rit / cpu_s / 1e9, 4)}
             log(f"pagerank oracle {kind or 'unweighted '}: {rit} iterations in {cpu_s:.1f}s on {threads} threads")
         log(f"pagerank oracle setup {time.time() - t2:.0f}s")
-        for name, g in got.items():
+        for name, (g, git) in got.items():
             ref, rit = refs["weighted_" if name.startswith("weighted") else ""]
             rel = np.abs(g - ref) / np.abs(ref)
-            r = res[name]
+            r = res["pull"][name[5:]] if name.startswith("pull_") else res[name]
+            r["iters_checked"] = git
             r["L1_vs_oracle"] = float(np.abs(g - ref).sum())
             r["maxabs_vs_oracle"] = float(np.abs(g - ref).max())
             r["max_rel_vs_oracle"] = float(rel.max())
@@ -702,15 +724,22 @@ def bench_pagerank(args, ua, gen):
         res["cpu_oracle"] = cpu
         res["parity"] = {
             "f64_max_rel": res["f64"]["max_rel_vs_oracle"],
-            "f64_iters_equal": res["f64"]["iters"] == refs[""][1],
+            "f64_iters_equal": got["f64"][1] == refs[""][1],
             "weighted_f64_max_rel": res["weighted_f64"]["max_rel_vs_oracle"],
-            "weighted_f64_iters_equal": res["weighted_f64"]["iters"] == refs["weighted_"][1],
+            "weighted_f64_iters_equal": got["weighted_f64"][1] == refs["weighted_"][1],
             "mixed_L1": res["mixed"]["L1_vs_oracle"],
             "weighted_mixed_L1": res["weighted_mixed"]["L1_vs_oracle"],
+            "pull_f64_max_rel": res["pull"]["f64"]["max_rel_vs_oracle"],
+            "pull_f64_iters_equal": got["pull_f64"][1] == refs[""][1],
+            "pull_mixed_L1": res["pull"]["mixed"]["L1_vs_oracle"],
             "within_bars": bool(res["f64"]["max_rel_vs_oracle"] <= 1e-12
                                 and res["weighted_f64"]["max_rel_vs_oracle"] <= 1e-12
+                                and res["pull"]["f64"]["max_rel_vs_oracle"] <= 1e-12
                                 and res["mixed"]["L1_vs_oracle"] <= 1e-9
-                                and res["weighted_mixed"]["L1_vs_oracle"] <= 1e-9),
+                                and res["pull"]["mixed"]["L1_vs_oracle"] <= 1e-9
+                                and res["weighted_mixed"]["L1_vs_oracle"] <= 1e-9
+                                and got["f64"][1] == refs[""][1] and got["pull_f64"][1] == refs[""][1]
+                                and got["weighted_f64"][1] == refs["weighted_"][1]),
         }
         log(f"pagerank parity: {json.dumps(res['parity'])}")
         del w_host, ot

@@ -766,7 +766,12 @@ static_assert(BN_RSPAN % (4 * BN_P2_BLOCK) == 0, "epilogue rows per thread: grou
 // instruction of the wave covers one contiguous span.
 constexpr uint32_t BN_WIN = 512;
 
-// 8 consecutive values from a 16-byte aligned address in 16-byte loads.
+// 8 consecutive values from a 16-byte aligned address in 16-byte loads. A
+// lane's pieces are 32 or 64 bytes apart, so each load instruction of a wave
+// covers its lines only partly and the following ones complete them. For f64
+// (four instructions per line set) plain loads keep the lines in L2 in
+// between, where streaming loads let them go and read them again (30 % more
+// bytes, 1.6x the time at R-MAT scale 24); f32 (two instructions) streams.
 template <class T>
 __device__ __forceinline__ void load8(const T* p, T (&v)[8]) {
     if constexpr (sizeof(T) == 8) {
@@ -774,7 +779,7 @@ __device__ __forceinline__ void load8(const T* p, T (&v)[8]) {
         const d2* q = reinterpret_cast<const d2*>(p);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const d2 t = __builtin_nontemporal_load(q + i);
+            const d2 t = q[i];
             v[2 * i] = t.x;
             v[2 * i + 1] = t.y;
         }
@@ -792,21 +797,36 @@ __device__ __forceinline__ void load8(const T* p, T (&v)[8]) {
     }
 }
 
-template <class T>
-struct Pair;
-template <>
-struct Pair<double> {
-    typedef double type __attribute__((ext_vector_type(2)));
+// G consecutive values of T (G * sizeof(T) = 4, 8 or 16 bytes) at an aligned address.
+template <class T, int G>
+struct Vec {
+    T v[G];
 };
-template <>
-struct Pair<float> {
-    typedef float type __attribute__((ext_vector_type(2)));
-};
+template <class T, int G>
+__device__ __forceinline__ Vec<T, G> load_vec(const T* p) {
+    typedef T V __attribute__((ext_vector_type(G)));
+    const V x = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+    Vec<T, G> r;
+#pragma unroll
+    for (int i = 0; i < G; ++i) r.v[i] = x[i];
+    return r;
+}
+template <class T, int G>
+__device__ __forceinline__ void store_vec(T* p, const Vec<T, G>& a) {
+    typedef T V __attribute__((ext_vector_type(G)));
+    V x;
+#pragma unroll
+    for (int i = 0; i < G; ++i) x[i] = a.v[i];
+    *reinterpret_cast<V*>(p) = x;  // plain store: the wave's stores to a tile's run merge in L2
+}
 
 // Phase 1: terms of the chunks of items (block c, source-major chunks
 // [q0, q1)), each slot's term stored at its chunk's bin-major place
 // place[chunk] * BN_CH + (slot % BN_CH), claimed through ctr[0]; also clears
-// phase 2's claim counter ctr[1] for this sweep.
+// phase 2's claim counter ctr[1] for this sweep. A wave takes windows of
+// BN_WIN slots; in each of BN_WIN / (64 G) steps lane L handles the G
+// consecutive slots at step * 64 G + G L (G = 16 bytes / term size), so every
+// load and store instruction of the wave covers one contiguous span.
 template <class CT, class W, bool WEIGHTED>
 __global__ __launch_bounds__(BN_P1_BLOCK) void k_bn_scatter(const uint16_t* __restrict__ bsrc,
                                                             const uint32_t* __restrict__ place,
@@ -816,8 +836,8 @@ __global__ __launch_bounds__(BN_P1_BLOCK) void k_bn_scatter(const uint16_t* __re
                                                             CT* __restrict__ term, uint32_t* __restrict__ ctr,
                                                             const PrCtl* ctl) {
     constexpr int WAVES = BN_P1_BLOCK / 64;
-    typedef typename Pair<CT>::type CT2;
-    typedef typename Pair<W>::type W2;
+    constexpr int G = 16 / sizeof(CT);
+    constexpr int STEPS = BN_WIN / (64 * G);
     __shared__ CT s_c[BN_SSPAN];
     __shared__ uint32_t s_it;
     if (ctl->done) return;
@@ -837,28 +857,28 @@ __global__ __launch_bounds__(BN_P1_BLOCK) void k_bn_scatter(const uint16_t* __re
         for (uint64_t W0 = k0 + (uint64_t)wave * BN_WIN; W0 < k1; W0 += (uint64_t)WAVES * BN_WIN) {
             // Loads without branches (slots past the item read its first slot and
             // store nothing), so that all of them are issued before the first wait.
-            uint32_t sp[4], pl[4];
-            W2 wv[4];
+            Vec<uint16_t, G> sv[STEPS];
+            Vec<W, G> wv[STEPS];
+            uint32_t pl[STEPS];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const uint64_t sl = W0 + 128 * i + 2 * lane;
+            for (int i = 0; i < STEPS; ++i) {
+                const uint64_t sl = W0 + (uint64_t)i * 64 * G + (uint64_t)G * lane;
                 const uint64_t sa = sl < k1 ? sl : k0;
-                sp[i] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(bsrc + sa));
+                sv[i] = load_vec<uint16_t, G>(bsrc + sa);
                 pl[i] = place[sa / BN_CH];
-                if constexpr (WEIGHTED) wv[i] = __builtin_nontemporal_load(reinterpret_cast<const W2*>(w + sa));
+                if constexpr (WEIGHTED) wv[i] = load_vec<W, G>(w + sa);
             }
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const uint64_t sl = W0 + 128 * i + 2 * lane;
+            for (int i = 0; i < STEPS; ++i) {
+                const uint64_t sl = W0 + (uint64_t)i * 64 * G + (uint64_t)G * lane;
                 if (sl < k1) {
-                    CT2 v;
-                    v.x = s_c[sp[i] & 0xFFFFu];
-                    v.y = s_c[sp[i] >> 16];
-                    if constexpr (WEIGHTED) {
-                        v.x = (CT)wv[i].x * v.x;
-                        v.y = (CT)wv[i].y * v.y;
+                    Vec<CT, G> v;
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        v.v[g] = s_c[sv[i].v[g]];
+                        if constexpr (WEIGHTED) v.v[g] = (CT)wv[i].v[g] * v.v[g];
                     }
-                    *reinterpret_cast<CT2*>(term + (uint64_t)pl[i] * BN_CH + (sl % BN_CH)) = v;
+                    store_vec<CT, G>(term + (uint64_t)pl[i] * BN_CH + (sl % BN_CH), v);
                 }
             }
         }
@@ -951,7 +971,14 @@ __device__ __forceinline__ void bn_add_lane(const uint32_t (&d)[8], const CT (&x
         }
         acc.add(x[j], scale);
     }
-    const uint32_t prev = __shfl_up(cur, 1, 64), next = __shfl_down(cur, 1, 64);
+    const uint32_t next = __shfl_down(cur, 1, 64);
+    if (__ballot(lane < 63 && next == cur && cur != BN_PAD) == 0) {
+        // No last run continues into the next lane (the usual case off the hub
+        // rows): every lane adds its own.
+        if (cur != BN_PAD) acc.flush(cur, s);
+        return;
+    }
+    const uint32_t prev = __shfl_up(cur, 1, 64);
     bool start = lane == 0 || prev != cur;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
