@@ -121,6 +121,13 @@ int ursa_hip_ppr(ursa_hip_topo* t, const uint32_t* seeds, uint64_t nseeds, doubl
 int ursa_hip_ppr_batch(ursa_hip_topo* t, uint32_t k, const uint64_t* seed_offsets,
                        const uint32_t* seeds, double damping, uint32_t max_iter, double tol,
                        double* out, uint32_t* iters_out);
+/* Personalized PageRank sweep schedule (default 0): 0 automatic (binned on
+ * graphs of 32 or more blocks of 8192 vertices with no row above 256
+ * in-edges and at least 8 edges per vertex, else pull), 1 pull, 2 binned (each lane's sweep streamed through
+ * PageRank's source-block x row-bin tiling, exact fixed-point row sums). The
+ * schedule never depends on the batch, so the bit-identity above holds under
+ * every schedule; pull and binned differ in the last bits. */
+int ursa_hip_ppr_schedule(int mode);
 
 /* Streaming device-to-device copy of `bytes` (multiple of 16) by a plain
  * kernel: the reference point for hardware byte counters (k_copy_probe moves

@@ -146,6 +146,133 @@ def test_ppr_edge_cases(ua):
     assert x.size == 0 and it == 0
 
 
+# ------------------------------------------------- binned PPR schedule
+@pytest.fixture
+def ppr_binned(ua):
+    """Forces the binned sweep (ursa_hip_ppr_schedule 2) for one test."""
+    lib = ua.load()
+    ua.check(lib.ursa_hip_ppr_schedule(2))
+    yield
+    ua.check(lib.ursa_hip_ppr_schedule(0))
+
+
+@pytest.mark.parametrize("case", GRAPHS, ids=IDS)
+def test_ppr_binned_matches_oracle(ua, case, ppr_binned):
+    name, n, s, d = case
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    for seeds in seed_sets(n):
+        got, it = ua.ppr(t, seeds, 0.85, 1000, 1e-12)
+        ref, rit = orc.ppr(o, seeds, 0.85, 1000, 1e-12)
+        assert np.abs(got - ref).sum() <= 1e-10, (name, seeds)
+        assert np.abs(got - ref).max() <= 1e-11, (name, seeds)
+        assert it > 0 and rit > 0, name
+
+
+@pytest.mark.parametrize("max_iter", [1, 3, 100])
+@pytest.mark.parametrize("tol", [1e-4, 1e-8])
+def test_ppr_binned_f64_lanes_follow_the_oracle(ua, max_iter, tol, ppr_binned):
+    """As the pull schedule: f64 lanes keep the oracle's iteration count and values
+    within 1e-11 relative (their row sums are exact, then rounded once)."""
+    n, s, d = rmat_dense(12, 3)
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    for seeds in seed_sets(n):
+        got, it = ua.ppr(t, seeds, 0.85, max_iter, tol)
+        ref, rit = orc.ppr(o, seeds, 0.85, max_iter, tol)
+        if tol < 2e-6 and max_iter > 16:
+            assert np.abs(got - ref).sum() <= 2e-7, seeds
+            continue
+        assert it == rit, (seeds, it, rit)
+        assert np.all(np.abs(got - ref) <= 1e-11 * np.abs(ref)), seeds
+
+
+def test_ppr_binned_serving_tolerance_and_batches(ua, ppr_binned):
+    """Serving tolerance against the oracle, and every lane of a batch bit-identical
+    to the query alone: each lane runs its own binned passes."""
+    n, s, d = rmat_dense(13, 6)
+    t = ua.Topology.build(n, s, d)
+    o = orc.Topology(n, s, d)
+    rng = np.random.default_rng(2)
+    lists = [rng.integers(0, n, int(rng.integers(1, 17))).tolist() for _ in range(18)]
+    lists[5] = [n + 3]
+    alone = [ua.ppr(t, sl, 0.85, 100, 1e-8) for sl in lists]
+    for sl, (got, _) in list(zip(lists, alone))[:4]:
+        ref, _ = orc.ppr(o, sl, 0.85, 100, 1e-8)
+        assert np.abs(got - ref).sum() <= 2e-7
+        assert abs(got.sum() - 1.0) < 1e-8 / 0.15
+    for group in (lists[:16], lists[2:5], lists[16:]):
+        out, iters = ua.ppr_batch(t, group, 0.85, 100, 1e-8)
+        for q, sl in enumerate(group):
+            ref, rit = alone[lists.index(sl)]
+            assert np.array_equal(out[q], ref) and iters[q] == rit, q
+    again = ua.ppr(t, lists[0], 0.85, 100, 1e-8)
+    assert np.array_equal(again[0], alone[0][0])
+
+
+def test_ppr_binned_networkx_fixtures_and_edge_cases(ua, ppr_binned):
+    from graphs import FX
+
+    for name in ("sparse400", "gnp100"):
+        n, s, d, users = orc.intern_i64(FX[f"{name}_src"], FX[f"{name}_dst"])
+        dense = {int(u): i for i, u in enumerate(users)}
+        t = ua.Topology.build(n, s, d)
+        for qi in range(3):
+            seeds = [dense[int(x)] for x in FX[f"{name}_ppr{qi}_seeds"]]
+            got, _ = ua.ppr(t, seeds, 0.85, 10000, 1e-15)
+            want = dict(zip(FX[f"{name}_ppr{qi}_ids"].tolist(), FX[f"{name}_ppr{qi}"].tolist()))
+            for u, i in dense.items():
+                assert abs(got[i] - want[u]) < 1e-11, (name, qi, u)
+    t = ua.Topology.build(4, [0, 1, 2], [1, 2, 0])
+    o = orc.Topology(4, [0, 1, 2], [1, 2, 0])
+    x, it = ua.ppr(t, [3], 0.85, 1000, 1e-13)
+    r, _ = orc.ppr(o, [3], 0.85, 1000, 1e-13)
+    assert np.abs(x - r).max() < 1e-12 and it > 0
+    x, it = ua.ppr(ua.Topology.build(0, [], []), [0], 0.85, 10, 1e-8)
+    assert x.size == 0 and it == 0
+
+
+def test_ppr_automatic_schedule_bins_large_sparse_graphs(ua):
+    """An Erdos-Renyi graph of 40 blocks of 8192 vertices (no row above 256
+    in-edges, 12 per vertex on average) takes the binned sweep automatically
+    (an edge factor below 8 would stay pull): the automatic result equals the
+    forced binned one bit for bit, stays within the bar of the oracle, and the
+    kernel profile names the binned sweep; an R-MAT graph (hub rows) stays pull."""
+    from ursa_amd import gen
+
+    lib = ua.load()
+    e = gen.erdos_renyi(40 * 8192, 12, 77)
+    t = e.topology()
+    _, _, sd, dd, _ = e.export()
+    seeds = [3, 5000, 200000]
+    ua.profile.enable(True)
+    ua.profile.reset()
+    try:
+        auto, it = ua.ppr(t, seeds, 0.85, 1000, 1e-12)
+        names = ua.profile.names()
+    finally:
+        ua.profile.enable(False)
+    assert "ppr_binned" in names and "ppr_rows" not in names
+    ua.check(lib.ursa_hip_ppr_schedule(2))
+    try:
+        forced, fit = ua.ppr(t, seeds, 0.85, 1000, 1e-12)
+    finally:
+        ua.check(lib.ursa_hip_ppr_schedule(0))
+    assert np.array_equal(auto, forced) and it == fit
+    o = orc.Topology(e.n, sd, dd)
+    ref, _ = orc.ppr(o, seeds, 0.85, 1000, 1e-12)
+    assert np.abs(auto - ref).sum() <= 1e-10 and np.abs(auto - ref).max() <= 1e-11
+    r = gen.rmat(19, 8, seed=9).topology()
+    ua.profile.enable(True)
+    ua.profile.reset()
+    try:
+        ua.ppr(r, [1, 2], 0.85, 30, 1e-8)
+        names = ua.profile.names()
+    finally:
+        ua.profile.enable(False)
+    assert "ppr_rows" in names and "ppr_binned" not in names
+
+
 # ---------------------------------------------------------------- server
 @pytest.fixture(scope="module")
 def pool(ua, sv):

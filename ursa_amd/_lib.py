@@ -82,6 +82,7 @@ SIGNATURES = {
     "ursa_hip_graphs_enable": (cint, [cint]),
     "ursa_hip_triangle_bitmaps": (cint, [cint]),
     "ursa_hip_pagerank_schedule": (cint, [cint]),
+    "ursa_hip_ppr_schedule": (cint, [cint]),
     "ursa_hip_exact_sum": (cint, [P, u64, P]),
     "ursa_hip_set_stream": (cint, [P]),
     "ursa_hip_intern_i64": (cint, [u64, P, P, P, P, P, C.POINTER(u64)]),

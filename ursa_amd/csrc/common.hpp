@@ -175,6 +175,13 @@ class DevBuf {
 
 bool is_device_ptr(const void* p);
 
+// Copies `bytes` from device memory to host memory at `dst` and synchronises
+// stream s. Pageable destinations above a few hundred KB go through the
+// calling thread's two pinned staging buffers: the DMA of one chunk overlaps
+// the host copy of the previous one, and no process-wide staging path is
+// shared with other threads' copies.
+void copy_to_host(void* dst, const void* src, size_t bytes, hipStream_t s);
+
 // A read-only argument: device pointers are used in place, host buffers
 // are uploaded to a temporary device copy.
 template <class T>
@@ -208,8 +215,9 @@ struct DevOut {
     }
     void finish(hipStream_t s) {
         if (user && n && !dev)
-            HIP_OK(hipMemcpyAsync(user, own.get(), n * sizeof(T), hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
+            copy_to_host(user, own.get(), n * sizeof(T), s);
+        else
+            HIP_OK(hipStreamSynchronize(s));
     }
 };
 

@@ -163,9 +163,12 @@ void join_side(hipStream_t main) {
 // they are never destroyed, so their handles are never reused, whereas a
 // caller's stream may be. A call that keeps a block synchronizes its stream
 // before returning (sync_kept_blocks), so calls still return with their
-// results written, as they did when hipFree waited. Blocks above kKeepMax are
-// never kept, the cache holds at most kCacheCap bytes, and a failed hipMalloc
-// waits for the device and releases the whole cache before it retries.
+// results written, as they did when hipFree waited. A request takes the
+// smallest idle block of at least its size and at most twice it (calls on
+// graphs of different sizes reuse each other's blocks). Blocks above kKeepMax
+// are never kept, the cache holds at most kCacheCap bytes, and a failed
+// hipMalloc waits for the device and releases the whole cache before it
+// retries.
 namespace {
 constexpr size_t kKeepMax = 512ull << 20;
 constexpr size_t kCacheCap = 4ull << 30;
@@ -239,7 +242,7 @@ void* dev_alloc(size_t bytes) {
     if (s) {
         std::lock_guard<std::mutex> lk(g_cache_mu);
         auto it = g_kept.lower_bound({s, r});
-        if (it != g_kept.end() && it->first.first == s && it->first.second <= r + r / 8) {
+        if (it != g_kept.end() && it->first.first == s && it->first.second <= 2 * r) {
             void* p = it->second;
             const size_t got = it->first.second;
             g_kept.erase(it);
@@ -322,6 +325,60 @@ void* pinned_scratch(size_t bytes) {
         t_pinned.bytes = want;
     }
     return t_pinned.p;
+}
+
+namespace {
+constexpr size_t kStageChunk = 4ull << 20;
+constexpr size_t kStageDirect = 256ull << 10;
+struct Staging {
+    char* buf = nullptr;  // two chunks
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    // Not freed at thread exit, for the same reason as the streams.
+};
+thread_local Staging t_stage;
+
+bool is_pinned_host(const void* p) {
+    hipPointerAttribute_t attr;
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeHost;
+}
+}  // namespace
+
+void copy_to_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes <= kStageDirect || is_pinned_host(dst)) {
+        if (bytes) HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        return;
+    }
+    ensure_device();
+    Staging& g = t_stage;
+    if (!g.buf) {
+        HIP_OK(hipHostMalloc((void**)&g.buf, 2 * kStageChunk, hipHostMallocDefault));
+        for (auto& e : g.ev) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    const char* from = static_cast<const char*>(src);
+    char* to = static_cast<char*>(dst);
+    const size_t chunks = (bytes + kStageChunk - 1) / kStageChunk;
+    auto issue = [&](size_t c) {
+        const size_t at = c * kStageChunk, len = std::min(kStageChunk, bytes - at);
+        char* st = g.buf + (c & 1) * kStageChunk;
+        HIP_OK(hipMemcpyAsync(st, from + at, len, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipEventRecord(g.ev[c & 1], s));
+    };
+    issue(0);
+    for (size_t c = 0; c < chunks; ++c) {
+        // Buffer (c+1)&1 held chunk c-1, which was copied out below in the
+        // previous round, so the next DMA may overwrite it.
+        if (c + 1 < chunks) issue(c + 1);
+        HIP_OK(hipEventSynchronize(g.ev[c & 1]));
+        const size_t at = c * kStageChunk, len = std::min(kStageChunk, bytes - at);
+        std::memcpy(to + at, g.buf + (c & 1) * kStageChunk, len);
+    }
+    HIP_OK(hipStreamSynchronize(s));
 }
 
 bool is_device_ptr(const void* p) {

@@ -36,14 +36,28 @@
 // and dangling mass of those rows in a fixed tree; the host-side reduction adds
 // the per-workgroup values in workgroup order.
 //
+// Binned schedule: on graphs whose contributions do not stay in cache (from
+// 32 blocks of 8192 vertices up, no row above PPR_HEAVY in-edges, i.e. no
+// hubs whose sources share lines, and a mean in-degree of at least
+// PPR_BN_MIN_DEGREE), each lane's sweep is PageRank's
+// propagation-blocked pair of passes over the same working copy (binned.hpp):
+// phase 1 streams the lane's contributions into terms in bin-major order,
+// phase 2 adds each bin's terms into exact fixed-point row sums in LDS (f32
+// phase: units of 2^-PPR_BN_K; f64 phases: fixed_sum.hpp) and applies the
+// epilogue below. A lane's delta and dangling mass are reduced per bin.
+// The schedule is a property of the graph (and of ursa_hip_ppr_schedule),
+// never of the batch, and each lane runs its own passes.
+//
 // Determinism: a lane's arithmetic does not depend on the other lanes (every
 // split above is fixed by the graph), so a query gives the same bits alone, in
 // any batch and in any lane.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <type_traits>
 #include <vector>
 
+#include "binned.hpp"
 #include "ppr.hpp"
 #include "topology.hpp"
 
@@ -60,6 +74,17 @@ constexpr uint32_t PPR_MAXSEG = 1024;
 constexpr int PPR_BATCH = 4;                 // slots read together per thread (2 at four lanes)
 constexpr double PPR_F32_TOL = 2e-6;         // the f32 phase ends once delta is below this
 constexpr uint32_t PPR_F32_MIN_ITER = 16;    // lanes with max_iter <= this run f64 only
+// Binned f32 phase: terms and row sums of a lane are below 4 (its iterate
+// sums to 1), so rows sum exactly in 64-bit integers of 2^-58 units.
+constexpr int PPR_BN_K = 58;
+// Automatic binned schedule: mean in-degree at least this (below it the
+// per-row work of the binned passes outweighs the gathers they replace;
+// Erdos-Renyi 1.4 M vertices: edge factor 5 pull 84 us vs binned 74 us per
+// sweep but slower per solve, edge factor 15: 266 vs 123 us).
+constexpr uint64_t PPR_BN_MIN_DEGREE = 8;
+
+// Sweep schedule (ursa_hip_ppr_schedule): 0 automatic, 1 pull, 2 binned.
+std::atomic<int> g_ppr_schedule{0};
 
 struct PprLane {
     double d, tol, inv_s, pb, prev, delta;
@@ -336,9 +361,21 @@ __global__ __launch_bounds__(PB) void k_ppr_heavy(uint64_t H, const uint32_t* __
         double a[KP];
 #pragma unroll
         for (int q = 0; q < KP; ++q) {
+            // Segment order; the loads of 16 segments are issued before their adds.
             double acc = 0.0;
-            if (ph[q] != 3)
-                for (uint32_t j = sb; j < se; ++j) acc += spart[(uint64_t)j * KP + q];
+            if (ph[q] != 3) {
+                for (uint32_t j0 = sb; j0 < se; j0 += 16) {
+                    double x[16];
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) {
+                        const uint32_t j = min(j0 + i, se - 1);
+                        x[i] = spart[(uint64_t)j * KP + q];
+                    }
+#pragma unroll
+                    for (int i = 0; i < 16; ++i)
+                        if (j0 + i < se) acc += x[i];
+                }
+            }
             a[q] = acc;
             s[q] = (float)acc;
             c[q] = 0.f;
@@ -354,12 +391,31 @@ __global__ __launch_bounds__(PB) void k_ppr_reduce(const double* __restrict__ pa
                                                    PprLane* __restrict__ lanes, PprCtl* __restrict__ ctl) {
     __shared__ double sh[PB];
     __shared__ double res[KP * 2];
+    // Thread t adds the partials b = t, t + PB, ... in that order for every
+    // value (q, w); the loads of U partials are issued before their adds.
+    constexpr int V = KP * 2, U = V >= 16 ? 1 : 16 / V;
+    double acc[V];
+#pragma unroll
+    for (int j = 0; j < V; ++j) acc[j] = 0.0;
+    for (uint32_t b0 = threadIdx.x; b0 < nparts; b0 += U * PB) {
+        double x[U][V];
+#pragma unroll
+        for (int i = 0; i < U; ++i) {
+            const uint32_t b = min(b0 + i * PB, nparts - 1);
+#pragma unroll
+            for (int j = 0; j < V; ++j) x[i][j] = part[(uint64_t)b * V + j];
+        }
+#pragma unroll
+        for (int i = 0; i < U; ++i)
+            if (b0 + i * PB < nparts) {
+#pragma unroll
+                for (int j = 0; j < V; ++j) acc[j] += x[i][j];
+            }
+    }
     for (int q = 0; q < KP; ++q) {
         if (lanes[q].phase == 3) continue;
         for (int w = 0; w < 2; ++w) {
-            double acc = 0.0;
-            for (uint32_t b = threadIdx.x; b < nparts; b += PB) acc += part[(uint64_t)b * KP * 2 + q * 2 + w];
-            sh[threadIdx.x] = acc;
+            sh[threadIdx.x] = acc[q * 2 + w];
             __syncthreads();
             for (int st = PB / 2; st > 0; st >>= 1) {
                 if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
@@ -395,6 +451,118 @@ __global__ __launch_bounds__(PB) void k_ppr_reduce(const double* __restrict__ pa
         if (L.phase != 3) ++active;
     }
     ctl->active = active;
+}
+
+// Binned sweep of lane q, phase 1: the lane's contributions (c32 in the f32
+// phase and the first f64 sweep, c64 after) become terms in bin-major order.
+// Also clears phase 2's claim counter ctr[1].
+template <int KP>
+__global__ __launch_bounds__(BN_P1_BLOCK) void k_ppr_bn_scatter(int q, const PprLane* __restrict__ lanes,
+                                                                const uint16_t* __restrict__ bsrc,
+                                                                const uint32_t* __restrict__ place,
+                                                                const uint32_t* __restrict__ items, uint32_t nitems,
+                                                                uint64_t n, const float* __restrict__ c32,
+                                                                const double* __restrict__ c64, void* term,
+                                                                uint32_t* __restrict__ ctr) {
+    __shared__ double s_c[BN_SSPAN];
+    __shared__ uint32_t s_it;
+    const int ph = lanes[q].phase;
+    if (ph == 3) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[1] = 0;
+    if (ph == 2)
+        bn_scatter_items<double, double, float, false>(bsrc, place, nullptr, items, nitems, n, c64 + q, (uint32_t)KP,
+                                                       static_cast<double*>(term), ctr, s_c, &s_it);
+    else
+        bn_scatter_items<float, float, float, false>(bsrc, place, nullptr, items, nitems, n, c32 + q, (uint32_t)KP,
+                                                     static_cast<float*>(term), ctr,
+                                                     reinterpret_cast<float*>(s_c), &s_it);
+}
+
+// Fixed-order sum over a BN_P2_BLOCK workgroup; valid in thread 0.
+__device__ __forceinline__ double ppr_bn_block_sum(double v, double* lds) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    double tot = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < BN_P2_BLOCK / 64; ++w) tot += lds[w];
+    __syncthreads();
+    return tot;
+}
+
+// Binned sweep of lane q, phase 2: one bin per work item (claimed through
+// ctr[1]; also clears phase 1's counter ctr[0]); exact row sums, the lane's
+// epilogue (as ppr_epilogue) and the bin's (delta, dangling) partial at
+// part[(bin * KP + q) * 2].
+template <int KP>
+__global__ __launch_bounds__(BN_P2_BLOCK, 4) void k_ppr_bn_gather(
+    int q, const PprLane* __restrict__ lanes, const uint16_t* __restrict__ bdst, const void* term,
+    const uint32_t* __restrict__ items, uint32_t nitems, const uint32_t* __restrict__ od,
+    const uint32_t* __restrict__ seedbits, float* __restrict__ r32, float* __restrict__ c32n,
+    double* __restrict__ r64, double* __restrict__ c64n, double* __restrict__ part, uint32_t* __restrict__ ctr) {
+    __shared__ unsigned long long s_acc[2 * BN_RSPAN];
+    __shared__ double s_red[BN_P2_BLOCK / 64];
+    __shared__ uint32_t s_it;
+    const int ph = lanes[q].phase;
+    if (ph == 3) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[0] = 0;
+    const double d = lanes[q].d, tele = lanes[q].pb * lanes[q].inv_s;
+    const float scale = __builtin_ldexpf(1.0f, PPR_BN_K - 32);
+    const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t parts = ph == 0 ? 1u : 2u;
+    for (;;) {
+        if (threadIdx.x == 0) s_it = atomicAdd(&ctr[1], 1u);
+        for (uint32_t i = threadIdx.x; i < parts * BN_RSPAN; i += BN_P2_BLOCK) s_acc[i] = 0ull;
+        __syncthreads();
+        const uint32_t it = s_it;
+        if (it >= nitems) break;
+        const uint32_t* I = items + 8 * (uint64_t)it;
+        const uint32_t b = I[0], rows = I[4];
+        const uint64_t k0 = (uint64_t)I[1] * BN_CH, k1 = (uint64_t)I[2] * BN_CH, r0 = I[3];
+        if (ph == 0)
+            bn_gather_windows<float, float>(bdst, static_cast<const float*>(term), k0, k1, wave, lane, scale, s_acc);
+        else if (ph == 1)
+            bn_gather_windows<float, double>(bdst, static_cast<const float*>(term), k0, k1, wave, lane, scale, s_acc);
+        else
+            bn_gather_windows<double, double>(bdst, static_cast<const double*>(term), k0, k1, wave, lane, scale,
+                                              s_acc);
+        __syncthreads();
+        double dl = 0.0, dm = 0.0;
+        for (uint32_t i = threadIdx.x; i < rows; i += BN_P2_BLOCK) {
+            const uint64_t u = r0 + i, x = u * KP + q;
+            const bool seed = (seedbits[u] >> q) & 1u;
+            const uint32_t odu = od[u];
+            if (ph == 0) {
+                const float acc = (float)__builtin_ldexp((double)(long long)s_acc[i], -PPR_BN_K);
+                const float nx = fmaf((float)d, acc, seed ? (float)tele : 0.f);
+                const float cur = r32[x];
+                r32[x] = nx;
+                dl += fabs((double)nx - (double)cur);
+                if (odu)
+                    c32n[x] = nx / (float)odu;
+                else
+                    dm += (double)nx;
+            } else {
+                const double acc = fx_value(s_acc[i], (long long)s_acc[BN_RSPAN + i]);
+                const double nx = fma(d, acc, seed ? tele : 0.0);
+                const double cur = ph == 1 ? (double)r32[x] : r64[x];
+                r64[x] = nx;
+                dl += fabs(nx - cur);
+                if (odu)
+                    c64n[x] = nx / (double)odu;
+                else
+                    dm += nx;
+            }
+        }
+        const double bd = ppr_bn_block_sum(dl, s_red);
+        const double bm = ppr_bn_block_sum(dm, s_red);
+        if (threadIdx.x == 0) {
+            part[((uint64_t)b * KP + q) * 2] = bd;
+            part[((uint64_t)b * KP + q) * 2 + 1] = bm;
+        }
+        __syncthreads();
+    }
 }
 
 // x0 = 1/n for every lane, with its f32 and f64 contributions (lanes in the
@@ -438,10 +606,13 @@ int lanes_padded(int k) {
 
 struct PprBufs {
     DevBuf<float> r32, c32a, c32b;
-    DevBuf<double> r64, c64a, c64b, part, spart;
-    DevBuf<uint32_t> seedbits, seeds, lane_of;
+    DevBuf<double> r64, c64a, c64b, part, spart, term;
+    DevBuf<uint32_t> seedbits, seeds, lane_of, bctr;
     DevBuf<PprLane> lanes;
     DevBuf<PprCtl> ctl;
+    bool binned = false;
+    int k = 0;                 // lanes that hold a query
+    unsigned g1 = 1, g2 = 1;   // binned grid sizes
 };
 
 template <int KP>
@@ -450,7 +621,7 @@ void ppr_iterate(ursa_hip_topo* t, PprBufs& B, uint32_t max_iter_all, hipStream_
     const uint32_t light_blocks = (uint32_t)((n + PPR_ROWS - 1) / PPR_ROWS);
     const uint32_t seg_blocks = (uint32_t)((t->ppr_S + PPR_GROUPS - 1) / PPR_GROUPS);
     const uint32_t heavy_blocks = (uint32_t)((t->ppr_H + PB - 1) / PB);
-    const uint32_t nparts = light_blocks + heavy_blocks;
+    const uint32_t nparts = B.binned ? (uint32_t)t->bn_bins : light_blocks + heavy_blocks;
     const double edge_bytes = 4.0 + 4.0 * KP;
     uint32_t it = 0;
     CapturedLaunches graph;  // a full batch always starts at an even sweep
@@ -464,6 +635,26 @@ void ppr_iterate(ursa_hip_topo* t, PprBufs& B, uint32_t max_iter_all, hipStream_
                 float* c32n = p ? B.c32a.get() : B.c32b.get();
                 const double* c64 = p ? B.c64b.get() : B.c64a.get();
                 double* c64n = p ? B.c64a.get() : B.c64b.get();
+                if (B.binned) {
+                    // Per lane: 2 + 8 (f64 term) + 8 + 2 bytes per slot at most.
+                    timed("ppr_binned", (20.0 * (double)t->bn_slots + 28.0 * (double)n) * B.k, s, [&] {
+                        for (int q = 0; q < B.k; ++q) {
+                            k_ppr_bn_scatter<KP><<<B.g1, BN_P1_BLOCK, 0, s>>>(
+                                q, B.lanes.get(), t->bn_src.get(), t->bn_place.get(), t->bn_item1.get(),
+                                (uint32_t)t->bn_n1, n, c32, c64, B.term.get(), B.bctr.get());
+                            k_ppr_bn_gather<KP><<<B.g2, BN_P2_BLOCK, 0, s>>>(
+                                q, B.lanes.get(), t->bn_dst.get(), B.term.get(), t->bn_item2.get(),
+                                (uint32_t)t->bn_n2, t->pr_outdeg.get(), B.seedbits.get(), B.r32.get(), c32n,
+                                B.r64.get(), c64n, B.part.get(), B.bctr.get());
+                        }
+                    });
+                    check_launch();
+                    timed("ppr_reduce", 16.0 * KP * (double)nparts, s, [&] {
+                        k_ppr_reduce<KP><<<1, PB, 0, s>>>(B.part.get(), nparts, B.lanes.get(), B.ctl.get());
+                    });
+                    check_launch();
+                    continue;
+                }
                 timed("ppr_rows", edge_bytes * (double)t->m + (16.0 + 8.0 * KP) * (double)n, s, [&] {
                     k_ppr_rows<KP><<<light_blocks + seg_blocks, PB, 0, s>>>(
                         n, t->pr_off.get(), t->pr_tgt.get(), t->pr_outdeg.get(), B.seedbits.get(), B.lanes.get(), c32,
@@ -472,13 +663,17 @@ void ppr_iterate(ursa_hip_topo* t, PprBufs& B, uint32_t max_iter_all, hipStream_
                 });
                 check_launch();
                 if (t->ppr_H) {
-                    k_ppr_heavy<KP><<<heavy_blocks, PB, 0, s>>>(t->ppr_H, t->ppr_hrow.get(), t->ppr_hsoff.get(),
-                                                                B.spart.get(), t->pr_outdeg.get(), B.seedbits.get(),
-                                                                B.lanes.get(), c32n, B.r32.get(), c64n, B.r64.get(),
-                                                                B.part.get() + (uint64_t)light_blocks * KP * 2);
+                    timed("ppr_heavy", 8.0 * KP * (double)t->ppr_S, s, [&] {
+                        k_ppr_heavy<KP><<<heavy_blocks, PB, 0, s>>>(
+                            t->ppr_H, t->ppr_hrow.get(), t->ppr_hsoff.get(), B.spart.get(), t->pr_outdeg.get(),
+                            B.seedbits.get(), B.lanes.get(), c32n, B.r32.get(), c64n, B.r64.get(),
+                            B.part.get() + (uint64_t)light_blocks * KP * 2);
+                    });
                     check_launch();
                 }
-                k_ppr_reduce<KP><<<1, PB, 0, s>>>(B.part.get(), nparts, B.lanes.get(), B.ctl.get());
+                timed("ppr_reduce", 16.0 * KP * (double)nparts, s, [&] {
+                    k_ppr_reduce<KP><<<1, PB, 0, s>>>(B.part.get(), nparts, B.lanes.get(), B.ctl.get());
+                });
                 check_launch();
             }
         };
@@ -504,9 +699,7 @@ void ppr_write_out(const ursa_hip_topo* t, uint64_t n, PprBufs& B, int k, double
 
 }  // namespace
 
-void ppr_prepare(ursa_hip_topo* t, hipStream_t s) {
-    NoArena keep_out_of_arena;
-    pagerank_prepare(t, s);
+static void ppr_prepare_rows(ursa_hip_topo* t, hipStream_t s) {
     std::lock_guard<std::mutex> lk(t->cache_mu);
     if (t->ppr_ready) return;
     const uint64_t n = t->n;
@@ -544,12 +737,49 @@ void ppr_prepare(ursa_hip_topo* t, hipStream_t s) {
     t->ppr_ready = true;
 }
 
+void ppr_prepare(ursa_hip_topo* t, hipStream_t s) {
+    NoArena keep_out_of_arena;
+    pagerank_prepare(t, s);
+    ppr_prepare_rows(t, s);
+    // Automatic schedule: binned from BN_MIN_BLOCKS source blocks up on graphs
+    // without heavy rows and with at least PPR_BN_MIN_DEGREE in-edges per row
+    // on average (decided once; the layout is built here, outside any query's
+    // workspace).
+    bool bn = false;
+    {
+        std::lock_guard<std::mutex> lk(t->cache_mu);
+        bn = !t->ppr_bn_auto && t->ppr_H == 0 && (t->n + BN_SSPAN - 1) / BN_SSPAN >= BN_MIN_BLOCKS &&
+             t->m >= PPR_BN_MIN_DEGREE * t->n;
+    }
+    if (bn && bn_fits(t, s)) {
+        bn_prepare(t, s);
+        std::lock_guard<std::mutex> lk(t->cache_mu);
+        t->ppr_bn_auto = true;
+    }
+}
+
+// Whether solves on t sweep binned under the current schedule.
+static bool ppr_binned(ursa_hip_topo* t, hipStream_t s) {
+    const int sched = g_ppr_schedule.load();
+    if (sched == 1) return false;
+    if (sched == 2) return bn_fits(t, s);
+    return t->ppr_bn_auto;
+}
+
 uint64_t ppr_workspace_bytes(const ursa_hip_topo* t, int k, bool host_out) {
     const uint64_t n = t->n, kp = (uint64_t)lanes_padded(std::max(k, 1));
     const uint64_t light_blocks = (n + PPR_ROWS - 1) / PPR_ROWS, heavy_blocks = (t->ppr_H + PB - 1) / PB;
     uint64_t b = n * kp * (4 + 8 + 8 + 16 + 16)  // r32, c32 x2, r64, c64 x2
                  + n * 4                         // seedbits
                  + (light_blocks + heavy_blocks + 1) * kp * 16 + (t->ppr_S + 1) * kp * 8 + 16 * 64 * 8;
+    const int sched = g_ppr_schedule.load();
+    if (sched == 2 || (sched == 0 && t->ppr_bn_auto)) {
+        // Term array (the layout's slots; a nonempty tile pads to at most 8x
+        // its slots when the layout is not built yet), claim counters, per-bin
+        // partials (at most n / BN_RSPAN + m / BN_T2 + 1 bins).
+        const uint64_t slots = t->bn_ready ? t->bn_slots : 8 * t->m;
+        b += (std::max<uint64_t>(slots, BN_CH) + 1) * 8 + 64 + (n / BN_RSPAN + t->m / BN_T2 + 2) * kp * 16;
+    }
     if (host_out) b += (uint64_t)k * n * 8;
     return b + (b >> 4) + (64u << 10) * 16;
 }
@@ -597,6 +827,16 @@ void ppr_solve(ursa_hip_topo* t, const PprQuery* qs, int k, double* const* outs,
     }
 
     PprBufs B;
+    B.k = k;
+    B.binned = ppr_binned(t, s);
+    if (B.binned) {
+        bn_prepare(t, s);
+        B.term.alloc(std::max<uint64_t>(t->bn_slots, BN_CH));
+        B.bctr.alloc(2);
+        HIP_OK(hipMemsetAsync(B.bctr.get(), 0, 8, s));
+        B.g1 = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(t->bn_n1, 4ull * cu_count()));
+        B.g2 = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(t->bn_n2, 2ull * cu_count()));
+    }
     B.r32.alloc(n * KP);
     B.c32a.alloc(n * KP);
     B.c32b.alloc(n * KP);
@@ -604,7 +844,7 @@ void ppr_solve(ursa_hip_topo* t, const PprQuery* qs, int k, double* const* outs,
     B.c64a.alloc(n * KP);
     B.c64b.alloc(n * KP);
     const uint64_t light_blocks = (n + PPR_ROWS - 1) / PPR_ROWS, heavy_blocks = (t->ppr_H + PB - 1) / PB;
-    B.part.alloc((light_blocks + heavy_blocks) * KP * 2);
+    B.part.alloc(std::max<uint64_t>(light_blocks + heavy_blocks, B.binned ? t->bn_bins : 0) * KP * 2);
     B.spart.alloc(std::max<uint64_t>(t->ppr_S, 1) * KP);
     B.seedbits.alloc(n);
     B.lanes.alloc(KP);
@@ -657,6 +897,13 @@ void ppr_solve(ursa_hip_topo* t, const PprQuery* qs, int k, double* const* outs,
 using namespace ursa;
 
 extern "C" {
+
+int ursa_hip_ppr_schedule(int mode) {
+    return guard([&] {
+        URSA_REQUIRE(mode >= 0 && mode <= 2, URSA_HIP_EINVAL, "schedule must be 0 (automatic), 1 (pull) or 2 (binned)");
+        g_ppr_schedule.store(mode);
+    });
+}
 
 int ursa_hip_ppr(ursa_hip_topo* t, const uint32_t* seeds, uint64_t nseeds, double damping, uint32_t max_iter,
                  double tol, double* out, uint32_t* iters_out) {

@@ -216,11 +216,10 @@ void ursa_hip_server::run_served(const std::vector<uint64_t>& b, uint32_t slot, 
                 for (uint64_t j = 0; j < k; ++j) {
                     Rec& r = recs[b[j]];
                     r.data.resize(n * 8);
-                    HIP_OK(hipMemcpyAsync(r.data.data(), outs[j], n * 8, hipMemcpyDeviceToHost, s));
+                    copy_to_host(r.data.data(), outs[j], n * 8, s);
                     r.r.iters = iters[j];
                     r.r.len = n;
                 }
-                HIP_OK(hipStreamSynchronize(s));
             }
         } else if (h.kind == URSA_HIP_Q_KHOP) {
             std::vector<uint32_t> seeds;
@@ -265,8 +264,7 @@ void ursa_hip_server::run_served(const std::vector<uint64_t>& b, uint32_t slot, 
                 triangle_count(t, o.get());
                 Rec& r0 = recs[b[0]];
                 r0.data.resize(n * 8);
-                HIP_OK(hipMemcpyAsync(r0.data.data(), o.get(), n * 8, hipMemcpyDeviceToHost, s));
-                HIP_OK(hipStreamSynchronize(s));
+                copy_to_host(r0.data.data(), o.get(), n * 8, s);
                 for (uint64_t j = 1; j < k; ++j) recs[b[j]].data = r0.data;
             }
             for (uint64_t i : b) recs[i].r.len = n;

@@ -100,6 +100,7 @@ struct ursa_hip_topo {
     ursa::DevBuf<uint32_t> ppr_hrow, ppr_hsoff, ppr_sbeg, ppr_send;
     uint64_t ppr_H = 0, ppr_S = 0, ppr_dangling = 0;
     bool ppr_ready = false;
+    bool ppr_bn_auto = false;  // the automatic schedule sweeps this graph binned
 
     ursa::TuneParams tune[3];  // per traversal direction
     uint64_t tune_key[3] = {0, 0, 0};
