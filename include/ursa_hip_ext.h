@@ -121,6 +121,17 @@ int ursa_hip_ppr(ursa_hip_topo* t, const uint32_t* seeds, uint64_t nseeds, doubl
 int ursa_hip_ppr_batch(ursa_hip_topo* t, uint32_t k, const uint64_t* seed_offsets,
                        const uint32_t* seeds, double damping, uint32_t max_iter, double tol,
                        double* out, uint32_t* iters_out);
+/* One query on each of `count` graphs, solved in cross-graph passes: query i
+ * runs on topos[i] with seeds seeds[seed_offsets[i] .. seed_offsets[i+1]) and
+ * writes f64[n_i] to outs[i] (host or device memory) and its iteration count
+ * to iters_out[i]. Graphs of at most 2^19 vertices on the pull schedule are
+ * swept together (up to 256 queries and 8 Mi vertices per pass, one launch per
+ * sweep phase for all of them); other graphs are solved one at a time. Every
+ * result is bit-identical to ursa_hip_ppr on the same arguments. No
+ * counterpart in ursa-core: the request server's cross-graph batching. */
+int ursa_hip_ppr_multi(uint32_t count, ursa_hip_topo* const* topos, const uint64_t* seed_offsets,
+                       const uint32_t* seeds, double damping, uint32_t max_iter, double tol,
+                       double* const* outs, uint32_t* iters_out);
 /* Personalized PageRank sweep schedule (default 0): 0 automatic (binned on
  * graphs of 32 or more blocks of 8192 vertices with no row above 256
  * in-edges and at least 8 edges per vertex, else pull), 1 pull, 2 binned (each lane's sweep streamed through

@@ -133,6 +133,88 @@ def test_ppr_batch_lanes_are_bit_identical_to_alone(ua):
     assert it == 0 and np.all(x == 1.0 / n)
 
 
+def multi_graphs(ua):
+    """Graphs for the cross-graph passes: every shared case (hub rows cut into
+    segments, isolated nodes, multigraph), R-MAT graphs of several sizes and a
+    graph used twice."""
+    out = [ua.Topology.build(n, s, d) for (_, n, s, d) in GRAPHS]
+    for scale, seed, ef in [(9, 1, 4), (12, 3, 16), (14, 7, 8), (11, 2, 24)]:
+        n, s, d = rmat_dense(scale, seed, edge_factor=ef)
+        out.append(ua.Topology.build(n, s, d))
+    out.append(out[-3])
+    return out
+
+
+@pytest.mark.parametrize("max_iter,tol", [(100, 1e-8), (1, 1e-8), (2, 1e-8), (3, 1e-8), (17, 1e-8),
+                                          (60, 1e-4), (1000, 1e-12)])
+def test_ppr_multi_is_bit_identical_to_alone(ua, max_iter, tol):
+    """One query per graph swept in cross-graph passes gives, for every query,
+    the bits and iteration count of the same query alone (f32 and f64 phases,
+    stops inside the f32 phase, coarse tolerances, no seed in range)."""
+    topos = multi_graphs(ua)
+    rng = np.random.default_rng(max_iter)
+    lists = []
+    for i, t in enumerate(topos):
+        n = t.n_nodes()
+        lists.append(rng.integers(0, n, int(rng.integers(1, 17))).tolist())
+    lists[1] = [topos[1].n_nodes() + 5]  # no seed in range
+    outs, iters = ua.ppr_multi(topos, lists, 0.85, max_iter, tol)
+    for i, (t, sl) in enumerate(zip(topos, lists)):
+        ref, rit = ua.ppr(t, sl, 0.85, max_iter, tol)
+        assert np.array_equal(outs[i], ref) and iters[i] == rit, (i, max_iter, tol)
+    assert iters[1] == 0 and np.all(outs[1] == 1.0 / topos[1].n_nodes())
+
+
+def test_ppr_multi_passes_split_and_fall_back(ua):
+    """More queries than one pass holds (several passes), device outputs, and
+    graphs that sweep binned (forced schedule) solved one at a time: every
+    result still equals the query alone."""
+    import ctypes as C
+
+    lib = ua.load()
+    small = [ua.Topology.build(*rmat_dense(6 + (i % 5), 100 + i, edge_factor=4 + i % 9)) for i in range(40)]
+    topos = small * 7  # 280 queries: more than the 256 of one pass
+    rng = np.random.default_rng(9)
+    lists = [rng.integers(0, t.n_nodes(), int(rng.integers(1, 5))).tolist() for t in topos]
+    outs, iters = ua.ppr_multi(topos, lists, 0.85, 100, 1e-8)
+    for i in range(0, len(topos), 3):
+        ref, rit = ua.ppr(topos[i], lists[i], 0.85, 100, 1e-8)
+        assert np.array_equal(outs[i], ref) and iters[i] == rit, i
+    # Device outputs.
+    k = 6
+    dev = []
+    for t in topos[:k]:
+        p = C.c_void_p()
+        ua.check(lib.ursa_hip_device_alloc(8 * t.n_nodes(), C.byref(p)))
+        dev.append(p)
+    try:
+        offs = np.zeros(k + 1, np.uint64)
+        for i in range(k):
+            offs[i + 1] = offs[i] + len(lists[i])
+        flat = np.concatenate([np.asarray(x, np.uint32) for x in lists[:k]])
+        handles = (C.c_void_p * k)(*[t._h for t in topos[:k]])
+        optrs = (C.c_void_p * k)(*[p.value for p in dev])
+        it = np.zeros(k, np.uint32)
+        ua.check(lib.ursa_hip_ppr_multi(k, handles, offs.ctypes.data_as(C.c_void_p), flat.ctypes.data_as(C.c_void_p),
+                                        0.85, 100, 1e-8, optrs, it.ctypes.data_as(C.c_void_p)))
+        for i in range(k):
+            got = np.zeros(topos[i].n_nodes(), np.float64)
+            ua.check(lib.ursa_hip_memcpy_to_host(got.ctypes.data_as(C.c_void_p), dev[i], got.nbytes))
+            assert np.array_equal(got, outs[i]) and it[i] == iters[i], i
+    finally:
+        for p in dev:
+            lib.ursa_hip_device_free(p)
+    # Forced binned schedule: no graph qualifies, each is solved alone.
+    ua.check(lib.ursa_hip_ppr_schedule(2))
+    try:
+        outs_b, iters_b = ua.ppr_multi(topos[:8], lists[:8], 0.85, 100, 1e-8)
+        for i in range(8):
+            ref, rit = ua.ppr(topos[i], lists[i], 0.85, 100, 1e-8)
+            assert np.array_equal(outs_b[i], ref) and iters_b[i] == rit, i
+    finally:
+        ua.check(lib.ursa_hip_ppr_schedule(0))
+
+
 def test_ppr_edge_cases(ua):
     t = ua.Topology.build(4, [0, 1, 2], [1, 2, 0])  # node 3 isolated and dangling
     o = orc.Topology(4, [0, 1, 2], [1, 2, 0])

@@ -43,6 +43,7 @@ __all__ = [
     "PinnedBuffer",
     "ppr",
     "ppr_batch",
+    "ppr_multi",
     "bfs_distances",
     "bfs_multi",
     "autotune",
@@ -338,6 +339,27 @@ def ppr_batch(topo: Topology, seed_lists, damping=0.85, max_iter=100, tol=1e-8):
     check(load().ursa_hip_ppr_batch(topo._h, k, _ptr(offs), _ptr(flat) if flat.size else None, damping, max_iter,
                                     tol, _ptr(out), _ptr(iters)))
     return out[:, :n], iters[:k]
+
+
+def ppr_multi(topos, seed_lists, damping=0.85, max_iter=100, tol=1e-8):
+    """One personalized PageRank query per graph, small graphs swept together in
+    cross-graph passes -> ([f64[n_i]], iterations u32[k]); entry i equals
+    ppr(topos[i], seed_lists[i])."""
+    k = len(topos)
+    if len(seed_lists) != k:
+        raise ValueError("one seed list per topology")
+    offs = np.zeros(k + 1, np.uint64)
+    for i, sl in enumerate(seed_lists):
+        offs[i + 1] = offs[i] + len(sl)
+    flat = np.ascontiguousarray(np.concatenate([np.asarray(x, np.uint32) for x in seed_lists])
+                                if k else np.zeros(0, np.uint32))
+    outs = [np.zeros(max(t.n_nodes(), 1), np.float64) for t in topos]
+    handles = (C.c_void_p * max(k, 1))(*[t._h for t in topos])
+    optrs = (C.c_void_p * max(k, 1))(*[o.ctypes.data for o in outs])
+    iters = np.zeros(max(k, 1), np.uint32)
+    check(load().ursa_hip_ppr_multi(k, handles, _ptr(offs), _ptr(flat) if flat.size else None, damping, max_iter,
+                                    tol, optrs, _ptr(iters)))
+    return [o[:t.n_nodes()] for o, t in zip(outs, topos)], iters[:k]
 
 
 def pagerank(topo: Topology, params: PageRankParams = PageRankParams()) -> np.ndarray:

@@ -99,6 +99,7 @@ SIGNATURES = {
     "ursa_hip_host_alloc": (cint, [u64, C.POINTER(P)]),
     "ursa_hip_host_free": (cint, [P]),
     "ursa_hip_ppr_batch": (cint, [P, u32, P, P, f64, u32, f64, P, P]),
+    "ursa_hip_ppr_multi": (cint, [u32, P, P, P, f64, u32, f64, P, P]),
     "ursa_hip_server_create": (cint, [P, C.POINTER(P)]),
     "ursa_hip_server_add_graph": (cint, [P, P, C.POINTER(u32)]),
     "ursa_hip_server_graph_bytes": (cint, [P, C.POINTER(u64)]),

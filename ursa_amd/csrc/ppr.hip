@@ -54,6 +54,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstring>
 #include <type_traits>
 #include <vector>
 
@@ -288,9 +289,10 @@ __device__ __forceinline__ void ppr_block_reduce(double* dl, double* dm, double*
 }
 
 // One sweep over the light rows (blocks [0, light_blocks), PPR_ROWS rows
-// each) and the heavy segments (PPR_GROUPS per block after them).
+// each) and the heavy segments (PPR_GROUPS per block after them); bid is the
+// block's index in that range (a whole workgroup of PB threads).
 template <int KP>
-__global__ __launch_bounds__(PB) void k_ppr_rows(uint64_t n, const uint32_t* __restrict__ off,
+__device__ __forceinline__ void ppr_rows_block(uint32_t bid, uint64_t n, const uint32_t* __restrict__ off,
                                                  const uint32_t* __restrict__ tgt, const uint32_t* __restrict__ od,
                                                  const uint32_t* __restrict__ seedbits,
                                                  const PprLane* __restrict__ lanes, const float* __restrict__ c32,
@@ -309,8 +311,8 @@ __global__ __launch_bounds__(PB) void k_ppr_rows(uint64_t n, const uint32_t* __r
     float s[KP], c[KP];
     double a[KP];
     const uint32_t g = threadIdx.x % PPR_G, grp = threadIdx.x / PPR_G;
-    if (blockIdx.x >= light_blocks) {
-        const uint64_t sg = (uint64_t)(blockIdx.x - light_blocks) * PPR_GROUPS + grp;
+    if (bid >= light_blocks) {
+        const uint64_t sg = (uint64_t)(bid - light_blocks) * PPR_GROUPS + grp;
         if (sg >= nseg) return;  // whole groups leave together
         ppr_row_sum<KP>(sbeg[sg], send[sg], g, tgt, ph, c32, c64, s, c, a);
         if (g == 0) {
@@ -323,7 +325,7 @@ __global__ __launch_bounds__(PB) void k_ppr_rows(uint64_t n, const uint32_t* __r
     double dl[KP], dm[KP];
 #pragma unroll
     for (int q = 0; q < KP; ++q) dl[q] = dm[q] = 0.0;
-    const uint64_t row0 = (uint64_t)blockIdx.x * PPR_ROWS;
+    const uint64_t row0 = (uint64_t)bid * PPR_ROWS;
     for (uint32_t r = grp; r < PPR_ROWS; r += PPR_GROUPS) {
         const uint64_t u = row0 + r;
         if (u >= n) break;
@@ -332,12 +334,28 @@ __global__ __launch_bounds__(PB) void k_ppr_rows(uint64_t n, const uint32_t* __r
         ppr_row_sum<KP>(b, e, g, tgt, ph, c32, c64, s, c, a);
         if (g == 0) ppr_epilogue<KP>(u, seedbits[u], od[u], lanes, ph, s, c, a, r32, c32n, r64, c64n, dl, dm);
     }
-    ppr_block_reduce<KP>(dl, dm, part + (uint64_t)blockIdx.x * KP * 2);
+    ppr_block_reduce<KP>(dl, dm, part + (uint64_t)bid * KP * 2);
 }
 
-// Heavy rows: add the row's segment partials in order, then the epilogue.
 template <int KP>
-__global__ __launch_bounds__(PB) void k_ppr_heavy(uint64_t H, const uint32_t* __restrict__ hrow,
+__global__ __launch_bounds__(PB) void k_ppr_rows(uint64_t n, const uint32_t* __restrict__ off,
+                                                 const uint32_t* __restrict__ tgt, const uint32_t* __restrict__ od,
+                                                 const uint32_t* __restrict__ seedbits,
+                                                 const PprLane* __restrict__ lanes, const float* __restrict__ c32,
+                                                 float* __restrict__ c32n, float* __restrict__ r32,
+                                                 const double* __restrict__ c64, double* __restrict__ c64n,
+                                                 double* __restrict__ r64, double* __restrict__ part,
+                                                 uint32_t light_blocks, uint64_t nseg,
+                                                 const uint32_t* __restrict__ sbeg,
+                                                 const uint32_t* __restrict__ send, double* __restrict__ spart) {
+    ppr_rows_block<KP>(blockIdx.x, n, off, tgt, od, seedbits, lanes, c32, c32n, r32, c64, c64n, r64, part,
+                       light_blocks, nseg, sbeg, send, spart);
+}
+
+// Heavy rows: add the row's segment partials in order, then the epilogue
+// (block bid of the heavy-row range).
+template <int KP>
+__device__ __forceinline__ void ppr_heavy_block(uint32_t bid, uint64_t H, const uint32_t* __restrict__ hrow,
                                                   const uint32_t* __restrict__ hsoff,
                                                   const double* __restrict__ spart, const uint32_t* __restrict__ od,
                                                   const uint32_t* __restrict__ seedbits,
@@ -353,7 +371,7 @@ __global__ __launch_bounds__(PB) void k_ppr_heavy(uint64_t H, const uint32_t* __
     double dl[KP], dm[KP];
 #pragma unroll
     for (int q = 0; q < KP; ++q) dl[q] = dm[q] = 0.0;
-    const uint64_t h = (uint64_t)blockIdx.x * PB + threadIdx.x;
+    const uint64_t h = (uint64_t)bid * PB + threadIdx.x;
     if (h < H) {
         const uint64_t u = hrow[h];
         const uint32_t sb = hsoff[h], se = hsoff[h + 1];
@@ -382,13 +400,25 @@ __global__ __launch_bounds__(PB) void k_ppr_heavy(uint64_t H, const uint32_t* __
         }
         ppr_epilogue<KP>(u, seedbits[u], od[u], lanes, ph, s, c, a, r32, c32n, r64, c64n, dl, dm);
     }
-    ppr_block_reduce<KP>(dl, dm, part + (uint64_t)blockIdx.x * KP * 2);
+    ppr_block_reduce<KP>(dl, dm, part + (uint64_t)bid * KP * 2);
 }
 
-// Sums the block partials per lane in a fixed order and advances each lane.
 template <int KP>
-__global__ __launch_bounds__(PB) void k_ppr_reduce(const double* __restrict__ part, uint32_t nparts,
-                                                   PprLane* __restrict__ lanes, PprCtl* __restrict__ ctl) {
+__global__ __launch_bounds__(PB) void k_ppr_heavy(uint64_t H, const uint32_t* __restrict__ hrow,
+                                                  const uint32_t* __restrict__ hsoff,
+                                                  const double* __restrict__ spart, const uint32_t* __restrict__ od,
+                                                  const uint32_t* __restrict__ seedbits,
+                                                  const PprLane* __restrict__ lanes, float* __restrict__ c32n,
+                                                  float* __restrict__ r32, double* __restrict__ c64n,
+                                                  double* __restrict__ r64, double* __restrict__ part) {
+    ppr_heavy_block<KP>(blockIdx.x, H, hrow, hsoff, spart, od, seedbits, lanes, c32n, r32, c64n, r64, part);
+}
+
+// Sums the block partials per lane in a fixed order and advances each lane
+// (one workgroup of PB threads).
+template <int KP>
+__device__ __forceinline__ void ppr_reduce_block(const double* __restrict__ part, uint32_t nparts,
+                                                 PprLane* __restrict__ lanes, PprCtl* __restrict__ ctl) {
     __shared__ double sh[PB];
     __shared__ double res[KP * 2];
     // Thread t adds the partials b = t, t + PB, ... in that order for every
@@ -451,6 +481,12 @@ __global__ __launch_bounds__(PB) void k_ppr_reduce(const double* __restrict__ pa
         if (L.phase != 3) ++active;
     }
     ctl->active = active;
+}
+
+template <int KP>
+__global__ __launch_bounds__(PB) void k_ppr_reduce(const double* __restrict__ part, uint32_t nparts,
+                                                   PprLane* __restrict__ lanes, PprCtl* __restrict__ ctl) {
+    ppr_reduce_block<KP>(part, nparts, lanes, ctl);
 }
 
 // Binned sweep of lane q, phase 1: the lane's contributions (c32 in the f32
@@ -568,20 +604,26 @@ __global__ __launch_bounds__(BN_P2_BLOCK, 4) void k_ppr_bn_gather(
 // x0 = 1/n for every lane, with its f32 and f64 contributions (lanes in the
 // f32 phase read c32, lanes that start in f64 read c64).
 template <int KP>
-__global__ void k_ppr_init(uint64_t n, const uint32_t* __restrict__ od, float* __restrict__ r32,
-                           float* __restrict__ c32, double* __restrict__ r64, double* __restrict__ c64) {
+__device__ __forceinline__ void ppr_init_vertex(uint64_t u, uint64_t n, const uint32_t* __restrict__ od,
+                                                float* __restrict__ r32, float* __restrict__ c32,
+                                                double* __restrict__ r64, double* __restrict__ c64) {
     const float x32 = (float)(1.0 / (double)n);
     const double x64 = 1.0 / (double)n;
-    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n; u += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t o = od[u];
+    const uint32_t o = od[u];
 #pragma unroll
-        for (int q = 0; q < KP; ++q) {
-            r32[u * KP + q] = x32;
-            r64[u * KP + q] = x64;
-            c32[u * KP + q] = o ? x32 / (float)o : 0.f;
-            c64[u * KP + q] = o ? x64 / (double)o : 0.0;
-        }
+    for (int q = 0; q < KP; ++q) {
+        r32[u * KP + q] = x32;
+        r64[u * KP + q] = x64;
+        c32[u * KP + q] = o ? x32 / (float)o : 0.f;
+        c64[u * KP + q] = o ? x64 / (double)o : 0.0;
     }
+}
+
+template <int KP>
+__global__ void k_ppr_init(uint64_t n, const uint32_t* __restrict__ od, float* __restrict__ r32,
+                           float* __restrict__ c32, double* __restrict__ r64, double* __restrict__ c64) {
+    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n; u += (uint64_t)gridDim.x * blockDim.x)
+        ppr_init_vertex<KP>(u, n, od, r32, c32, r64, c64);
 }
 
 __global__ void k_ppr_seedbits(const uint32_t* __restrict__ seeds, const uint32_t* __restrict__ lane_of,
@@ -596,6 +638,95 @@ __global__ void k_ppr_out(uint64_t n, int q, const uint32_t* __restrict__ new_of
                           double* __restrict__ out) {
     for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < n; u += (uint64_t)gridDim.x * blockDim.x)
         out[u] = r64[(uint64_t)new_of_old[u] * KP + q];
+}
+
+// ---- several one-lane solves on different graphs, swept together ----
+//
+// A cross-graph pass runs P one-lane solves (each on its own graph, on the
+// pull schedule) in lockstep: one launch per sweep phase covers every
+// problem, a workgroup finding its problem in a prefix table of block
+// counts. Every workgroup runs the one-graph kernels' block bodies
+// (ppr_rows_block, ppr_heavy_block, ppr_reduce_block) with its index inside
+// its own problem's grid, so each problem's arithmetic, and its result, is
+// that of its own solve alone.
+struct PprProb {
+    const uint32_t *off, *tgt, *od, *sbeg, *send, *hrow, *hsoff, *new_of_old;
+    uint32_t* seedbits;
+    PprLane* lanes;
+    PprCtl* ctl;
+    float *r32, *c32a, *c32b;
+    double *r64, *c64a, *c64b, *part, *spart, *out;
+    uint64_t n, nseg, H;
+    uint32_t light_blocks, nparts;
+};
+
+// Largest p < np with start[p] <= bid (start non-decreasing, start[0] = 0):
+// the problem whose block range holds bid.
+__device__ __forceinline__ uint32_t ppr_prob_of(const uint32_t* __restrict__ start, uint32_t np, uint32_t bid) {
+    uint32_t lo = 0, hi = np;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (start[mid] <= bid)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(PB) void k_pprm_rows(const PprProb* __restrict__ P, const uint32_t* __restrict__ start,
+                                                  uint32_t np, int parity) {
+    const uint32_t p = ppr_prob_of(start, np, blockIdx.x);
+    const PprProb& G = P[p];
+    const float* c32 = parity ? G.c32b : G.c32a;
+    float* c32n = parity ? G.c32a : G.c32b;
+    const double* c64 = parity ? G.c64b : G.c64a;
+    double* c64n = parity ? G.c64a : G.c64b;
+    ppr_rows_block<1>(blockIdx.x - start[p], G.n, G.off, G.tgt, G.od, G.seedbits, G.lanes, c32, c32n, G.r32, c64,
+                      c64n, G.r64, G.part, G.light_blocks, G.nseg, G.sbeg, G.send, G.spart);
+}
+
+__global__ __launch_bounds__(PB) void k_pprm_heavy(const PprProb* __restrict__ P, const uint32_t* __restrict__ start,
+                                                   uint32_t np, int parity) {
+    const uint32_t p = ppr_prob_of(start, np, blockIdx.x);
+    const PprProb& G = P[p];
+    float* c32n = parity ? G.c32a : G.c32b;
+    double* c64n = parity ? G.c64a : G.c64b;
+    ppr_heavy_block<1>(blockIdx.x - start[p], G.H, G.hrow, G.hsoff, G.spart, G.od, G.seedbits, G.lanes, c32n, G.r32,
+                       c64n, G.r64, G.part + (uint64_t)G.light_blocks * 2);
+}
+
+__global__ __launch_bounds__(PB) void k_pprm_reduce(const PprProb* __restrict__ P) {
+    const PprProb& G = P[blockIdx.x];
+    ppr_reduce_block<1>(G.part, G.nparts, G.lanes, G.ctl);
+}
+
+// x0 and cleared seed bits for every problem (block range start[p].. per problem).
+__global__ void k_pprm_init(const PprProb* __restrict__ P, const uint32_t* __restrict__ start, uint32_t np) {
+    const uint32_t p = ppr_prob_of(start, np, blockIdx.x);
+    const PprProb& G = P[p];
+    const uint64_t nb = start[p + 1] - start[p];
+    for (uint64_t u = (blockIdx.x - start[p]) * (uint64_t)blockDim.x + threadIdx.x; u < G.n; u += nb * blockDim.x) {
+        G.seedbits[u] = 0;
+        ppr_init_vertex<1>(u, G.n, G.od, G.r32, G.c32a, G.r64, G.c64a);
+    }
+}
+
+__global__ void k_pprm_seeds(const PprProb* __restrict__ P, const uint32_t* __restrict__ prob,
+                             const uint32_t* __restrict__ seed, uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) {
+        const PprProb& G = P[prob[i]];
+        atomicOr(&G.seedbits[G.new_of_old[seed[i]]], 1u);
+    }
+}
+
+__global__ void k_pprm_out(const PprProb* __restrict__ P, const uint32_t* __restrict__ start, uint32_t np) {
+    const uint32_t p = ppr_prob_of(start, np, blockIdx.x);
+    const PprProb& G = P[p];
+    const uint64_t nb = start[p + 1] - start[p];
+    for (uint64_t u = (blockIdx.x - start[p]) * (uint64_t)blockDim.x + threadIdx.x; u < G.n; u += nb * blockDim.x)
+        G.out[u] = G.r64[G.new_of_old[u]];
 }
 
 int lanes_padded(int k) {
@@ -784,6 +915,32 @@ uint64_t ppr_workspace_bytes(const ursa_hip_topo* t, int k, bool host_out) {
     return b + (b >> 4) + (64u << 10) * 16;
 }
 
+// The lane of query q on t: damping, tol, max_iter, the teleport share of
+// each of its distinct in-range seeds (returned in u, ascending) and its first
+// phase; phase 3 (nothing to run, x0 is the result) when no seed is in range
+// or max_iter is 0.
+static PprLane ppr_make_lane(const ursa_hip_topo* t, const PprQuery& q, std::vector<uint32_t>& u) {
+    const uint64_t n = t->n;
+    PprLane L{};
+    L.phase = 3;
+    u.clear();
+    for (uint32_t i = 0; i < q.nseeds; ++i)
+        if (q.seeds[i] < n) u.push_back(q.seeds[i]);
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    L.d = q.damping;
+    L.tol = q.tol;
+    L.max_iter = q.max_iter;
+    if (u.empty() || q.max_iter == 0) {
+        u.clear();
+        return L;
+    }
+    L.inv_s = 1.0 / (double)u.size();
+    L.pb = (1.0 - L.d) + L.d * ((double)t->ppr_dangling / (double)n);
+    L.phase = (L.tol < PPR_F32_TOL && L.max_iter > PPR_F32_MIN_ITER) ? 0 : 2;
+    return L;
+}
+
 void ppr_solve(ursa_hip_topo* t, const PprQuery* qs, int k, double* const* outs, uint32_t* iters_out) {
     URSA_REQUIRE(k >= 1 && k <= PPR_MAX_LANES, URSA_HIP_EINVAL, "a PPR solve takes 1..16 queries");
     const uint64_t n = t->n;
@@ -800,25 +957,15 @@ void ppr_solve(ursa_hip_topo* t, const PprQuery* qs, int k, double* const* outs,
 
     // Host-side lane set-up: distinct in-range seeds per lane.
     std::vector<PprLane> lanes(KP);
-    std::vector<uint32_t> seeds, lane_of;
+    std::vector<uint32_t> seeds, lane_of, u;
     uint32_t max_iter_all = 0;
     for (int q = 0; q < KP; ++q) {
         PprLane& L = lanes[q];
         L = PprLane{};
         L.phase = 3;
         if (q >= k) continue;
-        std::vector<uint32_t> u;
-        for (uint32_t i = 0; i < qs[q].nseeds; ++i)
-            if (qs[q].seeds[i] < n) u.push_back(qs[q].seeds[i]);
-        std::sort(u.begin(), u.end());
-        u.erase(std::unique(u.begin(), u.end()), u.end());
-        L.d = qs[q].damping;
-        L.tol = qs[q].tol;
-        L.max_iter = qs[q].max_iter;
-        if (u.empty() || qs[q].max_iter == 0) continue;
-        L.inv_s = 1.0 / (double)u.size();
-        L.pb = (1.0 - L.d) + L.d * ((double)t->ppr_dangling / (double)n);
-        L.phase = (L.tol < PPR_F32_TOL && L.max_iter > PPR_F32_MIN_ITER) ? 0 : 2;
+        L = ppr_make_lane(t, qs[q], u);
+        if (L.phase == 3) continue;
         max_iter_all = std::max(max_iter_all, L.max_iter);
         for (uint32_t v : u) {
             seeds.push_back(v);
@@ -892,6 +1039,246 @@ void ppr_solve(ursa_hip_topo* t, const PprQuery* qs, int k, double* const* outs,
         for (int q = 0; q < k; ++q) iters_out[q] = lanes[q].iters;
 }
 
+// ---- cross-graph passes (kernels above: k_pprm_*) ----
+
+namespace {
+
+uint32_t pprm_vec_blocks(uint64_t n) {
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 256));
+}
+
+// Per-problem device layout inside one allocation (byte offsets, each array
+// 256-byte aligned).
+struct PprmLayout {
+    uint64_t seedbits, r32, c32a, c32b, r64, c64a, c64b, part, spart, bytes;
+    uint32_t light_blocks, seg_blocks, heavy_blocks;
+};
+
+PprmLayout pprm_layout(const ursa_hip_topo* t) {
+    PprmLayout L{};
+    const uint64_t n = t->n;
+    L.light_blocks = (uint32_t)((n + PPR_ROWS - 1) / PPR_ROWS);
+    L.seg_blocks = (uint32_t)((t->ppr_S + PPR_GROUPS - 1) / PPR_GROUPS);
+    L.heavy_blocks = (uint32_t)((t->ppr_H + PB - 1) / PB);
+    uint64_t at = 0;
+    auto take = [&](uint64_t bytes) {
+        const uint64_t o = at;
+        at += (bytes + 255) & ~(uint64_t)255;
+        return o;
+    };
+    L.seedbits = take(n * 4);
+    L.r32 = take(n * 4);
+    L.c32a = take(n * 4);
+    L.c32b = take(n * 4);
+    L.r64 = take(n * 8);
+    L.c64a = take(n * 8);
+    L.c64b = take(n * 8);
+    L.part = take((uint64_t)(L.light_blocks + L.heavy_blocks) * 16);
+    L.spart = take(std::max<uint64_t>(t->ppr_S, 1) * 8);
+    L.bytes = at;
+    return L;
+}
+
+uint64_t align256(uint64_t b) { return (b + 255) & ~(uint64_t)255; }
+
+}  // namespace
+
+bool ppr_multi_eligible(ursa_hip_topo* t) {
+    hipStream_t s = stream();
+    ppr_prepare(t, s);
+    return t->n > 0 && t->n <= PPR_MULTI_MAX_N && !ppr_binned(t, s);
+}
+
+uint64_t ppr_multi_workspace_bytes(const ursa_hip_topo* const* ts, int count) {
+    uint64_t b = 0;
+    for (int i = 0; i < count; ++i) b += pprm_layout(ts[i]).bytes + align256(ts[i]->n * 8) + 256;
+    // Upload image: table, three prefix tables, lanes, controls, seeds.
+    b += align256((uint64_t)count * (sizeof(PprProb) + sizeof(PprLane) + sizeof(PprCtl) + 12 + 2 * 4 * 16) + 64) +
+         4 * 256;
+    return b + (b >> 6) + (64u << 10);
+}
+
+void ppr_solve_multi(ursa_hip_topo* const* ts, const PprQuery* qs, int count, double* const* outs,
+                     uint32_t* iters_out) {
+    if (count <= 0) return;
+    hipStream_t s = stream();
+    const uint32_t np = (uint32_t)count;
+    std::vector<PprmLayout> lay(np);
+    std::vector<PprLane> lanes(np);
+    std::vector<uint32_t> sprob, sval, u;
+    uint64_t state_bytes = 0, out_elems = 0;
+    uint32_t max_iter_all = 0;
+    for (uint32_t i = 0; i < np; ++i) {
+        ursa_hip_topo* t = ts[i];
+        URSA_REQUIRE(ppr_multi_eligible(t), URSA_HIP_EINVAL, "graph not eligible for a cross-graph pass");
+        URSA_REQUIRE(qs[i].nseeds == 0 || qs[i].seeds != nullptr, URSA_HIP_EINVAL, "seeds is NULL");
+        URSA_REQUIRE(qs[i].damping >= 0.0 && qs[i].damping <= 1.0, URSA_HIP_EINVAL, "damping must be in [0, 1]");
+        URSA_REQUIRE(outs[i] != nullptr, URSA_HIP_EINVAL, "out is NULL");
+        if (iters_out) iters_out[i] = 0;
+        lay[i] = pprm_layout(t);
+        state_bytes += lay[i].bytes;
+        out_elems += align256(t->n * 8) / 8;
+        lanes[i] = ppr_make_lane(t, qs[i], u);
+        if (lanes[i].phase != 3) max_iter_all = std::max(max_iter_all, lanes[i].max_iter);
+        for (uint32_t v : u) {
+            sprob.push_back(i);
+            sval.push_back(v);
+        }
+    }
+    // Upload image: [table | rows start | heavy start | vec start | lanes | ctl | seed problem | seed vertex].
+    const uint64_t o_tab = 0, o_rs = align256(np * sizeof(PprProb)), o_hs = o_rs + align256((np + 1) * 4),
+                   o_vs = o_hs + align256((np + 1) * 4), o_ln = o_vs + align256((np + 1) * 4),
+                   o_ct = o_ln + align256(np * sizeof(PprLane)), o_sp = o_ct + align256(np * sizeof(PprCtl)),
+                   o_sv = o_sp + align256(sprob.size() * 4), o_end = o_sv + align256(sval.size() * 4);
+    DevBuf<char> blob(o_end), state(std::max<uint64_t>(state_bytes, 256));
+    DevBuf<double> outbuf(std::max<uint64_t>(out_elems, 1));
+    std::vector<char> img(o_end, 0);
+    PprProb* tab = reinterpret_cast<PprProb*>(img.data() + o_tab);
+    uint32_t* rs = reinterpret_cast<uint32_t*>(img.data() + o_rs);
+    uint32_t* hs = reinterpret_cast<uint32_t*>(img.data() + o_hs);
+    uint32_t* vs = reinterpret_cast<uint32_t*>(img.data() + o_vs);
+    std::memcpy(img.data() + o_ln, lanes.data(), np * sizeof(PprLane));
+    if (!sprob.empty()) {
+        std::memcpy(img.data() + o_sp, sprob.data(), sprob.size() * 4);
+        std::memcpy(img.data() + o_sv, sval.data(), sval.size() * 4);
+    }
+    char* sb = state.get();
+    PprLane* d_lanes = reinterpret_cast<PprLane*>(blob.get() + o_ln);
+    PprCtl* d_ctl = reinterpret_cast<PprCtl*>(blob.get() + o_ct);
+    std::vector<bool> host_out(np);
+    std::vector<uint64_t> out_at(np);
+    uint64_t soff = 0, oat = 0;
+    rs[0] = hs[0] = vs[0] = 0;
+    for (uint32_t i = 0; i < np; ++i) {
+        const ursa_hip_topo* t = ts[i];
+        const PprmLayout& L = lay[i];
+        PprProb& G = tab[i];
+        G.off = t->pr_off.get();
+        G.tgt = t->pr_tgt.get();
+        G.od = t->pr_outdeg.get();
+        G.sbeg = t->ppr_sbeg.get();
+        G.send = t->ppr_send.get();
+        G.hrow = t->ppr_hrow.get();
+        G.hsoff = t->ppr_hsoff.get();
+        G.new_of_old = t->pr_new_of_old.get();
+        G.seedbits = reinterpret_cast<uint32_t*>(sb + soff + L.seedbits);
+        G.r32 = reinterpret_cast<float*>(sb + soff + L.r32);
+        G.c32a = reinterpret_cast<float*>(sb + soff + L.c32a);
+        G.c32b = reinterpret_cast<float*>(sb + soff + L.c32b);
+        G.r64 = reinterpret_cast<double*>(sb + soff + L.r64);
+        G.c64a = reinterpret_cast<double*>(sb + soff + L.c64a);
+        G.c64b = reinterpret_cast<double*>(sb + soff + L.c64b);
+        G.part = reinterpret_cast<double*>(sb + soff + L.part);
+        G.spart = reinterpret_cast<double*>(sb + soff + L.spart);
+        G.lanes = d_lanes + i;
+        G.ctl = d_ctl + i;
+        host_out[i] = !is_device_ptr(outs[i]);
+        out_at[i] = oat;
+        G.out = host_out[i] ? outbuf.get() + oat : outs[i];
+        oat += align256(t->n * 8) / 8;
+        G.n = t->n;
+        G.nseg = t->ppr_S;
+        G.H = t->ppr_H;
+        G.light_blocks = L.light_blocks;
+        G.nparts = L.light_blocks + L.heavy_blocks;
+        soff += L.bytes;
+        rs[i + 1] = rs[i] + L.light_blocks + L.seg_blocks;
+        hs[i + 1] = hs[i] + L.heavy_blocks;
+        vs[i + 1] = vs[i] + pprm_vec_blocks(t->n);
+    }
+    const uint32_t rows_total = rs[np], heavy_total = hs[np], vec_total = vs[np];
+    HIP_OK(hipMemcpyAsync(blob.get(), img.data(), o_end, hipMemcpyHostToDevice, s));
+    const PprProb* dP = reinterpret_cast<const PprProb*>(blob.get() + o_tab);
+    const uint32_t* d_rs = reinterpret_cast<const uint32_t*>(blob.get() + o_rs);
+    const uint32_t* d_hs = reinterpret_cast<const uint32_t*>(blob.get() + o_hs);
+    const uint32_t* d_vs = reinterpret_cast<const uint32_t*>(blob.get() + o_vs);
+    k_pprm_init<<<vec_total, 256, 0, s>>>(dP, d_vs, np);
+    check_launch();
+    if (!sprob.empty()) {
+        k_pprm_seeds<<<blocks_for(sprob.size(), 256), 256, 0, s>>>(
+            dP, reinterpret_cast<const uint32_t*>(blob.get() + o_sp), reinterpret_cast<const uint32_t*>(blob.get() + o_sv),
+            (uint32_t)sprob.size());
+        check_launch();
+    }
+    uint32_t it = 0;
+    while (it < max_iter_all) {
+        const uint32_t end = std::min<uint32_t>(max_iter_all, it + 8);
+        for (uint32_t i = it; i < end; ++i) {
+            const int par = (int)(i & 1);
+            timed("pprm_rows", 0.0, s, [&] { k_pprm_rows<<<rows_total, PB, 0, s>>>(dP, d_rs, np, par); });
+            if (heavy_total) k_pprm_heavy<<<heavy_total, PB, 0, s>>>(dP, d_hs, np, par);
+            k_pprm_reduce<<<np, PB, 0, s>>>(dP);
+            check_launch();
+        }
+        it = end;
+        PprCtl* h = static_cast<PprCtl*>(pinned_scratch(np * sizeof(PprCtl)));
+        HIP_OK(hipMemcpyAsync(h, d_ctl, np * sizeof(PprCtl), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        bool active = false;
+        for (uint32_t i = 0; i < np && !active; ++i) active = h[i].active != 0;
+        if (!active) break;
+    }
+    k_pprm_out<<<vec_total, 256, 0, s>>>(dP, d_vs, np);
+    check_launch();
+    HIP_OK(hipMemcpyAsync(lanes.data(), d_lanes, np * sizeof(PprLane), hipMemcpyDeviceToHost, s));
+    bool any_host = false;
+    for (uint32_t i = 0; i < np; ++i) any_host |= host_out[i];
+    if (any_host) {
+        const uint64_t bytes = oat * 8;
+        char* h = static_cast<char*>(pinned_scratch(bytes));
+        HIP_OK(hipMemcpyAsync(h, outbuf.get(), bytes, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        for (uint32_t i = 0; i < np; ++i)
+            if (host_out[i]) std::memcpy(outs[i], h + out_at[i] * 8, ts[i]->n * 8);
+    } else {
+        HIP_OK(hipStreamSynchronize(s));
+    }
+    if (iters_out)
+        for (uint32_t i = 0; i < np; ++i) iters_out[i] = lanes[i].iters;
+}
+
+bool ppr_multi_candidate(const ursa_hip_topo* t) {
+    return t->n > 0 && t->n <= PPR_MULTI_MAX_N && t->ppr_ready && g_ppr_schedule.load() != 2 &&
+           (g_ppr_schedule.load() == 1 || !t->ppr_bn_auto);
+}
+
+void ppr_solve_many(ursa_hip_topo* const* ts, const PprQuery* qs, int count, double* const* outs,
+                    uint32_t* iters_out) {
+    std::vector<ursa_hip_topo*> mt;
+    std::vector<PprQuery> mq;
+    std::vector<double*> mo;
+    std::vector<int> mi;
+    uint64_t mn = 0;
+    auto flush = [&] {
+        if (mt.empty()) return;
+        std::vector<uint32_t> it(mt.size(), 0);
+        ppr_solve_multi(mt.data(), mq.data(), (int)mt.size(), mo.data(), it.data());
+        if (iters_out)
+            for (size_t j = 0; j < mi.size(); ++j) iters_out[mi[j]] = it[j];
+        mt.clear();
+        mq.clear();
+        mo.clear();
+        mi.clear();
+        mn = 0;
+    };
+    for (int i = 0; i < count; ++i) {
+        URSA_REQUIRE(ts[i] != nullptr, URSA_HIP_EINVAL, "topology is NULL");
+        ursa_hip_topo* t = ts[i];
+        if (t->n == 0 || !ppr_multi_eligible(t)) {
+            double* o[1] = {outs[i]};
+            ppr_solve(t, &qs[i], 1, o, iters_out ? iters_out + i : nullptr);
+            continue;
+        }
+        if (mt.size() == PPR_MULTI_MAX_PASS || mn + t->n > PPR_MULTI_MAX_NODES) flush();
+        mt.push_back(t);
+        mq.push_back(qs[i]);
+        mo.push_back(outs[i]);
+        mi.push_back(i);
+        mn += t->n;
+    }
+    flush();
+}
+
 }  // namespace ursa
 
 using namespace ursa;
@@ -921,6 +1308,30 @@ int ursa_hip_ppr(ursa_hip_topo* t, const uint32_t* seeds, uint64_t nseeds, doubl
         PprQuery q{hs.data(), (uint32_t)nseeds, damping, max_iter, tol};
         double* o[1] = {out};
         ursa::ppr_solve(t, &q, 1, o, iters_out);
+    });
+}
+
+int ursa_hip_ppr_multi(uint32_t count, ursa_hip_topo* const* topos, const uint64_t* seed_offsets,
+                       const uint32_t* seeds, double damping, uint32_t max_iter, double tol, double* const* outs,
+                       uint32_t* iters_out) {
+    return guard([&] {
+        URSA_REQUIRE(count == 0 || (topos != nullptr && seed_offsets != nullptr && outs != nullptr), URSA_HIP_EINVAL,
+                     "topos/seed_offsets/outs is NULL");
+        std::vector<uint32_t> hs;
+        if (count && seeds && is_device_ptr(seeds)) {
+            hs.resize(seed_offsets[count]);
+            if (!hs.empty())
+                HIP_OK(hipMemcpy(hs.data(), seeds, hs.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            seeds = hs.data();
+        }
+        std::vector<PprQuery> qs(count);
+        for (uint32_t i = 0; i < count; ++i) {
+            const uint64_t a = seed_offsets[i], b = seed_offsets[i + 1];
+            URSA_REQUIRE(b >= a && b - a <= 0xFFFFFFFFull, URSA_HIP_EINVAL, "seed_offsets must not decrease");
+            URSA_REQUIRE(b == a || seeds != nullptr, URSA_HIP_EINVAL, "seeds is NULL");
+            qs[i] = PprQuery{seeds ? seeds + a : nullptr, (uint32_t)(b - a), damping, max_iter, tol};
+        }
+        ursa::ppr_solve_many(topos, qs.data(), (int)count, outs, iters_out);
     });
 }
 

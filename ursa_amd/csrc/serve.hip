@@ -132,11 +132,28 @@ struct ursa_hip_server {
 
     bool serial() const { return cfg.serial != 0; }
 
-    // Queries joined to the oldest waiting one (caller holds mu).
-    std::vector<uint64_t> form_batch() {
+    // Queries joined to the oldest waiting one (caller holds mu). A PPR query
+    // on a small graph takes every waiting PPR query on such graphs (any
+    // graph) into one cross-graph pass; otherwise a batch is same-kind,
+    // same-graph.
+    std::vector<uint64_t> form_batch(bool& cross) {
+        cross = false;
         std::vector<uint64_t> b{pending.front()};
         if (serial()) return b;
         const ursa_hip_query& h = recs[pending.front()].q;
+        if (h.kind == URSA_HIP_Q_PPR && ppr_multi_candidate(graphs[h.graph])) {
+            uint64_t nodes = graphs[h.graph]->n;
+            for (size_t i = 1; i < pending.size() && b.size() < cfg.max_batch; ++i) {
+                const ursa_hip_query& q = recs[pending[i]].q;
+                if (q.kind != URSA_HIP_Q_PPR || !ppr_multi_candidate(graphs[q.graph])) continue;
+                const uint64_t n = graphs[q.graph]->n;
+                if (nodes + n > PPR_MULTI_MAX_NODES) continue;
+                b.push_back(pending[i]);
+                nodes += n;
+            }
+            cross = b.size() > 1;
+            if (cross) return b;
+        }
         uint32_t cap = cfg.max_batch;
         if (h.kind == URSA_HIP_Q_PPR) cap = std::min<uint32_t>(cap, PPR_MAX_LANES);
         for (size_t i = 1; i < pending.size() && b.size() < cap; ++i) {
@@ -149,8 +166,19 @@ struct ursa_hip_server {
         return b;
     }
 
-    uint64_t workspace_need(const std::vector<uint64_t>& b) {
+    uint64_t workspace_need(const std::vector<uint64_t>& b, bool cross) {
         const ursa_hip_query& h = recs[b[0]].q;
+        if (cross) {
+            // The cross-graph pass, or (if a graph stops qualifying) one
+            // single-lane solve at a time inside the same reservation.
+            std::vector<const ursa_hip_topo*> ts;
+            uint64_t single = 0;
+            for (uint64_t i : b) {
+                ts.push_back(graphs[recs[i].q.graph]);
+                single = std::max(single, ppr_workspace_bytes(ts.back(), 1, true));
+            }
+            return std::max(ppr_multi_workspace_bytes(ts.data(), (int)ts.size()), single) + 4096;
+        }
         const ursa_hip_topo* t = graphs[h.graph];
         const uint64_t n = t->n, k = b.size();
         switch (h.kind) {
@@ -178,13 +206,14 @@ struct ursa_hip_server {
         cv_hash.notify_all();
     }
 
-    void run_served(const std::vector<uint64_t>& b, uint32_t slot, uint64_t off, uint64_t need);
+    void run_served(const std::vector<uint64_t>& b, bool cross, uint32_t slot, uint64_t off, uint64_t need);
     void run_serial(uint64_t i);
     void worker(uint32_t slot);
     void hasher();
 };
 
-void ursa_hip_server::run_served(const std::vector<uint64_t>& b, uint32_t slot, uint64_t off, uint64_t need) {
+void ursa_hip_server::run_served(const std::vector<uint64_t>& b, bool cross, uint32_t slot, uint64_t off,
+                                 uint64_t need) {
     const auto t_start = Clock::now();
     for (uint64_t i : b) {
         recs[i].start = t_start;
@@ -201,7 +230,24 @@ void ursa_hip_server::run_served(const std::vector<uint64_t>& b, uint32_t slot, 
         ScopedArena scope(&arena);
         hipStream_t s = stream();
         const uint64_t k = b.size();
-        if (h.kind == URSA_HIP_Q_PPR) {
+        if (cross) {
+            std::vector<ursa_hip_topo*> ts(k);
+            std::vector<PprQuery> qs(k);
+            std::vector<double*> outs(k);
+            std::vector<uint32_t> iters(k, 0);
+            for (uint64_t j = 0; j < k; ++j) {
+                Rec& r = recs[b[j]];
+                ts[j] = graphs[r.q.graph];
+                qs[j] = PprQuery{r.q.seeds, r.q.nseeds, r.q.damping, r.q.max_iter, r.q.tol};
+                r.data.resize(ts[j]->n * 8);
+                outs[j] = reinterpret_cast<double*>(r.data.data());
+            }
+            ppr_solve_many(ts.data(), qs.data(), (int)k, outs.data(), iters.data());
+            for (uint64_t j = 0; j < k; ++j) {
+                recs[b[j]].r.iters = iters[j];
+                recs[b[j]].r.len = ts[j]->n;
+            }
+        } else if (h.kind == URSA_HIP_Q_PPR) {
             std::vector<PprQuery> qs(k);
             for (uint64_t j = 0; j < k; ++j) {
                 const ursa_hip_query& q = recs[b[j]].q;
@@ -340,10 +386,11 @@ void ursa_hip_server::worker(uint32_t slot) {
     for (;;) {
         cv_work.wait(lk, [&] { return stop || !pending.empty(); });
         if (stop) return;
-        std::vector<uint64_t> b = form_batch();
+        bool cross = false;
+        std::vector<uint64_t> b = form_batch(cross);
         uint64_t need = 0, off = 0;
         if (!serial()) {
-            need = workspace_need(b);
+            need = workspace_need(b, cross);
             if (!ra.alloc(need, off)) {
                 if (inflight == 0) {
                     // Larger than the whole budget: it can never run.
@@ -367,7 +414,7 @@ void ursa_hip_server::worker(uint32_t slot) {
         if (serial())
             run_serial(b[0]);
         else
-            run_served(b, slot, off, need);
+            run_served(b, cross, slot, off, need);
         lk.lock();
         if (!serial()) ra.release(off, need);
         --inflight;

@@ -33,9 +33,11 @@ constexpr int TW = TB / 64;
 
 struct OCtr {
     uint32_t count;
-    uint32_t hit;    // target reached (shortest_path)
-    uint32_t heavy;  // frontier positions deferred to k_oexpand_heavy
-    uint32_t pad;
+    uint32_t hit;  // target reached (shortest_path)
+    // Frontier positions deferred to k_oexpand_heavy (high 32 bits) and the
+    // 64-slot chunks of their lists (low 32 bits), taken by one atomic add
+    // each, so deferred entry h owns the chunks from its low word on.
+    unsigned long long hpack;
 };
 
 constexpr uint32_t O_HEAVY = 1024;  // frontier vertices with more edges go grid-wide
@@ -57,6 +59,15 @@ __device__ __forceinline__ void o_flush(OBuf& ob, uint32_t* __restrict__ cand, O
     ob.cnt = 0;
 }
 
+// Appends the wave's fresh candidates to its LDS buffer.
+__device__ __forceinline__ void o_push(bool fresh, uint32_t v, uint32_t* __restrict__ cand, OCtr* ctr, OBuf& ob) {
+    const uint64_t mask = __ballot(fresh);
+    if (!mask) return;
+    if (fresh) ob.s[ob.cnt + (uint32_t)__popcll(mask & lanemask_lt())] = v;
+    ob.cnt += (uint32_t)__popcll(mask);  // < 128: flushed below once it reaches 64
+    if (ob.cnt >= 64) o_flush(ob, cand, ctr);
+}
+
 // Offers v, reached by frontier position p through its slot j: atomicMin of
 // the discovery key, and the thread that replaced the initial key appends v to
 // the candidate list.
@@ -65,11 +76,7 @@ __device__ __forceinline__ void o_offer(bool active, uint32_t v, uint64_t p, uin
                                         uint32_t* __restrict__ cand, OCtr* ctr, OBuf& ob) {
     bool fresh = false;
     if (active && level[v] < 0) fresh = atomicMin(&key[v], ((unsigned long long)p << 32) | j) == ~0ull;
-    const uint64_t mask = __ballot(fresh);
-    if (!mask) return;
-    if (fresh) ob.s[ob.cnt + (uint32_t)__popcll(mask & lanemask_lt())] = v;
-    ob.cnt += (uint32_t)__popcll(mask);  // < 128: flushed below once it reaches 64
-    if (ob.cnt >= 64) o_flush(ob, cand, ctr);
+    o_push(fresh, v, cand, ctr, ob);
 }
 
 // Expand frontier F[0..flen): candidates get atomicMin(key, p<<32 | j) and are
@@ -78,7 +85,8 @@ __device__ __forceinline__ void o_offer(bool active, uint32_t v, uint64_t p, uin
 __global__ __launch_bounds__(TB) void k_oexpand(const uint32_t* __restrict__ F, uint32_t flen, NbrView fwd,
                                                 const int32_t* __restrict__ level,
                                                 unsigned long long* __restrict__ key, uint32_t* __restrict__ cand,
-                                                OCtr* ctr, uint32_t* __restrict__ heavy) {
+                                                OCtr* ctr, uint32_t* __restrict__ heavy,
+                                                uint32_t* __restrict__ hbase) {
     __shared__ uint32_t s_buf[TW][128];
     const int lane = lane_id();
     const uint64_t gw = blockIdx.x * (uint64_t)TW + (threadIdx.x >> 6);
@@ -93,7 +101,9 @@ __global__ __launch_bounds__(TB) void k_oexpand(const uint32_t* __restrict__ F, 
             sa = fwd.off_a[u];
             if (fwd.off_b) sb = fwd.off_b[u];
             if (d > O_HEAVY) {
-                heavy[atomicAdd(&ctr->heavy, 1u)] = (uint32_t)idx;
+                const unsigned long long old = atomicAdd(&ctr->hpack, (1ull << 32) | (unsigned long long)((d + 63) / 64));
+                heavy[old >> 32] = (uint32_t)idx;
+                hbase[old >> 32] = (uint32_t)old;
                 d = 0;
             }
         }
@@ -115,31 +125,67 @@ __global__ __launch_bounds__(TB) void k_oexpand(const uint32_t* __restrict__ F, 
     o_flush(ob, cand, ctr);
 }
 
-// Deferred frontier vertices: every wave of the grid takes a stride of each
-// one's slots, so a hub's edges spread over the device.
+// Deferred frontier vertices. Their 64-slot chunks are numbered in one
+// sequence (entry h owns chunks hbase[h] .. hbase[h+1]-1, the last entry up to
+// the total) and every wave takes a contiguous run of that sequence, so a
+// hub's slots spread over the grid and no wave walks the deferred list. The
+// keys are atomic minima, so which wave offers a slot changes nothing.
 __global__ __launch_bounds__(TB) void k_oexpand_heavy(const uint32_t* __restrict__ F, NbrView fwd,
                                                       const int32_t* __restrict__ level,
                                                       unsigned long long* __restrict__ key,
                                                       uint32_t* __restrict__ cand, OCtr* ctr,
-                                                      const uint32_t* __restrict__ heavy) {
+                                                      const uint32_t* __restrict__ heavy,
+                                                      const uint32_t* __restrict__ hbase) {
     __shared__ uint32_t s_buf[TW][128];
     OBuf ob{s_buf[threadIdx.x >> 6], 0};
-    const uint32_t cnt = ctr->heavy;
-    const uint64_t wave_base = (blockIdx.x * (uint64_t)TW + (threadIdx.x >> 6)) * 64;
-    const uint64_t stride = (uint64_t)gridDim.x * TB;
-    for (uint32_t h = 0; h < cnt; ++h) {
-        const uint32_t p = heavy[h];
+    const unsigned long long hp = ctr->hpack;
+    const uint32_t cnt = (uint32_t)(hp >> 32), total = (uint32_t)hp;
+    if (cnt == 0) return;
+    const int lane = lane_id();
+    const uint64_t nwaves = (uint64_t)gridDim.x * TW;
+    const uint64_t gw = blockIdx.x * (uint64_t)TW + (threadIdx.x >> 6);
+    const uint64_t per = (total + nwaves - 1) / nwaves;
+    uint32_t c = (uint32_t)min((uint64_t)total, gw * per);
+    const uint32_t cend = (uint32_t)min((uint64_t)total, gw * per + per);
+    if (c >= cend) return;
+    uint32_t lo = 0, hi = cnt;  // largest h with hbase[h] <= c
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (hbase[mid] <= c)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    for (uint32_t h = lo; c < cend; ++h) {
+        const uint32_t p = heavy[h], hb = hbase[h];
+        const uint32_t hn = h + 1 < cnt ? hbase[h + 1] : total;
         const uint32_t u = F[p];
         uint32_t da;
         const uint32_t d = nv_degree(fwd, u, da);
         const uint32_t sa = fwd.off_a[u];
         const uint32_t sb = fwd.off_b ? fwd.off_b[u] : 0;
-        for (uint64_t e0 = wave_base; e0 < d; e0 += stride) {
-            const uint32_t e = (uint32_t)(e0 + lane_id());
-            uint32_t v = 0;
-            if (e < d) v = e < da ? fwd.tgt_a[sa + e] : fwd.tgt_b[sb + (e - da)];
-            o_offer(e < d, v, p, e, level, key, cand, ctr, ob);
+        const uint32_t c1 = min(cend, hn);
+        const uint32_t e1 = min(d, (c1 - hb) * 64u);
+        // Four 64-slot steps in flight: their targets, then their levels, then the offers.
+        for (uint32_t eb = (c - hb) * 64u; eb < e1; eb += 256) {
+            uint32_t v[4];
+            int32_t lv[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t e = eb + 64u * j + lane;
+                v[j] = e < e1 ? (e < da ? fwd.tgt_a[sa + e] : fwd.tgt_b[sb + (e - da)]) : 0u;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) lv[j] = eb + 64u * j + lane < e1 ? level[v[j]] : 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t e = eb + 64u * j + lane;
+                bool fresh = false;
+                if (lv[j] < 0) fresh = atomicMin(&key[v[j]], ((unsigned long long)p << 32) | e) == ~0ull;
+                o_push(fresh, v[j], cand, ctr, ob);
+            }
         }
+        c = c1;
     }
     o_flush(ob, cand, ctr);
 }
@@ -195,10 +241,10 @@ struct OWork {
     DevBuf<unsigned long long> key;
     DevBuf<uint32_t> order, cand, sorted, parent;
     DevBuf<uint64_t> ckeys, skeys;
-    DevBuf<uint32_t> heavy;
+    DevBuf<uint32_t> heavy, hbase;
     DevBuf<OCtr> ctr;
     OWork(uint64_t n_, bool with_parent, hipStream_t s)
-        : n(n_), level(n_), key(n_), order(n_), cand(n_), sorted(n_), ckeys(n_), skeys(n_), heavy(n_), ctr(1) {
+        : n(n_), level(n_), key(n_), order(n_), cand(n_), sorted(n_), ckeys(n_), skeys(n_), heavy(n_), hbase(n_), ctr(1) {
         if (with_parent) parent.alloc(n_);
         k_fill_i32b<<<blocks_for(n, 256), 256, 0, s>>>(level.get(), n, -1);
         k_fill_u64<<<blocks_for(n, 256), 256, 0, s>>>(key.get(), n, ~0ull);
@@ -226,10 +272,10 @@ bool ordered_bfs(uint32_t seed, uint32_t max_levels, uint32_t target, const NbrV
         const uint32_t* F = w.order.get() + fstart;
         k_oexpand<<<blocks_for((flen + 63) / 64, TW, 1u << 16), TB, 0, s>>>(F, (uint32_t)flen, fwd, w.level.get(),
                                                                            w.key.get(), w.cand.get(), w.ctr.get(),
-                                                                           w.heavy.get());
+                                                                           w.heavy.get(), w.hbase.get());
         if (may_have_heavy)
-            k_oexpand_heavy<<<512, TB, 0, s>>>(F, fwd, w.level.get(), w.key.get(), w.cand.get(), w.ctr.get(),
-                                               w.heavy.get());
+            k_oexpand_heavy<<<2048, TB, 0, s>>>(F, fwd, w.level.get(), w.key.get(), w.cand.get(), w.ctr.get(),
+                                                w.heavy.get(), w.hbase.get());
         check_launch();
         const uint32_t cnt = w.read(s).count;
         if (cnt == 0) break;
@@ -274,10 +320,10 @@ __global__ void k_opath(const uint32_t* __restrict__ parent, uint32_t src, uint3
 }  // namespace
 
 uint64_t traverse_workspace_bytes(uint64_t n) {
-    // OWork (level, key, order, cand, sorted, parent, ckeys, skeys, heavy: 48 B
-    // per node) plus the radix sort's temporary storage for up to n (u64, u32)
-    // pairs (a double buffer of 12 B per pair and its histograms).
-    return n * 80 + (2u << 20);
+    // OWork (level, key, order, cand, sorted, parent, ckeys, skeys, heavy,
+    // hbase: 52 B per node) plus the radix sort's temporary storage for up to n
+    // (u64, u32) pairs (a double buffer of 12 B per pair and its histograms).
+    return n * 84 + (2u << 20);
 }
 
 void k_hop_seq(ursa_hip_topo* t, const uint32_t* seeds, uint64_t nseeds, uint32_t k, int dir,
