@@ -202,6 +202,20 @@ def bench_c2(args, ua, gen):
             kernels[g]["moved_frac"] = round(moved / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
     kernels["bfs"]["algo_model"] = ("4 E_reach + 12 n summed over the 64 sources; the 64 lanes share every edge "
                                     "scan, so moved_GB is the bus figure")
+    # Triangle work: the walk model (what the one-sided probe reads) is the
+    # algorithmic figure; the sorted-merge model (both lists of every pair) is
+    # reported beside it.
+    e_or, merge, walk = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+    check(lib.ursa_hip_triangle_work(topo._h, C.byref(e_or), C.byref(merge), C.byref(walk)))
+    tri_ms = stats["triangle"][0]
+    merge_bytes = 4.0 * merge.value + 12.0 * n
+    kernels["triangle"]["algo_model"] = ("one-sided walk: 4 B per walked list entry + 12 B per oriented edge "
+                                         "(partner id, two offsets) + 12 B per node")
+    kernels["triangle"]["walk_steps"] = walk.value
+    kernels["triangle"]["oriented_edges"] = e_or.value
+    kernels["triangle"]["merge_model_GB"] = round(merge_bytes / 1e9, 4)
+    if tri_ms > 0:
+        kernels["triangle"]["merge_model_frac"] = round(merge_bytes / (tri_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
     log("kernels: " + json.dumps(kernels))
     dom = max(("bfs", "cc", "triangle"), key=lambda g: stats[g][0])
     if dom == "bfs":

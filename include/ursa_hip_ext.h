@@ -77,6 +77,12 @@ int ursa_hip_graphs_enable(int on);
  * hash sets. With 0, topologies whose triangle orientation is built afterwards
  * use hash sets only. Counts are identical either way. */
 int ursa_hip_triangle_bitmaps(int on);
+/* Work figures of triangle_count on t (builds its orientation if needed):
+ * oriented edges |E+|, the sorted-merge intersection length
+ * sum over (u,v) in E+ of d+(u) + d+(v), and the list entries the one-sided
+ * walks read (each pair's shorter side, as the kernels walk it). */
+int ursa_hip_triangle_work(ursa_hip_topo* t, uint64_t* oriented_edges, uint64_t* merge_steps,
+                           uint64_t* walk_steps);
 
 /* PageRank sweep schedule (default 0): 0 automatic (binned from 32 blocks of
  * 8192 nodes up), 1 pull (each in-row sums its sources' contributions gathered

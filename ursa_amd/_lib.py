@@ -81,6 +81,7 @@ SIGNATURES = {
     "ursa_hip_ppr": (cint, [P, P, u64, f64, u32, f64, P, C.POINTER(u32)]),
     "ursa_hip_graphs_enable": (cint, [cint]),
     "ursa_hip_triangle_bitmaps": (cint, [cint]),
+    "ursa_hip_triangle_work": (cint, [P, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)]),
     "ursa_hip_pagerank_schedule": (cint, [cint]),
     "ursa_hip_ppr_schedule": (cint, [cint]),
     "ursa_hip_exact_sum": (cint, [P, u64, P]),

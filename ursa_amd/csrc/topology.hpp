@@ -93,7 +93,8 @@ struct ursa_hip_topo {
     };
     TcItems tc_items[2][4];
     uint64_t tc_slots = 0;
-    double tc_pair_work = 0.0;
+    double tc_pair_work = 0.0;     // sum over oriented pairs of d+(u) + d+(v) (sorted-merge steps)
+    uint64_t tc_walk_steps = 0;     // entries the one-sided walks read, both item kinds
     bool tc_built = false;
     // Personalized PageRank's split of high in-degree rows into fixed
     // segments (see ppr.hip); depends on the graph only.

@@ -55,9 +55,9 @@ constexpr uint32_t MID_CAP = 1024;   // N+(u) entries staged per workgroup in th
 constexpr uint32_t BIG_CAP = 2048;   // N+(u) entries staged in LDS per workgroup (24 KB table)
 constexpr uint32_t BM_BITS = 65536;  // bitmap tables cover new ids [0, BM_BITS) (8 KB)
 constexpr uint32_t BM_WORDS = BM_BITS / 32;
-constexpr uint64_t TC_CHUNK = 32768;  // intersection steps per work item (about)
+constexpr uint64_t TC_CHUNK = 65536;  // intersection steps per work item (about)
 constexpr uint32_t TC_MAX_SLOTS = 32768;  // list slots per work item at most (keeps hit counters below 2^16)
-constexpr int TC_ILP = 2;            // 64-step groups whose loads are in flight together
+constexpr int TC_ILP = 4;            // 64-step groups whose loads are in flight together (s22: 2 -> 4 cut 25.9 to 21.4 ms, 8 gave 22.1)
 constexpr uint32_t TC_LONG = 32;     // walks at least this long are taken by the whole wave
 constexpr int TC_LILP = 8;           // 64-step chunks of long walks in flight together
 
@@ -68,12 +68,14 @@ constexpr int TC_LILP = 8;           // 64-step chunks of long walks in flight t
 // shorter list is walked wherever the longer one is a hub's; pairs among short
 // lists stay on x's side, where each table is staged once and no partner needs
 // an atomic credit (on graphs of even, small degrees splitting them as well
-// doubled the staging and the launches). At s22 a hub threshold of 32 to 128
-// measured 25.4 to 25.6 ms, 256 26.5 ms.
+// doubled the staging and the launches). At s22, with four short-walk groups
+// in flight and items of 65536 steps, a hub threshold of 128 measured 21.8 ms,
+// 256 20.4 ms, 512 21.2 ms and 1024 33.4 ms (on the serving mix's densest
+// R-MAT graph, 1.41 M nodes and 59.7 M edges: 256 23.7 ms, 512 23.3 ms).
 // A triangle x < y < z closes at w = z in N+(x) and N+(y). Walk length of a
 // slot, with `owner` the table's vertex and `partner` the other end; a pair
 // needs |N+(x)| >= 2.
-constexpr uint32_t TC_HUB = 128;
+constexpr uint32_t TC_HUB = 256;
 __device__ __forceinline__ uint32_t tc_walk(bool in, uint32_t d_owner, uint32_t d_partner) {
     if (in) return d_partner >= 2 && d_partner < d_owner && d_owner > TC_HUB ? d_partner : 0u;
     return d_owner >= 2 && (d_partner <= d_owner || d_partner <= TC_HUB) ? d_partner : 0u;
@@ -688,6 +690,7 @@ static void tc_orient(ursa_hip_topo* t, hipStream_t s) {
         offsets_from_sorted(nullptr, 0, n, t->tc_ioff.get(), s);
     }
     // Work items (see the file comment), for out-lists ([0]) and in-lists ([1]).
+    uint64_t walk_steps = 0;  // list entries walked against a table, both kinds
     for (int kind = 0; kind < 2; ++kind) {
         auto& items = t->tc_items[kind];
         const uint32_t* loff = kind ? t->tc_ioff.get() : t->tc_off.get();
@@ -707,6 +710,14 @@ static void tc_orient(ursa_hip_topo* t, hipStream_t s) {
                                                        cost.get());
         check_launch();
         prim::exclusive_sum_u32_to_u64(cost.get(), pre.get(), kept, s);
+        {
+            uint64_t last_pre = 0;
+            uint32_t last_cost = 0;
+            HIP_OK(hipMemcpyAsync(&last_pre, pre.get() + (kept - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+            HIP_OK(hipMemcpyAsync(&last_cost, cost.get() + (kept - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            HIP_OK(hipStreamSynchronize(s));
+            walk_steps += last_pre + last_cost;
+        }
         k_tc_item_flags<<<blocks_for(kept, 256), 256, 0, s>>>(loff, t->tc_off.get(), owner, cost.get(), pre.get(),
                                                              kept, flag.get(), iota.get());
         check_launch();
@@ -748,6 +759,7 @@ static void tc_orient(ursa_hip_topo* t, hipStream_t s) {
     HIP_OK(hipStreamSynchronize(s));
     t->tc_slots = kept;
     t->tc_pair_work = (double)hp;
+    t->tc_walk_steps = walk_steps;
     t->tc_built = true;
 }
 
@@ -769,7 +781,9 @@ void triangle_count(ursa_hip_topo* t, uint64_t* out_user) {
     for (const auto& kind : t->tc_items)
         for (const auto& it : kind) nitems += it.n;
     if (nitems) {
-        const double bytes = 4.0 * t->tc_pair_work + 12.0 * (double)n;
+        // Bytes the one-sided walk must read: each walked entry (4), each
+        // oriented pair's partner id and offsets (12), per node 12.
+        const double bytes = 4.0 * (double)t->tc_walk_steps + 12.0 * (double)t->tc_slots + 12.0 * (double)n;
         auto* tri = tnew.get();
         const uint32_t* ooff = t->tc_off.get();
         const uint32_t* otgt = t->tc_tgt.get();
@@ -831,6 +845,17 @@ void triangle_count(ursa_hip_topo* t, uint64_t* out_user) {
 extern "C" int ursa_hip_triangle_bitmaps(int on) {
     ursa::g_tc_bitmaps.store(on != 0);
     return URSA_HIP_OK;
+}
+
+extern "C" int ursa_hip_triangle_work(ursa_hip_topo* t, uint64_t* oriented_edges, uint64_t* merge_steps,
+                                      uint64_t* walk_steps) {
+    return ursa::guard([&] {
+        URSA_REQUIRE(t != nullptr, URSA_HIP_EINVAL, "topology is NULL");
+        ursa::triangle_prepare(t);
+        if (oriented_edges) *oriented_edges = t->tc_slots;
+        if (merge_steps) *merge_steps = (uint64_t)t->tc_pair_work;
+        if (walk_steps) *walk_steps = t->tc_walk_steps;
+    });
 }
 
 extern "C" int ursa_hip_triangle_count(ursa_hip_topo* t, uint64_t* out) {
