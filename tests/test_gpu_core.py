@@ -304,6 +304,28 @@ def test_intern_i64_matches_oracle(ua):
     assert n == 0 and sd.size == 0
 
 
+def test_intern_i64_table_edge_cases(ua):
+    """The device hash table marks empty slots with the id -2^63, which is still a
+    valid id (counted beside the table, first seen anywhere in the rows); and more
+    distinct ids than the first 2^23-slot table holds (12 M) force its rebuild."""
+    lo = np.int64(-2**63)
+    for s, d in [(np.array([lo, 5, 7, lo]), np.array([5, lo, lo, 9])),
+                 (np.array([3, 4, 5]), np.array([lo, 4, 3])),
+                 (np.array([lo]), np.array([lo]))]:
+        n, sd, dd, users = ua.intern_i64(s, d)
+        rn, rs, rd, ru = orc.intern_i64(s, d)
+        assert n == rn and np.array_equal(sd, rs) and np.array_equal(dd, rd) and np.array_equal(users, ru)
+    rng = np.random.default_rng(8)
+    m = 6_000_000
+    s = rng.integers(-2**62, 2**62, m, dtype=np.int64)
+    d = rng.integers(-2**62, 2**62, m, dtype=np.int64)
+    d[::97] = s[::97]  # some repeats
+    n, sd, dd, users = ua.intern_i64(s, d)
+    rn, rs, rd, ru = orc.intern_i64(s, d)
+    assert n == rn and n > (1 << 23)
+    assert np.array_equal(sd, rs) and np.array_equal(dd, rd) and np.array_equal(users, ru)
+
+
 def test_intern_i64_batches_matches_oracle(ua):
     """Interning across batches (id_map.rs:220-292): the two-chunk golden of
     id_map.rs:529-550, the empty batch list, empty batches between full ones, and an
