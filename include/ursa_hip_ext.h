@@ -38,13 +38,15 @@ uint64_t ursa_hip_profile_names(char* buf, uint64_t cap);
 int ursa_hip_bfs_ex(ursa_hip_topo* t, const uint32_t* sources, uint32_t k, int direction,
                     int32_t* levels, uint64_t* reached_edges, int mode);
 /* Traversal schedule autotuner. Times the BFS direction switch (alpha, beta in
- * {4, 15, 60} x {6, 18, 54}) on 4 single-source traversals and the multi-source
- * push rule (divisor in {5, 20, 80}) on one 64-source batch of this graph, and
- * keeps the fastest for `direction` on the topology. The choice is also cached
- * under a key of the forward degree distribution (log2 n, mean degree, max/mean
- * degree, share of zero-degree vertices); a later graph with the same key takes
- * it without timing unless `force`. *source: 1 measured, 2 from the key cache.
- * Results of every traversal are identical under any parameters. */
+ * {4, 15, 60} x {6, 18, 54}) on 8 single-source traversals and the multi-source
+ * push rule (divisor in {5, 20, 80}) on two 64-source batches of this graph
+ * (sources in a seeded random order over the vertices with forward edges), and
+ * keeps the fastest for `direction` on the topology; a candidate displaces the
+ * defaults (15, 18; 20) only when more than 5 % faster. The choice is also
+ * cached under a key of the forward degree distribution (log2 n, mean degree,
+ * max/mean degree, share of zero-degree vertices); a later graph with the same
+ * key takes it without timing unless `force`. *source: 1 measured, 2 from the
+ * key cache. Results of every traversal are identical under any parameters. */
 int ursa_hip_autotune(ursa_hip_topo* t, int direction, int force, double* alpha, double* beta,
                       uint32_t* push_div, int* source);
 
