@@ -15,7 +15,11 @@
  * same kind on the same graph (PPR up to 16 lanes of one solve, k-hop and
  * shortest-path queries in one workspace, identical triangle queries served by
  * one count), reserves the pass's device workspace from the HBM budget and
- * runs it; queries that arrive meanwhile wait for the next free slot. A pass
+ * runs it; queries that arrive meanwhile wait for the next free slot. Queries
+ * on small graphs are also joined across graphs: PPR on graphs of at most 2^19
+ * nodes (one launch per sweep phase for all of them, up to max_batch), k-hop
+ * and shortest path on graphs of at most 2^16 nodes (one workgroup per query,
+ * one launch per pass, up to cross_batch). A pass
  * whose workspace does not fit waits for running passes to release memory
  * (first come, first served). Passes on different slots overlap on the device.
  * Every query's result is identical to the plain entry point's on the same
@@ -74,9 +78,11 @@ typedef struct {
     uint64_t workspace_bytes; /* HBM budget for in-flight passes (0: 25% of free memory, at most 32 GiB) */
     uint64_t graph_bytes;     /* HBM budget for resident graphs (0: unlimited) */
     uint32_t slots;           /* concurrent passes, one stream each (0: 8) */
-    uint32_t max_batch;       /* queries per pass (0: 16; PPR passes take at most 16) */
+    uint32_t max_batch;       /* queries per pass (0: 16; PPR passes take at most 16; 1: no batching) */
     int32_t serial;           /* 1: plain serial loop (slots, max_batch, budget ignored) */
     int32_t keep_results;     /* keep every result for ursa_hip_server_result_data */
+    uint32_t cross_batch;     /* k-hop/shortest-path queries per cross-graph pass (0: 256; at least max_batch) */
+    uint32_t reserved;
 } ursa_hip_server_config;
 
 int ursa_hip_server_create(const ursa_hip_server_config* cfg, ursa_hip_server** out);

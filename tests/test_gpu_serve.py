@@ -429,6 +429,48 @@ def test_server_mix_equals_serial_alone_and_oracle(ua, sv, pool):
             assert r1.digest == rm[i].digest and r1.batch == 1, i
 
 
+def test_server_cross_graph_traversals_equal_plain_and_oracle(ua, sv, pool):
+    """k-hop and shortest-path queries on small graphs waiting together run as
+    cross-graph passes (one workgroup per query): every result equals the plain
+    serial loop's and the oracle's, over all three directions, depths 0-4,
+    several seeds per k-hop query (one unknown), unreachable, equal and
+    out-of-range path endpoints."""
+    topos, dense = pool
+    ns = [t.n_nodes() for t in topos]
+    rng = np.random.default_rng(31)
+    qs = []
+    for i in range(160):
+        g = int(rng.integers(0, len(topos)))
+        n = ns[g]
+        d = int(rng.integers(0, 3))
+        if i % 2 == 0:
+            seeds = rng.integers(0, n, int(rng.integers(1, 5))).tolist()
+            if i % 10 == 0:
+                seeds.append(n + 7)  # unknown seed (hop.rs:91-93)
+            qs.append(sv.khop_query(g, seeds, k=int(rng.integers(0, 5)), direction=d))
+        else:
+            a, b = (int(x) for x in rng.integers(0, n, 2))
+            if i % 9 == 1:
+                b = a
+            if i % 13 == 1:
+                b = n + 3
+            qs.append(sv.path_query(g, a, b, direction=d))
+    with sv.Server(serial=True) as ser:
+        for t in topos:
+            ser.add_graph(t)
+        rs = ser.run(qs)
+    with sv.Server(slots=4, max_batch=16, keep_results=True) as srv:
+        for t in topos:
+            srv.add_graph(t)
+        rm = srv.run(qs)
+        assert max(r.batch for r in rm) > 1
+        orcs = [orc.Topology(n, s, d) for (n, s, d) in dense]
+        for i, q in enumerate(qs):
+            assert rs[i].status == 0 and rm[i].status == 0, (i, rs[i].status, rm[i].status)
+            assert rs[i].digest == rm[i].digest and rs[i].len == rm[i].len, (i, q.kind)
+            assert same(q.kind, srv.result(i, q), oracle_result(orcs[q.graph], q, sv), sv), (i, q.kind)
+
+
 def test_server_timed_arrivals_and_latency_fields(ua, sv, pool):
     topos, _ = pool
     ns = [t.n_nodes() for t in topos]

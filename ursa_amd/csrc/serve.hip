@@ -154,6 +154,18 @@ struct ursa_hip_server {
             cross = b.size() > 1;
             if (cross) return b;
         }
+        if ((h.kind == URSA_HIP_Q_KHOP || h.kind == URSA_HIP_Q_PATH) && traverse_multi_eligible(graphs[h.graph])) {
+            // A workgroup per query: a pass takes many (each runs on its own CU).
+            const uint32_t tcap = cfg.max_batch <= 1 ? 1u : std::max(cfg.max_batch, cfg.cross_batch);
+            for (size_t i = 1; i < pending.size() && b.size() < tcap; ++i) {
+                const ursa_hip_query& q = recs[pending[i]].q;
+                if ((q.kind == URSA_HIP_Q_KHOP || q.kind == URSA_HIP_Q_PATH) && traverse_multi_eligible(graphs[q.graph]))
+                    b.push_back(pending[i]);
+            }
+            cross = b.size() > 1;
+            if (cross) return b;
+            b.resize(1);
+        }
         uint32_t cap = cfg.max_batch;
         if (h.kind == URSA_HIP_Q_PPR) cap = std::min<uint32_t>(cap, PPR_MAX_LANES);
         for (size_t i = 1; i < pending.size() && b.size() < cap; ++i) {
@@ -166,8 +178,28 @@ struct ursa_hip_server {
         return b;
     }
 
+    // The cross-graph traversal tasks of a batch (results not yet filled).
+    std::vector<TravTask> trav_tasks(const std::vector<uint64_t>& b) {
+        std::vector<TravTask> ts(b.size());
+        for (size_t j = 0; j < b.size(); ++j) {
+            const ursa_hip_query& q = recs[b[j]].q;
+            ts[j].t = graphs[q.graph];
+            ts[j].path = q.kind == URSA_HIP_Q_PATH;
+            ts[j].seeds.assign(q.seeds, q.seeds + (ts[j].path ? 2 : q.nseeds));
+            ts[j].k = q.k;
+            ts[j].dir = q.direction;
+        }
+        return ts;
+    }
+
     uint64_t workspace_need(const std::vector<uint64_t>& b, bool cross) {
         const ursa_hip_query& h = recs[b[0]].q;
+        if (cross && h.kind != URSA_HIP_Q_PPR) {
+            const std::vector<TravTask> ts = trav_tasks(b);
+            std::vector<const TravTask*> ps;
+            for (const TravTask& t : ts) ps.push_back(&t);
+            return traverse_multi_workspace_bytes(ps) + 4096;
+        }
         if (cross) {
             // The cross-graph pass, or (if a graph stops qualifying) one
             // single-lane solve at a time inside the same reservation.
@@ -230,7 +262,30 @@ void ursa_hip_server::run_served(const std::vector<uint64_t>& b, bool cross, uin
         ScopedArena scope(&arena);
         hipStream_t s = stream();
         const uint64_t k = b.size();
-        if (cross) {
+        if (cross && h.kind != URSA_HIP_Q_PPR) {
+            std::vector<TravTask> ts = trav_tasks(b);
+            std::vector<TravTask*> ps;
+            for (TravTask& t : ts) ps.push_back(&t);
+            traverse_multi(ps);
+            for (uint64_t j = 0; j < k; ++j) {
+                Rec& r = recs[b[j]];
+                const TravTask& q = ts[j];
+                if (q.path) {
+                    r.r.len = q.found.size();
+                    r.data.resize(q.found.size() * 4);
+                    if (!q.found.empty()) std::memcpy(r.data.data(), q.found.data(), q.found.size() * 4);
+                } else {
+                    const uint64_t cnt = q.dst.size();
+                    r.r.len = cnt;
+                    r.data.resize(cnt * 8);
+                    uint32_t* src = reinterpret_cast<uint32_t*>(r.data.data());
+                    uint64_t at = 0;
+                    for (size_t i = 0; i < q.seeds.size(); ++i)
+                        for (; at < q.ends[i]; ++at) src[at] = q.seeds[i];
+                    if (cnt) std::memcpy(r.data.data() + cnt * 4, q.dst.data(), cnt * 4);
+                }
+            }
+        } else if (cross) {
             std::vector<ursa_hip_topo*> ts(k);
             std::vector<PprQuery> qs(k);
             std::vector<double*> outs(k);
@@ -475,6 +530,7 @@ int ursa_hip_server_create(const ursa_hip_server_config* cfg, ursa_hip_server** 
             }
             if (S->cfg.slots == 0) S->cfg.slots = 8;
             if (S->cfg.max_batch == 0) S->cfg.max_batch = 16;
+            if (S->cfg.cross_batch == 0) S->cfg.cross_batch = 256;
             URSA_REQUIRE(S->cfg.slots <= 64, URSA_HIP_EINVAL, "at most 64 slots");
             if (!S->serial()) {
                 uint64_t ws = S->cfg.workspace_bytes;

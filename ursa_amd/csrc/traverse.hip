@@ -12,6 +12,7 @@
 // and parent[v] = frontier[key >> 32] is their serial first discoverer.
 // Outputs are therefore byte-identical to the reference's.
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "frontier.hpp"
@@ -317,6 +318,173 @@ __global__ void k_opath(const uint32_t* __restrict__ parent, uint32_t src, uint3
     *len = l;
 }
 
+// ---- cross-graph passes: one workgroup runs one query's whole traversal ----
+//
+// For graphs of at most TRAV_MULTI_MAX_N nodes a query (k-hop from each of its
+// seeds in turn, or one shortest path) is a workgroup of WT threads, and one
+// launch runs every query of a pass. The serial order is kept without a sort
+// and without global atomics: a level's frontier edges are walked in (frontier
+// position, slot) order, WT at a time; among the edges of one batch that reach
+// a node not yet on any level, an LDS table keeps the smallest edge index per
+// node, and that edge is the node's serial first discoverer (edges of earlier
+// batches came first and have already put their nodes on the level). The
+// winners are appended in edge order by a block scan. Only this workgroup
+// touches the query's level, order and parent arrays, so its barriers order
+// every access (the level array is -1 again after each traversal).
+constexpr int WT = 1024;
+constexpr uint32_t WT_HASH = 2048;
+
+struct TravProb {
+    NbrView fwd;
+    const uint32_t* seeds;  // k-hop: its seeds; path: source, target
+    int32_t* level;         // n, -1 on entry and exit
+    uint32_t* order;        // n: discovery order of the current traversal
+    uint32_t* parent;       // n (paths)
+    uint32_t* out;          // k-hop: reached nodes of every seed in turn; path: source..target
+    uint32_t* ends;         // k-hop: out's length after each seed; path: [0] = path length (0 = none)
+    uint32_t n, nseeds, k;  // k-hop depth (paths: 0 = unbounded)
+    uint32_t path;
+};
+
+// Exclusive block scan over WT threads; *total receives the sum (all threads).
+__device__ __forceinline__ uint32_t wt_excl_scan(uint32_t x, uint32_t* s_wave, uint32_t* s_tot) {
+    const uint32_t incl = wave_incl_scan(x);
+    const int lane = lane_id(), wave = (int)(threadIdx.x >> 6);
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    if (wave == 0) {
+        const uint32_t w = lane < WT / 64 ? s_wave[lane] : 0u;
+        const uint32_t wi = wave_incl_scan(w);
+        if (lane < WT / 64) s_wave[lane] = wi - w;
+        if (lane == WT / 64 - 1) *s_tot = wi;
+    }
+    __syncthreads();
+    return s_wave[wave] + incl - x;
+}
+
+__global__ __launch_bounds__(WT) void k_otrav_wg(const TravProb* __restrict__ probs) {
+    __shared__ uint32_t s_pre[WT], s_u[WT], s_da[WT];
+    __shared__ uint32_t s_hkey[WT_HASH], s_hmin[WT_HASH];
+    __shared__ uint32_t s_wave[WT / 64];
+    __shared__ uint32_t s_tot, s_hit;
+    const TravProb Q = probs[blockIdx.x];
+    const uint32_t tid = threadIdx.x, n = Q.n;
+    const NbrView fwd = Q.fwd;
+    const uint32_t runs = Q.path ? 1u : Q.nseeds;
+    const uint32_t target = Q.path ? Q.seeds[1] : 0xFFFFFFFFu;
+    uint32_t outpos = 0;
+    for (uint32_t si = 0; si < runs; ++si) {
+        const uint32_t seed = Q.seeds[si];
+        if (seed >= n || (Q.path && target >= n) || (!Q.path && Q.k == 0)) {  // hop.rs:53-56, 91-93; bfs.rs:55-57
+            if (tid == 0) Q.ends[si] = outpos;
+            continue;
+        }
+        if (tid == 0) {
+            Q.level[seed] = 0;
+            Q.order[0] = seed;
+            if (Q.path) Q.parent[seed] = seed;
+            s_hit = 0u;
+        }
+        __syncthreads();
+        uint32_t fs = 0, flen = 1, filled = 1;
+        bool hit = false;
+        for (int32_t L = 1; flen > 0 && (Q.k == 0 || (uint32_t)L <= Q.k) && !hit; ++L) {
+            uint32_t added = 0;
+            for (uint32_t c0 = 0; c0 < flen && !hit; c0 += WT) {
+                uint32_t u = 0, d = 0, da = 0;
+                if (c0 + tid < flen) {
+                    u = Q.order[fs + c0 + tid];
+                    d = nv_degree(fwd, u, da);
+                }
+                const uint32_t pre = wt_excl_scan(d, s_wave, &s_tot);
+                const uint32_t E = s_tot;
+                s_pre[tid] = pre;
+                s_u[tid] = u;
+                s_da[tid] = da;
+                __syncthreads();
+                for (uint32_t e0 = 0; e0 < E && !hit; e0 += WT) {
+                    for (uint32_t h = tid; h < WT_HASH; h += WT) {
+                        s_hkey[h] = 0xFFFFFFFFu;
+                        s_hmin[h] = 0xFFFFFFFFu;
+                    }
+                    __syncthreads();
+                    const uint32_t e = e0 + tid;
+                    uint32_t v = 0, owner = 0;
+                    int slot = -1;
+                    if (e < E) {
+                        uint32_t lo = 0, hi = WT;  // largest owner with s_pre[owner] <= e
+                        while (hi - lo > 1) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (s_pre[mid] <= e)
+                                lo = mid;
+                            else
+                                hi = mid;
+                        }
+                        owner = lo;
+                        const uint32_t j = e - s_pre[owner], uu = s_u[owner], a = s_da[owner];
+                        v = j < a ? fwd.tgt_a[fwd.off_a[uu] + j] : fwd.tgt_b[fwd.off_b[uu] + (j - a)];
+                        if (Q.level[v] < 0) {
+                            uint32_t h = (v * 0x9E3779B1u) >> 21;  // 11 bits: WT_HASH slots
+                            for (uint32_t step = 0; step < WT_HASH; ++step, h = (h + 1) & (WT_HASH - 1)) {
+                                const uint32_t k = atomicCAS(&s_hkey[h], 0xFFFFFFFFu, v);
+                                if (k == 0xFFFFFFFFu || k == v) {
+                                    atomicMin(&s_hmin[h], e);
+                                    slot = (int)h;
+                                    break;
+                                }
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    const bool win = slot >= 0 && s_hmin[slot] == e;
+                    const uint32_t pos = wt_excl_scan(win ? 1u : 0u, s_wave, &s_tot);
+                    const uint32_t cnt = s_tot;
+                    if (win) {
+                        Q.order[filled + pos] = v;
+                        Q.level[v] = L;
+                        if (Q.path) {
+                            Q.parent[v] = s_u[owner];
+                            if (v == target) s_hit = 1u;
+                        }
+                    }
+                    filled += cnt;
+                    added += cnt;
+                    __syncthreads();
+                    hit = s_hit != 0u;
+                }
+            }
+            fs += flen;
+            flen = added;
+        }
+        if (Q.path) {
+            if (tid == 0) {
+                uint32_t len = 0;
+                if (hit) {
+                    uint32_t cur = target;
+                    len = 1;
+                    while (cur != seed && len < n) {
+                        cur = Q.parent[cur];
+                        ++len;
+                    }
+                    cur = target;
+                    for (uint32_t i = len; i-- > 0;) {
+                        Q.out[i] = cur;
+                        cur = Q.parent[cur];
+                    }
+                }
+                Q.ends[0] = len;
+            }
+        } else {
+            for (uint32_t i = 1 + tid; i < filled; i += WT) Q.out[outpos + i - 1] = Q.order[i];
+            outpos += filled - 1;
+            if (tid == 0) Q.ends[si] = outpos;
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < filled; i += WT) Q.level[Q.order[i]] = -1;
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 uint64_t traverse_workspace_bytes(uint64_t n) {
@@ -407,6 +575,122 @@ void shortest_paths_seq(ursa_hip_topo* t, const uint32_t* src, const uint32_t* t
         reset_touched(w, visited, s);
         HIP_OK(hipStreamSynchronize(s));
         if (hit) std::reverse(paths[i].begin(), paths[i].end());
+    }
+}
+
+namespace {
+
+uint64_t al256(uint64_t b) { return (b + 255) & ~(uint64_t)255; }
+
+// Device bytes of one task's arrays (level, order, parent, out, ends).
+uint64_t trav_task_bytes(const TravTask& q) {
+    const uint64_t n = q.t->n;
+    const uint64_t outs = q.path ? n : (uint64_t)q.seeds.size() * n;
+    return al256(n * 4) * (q.path ? 3 : 2) + al256(outs * 4) + al256(std::max<size_t>(q.seeds.size(), 1) * 4);
+}
+
+}  // namespace
+
+bool traverse_multi_eligible(const ursa_hip_topo* t) { return t->n > 0 && t->n <= TRAV_MULTI_MAX_N; }
+
+uint64_t traverse_multi_workspace_bytes(const std::vector<const TravTask*>& tasks) {
+    uint64_t b = 0;
+    for (const TravTask* q : tasks) b += trav_task_bytes(*q) + al256(q->seeds.size() * 4);
+    return b + al256(tasks.size() * sizeof(TravProb)) + (64u << 10);
+}
+
+void traverse_multi(const std::vector<TravTask*>& tasks) {
+    hipStream_t s = stream();
+    std::vector<TravTask*> run;
+    for (TravTask* q : tasks) {
+        q->dst.clear();
+        q->ends.assign(q->path ? 0 : q->seeds.size(), 0);
+        q->found.clear();
+        const uint64_t n = q->t->n;
+        URSA_REQUIRE(traverse_multi_eligible(q->t), URSA_HIP_EINVAL, "graph not eligible for a cross-graph pass");
+        if (q->path) {
+            URSA_REQUIRE(q->seeds.size() == 2, URSA_HIP_EINVAL, "a path task takes a source and a target");
+            if (q->seeds[0] >= n || q->seeds[1] >= n) continue;  // bfs.rs:55-57
+            if (q->seeds[0] == q->seeds[1]) {                    // bfs.rs:58-60
+                q->found.push_back(q->seeds[0]);
+                continue;
+            }
+        } else if (q->k == 0 || q->seeds.empty()) {  // hop.rs:53-56
+            continue;
+        }
+        run.push_back(q);
+    }
+    if (run.empty()) return;
+    const uint32_t np = (uint32_t)run.size();
+    // One state allocation: every task's level array first (one fill to -1),
+    // then ends and outs of every task (one copy back), then order and parent.
+    uint64_t lv_bytes = 0, eo_bytes = 0, rest = 0;
+    for (TravTask* q : run) {
+        const uint64_t n = q->t->n;
+        lv_bytes += al256(n * 4);
+        eo_bytes += al256(std::max<size_t>(q->seeds.size(), 1) * 4) + al256((q->path ? n : q->seeds.size() * n) * 4);
+        rest += al256(n * 4) * (q->path ? 2 : 1);
+    }
+    DevBuf<char> state(lv_bytes + eo_bytes + rest);
+    std::vector<TravProb> tab(np);
+    std::vector<uint32_t> seeds;
+    std::vector<uint64_t> ends_at(np), out_at(np), seed_at(np);
+    uint64_t lv = 0, eo = lv_bytes, rs = lv_bytes + eo_bytes;
+    for (uint32_t i = 0; i < np; ++i) {
+        TravTask* q = run[i];
+        const uint64_t n = q->t->n;
+        NbrView fwd, rev;
+        direction_views(q->t, q->dir, s, fwd, rev);
+        TravProb& P = tab[i];
+        P.fwd = fwd;
+        P.n = (uint32_t)n;
+        P.nseeds = (uint32_t)q->seeds.size();
+        P.k = q->path ? 0u : q->k;
+        P.path = q->path ? 1u : 0u;
+        P.level = reinterpret_cast<int32_t*>(state.get() + lv);
+        lv += al256(n * 4);
+        ends_at[i] = eo;
+        P.ends = reinterpret_cast<uint32_t*>(state.get() + eo);
+        eo += al256(std::max<size_t>(q->seeds.size(), 1) * 4);
+        out_at[i] = eo;
+        P.out = reinterpret_cast<uint32_t*>(state.get() + eo);
+        eo += al256((q->path ? n : q->seeds.size() * n) * 4);
+        P.order = reinterpret_cast<uint32_t*>(state.get() + rs);
+        rs += al256(n * 4);
+        P.parent = nullptr;
+        if (q->path) {
+            P.parent = reinterpret_cast<uint32_t*>(state.get() + rs);
+            rs += al256(n * 4);
+        }
+        seed_at[i] = seeds.size();
+        seeds.insert(seeds.end(), q->seeds.begin(), q->seeds.end());
+    }
+    const uint64_t tab_bytes = al256(np * sizeof(TravProb));
+    DevBuf<char> blob(tab_bytes + seeds.size() * 4);
+    const uint32_t* d_seeds = reinterpret_cast<const uint32_t*>(blob.get() + tab_bytes);
+    for (uint32_t i = 0; i < np; ++i) tab[i].seeds = d_seeds + seed_at[i];
+    std::vector<char> img(tab_bytes + seeds.size() * 4);
+    std::memcpy(img.data(), tab.data(), np * sizeof(TravProb));
+    std::memcpy(img.data() + tab_bytes, seeds.data(), seeds.size() * 4);
+    HIP_OK(hipMemcpyAsync(blob.get(), img.data(), img.size(), hipMemcpyHostToDevice, s));
+    k_fill_i32b<<<blocks_for(lv_bytes / 4, 256), 256, 0, s>>>(reinterpret_cast<int32_t*>(state.get()), lv_bytes / 4, -1);
+    check_launch();
+    k_otrav_wg<<<np, WT, 0, s>>>(reinterpret_cast<const TravProb*>(blob.get()));
+    check_launch();
+    char* h = static_cast<char*>(pinned_scratch(eo_bytes));
+    HIP_OK(hipMemcpyAsync(h, state.get() + lv_bytes, eo_bytes, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    for (uint32_t i = 0; i < np; ++i) {
+        TravTask* q = run[i];
+        const uint32_t* ends = reinterpret_cast<const uint32_t*>(h + (ends_at[i] - lv_bytes));
+        const uint32_t* out = reinterpret_cast<const uint32_t*>(h + (out_at[i] - lv_bytes));
+        if (q->path) {
+            q->found.assign(out, out + ends[0]);
+        } else {
+            const uint32_t total = ends[q->seeds.size() - 1];
+            q->dst.assign(out, out + total);
+            for (size_t j = 0; j < q->seeds.size(); ++j) q->ends[j] = ends[j];
+        }
     }
 }
 

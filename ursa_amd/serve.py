@@ -64,6 +64,8 @@ class Config(C.Structure):
         ("max_batch", C.c_uint32),
         ("serial", C.c_int32),
         ("keep_results", C.c_int32),
+        ("cross_batch", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 
@@ -105,9 +107,10 @@ def digest(data) -> int:
 
 
 class Server:
-    def __init__(self, slots=8, max_batch=16, workspace_bytes=0, graph_bytes=0, serial=False, keep_results=False):
+    def __init__(self, slots=8, max_batch=16, workspace_bytes=0, graph_bytes=0, serial=False, keep_results=False,
+                 cross_batch=0):
         cfg = Config(workspace_bytes=workspace_bytes, graph_bytes=graph_bytes, slots=slots, max_batch=max_batch,
-                     serial=int(serial), keep_results=int(keep_results))
+                     serial=int(serial), keep_results=int(keep_results), cross_batch=cross_batch)
         h = C.c_void_p()
         check(load().ursa_hip_server_create(C.byref(cfg), C.byref(h)))
         self._h = h
