@@ -42,7 +42,8 @@ int ursa_hip_bfs_ex(ursa_hip_topo* t, const uint32_t* sources, uint32_t k, int d
  * push rule (divisor in {5, 20, 80}) on two 64-source batches of this graph
  * (sources in a seeded random order over the vertices with forward edges), and
  * keeps the fastest for `direction` on the topology; a candidate displaces the
- * defaults (15, 18; 20) only when more than 5 % faster. The choice is also
+ * defaults (15, 18; 20) only when more than 5 % faster on each half of the
+ * sample (4 traversals, one batch) separately. The choice is also
  * cached under a key of the forward degree distribution (log2 n, mean degree,
  * max/mean degree, share of zero-degree vertices); a later graph with the same
  * key takes it without timing unless `force`. *source: 1 measured, 2 from the

@@ -518,8 +518,9 @@ void bfs_levels(ursa_hip_topo* t, const uint32_t* sources, uint64_t k, int dir, 
 }
 
 // Empirical schedule search on this graph: the direction-switch pair (alpha,
-// beta) over a 3 x 3 grid, timed on 8 single-source traversals, and the
-// multi-source push rule over 3 divisors, timed on two 64-source batches. The
+// beta) over a 3 x 3 grid, timed on two sets of 4 single-source traversals,
+// and the multi-source push rule over 3 divisors, timed on two 64-source
+// batches. The
 // winner is kept on the topology and cached for its degree-distribution key,
 // so a graph with the same key reuses it without timing.
 constexpr uint64_t kTuneSeed = 0x74756E65ull;  // sources the autotuner times on
@@ -576,19 +577,25 @@ TuneParams autotune(ursa_hip_topo* t, int dir, bool force) {
         return ms;
     };
     // A candidate replaces the defaults (first in each list) only when it is
-    // more than 5 % faster, so timing noise leaves them in place.
+    // more than 5 % faster on each half of the sample separately, so that a
+    // choice which helps one set of sources and hurts another stays out.
     const double alphas[3] = {15.0, 4.0, 60.0}, betas[3] = {18.0, 6.0, 54.0};
-    const size_t ns = std::min<size_t>(8, cand.size());
-    float best_ms = 1e30f;
+    const size_t ns = std::min<size_t>(8, cand.size()), nh = (ns + 1) / 2;
+    float best_a = 1e30f, best_b = 1e30f;
     bfs_one(t, cand[0], fwd, rev, fwd_edges, lv.get(), w, s, nullptr, 15.0, 18.0);  // warm-up
     for (double al : alphas)
         for (double be : betas) {
-            const float ms = time_it([&] {
-                for (size_t i = 0; i < ns; ++i)
+            const float ta = time_it([&] {
+                for (size_t i = 0; i < nh; ++i)
                     bfs_one(t, cand[i], fwd, rev, fwd_edges, lv.get() + i * n, w, s, nullptr, al, be);
             });
-            if (ms < 0.95f * best_ms || best_ms == 1e30f) {
-                best_ms = ms;
+            const float tb = time_it([&] {
+                for (size_t i = nh; i < ns; ++i)
+                    bfs_one(t, cand[i], fwd, rev, fwd_edges, lv.get() + i * n, w, s, nullptr, al, be);
+            });
+            if (best_a == 1e30f || (ta < 0.95f * best_a && tb <= 0.95f * best_b)) {
+                best_a = ta;
+                best_b = tb;
                 best.bfs_alpha = al;
                 best.bfs_beta = be;
             }
@@ -597,15 +604,14 @@ TuneParams autotune(ursa_hip_topo* t, int dir, bool force) {
         const uint32_t divs[3] = {20, 5, 80};
         const std::vector<uint32_t> b0(cand.begin(), cand.begin() + std::min<size_t>(64, cand.size()));
         const std::vector<uint32_t> b1(cand.begin() + b0.size(), cand.end());
-        float best_ms2 = 1e30f;
+        float best0 = 1e30f, best1 = 1e30f;
         msbfs_levels(t, b0, dir, lv.get(), nullptr, 20);  // warm-up
         for (uint32_t dv : divs) {
-            const float ms = time_it([&] {
-                msbfs_levels(t, b0, dir, lv.get(), nullptr, dv);
-                if (!b1.empty()) msbfs_levels(t, b1, dir, lv.get(), nullptr, dv);
-            });
-            if (ms < 0.95f * best_ms2 || best_ms2 == 1e30f) {
-                best_ms2 = ms;
+            const float t0 = time_it([&] { msbfs_levels(t, b0, dir, lv.get(), nullptr, dv); });
+            const float t1 = b1.empty() ? 0.f : time_it([&] { msbfs_levels(t, b1, dir, lv.get(), nullptr, dv); });
+            if (best0 == 1e30f || (t0 < 0.95f * best0 && t1 <= 0.95f * best1)) {
+                best0 = t0;
+                best1 = t1;
                 best.ms_push_div = dv;
             }
         }
