@@ -272,9 +272,18 @@ __device__ __forceinline__ void ms_pull_heavy(const uint32_t* __restrict__ heavy
         const uint32_t e0 = ((uint32_t)c - cbase[h]) * PULL_CHUNK;
         const uint32_t e1 = min(d, e0 + PULL_CHUNK);
         unsigned long long acc = 0;
-        for (uint32_t e = e0 + lane_id(); e < e1; e += 64) {
-            const uint32_t w = e < da ? rev.tgt_a[rev.off_a[v] + e] : rev.tgt_b[rev.off_b[v] + (e - da)];
-            acc |= front[w];
+        const uint32_t sa = rev.off_a[v], sb = rev.off_b ? rev.off_b[v] : 0u;
+        // Four 64-slot steps in flight: their slots, then their masks.
+        for (uint32_t eb = e0; eb < e1; eb += 256) {
+            uint32_t w[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t e = eb + 64u * r + lane_id();
+                w[r] = e < e1 ? (e < da ? rev.tgt_a[sa + e] : rev.tgt_b[sb + (e - da)]) : 0xFFFFFFFFu;
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (w[r] != 0xFFFFFFFFu) acc |= front[w[r]];
         }
         for (int off = 32; off > 0; off >>= 1) acc |= __shfl_down(acc, off, 64);
         acc &= want;
