@@ -230,7 +230,13 @@ __global__ void k_scale(double* p, uint64_t n, double s) {
 // wavefront. State for (vertex v, source b) lives at v * 64 + b, so one edge
 // scan of v serves every source whose frontier holds v, and the per-source
 // reads and atomics for a neighbour w are one contiguous 256 / 512 B access.
-constexpr uint32_t VM_HEAVY = 2048;  // vertices with more out-edges are split into chunks of this size
+constexpr uint32_t VM_HEAVY = 256;  // backward: vertices with more out-edges are split into chunks of this size
+// Forward: vertices with more out-edges are split into chunks of this size. A
+// wave walks its vertex's (or chunk's) edges four at a time for all 64
+// sources, so a long list is a long serial chain that ends the level last;
+// the forward sums are exact integers, so the split changes no result (the
+// backward chunking fixes the f64 order of its partial sums and stays).
+constexpr uint32_t VM_FHEAVY = 64;
 
 struct VmCtr {
     uint32_t len;
@@ -339,37 +345,44 @@ __device__ __forceinline__ void vm_expand(uint32_t v, uint32_t e0, uint32_t e1, 
     if (count_edges && b == 0) edges += (unsigned long long)deg * __popcll(fmask);
     const double sv = fl ? sigma[(uint64_t)v * 64 + b] : 0.0;
     for (uint32_t e0q = e0; e0q < e1; e0q += 4) {
-        // four slots' targets and current levels read together, then taken in slot order
+        // Four slots at a time, in phases so that their atomics overlap:
+        // targets and current levels, then the distance claims, then the
+        // path-count adds, then the queue marks. (Path counts are integers, exact
+        // below 2^53 whatever the order of the adds, and the queue order never
+        // reaches a result.)
         uint32_t wq[4];
         int32_t dq[4];
+        bool claimed[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) wq[j] = e0q + j < e1 ? tgt[e0q + j] : 0u;
 #pragma unroll
         for (int j = 0; j < 4; ++j) dq[j] = (fl && e0q + j < e1) ? dist[(uint64_t)wq[j] * 64 + b] : 0;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            if (e0q + j >= e1) break;  // wave-uniform
-            const uint32_t w = wq[j];
-            bool claimed = false;
-            if (fl) {
-                int32_t* dp = &dist[(uint64_t)w * 64 + b];
-                int32_t dw = dq[j];
-                if (dw < 0) {  // a repeated target within the four sees the claim through the CAS
-                    const int32_t old = atomicCAS(dp, -1, L);
-                    claimed = old == -1;
-                    dw = claimed ? L : old;
-                }
-                if (dw == L) big |= sigma_add(&sigma[(uint64_t)w * 64 + b], sv);
+            claimed[j] = false;
+            if (fl && e0q + j < e1 && dq[j] < 0) {  // a repeated target sees the claim through the CAS
+                const int32_t old = atomicCAS(&dist[(uint64_t)wq[j] * 64 + b], -1, L);
+                claimed[j] = old == -1;
+                dq[j] = claimed[j] ? L : old;
             }
-            const uint64_t cm = __ballot(claimed);
-            if (cm) {
-                bool fresh = false;
-                if (b == __ffsll((unsigned long long)cm) - 1)
-                    fresh = atomicMax(&mark[w], (uint32_t)(L + 1)) < (uint32_t)(L + 1);
-                if (__ballot(fresh)) {  // w is the same on every lane
-                    if ((uint32_t)b == ap.cnt) ap.buf = w;
-                    if (++ap.cnt == 64) vm_append_flush(ap, nq, ctr);
-                }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (fl && e0q + j < e1 && dq[j] == L) big |= sigma_add(&sigma[(uint64_t)wq[j] * 64 + b], sv);
+        uint64_t cm[4];
+        bool fresh[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            cm[j] = __ballot(claimed[j]);
+            fresh[j] = false;
+            if (cm[j] && b == __ffsll((unsigned long long)cm[j]) - 1)
+                fresh[j] = atomicMax(&mark[wq[j]], (uint32_t)(L + 1)) < (uint32_t)(L + 1);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (cm[j] && __ballot(fresh[j])) {  // wq[j] is the same on every lane
+                if ((uint32_t)b == ap.cnt) ap.buf = wq[j];
+                if (++ap.cnt == 64) vm_append_flush(ap, nq, ctr);
             }
         }
     }
@@ -392,7 +405,7 @@ __global__ __launch_bounds__(BB) void k_vm_forward(uint32_t* __restrict__ order,
     for (uint64_t i = gw; i < qlen; i += nwaves) {
         const uint32_t v = q[i];
         const uint32_t a = off[v], z = off[v + 1];
-        if (z - a > VM_HEAVY) continue;  // chunked by k_vm_forward_heavy
+        if (z - a > VM_FHEAVY) continue;  // chunked by k_vm_forward_heavy
         vm_expand(v, a, z, tgt, dist, sigma, L, mark, nq, ctr, true, z - a, edges, ap, big);
     }
     vm_append_flush(ap, nq, ctr);
@@ -400,8 +413,9 @@ __global__ __launch_bounds__(BB) void k_vm_forward(uint32_t* __restrict__ order,
     block_add<BB>(edges, &ctr->edges);
 }
 
-// Heavy vertices, one wave per chunk of VM_HEAVY edges (chunk c of heavy
-// vertex h lies in [cbase[h], cbase[h+1])).
+// Heavy vertices, one wave per chunk of VM_FHEAVY edges (forward) or
+// VM_HEAVY edges (backward); chunk c of heavy vertex h lies in
+// [cbase[h], cbase[h+1]).
 __device__ __forceinline__ uint32_t vm_chunk_owner(const uint32_t* cbase, uint32_t nh, uint32_t c) {
     uint32_t lo = 0, hi = nh;  // largest h with cbase[h] <= c
     while (hi - lo > 1) {
@@ -432,8 +446,8 @@ __global__ __launch_bounds__(BB) void k_vm_forward_heavy(const uint32_t* __restr
         const uint32_t h = vm_chunk_owner(cbase, nh, (uint32_t)c);
         const uint32_t v = heavy[h];
         const uint32_t a = off[v], z = off[v + 1];
-        const uint32_t e0 = a + ((uint32_t)c - cbase[h]) * VM_HEAVY;
-        const uint32_t e1 = min(z, e0 + VM_HEAVY);
+        const uint32_t e0 = a + ((uint32_t)c - cbase[h]) * VM_FHEAVY;
+        const uint32_t e1 = min(z, e0 + VM_FHEAVY);
         vm_expand(v, e0, e1, tgt, dist, sigma, L, mark, nq, ctr, c == cbase[h], z - a, edges, ap, big);
     }
     vm_append_flush(ap, nq, ctr);
@@ -566,13 +580,13 @@ __global__ void k_vm_accumulate(const int32_t* __restrict__ dist, const double* 
     }
 }
 
-__global__ void k_vm_heavy_flags(const uint32_t* __restrict__ off, uint64_t n, uint8_t* flag, uint32_t* iota,
-                                 uint32_t* chunks) {
+__global__ void k_vm_heavy_flags(const uint32_t* __restrict__ off, uint64_t n, uint32_t thr, uint8_t* flag,
+                                 uint32_t* iota, uint32_t* chunks) {
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t d = off[v + 1] - off[v];
-        flag[v] = d > VM_HEAVY;
+        flag[v] = d > thr;
         iota[v] = (uint32_t)v;
-        chunks[v] = (d + VM_HEAVY - 1) / VM_HEAVY;
+        chunks[v] = (d + thr - 1) / thr;
     }
 }
 
@@ -590,29 +604,42 @@ static void betweenness_vm(ursa_hip_topo* t, const std::vector<uint32_t>& src, d
                            uint64_t* reached_edges) {
     const uint64_t n = t->n;
     const DevCsr& o = t->out;
-    // Heavy vertices and their chunk bases (static for the topology).
-    DevBuf<uint8_t> flag(n);
-    DevBuf<uint32_t> iota(n), chunks(n), heavy(n), dcnt_buf(2);
-    DevBuf<uint64_t> dcnt(1);
-    k_vm_heavy_flags<<<blocks_for(n, 256, 1u << 16), 256, 0, s>>>(o.off.get(), n, flag.get(), iota.get(), chunks.get());
-    check_launch();
-    prim::select_flagged_u32(iota.get(), flag.get(), heavy.get(), dcnt.get(), n, s);
-    uint64_t nh64 = 0;
-    HIP_OK(hipMemcpyAsync(&nh64, dcnt.get(), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
-    const uint32_t nh = (uint32_t)nh64;
-    DevBuf<uint32_t> hch(nh ? nh : 1), cbase(nh + 1);
-    uint32_t nchunks = 0;
-    if (nh) {
-        k_vm_gather_chunks<<<blocks_for(nh, 256), 256, 0, s>>>(heavy.get(), nh, chunks.get(), hch.get());
+    // Heavy vertices and their chunk bases, for the forward (VM_FHEAVY) and the
+    // backward (VM_HEAVY) kernels.
+    struct HeavySet {
+        DevBuf<uint32_t> heavy, cbase;
+        uint32_t nh = 0, nchunks = 0;
+    };
+    auto heavy_set = [&](uint32_t thr) {
+        HeavySet hs;
+        DevBuf<uint8_t> flag(n);
+        DevBuf<uint32_t> iota(n), chunks(n);
+        DevBuf<uint64_t> dcnt(1);
+        hs.heavy.alloc(n);
+        k_vm_heavy_flags<<<blocks_for(n, 256, 1u << 16), 256, 0, s>>>(o.off.get(), n, thr, flag.get(), iota.get(),
+                                                                      chunks.get());
         check_launch();
-        prim::exclusive_sum_u32(hch.get(), cbase.get(), nh, s);
-        HIP_OK(hipMemcpyAsync(&nchunks, cbase.get() + nh, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        prim::select_flagged_u32(iota.get(), flag.get(), hs.heavy.get(), dcnt.get(), n, s);
+        uint64_t nh64 = 0;
+        HIP_OK(hipMemcpyAsync(&nh64, dcnt.get(), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
-    }
-    flag.reset();
-    iota.reset();
-    chunks.reset();
+        hs.nh = (uint32_t)nh64;
+        hs.cbase.alloc(hs.nh + 1);
+        if (hs.nh) {
+            DevBuf<uint32_t> hch(hs.nh);
+            k_vm_gather_chunks<<<blocks_for(hs.nh, 256), 256, 0, s>>>(hs.heavy.get(), hs.nh, chunks.get(), hch.get());
+            check_launch();
+            prim::exclusive_sum_u32(hch.get(), hs.cbase.get(), hs.nh, s);
+            HIP_OK(hipMemcpyAsync(&hs.nchunks, hs.cbase.get() + hs.nh, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            HIP_OK(hipStreamSynchronize(s));
+        }
+        return hs;
+    };
+    HeavySet fwd_heavy = heavy_set(VM_FHEAVY);
+    HeavySet bwd_heavy = heavy_set(VM_HEAVY);
+    DevBuf<uint32_t>& heavy = bwd_heavy.heavy;
+    DevBuf<uint32_t>& cbase = bwd_heavy.cbase;
+    const uint32_t nh = bwd_heavy.nh, nchunks = bwd_heavy.nchunks;
     DevBuf<int32_t> dist(n * 64);
     DevBuf<double> sigma(n * 64), delta(n * 64), part((uint64_t)std::max<uint32_t>(nchunks, 1) * 64);
     DevBuf<uint32_t> mark(n), order(n * 64), dsrc(64);
@@ -628,6 +655,7 @@ static void betweenness_vm(ursa_hip_topo* t, const std::vector<uint32_t>& src, d
     };
     uint64_t total_edges = 0;
     const unsigned hgrid = blocks_for(std::max<uint32_t>(nchunks, 1), BW, GRID_CAP);
+    const unsigned fhgrid = blocks_for(std::max<uint32_t>(fwd_heavy.nchunks, 1), BW, GRID_CAP);
     const unsigned fgrid = blocks_for(n, BW, GRID_CAP);
     // Forward levels in groups of 8, every launch reading its level, queue range
     // and counters from the control block: one recorded graph replayed for every
@@ -642,10 +670,11 @@ static void betweenness_vm(ursa_hip_topo* t, const std::vector<uint32_t>& src, d
             timed("brandes_forward", 0.0, s, [&] {
                 k_vm_forward<<<fgrid, BB, 0, s>>>(order.get(), ctl.get(), o.off.get(), o.tgt.get(), dist.get(),
                                                   sigma.get(), mark.get(), ctr.get());
-                if (nchunks)
-                    k_vm_forward_heavy<<<hgrid, BB, 0, s>>>(heavy.get(), cbase.get(), nh, nchunks, o.off.get(),
-                                                            o.tgt.get(), dist.get(), sigma.get(), mark.get(),
-                                                            order.get(), ctl.get(), ctr.get());
+                if (fwd_heavy.nchunks)
+                    k_vm_forward_heavy<<<fhgrid, BB, 0, s>>>(fwd_heavy.heavy.get(), fwd_heavy.cbase.get(), fwd_heavy.nh,
+                                                             fwd_heavy.nchunks, o.off.get(), o.tgt.get(), dist.get(),
+                                                             sigma.get(), mark.get(), order.get(), ctl.get(),
+                                                             ctr.get());
             });
             if (recount)
                 k_vm_sigma_pull<<<fgrid, BB, 0, s>>>(order.get(), ctl.get(), ctr.get(), in_csr->off.get(),
