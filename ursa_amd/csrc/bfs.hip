@@ -38,8 +38,11 @@ constexpr int BFS_BLOCK = 256;
 constexpr int WAVES = BFS_BLOCK / 64;
 
 struct BfsCtr {
-    uint32_t qlen;   // next-queue length (top-down) / newly visited count (bottom-up)
-    uint32_t heavy;  // frontier vertices deferred to the grid-wide expansion
+    uint32_t qlen;  // next-queue length (top-down) / newly visited count (bottom-up)
+    uint32_t pad;
+    // Frontier vertices deferred to the grid-wide expansion (high 32 bits) and
+    // the 64-slot chunks of their lists (low 32 bits), one atomic add each.
+    unsigned long long hpack;
     unsigned long long edges;  // forward degree sum of the newly visited vertices
 };
 
@@ -52,7 +55,7 @@ constexpr uint32_t TD_HEAVY = 1024;
 __device__ __forceinline__ void ctr_clear(BfsCtr* c) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         c->qlen = 0;
-        c->heavy = 0;
+        c->hpack = 0;
         c->edges = 0;
     }
 }
@@ -64,7 +67,7 @@ __global__ void k_bfs_init(int32_t* level, uint64_t n, uint32_t src, uint32_t* q
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         queue[0] = src;
         ctr->qlen = 1;
-        ctr->heavy = 0;
+        ctr->hpack = 0;
         ctr->edges = nv_degree(fwd, src);
     }
     ctr_clear(next_ctr);
@@ -85,7 +88,8 @@ __device__ __forceinline__ void td_visit(bool active, uint32_t v, const NbrView&
 __global__ __launch_bounds__(BFS_BLOCK) void k_bfs_td(const uint32_t* __restrict__ q, uint32_t qlen, NbrView fwd,
                                                       int32_t* __restrict__ level, int32_t L,
                                                       uint32_t* __restrict__ nq, BfsCtr* ctr,
-                                                      uint32_t* __restrict__ heavy, BfsCtr* next_ctr) {
+                                                      uint32_t* __restrict__ heavy, uint32_t* __restrict__ hbase,
+                                                      BfsCtr* next_ctr) {
     ctr_clear(next_ctr);
     const int lane = lane_id();
     const uint64_t gw = blockIdx.x * (uint64_t)WAVES + (threadIdx.x >> 6);
@@ -100,7 +104,9 @@ __global__ __launch_bounds__(BFS_BLOCK) void k_bfs_td(const uint32_t* __restrict
             sa = fwd.off_a[u];
             if (fwd.off_b) sb = fwd.off_b[u];
             if (d > TD_HEAVY) {
-                heavy[atomicAdd(&ctr->heavy, 1u)] = u;
+                const unsigned long long old = atomicAdd(&ctr->hpack, (1ull << 32) | (unsigned long long)((d + 63) / 64));
+                heavy[old >> 32] = u;
+                hbase[old >> 32] = (uint32_t)old;
                 d = 0;
             }
         }
@@ -123,30 +129,54 @@ __global__ __launch_bounds__(BFS_BLOCK) void k_bfs_td(const uint32_t* __restrict
     if (lane == 0 && edges) atomicAdd(&ctr->edges, edges);
 }
 
-// Heavy frontier vertices: every wave of the grid takes a stride of each
-// heavy vertex's edge list, so a hub's edges spread over the whole device.
-__global__ __launch_bounds__(BFS_BLOCK) void k_bfs_td_heavy(const uint32_t* __restrict__ heavy, NbrView fwd,
+// Heavy frontier vertices. Their 64-slot chunks are numbered in one sequence
+// (entry h owns chunks hbase[h] .. hbase[h+1]-1, the last up to the total) and
+// every wave takes a contiguous run of it, so hub lists spread over the grid
+// and no wave walks the deferred list (which took milliseconds when a level
+// held thousands of hubs).
+__global__ __launch_bounds__(BFS_BLOCK) void k_bfs_td_heavy(const uint32_t* __restrict__ heavy,
+                                                            const uint32_t* __restrict__ hbase, NbrView fwd,
                                                             int32_t* __restrict__ level, int32_t L,
                                                             uint32_t* __restrict__ nq, BfsCtr* ctr) {
-    const uint32_t cnt = ctr->heavy;
-    const uint64_t wave_base = (blockIdx.x * (uint64_t)WAVES + (threadIdx.x >> 6)) * 64;
-    const uint64_t stride = (uint64_t)gridDim.x * BFS_BLOCK;
+    const unsigned long long hp = ctr->hpack;
+    const uint32_t cnt = (uint32_t)(hp >> 32), total = (uint32_t)hp;
+    if (cnt == 0) return;
+    const int lane = lane_id();
+    const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
+    const uint64_t gw = blockIdx.x * (uint64_t)WAVES + (threadIdx.x >> 6);
+    const uint64_t per = (total + nwaves - 1) / nwaves;
+    uint32_t c = (uint32_t)min((uint64_t)total, gw * per);
+    const uint32_t cend = (uint32_t)min((uint64_t)total, gw * per + per);
     uint64_t edges = 0;
-    for (uint32_t h = 0; h < cnt; ++h) {
-        const uint32_t u = heavy[h];
-        uint32_t da;
-        const uint32_t d = nv_degree(fwd, u, da);
-        const uint32_t sa = fwd.off_a[u];
-        const uint32_t sb = fwd.off_b ? fwd.off_b[u] : 0;
-        for (uint64_t e0 = wave_base; e0 < d; e0 += stride) {
-            const uint64_t e = e0 + lane_id();
-            uint32_t v = 0;
-            if (e < d) v = e < da ? fwd.tgt_a[sa + e] : fwd.tgt_b[sb + (e - da)];
-            td_visit(e < d, v, fwd, level, L, nq, ctr, edges);
+    if (c < cend) {
+        uint32_t lo = 0, hi = cnt;  // largest h with hbase[h] <= c
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (hbase[mid] <= c)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        for (uint32_t h = lo; c < cend; ++h) {
+            const uint32_t u = heavy[h], hb = hbase[h];
+            const uint32_t hn = h + 1 < cnt ? hbase[h + 1] : total;
+            uint32_t da;
+            const uint32_t d = nv_degree(fwd, u, da);
+            const uint32_t sa = fwd.off_a[u];
+            const uint32_t sb = fwd.off_b ? fwd.off_b[u] : 0;
+            const uint32_t c1 = min(cend, hn);
+            const uint32_t e1 = min(d, (c1 - hb) * 64u);
+            for (uint32_t e0 = (c - hb) * 64u; e0 < e1; e0 += 64) {
+                const uint32_t e = e0 + lane;
+                uint32_t v = 0;
+                if (e < e1) v = e < da ? fwd.tgt_a[sa + e] : fwd.tgt_b[sb + (e - da)];
+                td_visit(e < e1, v, fwd, level, L, nq, ctr, edges);
+            }
+            c = c1;
         }
     }
     edges = wave_sum_u64(edges);
-    if (lane_id() == 0 && edges) atomicAdd(&ctr->edges, edges);
+    if (lane == 0 && edges) atomicAdd(&ctr->edges, edges);
 }
 
 // Bottom-up step: vertices of level L are those unvisited with a reverse
@@ -342,11 +372,12 @@ void direction_views(ursa_hip_topo* t, int dir, hipStream_t s, NbrView& fwd, Nbr
 }
 
 struct BfsWork {
-    DevBuf<uint32_t> q0, q1, b0, b1, heavy;
+    DevBuf<uint32_t> q0, q1, b0, b1, heavy, hbase;
     DevBuf<BfsCtr> ctr;  // two slots, used by alternate steps
     BfsCtr* h = nullptr;  // pinned mirror of the current slot
     explicit BfsWork(uint64_t n)
-        : q0(n ? n : 1), q1(n ? n : 1), b0(((n + 63) / 64) * 2), b1(((n + 63) / 64) * 2), heavy(n ? n : 1), ctr(2) {
+        : q0(n ? n : 1), q1(n ? n : 1), b0(((n + 63) / 64) * 2), b1(((n + 63) / 64) * 2), heavy(n ? n : 1),
+          hbase(n ? n : 1), ctr(2) {
         HIP_OK(hipHostMalloc((void**)&h, sizeof(BfsCtr), hipHostMallocDefault));
     }
     ~BfsWork() {
@@ -413,10 +444,11 @@ static void bfs_one(ursa_hip_topo* t, uint32_t src, const NbrView& fwd, const Nb
             edges_to_check -= (int64_t)scout;
             const unsigned tb = blocks_for((qlen + 63) / 64, WAVES, 1u << 16);
             timed("bfs_top_down", 4.0 * (double)scout + 8.0 * (double)qlen, s, [&] {
-                k_bfs_td<<<tb, BFS_BLOCK, 0, s>>>(q, (uint32_t)qlen, fwd, level, L, nq, cur(), w.heavy.get(), nxt());
+                k_bfs_td<<<tb, BFS_BLOCK, 0, s>>>(q, (uint32_t)qlen, fwd, level, L, nq, cur(), w.heavy.get(),
+                                                  w.hbase.get(), nxt());
                 if (scout > TD_HEAVY) {
                     const unsigned hb = blocks_for(scout, BFS_BLOCK, 2048);
-                    k_bfs_td_heavy<<<hb, BFS_BLOCK, 0, s>>>(w.heavy.get(), fwd, level, L, nq, cur());
+                    k_bfs_td_heavy<<<hb, BFS_BLOCK, 0, s>>>(w.heavy.get(), w.hbase.get(), fwd, level, L, nq, cur());
                 }
             });
             check_launch();
