@@ -42,7 +42,10 @@ constexpr int MW = MB / 64;
 
 struct MsCtr {
     uint32_t qlen;
-    uint32_t heavy;              // push-frontier vertices deferred to the grid-wide pass
+    uint32_t pad;
+    // Push-frontier vertices deferred to the grid-wide pass (high 32 bits) and
+    // the 64-slot chunks of their lists (low 32 bits), one atomic add each.
+    unsigned long long hpack;
     unsigned long long newbits;  // (source, vertex) pairs settled this level
     unsigned long long edges;    // forward degree sum of the next frontier (push levels)
 };
@@ -333,7 +336,8 @@ __device__ __forceinline__ void ms_push(uint32_t* __restrict__ q0, uint32_t* __r
                                                 const unsigned long long* __restrict__ front,
                                                 unsigned long long* __restrict__ next,
                                                 unsigned long long* __restrict__ visit, uint64_t n, const MsCtl* ctl,
-                                                MsCtr* ctr, uint32_t* __restrict__ heavy) {
+                                                MsCtr* ctr, uint32_t* __restrict__ heavy,
+                                                uint32_t* __restrict__ hbase) {
     const uint32_t* q = ctl->qsel ? q1 : q0;
     uint32_t* nq = ctl->qsel ? q0 : q1;
     const uint64_t qlen = ctl->qlen;
@@ -354,7 +358,9 @@ __device__ __forceinline__ void ms_push(uint32_t* __restrict__ q0, uint32_t* __r
             if (fwd.off_b) sb = fwd.off_b[u];
             fm = front[u];
             if (d > MS_HEAVY) {
-                heavy[atomicAdd(&ctr->heavy, 1u)] = u;
+                const unsigned long long old = atomicAdd(&ctr->hpack, (1ull << 32) | (unsigned long long)((d + 63) / 64));
+                heavy[old >> 32] = u;
+                hbase[old >> 32] = (uint32_t)old;
                 d = 0;
             }
         }
@@ -390,46 +396,68 @@ __device__ __forceinline__ void ms_push(uint32_t* __restrict__ q0, uint32_t* __r
     }
 }
 
-// Heavy push-frontier vertices: every wave of the grid takes a stride of each
-// heavy vertex's edges.
-__device__ __forceinline__ void ms_push_heavy(const uint32_t* __restrict__ heavy, NbrView fwd,
-                                                      const unsigned long long* __restrict__ front,
-                                                      unsigned long long* __restrict__ next,
-                                                      unsigned long long* __restrict__ visit, uint64_t n,
-                                                      uint32_t* __restrict__ q0, uint32_t* __restrict__ q1,
-                                                      const MsCtl* ctl, MsCtr* ctr) {
+// Heavy push-frontier vertices. Their 64-slot chunks are numbered in one
+// sequence (entry h owns chunks hbase[h] .. hbase[h+1]-1, the last up to the
+// total) and every wave takes a contiguous run of it, so hub lists spread over
+// the grid and no wave walks the deferred list.
+__device__ __forceinline__ void ms_push_heavy(const uint32_t* __restrict__ heavy, const uint32_t* __restrict__ hbase,
+                                              NbrView fwd, const unsigned long long* __restrict__ front,
+                                              unsigned long long* __restrict__ next,
+                                              unsigned long long* __restrict__ visit, uint64_t n,
+                                              uint32_t* __restrict__ q0, uint32_t* __restrict__ q1,
+                                              const MsCtl* ctl, MsCtr* ctr) {
     uint32_t* nq = ctl->qsel ? q0 : q1;
     int32_t* levels = ctl->levels;
     const int32_t L = ctl->L;
-    const uint32_t cnt_h = ctr->heavy;
-    const uint64_t wave_base = (blockIdx.x * (uint64_t)MW + (threadIdx.x >> 6)) * 64;
-    const uint64_t stride = (uint64_t)gridDim.x * MB;
+    const unsigned long long hp = ctr->hpack;
+    const uint32_t cnt_h = (uint32_t)(hp >> 32), total = (uint32_t)hp;
+    if (cnt_h == 0) return;
+    const int lane = lane_id();
+    const uint64_t nwaves = (uint64_t)gridDim.x * MW;
+    const uint64_t gw = blockIdx.x * (uint64_t)MW + (threadIdx.x >> 6);
+    const uint64_t per = (total + nwaves - 1) / nwaves;
+    uint32_t c = (uint32_t)min((uint64_t)total, gw * per);
+    const uint32_t cend = (uint32_t)min((uint64_t)total, gw * per + per);
     unsigned long long cnt = 0, edges = 0;
-    for (uint32_t h = 0; h < cnt_h; ++h) {
-        const uint32_t u = heavy[h];
-        uint32_t da;
-        const uint32_t d = nv_degree(fwd, u, da);
-        const uint32_t sa = fwd.off_a[u];
-        const uint32_t sb = fwd.off_b ? fwd.off_b[u] : 0;
-        const unsigned long long fm = front[u];
-        for (uint64_t e0 = wave_base; e0 < d; e0 += stride) {
-            const uint64_t e = e0 + lane_id();
-            bool first = false;
-            uint32_t v = 0;
-            if (e < d) {
-                v = e < da ? fwd.tgt_a[sa + e] : fwd.tgt_b[sb + (e - da)];
-                first = ms_push_edge(v, fm, visit, next, levels, n, L, cnt);
+    if (c < cend) {
+        uint32_t lo = 0, hi = cnt_h;  // largest h with hbase[h] <= c
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (hbase[mid] <= c)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        for (uint32_t h = lo; c < cend; ++h) {
+            const uint32_t u = heavy[h], hb = hbase[h];
+            const uint32_t hn = h + 1 < cnt_h ? hbase[h + 1] : total;
+            uint32_t da;
+            const uint32_t d = nv_degree(fwd, u, da);
+            const uint32_t sa = fwd.off_a[u];
+            const uint32_t sb = fwd.off_b ? fwd.off_b[u] : 0;
+            const unsigned long long fm = front[u];
+            const uint32_t c1 = min(cend, hn);
+            const uint32_t e1 = min(d, (c1 - hb) * 64u);
+            for (uint32_t e0 = (c - hb) * 64u; e0 < e1; e0 += 64) {
+                const uint32_t e = e0 + lane;
+                bool first = false;
+                uint32_t v = 0;
+                if (e < e1) {
+                    v = e < da ? fwd.tgt_a[sa + e] : fwd.tgt_b[sb + (e - da)];
+                    first = ms_push_edge(v, fm, visit, next, levels, n, L, cnt);
+                }
+                const uint32_t slot = wave_append_slot(first, &ctr->qlen);
+                if (first) {
+                    nq[slot] = v;
+                    edges += nv_degree(fwd, v);
+                }
             }
-            const uint32_t slot = wave_append_slot(first, &ctr->qlen);
-            if (first) {
-                nq[slot] = v;
-                edges += nv_degree(fwd, v);
-            }
+            c = c1;
         }
     }
     cnt = wave_sum_u64(cnt);
     edges = wave_sum_u64(edges);
-    if (lane_id() == 0) {
+    if (lane == 0) {
         if (cnt) atomicAdd(&ctr->newbits, cnt);
         if (edges) atomicAdd(&ctr->edges, edges);
     }
@@ -441,17 +469,19 @@ __global__ __launch_bounds__(MB) void k_ms_expand(uint32_t* __restrict__ q0, uin
                                                   NbrView rev, const unsigned long long* __restrict__ front,
                                                   unsigned long long* __restrict__ next,
                                                   unsigned long long* __restrict__ visit, uint64_t n, const MsCtl* ctl,
-                                                  MsCtr* ctr, uint32_t* __restrict__ heavy) {
+                                                  MsCtr* ctr, uint32_t* __restrict__ heavy,
+                                                  uint32_t* __restrict__ hbase) {
     if (ctl->done) return;
     if (ctl->push)
-        ms_push(q0, q1, fwd, front, next, visit, n, ctl, ctr, heavy);
+        ms_push(q0, q1, fwd, front, next, visit, n, ctl, ctr, heavy, hbase);
     else
         ms_pull(n, rev, front, next, visit, ctl, ctr);
 }
 
 // The heavy vertices of the level: the push frontier's long forward lists
 // (queued by the push) or the pull side's long reverse lists, chunk by chunk.
-__global__ __launch_bounds__(MB) void k_ms_expand_heavy(const uint32_t* __restrict__ heavy, NbrView fwd, NbrView rev,
+__global__ __launch_bounds__(MB) void k_ms_expand_heavy(const uint32_t* __restrict__ heavy,
+                                                        const uint32_t* __restrict__ hbase, NbrView fwd, NbrView rev,
                                                         const unsigned long long* __restrict__ front,
                                                         unsigned long long* __restrict__ next,
                                                         unsigned long long* __restrict__ visit, uint64_t n,
@@ -462,7 +492,7 @@ __global__ __launch_bounds__(MB) void k_ms_expand_heavy(const uint32_t* __restri
                                                         const MsCtl* ctl, MsCtr* ctr) {
     if (ctl->done) return;
     if (ctl->push)
-        ms_push_heavy(heavy, fwd, front, next, visit, n, q0, q1, ctl, ctr);
+        ms_push_heavy(heavy, hbase, fwd, front, next, visit, n, q0, q1, ctl, ctr);
     else if (pnh)
         ms_pull_heavy(pheavy, pcbase, pnh, pchunks, rev, front, visit, ctl, hacc);
 }
@@ -507,7 +537,7 @@ void direction_views(ursa_hip_topo* t, int dir, hipStream_t s, NbrView& fwd, Nbr
 struct MsWork {
     uint64_t n;
     DevBuf<unsigned long long> visit, f0, f1, redges, hacc;
-    DevBuf<uint32_t> q0, q1, dsrc, heavy;
+    DevBuf<uint32_t> q0, q1, dsrc, heavy, hbase;
     const uint32_t* pheavy = nullptr;
     const uint32_t* pcbase = nullptr;
     uint32_t pnh = 0, pchunks = 0;  // pull-side heavy vertices and their chunk count
@@ -515,7 +545,8 @@ struct MsWork {
     DevBuf<MsCtl> ctl;
     CapturedLaunches group;
     explicit MsWork(uint64_t n_)
-        : n(n_), visit(n_), f0(n_), f1(n_), redges(1), q0(n_), q1(n_), dsrc(64), heavy(n_), ctr(1), ctl(1) {}
+        : n(n_), visit(n_), f0(n_), f1(n_), redges(1), q0(n_), q1(n_), dsrc(64), heavy(n_), hbase(n_), ctr(1),
+          ctl(1) {}
     MsCtl read_ctl(hipStream_t s) {
         MsCtl* h = static_cast<MsCtl*>(pinned_scratch(sizeof(MsCtl)));
         HIP_OK(hipMemcpyAsync(h, ctl.get(), sizeof(MsCtl), hipMemcpyDeviceToHost, s));
@@ -596,8 +627,9 @@ void msbfs_batch(uint64_t n, uint64_t fwd_edges, const uint32_t* src_host, uint3
             k_ms_clear_next<<<gv, MB, 0, s>>>(ctl, next, n);
             timed("msbfs_expand", 0.0, s, [&] {
                 k_ms_expand<<<GRID_CAP, MB, 0, s>>>(w.q0.get(), w.q1.get(), fwd, rev, front, next, w.visit.get(), n,
-                                                    ctl, ctr, w.heavy.get());
-                k_ms_expand_heavy<<<GRID_CAP, MB, 0, s>>>(w.heavy.get(), fwd, rev, front, next, w.visit.get(), n,
+                                                    ctl, ctr, w.heavy.get(), w.hbase.get());
+                k_ms_expand_heavy<<<GRID_CAP, MB, 0, s>>>(w.heavy.get(), w.hbase.get(), fwd, rev, front, next,
+                                                          w.visit.get(), n,
                                                           w.q0.get(), w.q1.get(), w.pheavy, w.pcbase, w.pnh,
                                                           w.pchunks, w.hacc.get(), ctl, ctr);
                 if (w.pnh)
