@@ -98,7 +98,8 @@ uint64_t topo_device_bytes(const ursa_hip_topo* t) {
     b += bytes_of(t->out_deg);
     b += bytes_of(t->pr_old_of_new) + bytes_of(t->pr_new_of_old) + bytes_of(t->pr_off) + bytes_of(t->pr_tgt) +
          bytes_of(t->pr_eid) + bytes_of(t->pr_outdeg) + bytes_of(t->pr_head) + bytes_of(t->pr_crow) + bytes_of(t->pr_cflag);
-    b += bytes_of(t->tc_off) + bytes_of(t->tc_tgt) + bytes_of(t->tc_ioff) + bytes_of(t->tc_isrc) + bytes_of(t->tc_new);
+    b += bytes_of(t->tc_off) + bytes_of(t->tc_tgt) + bytes_of(t->tc_ioff) + bytes_of(t->tc_isrc) + bytes_of(t->tc_new) +
+         bytes_of(t->tc_iwalk);
     for (const auto& kind : t->tc_items)
         for (const auto& it : kind) b += bytes_of(it.start) + bytes_of(it.end) + bytes_of(it.owner);
     b += bytes_of(t->ppr_hrow) + bytes_of(t->ppr_hsoff) + bytes_of(t->ppr_sbeg) + bytes_of(t->ppr_send);

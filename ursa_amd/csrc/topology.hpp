@@ -87,6 +87,7 @@ struct ursa_hip_topo {
     // workgroup with the mid-size hash table, [2] with the large one, [3] with
     // a bitmap table).
     ursa::DevBuf<uint32_t> tc_off, tc_tgt, tc_ioff, tc_isrc, tc_new;
+    ursa::DevBuf<uint32_t> tc_iwalk;  // per in-list slot x -> v: entries of N+(x) walked on v's side (before v), or 0
     struct TcItems {
         ursa::DevBuf<uint32_t> start, end, owner;
         uint64_t n = 0;

@@ -11,9 +11,12 @@
 // are made per new id and mapped back to the caller's ids at the end.
 // A triangle u < v < w (in rank) is found exactly once, as w in
 // N+(u) ∩ N+(v) for the pair u -> v, intersected from one side: N+(v) walked
-// against a table of N+(u), or N+(u) walked against a table of N+(v) when
-// N+(v) is much longer (tc_walk); the second kind of work runs over the
-// in-lists In+(v) of the oriented graph.
+// against a table of N+(u), or, when N+(v) is a hub's list, the entries of
+// N+(u) before v (the only ones that can close a triangle: w lies in [0, v))
+// walked against a table of N+(v) (tc_walk_out / tc_walk_in); the second kind
+// of work runs over the in-lists In+(v) of the oriented graph. At s22 the cut
+// halves that kind's walks (6.87 G to 3.44 G entries under the d+(u) < d+(v)
+// rule) and lets hub pairs take the shorter side.
 // Tables: an LDS hash set (lists of up to BIG_CAP entries), or, for an owner
 // u below BM_BITS whose list is longer than STAGE_CAP, a bitmap of N+(u) over
 // [0, u) with per-word prefix counts (a lookup is one LDS word; the member's
@@ -32,8 +35,10 @@
 // lanes with a degree prefix-sum. Integer sums are order-independent, so
 // counts are exact and identical to the node-iterator reference.
 //
-// Algorithmic bytes (the sorted-merge model): 4 * sum over oriented edges (u,v)
-// of (d+(u) + d+(v)), plus 12 B per node.
+// Algorithmic bytes: 4 B per walked entry, 12 B per oriented pair (the
+// partner id and its list offsets) and 12 B per node (the one-sided walk
+// model); the sorted-merge model, 4 * sum over oriented edges (u,v) of
+// (d+(u) + d+(v)) plus 12 B per node, is reported beside it.
 #include <atomic>
 #include <vector>
 
@@ -61,31 +66,56 @@ constexpr int TC_ILP = 4;            // 64-step groups whose loads are in flight
 constexpr uint32_t TC_LONG = 32;     // walks at least this long are taken by the whole wave
 constexpr int TC_LILP = 8;           // 64-step chunks of long walks in flight together
 
-// Each oriented edge x -> y (a pair) is intersected once: from y's side (N+(x)
-// walked against y's table, by an item over y's in-list In+(y), kind 1) when
-// N+(y) is longer than N+(x) and than TC_HUB entries, else from x's side (N+(y)
-// walked against x's table, by an item over x's out-list, kind 0). So the
-// shorter list is walked wherever the longer one is a hub's; pairs among short
-// lists stay on x's side, where each table is staged once and no partner needs
-// an atomic credit (on graphs of even, small degrees splitting them as well
-// doubled the staging and the launches). At s22, with four short-walk groups
-// in flight and items of 65536 steps, a hub threshold of 128 measured 21.8 ms,
-// 256 20.4 ms, 512 21.2 ms and 1024 33.4 ms (on the serving mix's densest
-// R-MAT graph, 1.41 M nodes and 59.7 M edges: 256 23.7 ms, 512 23.3 ms).
-// A triangle x < y < z closes at w = z in N+(x) and N+(y). Walk length of a
-// slot, with `owner` the table's vertex and `partner` the other end; a pair
-// needs |N+(x)| >= 2.
+// Each oriented edge x -> y (a pair; y in N+(x), so y < x) is intersected once.
+// A triangle z < y < x closes at z in N+(x) ∩ N+(y), and every such z lies
+// below y: in N+(x) only the entries before y can close one. So a pair whose y
+// is the first entry of N+(x) (position 0) has no triangle, and otherwise the
+// pair is walked from y's side (the first pos(y) entries of N+(x) walked
+// against y's table, by an item over y's in-list In+(y), kind 1) when N+(y) is
+// longer than TC_HUB entries and than that prefix, else from x's side (all of
+// N+(y) walked against x's table, by an item over x's out-list, kind 0). So the
+// shorter walk is taken wherever y's list is a hub's; pairs among short lists
+// stay on x's side, where each table is staged once and no partner needs an
+// atomic credit (on graphs of even, small degrees splitting them as well
+// doubled the staging and the launches). With whole out-lists walked on y's
+// side and the rule d+(x) < d+(y), a hub threshold of 128 measured 21.8 ms at
+// s22, 256 20.4 ms, 512 21.2 ms and 1024 33.4 ms.
 constexpr uint32_t TC_HUB = 256;
-__device__ __forceinline__ uint32_t tc_walk(bool in, uint32_t d_owner, uint32_t d_partner) {
-    if (in) return d_partner >= 2 && d_partner < d_owner && d_owner > TC_HUB ? d_partner : 0u;
-    return d_owner >= 2 && (d_partner <= d_owner || d_partner <= TC_HUB) ? d_partner : 0u;
+// Kind 1 when y's list is a hub's and longer than the prefix of N+(x) below y.
+__device__ __forceinline__ bool tc_from_owner(uint32_t d_y, uint32_t pos) { return d_y > TC_HUB && pos < d_y; }
+// Walk length of the pair at position pos of N+(x) on x's side (kind 0).
+__device__ __forceinline__ uint32_t tc_walk_out(uint32_t pos, uint32_t d_y) {
+    return pos >= 1 && !tc_from_owner(d_y, pos) ? d_y : 0u;
+}
+// Walk length on y's side (kind 1).
+__device__ __forceinline__ uint32_t tc_walk_in(uint32_t pos, uint32_t d_y) {
+    return pos >= 1 && tc_from_owner(d_y, pos) ? pos : 0u;
 }
 
-__global__ void k_tc_cost(const uint32_t* __restrict__ ooff, const uint32_t* __restrict__ partner,
-                          const uint32_t* __restrict__ owner, uint64_t kept, bool in, uint32_t* __restrict__ cost) {
+// Walk length of every out-list slot (kind 0).
+__global__ void k_tc_cost_out(const uint32_t* __restrict__ ooff, const uint32_t* __restrict__ otgt,
+                              const uint32_t* __restrict__ oown, uint64_t kept, uint32_t* __restrict__ cost) {
     for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < kept; k += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t o = owner[k], p = partner[k];
-        cost[k] = tc_walk(in, ooff[o + 1] - ooff[o], ooff[p + 1] - ooff[p]);
+        const uint32_t x = oown[k], y = otgt[k];
+        cost[k] = tc_walk_out((uint32_t)(k - ooff[x]), ooff[y + 1] - ooff[y]);
+    }
+}
+
+__global__ void k_tc_iota(uint64_t kept, uint32_t* __restrict__ out) {
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < kept; k += (uint64_t)gridDim.x * blockDim.x)
+        out[k] = (uint32_t)k;
+}
+
+// In-list slot i holds the out-slot islot[i] of its pair x -> y: the partner
+// x, and the walk length on y's side (kind 1), the entries of N+(x) before y.
+__global__ void k_tc_in_lists(const uint32_t* __restrict__ islot, const uint32_t* __restrict__ oown,
+                              const uint32_t* __restrict__ ooff, const uint32_t* __restrict__ otgt, uint64_t kept,
+                              uint32_t* __restrict__ isrc, uint32_t* __restrict__ iwalk) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < kept; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = islot[i];
+        const uint32_t x = oown[k], y = otgt[k];
+        isrc[i] = x;
+        iwalk[i] = tc_walk_in(k - ooff[x], ooff[y + 1] - ooff[y]);
     }
 }
 
@@ -283,8 +313,9 @@ __device__ __forceinline__ bool tc_member(bool staged, const uint32_t* key, cons
 // group of 64 slots starting at s0 + first. With `staged`, N+(u) sits in an
 // LDS open-addressing hash set whose slots also carry hit counters; otherwise
 // membership is a binary search of N+(u) in HBM. For each partner v of the
-// range whose pair falls to this side (tc_walk), the wave walks N+(v) from HBM
-// and tests each entry against N+(u). A v whose walk has at least TC_LONG steps
+// range whose pair falls to this side, the wave walks N+(v) from HBM — all of
+// it (out-lists, tc_walk_out), or the entries before u (in-lists: `iwalk`,
+// tc_walk_in) — and tests each entry against N+(u). A v whose walk has at least TC_LONG steps
 // is walked by the whole wave in 64-step chunks; the shorter walks of the group
 // are flattened over the 64 lanes with a prefix sum and an owner search.
 // A hit w closes a triangle of u, v and w: the returned count (wave-uniform) is
@@ -294,7 +325,8 @@ __device__ __forceinline__ bool tc_member(bool staged, const uint32_t* key, cons
 template <bool IN, bool BM = false>
 __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict__ ooff,
                                                       const uint32_t* __restrict__ otgt,
-                                                      const uint32_t* __restrict__ plist, uint32_t a, uint32_t du,
+                                                      const uint32_t* __restrict__ plist,
+                                                      const uint32_t* __restrict__ iwalk, uint32_t a, uint32_t du,
                                                       uint32_t s0, uint32_t s1, uint32_t first, uint32_t step,
                                                       bool staged, const uint32_t* key, uint32_t* cnt,
                                                       uint32_t hmask,
@@ -312,7 +344,7 @@ __device__ __forceinline__ unsigned long long tc_item(const uint32_t* __restrict
         if (i < s1) {
             v = plist[i];
             sv = ooff[v];
-            c = tc_walk(IN, du, ooff[v + 1] - sv);
+            c = IN ? iwalk[i] : tc_walk_out(i - a, ooff[v + 1] - sv);
             if (vtable && c) vslot = tc_find<BM>(key, pre, hmask, v);
         }
         // Long walks: cut into 64-step chunks walked by the whole wave, lane l
@@ -462,6 +494,7 @@ __device__ __forceinline__ void tc_flush(const uint32_t* key, const uint32_t* cn
 template <bool IN>
 __global__ __launch_bounds__(TCB, 8) void k_tc_count(const uint32_t* __restrict__ ooff, const uint32_t* __restrict__ otgt,
                                                   const uint32_t* __restrict__ plist,
+                                                  const uint32_t* __restrict__ iwalk,
                                                   const uint32_t* __restrict__ istart, const uint32_t* __restrict__ iend,
                                                   const uint32_t* __restrict__ iu, uint64_t nwork,
                                                   unsigned long long* __restrict__ tri) {
@@ -479,7 +512,7 @@ __global__ __launch_bounds__(TCB, 8) void k_tc_count(const uint32_t* __restrict_
         const bool staged = du <= STAGE_CAP;
         const uint32_t hmask = tc_stage<true>(otgt, a, staged ? du : 0, lane, 64, key, cnt);
         const unsigned long long cu =
-            tc_item<IN>(ooff, otgt, plist, a, du, istart[wi], iend[wi], 0, 64, staged, key, cnt, hmask, tri);
+            tc_item<IN>(ooff, otgt, plist, iwalk, a, du, istart[wi], iend[wi], 0, 64, staged, key, cnt, hmask, tri);
         if (lane == 0 && cu) atomicAdd(&tri[u], cu);
         tc_flush<true>(key, cnt, (hmask + 1) * TC_BUCKET, lane, 64, tri);
     }
@@ -493,6 +526,7 @@ template <uint32_t CAP, bool IN>
 __global__ __launch_bounds__(TCB_BIG) void k_tc_count_big(const uint32_t* __restrict__ ooff,
                                                              const uint32_t* __restrict__ otgt,
                                                              const uint32_t* __restrict__ plist,
+                                                             const uint32_t* __restrict__ iwalk,
                                                              const uint32_t* __restrict__ istart,
                                                              const uint32_t* __restrict__ iend,
                                                              const uint32_t* __restrict__ iu, uint64_t nwork,
@@ -506,7 +540,7 @@ __global__ __launch_bounds__(TCB_BIG) void k_tc_count_big(const uint32_t* __rest
         const uint32_t a = ooff[u];
         const uint32_t du = ooff[u + 1] - a;
         const uint32_t hmask = tc_stage<false>(otgt, a, du, threadIdx.x, TCB_BIG, b_key, b_cnt);
-        const unsigned long long cu = tc_item<IN>(ooff, otgt, plist, a, du, istart[wi], iend[wi], 64u * wid,
+        const unsigned long long cu = tc_item<IN>(ooff, otgt, plist, iwalk, a, du, istart[wi], iend[wi], 64u * wid,
                                                   TCB_BIG, true, b_key, b_cnt, hmask, tri);
         if (lane == 0 && cu) atomicAdd(&tri[u], cu);
         tc_flush<false>(b_key, b_cnt, (hmask + 1) * TC_BUCKET, threadIdx.x, TCB_BIG, tri);
@@ -523,6 +557,7 @@ template <bool IN>
 __global__ __launch_bounds__(TCB_BIG) void k_tc_count_bm(const uint32_t* __restrict__ ooff,
                                                         const uint32_t* __restrict__ otgt,
                                                         const uint32_t* __restrict__ plist,
+                                                        const uint32_t* __restrict__ iwalk,
                                                         const uint32_t* __restrict__ istart,
                                                         const uint32_t* __restrict__ iend,
                                                         const uint32_t* __restrict__ iu, uint64_t nwork,
@@ -570,7 +605,7 @@ __global__ __launch_bounds__(TCB_BIG) void k_tc_count_bm(const uint32_t* __restr
             }
         }
         __syncthreads();
-        const unsigned long long cu = tc_item<IN, true>(ooff, otgt, plist, a, du, istart[wi], iend[wi], 64u * wid,
+        const unsigned long long cu = tc_item<IN, true>(ooff, otgt, plist, iwalk, a, du, istart[wi], iend[wi], 64u * wid,
                                                         TCB_BIG, true, s_bm, s_cnt, nw * 32u, tri, s_pre);
         if (lane == 0 && cu) atomicAdd(&tri[u], cu);
         __syncthreads();
@@ -674,19 +709,31 @@ static void tc_orient(ursa_hip_topo* t, hipStream_t s) {
         t->tc_tgt.alloc(1);
         offsets_from_sorted(nullptr, 0, n, t->tc_off.get(), s);
     }
-    // In-lists of the DAG: the kept (owner, target) pairs regrouped by target
-    // with a stable sort, so each In+(v) lists its sources in increasing order.
+    // In-lists of the DAG: the kept out-slots regrouped by target with a
+    // stable sort, so each In+(v) lists its sources in increasing order; each
+    // in-slot keeps its source and its walk length on the target's side.
     DevBuf<uint32_t> iown;  // target of each in-list slot
     t->tc_ioff.alloc(n + 1);
     if (kept) {
         iown.alloc(kept);
         t->tc_isrc.alloc(kept);
+        t->tc_iwalk.alloc(kept);
         int bits = 1;
         while (bits < 32 && (1ull << bits) < n) ++bits;
-        prim::sort_pairs_u32(t->tc_tgt.get(), iown.get(), oown.get(), t->tc_isrc.get(), kept, bits, s);
+        {
+            DevBuf<uint32_t> slot(kept), islot(kept);
+            k_tc_iota<<<blocks_for(kept, 256), 256, 0, s>>>(kept, slot.get());
+            check_launch();
+            prim::sort_pairs_u32(t->tc_tgt.get(), iown.get(), slot.get(), islot.get(), kept, bits, s);
+            k_tc_in_lists<<<blocks_for(kept, 256), 256, 0, s>>>(islot.get(), oown.get(), t->tc_off.get(),
+                                                               t->tc_tgt.get(), kept, t->tc_isrc.get(),
+                                                               t->tc_iwalk.get());
+            check_launch();
+        }
         offsets_from_sorted(iown.get(), kept, n, t->tc_ioff.get(), s);
     } else {
         t->tc_isrc.alloc(1);
+        t->tc_iwalk.alloc(1);
         offsets_from_sorted(nullptr, 0, n, t->tc_ioff.get(), s);
     }
     // Work items (see the file comment), for out-lists ([0]) and in-lists ([1]).
@@ -694,7 +741,6 @@ static void tc_orient(ursa_hip_topo* t, hipStream_t s) {
     for (int kind = 0; kind < 2; ++kind) {
         auto& items = t->tc_items[kind];
         const uint32_t* loff = kind ? t->tc_ioff.get() : t->tc_off.get();
-        const uint32_t* partner = kind ? t->tc_isrc.get() : t->tc_tgt.get();
         const uint32_t* owner = kind ? iown.get() : oown.get();
         for (auto& it : items) {
             it.start.alloc(1);
@@ -703,22 +749,25 @@ static void tc_orient(ursa_hip_topo* t, hipStream_t s) {
             it.n = 0;
         }
         if (!kept) continue;
-        DevBuf<uint32_t> cost(kept), iota(kept);
+        DevBuf<uint32_t> cost(kind ? 1 : kept), iota(kept);
         DevBuf<uint64_t> pre(kept + 1);
         DevBuf<uint8_t> flag(kept);
-        k_tc_cost<<<blocks_for(kept, 256), 256, 0, s>>>(t->tc_off.get(), partner, owner, kept, kind == 1,
-                                                       cost.get());
-        check_launch();
-        prim::exclusive_sum_u32_to_u64(cost.get(), pre.get(), kept, s);
+        if (kind == 0) {
+            k_tc_cost_out<<<blocks_for(kept, 256), 256, 0, s>>>(t->tc_off.get(), t->tc_tgt.get(), owner, kept,
+                                                               cost.get());
+            check_launch();
+        }
+        const uint32_t* costp = kind ? t->tc_iwalk.get() : cost.get();
+        prim::exclusive_sum_u32_to_u64(costp, pre.get(), kept, s);
         {
             uint64_t last_pre = 0;
             uint32_t last_cost = 0;
             HIP_OK(hipMemcpyAsync(&last_pre, pre.get() + (kept - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-            HIP_OK(hipMemcpyAsync(&last_cost, cost.get() + (kept - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            HIP_OK(hipMemcpyAsync(&last_cost, costp + (kept - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
             HIP_OK(hipStreamSynchronize(s));
             walk_steps += last_pre + last_cost;
         }
-        k_tc_item_flags<<<blocks_for(kept, 256), 256, 0, s>>>(loff, t->tc_off.get(), owner, cost.get(), pre.get(),
+        k_tc_item_flags<<<blocks_for(kept, 256), 256, 0, s>>>(loff, t->tc_off.get(), owner, costp, pre.get(),
                                                              kept, flag.get(), iota.get());
         check_launch();
         DevBuf<uint32_t> istart(kept), iu(kept), iend(kept);
@@ -798,35 +847,36 @@ void triangle_count(ursa_hip_topo* t, uint64_t* out_user) {
                 const auto& it = t->tc_items[kind];
                 const hipStream_t ks = kind ? s1 : s;
                 const uint32_t* plist = kind ? t->tc_isrc.get() : otgt;
+                const uint32_t* iwalk = kind ? t->tc_iwalk.get() : nullptr;
                 auto grid_big = [](uint64_t k) { return (unsigned)std::min<uint64_t>(k, 1u << 16); };
                 const unsigned grid_small = (unsigned)std::min<uint64_t>((it[0].n + TCW - 1) / TCW, TC_SMALL_GRID);
                 if (it[3].n) {
                     if (kind == 0)
                         k_tc_count_bm<false><<<grid_big(it[3].n), TCB_BIG, 0, ks>>>(
-                            ooff, otgt, plist, it[3].start.get(), it[3].end.get(), it[3].owner.get(), it[3].n, tri);
+                            ooff, otgt, plist, iwalk, it[3].start.get(), it[3].end.get(), it[3].owner.get(), it[3].n, tri);
                     else
                         k_tc_count_bm<true><<<grid_big(it[3].n), TCB_BIG, 0, ks>>>(
-                            ooff, otgt, plist, it[3].start.get(), it[3].end.get(), it[3].owner.get(), it[3].n, tri);
+                            ooff, otgt, plist, iwalk, it[3].start.get(), it[3].end.get(), it[3].owner.get(), it[3].n, tri);
                 }
                 if (kind == 0) {
                     if (it[2].n)
                         k_tc_count_big<BIG_CAP, false><<<grid_big(it[2].n), TCB_BIG, 0, ks>>>(
-                            ooff, otgt, plist, it[2].start.get(), it[2].end.get(), it[2].owner.get(), it[2].n, tri);
+                            ooff, otgt, plist, iwalk, it[2].start.get(), it[2].end.get(), it[2].owner.get(), it[2].n, tri);
                     if (it[1].n)
                         k_tc_count_big<MID_CAP, false><<<grid_big(it[1].n), TCB_BIG, 0, ks>>>(
-                            ooff, otgt, plist, it[1].start.get(), it[1].end.get(), it[1].owner.get(), it[1].n, tri);
+                            ooff, otgt, plist, iwalk, it[1].start.get(), it[1].end.get(), it[1].owner.get(), it[1].n, tri);
                     if (it[0].n)
-                        k_tc_count<false><<<grid_small, TCB, 0, ks>>>(ooff, otgt, plist, it[0].start.get(),
+                        k_tc_count<false><<<grid_small, TCB, 0, ks>>>(ooff, otgt, plist, iwalk, it[0].start.get(),
                                                                       it[0].end.get(), it[0].owner.get(), it[0].n, tri);
                 } else {
                     if (it[2].n)
                         k_tc_count_big<BIG_CAP, true><<<grid_big(it[2].n), TCB_BIG, 0, ks>>>(
-                            ooff, otgt, plist, it[2].start.get(), it[2].end.get(), it[2].owner.get(), it[2].n, tri);
+                            ooff, otgt, plist, iwalk, it[2].start.get(), it[2].end.get(), it[2].owner.get(), it[2].n, tri);
                     if (it[1].n)
                         k_tc_count_big<MID_CAP, true><<<grid_big(it[1].n), TCB_BIG, 0, ks>>>(
-                            ooff, otgt, plist, it[1].start.get(), it[1].end.get(), it[1].owner.get(), it[1].n, tri);
+                            ooff, otgt, plist, iwalk, it[1].start.get(), it[1].end.get(), it[1].owner.get(), it[1].n, tri);
                     if (it[0].n)
-                        k_tc_count<true><<<grid_small, TCB, 0, ks>>>(ooff, otgt, plist, it[0].start.get(),
+                        k_tc_count<true><<<grid_small, TCB, 0, ks>>>(ooff, otgt, plist, iwalk, it[0].start.get(),
                                                                      it[0].end.get(), it[0].owner.get(), it[0].n, tri);
                 }
             }
