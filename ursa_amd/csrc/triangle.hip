@@ -60,10 +60,10 @@ constexpr uint32_t MID_CAP = 1024;   // N+(u) entries staged per workgroup in th
 constexpr uint32_t BIG_CAP = 2048;   // N+(u) entries staged in LDS per workgroup (24 KB table)
 constexpr uint32_t BM_BITS = 65536;  // bitmap tables cover new ids [0, BM_BITS) (8 KB)
 constexpr uint32_t BM_WORDS = BM_BITS / 32;
-constexpr uint64_t TC_CHUNK = 65536;  // intersection steps per work item (about)
+constexpr uint64_t TC_CHUNK = 131072;  // intersection steps per work item (about; 65536 measured 3-4 % slower once hub-side walks stop below the owner)
 constexpr uint32_t TC_MAX_SLOTS = 32768;  // list slots per work item at most (keeps hit counters below 2^16)
 constexpr int TC_ILP = 4;            // 64-step groups whose loads are in flight together (s22: 2 -> 4 cut 25.9 to 21.4 ms, 8 gave 22.1)
-constexpr uint32_t TC_LONG = 32;     // walks at least this long are taken by the whole wave
+constexpr uint32_t TC_LONG = 16;     // walks at least this long are taken by the whole wave (8 and 16 alone measured alike, 64 20 % slower)
 constexpr int TC_LILP = 8;           // 64-step chunks of long walks in flight together
 
 // Each oriented edge x -> y (a pair; y in N+(x), so y < x) is intersected once.
@@ -131,7 +131,7 @@ __global__ void k_tc_item_flags(const uint32_t* __restrict__ loff, const uint32_
         const uint32_t o = owner[k];
         const uint32_t a = loff[o], e = loff[o + 1], du = ooff[o + 1] - ooff[o];
         // Lists staged in the mid-size workgroup table are cut 8x coarser (about
-        // 256 K steps), so their larger staging and count flush are paid less
+        // 1 M steps), so their larger staging and count flush are paid less
         // often, while the longest lists still spread over several workgroups
         // instead of finishing the launch alone (at s22: 30.8 ms against 32.5
         // uncut, 31.8 at 32x and 37.8 at 2x). The few lists of the large table
