@@ -296,6 +296,38 @@ def test_triangles_dense_rmat_both_tables(ua):
                 assert np.array_equal(ua.triangle_count(t), want), bitmaps
 
 
+def test_triangles_clique_with_satellites_closed_form(ua):
+    # K_300 plus 3000 satellites, each joined to k in {1, 2, 20, 40} distinct
+    # clique vertices. Oriented by (degree, id), the 43 lowest-ranked clique
+    # vertices have more than 256 out-entries, so satellite pairs into them
+    # are walked from the hub's side (only the satellite's entries before the
+    # hub), pairs at position 0 have no triangle, and the rest from the
+    # satellite's side. Closed form: a satellite with k neighbours is in
+    # C(k, 2) triangles, a clique vertex in C(299, 2) plus k - 1 for every
+    # satellite joined to it.
+    rng = np.random.default_rng(11)
+    c, ns = 300, 3000
+    iu, ju = np.triu_indices(c, 1)
+    src, dst = [iu], [ju]
+    want = np.zeros(c + ns, np.uint64)
+    want[:c] = (c - 1) * (c - 2) // 2
+    for x in range(ns):
+        k = int(rng.choice([1, 2, 20, 40]))
+        nb = rng.choice(c, size=k, replace=False)
+        src.append(np.full(k, c + x))
+        dst.append(nb)
+        want[c + x] = k * (k - 1) // 2
+        want[nb] += k - 1
+    s = np.concatenate(src).astype(np.uint32)
+    d = np.concatenate(dst).astype(np.uint32)
+    n = c + ns
+    assert np.array_equal(orc.triangle_count(orc.Topology(n, s, d), nthreads=8), want)
+    for bitmaps in (True, False):
+        with tri_tables(ua, bitmaps):
+            t = ua.Topology.build(n, s, d)
+            assert np.array_equal(ua.triangle_count(t), want), bitmaps
+
+
 def test_triangles_goldens(ua):  # triangle.rs:70-99
     t = ua.Topology.build(4, [0, 0, 0, 1, 1, 2], [1, 2, 3, 2, 3, 3])
     assert list(ua.triangle_count(t)) == [3, 3, 3, 3]
