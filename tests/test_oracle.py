@@ -308,6 +308,29 @@ def test_triangles_vs_networkx(name):
         assert got[k] == v
 
 
+def test_triangles_clique_with_satellites_closed_form():
+    # K_60 plus 400 satellites, each joined to k in {1, 2, 5, 10} distinct
+    # clique vertices: a satellite is in C(k, 2) triangles, a clique vertex in
+    # C(59, 2) plus k - 1 per satellite joined to it.
+    rng = np.random.default_rng(5)
+    c, ns = 60, 400
+    iu, ju = np.triu_indices(c, 1)
+    src, dst = [iu], [ju]
+    want = np.zeros(c + ns, np.uint64)
+    want[:c] = (c - 1) * (c - 2) // 2
+    for x in range(ns):
+        k = int(rng.choice([1, 2, 5, 10]))
+        nb = rng.choice(c, size=k, replace=False)
+        # both orientations of the satellite edges, so the undirected view dedups them
+        src += [np.full(k, c + x), nb]
+        dst += [nb, np.full(k, c + x)]
+        want[c + x] = k * (k - 1) // 2
+        want[nb] += k - 1
+    s = np.concatenate(src).astype(np.uint32)
+    d = np.concatenate(dst).astype(np.uint32)
+    assert np.array_equal(orc.triangle_count(orc.Topology(c + ns, s, d)), want)
+
+
 def test_triangles_brute_force():
     rng = np.random.default_rng(3)
     n = 40
