@@ -79,7 +79,8 @@ constexpr int TC_LILP = 8;           // 64-step chunks of long walks in flight t
 // atomic credit (on graphs of even, small degrees splitting them as well
 // doubled the staging and the launches). With whole out-lists walked on y's
 // side and the rule d+(x) < d+(y), a hub threshold of 128 measured 21.8 ms at
-// s22, 256 20.4 ms, 512 21.2 ms and 1024 33.4 ms.
+// s22, 256 20.4 ms, 512 21.2 ms and 1024 33.4 ms; with the prefix walks and
+// the rule above, 64 measured 21.5 ms, 128 21.1 ms, 256 18.6 ms and 512 20.1 ms.
 constexpr uint32_t TC_HUB = 256;
 // Kind 1 when y's list is a hub's and longer than the prefix of N+(x) below y.
 __device__ __forceinline__ bool tc_from_owner(uint32_t d_y, uint32_t pos) { return d_y > TC_HUB && pos < d_y; }
